@@ -1,0 +1,261 @@
+/*
+ * mck.h - C-ABI of the MI355X implementation of MC Kernel's per-point kernel
+ * evaluation and Monte Carlo integration path.
+ *
+ * The boundary is the reference's work unit: `slave_work` in src/slave.f90:273-810,
+ * called once per task by `do_slave` (src/slave.f90:205-219), plus the once-per-process
+ * setup that `do_slave` performs before its task loop (src/slave.f90:94-173).
+ * Every entry point takes plain pointers and sizes; arrays are caller-owned and use
+ * Fortran (column-major) order where the reference passes Fortran arrays, so a Fortran
+ * host binds them with iso_c_binding without copies.
+ *
+ * Error convention: every int-returning function returns 0 on success and a nonzero
+ * MCK_E* code on failure; mck_last_error() then describes the failure.  The reference
+ * aborts the whole MPI job instead (pabort, src/commpi.F90:515-556); a Fortran host keeps
+ * that behaviour by calling pabort on a nonzero return.
+ *
+ * Threading: one call in flight per process; one GPU per process (the device that is
+ * current when mck_init is called).
+ */
+#ifndef MCK_H
+#define MCK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------------------ */
+#define MCK_OK              0
+#define MCK_EINVAL          1   /* invalid argument / inconsistent setup               */
+#define MCK_ENOTINIT        2   /* mck_init has not been called                        */
+#define MCK_ENOMEM          3   /* device or host allocation failed                    */
+#define MCK_EHIP            4   /* a HIP runtime call failed                           */
+#define MCK_ENOTFOUND       5   /* point not inside any of the 12 nearest elements     */
+                                /* (reference: 'element not found', readfields.f90:1100) */
+#define MCK_ENAN            6   /* NaN in an MC integral (mc_integration.f90:113-130)  */
+#define MCK_EUNSUPPORTED    7   /* feature the reference aborts on (e.g. rho kernels)  */
+
+/* ---- model parameter codes (kernel.f90:665-832, type_parameter.f90:487-491) ---------- */
+#define MCK_PARAM_LAM  0
+#define MCK_PARAM_VP   1
+#define MCK_PARAM_VS   2
+#define MCK_PARAM_MU   3
+#define MCK_PARAM_VSH  4
+#define MCK_PARAM_VSV  5
+#define MCK_PARAM_VPH  6
+#define MCK_PARAM_VPV  7
+#define MCK_PARAM_ETA  8
+#define MCK_PARAM_RHO  9   /* rejected with MCK_EUNSUPPORTED: kernel.f90:554-556,824-826 */
+
+/* ---- misfit types and integration schemes (kernel.f90:410-510) --------------------- */
+#define MCK_MISFIT_CC  0
+#define MCK_MISFIT_AM  1
+#define MCK_INT_PARSEVAL     0
+#define MCK_INT_TRAPEZOIDAL  1
+
+/* ---- filter classes (filtering.f90:78-230) ---------------------------------------- */
+#define MCK_FILTER_IDENT       0
+#define MCK_FILTER_GABOR       1   /* freq = [centre period, sigma/period, tshift, -]   */
+#define MCK_FILTER_BUTTER_LP   2   /* freq = [period, -, order, -]                       */
+#define MCK_FILTER_BUTTER_HP   3
+#define MCK_FILTER_BUTTER_BP   4   /* freq = [long period, short period, order, -]       */
+#define MCK_FILTER_BUTTER_LP_ZP 5
+#define MCK_FILTER_BUTTER_HP_ZP 6
+#define MCK_FILTER_BUTTER_BP_ZP 7
+
+/*
+ * One merged AxiSEM wavefield database (one of fwd / bwd), already in host memory.
+ * Mirrors the merged NetCDF layout read by readfields.f90 (group `Mesh` and variable
+ * `merged_snapshots`, readfields.f90:2925-2998, 2673-2768; repack_database.py:151-187).
+ */
+typedef struct {
+    int32_t npol;              /* polynomial order, 4                                    */
+    int32_t nelem;             /* number of spectral elements                            */
+    int32_t npoints;           /* number of global GLL points                            */
+    const float   *mesh_s;     /* [npoints] s coordinate of GLL points (m)  Mesh/mesh_S  */
+    const float   *mesh_z;     /* [npoints]                                  Mesh/mesh_Z */
+    const float   *mp_s;       /* [nelem] element midpoints (m)           Mesh/mp_mesh_S */
+    const float   *mp_z;       /* [nelem]                                 Mesh/mp_mesh_Z */
+    const int32_t *fem_mesh;   /* [nelem][4] 0-based corner point ids      Mesh/fem_mesh */
+    const int32_t *eltype;     /* [nelem] 0 spheroid, 1 subpar, 2 semino, 3 semiso       */
+    const int32_t *axis;       /* [nelem] 1 if the element touches the axis              */
+    int32_t nvar;              /* 10 (fwd: MZZ s,z; MXX s,z; MXZ s,p,z; MXY s,p,z)       */
+                               /*  5 (bwd: PZ s,z; PX s,p,z)                             */
+    const float   *snapshots;  /* merged_snapshots, C dims [nelem][nvar][npol+1][npol+1][nt] */
+                               /* (= Fortran u(nt, ipol, jpol, nvar, nelem))             */
+    double amplitude;          /* global attribute "scalar source magnitude"             */
+} mck_db_t;
+
+/* One seismic filter as read from filters.dat (type_parameter.f90:695-766). */
+typedef struct {
+    int32_t filterclass;       /* MCK_FILTER_*                                           */
+    double  frequencies[4];    /* class-dependent parameters, see MCK_FILTER_*           */
+} mck_filter_t;
+
+/*
+ * Once-per-process setup: the products of do_slave's setup phase
+ * (src/slave.f90:94-173: read_parameters, read_source, read_receiver, read_filter,
+ * read_kernel, sem_data%open_files/read_meshes/load_seismogram).
+ */
+typedef struct {
+    mck_db_t fwd;              /* forward (source) database                              */
+    mck_db_t bwd;              /* backward (receiver, reciprocity) database              */
+    int32_t nt;                /* number of strain dumps (ndumps)                        */
+    double  dt;                /* strain dump sampling rate in s                         */
+    double  planet_radius;     /* m                                                      */
+    double  timeshift_fwd;     /* "source shift factor in sec" of fwd / bwd databases,   */
+    double  timeshift_bwd;     /*  applied when deconv_stf == 0 (slave.f90:531,596)      */
+    const double *stf_fwd;     /* [nt] stf_dump of the fwd DB (used when deconv_stf)     */
+
+    /* source (source.f90:67-110): lat/lon in degrees, depth in km, Mij in N m,          */
+    /* order rr, tt, pp, rt, rp, tp                                                      */
+    double  src_lat, src_lon, src_depth_km;
+    double  src_mij[6];
+    const double *stf_src;     /* [nt] earthquake STF sampled at dt (deconv_stf only),   */
+                               /* NULL = unit impulse                                    */
+    double  stf_src_shift;     /* time shift of the earthquake STF in s                  */
+
+    /* receivers (type_parameter.f90:396-517)                                            */
+    int32_t nrec;
+    const double *rec_lat;     /* [nrec] degrees                                         */
+    const double *rec_lon;     /* [nrec] degrees                                         */
+    char    component;         /* 'Z', 'R' or 'T'                                        */
+    const double *seis;        /* [nrec][nt] velocity seismogram of each receiver        */
+                               /* (sem_data%seis, readfields.f90:1004-1048)              */
+
+    /* filters and kernels (type_parameter.f90:521-691)                                  */
+    int32_t nfilter;
+    const mck_filter_t *filters;
+    int32_t nkernel;           /* kernels are grouped by receiver in receiver order      */
+    const int32_t *kernel_receiver; /* [nkernel] 0-based receiver of each kernel         */
+    const int32_t *kernel_filter;   /* [nkernel] 0-based filter index                    */
+    const int32_t *kernel_misfit;   /* [nkernel] MCK_MISFIT_*                            */
+    const int32_t *kernel_param;    /* [nkernel] MCK_PARAM_*                             */
+    const double  *kernel_window;   /* [nkernel][2] time window (s)                      */
+
+    /* 1-D background model on a radial grid (interpolate_mesh.f90:45-109)               */
+    int32_t model_ndepth;
+    float   model_dr;          /* 100 m in the reference                                 */
+    const float *model_vp, *model_vs, *model_rho, *model_phi, *model_xi, *model_eta;
+    float   vp_max, vs_max;    /* maximum velocities of the model (travel-time mask)     */
+
+    /* Monte Carlo and kernel options (inparam_basic, type_parameter.f90:139-274)        */
+    int32_t npoints_per_step;  /* POINTS_PER_MC_STEP                                     */
+    int32_t max_iter;          /* MAXIMUM_ITERATIONS                                     */
+    double  allowed_error;     /* ALLOWED_ERROR                                          */
+    double  allowed_relative_error; /* ALLOWED_RELATIVE_ERROR                            */
+    int32_t int_over_volume;   /* !NO_INT_OVER_VOLUME                                    */
+    int32_t relative_kernel;   /* !KERNEL_FOR_ABSOLUTE_PERTURBATIONS                     */
+    int32_t quasirandom;       /* !USE_PSEUDORANDOM_NUMBERS (only 1 is supported)        */
+    int32_t int_scheme;        /* MCK_INT_*                                              */
+    int32_t deconv_stf;        /* !NO_DECONVOLVE_STF                                     */
+    double  damp_radius;       /* DAMP_RADIUS_SOURCE_RECEIVER (m); <= 0 disables         */
+    int32_t mask_src_rec;      /* MASK_SOURCE_RECEIVER                                   */
+} mck_setup_t;
+
+/* Work-unit statistics of the last mck_slave_work call (device timings via hipEvents). */
+typedef struct {
+    int64_t evaluations;       /* computed (voxel, MC point, kernel) values              */
+    int64_t points;            /* MC points drawn                                        */
+    int64_t fwd_locations;     /* point locations in the fwd mesh                        */
+    int64_t bwd_locations;     /* point locations in the bwd mesh (sum over receivers)   */
+    int64_t fwd_bytes;         /* algorithmic DB bytes of the strain gathers (fwd)       */
+    int64_t bwd_bytes;         /* algorithmic DB bytes of the strain gathers (bwd)       */
+    int32_t mc_steps;          /* step-synchronous MC iterations performed               */
+    int32_t strain_launches;   /* launches of the strain-gather/FFT kernel               */
+    double  device_ms;         /* device time of the whole call (hipEvent)               */
+    double  strain_fwd_ms;     /* summed time of the fwd strain-gather/FFT kernel        */
+    double  strain_bwd_ms;     /* summed time of the bwd strain-gather/FFT kernel        */
+    double  wall_ms;           /* host wall time of the whole call                       */
+} mck_stats_t;
+
+/*
+ * Replaces the setup half of do_slave (src/slave.f90:94-173): uploads both databases to
+ * HBM, builds the point-location grids, the derivative matrices (calc_gradient_terms,
+ * readfields.f90:3084-3150), the filters (filtering.f90:78-470) and the per-kernel
+ * window/seismogram integration weights (kernel.f90:181-385).
+ */
+int mck_init(const mck_setup_t *setup);
+
+/*
+ * Replaces slave_work (src/slave.f90:273-810) for one task of inversion elements.
+ *  elem_type       4 = tetrahedron (inversion_mesh.f90:766-805 ielem_type); others rejected
+ *  nvpe            vertices per element (4)
+ *  nelems          elements in this task
+ *  nbasis          basis functions per element (1 = volumetric)
+ *  vertices        [nvertices][3] metres (Fortran vertices(3, nvertices))
+ *  conn            [nelems][nvpe] 1-based vertex ids (Fortran connectivity(nvpe, nelems));
+ *                  elements whose first id is <= 0 are skipped (empty slots of a last task)
+ *  kernel_values   out [nelems][nbasis][nkernel]   (Fortran (nkernel, nbasis, nelems))
+ *  kernel_variance out [nelems][nbasis][nkernel]
+ *  niterations     out [nelems][nkernel]           (Fortran (nkernel, nelems))
+ *  computation_time out [nelems] seconds (device time of the task / live elements)
+ *  halton_step     in/out position of the Halton stream (halton_sequence.f90:109-153)
+ *  halton_offsets  optional [nelems] Halton stream position at which each element starts;
+ *                  NULL = each element reserves (max_iter+1)*npoints_per_step draws of the
+ *                  stream starting at *halton_step (voxels are then independent)
+ */
+int mck_slave_work(int32_t elem_type, int32_t nvpe, int32_t nelems, int32_t nbasis,
+                   int32_t nvertices, const double *vertices, const int32_t *conn,
+                   double *kernel_values, double *kernel_variance, int32_t *niterations,
+                   double *computation_time, int64_t *halton_step,
+                   const int64_t *halton_offsets);
+
+/*
+ * The same work unit evaluated the way the reference loop runs it: one element and one
+ * MC step at a time, each step a separate sequence of launches over npoints_per_step
+ * points.  Same arguments and results as mck_slave_work; used as the one-point-at-a-time
+ * comparator for the batched form.
+ */
+int mck_slave_work_plain(int32_t elem_type, int32_t nvpe, int32_t nelems, int32_t nbasis,
+                         int32_t nvertices, const double *vertices, const int32_t *conn,
+                         double *kernel_values, double *kernel_variance,
+                         int32_t *niterations, double *computation_time,
+                         int64_t *halton_step, const int64_t *halton_offsets);
+
+/*
+ * Point location, replacing the front half of load_fw_points / load_bw_points
+ * (readfields.f90:1192-1226, 1359-1418) and find_element (readfields.f90:1052-1134).
+ *  frame      0 = fwd (source frame), 1+r = bwd frame of receiver r
+ *  xyz        [npts][3] metres
+ *  elem       out [npts] 1-based element id (the reference's kd-tree index)
+ *  xi, eta    out [npts] reference coordinates
+ *  ties       out [npts] 1 where two of the 13 nearest midpoints have equal fp32
+ *             distance (the reference's heap order then decides; NULL allowed)
+ */
+int mck_locate_points(int32_t frame, int32_t npts, const double *xyz,
+                      int32_t *elem, double *xi, double *eta, int32_t *ties);
+
+/*
+ * Strain traces at points, replacing load_fw_points (readfields.f90:1138-1308) for
+ * frame 0 and load_bw_points (readfields.f90:1312-1465) for frame 1+r.
+ *  traces     out [npts][nd][nt] float (Fortran (nt, nd, npts)); nd = 6 for full strain,
+ *             1 for the trace.  The fwd result is before cumsum_trapezoidal.
+ */
+int mck_load_strain_points(int32_t frame, int32_t npts, const double *xyz, float *traces);
+
+/*
+ * Base-kernel values for given points, replacing the receiver loop body of slave_work
+ * (slave.f90:588-640: load_bw_points, rfft, calc_basekernel, apply_filter, irfft,
+ * calc_misfit_kernel) for all receivers, one MC batch, without masks or damping.
+ *  base       out [npts][nkernel][6] double (Fortran kernelvalue_basekers(npts, nkernel, 6)
+ *             transposed): base kernels lambda, mu, rho, a, b, c of each kernel; slots the
+ *             kernel's model parameter does not need are 0
+ */
+int mck_base_kernels(int32_t npts, const double *xyz, double *base);
+
+/* Statistics of the last mck_slave_work / mck_slave_work_plain call. */
+int mck_get_stats(mck_stats_t *stats);
+
+/* Frees device memory and resets the library (reference: freeme routines). */
+void mck_finalize(void);
+
+/* Human-readable description of the last error (thread-local, never NULL). */
+const char *mck_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MCK_H */

@@ -1,0 +1,117 @@
+"""MI355X implementation of MC Kernel's per-point kernel evaluation and Monte Carlo
+integration path (the work unit `slave_work` of src/slave.f90).
+
+The compute path is the HIP library libmck.so behind the C-ABI of include/mck.h; this
+package is its Python binding.  There is no CPU fallback: every call goes to the GPU and
+fails loudly when the library or a GPU is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+__all__ = ["MCKernel", "MckError", "load_library", "abi"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+class MckError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("mck error %d: %s" % (code, msg))
+        self.code = code
+
+
+def library_path():
+    return os.path.join(_HERE, "libmck.so")
+
+
+def load_library():
+    """Loads libmck.so (built in-tree by __graft_entry__.build())."""
+    global _LIB
+    if _LIB is None:
+        path = library_path()
+        if not os.path.exists(path):
+            raise RuntimeError("libmck.so is not built (run __graft_entry__.build()); "
+                               "there is no CPU fallback")
+        _LIB = abi.declare(C.CDLL(path))
+    return _LIB
+
+
+def _check(rc):
+    if rc != 0:
+        raise MckError(rc, load_library().mck_last_error().decode())
+
+
+class MCKernel:
+    """One process-wide instance of the GPU path (mck_init / mck_finalize)."""
+
+    def __init__(self, setup):
+        self.lib = load_library()
+        self.setup = setup          # the caller keeps the arrays behind it alive
+        self.nk = setup.nkernel
+        self.nt = setup.nt
+        self.R = setup.nrec
+        _check(self.lib.mck_init(C.byref(setup)))
+        self.halton_step = 0
+        full = any(p not in (0, 1, 6, 7) for p in
+                   np.ctypeslib.as_array(setup.kernel_param, shape=(setup.nkernel,)))
+        self.nd = 6 if full else 1
+
+    def close(self):
+        if self.lib is not None:
+            self.lib.mck_finalize()
+            self.lib = None
+
+    def slave_work(self, verts, conn, halton_offsets=None, plain=False):
+        """mck_slave_work for a tetrahedral task; returns a dict of numpy arrays."""
+        verts = np.ascontiguousarray(verts, np.float64)
+        conn = np.ascontiguousarray(conn, np.int32)
+        ne = conn.shape[0]
+        vals = np.zeros((ne, self.nk))
+        var = np.zeros((ne, self.nk))
+        nit = np.zeros((ne, self.nk), np.int32)
+        tm = np.zeros(ne)
+        step = np.array([self.halton_step], np.int64)
+        offs = None if halton_offsets is None else np.ascontiguousarray(halton_offsets, np.int64)
+        fn = self.lib.mck_slave_work_plain if plain else self.lib.mck_slave_work
+        _check(fn(4, 4, ne, 1, verts.shape[0], abi.ptr(verts, C.c_double),
+                  abi.ptr(conn, C.c_int32), abi.ptr(vals, C.c_double), abi.ptr(var, C.c_double),
+                  abi.ptr(nit, C.c_int32), abi.ptr(tm, C.c_double), abi.ptr(step, C.c_int64),
+                  abi.ptr(offs, C.c_int64)))
+        self.halton_step = int(step[0])
+        return dict(values=vals, variance=var, niterations=nit, computation_time=tm,
+                    halton_step=int(step[0]), stats=self.stats())
+
+    def stats(self):
+        s = abi.MckStats()
+        _check(self.lib.mck_get_stats(C.byref(s)))
+        return {f: getattr(s, f) for f, _ in abi.MckStats._fields_}
+
+    def locate(self, frame, xyz):
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        n = xyz.shape[0]
+        elem = np.zeros(n, np.int32)
+        xi = np.zeros(n)
+        eta = np.zeros(n)
+        ties = np.zeros(n, np.int32)
+        _check(self.lib.mck_locate_points(frame, n, abi.ptr(xyz, C.c_double),
+                                          abi.ptr(elem, C.c_int32), abi.ptr(xi, C.c_double),
+                                          abi.ptr(eta, C.c_double), abi.ptr(ties, C.c_int32)))
+        return elem, xi, eta, ties
+
+    def load_strain(self, frame, xyz):
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        out = np.zeros((xyz.shape[0], self.nd, self.nt), np.float32)
+        _check(self.lib.mck_load_strain_points(frame, xyz.shape[0], abi.ptr(xyz, C.c_double),
+                                               abi.ptr(out, C.c_float)))
+        return out
+
+    def base_kernels(self, xyz):
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        out = np.zeros((xyz.shape[0], self.nk, 6))
+        _check(self.lib.mck_base_kernels(xyz.shape[0], abi.ptr(xyz, C.c_double),
+                                         abi.ptr(out, C.c_double)))
+        return out

@@ -1,0 +1,100 @@
+"""Native build of the package: the HIP C-ABI library (libmck.so, gfx950), the synthetic
+workload generator (libmck_synth.so) and, as test infrastructure, the CPU oracle
+(oracle/liboracle.so) and the reference's own kdtree2 (oracle/_ref)."""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+
+# Packed FP32 instructions are disabled for every HIP translation unit; FMA contraction
+# is disabled so that the point-location arithmetic matches the reference bit for bit.
+HIP_FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-ffp-contract=off",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function"]
+
+
+def _run(cmd, cwd=None):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, cwd=cwd)
+
+
+def _stale(target, sources):
+    if not os.path.exists(target):
+        return True
+    t = os.path.getmtime(target)
+    return any(os.path.getmtime(s) > t for s in sources)
+
+
+def build_synth(force=True):
+    src = os.path.join(CSRC, "synth.cpp")
+    out = os.path.join(HERE, "libmck_synth.so")
+    if force or _stale(out, [src]):
+        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", src, "-o", out])
+    return out
+
+
+def hip_sources():
+    return [os.path.join(CSRC, f) for f in sorted(os.listdir(CSRC)) if f.endswith(".hip")]
+
+
+def build_hip(force=True):
+    srcs = hip_sources()
+    hdrs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+    hdrs.append(os.path.join(ROOT, "include", "mck.h"))
+    out = os.path.join(HERE, "libmck.so")
+    if not srcs:
+        raise RuntimeError("no HIP sources")
+    if force or _stale(out, srcs + hdrs):
+        hipcc = os.path.join(ROCM, "bin", "hipcc")
+        objs = []
+        for s in srcs:
+            o = os.path.join(HERE, "_build", os.path.basename(s) + ".o")
+            os.makedirs(os.path.dirname(o), exist_ok=True)
+            _run([hipcc] + HIP_FLAGS + ["-I", os.path.join(ROOT, "include"), "-I", CSRC,
+                                        "-c", s, "-o", o])
+            objs.append(o)
+        _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs +
+             ["-o", out, "-Wl,--no-undefined", "-lamdhip64"])
+    return out
+
+
+def build_oracle(force=True):
+    odir = os.path.join(ROOT, "oracle")
+    srcs = [os.path.join(odir, f) for f in ("mck_oracle_core.c", "mck_oracle_path.c")]
+    out = os.path.join(odir, "liboracle.so")
+    if force or _stale(out, srcs + [os.path.join(odir, "mck_oracle.h")]):
+        _run(["gcc", "-O2", "-std=gnu11", "-fPIC", "-shared", "-ffp-contract=off",
+              "-fno-fast-math", "-fexcess-precision=standard"] + srcs + ["-o", out, "-lm"])
+    return out
+
+
+def build_ref(force=True):
+    """oracle/_ref: the reference's kdtree2.f90 compiled where it lies, plus this repo's
+    bind(c) wrapper; skipped when the reference tree is absent."""
+    odir = os.path.join(ROOT, "oracle")
+    script = os.path.join(odir, "build_ref.sh")
+    if not os.path.isdir("/root/reference/src") or not shutil.which("bash"):
+        return None
+    out = os.path.join(odir, "_ref", "libref_kdtree.so")
+    if force or not os.path.exists(out):
+        _run(["bash", script])
+    return out
+
+
+def build_all(force=True):
+    build_synth(force)
+    build_oracle(force)
+    try:
+        build_ref(force)
+    except subprocess.CalledProcessError as e:  # the reference build is optional
+        print("oracle/_ref build failed:", e, file=sys.stderr)
+    build_hip(force)
+
+
+if __name__ == "__main__":
+    build_all(force="--force" in sys.argv)

@@ -1,0 +1,1263 @@
+// mck_api.hip - host side of the C-ABI (include/mck.h): once-per-process setup (HBM
+// residency of the merged databases, point-location grids, derivative matrices,
+// filters and per-kernel window spectra) and the step-synchronous batched evaluation
+// of slave_work (src/slave.f90:273-810) over all voxels of a task.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "mck.h"
+#include "mck_device.h"
+
+#include "mck_kernels.inc.h"
+
+using namespace mck;
+using cd = std::complex<double>;
+
+namespace {
+
+thread_local std::string g_err = "no error";
+const double kPi = 3.1415926535898;  // global_parameters.f90:40
+const double kDeg2Rad = kPi / 180.0;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHECK(x)                                                                   \
+    do {                                                                              \
+        hipError_t e_ = (x);                                                          \
+        if (e_ != hipSuccess)                                                         \
+            return fail(MCK_EHIP, std::string(#x) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t want) {
+        if (want <= n) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc((void **)&p, std::max<size_t>(want, 1) * sizeof(T));
+        if (e == hipSuccess) n = want;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+// ---- host FFT (double), used for the per-kernel seismogram set-up only --------------
+void fft_inplace(std::vector<cd> &a, int sign) {
+    const size_t n = a.size();
+    for (size_t i = 1, j = 0; i < n; ++i) {
+        size_t bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(a[i], a[j]);
+    }
+    for (size_t len = 2; len <= n; len <<= 1) {
+        for (size_t k = 0; k < len / 2; ++k) {
+            double ang = sign * 2.0 * M_PI * (double)k / (double)len;
+            cd w(std::cos(ang), std::sin(ang));
+            for (size_t m = k; m < n; m += len) {
+                cd t = a[m + len / 2] * w;
+                a[m + len / 2] = a[m] - t;
+                a[m] += t;
+            }
+        }
+    }
+}
+// X = dt * DFT(x) (r2c), x = df * IDFT(X) (c2r, imaginary parts of DC and Nyquist ignored)
+std::vector<cd> rfft(const std::vector<double> &x, int N, double dt) {
+    std::vector<cd> a(N, 0.0);
+    for (int i = 0; i < (int)x.size() && i < N; ++i) a[i] = x[i];
+    fft_inplace(a, -1);
+    std::vector<cd> X(N / 2 + 1);
+    for (int k = 0; k <= N / 2; ++k) X[k] = a[k] * dt;
+    return X;
+}
+std::vector<double> irfft(const std::vector<cd> &X, int N, double df) {
+    std::vector<cd> a(N);
+    a[0] = X[0].real();
+    a[N / 2] = X[N / 2].real();
+    for (int k = 1; k < N / 2; ++k) { a[k] = X[k]; a[N - k] = std::conj(X[k]); }
+    fft_inplace(a, +1);
+    std::vector<double> x(N);
+    for (int i = 0; i < N; ++i) x[i] = a[i].real() * df;
+    return x;
+}
+
+int nextpow2(int n) { int p = 2; while (p < n) p *= 2; return p; }
+
+// ---- filters (filtering.f90:78-230, 699-794) -----------------------------------------
+std::vector<cd> butter_lp(const std::vector<double> &f, double pc, int n) {
+    double wc = 2 * kPi / pc;
+    std::vector<cd> s(n), out(f.size());
+    for (int k = 1; k <= n; ++k) {
+        double th = (double)(2 * k + n - 1) * kPi / (double)(2 * n);
+        s[k - 1] = cd(wc * std::cos(th), wc * std::sin(th));
+    }
+    for (size_t i = 0; i < f.size(); ++i) {
+        cd v = 1.0;
+        for (int k = 0; k < n; ++k) v /= (cd(0.0, 2 * kPi * f[i]) - s[k]) / wc;
+        out[i] = v;
+    }
+    return out;
+}
+std::vector<cd> butter_hp(const std::vector<double> &f, double pc, int n) {
+    double wc = 2 * kPi / pc;
+    std::vector<cd> s(n), out(f.size());
+    for (int k = 1; k <= n; ++k) {
+        double th = (double)(2 * k + n - 1) * kPi / (double)(2 * n);
+        s[k - 1] = cd(wc * std::cos(th), wc * std::sin(th));
+    }
+    for (size_t i = 0; i < f.size(); ++i) {
+        if (i == 0) { out[0] = 0.0; continue; }
+        cd v = 1.0;
+        for (int k = 0; k < n; ++k) v /= wc / cd(0.0, 2 * kPi * f[i]) - s[k] / wc;
+        out[i] = v;
+    }
+    return out;
+}
+std::vector<cd> filter_tf(const mck_filter_t &flt, const std::vector<double> &f) {
+    const double *fr = flt.frequencies;
+    std::vector<cd> tf(f.size());
+    switch (flt.filterclass) {
+    case MCK_FILTER_IDENT: std::fill(tf.begin(), tf.end(), cd(1.0, 0.0)); break;
+    case MCK_FILTER_GABOR: {
+        double fc = 1 / fr[0], ls = std::log(fr[1]);
+        for (size_t i = 0; i < f.size(); ++i) {
+            double g = 0.0;
+            if (f[i] > 0) { double l = std::log(f[i] / fc); g = std::exp(-(l * l) / (2 * (ls * ls))); }
+            double th = (-2.0 * kPi) * f[i] * fr[2];
+            tf[i] = cd(g * std::cos(th), g * std::sin(th));
+        }
+        break;
+    }
+    case MCK_FILTER_BUTTER_LP: case MCK_FILTER_BUTTER_LP_ZP: tf = butter_lp(f, fr[0], (int)fr[2]); break;
+    case MCK_FILTER_BUTTER_HP: case MCK_FILTER_BUTTER_HP_ZP: tf = butter_hp(f, fr[0], (int)fr[2]); break;
+    default: {
+        auto a = butter_hp(f, fr[0], (int)fr[2]), b = butter_lp(f, fr[1], (int)fr[2]);
+        for (size_t i = 0; i < f.size(); ++i) tf[i] = a[i] * b[i];
+    }
+    }
+    if (flt.filterclass >= MCK_FILTER_BUTTER_LP_ZP)
+        for (auto &v : tf) v = v * std::conj(v);
+    return tf;
+}
+
+std::vector<double> taper_zeropad(const double *x, int nin, int nout, int ntaper, bool end_only) {
+    std::vector<double> out(nout, 0.0), win(nin, 1.0);
+    const float D = 3.3f;
+    if (ntaper > 0) {
+        if (!end_only)
+            for (int i = 1; i <= ntaper; ++i) {
+                float q = D * (float)(ntaper - i + 1) / (float)ntaper;
+                win[i - 1] = (double)expf(-(q * q));
+            }
+        for (int i = 0; i < ntaper; ++i) {
+            float q = D * (float)(ntaper - i) / (float)ntaper;
+            win[nin - 1 - i] = (double)expf(-(q * q));
+        }
+    }
+    for (int i = 0; i < nin; ++i) out[i] = x[i] * win[i];
+    return out;
+}
+
+std::vector<double> cumsum_trap(const double *x, int n, double dt) {
+    std::vector<double> c(n);
+    c[0] = x[0];
+    if (n > 1) c[1] = c[0] + x[1];
+    for (int i = 2; i < n; ++i) c[i] = c[i - 1] + x[i - 1] + x[i];
+    for (auto &v : c) v = v * dt * 0.5;
+    return c;
+}
+
+// ---- spectral basis: GLL / GLJ(0,1) nodes and Lagrange derivative matrices ---------
+void jacobi(int n, double a, double b, double x, double &p, double &dp) {
+    auto val = [](int n_, double a_, double b_, double x_) {
+        double p0 = 1.0, p1 = 0.5 * ((a_ + b_ + 2.0) * x_ + (a_ - b_)), pn = n_ == 0 ? 1.0 : p1;
+        for (int k = 2; k <= n_; ++k) {
+            double c = 2.0 * k + a_ + b_;
+            double a1 = 2.0 * k * (k + a_ + b_) * (c - 2.0), a2 = (c - 1.0) * (a_ * a_ - b_ * b_);
+            double a3 = (c - 2.0) * (c - 1.0) * c, a4 = 2.0 * (k + a_ - 1.0) * (k + b_ - 1.0) * c;
+            pn = ((a2 + a3 * x_) * p1 - a4 * p0) / a1;
+            p0 = p1; p1 = pn;
+        }
+        return pn;
+    };
+    p = val(n, a, b, x);
+    dp = n == 0 ? 0.0 : 0.5 * (n + a + b + 1.0) * val(n - 1, a + 1, b + 1, x);
+}
+void lobatto(int n, double a, double b, double *x) {
+    x[0] = -1.0; x[n] = 1.0;
+    int m = n - 1;
+    for (int k = 0; k < m; ++k) {
+        double r = -std::cos((2.0 * k + 1.0) * M_PI / (2.0 * m));
+        if (k > 0) r = 0.5 * (r + x[k]);
+        for (int it = 0; it < 100; ++it) {
+            double p, dp, s = 0.0;
+            for (int i = 0; i < k; ++i) s += 1.0 / (r - x[i + 1]);
+            jacobi(m, a + 1, b + 1, r, p, dp);
+            double d = -p / (dp - s * p);
+            r += d;
+            if (std::fabs(d) < 1e-17) break;
+        }
+        x[k + 1] = r;
+    }
+}
+void deriv_matrix(const double *x, double *G) {
+    double lam[5];
+    for (int i = 0; i < 5; ++i) {
+        double p = 1.0;
+        for (int m = 0; m < 5; ++m) if (m != i) p *= (x[i] - x[m]);
+        lam[i] = 1.0 / p;
+    }
+    for (int j = 0; j < 5; ++j)
+        for (int i = 0; i < 5; ++i) {
+            double v = 0.0;
+            if (i == j) { for (int m = 0; m < 5; ++m) if (m != i) v += 1.0 / (x[i] - x[m]); }
+            else v = lam[i] / (lam[j] * (x[j] - x[i]));
+            G[i + 5 * j] = v;
+        }
+}
+
+// ---- element constants (same arithmetic as finite_elem_mapping.f90's helpers) ------
+void element_constants(const float *ms, const float *mz, const int *fem, int type, double *E) {
+    for (int c = 0; c < 4; ++c) { E[c] = (double)ms[fem[c]]; E[4 + c] = (double)mz[fem[c]]; }
+    for (int c = 8; c < 16; ++c) E[c] = 0.0;
+    auto ab = [](double &a, double &b, double s1, double z1, double s2, double z2) {
+        a = std::sqrt(std::fabs((s2 * s2 * (z1 * z1) - z2 * z2 * (s1 * s1)) / (z1 * z1 - z2 * z2)));
+        b = std::sqrt(std::fabs((z1 * z1 * (s2 * s2) - z2 * z2 * (s1 * s1)) / (s2 * s2 - s1 * s1)));
+    };
+    auto theta = [](double s, double z, double a, double b) {
+        if (s >= 0.0) return std::atan2(z * a, s * b);
+        if (z > 0) return kPi / 2;
+        if (z < 0) return -kPi / 2;
+        return 0.0;
+    };
+    if (type == 0) {
+        for (int i = 0; i < 4; ++i) {
+            double r = std::sqrt(E[i] * E[i] + E[4 + i] * E[4 + i]);
+            E[8 + i] = r;
+            E[12 + i] = r != 0.0 ? std::acos(E[4 + i] / r) : 0.0;
+        }
+    } else if (type == 2) {
+        double a, b;
+        ab(a, b, E[3], E[7], E[2], E[6]);
+        double t3 = theta(E[2], E[6], a, b), t4 = theta(E[3], E[7], a, b);
+        E[8] = a; E[9] = b; E[10] = (t4 + t3) / 2; E[11] = t3 - t4;
+    } else if (type == 3) {
+        double a, b;
+        ab(a, b, E[0], E[4], E[1], E[5]);
+        double t2 = theta(E[1], E[5], a, b), t1 = theta(E[0], E[4], a, b);
+        E[8] = a; E[9] = b; E[10] = (t1 + t2) / 2; E[11] = t2 - t1;
+    }
+}
+
+double norm2_3(double a, double b, double c) {
+    double v[3] = {a, b, c}, scale = 0.0, res = 1.0;
+    for (double x : v)
+        if (x != 0.0) {
+            double ax = std::fabs(x);
+            if (scale < ax) { double q = scale / ax; res = 1.0 + res * q * q; scale = ax; }
+            else { double q = ax / scale; res = res + q * q; }
+        }
+    return std::sqrt(res) * scale;
+}
+
+double tet_volume(const double *v) {
+    __float128 a[4][4];
+    for (int i = 0; i < 4; ++i) {
+        for (int k = 0; k < 3; ++k) a[i][k] = v[i * 3 + k];
+        a[i][3] = 1;
+    }
+#define A(r, c) a[(r) - 1][(c) - 1]
+    __float128 d =
+          A(1,1) * ( A(2,2) * ( A(3,3) * A(4,4) - A(3,4) * A(4,3) )
+                   - A(2,3) * ( A(3,2) * A(4,4) - A(3,4) * A(4,2) )
+                   + A(2,4) * ( A(3,2) * A(4,3) - A(3,3) * A(4,2) ) )
+        - A(1,2) * ( A(2,1) * ( A(3,3) * A(4,4) - A(3,4) * A(4,3) )
+                   - A(2,3) * ( A(3,1) * A(4,4) - A(3,4) * A(4,1) )
+                   + A(2,4) * ( A(3,1) * A(4,3) - A(3,3) * A(4,1) ) )
+        + A(1,3) * ( A(2,1) * ( A(3,2) * A(4,4) - A(3,4) * A(4,2) )
+                   - A(2,2) * ( A(3,1) * A(4,4) - A(3,4) * A(4,1) )
+                   + A(2,4) * ( A(3,1) * A(4,2) - A(3,2) * A(4,1) ) )
+        - A(1,4) * ( A(2,1) * ( A(3,2) * A(4,3) - A(3,3) * A(4,2) )
+                   - A(2,2) * ( A(3,1) * A(4,3) - A(3,3) * A(4,1) )
+                   + A(2,3) * ( A(3,1) * A(4,2) - A(3,2) * A(4,1) ) );
+#undef A
+    return std::fabs((double)d) / 6.0;
+}
+
+struct DbDev {
+    DevDb d{};
+    float *u_owned = nullptr;      // device copy of the snapshots (null if caller-resident)
+    DBuf<float> mp_s, mp_z;
+    DBuf<double> econ;
+    DBuf<int> eltype, axis, cell_start, cell_elems;
+    std::vector<double> econ_host;
+    std::vector<int> eltype_host, axis_host;
+    int nvar = 0;
+};
+
+struct State {
+    bool init = false;
+    mck_setup_t S{};
+    PathConf pc{};
+    DbDev fwd, bwd;
+    int R = 0, nk = 0, nframes = 0;
+    DBuf<DevFrame> frames;
+    std::vector<DevFrame> frames_host;
+    DBuf<float2> tw, tw2, H;
+    DBuf<RecDesc> recs;
+    DBuf<KernDesc> kerns;
+    DBuf<float> model;
+    std::vector<int> k_rec, k_nbk, rec_first, rec_count;
+    DBuf<int> d_k_rec, d_k_nbk, d_rec_first, d_rec_count;
+    std::vector<double> t_last_p, t_last_s, rec_r;  // rec_r [R][3]
+    double src_r[3];
+    // per-task and per-chunk buffers
+    DBuf<double> tet, volume, allowed_var;
+    DBuf<int64_t> hbase;
+    DBuf<uint8_t> masked, live, working, active;
+    DBuf<int> work_list, nwork;
+    DBuf<double> fsum, f2sum, integral, variance;
+    DBuf<uint8_t> conv, done;
+    DBuf<int> niter, draws, steps;
+    DBuf<int64_t> nmodels;
+    DBuf<double> xyz;
+    DBuf<int> model_idx;
+    DBuf<Located> loc;
+    DBuf<float2> Fspec, Bspec;
+    DBuf<double> phys;
+    DBuf<unsigned long long> evals;
+    DBuf<int> err;
+    DBuf<unsigned int> elem_mark;
+    DBuf<unsigned long long> elem_count;
+    int chunk_points = 0;
+    hipStream_t stream = nullptr;
+    mck_stats_t stats{};
+};
+State g;
+
+int upload_db(DbDev &D, const mck_db_t &db, int nt, const char *name) {
+    if (db.npol != 4) return fail(MCK_EINVAL, std::string(name) + ": only npol = 4 is supported");
+    if (db.nelem <= 0 || !db.snapshots || !db.mp_s || !db.fem_mesh)
+        return fail(MCK_EINVAL, std::string(name) + ": incomplete mesh");
+    const int ne = db.nelem;
+    D.nvar = db.nvar;
+    // element constants and grid
+    D.econ_host.assign((size_t)ne * 16, 0.0);
+    D.eltype_host.assign(db.eltype, db.eltype + ne);
+    D.axis_host.assign(db.axis, db.axis + ne);
+    for (int e = 0; e < ne; ++e) {
+        int t = db.eltype[e];
+        if (t < 0 || t > 3) return fail(MCK_EINVAL, std::string(name) + ": unknown element type");
+        element_constants(db.mesh_s, db.mesh_z, db.fem_mesh + 4 * e, t, &D.econ_host[(size_t)e * 16]);
+    }
+    float smin = 1e30f, smax = -1e30f, zmin = 1e30f, zmax = -1e30f;
+    for (int e = 0; e < ne; ++e) {
+        smin = std::min(smin, db.mp_s[e]); smax = std::max(smax, db.mp_s[e]);
+        zmin = std::min(zmin, db.mp_z[e]); zmax = std::max(zmax, db.mp_z[e]);
+    }
+    double area = std::max(1.0, (double)(smax - smin)) * std::max(1.0, (double)(zmax - zmin));
+    double h = std::sqrt(area * 2.0 / ne);
+    int gnx = std::max(1, (int)std::ceil((smax - smin) / h) + 1);
+    int gnz = std::max(1, (int)std::ceil((zmax - zmin) / h) + 1);
+    std::vector<int> cnt((size_t)gnx * gnz + 1, 0), cell(ne);
+    for (int e = 0; e < ne; ++e) {
+        int ix = std::min(gnx - 1, std::max(0, (int)std::floor((db.mp_s[e] - smin) / h)));
+        int iz = std::min(gnz - 1, std::max(0, (int)std::floor((db.mp_z[e] - zmin) / h)));
+        cell[e] = iz * gnx + ix;
+        cnt[cell[e] + 1]++;
+    }
+    for (size_t i = 1; i < cnt.size(); ++i) cnt[i] += cnt[i - 1];
+    std::vector<int> elems(ne), fill(cnt.begin(), cnt.end() - 1);
+    for (int e = 0; e < ne; ++e) elems[fill[cell[e]]++] = e;
+    HIPCHECK(D.mp_s.ensure(ne));
+    HIPCHECK(D.mp_z.ensure(ne));
+    HIPCHECK(D.econ.ensure((size_t)ne * 16));
+    HIPCHECK(D.eltype.ensure(ne));
+    HIPCHECK(D.axis.ensure(ne));
+    HIPCHECK(D.cell_start.ensure(cnt.size()));
+    HIPCHECK(D.cell_elems.ensure(ne));
+    HIPCHECK(hipMemcpy(D.mp_s.p, db.mp_s, ne * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(D.mp_z.p, db.mp_z, ne * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(D.econ.p, D.econ_host.data(), (size_t)ne * 16 * 8, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(D.eltype.p, db.eltype, ne * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(D.axis.p, db.axis, ne * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(D.cell_start.p, cnt.data(), cnt.size() * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(D.cell_elems.p, elems.data(), ne * 4, hipMemcpyHostToDevice));
+    // snapshots: use in place when they already live in device memory
+    const size_t nbytes = (size_t)ne * db.nvar * 25 * nt * sizeof(float);
+    hipPointerAttribute_t attr;
+    bool on_device = false;
+    if (hipPointerGetAttributes(&attr, db.snapshots) == hipSuccess)
+        on_device = (attr.type == hipMemoryTypeDevice);
+    (void)hipGetLastError();
+    const float *u = db.snapshots;
+    if (!on_device) {
+        if (D.u_owned) { (void)hipFree(D.u_owned); D.u_owned = nullptr; }
+        HIPCHECK(hipMalloc((void **)&D.u_owned, nbytes));
+        HIPCHECK(hipMemcpy(D.u_owned, db.snapshots, nbytes, hipMemcpyHostToDevice));
+        u = D.u_owned;
+    }
+    D.d.nelem = ne; D.d.nvar = db.nvar; D.d.nt = nt;
+    D.d.u = u;
+    D.d.mp_s = D.mp_s.p; D.d.mp_z = D.mp_z.p;
+    D.d.econ = D.econ.p; D.d.eltype = D.eltype.p; D.d.axis = D.axis.p;
+    D.d.gnx = gnx; D.d.gnz = gnz; D.d.gs0 = smin; D.d.gz0 = zmin; D.d.gh = h;
+    D.d.cell_start = D.cell_start.p; D.d.cell_elems = D.cell_elems.p;
+    D.d.amplitude = db.amplitude;
+    return MCK_OK;
+}
+
+int needs_of(int param, int *nb) {
+    std::fill(nb, nb + 6, 0);
+    switch (param) {
+    case MCK_PARAM_LAM: case MCK_PARAM_VP: nb[0] = 1; break;
+    case MCK_PARAM_VS: nb[0] = nb[1] = 1; break;
+    case MCK_PARAM_MU: nb[1] = 1; break;
+    case MCK_PARAM_VSH: nb[0] = nb[1] = nb[4] = nb[5] = 1; break;
+    case MCK_PARAM_VSV: nb[4] = 1; break;
+    case MCK_PARAM_VPH: nb[3] = nb[5] = 1; break;
+    case MCK_PARAM_VPV: case MCK_PARAM_ETA: nb[0] = nb[3] = nb[5] = 1; break;
+    default: return -1;
+    }
+    return 0;
+}
+
+// Per-kernel window spectrum H_k = -norm df c_k S_k TF_fwd_k TF_bwd_k [TS_fwd TS_bwd],
+// the frequency-domain form of apply_filter + irfft + calc_misfit_kernel
+// (kernel.f90:410-510, 946-1020): value = sum_k Re(basekernel_k H_k).
+int build_kernels(State &G) {
+    const mck_setup_t &S = G.S;
+    const int nt = S.nt, N = 2 * nextpow2(nt), nw = N / 2 + 1;
+    const double dt = S.dt, df = 1.0 / (dt * N);
+    std::vector<double> f(nw);
+    for (int k = 0; k < nw; ++k) f[k] = df * k;
+    std::vector<std::vector<cd>> tff(S.nfilter), tfb(S.nfilter);
+    for (int i = 0; i < S.nfilter; ++i) {
+        tff[i] = filter_tf(S.filters[i], f);
+        tfb[i] = tff[i];
+        if (S.deconv_stf) {
+            // add_stfs (filtering.f90:241-470)
+            if (!S.stf_fwd) return fail(MCK_EINVAL, "deconv_stf needs stf_fwd");
+            auto Fs = rfft(taper_zeropad(S.stf_fwd, nt, N, 2, false), N, dt);
+            std::vector<double> e(nt, 0.0);
+            for (int t = 0; t < nt; ++t) e[t] = S.stf_src ? S.stf_src[t] : (t == 0 ? 1.0 : 0.0);
+            auto Fe = rfft(taper_zeropad(e.data(), nt, N, 2, true), N, dt);
+            auto lp = butter_lp(f, 1.0 / (f[nw - 1] * 0.75), 12);
+            double mf = 0, mb = 0;
+            for (int k = 0; k < nw; ++k) {
+                cd l = lp[k] * std::conj(lp[k]);
+                cd a = tff[i][k] / Fs[k] * S.fwd.amplitude;
+                cd b = a;
+                double th = 2 * kPi * f[k] * S.stf_src_shift;
+                a = a * Fe[k] * cd(std::cos(th), std::sin(th)) * l;
+                b = b * l;
+                if (std::isnan(std::abs(a))) a = 0.0;
+                if (std::isnan(std::abs(b))) b = 0.0;
+                tff[i][k] = a; tfb[i][k] = b;
+                mf = std::max(mf, std::abs(a)); mb = std::max(mb, std::abs(b));
+            }
+            if (std::abs(tff[i][nw - 1]) / mf > 1e-2 || std::abs(tfb[i][nw - 1]) / mb > 1e-2)
+                return fail(MCK_EINVAL, "filter is not vanishing fast enough for high frequencies");
+        }
+    }
+    std::vector<cd> ts(nw, 1.0);
+    if (!S.deconv_stf)
+        for (int k = 0; k < nw; ++k) {
+            double a = 2 * kPi * f[k] * S.timeshift_fwd, b = 2 * kPi * f[k] * S.timeshift_bwd;
+            ts[k] = cd(std::cos(a), std::sin(a)) * cd(std::cos(b), std::sin(b));
+        }
+    // seismograms per receiver (cut_and_add_seismogram, kernel.f90:181-385)
+    std::map<std::tuple<int, int, int, int>, int> hidx;  // (rec, filter, misfit, window) -> H
+    std::vector<std::vector<cd>> Hs;
+    std::vector<RecDesc> recs(G.R);
+    std::vector<KernDesc> kerns(G.nk);
+    for (int r = 0; r < G.R; ++r) { recs[r].nslot = 0; recs[r].first_kernel = -1; recs[r].nkernel = 0; }
+    G.k_rec.assign(G.nk, 0);
+    G.k_nbk.assign(G.nk, 0);
+    G.t_last_p.assign(G.R, 0.0);
+    G.t_last_s.assign(G.R, 0.0);
+    for (int k = 0; k < G.nk; ++k) {
+        int r = S.kernel_receiver[k], fi = S.kernel_filter[k], mf = S.kernel_misfit[k], pa = S.kernel_param[k];
+        if (r < 0 || r >= G.R || fi < 0 || fi >= S.nfilter) return fail(MCK_EINVAL, "kernel references unknown receiver/filter");
+        if (k > 0 && r < S.kernel_receiver[k - 1]) return fail(MCK_EINVAL, "kernels must be grouped by receiver");
+        if (pa == MCK_PARAM_RHO) return fail(MCK_EUNSUPPORTED, "density kernels are not implemented (kernel.f90:554)");
+        int nb[6];
+        if (needs_of(pa, nb)) return fail(MCK_EINVAL, "unknown model parameter");
+        if (!G.pc.full && (nb[1] || nb[3] || nb[4] || nb[5]))
+            return fail(MCK_EINVAL, "kernel needs the full strain tensor");
+        RecDesc &rd = recs[r];
+        if (rd.first_kernel < 0) rd.first_kernel = k;
+        rd.nkernel++;
+        G.k_rec[k] = r;
+        for (int b = 0; b < 6; ++b) G.k_nbk[k] += nb[b];
+        double tw0 = S.kernel_window[2 * k], tw1 = S.kernel_window[2 * k + 1];
+        if (pa == MCK_PARAM_VP || pa == MCK_PARAM_VPH || pa == MCK_PARAM_VPV || pa == MCK_PARAM_LAM)
+            G.t_last_p[r] = std::max(G.t_last_p[r], tw1);
+        else
+            G.t_last_s[r] = std::max(G.t_last_s[r], tw1);
+        if (tw0 < 0.0 || tw1 > (N - 1) * dt || tw1 - tw0 <= 0)
+            return fail(MCK_EINVAL, "kernel time window outside the seismogram");
+        int ifirst = -1, ncut = 0;
+        for (int i = 0; i < N; ++i) {
+            double t = i * dt;
+            if (t <= tw1 && t >= tw0) { if (ifirst < 0) ifirst = i; ++ncut; }
+            if (t > tw1) break;
+        }
+        if (ncut == 0) return fail(MCK_EINVAL, "kernel time window length is zero");
+        auto key = std::make_tuple(r, fi, mf, ifirst * 100000 + ncut);
+        int h;
+        auto it = hidx.find(key);
+        if (it != hidx.end()) h = it->second;
+        else {
+            const double *seis = S.seis + (size_t)r * nt;
+            std::vector<double> x = (mf == MCK_MISFIT_AM) ? cumsum_trap(seis, nt, dt)
+                                                          : std::vector<double>(seis, seis + nt);
+            auto X = rfft(x, N, dt);
+            for (int q = 0; q < nw; ++q) {
+                if (!S.deconv_stf) {
+                    double a = 2 * kPi * f[q] * S.timeshift_fwd;
+                    X[q] *= cd(std::cos(a), std::sin(a));
+                }
+                X[q] *= tff[fi][q];
+            }
+            auto xs = irfft(X, N, df);
+            std::vector<double> w(ncut);
+            double norm_term = 0.0;
+            for (int n = 0; n < ncut; ++n) norm_term += xs[ifirst + n] * xs[ifirst + n];
+            norm_term *= dt;
+            double norm = norm_term > 1e-100 ? 1.0 / norm_term : 0.0;
+            for (int n = 0; n < ncut; ++n) {
+                double om = 1.0;
+                if (S.int_scheme == MCK_INT_TRAPEZOIDAL) {
+                    om = 0.0;
+                    if (n == 0) om += 0.5;
+                    if (n == ncut - 1) om += 0.5;
+                    if (n > 0 && n < ncut - 1) om = 1.0;
+                }
+                w[n] = dt * om * xs[ifirst + n];
+            }
+            std::vector<cd> Hk(nw);
+            for (int q = 0; q < nw; ++q) {
+                cd Sk = 0.0;
+                for (int n = 0; n < ncut; ++n) {
+                    long long m = ((long long)q * (ifirst + n)) % N;
+                    double ang = 2 * kPi * (double)m / (double)N;
+                    Sk += w[n] * cd(std::cos(ang), std::sin(ang));
+                }
+                double ck = (q == 0 || q == nw - 1) ? 1.0 : 2.0;
+                Hk[q] = -norm * df * ck * Sk * tff[fi][q] * tfb[fi][q] * ts[q];
+            }
+            h = (int)Hs.size();
+            Hs.push_back(Hk);
+            hidx[key] = h;
+        }
+        KernDesc &kd = kerns[k];
+        kd.param = pa;
+        for (int b = 0; b < 6; ++b) {
+            kd.slot[b] = -1;
+            if (!nb[b]) continue;
+            int s = -1;
+            for (int q = 0; q < rd.nslot; ++q)
+                if (rd.slot_h[q] == h && rd.slot_bk[q] == b) s = q;
+            if (s < 0) {
+                if (rd.nslot >= kMaxSlots) return fail(MCK_EUNSUPPORTED, "too many distinct windows per receiver");
+                s = rd.nslot++;
+                rd.slot_h[s] = h;
+                rd.slot_bk[s] = b;
+            }
+            kd.slot[b] = s;
+        }
+    }
+    for (int r = 0; r < G.R; ++r)
+        if (recs[r].first_kernel < 0) return fail(MCK_EINVAL, "receiver without kernels");
+    std::vector<float2> Hh(Hs.size() * nw);
+    for (size_t h = 0; h < Hs.size(); ++h)
+        for (int q = 0; q < nw; ++q) Hh[h * nw + q] = make_float2((float)Hs[h][q].real(), (float)Hs[h][q].imag());
+    HIPCHECK(G.H.ensure(Hh.size()));
+    HIPCHECK(hipMemcpy(G.H.p, Hh.data(), Hh.size() * sizeof(float2), hipMemcpyHostToDevice));
+    HIPCHECK(G.recs.ensure(G.R));
+    HIPCHECK(hipMemcpy(G.recs.p, recs.data(), G.R * sizeof(RecDesc), hipMemcpyHostToDevice));
+    HIPCHECK(G.kerns.ensure(G.nk));
+    HIPCHECK(hipMemcpy(G.kerns.p, kerns.data(), G.nk * sizeof(KernDesc), hipMemcpyHostToDevice));
+    G.rec_first.resize(G.R);
+    G.rec_count.resize(G.R);
+    for (int r = 0; r < G.R; ++r) { G.rec_first[r] = recs[r].first_kernel; G.rec_count[r] = recs[r].nkernel; }
+    HIPCHECK(G.d_k_rec.ensure(G.nk));
+    HIPCHECK(G.d_k_nbk.ensure(G.nk));
+    HIPCHECK(G.d_rec_first.ensure(G.R));
+    HIPCHECK(G.d_rec_count.ensure(G.R));
+    HIPCHECK(hipMemcpy(G.d_k_rec.p, G.k_rec.data(), G.nk * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(G.d_k_nbk.p, G.k_nbk.data(), G.nk * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(G.d_rec_first.p, G.rec_first.data(), G.R * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(G.d_rec_count.p, G.rec_count.data(), G.R * 4, hipMemcpyHostToDevice));
+    return MCK_OK;
+}
+
+
+size_t strain_smem(int M) {
+    return ((sizeof(OpShared) + 15) / 16) * 16 + 2 * (size_t)M * sizeof(float2);
+}
+
+// samples per thread of the strain kernel for a trace length
+int spt_for(int nt) {
+    int spt = (nt + kStrainThreads - 1) / kStrainThreads;
+    int p = 1;
+    while (p < spt) p *= 2;
+    return p;
+}
+
+hipError_t launch_strain(State &G, int fwd, int npairs, int P, const Located *loc, int loc_stride,
+                         const uint8_t *active, float2 *out, float *trace_out) {
+    if (npairs <= 0) return hipSuccess;
+    const DbDev &D = fwd ? G.fwd : G.bwd;
+    size_t smem = strain_smem(G.pc.M);
+    dim3 grid(npairs), block(kStrainThreads);
+    switch (spt_for(G.pc.nt)) {
+#define CASE(S)                                                                                  \
+    case S:                                                                                      \
+        hipLaunchKernelGGL((k_strain_fft<S>), grid, block, smem, G.stream, D.d, G.pc, G.frames.p, \
+                           fwd, npairs, G.R, P, loc, loc_stride, active, G.tw.p, G.tw2.p, out,   \
+                           trace_out);                                                           \
+        break;
+        CASE(1) CASE(2) CASE(4) CASE(8) CASE(16)
+#undef CASE
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+int setup_tables(State &G) {
+    const mck_setup_t &S = G.S;
+    // basis
+    Basis B;
+    lobatto(4, 0.0, 0.0, B.gll);
+    lobatto(4, 0.0, 1.0, B.glj);
+    deriv_matrix(B.gll, B.G2);
+    deriv_matrix(B.glj, B.G1);
+    HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_basis), &B, sizeof(Basis)));
+    // frames: cos/sin of lon and colat, evaluated on the host
+    G.frames_host.resize(1 + G.R);
+    auto mk = [](double latd, double lond) {
+        DevFrame f;
+        double lon = lond * kDeg2Rad, colat = (90 - latd) * kDeg2Rad;
+        f.cos_lon = std::cos(lon); f.sin_lon = std::sin(lon);
+        f.cos_colat = std::cos(colat); f.sin_colat = std::sin(colat);
+        return f;
+    };
+    G.frames_host[0] = mk(S.src_lat, S.src_lon);
+    for (int r = 0; r < G.R; ++r) G.frames_host[1 + r] = mk(S.rec_lat[r], S.rec_lon[r]);
+    HIPCHECK(G.frames.ensure(1 + G.R));
+    HIPCHECK(hipMemcpy(G.frames.p, G.frames_host.data(), (1 + G.R) * sizeof(DevFrame), hipMemcpyHostToDevice));
+    // source / receiver positions (source.f90:115-131, receiver.f90 get_r)
+    {
+        double lat = S.src_lat * kDeg2Rad, lon = S.src_lon * kDeg2Rad;
+        double rad = S.planet_radius - S.src_depth_km * 1e3;
+        G.src_r[0] = std::cos(lat) * std::cos(lon) * rad;
+        G.src_r[1] = std::cos(lat) * std::sin(lon) * rad;
+        G.src_r[2] = std::sin(lat) * rad;
+        G.rec_r.resize(3 * G.R);
+        for (int r = 0; r < G.R; ++r) {
+            double la = S.rec_lat[r] * kDeg2Rad, lo = S.rec_lon[r] * kDeg2Rad;
+            G.rec_r[3 * r] = std::cos(la) * std::cos(lo) * S.planet_radius;
+            G.rec_r[3 * r + 1] = std::cos(la) * std::sin(lo) * S.planet_radius;
+            G.rec_r[3 * r + 2] = std::sin(la) * S.planet_radius;
+        }
+    }
+    // twiddles
+    const int M = G.pc.M, N = 2 * M;
+    std::vector<float2> tw(M), tw2(M + 1);
+    for (int m = 0; m < M; ++m) {
+        double a = -2.0 * M_PI * m / M;
+        tw[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    for (int k = 0; k <= M; ++k) {
+        double a = -2.0 * M_PI * k / N;
+        tw2[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    HIPCHECK(G.tw.ensure(M));
+    HIPCHECK(G.tw2.ensure(M + 1));
+    HIPCHECK(hipMemcpy(G.tw.p, tw.data(), M * sizeof(float2), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(G.tw2.p, tw2.data(), (M + 1) * sizeof(float2), hipMemcpyHostToDevice));
+    // radial model table [6][ndepth]
+    const int nd = S.model_ndepth;
+    std::vector<float> mt((size_t)6 * nd);
+    const float *src[6] = {S.model_vp, S.model_vs, S.model_rho, S.model_phi, S.model_xi, S.model_eta};
+    for (int c = 0; c < 6; ++c) {
+        if (!src[c]) return fail(MCK_EINVAL, "model table missing");
+        std::memcpy(&mt[(size_t)c * nd], src[c], nd * sizeof(float));
+    }
+    HIPCHECK(G.model.ensure(mt.size()));
+    HIPCHECK(hipMemcpy(G.model.p, mt.data(), mt.size() * 4, hipMemcpyHostToDevice));
+    return MCK_OK;
+}
+
+int ensure_chunk(State &G, int points) {
+    const int R = G.R, nd = G.pc.nd, nw = G.pc.nomega;
+    HIPCHECK(G.xyz.ensure((size_t)points * 3));
+    HIPCHECK(G.model_idx.ensure(points));
+    HIPCHECK(G.loc.ensure((size_t)points * (1 + R)));
+    HIPCHECK(G.Fspec.ensure((size_t)points * nd * nw));
+    HIPCHECK(G.Bspec.ensure((size_t)points * R * nd * nw));
+    return MCK_OK;
+}
+
+int init_impl(const mck_setup_t *s) {
+    if (!s) return fail(MCK_EINVAL, "null setup");
+    State &G = g;
+    G.S = *s;
+    const mck_setup_t &S = G.S;
+    if (S.nt < 16 || S.nt > 4096) return fail(MCK_EUNSUPPORTED, "nt must be in [16, 4096]");
+    if (S.nrec <= 0 || S.nkernel <= 0) return fail(MCK_EINVAL, "no receivers or kernels");
+    if (S.fwd.nvar != 10 || S.bwd.nvar != 5) return fail(MCK_EINVAL, "merged databases need 10 (fwd) / 5 (bwd) variables");
+    if (S.quasirandom != 1) return fail(MCK_EUNSUPPORTED, "only quasi-random (Halton) points are supported");
+    if (S.mask_src_rec) return fail(MCK_EUNSUPPORTED, "MASK_SOURCE_RECEIVER is not supported");
+    if (S.damp_radius > 0) return fail(MCK_EUNSUPPORTED, "DAMP_RADIUS_SOURCE_RECEIVER > 0 is not supported");
+    if (S.component != 'Z' && S.component != 'R' && S.component != 'T')
+        return fail(MCK_EINVAL, "component must be Z, R or T");
+    if (!G.stream) HIPCHECK(hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking));
+    G.R = S.nrec;
+    G.nk = S.nkernel;
+    G.nframes = 1 + G.R;
+    PathConf &pc = G.pc;
+    pc = PathConf{};
+    pc.nt = S.nt;
+    pc.M = nextpow2(S.nt);
+    pc.nomega = pc.M + 1;
+    pc.dt = S.dt;
+    // strain type (type_parameter.f90:477-486)
+    pc.full = 0;
+    for (int k = 0; k < S.nkernel; ++k) {
+        int p = S.kernel_param[k];
+        if (!(p == MCK_PARAM_LAM || p == MCK_PARAM_VP || p == MCK_PARAM_VPH || p == MCK_PARAM_VPV)) pc.full = 1;
+    }
+    pc.nd = pc.full ? 6 : 1;
+    pc.planet_radius = S.planet_radius;
+    for (int i = 0; i < 6; ++i) pc.mij[i] = S.src_mij[i];
+    // simulations with a non-zero azimuthal factor and the variables they read
+    const double *m = S.src_mij;
+    bool act[4] = {m[0] != 0.0, (m[1] + m[2]) != 0.0, m[3] != 0.0 || m[4] != 0.0,
+                   (m[1] - m[2]) != 0.0 || m[5] != 0.0};
+    const int v0[4] = {0, 2, 4, 7}, nv[4] = {2, 2, 3, 3};
+    pc.nsim_fwd = 0;
+    pc.nvar_fwd = 0;
+    for (int i = 0; i < 4; ++i) {
+        if (!act[i]) continue;
+        pc.fwd_sim[pc.nsim_fwd++] = i + 1;
+        for (int q = 0; q < nv[i]; ++q) pc.fwd_vars[pc.nvar_fwd++] = v0[i] + q;
+    }
+    if (pc.nsim_fwd == 0) return fail(MCK_EINVAL, "moment tensor is zero");
+    if (S.component == 'Z') {
+        pc.bwd_isim = 1; pc.nvar_bwd = 2; pc.bwd_vars[0] = 0; pc.bwd_vars[1] = 1;
+    } else {
+        pc.bwd_isim = 2; pc.nvar_bwd = 3; pc.bwd_vars[0] = 2; pc.bwd_vars[1] = 3; pc.bwd_vars[2] = 4;
+        pc.bwd_fi[0] = 0; pc.bwd_fi[1] = S.component == 'R' ? 1.0 : 0.0; pc.bwd_fi[2] = S.component == 'T' ? 1.0 : 0.0;
+    }
+    pc.ntaper = 10;
+    for (int j = 0; j < 10; ++j) {
+        float q = 3.3f * (float)(10 - j) / 10.0f;
+        pc.taper[j] = expf(-(q * q));
+    }
+    int rc;
+    if ((rc = upload_db(G.fwd, S.fwd, S.nt, "fwd"))) return rc;
+    if ((rc = upload_db(G.bwd, S.bwd, S.nt, "bwd"))) return rc;
+    if ((rc = setup_tables(G))) return rc;
+    if ((rc = build_kernels(G))) return rc;
+    HIPCHECK(G.evals.ensure(1));
+    HIPCHECK(G.err.ensure(1));
+    HIPCHECK(G.nwork.ensure(1));
+    // points per chunk: spectra of (1 + R) frames within a memory budget
+    size_t free_b = 0, total_b = 0;
+    HIPCHECK(hipMemGetInfo(&free_b, &total_b));
+    size_t per_point = (size_t)(1 + G.R) * pc.nd * pc.nomega * sizeof(float2) + (size_t)(1 + G.R) * sizeof(Located) + 64;
+    size_t budget = std::min<size_t>(free_b / 3, (size_t)48 << 30);
+    G.chunk_points = (int)std::max<size_t>(1024, std::min<size_t>(budget / per_point, 1 << 20));
+    G.init = true;
+    return MCK_OK;
+}
+
+struct Task {
+    int nelems = 0, P = 0;
+    std::vector<double> tet;     // [ne][4][3]
+    std::vector<double> vol, allowed_var;
+    std::vector<uint8_t> masked, live;
+    std::vector<int64_t> hbase;
+};
+
+int prepare_task(State &G, Task &T, int elem_type, int nvpe, int nelems, int nbasis, int nvert,
+                 const double *vertices, const int32_t *conn, const int64_t *halton_step,
+                 const int64_t *halton_offsets) {
+    const mck_setup_t &S = G.S;
+    if (elem_type != 4 || nvpe != 4) return fail(MCK_EUNSUPPORTED, "only tetrahedral inversion meshes are supported");
+    if (nbasis != 1) return fail(MCK_EUNSUPPORTED, "only volumetric basis functions are supported");
+    if (nelems <= 0) return fail(MCK_EINVAL, "empty task");
+    T.nelems = nelems;
+    T.P = S.npoints_per_step;
+    T.tet.assign((size_t)nelems * 12, 0.0);
+    T.vol.assign(nelems, 1.0);
+    T.allowed_var.assign(nelems, 0.0);
+    T.masked.assign((size_t)nelems * G.R, 0);
+    T.live.assign(nelems, 1);
+    T.hbase.assign(nelems, 0);
+    const int64_t stride = (int64_t)(S.max_iter + 1) * T.P;
+    for (int e = 0; e < nelems; ++e) {
+        if (conn[4 * e] <= 0) { T.live[e] = 0; continue; }
+        for (int i = 0; i < 4; ++i) {
+            int vid = conn[4 * e + i] - 1;
+            if (vid < 0 || vid >= nvert) return fail(MCK_EINVAL, "connectivity out of range");
+            for (int c = 0; c < 3; ++c) T.tet[(size_t)e * 12 + 3 * i + c] = vertices[3 * vid + c];
+        }
+        const double *v = &T.tet[(size_t)e * 12];
+        double vol = S.int_over_volume ? tet_volume(v) : 1.0;
+        T.vol[e] = vol;
+        double ae = S.allowed_error * vol;
+        T.allowed_var[e] = ae * ae;
+        double c[3];
+        for (int i = 0; i < 3; ++i) c[i] = (v[i] + v[3 + i] + v[6 + i] + v[9 + i]) / 4.0;
+        // travel_time_larger_Tmax (slave.f90:928-950)
+        for (int r = 0; r < G.R; ++r) {
+            double d = norm2_3(G.src_r[0] - c[0], G.src_r[1] - c[1], G.src_r[2] - c[2])
+                     + norm2_3(G.rec_r[3 * r] - c[0], G.rec_r[3 * r + 1] - c[1], G.rec_r[3 * r + 2] - c[2]);
+            double tp = d / (double)S.vp_max, ts = d / (double)S.vs_max;
+            T.masked[(size_t)e * G.R + r] = (tp > G.t_last_p[r] && ts > G.t_last_s[r]) ? 1 : 0;
+        }
+        T.hbase[e] = halton_offsets ? halton_offsets[e] : (*halton_step + (int64_t)e * stride);
+    }
+    const int ne = nelems, nk = G.nk;
+    HIPCHECK(G.tet.ensure((size_t)ne * 12));
+    HIPCHECK(G.volume.ensure(ne));
+    HIPCHECK(G.allowed_var.ensure(ne));
+    HIPCHECK(G.hbase.ensure(ne));
+    HIPCHECK(G.masked.ensure((size_t)ne * G.R));
+    HIPCHECK(G.live.ensure(ne));
+    HIPCHECK(G.working.ensure(ne));
+    HIPCHECK(G.work_list.ensure(ne));
+    HIPCHECK(G.active.ensure((size_t)ne * G.R));
+    HIPCHECK(G.fsum.ensure((size_t)ne * nk));
+    HIPCHECK(G.f2sum.ensure((size_t)ne * nk));
+    HIPCHECK(G.integral.ensure((size_t)ne * nk));
+    HIPCHECK(G.variance.ensure((size_t)ne * nk));
+    HIPCHECK(G.conv.ensure((size_t)ne * nk));
+    HIPCHECK(G.niter.ensure((size_t)ne * nk));
+    HIPCHECK(G.nmodels.ensure(ne));
+    HIPCHECK(G.draws.ensure(ne));
+    HIPCHECK(G.done.ensure(ne));
+    HIPCHECK(G.steps.ensure(ne));
+    hipStream_t st = G.stream;
+    HIPCHECK(hipMemcpyAsync(G.tet.p, T.tet.data(), T.tet.size() * 8, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(G.volume.p, T.vol.data(), ne * 8, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(G.allowed_var.p, T.allowed_var.data(), ne * 8, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(G.hbase.p, T.hbase.data(), ne * 8, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(G.masked.p, T.masked.data(), T.masked.size(), hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(G.live.p, T.live.data(), ne, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemsetAsync(G.fsum.p, 0, (size_t)ne * nk * 8, st));
+    HIPCHECK(hipMemsetAsync(G.f2sum.p, 0, (size_t)ne * nk * 8, st));
+    HIPCHECK(hipMemsetAsync(G.integral.p, 0, (size_t)ne * nk * 8, st));
+    HIPCHECK(hipMemsetAsync(G.variance.p, 0, (size_t)ne * nk * 8, st));
+    HIPCHECK(hipMemsetAsync(G.conv.p, 0, (size_t)ne * nk, st));
+    HIPCHECK(hipMemsetAsync(G.niter.p, 0, (size_t)ne * nk * 4, st));
+    HIPCHECK(hipMemsetAsync(G.nmodels.p, 0, ne * 8, st));
+    HIPCHECK(hipMemsetAsync(G.draws.p, 0, ne * 4, st));
+    HIPCHECK(hipMemsetAsync(G.done.p, 0, ne, st));
+    HIPCHECK(hipMemsetAsync(G.steps.p, 0, ne * 4, st));
+    HIPCHECK(hipMemsetAsync(G.evals.p, 0, 8, st));
+    HIPCHECK(hipMemsetAsync(G.err.p, 0, 4, st));
+    return MCK_OK;
+}
+
+VoxState vox_state(State &G) {
+    VoxState v;
+    v.fsum = G.fsum.p; v.f2sum = G.f2sum.p; v.integral = G.integral.p; v.variance = G.variance.p;
+    v.conv = G.conv.p; v.niter = G.niter.p; v.nmodels = G.nmodels.p; v.draws = G.draws.p;
+    v.done = G.done.p; v.steps = G.steps.p;
+    return v;
+}
+
+// Evaluates the MC points of working voxels [w0, w0 + nw) of the current step; writes
+// physical kernel values phys[(w0 + w) * P + p][k].
+int eval_chunk(State &G, int P, int w0, int nw, std::vector<std::pair<hipEvent_t, hipEvent_t>> &evs) {
+    const int np = nw * P, R = G.R, nf = 1 + R;
+    const int *wl = G.work_list.p + w0;
+    const uint8_t *act = G.active.p + (size_t)w0 * R;
+    hipStream_t st = G.stream;
+    int rc;
+    if ((rc = ensure_chunk(G, np))) return rc;
+    hipLaunchKernelGGL(k_points, dim3((np + 255) / 256), dim3(256), 0, st, nw, P, wl, G.hbase.p,
+                       G.draws.p, G.tet.p, G.S.model_ndepth, G.S.model_dr, G.xyz.p, G.model_idx.p);
+    HIPCHECK(hipGetLastError());
+    const long long nloc = (long long)np * nf;
+    // fwd and bwd meshes may differ: locate frame 0 in the fwd mesh, frames 1.. in the bwd mesh
+    hipLaunchKernelGGL(k_locate, dim3((unsigned)((np + 127) / 128)), dim3(128), 0, st, G.fwd.d,
+                       G.frames.p, 0, 1, np, G.xyz.p, G.S.planet_radius, nullptr, P, R,
+                       G.loc.p + (size_t)G.chunk_points * 0, G.err.p);
+    (void)nloc;
+    HIPCHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_locate, dim3((unsigned)(((long long)np * R + 127) / 128)), dim3(128), 0, st,
+                       G.bwd.d, G.frames.p, 1, R, np, G.xyz.p, G.S.planet_radius, act, P, R,
+                       G.loc.p + (size_t)np, G.err.p);
+    HIPCHECK(hipGetLastError());
+    hipEvent_t a, b;
+    HIPCHECK(hipEventCreate(&a));
+    HIPCHECK(hipEventCreate(&b));
+    HIPCHECK(hipEventRecord(a, st));
+    HIPCHECK(launch_strain(G, 1, np, P, G.loc.p, 1, nullptr, G.Fspec.p, nullptr));
+    HIPCHECK(hipEventRecord(b, st));
+    evs.push_back({a, b});
+    HIPCHECK(hipEventCreate(&a));
+    HIPCHECK(hipEventCreate(&b));
+    HIPCHECK(hipEventRecord(a, st));
+    HIPCHECK(launch_strain(G, 0, np * R, P, G.loc.p + (size_t)np - 1, R, act, G.Bspec.p, nullptr));
+    HIPCHECK(hipEventRecord(b, st));
+    evs.push_back({a, b});
+    hipLaunchKernelGGL(k_eval, dim3(np * R), dim3(kEvalThreads), 0, st, G.pc, R, np * R, P, G.Fspec.p,
+                       G.Bspec.p, act, G.recs.p, G.kerns.p, G.H.p, G.model_idx.p, G.model.p,
+                       G.S.model_ndepth, G.S.relative_kernel, G.nk,
+                       G.phys.p + (size_t)w0 * P * G.nk, nullptr);
+    HIPCHECK(hipGetLastError());
+    G.stats.strain_launches += 2;
+    return MCK_OK;
+}
+
+int collect_events(State &G, std::vector<std::pair<hipEvent_t, hipEvent_t>> &evs) {
+    for (size_t i = 0; i < evs.size(); ++i) {
+        float ms = 0.f;
+        HIPCHECK(hipEventSynchronize(evs[i].second));
+        HIPCHECK(hipEventElapsedTime(&ms, evs[i].first, evs[i].second));
+        if (i % 2 == 0) G.stats.strain_fwd_ms += ms; else G.stats.strain_bwd_ms += ms;
+        (void)hipEventDestroy(evs[i].first);
+        (void)hipEventDestroy(evs[i].second);
+    }
+    evs.clear();
+    return MCK_OK;
+}
+
+// One MC step of every voxel that is still iterating.  Returns the number of voxels
+// evaluated in *nwork_out.
+int mc_step(State &G, const Task &T, int *nwork_out, std::vector<std::pair<hipEvent_t, hipEvent_t>> &evs) {
+    const int ne = T.nelems, P = T.P, R = G.R, nk = G.nk;
+    hipStream_t st = G.stream;
+    VoxState vs = vox_state(G);
+    hipLaunchKernelGGL(k_step_begin, dim3((ne + 255) / 256), dim3(256), 0, st, ne, nk, G.S.max_iter, vs,
+                       G.live.p, G.working.p);
+    HIPCHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, st, ne, G.working.p, G.work_list.p, G.nwork.p);
+    HIPCHECK(hipGetLastError());
+    int nwork = 0;
+    HIPCHECK(hipMemcpyAsync(&nwork, G.nwork.p, 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    *nwork_out = nwork;
+    if (nwork == 0) return MCK_OK;
+    hipLaunchKernelGGL(k_receiver_active, dim3((nwork * R + 255) / 256), dim3(256), 0, st, nwork, R, nk,
+                       G.work_list.p, G.masked.p, G.d_rec_first.p, G.d_rec_count.p, vs, G.active.p);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(G.phys.ensure((size_t)nwork * P * nk));
+    const int wc = std::max(1, G.chunk_points / P);
+    int rc;
+    for (int w0 = 0; w0 < nwork; w0 += wc)
+        if ((rc = eval_chunk(G, P, w0, std::min(wc, nwork - w0), evs))) return rc;
+    hipLaunchKernelGGL(k_mc, dim3(nwork), dim3(256), 0, st, nwork, P, nk, R, G.work_list.p, G.phys.p,
+                       G.masked.p, G.active.p, G.d_k_rec.p, G.d_k_nbk.p, G.volume.p, G.allowed_var.p,
+                       G.S.allowed_relative_error, vs, G.evals.p, G.err.p);
+    HIPCHECK(hipGetLastError());
+    G.stats.points += (int64_t)nwork * P;
+    G.stats.mc_steps += 1;
+    return MCK_OK;
+}
+
+int finish_task(State &G, const Task &T, double *values, double *variance, int32_t *niterations,
+                double *ctime, int64_t *halton_step, const int64_t *halton_offsets, double dev_ms) {
+    const int ne = T.nelems, nk = G.nk;
+    hipStream_t st = G.stream;
+    std::vector<int> draws(ne), steps(ne);
+    unsigned long long ev = 0;
+    int err = 0;
+    HIPCHECK(hipMemcpyAsync(values, G.integral.p, (size_t)ne * nk * 8, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(variance, G.variance.p, (size_t)ne * nk * 8, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(niterations, G.niter.p, (size_t)ne * nk * 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(draws.data(), G.draws.p, ne * 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(steps.data(), G.steps.p, ne * 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(&ev, G.evals.p, 8, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(&err, G.err.p, 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    G.stats.evaluations = (int64_t)ev;
+    if (err & 1) return fail(MCK_ENOTFOUND, "element not found: a point is outside the 12 nearest elements");
+    if (err & 2) return fail(MCK_ENAN, "NaN in a Monte Carlo integral");
+    long long tot_steps = 0;
+    for (int e = 0; e < ne; ++e) tot_steps += steps[e];
+    for (int e = 0; e < ne; ++e) {
+        if (!T.live[e]) {
+            for (int k = 0; k < nk; ++k) { values[(size_t)e * nk + k] = 0; variance[(size_t)e * nk + k] = 0; niterations[(size_t)e * nk + k] = 0; }
+            if (ctime) ctime[e] = 0.0;
+            continue;
+        }
+        if (ctime) ctime[e] = tot_steps ? dev_ms * 1e-3 * steps[e] / (double)tot_steps : 0.0;
+    }
+    // the Halton stream position after the task
+    if (halton_offsets) {
+        int last = -1;
+        for (int e = 0; e < ne; ++e) if (T.live[e]) last = e;
+        if (last >= 0) *halton_step = T.hbase[last] + (int64_t)draws[last] * T.P;
+    } else {
+        *halton_step += (int64_t)ne * (G.S.max_iter + 1) * T.P;
+    }
+    return MCK_OK;
+}
+
+int slave_work_impl(bool plain, int32_t elem_type, int32_t nvpe, int32_t nelems, int32_t nbasis,
+                    int32_t nvertices, const double *vertices, const int32_t *conn, double *values,
+                    double *variance, int32_t *niterations, double *ctime, int64_t *halton_step,
+                    const int64_t *halton_offsets) {
+    State &G = g;
+    if (!G.init) return fail(MCK_ENOTINIT, "mck_init has not been called");
+    if (!vertices || !conn || !values || !variance || !niterations || !halton_step)
+        return fail(MCK_EINVAL, "null argument");
+    auto t0 = std::chrono::steady_clock::now();
+    G.stats = mck_stats_t{};
+    Task T;
+    int rc;
+    if ((rc = prepare_task(G, T, elem_type, nvpe, nelems, nbasis, nvertices, vertices, conn, halton_step,
+                           halton_offsets)))
+        return rc;
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    HIPCHECK(hipEventRecord(e0, G.stream));
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
+    if (!plain) {
+        for (;;) {
+            int nwork = 0;
+            if ((rc = mc_step(G, T, &nwork, evs))) return rc;
+            if ((rc = collect_events(G, evs))) return rc;
+            if (nwork == 0) break;
+        }
+    } else {
+        // reference loop order: one element at a time, one MC step at a time
+        std::vector<uint8_t> only(nelems, 0);
+        for (int e = 0; e < nelems; ++e) {
+            if (!T.live[e]) continue;
+            only.assign(nelems, 0);
+            only[e] = 1;
+            HIPCHECK(hipMemcpyAsync(G.live.p, only.data(), nelems, hipMemcpyHostToDevice, G.stream));
+            for (;;) {
+                int nwork = 0;
+                if ((rc = mc_step(G, T, &nwork, evs))) return rc;
+                if ((rc = collect_events(G, evs))) return rc;
+                if (nwork == 0) break;
+            }
+        }
+    }
+    HIPCHECK(hipEventRecord(e1, G.stream));
+    HIPCHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    G.stats.device_ms = ms;
+    if ((rc = finish_task(G, T, values, variance, niterations, ctime, halton_step, halton_offsets, ms))) return rc;
+    G.stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return MCK_OK;
+}
+
+// point-level evaluation shared by mck_locate_points / mck_load_strain_points /
+// mck_base_kernels: npts points, all receivers active
+int points_upload(State &G, int npts, const double *xyz) {
+    int rc;
+    if ((rc = ensure_chunk(G, npts))) return rc;
+    HIPCHECK(hipMemcpyAsync(G.xyz.p, xyz, (size_t)npts * 24, hipMemcpyHostToDevice, G.stream));
+    HIPCHECK(hipMemsetAsync(G.err.p, 0, 4, G.stream));
+    return MCK_OK;
+}
+
+int locate_frame(State &G, int frame, int npts) {
+    const DbDev &D = frame == 0 ? G.fwd : G.bwd;
+    // results for frame f of point p at loc[p] (stride 1)
+    hipLaunchKernelGGL(k_locate, dim3((npts + 127) / 128), dim3(128), 0, G.stream, D.d, G.frames.p, frame,
+                       1, npts, G.xyz.p, G.S.planet_radius, nullptr, 1, G.R, G.loc.p, G.err.p);
+    HIPCHECK(hipGetLastError());
+    return MCK_OK;
+}
+
+}  // namespace
+
+// =====================================================================================
+// C-ABI
+// =====================================================================================
+extern "C" {
+
+int mck_init(const mck_setup_t *setup) {
+    if (g.init) mck_finalize();
+    int rc = init_impl(setup);
+    if (rc) g.init = false;
+    return rc;
+}
+
+int mck_slave_work(int32_t elem_type, int32_t nvpe, int32_t nelems, int32_t nbasis, int32_t nvertices,
+                   const double *vertices, const int32_t *conn, double *kernel_values,
+                   double *kernel_variance, int32_t *niterations, double *computation_time,
+                   int64_t *halton_step, const int64_t *halton_offsets) {
+    return slave_work_impl(false, elem_type, nvpe, nelems, nbasis, nvertices, vertices, conn, kernel_values,
+                           kernel_variance, niterations, computation_time, halton_step, halton_offsets);
+}
+
+int mck_slave_work_plain(int32_t elem_type, int32_t nvpe, int32_t nelems, int32_t nbasis,
+                         int32_t nvertices, const double *vertices, const int32_t *conn,
+                         double *kernel_values, double *kernel_variance, int32_t *niterations,
+                         double *computation_time, int64_t *halton_step,
+                         const int64_t *halton_offsets) {
+    return slave_work_impl(true, elem_type, nvpe, nelems, nbasis, nvertices, vertices, conn, kernel_values,
+                           kernel_variance, niterations, computation_time, halton_step, halton_offsets);
+}
+
+int mck_locate_points(int32_t frame, int32_t npts, const double *xyz, int32_t *elem, double *xi,
+                      double *eta, int32_t *ties) {
+    State &G = g;
+    if (!G.init) return fail(MCK_ENOTINIT, "mck_init has not been called");
+    if (frame < 0 || frame > G.R || npts < 0 || (npts && (!xyz || !elem || !xi || !eta)))
+        return fail(MCK_EINVAL, "invalid argument");
+    if (npts == 0) return MCK_OK;
+    int rc;
+    if ((rc = points_upload(G, npts, xyz))) return rc;
+    if ((rc = locate_frame(G, frame, npts))) return rc;
+    std::vector<Located> L(npts);
+    HIPCHECK(hipMemcpyAsync(L.data(), G.loc.p, npts * sizeof(Located), hipMemcpyDeviceToHost, G.stream));
+    HIPCHECK(hipStreamSynchronize(G.stream));
+    for (int i = 0; i < npts; ++i) {
+        elem[i] = L[i].elem + 1;   // 1-based, 0 = not found
+        xi[i] = L[i].xi;
+        eta[i] = L[i].eta;
+        if (ties) ties[i] = L[i].tie;
+    }
+    return MCK_OK;
+}
+
+int mck_load_strain_points(int32_t frame, int32_t npts, const double *xyz, float *traces) {
+    State &G = g;
+    if (!G.init) return fail(MCK_ENOTINIT, "mck_init has not been called");
+    if (frame < 0 || frame > G.R || npts < 0 || (npts && (!xyz || !traces)))
+        return fail(MCK_EINVAL, "invalid argument");
+    if (npts == 0) return MCK_OK;
+    int rc;
+    if ((rc = points_upload(G, npts, xyz))) return rc;
+    if ((rc = locate_frame(G, frame, npts))) return rc;
+    float *d_tr = nullptr;
+    const size_t n = (size_t)npts * G.pc.nd * G.pc.nt;
+    HIPCHECK(hipMalloc((void **)&d_tr, n * sizeof(float)));
+    // fwd: pairs = points with frame 0; bwd: pair = p * R + r with a per-point location
+    hipError_t e;
+    if (frame == 0) {
+        e = launch_strain(G, 1, npts, 1, G.loc.p, 1, nullptr, nullptr, d_tr);
+    } else {
+        // R = 1 view: pair p -> frame index 1; locations at loc[p] (stride 1, frame offset -1)
+        int R_save = G.R;
+        G.R = 1;
+        DevFrame fsave;
+        HIPCHECK(hipMemcpy(&fsave, G.frames.p + 1, sizeof(DevFrame), hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(G.frames.p + 1, &G.frames_host[frame], sizeof(DevFrame), hipMemcpyHostToDevice));
+        e = launch_strain(G, 0, npts, 1, G.loc.p - 1, 1, nullptr, nullptr, d_tr);
+        G.R = R_save;
+        (void)hipStreamSynchronize(G.stream);
+        HIPCHECK(hipMemcpy(G.frames.p + 1, &fsave, sizeof(DevFrame), hipMemcpyHostToDevice));
+    }
+    if (e != hipSuccess) { (void)hipFree(d_tr); return fail(MCK_EHIP, hipGetErrorString(e)); }
+    HIPCHECK(hipMemcpyAsync(traces, d_tr, n * sizeof(float), hipMemcpyDeviceToHost, G.stream));
+    HIPCHECK(hipStreamSynchronize(G.stream));
+    (void)hipFree(d_tr);
+    int err = 0;
+    HIPCHECK(hipMemcpy(&err, G.err.p, 4, hipMemcpyDeviceToHost));
+    if (err & 1) return fail(MCK_ENOTFOUND, "element not found");
+    return MCK_OK;
+}
+
+int mck_base_kernels(int32_t npts, const double *xyz, double *base) {
+    State &G = g;
+    if (!G.init) return fail(MCK_ENOTINIT, "mck_init has not been called");
+    if (npts < 0 || (npts && (!xyz || !base))) return fail(MCK_EINVAL, "invalid argument");
+    if (npts == 0) return MCK_OK;
+    const int R = G.R;
+    int rc;
+    if ((rc = points_upload(G, npts, xyz))) return rc;
+    // model index of every point (k_points computes it from Halton draws; here from xyz)
+    std::vector<int> midx(npts);
+    for (int p = 0; p < npts; ++p) {
+        double r = norm2_3(xyz[3 * p], xyz[3 * p + 1], xyz[3 * p + 2]);
+        int id = (int)(r / (double)G.S.model_dr + 1);
+        id = std::max(1, std::min(G.S.model_ndepth, id));
+        midx[p] = id - 1;
+    }
+    HIPCHECK(hipMemcpyAsync(G.model_idx.p, midx.data(), npts * 4, hipMemcpyHostToDevice, G.stream));
+    hipLaunchKernelGGL(k_locate, dim3((npts + 127) / 128), dim3(128), 0, G.stream, G.fwd.d, G.frames.p, 0, 1,
+                       npts, G.xyz.p, G.S.planet_radius, nullptr, 1, R, G.loc.p, G.err.p);
+    hipLaunchKernelGGL(k_locate, dim3(((long long)npts * R + 127) / 128), dim3(128), 0, G.stream, G.bwd.d,
+                       G.frames.p, 1, R, npts, G.xyz.p, G.S.planet_radius, nullptr, 1, R, G.loc.p + npts, G.err.p);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(launch_strain(G, 1, npts, 1, G.loc.p, 1, nullptr, G.Fspec.p, nullptr));
+    HIPCHECK(launch_strain(G, 0, npts * R, 1, G.loc.p + npts - 1, R, nullptr, G.Bspec.p, nullptr));
+    double *d_base = nullptr, *d_phys = nullptr;
+    HIPCHECK(hipMalloc((void **)&d_base, (size_t)npts * G.nk * 6 * 8));
+    HIPCHECK(hipMalloc((void **)&d_phys, (size_t)npts * G.nk * 8));
+    HIPCHECK(hipMemsetAsync(d_base, 0, (size_t)npts * G.nk * 6 * 8, G.stream));
+    hipLaunchKernelGGL(k_eval, dim3(npts * R), dim3(kEvalThreads), 0, G.stream, G.pc, R, npts * R, 1, G.Fspec.p,
+                       G.Bspec.p, nullptr, G.recs.p, G.kerns.p, G.H.p, G.model_idx.p, G.model.p,
+                       G.S.model_ndepth, G.S.relative_kernel, G.nk, d_phys, d_base);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(base, d_base, (size_t)npts * G.nk * 6 * 8, hipMemcpyDeviceToHost, G.stream));
+    HIPCHECK(hipStreamSynchronize(G.stream));
+    (void)hipFree(d_base);
+    (void)hipFree(d_phys);
+    int err = 0;
+    HIPCHECK(hipMemcpy(&err, G.err.p, 4, hipMemcpyDeviceToHost));
+    if (err & 1) return fail(MCK_ENOTFOUND, "element not found");
+    return MCK_OK;
+}
+
+int mck_get_stats(mck_stats_t *stats) {
+    if (!stats) return fail(MCK_EINVAL, "null stats");
+    *stats = g.stats;
+    return MCK_OK;
+}
+
+void mck_finalize(void) {
+    State &G = g;
+    if (G.stream) (void)hipStreamSynchronize(G.stream);
+    for (DbDev *D : {&G.fwd, &G.bwd}) {
+        if (D->u_owned) (void)hipFree(D->u_owned);
+        D->u_owned = nullptr;
+        D->mp_s.release(); D->mp_z.release(); D->econ.release(); D->eltype.release();
+        D->axis.release(); D->cell_start.release(); D->cell_elems.release();
+    }
+    G.frames.release(); G.tw.release(); G.tw2.release(); G.H.release(); G.recs.release();
+    G.kerns.release(); G.model.release(); G.d_k_rec.release(); G.d_k_nbk.release();
+    G.d_rec_first.release(); G.d_rec_count.release(); G.tet.release(); G.volume.release();
+    G.allowed_var.release(); G.hbase.release(); G.masked.release(); G.live.release();
+    G.working.release(); G.active.release(); G.work_list.release(); G.nwork.release();
+    G.fsum.release(); G.f2sum.release(); G.integral.release(); G.variance.release();
+    G.conv.release(); G.done.release(); G.niter.release(); G.draws.release(); G.steps.release();
+    G.nmodels.release(); G.xyz.release(); G.model_idx.release(); G.loc.release();
+    G.Fspec.release(); G.Bspec.release(); G.phys.release(); G.evals.release(); G.err.release();
+    G.elem_mark.release(); G.elem_count.release();
+    G.init = false;
+}
+
+const char *mck_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"

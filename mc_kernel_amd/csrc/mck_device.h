@@ -1,0 +1,109 @@
+// mck_device.h - device-side data layout shared by the HIP kernels and the host driver.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mck {
+
+constexpr int kMaxSim = 4;       // fwd simulations MZZ, MXX+MYY, MXZ, MXY
+constexpr int kMaxVar = 10;      // merged variables per element (fwd)
+constexpr int kMaxKernelsPerRec = 256;
+constexpr int kNN = 12;          // nearest midpoints tested (readfields.f90:1067)
+constexpr int kStrainThreads = 256;
+constexpr int kEvalThreads = 256;
+constexpr int kMaxSlots = 64;    // distinct (window-filter, base kernel) pairs per receiver
+
+// GLL/GLJ collocation points and Lagrange derivative matrices for npol = 4
+// (calc_gradient_terms, readfields.f90:3084-3150); G[i + 5 j] = l_i'(x_j).
+struct Basis {
+    double gll[5], glj[5];
+    double G2[25], G1[25];
+};
+
+// One merged database in HBM.
+struct DevDb {
+    int nelem, nvar, nt;
+    const float *u;           // [nelem][nvar][5][5][nt]
+    const float *mp_s, *mp_z; // element midpoints
+    const double *econ;       // [nelem][16] element constants (see host setup)
+    const int *eltype, *axis;
+    // uniform grid over the midpoints for the exact 12-nearest search
+    int gnx, gnz;
+    double gs0, gz0, gh;      // origin and cell size (m)
+    const int *cell_start;    // [gnx*gnz+1]
+    const int *cell_elems;    // element ids sorted by cell
+    double amplitude;
+};
+
+// Observation frame: source (frame 0) or receiver r (frame 1 + r).
+struct DevFrame {
+    double cos_lon, sin_lon, cos_colat, sin_colat;
+};
+
+// Result of locating one point in one frame.
+struct Located {
+    int elem;                 // 0-based, -1 = not found
+    int tie;                  // equal fp32 distances among the 13 nearest
+    double xi, eta, phi;
+};
+
+// Path-wide configuration passed by value to the kernels.
+struct PathConf {
+    int nt, M, nomega;        // samples, FFT half length (N/2) and N/2+1
+    double dt;
+    int nd;                   // 1 (trace) or 6 (full strain)
+    int full;
+    double planet_radius;
+    // fwd frame: active simulations and the variables they read
+    int nsim_fwd;
+    int fwd_sim[kMaxSim];     // 1..4
+    int nvar_fwd;             // variables read from the fwd DB
+    int fwd_vars[kMaxVar];
+    double mij[6];
+    // bwd frame
+    int bwd_isim;             // 1 (Z) or 2 (R, T)
+    int nvar_bwd;
+    int bwd_vars[3];
+    double bwd_fi[3];
+    float taper[10];          // Gaussian taper samples (fft.f90:586-593), fp32
+    int ntaper;
+};
+
+// LDS layout of the per-point operator set-up
+struct OpShared {
+    double invj[25][4];     // [ipol + 5 jpol]
+    double sinv[25];        // 1 / s at the nodes
+    double li[5], lj[5];
+    double M[6][kMaxSim][6];// output comp x simulation x strain comp
+    float W[6 * kMaxVar * 25];
+    int elem;
+    int axial;
+    int type;
+};
+
+// receiver: distinct (window spectrum, base kernel) pairs ('slots') and its kernels
+struct RecDesc {
+    int nslot;
+    int first_kernel, nkernel;
+    int slot_h[kMaxSlots];
+    int slot_bk[kMaxSlots];    // 0 lambda, 1 mu, 3 a, 4 b, 5 c
+};
+
+// kernel: model parameter and the slot of every base kernel it needs (-1 = none)
+struct KernDesc {
+    int param;                 // MCK_PARAM_*
+    int slot[6];               // slot of each base kernel the parameter needs (-1 = none)
+};
+
+// Monte Carlo state of the voxels of one task
+struct VoxState {
+    double *fsum, *f2sum, *integral, *variance;   // [nvox][nk]
+    uint8_t *conv;                                // [nvox][nk]
+    int *niter;                                   // [nvox][nk]
+    int64_t *nmodels;                             // [nvox]
+    int *draws;                                   // [nvox] Halton draws so far
+    uint8_t *done;                                // [nvox]
+    int *steps;                                   // [nvox] MC steps evaluated
+};
+
+}  // namespace mck

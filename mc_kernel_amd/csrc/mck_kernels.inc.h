@@ -1,0 +1,927 @@
+// mck_kernels.hip - HIP kernels of the MC Kernel per-point kernel-evaluation and Monte
+// Carlo integration path for gfx950.
+//
+//  k_points       Halton points in tetrahedra (halton_sequence.f90:109-153,
+//                 polytopes.f90:47-115) and the radial model index of every point
+//  k_locate       point location in one frame: rotation into the source/receiver frame
+//                 (rotations.f90:737-782), clamp to the surface, exact 12 nearest element
+//                 midpoints with single-precision distances (kdtree2.f90:1541-1552) and
+//                 the Newton inverse mappings (finite_elem_mapping.f90) - bit-identical
+//                 element indices
+//  k_strain_fft   fused gather + strain + interpolation + azimuth + rotation
+//                 (sem_derivatives.f90, spectral_basis.f90:181-314, rotations.f90,
+//                 readfields.f90:1138-1465), cumsum (simple_routines.f90:314-341),
+//                 taper + zero pad (fft.f90:546-601) and LDS Stockham real FFT
+//                 (replacing FFTW in fft.f90:375-413)
+//  k_eval         fwd x bwd base kernels (kernel.f90:514-570) and the filtered Parseval
+//                 window integral (kernel.f90:410-510, 946-1020) in the frequency domain
+//                 with FP64 accumulation; physical kernels (kernel.f90:665-832)
+//  k_mc           per-voxel MC sums, variance and convergence (mc_integration.f90:69-143)
+//                 and the iteration bookkeeping of slave_work (slave.f90:501-757)
+// Included once by mck_api.hip (single translation unit).
+#pragma once
+#include "mck_device.h"
+
+namespace mck {
+
+__constant__ Basis c_basis;
+
+// ------------------------------------------------------------------------------------
+// element mappings (finite_elem_mapping.f90); E = element constants:
+//  E[0..3] corner s, E[4..7] corner z (fp32 values), spheroid: E[8..11] r, E[12..15]
+//  theta; semino/semiso: E[8] a, E[9] b, E[10] theta_bar, E[11] dtheta.
+// ------------------------------------------------------------------------------------
+__device__ inline void d_mapping(double xi, double eta, const double *E, int type, double *sz)
+{
+    if (type == 0) {
+        const double *r = E + 8, *th = E + 12;
+        sz[0] = (1 + eta) * r[3] / 2 * sin(((1 - xi) * th[3] + (1 + xi) * th[2]) / 2)
+              + (1 - eta) * r[0] / 2 * sin(((1 - xi) * th[0] + (1 + xi) * th[1]) / 2);
+        sz[1] = (1 + eta) * r[3] / 2 * cos(((1 - xi) * th[3] + (1 + xi) * th[2]) / 2)
+              + (1 - eta) * r[0] / 2 * cos(((1 - xi) * th[0] + (1 + xi) * th[1]) / 2);
+    } else if (type == 1) {
+        double xip = 1 + xi, xim = 1 - xi, etap = 1 + eta, etam = 1 - eta;
+        double shp0 = xim * etam / 4, shp1 = xip * etam / 4, shp2 = xip * etap / 4, shp3 = xim * etap / 4;
+        double s = 0.0, z = 0.0;
+        s = s + shp0 * E[0]; z = z + shp0 * E[4];
+        s = s + shp1 * E[1]; z = z + shp1 * E[5];
+        s = s + shp2 * E[2]; z = z + shp2 * E[6];
+        s = s + shp3 * E[3]; z = z + shp3 * E[7];
+        sz[0] = s; sz[1] = z;
+    } else if (type == 2) {
+        double a = E[8], b = E[9], tb = E[10], dt = E[11];
+        double sb = ((1 + xi) * E[1] + (1 - xi) * E[0]) / 2;
+        double zb = ((1 + xi) * E[5] + (1 - xi) * E[4]) / 2;
+        double st = a * cos(tb + xi * dt / 2);
+        double zt = b * sin(tb + xi * dt / 2);
+        sz[0] = (sb + st) / 2 + eta * (st - sb) / 2;
+        sz[1] = (zb + zt) / 2 + eta * (zt - zb) / 2;
+    } else {
+        double a = E[8], b = E[9], tb = E[10], dt = E[11];
+        double st = ((1 + xi) * E[2] + (1 - xi) * E[3]) / 2;
+        double zt = ((1 + xi) * E[6] + (1 - xi) * E[7]) / 2;
+        double sb = a * cos(tb + xi * dt / 2);
+        double zb = b * sin(tb + xi * dt / 2);
+        sz[0] = (sb + st) / 2 + eta * (st - sb) / 2;
+        sz[1] = (zb + zt) / 2 + eta * (zt - zb) / 2;
+    }
+}
+
+// inverse Jacobian, column-major: ij[0]=(1,1) ij[1]=(2,1) ij[2]=(1,2) ij[3]=(2,2)
+__device__ inline void d_inv_jacobian(double xi, double eta, const double *E, int type, double *ij)
+{
+    double J0, J1, J2, J3;
+    if (type == 0) {
+        const double *r = E + 8, *th = E + 12;
+        J0 = (1 + eta) * r[3] * (th[2] - th[3]) / 4 * cos(((1 - xi) * th[3] + (1 + xi) * th[2]) / 2)
+           + (1 - eta) * r[0] * (th[1] - th[0]) / 4 * cos(((1 - xi) * th[0] + (1 + xi) * th[1]) / 2);
+        J2 = 0.5 * (r[3] * sin(((1 - xi) * th[3] + (1 + xi) * th[2]) / 2)
+                  - r[0] * sin(((1 - xi) * th[0] + (1 + xi) * th[1]) / 2));
+        J1 = -(1 + eta) * r[3] * (th[2] - th[3]) / 4 * sin(((1 - xi) * th[3] + (1 + xi) * th[2]) / 2)
+           - (1 - eta) * r[0] * (th[1] - th[0]) / 4 * sin(((1 - xi) * th[0] + (1 + xi) * th[1]) / 2);
+        J3 = 0.5 * (r[3] * cos(((1 - xi) * th[3] + (1 + xi) * th[2]) / 2)
+                  - r[0] * cos(((1 - xi) * th[0] + (1 + xi) * th[1]) / 2));
+    } else if (type == 1) {
+        double xip = 1 + xi, xim = 1 - xi, etap = 1 + eta, etam = 1 - eta;
+        double d0[4] = {-etam / 4, etam / 4, etap / 4, -etap / 4};
+        double d1[4] = {-xim / 4, -xip / 4, xip / 4, xim / 4};
+        J0 = J1 = J2 = J3 = 0.0;
+        for (int i = 0; i < 4; ++i) {
+            J0 = J0 + E[i] * d0[i];
+            J2 = J2 + E[i] * d1[i];
+            J1 = J1 + E[4 + i] * d0[i];
+            J3 = J3 + E[4 + i] * d1[i];
+        }
+    } else {
+        double a = E[8], b = E[9], tb = E[10], dt = E[11];
+        double sc = a * cos(tb + xi * dt / 2), zc = b * sin(tb + xi * dt / 2);
+        double dsc = -a * dt * sin(tb + xi * dt / 2) / 2;
+        double dzc = b * dt * cos(tb + xi * dt / 2) / 2;
+        double sb, zb, st, zt, dsb, dzb, dst, dzt;
+        if (type == 2) {
+            sb = ((1 + xi) * E[1] + (1 - xi) * E[0]) / 2;
+            zb = ((1 + xi) * E[5] + (1 - xi) * E[4]) / 2;
+            st = sc; zt = zc;
+            dsb = (E[1] - E[0]) / 2;
+            dzb = (E[5] - E[4]) / 2;
+            dst = dsc; dzt = dzc;
+        } else {
+            st = ((1 + xi) * E[2] + (1 - xi) * E[3]) / 2;
+            zt = ((1 + xi) * E[6] + (1 - xi) * E[7]) / 2;
+            sb = sc; zb = zc;
+            dsb = dsc; dzb = dzc;
+            dst = (E[2] - E[3]) / 2;
+            dzt = (E[6] - E[7]) / 2;
+        }
+        J0 = (dsb + dst) / 2 + eta * (dst - dsb) / 2;
+        J2 = (st - sb) / 2;
+        J1 = (dzb + dzt) / 2 + eta * (dzt - dzb) / 2;
+        J3 = (zt - zb) / 2;
+    }
+    double det = J0 * J3 - J1 * J2;
+    ij[0] = J3 / det;
+    ij[1] = -J1 / det;
+    ij[2] = -J2 / det;
+    ij[3] = J0 / det;
+}
+
+// inv_mapping_* + inside_element (finite_elem_mapping.f90:101-141, 137-200, 403-433,
+// 505-535): 10 Newton steps from the centre; single-precision threshold 1e-7**2 except
+// for spheroids (1d-7**2).
+__device__ inline bool d_inside(double s, double z, const double *E, int type, double *xi_out,
+                                double *eta_out)
+{
+    const double thr = (type == 0) ? (1e-7 * 1e-7) : (double)(1e-7f * 1e-7f);
+    double xi = 0.0, eta = 0.0, sz[2], ij[4];
+    for (int i = 1; i <= 10; ++i) {
+        d_mapping(xi, eta, E, type, sz);
+        double ds = s - sz[0], dz = z - sz[1];
+        if ((ds * ds + dz * dz) / (s * s + z * z) < thr) break;
+        d_inv_jacobian(xi, eta, E, type, ij);
+        xi = xi + ij[0] * ds + ij[2] * dz;
+        eta = eta + ij[1] * ds + ij[3] * dz;
+    }
+    *xi_out = xi;
+    *eta_out = eta;
+    const double tol = 1e-3;
+    return (xi >= -1 - tol && xi <= 1 + tol && eta >= -1 - tol && eta <= 1 + tol);
+}
+
+// gfortran NORM2 of a 3-vector (scaled sum of squares)
+__device__ inline double d_norm2(double a, double b, double c)
+{
+    double v[3] = {a, b, c}, scale = 0.0, res = 1.0;
+    for (int i = 0; i < 3; ++i) {
+        if (v[i] != 0.0) {
+            double ax = fabs(v[i]);
+            if (scale < ax) { double q = scale / ax; res = 1.0 + res * q * q; scale = ax; }
+            else { double q = ax / scale; res = res + q * q; }
+        }
+    }
+    return sqrt(res) * scale;
+}
+
+// ------------------------------------------------------------------------------------
+// k_points: one thread per MC point of the working voxels of this chunk
+// ------------------------------------------------------------------------------------
+__global__ void k_points(int nwork, int P, const int *work_voxel, const int64_t *halton_base,
+                         const int *voxel_draws, const double *tet /* [nvox][4][3] */,
+                         int ndepth, float dr, double *xyz, int *model_idx)
+{
+    int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= nwork * P) return;
+    int w = gid / P, p = gid - w * P;
+    int v = work_voxel[w];
+    // the current draw of voxel v is its (voxel_draws[v] - 1)-th
+    int64_t idx = halton_base[v] + (int64_t)(voxel_draws[v] - 1) * P + p;
+    double c[4];
+    const int base[3] = {2, 3, 5};
+    for (int d = 0; d < 3; ++d) {
+        int64_t s2 = 20 + idx * 2;
+        double bk = (double)base[d];
+        double acc = 0.0;
+        while (s2 != 0) {
+            int64_t digit = s2 % base[d];
+            acc = acc + (double)digit / bk;   // = dp(digit * qp(1/b^k)), see halton tests
+            bk = bk * (double)base[d];
+            s2 = s2 / base[d];
+        }
+        c[d] = acc;
+    }
+    // Rocchini-Cignoni fold (polytopes.f90:88-110)
+    if (c[0] + c[1] > 1.0) { c[0] = 1.0 - c[0]; c[1] = 1.0 - c[1]; }
+    if (c[1] + c[2] > 1.0) {
+        double t = c[2];
+        c[2] = 1.0 - c[0] - c[1];
+        c[1] = 1.0 - t;
+    } else if (c[0] + c[1] + c[2] > 1.0) {
+        double t = c[2];
+        c[2] = c[0] + c[1] + c[2] - 1.0;
+        c[0] = 1.0 - c[1] - t;
+    }
+    c[3] = 1.0 - (c[0] + c[1] + c[2]);
+    const double *V = tet + (size_t)v * 12;
+    double x[3];
+    for (int i = 0; i < 3; ++i) {
+        double a = 0.0;
+        a = a + V[0 * 3 + i] * c[0];
+        a = a + V[1 * 3 + i] * c[1];
+        a = a + V[2 * 3 + i] * c[2];
+        a = a + V[3 * 3 + i] * c[3];
+        x[i] = a;
+    }
+    xyz[3 * (size_t)gid + 0] = x[0];
+    xyz[3 * (size_t)gid + 1] = x[1];
+    xyz[3 * (size_t)gid + 2] = x[2];
+    double r = d_norm2(x[0], x[1], x[2]);
+    int id = (int)(r / (double)dr + 1);
+    id = id < 1 ? 1 : (id > ndepth ? ndepth : id);
+    model_idx[gid] = id - 1;
+}
+
+// ------------------------------------------------------------------------------------
+// k_locate: one thread per (point, frame).  Frame 0 = source, 1 + r = receiver r.
+// active (nullable) gates bwd frames: active[slot_of_point * R + r].
+// ------------------------------------------------------------------------------------
+__device__ inline void knn_insert(float d, int e, float *bd, int *bi, int &cnt)
+{
+    const int K = kNN + 1;
+    if (cnt == K && !(d < bd[K - 1] || (d == bd[K - 1] && e < bi[K - 1]))) return;
+    int p = (cnt < K) ? cnt++ : K - 1;
+    while (p > 0 && (bd[p - 1] > d || (bd[p - 1] == d && bi[p - 1] > e))) {
+        bd[p] = bd[p - 1];
+        bi[p] = bi[p - 1];
+        --p;
+    }
+    bd[p] = d;
+    bi[p] = e;
+}
+
+__global__ void k_locate(DevDb db, const DevFrame *frames, int frame0, int nframes, int npts,
+                         const double *xyz, double planet_radius, const uint8_t *active,
+                         int P, int R, Located *out, int *err)
+{
+    int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= npts * nframes) return;
+    int p = gid / nframes, f = frame0 + (gid - (gid / nframes) * nframes);
+    Located res;
+    res.elem = -1; res.tie = 0; res.xi = 0; res.eta = 0; res.phi = 0;
+    if (f > 0 && active && !active[(p / P) * R + (f - 1)]) {
+        out[gid] = res;
+        return;
+    }
+    const DevFrame fr = frames[f];
+    double x = xyz[3 * (size_t)p], y = xyz[3 * (size_t)p + 1], zz = xyz[3 * (size_t)p + 2];
+    // rotate_frame_rd
+    double xc = x * fr.cos_lon + y * fr.sin_lon;
+    double yc = -x * fr.sin_lon + y * fr.cos_lon;
+    double zc = zz;
+    double xp = xc * fr.cos_colat - zc * fr.sin_colat;
+    double yp = yc;
+    double zp = xc * fr.sin_colat + zc * fr.cos_colat;
+    double s = sqrt(xp * xp + yp * yp), z = zp;
+    double pc = atan2(yp, xp);
+    res.phi = (pc < 0.0) ? 2.0 * 3.1415926535898 + pc : pc;
+    double d2 = s * s + z * z, Rp = planet_radius;
+    if (d2 > Rp * Rp) { s = s / sqrt(d2) * Rp; z = z / sqrt(d2) * Rp; }
+    // exact 13 nearest midpoints (12 tested, the 13th detects ties at the boundary)
+    float qs = (float)s, qz = (float)z;
+    float bd[kNN + 1];
+    int bi[kNN + 1], cnt = 0;
+    int cx = (int)floor((s - db.gs0) / db.gh), cz = (int)floor((z - db.gz0) / db.gh);
+    cx = cx < 0 ? 0 : (cx >= db.gnx ? db.gnx - 1 : cx);
+    cz = cz < 0 ? 0 : (cz >= db.gnz ? db.gnz - 1 : cz);
+    for (int ring = 0;; ++ring) {
+        int x0 = cx - ring, x1 = cx + ring, z0 = cz - ring, z1 = cz + ring;
+        for (int iz = z0; iz <= z1; ++iz) {
+            if (iz < 0 || iz >= db.gnz) continue;
+            bool edge_z = (iz == z0 || iz == z1);
+            for (int ix = x0; ix <= x1; ++ix) {
+                if (ix < 0 || ix >= db.gnx) continue;
+                if (!edge_z && ix != x0 && ix != x1) continue;
+                int c = iz * db.gnx + ix;
+                for (int k = db.cell_start[c]; k < db.cell_start[c + 1]; ++k) {
+                    int e = db.cell_elems[k];
+                    float dx = db.mp_s[e] - qs, dz = db.mp_z[e] - qz;
+                    float sd = 0.0f;
+                    sd = sd + dx * dx;
+                    sd = sd + dz * dz;
+                    knn_insert(sd, e, bd, bi, cnt);
+                }
+            }
+        }
+        bool all = (x0 <= 0 && z0 <= 0 && x1 >= db.gnx - 1 && z1 >= db.gnz - 1);
+        if (all) break;
+        if (cnt == kNN + 1) {
+            double inf = 1e300;
+            double lo_s = x0 <= 0 ? -inf : db.gs0 + x0 * db.gh, hi_s = x1 >= db.gnx - 1 ? inf : db.gs0 + (x1 + 1) * db.gh;
+            double lo_z = z0 <= 0 ? -inf : db.gz0 + z0 * db.gh, hi_z = z1 >= db.gnz - 1 ? inf : db.gz0 + (z1 + 1) * db.gh;
+            double dm = fmin(fmin((double)qs - lo_s, hi_s - (double)qs), fmin((double)qz - lo_z, hi_z - (double)qz));
+            if (dm > 0 && (double)bd[kNN] < dm * dm * (1.0 - 1e-6)) break;
+        }
+    }
+    for (int j = 1; j < cnt; ++j) if (bd[j] == bd[j - 1]) res.tie = 1;
+    for (int c = 0; c < kNN && c < cnt; ++c) {
+        int e = bi[c];
+        double xi, eta;
+        if (d_inside(s, z, db.econ + (size_t)e * 16, db.eltype[e], &xi, &eta)) {
+            res.elem = e; res.xi = xi; res.eta = eta;
+            break;
+        }
+    }
+    if (res.elem < 0 && err) atomicOr(err, 1);
+    out[gid] = res;
+}
+
+// ------------------------------------------------------------------------------------
+// Strain operator tables.  Strain components (Voigt ss, pp, zz, zp, sz, sp) of one
+// simulation as linear combinations of d/ds, d/dz and 1/s of its variables
+// (sem_derivatives.f90:281-649).  op: 0 = d/ds, 1 = d/dz, 2 = 1/s.
+// ------------------------------------------------------------------------------------
+struct Term { signed char comp, var, op; float coef; };
+// monopole (a = u_s, b = u_z)
+__constant__ Term kMono[] = {{0, 0, 0, 1.f}, {1, 0, 2, 1.f}, {2, 1, 1, 1.f},
+                             {4, 0, 1, .5f}, {4, 1, 0, .5f}};
+__constant__ Term kDi[] = {{0, 0, 0, 1.f}, {1, 0, 2, 1.f}, {1, 1, 2, -1.f}, {2, 2, 1, 1.f},
+                           {3, 2, 2, -.5f}, {3, 1, 1, -.5f}, {4, 0, 1, .5f}, {4, 2, 0, .5f},
+                           {5, 0, 2, -.5f}, {5, 1, 2, .5f}, {5, 1, 0, -.5f}};
+__constant__ Term kQuad[] = {{0, 0, 0, 1.f}, {1, 0, 2, 1.f}, {1, 1, 2, -2.f}, {2, 2, 1, 1.f},
+                             {3, 2, 2, -1.f}, {3, 1, 1, -.5f}, {4, 0, 1, .5f}, {4, 2, 0, .5f},
+                             {5, 1, 2, .5f}, {5, 0, 2, -1.f}, {5, 1, 0, -.5f}};
+// traces (straintrace_*_td): comp field unused
+__constant__ Term kTrMono[] = {{0, 0, 0, 1.f}, {0, 0, 2, 1.f}, {0, 1, 1, 1.f}};
+__constant__ Term kTrDi[] = {{0, 0, 0, 1.f}, {0, 0, 2, 1.f}, {0, 1, 2, -1.f}, {0, 2, 1, 1.f}};
+__constant__ Term kTrQuad[] = {{0, 0, 0, 1.f}, {0, 0, 2, 1.f}, {0, 1, 2, -2.f}, {0, 2, 1, 1.f}};
+
+// Voigt rotations (rotations.f90:382-419 and 555-612) applied to a 6-vector
+__device__ inline void rot_src_to_xyz(double *t, double sp, double cp)
+{
+    double a1 = t[0], a2 = t[1], a3 = t[2], a4 = t[3], a5 = t[4], a6 = t[5];
+    t[0] = a1 * (cp * cp) + sp * (-2 * a6 * cp + a2 * sp);
+    t[1] = a2 * (cp * cp) + sp * (2 * a6 * cp + a1 * sp);
+    t[2] = a3;
+    t[3] = a4 * cp + a5 * sp;
+    t[4] = a5 * cp - a4 * sp;
+    t[5] = (a1 - a2) * cp * sp + a6 * (cp * cp - sp * sp);
+}
+
+__device__ inline void rot_xyz_to_earth(double *t, double sp, double cp, double st, double ct)
+{
+    double a1 = t[0], a2 = t[1], a3 = t[2], a4 = t[3], a5 = t[4], a6 = t[5];
+    t[0] = a1 * (cp * cp) * (ct * ct) - 2 * a6 * cp * ct * sp + a2 * (sp * sp)
+         + 2 * a5 * (cp * cp) * ct * st - 2 * a4 * cp * sp * st + a3 * (cp * cp) * (st * st);
+    t[1] = a2 * (cp * cp) + sp * (2 * a6 * cp * ct + a1 * (ct * ct) * sp
+                                  + st * (2 * a4 * cp + 2 * a5 * ct * sp + a3 * sp * st));
+    t[2] = a3 * (ct * ct) + st * (-2 * a5 * ct + a1 * st);
+    t[3] = ct * (a4 * cp + a5 * ct * sp + a3 * sp * st) - st * (a6 * cp + a1 * ct * sp + a5 * sp * st);
+    t[4] = ct * (a5 * cp * ct - a4 * sp + a3 * cp * st) - st * (a1 * cp * ct - a6 * sp + a5 * cp * st);
+    t[5] = cp * st * (a4 * cp + a5 * ct * sp + a3 * sp * st) - sp * (a2 * cp + a6 * ct * sp + a4 * sp * st)
+         + cp * ct * (a6 * cp + a1 * ct * sp + a5 * sp * st);
+}
+
+__device__ inline double azim_factor(double phi, const double *mij, int isim, int ikind)
+{
+    switch (isim) {
+    case 1: return ikind == 1 ? mij[0] : 0.0;
+    case 2: return ikind == 1 ? 0.5 * (mij[1] + mij[2]) : 0.0;
+    case 3: return ikind == 1 ? mij[3] * cos(phi) + mij[4] * sin(phi) : -mij[3] * sin(phi) + mij[4] * cos(phi);
+    default: return ikind == 1 ? 0.5 * (mij[1] - mij[2]) * cos(2.0 * phi) + mij[5] * sin(2.0 * phi)
+                               : -0.5 * (mij[1] - mij[2]) * sin(2.0 * phi) + mij[5] * cos(2.0 * phi);
+    }
+}
+
+
+// Builds the per-point weights W[c][v][jpol*5+ipol] such that the frame's output
+// component c at this point is sum_v sum_node W * u_v(node, t).
+__device__ void build_weights(OpShared &S, const DevDb &db, const PathConf &pc, const DevFrame &fr,
+                              const Located &L, bool fwd, int nvar, const int *vars)
+{
+    const int tid = threadIdx.x;
+    const int e = L.elem;
+    const double *E = db.econ + (size_t)e * 16;
+    if (tid == 0) { S.elem = e; S.axial = db.axis[e]; S.type = db.eltype[e]; }
+    __syncthreads();
+    const bool axial = S.axial != 0;
+    const double *xp = axial ? c_basis.glj : c_basis.gll;
+    if (tid < 25) {
+        int i = tid % 5, j = tid / 5;
+        double ij[4], sz[2];
+        d_inv_jacobian(xp[i], c_basis.gll[j], E, S.type, ij);
+        d_mapping(xp[i], c_basis.gll[j], E, S.type, sz);
+        S.invj[tid][0] = ij[0]; S.invj[tid][1] = ij[1]; S.invj[tid][2] = ij[2]; S.invj[tid][3] = ij[3];
+        S.sinv[tid] = 1.0 / sz[0];
+    } else if (tid == 32 || tid == 33) {
+        // Lagrange basis at (xi, eta) (spectral_basis.f90:203-217)
+        const double *pts = tid == 32 ? xp : c_basis.gll;
+        double x = tid == 32 ? L.xi : L.eta;
+        double *l = tid == 32 ? S.li : S.lj;
+        for (int i = 0; i < 5; ++i) {
+            double v = 1.0;
+            for (int m = 0; m < 5; ++m) {
+                if (m == i) continue;
+                v = v * (x - pts[m]) / (pts[i] - pts[m]);
+            }
+            l[i] = v;
+        }
+    } else if (tid == 64) {
+        // output transform: sign flip of comps 4, 6 (readfields.f90:2180-2184), azimuthal
+        // factors, 1/amplitude, src -> xyz -> earth rotations (full strain only)
+        double amp = db.amplitude;
+        int nsim = fwd ? pc.nsim_fwd : 1;
+        for (int si = 0; si < nsim; ++si) {
+            double az[6];
+            if (fwd) {
+                int isim = pc.fwd_sim[si];
+                double a1 = azim_factor(L.phi, pc.mij, isim, 1), a2 = azim_factor(L.phi, pc.mij, isim, 2);
+                az[0] = a1; az[1] = a1; az[2] = a1; az[3] = a2; az[4] = a1; az[5] = a2;
+            } else if (pc.bwd_isim == 1) {
+                for (int m = 0; m < 6; ++m) az[m] = 1.0;
+            } else {
+                double a1 = pc.bwd_fi[1] * cos(L.phi) + pc.bwd_fi[2] * sin(L.phi);
+                double a2 = -pc.bwd_fi[1] * sin(L.phi) + pc.bwd_fi[2] * cos(L.phi);
+                az[0] = a1; az[1] = a1; az[2] = a1; az[3] = a2; az[4] = a1 * 2; az[5] = a2 * 2;
+            }
+            if (!pc.full) {
+                for (int m = 0; m < 6; ++m) S.M[0][si][m] = (m == 0) ? az[0] / amp : 0.0;
+            } else {
+                double sp = sin(L.phi), cp = cos(L.phi);
+                for (int m = 0; m < 6; ++m) {
+                    double t[6] = {0, 0, 0, 0, 0, 0};
+                    t[m] = 1.0;
+                    rot_src_to_xyz(t, sp, cp);
+                    rot_xyz_to_earth(t, fr.sin_lon, fr.cos_lon, fr.sin_colat, fr.cos_colat);
+                    double f = ((m == 3 || m == 5) ? -1.0 : 1.0) * az[m] / amp;
+                    for (int c = 0; c < 6; ++c) S.M[c][si][m] = t[c] * f;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const int nd = pc.nd;
+    const int total = nd * nvar * 25;
+    const double *Gx = axial ? c_basis.G1 : c_basis.G2;
+    for (int idx = tid; idx < total; idx += blockDim.x) {
+        int n = idx % 25, v = (idx / 25) % nvar, c = idx / (25 * nvar);
+        int k = n % 5, l = n / 5;
+        int var = vars[v];
+        // simulation of this variable and its local index
+        int si, pole, vl;
+        if (fwd) {
+            // variables 0-1 MZZ, 2-3 MXX+MYY, 4-6 MXZ, 7-9 MXY
+            int isim = var < 2 ? 1 : (var < 4 ? 2 : (var < 7 ? 3 : 4));
+            pole = isim <= 2 ? 0 : isim - 2;
+            vl = var - (isim == 1 ? 0 : isim == 2 ? 2 : isim == 3 ? 4 : 7);
+            si = 0;
+            for (int q = 0; q < pc.nsim_fwd; ++q) if (pc.fwd_sim[q] == isim) si = q;
+        } else {
+            pole = pc.bwd_isim == 1 ? 0 : 1;
+            vl = var - (pc.bwd_isim == 1 ? 0 : 2);
+            si = 0;
+        }
+        const Term *tab;
+        int nt;
+        if (pc.full) {
+            tab = pole == 0 ? kMono : (pole == 1 ? kDi : kQuad);
+            nt = pole == 0 ? 5 : 11;
+        } else {
+            tab = pole == 0 ? kTrMono : (pole == 1 ? kTrDi : kTrQuad);
+            nt = pole == 0 ? 3 : 4;
+        }
+        double as = 0.0, azz = 0.0, af = 0.0;
+        for (int q = 0; q < nt; ++q) {
+            if (tab[q].var != vl) continue;
+            double m = S.M[c][si][pc.full ? tab[q].comp : 0] * (double)tab[q].coef;
+            if (tab[q].op == 0) as += m; else if (tab[q].op == 1) azz += m; else af += m;
+        }
+        double w = 0.0;
+        // xi-derivative part: sum_i Pxi(i, l) * GT(i, k),  GT(i, k) = Gx[k + 5 i]
+        for (int i = 0; i < 5; ++i) {
+            const double *ij = S.invj[i + 5 * l];
+            double Lil = S.li[i] * S.lj[l];
+            double pxi = Lil * (as * ij[0] + azz * ij[2]);
+            if (axial && i == 0) pxi += af * Lil * ij[0];
+            w += pxi * Gx[k + 5 * i];
+        }
+        // eta-derivative part: sum_j Peta(k, j) * G(l, j),  G(l, j) = G2[l + 5 j]
+        for (int j = 0; j < 5; ++j) {
+            const double *ij = S.invj[k + 5 * j];
+            double Lkj = S.li[k] * S.lj[j];
+            double peta = Lkj * (as * ij[1] + azz * ij[3]);
+            if (axial && k == 0) peta += af * Lkj * ij[1];
+            w += peta * c_basis.G2[l + 5 * j];
+        }
+        if (!(axial && k == 0)) w += af * S.li[k] * S.lj[l] * S.sinv[k + 5 * l];
+        S.W[(c * nvar + v) * 25 + n] = (float)w;
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------------------------
+// k_strain_fft: one workgroup per (point, frame) pair.
+//  fwd: pairs are points; bwd: pairs are (point, receiver) = p * R + r.
+//  out: [pair][nd][nomega] float2 (spectrum times dt, FFTW r2c convention)
+// ------------------------------------------------------------------------------------
+template <int SPT>
+__global__ __launch_bounds__(kStrainThreads) void k_strain_fft(
+    DevDb db, PathConf pc, const DevFrame *frames, int fwd, int npairs, int R, int P,
+    const Located *loc, int loc_stride, const uint8_t *active, const float2 *tw,
+    const float2 *tw2, float2 *out, float *trace_out)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    OpShared &S = *reinterpret_cast<OpShared *>(smem_raw);
+    float2 *bufA = reinterpret_cast<float2 *>(smem_raw + ((sizeof(OpShared) + 15) / 16) * 16);
+    float2 *bufB = bufA + pc.M;
+    __shared__ float wave_tot[kStrainThreads / 64];
+    __shared__ float edge_x[kStrainThreads + 1];
+
+    const int pair = blockIdx.x;
+    if (pair >= npairs) return;
+    int p, f;
+    if (fwd) { p = pair; f = 0; }
+    else { p = pair / R; f = 1 + (pair - p * R); }
+    if (!fwd && active && !active[(p / P) * R + (f - 1)]) return;
+    const Located L = loc[(size_t)p * loc_stride + (fwd ? 0 : f)];
+    if (L.elem < 0) return;
+    const int nvar = fwd ? pc.nvar_fwd : pc.nvar_bwd;
+    const int *vars = fwd ? pc.fwd_vars : pc.bwd_vars;
+    build_weights(S, db, pc, frames[f], L, fwd != 0, nvar, vars);
+
+    const int tid = threadIdx.x;
+    const int nt = pc.nt, nd = pc.nd, M = pc.M;
+    const int t0 = tid * SPT;
+    float acc[6][SPT];
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+#pragma unroll
+        for (int q = 0; q < SPT; ++q) acc[c][q] = 0.0f;
+    const size_t blk = (size_t)25 * nt;
+    const float *ue = db.u + (size_t)S.elem * db.nvar * blk;
+    if (t0 < nt) {
+        const bool full_vec = (t0 + SPT <= nt) && ((nt & 3) == 0);
+        for (int v = 0; v < nvar; ++v) {
+            const float *uv = ue + (size_t)vars[v] * blk + t0;
+            for (int n = 0; n < 25; ++n) {
+                float x[SPT];
+                const float *src = uv + (size_t)n * nt;
+                if (full_vec && (SPT % 4) == 0) {
+#pragma unroll
+                    for (int q = 0; q < SPT; q += 4) {
+                        float4 t = *reinterpret_cast<const float4 *>(src + q);
+                        x[q] = t.x; x[q + 1] = t.y; x[q + 2] = t.z; x[q + 3] = t.w;
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < SPT; ++q) x[q] = (t0 + q < nt) ? src[q] : 0.0f;
+                }
+                if (nd == 6) {
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) {
+                        float w = S.W[(c * nvar + v) * 25 + n];
+#pragma unroll
+                        for (int q = 0; q < SPT; ++q) acc[c][q] = __builtin_fmaf(w, x[q], acc[c][q]);
+                    }
+                } else {
+                    float w = S.W[v * 25 + n];
+#pragma unroll
+                    for (int q = 0; q < SPT; ++q) acc[0][q] = __builtin_fmaf(w, x[q], acc[0][q]);
+                }
+            }
+        }
+    }
+
+    if (trace_out) {
+        // raw strain traces (before cumsum): [pair][nd][nt]
+        for (int c = 0; c < nd; ++c)
+#pragma unroll
+            for (int q = 0; q < SPT; ++q)
+                if (t0 + q < nt) trace_out[((size_t)pair * nd + c) * nt + t0 + q] = acc[c][q];
+        return;
+    }
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int c = 0; c < nd; ++c) {
+        float x[SPT];
+#pragma unroll
+        for (int q = 0; q < SPT; ++q) x[q] = (t0 + q < nt) ? acc[c][q] : 0.0f;
+        if (fwd) {
+            // cumsum_trapezoidal: c0 = x0, c1 = c0 + x1, ct = c(t-1) + x(t-1) + xt
+            edge_x[tid + 1] = x[SPT - 1];
+            if (tid == 0) edge_x[0] = 0.0f;
+            __syncthreads();
+            float prev = edge_x[tid];
+            float d[SPT], run = 0.0f;
+#pragma unroll
+            for (int q = 0; q < SPT; ++q) {
+                int t = t0 + q;
+                float xm1 = q == 0 ? prev : x[q - 1];
+                d[q] = (t <= 1) ? x[q] : (xm1 + x[q]);
+                run += d[q];
+            }
+            // exclusive scan of the per-thread totals
+            float incl = run;
+            for (int o = 1; o < 64; o <<= 1) {
+                float y = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += y;
+            }
+            if (lane == 63) wave_tot[wave] = incl;
+            __syncthreads();
+            float wbase = 0.0f;
+            for (int w2 = 0; w2 < wave; ++w2) wbase += wave_tot[w2];
+            float excl = wbase + incl - run;
+            float cs = excl;
+            const float h = (float)(pc.dt * 0.5);
+#pragma unroll
+            for (int q = 0; q < SPT; ++q) {
+                cs += d[q];
+                x[q] = (t0 + q < nt) ? cs * h : 0.0f;
+            }
+            __syncthreads();
+        }
+        // taper (10 samples at both ends) and packing into z[n] = x[2n] + i x[2n+1]
+#pragma unroll
+        for (int q = 0; q < SPT; ++q) {
+            int t = t0 + q;
+            if (t < pc.ntaper) x[q] *= pc.taper[t];
+            else if (t >= nt - pc.ntaper && t < nt) x[q] *= pc.taper[nt - 1 - t];
+        }
+        for (int n = tid; n < M; n += kStrainThreads) bufA[n] = make_float2(0.0f, 0.0f);
+        __syncthreads();
+        {
+            float *fa = reinterpret_cast<float *>(bufA);
+#pragma unroll
+            for (int q = 0; q < SPT; ++q)
+                if (t0 + q < nt) fa[t0 + q] = x[q];
+        }
+        __syncthreads();
+        // Stockham autosort FFT of length M, radix 4 (radix 2 first when log2 M is odd)
+        float2 *in = bufA, *ob = bufB;
+        for (int Ns = 1; Ns < M;) {
+            int Rr = ((M / Ns) % 4 == 0) ? 4 : 2;
+            int Mq = M / Rr;
+            int stride = M / (Ns * Rr);
+            for (int j = tid; j < Mq; j += kStrainThreads) {
+                int kk = j % Ns;
+                float2 v0 = in[j], v1 = in[j + Mq];
+                if (Rr == 2) {
+                    float2 w = tw[kk * stride];
+                    float2 t1 = make_float2(v1.x * w.x - v1.y * w.y, v1.x * w.y + v1.y * w.x);
+                    int o = (j / Ns) * Ns * 2 + kk;
+                    ob[o] = make_float2(v0.x + t1.x, v0.y + t1.y);
+                    ob[o + Ns] = make_float2(v0.x - t1.x, v0.y - t1.y);
+                } else {
+                    float2 v2 = in[j + 2 * Mq], v3 = in[j + 3 * Mq];
+                    float2 w1 = tw[kk * stride], w2 = tw[2 * kk * stride], w3 = tw[3 * kk * stride];
+                    float2 a1 = make_float2(v1.x * w1.x - v1.y * w1.y, v1.x * w1.y + v1.y * w1.x);
+                    float2 a2 = make_float2(v2.x * w2.x - v2.y * w2.y, v2.x * w2.y + v2.y * w2.x);
+                    float2 a3 = make_float2(v3.x * w3.x - v3.y * w3.y, v3.x * w3.y + v3.y * w3.x);
+                    float2 s02 = make_float2(v0.x + a2.x, v0.y + a2.y), d02 = make_float2(v0.x - a2.x, v0.y - a2.y);
+                    float2 s13 = make_float2(a1.x + a3.x, a1.y + a3.y), d13 = make_float2(a1.x - a3.x, a1.y - a3.y);
+                    int o = (j / Ns) * Ns * 4 + kk;
+                    ob[o] = make_float2(s02.x + s13.x, s02.y + s13.y);
+                    // y1 = d02 - i d13, y3 = d02 + i d13
+                    ob[o + Ns] = make_float2(d02.x + d13.y, d02.y - d13.x);
+                    ob[o + 2 * Ns] = make_float2(s02.x - s13.x, s02.y - s13.y);
+                    ob[o + 3 * Ns] = make_float2(d02.x - d13.y, d02.y + d13.x);
+                }
+            }
+            __syncthreads();
+            float2 *tmp = in; in = ob; ob = tmp;
+            Ns *= Rr;
+        }
+        // real-FFT post-processing: X_k = E_k + e^{-2 pi i k/N} O_k, k = 0..M; times dt
+        float2 *dst = out + ((size_t)pair * nd + c) * pc.nomega;
+        const float fdt = (float)pc.dt;
+        for (int k = tid; k <= M; k += kStrainThreads) {
+            float2 zk = in[k == M ? 0 : k];
+            float2 zm = in[(M - k) & (M - 1)];
+            float2 zc = make_float2(zm.x, -zm.y);
+            float2 E = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y + zc.y));
+            float2 dd = make_float2(zk.x - zc.x, zk.y - zc.y);
+            float2 O = make_float2(0.5f * dd.y, -0.5f * dd.x);
+            float2 w = tw2[k];
+            float2 X = make_float2(E.x + w.x * O.x - w.y * O.y, E.y + w.x * O.y + w.y * O.x);
+            dst[k] = make_float2(X.x * fdt, X.y * fdt);
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// k_eval: one workgroup per (point, receiver).  Slot s of receiver r is a pair
+// (H index, base kernel); value = sum_k Re(basekernel_k * H_k) in FP64.
+// ------------------------------------------------------------------------------------
+
+__device__ inline float2 cmulf(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+
+__device__ inline double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(kEvalThreads) void k_eval(
+    PathConf pc, int R, int npairs, int P, const float2 *F, const float2 *B, const uint8_t *active,
+    const RecDesc *recs, const KernDesc *kern, const float2 *H, const int *model_idx,
+    const float *model /* [6][ndepth] */, int ndepth, int relative, int nk, double *phys,
+    double *base_out)
+{
+    const int pair = blockIdx.x;
+    if (pair >= npairs) return;
+    const int p = pair / R, r = pair - (pair / R) * R;
+    if (active && !active[(p / P) * R + r]) return;
+    const RecDesc &rd = recs[r];
+    const int tid = threadIdx.x, nd = pc.nd, nw = pc.nomega;
+    const float2 *Fp = F + (size_t)p * nd * nw;
+    const float2 *Bp = B + (size_t)pair * nd * nw;
+    __shared__ double red[kEvalThreads / 64][16];
+    __shared__ double slotval[kMaxSlots];
+    for (int s0 = 0; s0 < rd.nslot; s0 += 16) {
+        const int ns = min(16, rd.nslot - s0);
+        double acc[16];
+#pragma unroll
+        for (int s = 0; s < 16; ++s) acc[s] = 0.0;
+        for (int k = tid; k < nw; k += kEvalThreads) {
+            float2 bk[6];
+            if (nd == 1) {
+                float2 f0 = Fp[k], b0 = Bp[k];
+                float2 pr = cmulf(f0, b0);
+                bk[0] = make_float2(-pr.x, -pr.y);
+                bk[1] = make_float2(-2.0f * pr.x, -2.0f * pr.y);
+                bk[3] = bk[4] = bk[5] = make_float2(0.f, 0.f);
+            } else {
+                float2 f[6], b[6];
+#pragma unroll
+                for (int c = 0; c < 6; ++c) { f[c] = Fp[(size_t)c * nw + k]; b[c] = Bp[(size_t)c * nw + k]; }
+                float2 fs = make_float2(f[0].x + f[1].x + f[2].x, f[0].y + f[1].y + f[2].y);
+                float2 bs = make_float2(b[0].x + b[1].x + b[2].x, b[0].y + b[1].y + b[2].y);
+                float2 l = cmulf(fs, bs);
+                bk[0] = make_float2(-l.x, -l.y);
+                float2 m = make_float2(0.f, 0.f);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) { float2 q = cmulf(f[c], b[c]); m.x += q.x; m.y += q.y; }
+                bk[1] = make_float2(-2.0f * m.x, -2.0f * m.y);
+                bk[3] = cmulf(make_float2(f[1].x + f[0].x, f[1].y + f[0].y), make_float2(b[1].x + b[0].x, b[1].y + b[0].y));
+                float2 q5 = cmulf(b[4], f[4]), q4 = cmulf(b[3], f[3]);
+                bk[4] = make_float2(4.0f * (q5.x + q4.x), 4.0f * (q5.y + q4.y));
+                float2 c1 = cmulf(make_float2(f[0].x + f[1].x, f[0].y + f[1].y), b[2]);
+                float2 c2 = cmulf(cmulf(b[0], b[1]), f[2]);
+                bk[5] = make_float2(c1.x + c2.x, c1.y + c2.y);
+            }
+            bk[2] = make_float2(0.f, 0.f);
+            for (int s = 0; s < ns; ++s) {
+                float2 h = H[(size_t)rd.slot_h[s0 + s] * nw + k];
+                float2 v = bk[rd.slot_bk[s0 + s]];
+                acc[s] += (double)v.x * (double)h.x - (double)v.y * (double)h.y;
+            }
+        }
+        const int lane = tid & 63, wave = tid >> 6;
+        for (int s = 0; s < ns; ++s) {
+            double t = wave_sum(acc[s]);
+            if (lane == 0) red[wave][s] = t;
+        }
+        __syncthreads();
+        if (tid < ns) {
+            double t = 0.0;
+            for (int w = 0; w < kEvalThreads / 64; ++w) t += red[w][tid];
+            slotval[s0 + tid] = t;
+        }
+        __syncthreads();
+    }
+    // physical kernels (kernel.f90:665-832) with the model at the point (fp32 values)
+    const int mi = model_idx[p];
+    const float vp = model[0 * ndepth + mi], vs = model[1 * ndepth + mi], rho_f = model[2 * ndepth + mi];
+    const float phi_m = model[3 * ndepth + mi], xi_m = model[4 * ndepth + mi], eta_m = model[5 * ndepth + mi];
+    const double rho = rho_f;
+    for (int kk = tid; kk < rd.nkernel; kk += kEvalThreads) {
+        int k = rd.first_kernel + kk;
+        const KernDesc &kd = kern[k];
+        double b[6];
+        for (int q = 0; q < 6; ++q) b[q] = kd.slot[q] >= 0 ? slotval[kd.slot[q]] : 0.0;
+        if (base_out) {
+            for (int q = 0; q < 6; ++q) base_out[((size_t)p * nk + k) * 6 + q] = b[q];
+            continue;
+        }
+        double v = 0.0;
+        float vsh = vs, vph = vp, vsv = vsh / sqrtf(xi_m), vpv = vph * sqrtf(phi_m);
+        switch (kd.param) {
+        case 0: v = relative ? b[0] * (double)(rho_f * (vp * vp - 2 * (vs * vs))) : b[0]; break;
+        case 3: v = relative ? b[1] * (double)(rho_f * (vs * vs)) : b[1]; break;
+        case 1: v = relative ? 2.0 * rho * (double)(vp * vp) * b[0] : 2.0 * rho * (double)vp * b[0]; break;
+        case 2: v = (relative ? 2.0 * rho * (double)(vs * vs) : 2.0 * rho * (double)vs) * (b[1] - 2 * b[0]); break;
+        case 4: v = (relative ? 2.0 * rho * (double)(vsh * vsh) : 2.0 * rho * (double)vsh)
+                    * (b[1] - 2 * b[0] - b[4] + 2 * b[5] * (double)(1 - eta_m)); break;
+        case 5: v = (relative ? 2.0 * rho * (double)(vsv * vsv) : 2.0 * rho * (double)vsv) * b[4]; break;
+        case 6: v = (relative ? 2.0 * rho * (double)(vph * vph) : 2.0 * rho * (double)vph) * (b[3] + b[5] * (double)eta_m); break;
+        case 7: v = (relative ? 2.0 * rho * (double)(vpv * vpv) : 2.0 * rho * (double)vpv) * (b[0] - b[3] - b[5]); break;
+        case 8: v = relative ? 2.0 * rho * (double)(vph * vph - vsh * vsh) * b[5] * (double)eta_m
+                             : 2.0 * rho * (double)(vph * vph - vsh * vsh) * b[5]; break;
+        default: v = 0.0;
+        }
+        phys[(size_t)p * nk + k] = v;
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// Monte Carlo bookkeeping
+// ------------------------------------------------------------------------------------
+
+// slave.f90:501-511: loop while not all converged; draw points; exit when any kernel
+// has niterations == max_iter.  Produces the ordered list of voxels to evaluate.
+__global__ void k_step_begin(int nvox, int nk, int max_iter, VoxState st, const uint8_t *live,
+                             uint8_t *working)
+{
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvox) return;
+    working[v] = 0;
+    if (st.done[v] || !live[v]) return;
+    bool all = true, anymax = false;
+    for (int k = 0; k < nk; ++k) {
+        if (!st.conv[(size_t)v * nk + k]) all = false;
+        if (st.niter[(size_t)v * nk + k] == max_iter) anymax = true;
+    }
+    if (all) { st.done[v] = 1; return; }
+    st.draws[v] += 1;
+    if (anymax) { st.done[v] = 1; return; }
+    working[v] = 1;
+}
+
+// single-workgroup ordered compaction of the working flags
+__global__ __launch_bounds__(1024) void k_compact(int n, const uint8_t *flag, int *list, int *count)
+{
+    __shared__ int part[1024];
+    const int tid = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int b0 = tid * per, b1 = min(n, b0 + per);
+    int c = 0;
+    for (int i = b0; i < b1; ++i) c += flag[i] ? 1 : 0;
+    part[tid] = c;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        int y = tid >= o ? part[tid - o] : 0;
+        __syncthreads();
+        part[tid] += y;
+        __syncthreads();
+    }
+    int pos = part[tid] - c;
+    for (int i = b0; i < b1; ++i) if (flag[i]) list[pos++] = i;
+    if (tid == 1023) *count = part[1023];
+}
+
+// receivers evaluated this step: not masked and not all converged (slave.f90:539-584)
+__global__ void k_receiver_active(int nwork, int R, int nk, const int *work_voxel,
+                                  const uint8_t *masked, const int *rec_first, const int *rec_count,
+                                  VoxState st, uint8_t *active)
+{
+    int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= nwork * R) return;
+    int w = gid / R, r = gid - (gid / R) * R;
+    int v = work_voxel[w];
+    uint8_t a = 0;
+    if (!masked[(size_t)v * R + r]) {
+        for (int k = rec_first[r]; k < rec_first[r] + rec_count[r]; ++k)
+            if (!st.conv[(size_t)v * nk + k]) { a = 1; break; }
+    }
+    active[gid] = a;
+}
+
+// check_montecarlo_integral (mc_integration.f90:69-143) and niterations bookkeeping
+__global__ void k_mc(int nwork, int P, int nk, int R, const int *work_voxel, const double *phys,
+                     const uint8_t *masked, const uint8_t *active, const int *k_rec,
+                     const int *k_nbk, const double *volume, const double *allowed_var, double allowed_rel, VoxState st,
+                     unsigned long long *evals, int *err)
+{
+    const int w = blockIdx.x;
+    if (w >= nwork) return;
+    const int v = work_voxel[w];
+    const int64_t n_old = st.nmodels[v];
+    const int64_t n = n_old + P;
+    const double V = volume[v];
+    unsigned long long my_evals = 0;
+    for (int k = threadIdx.x; k < nk; k += blockDim.x) {
+        const size_t vk = (size_t)v * nk + k;
+        const int r = k_rec[k];
+        const bool msk = masked[(size_t)v * R + r] != 0;
+        const bool was_conv = st.conv[vk] != 0;
+        if (msk) st.niter[vk] += 1;
+        else if (active[(size_t)w * R + r] && !was_conv) { st.niter[vk] += k_nbk[k]; my_evals += P; }
+        if (was_conv) continue;
+        double s = 0.0, s2 = 0.0;
+        if (!msk) {
+            for (int p = 0; p < P; ++p) s += phys[((size_t)w * P + p) * nk + k];
+            for (int p = 0; p < P; ++p) {
+                double x = phys[((size_t)w * P + p) * nk + k];
+                s2 += x * x;
+            }
+        }
+        double fs = st.fsum[vk] + s, f2 = st.f2sum[vk] + s2;
+        st.fsum[vk] = fs;
+        st.f2sum[vk] = f2;
+        double integral = fs * V / (double)n, variance;
+        bool conv = false;
+        if (n > 2) {
+            variance = V * V / (double)(n - 1) * ((f2 / (double)n) - (fs / (double)n) * (fs / (double)n));
+            if (variance < allowed_var[v]) conv = true;
+            else if (sqrt(variance) < allowed_rel * fabs(integral)) conv = true;
+        } else {
+            variance = integral * 1e10;
+        }
+        st.integral[vk] = integral;
+        st.variance[vk] = variance;
+        st.conv[vk] = conv ? 1 : 0;
+        if (integral != integral || variance != variance) atomicOr(err, 2);
+    }
+    // block sum of evaluations
+    __shared__ unsigned long long red[16];
+    unsigned long long t = my_evals;
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tot = 0;
+        for (int i = 0; i < (int)(blockDim.x + 63) / 64; ++i) tot += red[i];
+        atomicAdd(evals, tot);
+        st.nmodels[v] = n;
+        st.steps[v] += 1;
+    }
+}
+
+}  // namespace mck

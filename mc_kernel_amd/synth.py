@@ -1,0 +1,366 @@
+"""Seeded synthetic workloads: AxiSEM-style merged databases, receivers, kernels and
+tetrahedral inversion meshes, assembled into an `MckSetup` (include/mck.h).
+
+The workloads follow the benchmark configurations of this path:
+  c0    2,000-quad DB, nt=1024, dt=1 s, 1 explosive source, 1 receiver at 30 deg,
+        2 CC vp kernels (Gabor 40 s / 30 s, 60 s windows), 500 mantle tetrahedra,
+        Halton points, 32 points per MC step, up to 8 steps.
+  c1    20,000-quad DB, nt=2048, 10 receivers at 10..100 deg, Gabor 20/30/40/60 s,
+        CC and AM for vp and vs (160 kernels), 50,000 tetrahedra, relative MC
+        tolerance 1e-3, 16 points per step, MAXIMUM_ITERATIONS 64 (<= 1,024 points).
+  tiny  a few-hundred-quad DB with nt=256, dt=4 s for CPU tests.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SYNTH = None
+
+
+def synth_lib():
+    global _SYNTH
+    if _SYNTH is None:
+        path = os.path.join(_HERE, "libmck_synth.so")
+        if not os.path.exists(path):
+            raise RuntimeError("libmck_synth.so is not built; run __graft_entry__.build()")
+        lib = C.CDLL(path)
+        lib.synth_mesh_create.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]
+        lib.synth_mesh_create.restype = C.c_void_p
+        lib.synth_mesh_sizes.argtypes = [C.c_void_p, abi.i32p, abi.i32p]
+        lib.synth_mesh_get.argtypes = [C.c_void_p, abi.f32p, abi.f32p, abi.f32p, abi.f32p,
+                                       abi.i32p, abi.i32p, abi.i32p, abi.i32p]
+        lib.synth_mesh_free.argtypes = [C.c_void_p]
+        lib.synth_db_fill.argtypes = [C.c_int32, abi.i32p, C.c_int32, C.c_int32, C.c_double,
+                                      C.c_uint64, abi.f32p, C.c_int32]
+        lib.synth_model_table.argtypes = [C.c_int32, abi.f32p, abi.f32p, C.c_double, C.c_float,
+                                          abi.f32p]
+        lib.synth_model_table.restype = C.c_int32
+        _SYNTH = lib
+    return _SYNTH
+
+
+R_EARTH = 6371.0e3
+R_CMB = 3480.0e3
+R_CENTRAL = 1200.0e3
+
+# approximate P travel times (s) against epicentral distance (deg)
+_TP_DEG = np.array([0, 10, 20, 30, 40, 50, 60, 70, 80, 90, 100, 120, 140, 160, 180.0])
+_TP_SEC = np.array([0, 145, 280, 370, 460, 540, 610, 675, 735, 780, 820, 900, 980, 1060, 1140.0])
+
+
+def p_time(delta_deg):
+    return float(np.interp(delta_deg, _TP_DEG, _TP_SEC))
+
+
+def mesh_params(nelem_target):
+    """Quarter divisions n, ring layers and shell layers giving ~nelem_target quads with
+    near-unit aspect ratio."""
+    n = max(3, int(round(math.sqrt(nelem_target / 10.5))))
+    mring = 2
+    nshell = max(4, int(round((nelem_target - 2 * n * n - 4 * n * mring) / (4 * n))))
+    return n, mring, nshell
+
+
+class Mesh:
+    def __init__(self, nelem_target, R=R_EARTH, rc=R_CENTRAL):
+        lib = synth_lib()
+        n, mring, nshell = mesh_params(nelem_target)
+        h = lib.synth_mesh_create(R, rc, n, mring, nshell)
+        ne, npnt = C.c_int32(), C.c_int32()
+        lib.synth_mesh_sizes(h, C.byref(ne), C.byref(npnt))
+        self.nelem, self.npoints = ne.value, npnt.value
+        self.mesh_s = np.zeros(self.npoints, np.float32)
+        self.mesh_z = np.zeros(self.npoints, np.float32)
+        self.mp_s = np.zeros(self.nelem, np.float32)
+        self.mp_z = np.zeros(self.nelem, np.float32)
+        self.fem = np.zeros((self.nelem, 4), np.int32)
+        self.sem = np.zeros((self.nelem, 5, 5), np.int32)
+        self.eltype = np.zeros(self.nelem, np.int32)
+        self.axis = np.zeros(self.nelem, np.int32)
+        P = abi.ptr
+        lib.synth_mesh_get(h, P(self.mesh_s, C.c_float), P(self.mesh_z, C.c_float),
+                           P(self.mp_s, C.c_float), P(self.mp_z, C.c_float),
+                           P(self.fem, C.c_int32), P(self.sem, C.c_int32),
+                           P(self.eltype, C.c_int32), P(self.axis, C.c_int32))
+        lib.synth_mesh_free(h)
+        self.R = R
+
+
+def fill_db(mesh, nvar, nt, dt, seed, nthreads=None):
+    """merged_snapshots [nelem][nvar][5][5][nt] float32 (CPU generator)."""
+    lib = synth_lib()
+    out = np.empty((mesh.nelem, nvar, 5, 5, nt), np.float32)
+    nth = nthreads or min(16, os.cpu_count() or 1)
+    lib.synth_db_fill(mesh.nelem, abi.ptr(mesh.sem, C.c_int32), nvar, nt, dt, seed,
+                      abi.ptr(out, C.c_float), nth)
+    return out
+
+
+def model_table(mesh, dz=100.0):
+    lib = synth_lib()
+    nd = lib.synth_model_table(mesh.npoints, abi.ptr(mesh.mesh_s, C.c_float),
+                               abi.ptr(mesh.mesh_z, C.c_float), mesh.R, dz, None)
+    out = np.zeros((6, nd), np.float32)
+    lib.synth_model_table(mesh.npoints, abi.ptr(mesh.mesh_s, C.c_float),
+                          abi.ptr(mesh.mesh_z, C.c_float), mesh.R, dz,
+                          abi.ptr(out, C.c_float))
+    return out
+
+
+def destination(lat, lon, delta, az):
+    """Point at epicentral distance delta (deg) and azimuth az (deg) from (lat, lon)."""
+    la, lo, d, a = map(math.radians, (lat, lon, delta, az))
+    la2 = math.asin(math.sin(la) * math.cos(d) + math.cos(la) * math.sin(d) * math.cos(a))
+    lo2 = lo + math.atan2(math.sin(a) * math.sin(d) * math.cos(la),
+                          math.cos(d) - math.sin(la) * math.sin(la2))
+    return math.degrees(la2), (math.degrees(lo2) + 540.0) % 360.0 - 180.0
+
+
+def morton3(q):
+    """Morton codes of integer coordinates q[n][3] (21 bits each)."""
+    def spread(v):
+        v = v.astype(np.uint64) & np.uint64(0x1fffff)
+        v = (v | (v << np.uint64(32))) & np.uint64(0x1f00000000ffff)
+        v = (v | (v << np.uint64(16))) & np.uint64(0x1f0000ff0000ff)
+        v = (v | (v << np.uint64(8))) & np.uint64(0x100f00f00f00f00f)
+        v = (v | (v << np.uint64(4))) & np.uint64(0x10c30c30c30c30c3)
+        v = (v | (v << np.uint64(2))) & np.uint64(0x1249249249249249)
+        return v
+    return spread(q[:, 0]) | (spread(q[:, 1]) << np.uint64(1)) | (spread(q[:, 2]) << np.uint64(2))
+
+
+def random_tets(ntet, edge, seed, rmin=R_CMB, rmax=R_EARTH, accept=None, accept_frac=0.0):
+    """Regular tetrahedra of the given edge length (m) with random orientation, centres
+    uniform in the shell [rmin + edge, rmax - edge], in Morton order of their centres.
+    A fraction accept_frac of the centres is drawn from the region where accept(c) holds.
+    Returns vertices [4*ntet][3] and 1-based connectivity [ntet][4]."""
+    rng = np.random.default_rng(seed)
+    lo, hi = (rmin + edge) ** 3, (rmax - edge) ** 3
+
+    def draw(n):
+        r = np.cbrt(rng.uniform(lo, hi, n))
+        v = rng.normal(size=(n, 3))
+        v /= np.linalg.norm(v, axis=1)[:, None]
+        return v * r[:, None]
+
+    nacc = int(round(ntet * accept_frac)) if accept is not None else 0
+    parts = [draw(ntet - nacc)]
+    got = 0
+    while got < nacc:
+        c = draw(max(64, 8 * nacc))
+        c = c[accept(c)][: nacc - got]
+        parts.append(c)
+        got += len(c)
+    centres = np.concatenate(parts)
+    base = np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]], float)
+    base *= edge / (2.0 * math.sqrt(2.0))
+    # random rotations from normalised quaternions
+    q = rng.normal(size=(ntet, 4))
+    q /= np.linalg.norm(q, axis=1)[:, None]
+    w, x, y, z = q.T
+    rot = np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
+        np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
+        np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1),
+    ], 1)
+    verts = centres[:, None, :] + np.einsum("nij,kj->nki", rot, base)
+    qc = ((centres / rmax + 1.0) * 0.5 * (2 ** 21 - 1)).astype(np.int64)
+    order = np.argsort(morton3(qc), kind="stable")
+    verts = np.ascontiguousarray(verts[order].reshape(-1, 3))
+    conn = np.arange(1, 4 * ntet + 1, dtype=np.int32).reshape(ntet, 4)
+    return verts, conn
+
+
+def geo_xyz(lat, lon, r):
+    la, lo = math.radians(lat), math.radians(lon)
+    return np.array([math.cos(la) * math.cos(lo), math.cos(la) * math.sin(lo), math.sin(la)]) * r
+
+
+def wavelet(t, t0, period):
+    x = t - t0
+    return np.exp(-(x / period) ** 2) * np.cos(2 * np.pi * x / period)
+
+
+class Problem:
+    """All inputs of one workload plus the MckSetup that points into them."""
+
+    def __init__(self, name, nelem_target, nt, dt, receivers_deg, filters, kernels, ntet,
+                 tet_edge, npoints_per_step, max_iter, allowed_error=1e-4,
+                 allowed_relative_error=1e-2, component="Z", mij=None, seed=1,
+                 mesh_seed=2, src=(20.0, 30.0, 100.0), azimuth=135.0, deconv_stf=0,
+                 int_scheme="parseval", db_threads=None, fill=True, unmasked_frac=0.0):
+        self.name = name
+        self.mesh = Mesh(nelem_target)
+        self.nt, self.dt = nt, dt
+        m = self.mesh
+        self.fwd_u = fill_db(m, 10, nt, dt, seed * 1000 + 1, db_threads) if fill else None
+        self.bwd_u = fill_db(m, 5, nt, dt, seed * 1000 + 2, db_threads) if fill else None
+        self.model = model_table(m)
+        self.src_lat, self.src_lon, self.src_depth = src
+        if mij is None:
+            mij = [1.0e20, 1.0e20, 1.0e20, 0.0, 0.0, 0.0]   # explosion, N m
+        self.mij = np.array(mij, np.float64)
+        lats, lons = [], []
+        for d in receivers_deg:
+            la, lo = destination(self.src_lat, self.src_lon, d, azimuth)
+            lats.append(la)
+            lons.append(lo)
+        self.rec_lat = np.array(lats, np.float64)
+        self.rec_lon = np.array(lons, np.float64)
+        self.nrec = len(receivers_deg)
+        self.component = component
+        t = np.arange(nt) * dt
+        seis = np.zeros((self.nrec, nt))
+        for r, d in enumerate(receivers_deg):
+            tp = p_time(d)
+            for per in (20.0, 30.0, 40.0, 60.0):
+                seis[r] += wavelet(t, tp + 20.0, per) + 0.7 * wavelet(t, 1.8 * tp + 20.0, per)
+        self.seis = np.ascontiguousarray(seis * 1e-6)
+        self.stf = np.exp(-((t - 10.0) / (2.0 * dt)) ** 2)
+        self.stf /= self.stf.sum() * dt
+        # filters: list of (class, (f1, f2, f3, f4)); kernels: list of dicts
+        self.filters = (abi.MckFilter * len(filters))()
+        for i, (cls, fr) in enumerate(filters):
+            self.filters[i].filterclass = abi.FILTERS[cls]
+            for j in range(4):
+                self.filters[i].frequencies[j] = fr[j]
+        nk = len(kernels)
+        self.kernel_receiver = np.array([k["rec"] for k in kernels], np.int32)
+        self.kernel_filter = np.array([k["filter"] for k in kernels], np.int32)
+        self.kernel_misfit = np.array([abi.MISFITS[k["misfit"]] for k in kernels], np.int32)
+        self.kernel_param = np.array([abi.PARAMS[k["param"]] for k in kernels], np.int32)
+        win = []
+        for k in kernels:
+            tp = p_time(receivers_deg[k["rec"]])
+            t0 = (tp if k["param"] in ("vp", "lam", "vph", "vpv") else 1.8 * tp) + 10.0
+            win.append((t0, t0 + 60.0))
+        self.kernel_window = np.array(win, np.float64)
+        self.nkernel = nk
+        src_xyz = geo_xyz(self.src_lat, self.src_lon, R_EARTH - self.src_depth * 1e3)
+        rec_xyz = geo_xyz(self.rec_lat[0], self.rec_lon[0], R_EARTH)
+        vmax = float(self.model[0].max())
+        tlast = float(self.kernel_window[self.kernel_receiver == 0][:, 1].max())
+
+        def inside(c):   # not travel-time masked for the first receiver (slave.f90:928-950)
+            d = np.linalg.norm(c - src_xyz, axis=1) + np.linalg.norm(c - rec_xyz, axis=1)
+            return d / vmax < 0.9 * tlast
+
+        self.verts, self.conn = random_tets(ntet, tet_edge, mesh_seed, accept=inside,
+                                            accept_frac=unmasked_frac)
+        self.ntet = ntet
+        self.npoints_per_step = npoints_per_step
+        self.max_iter = max_iter
+        self.allowed_error = allowed_error
+        self.allowed_relative_error = allowed_relative_error
+        self.deconv_stf = deconv_stf
+        self.int_scheme = abi.INT_SCHEMES[int_scheme]
+        self.setup = self._make_setup()
+
+    def _db(self, u, nvar):
+        m = self.mesh
+        d = abi.MckDb()
+        d.npol, d.nelem, d.npoints = 4, m.nelem, m.npoints
+        d.mesh_s = abi.ptr(m.mesh_s, C.c_float)
+        d.mesh_z = abi.ptr(m.mesh_z, C.c_float)
+        d.mp_s = abi.ptr(m.mp_s, C.c_float)
+        d.mp_z = abi.ptr(m.mp_z, C.c_float)
+        d.fem_mesh = abi.ptr(m.fem, C.c_int32)
+        d.eltype = abi.ptr(m.eltype, C.c_int32)
+        d.axis = abi.ptr(m.axis, C.c_int32)
+        d.nvar = nvar
+        d.snapshots = abi.ptr(u, C.c_float)
+        d.amplitude = 1.0e20
+        return d
+
+    def _make_setup(self):
+        s = abi.MckSetup()
+        s.fwd = self._db(self.fwd_u, 10)
+        s.bwd = self._db(self.bwd_u, 5)
+        s.nt, s.dt, s.planet_radius = self.nt, self.dt, self.mesh.R
+        s.timeshift_fwd, s.timeshift_bwd = 15.0, 15.0
+        s.stf_fwd = abi.ptr(self.stf, C.c_double)
+        s.src_lat, s.src_lon, s.src_depth_km = self.src_lat, self.src_lon, self.src_depth
+        for i in range(6):
+            s.src_mij[i] = self.mij[i]
+        s.stf_src = None
+        s.stf_src_shift = 0.0
+        s.nrec = self.nrec
+        s.rec_lat = abi.ptr(self.rec_lat, C.c_double)
+        s.rec_lon = abi.ptr(self.rec_lon, C.c_double)
+        s.component = self.component.encode()
+        s.seis = abi.ptr(self.seis, C.c_double)
+        s.nfilter = len(self.filters)
+        s.filters = self.filters
+        s.nkernel = self.nkernel
+        s.kernel_receiver = abi.ptr(self.kernel_receiver, C.c_int32)
+        s.kernel_filter = abi.ptr(self.kernel_filter, C.c_int32)
+        s.kernel_misfit = abi.ptr(self.kernel_misfit, C.c_int32)
+        s.kernel_param = abi.ptr(self.kernel_param, C.c_int32)
+        s.kernel_window = abi.ptr(self.kernel_window, C.c_double)
+        s.model_ndepth = self.model.shape[1]
+        s.model_dr = 100.0
+        names = ["model_vp", "model_vs", "model_rho", "model_phi", "model_xi", "model_eta"]
+        for i, nm in enumerate(names):
+            setattr(s, nm, abi.ptr(self.model[i], C.c_float))
+        s.vp_max = float(self.model[0].max())
+        s.vs_max = float(self.model[1].max())
+        s.npoints_per_step = self.npoints_per_step
+        s.max_iter = self.max_iter
+        s.allowed_error = self.allowed_error
+        s.allowed_relative_error = self.allowed_relative_error
+        s.int_over_volume = 1
+        s.relative_kernel = 1
+        s.quasirandom = 1
+        s.int_scheme = self.int_scheme
+        s.deconv_stf = self.deconv_stf
+        s.damp_radius = -1.0
+        s.mask_src_rec = 0
+        return s
+
+    def sample_points(self, npts, seed=3):
+        """Uniform points inside the first tetrahedra (for point-level parity tests)."""
+        rng = np.random.default_rng(seed)
+        tet = rng.integers(0, self.ntet, npts)
+        b = rng.dirichlet(np.ones(4), npts)
+        v = self.verts.reshape(self.ntet, 4, 3)[tet]
+        return np.ascontiguousarray(np.einsum("nk,nkj->nj", b, v))
+
+
+def _kernels(nrec, nfilter, misfits, params):
+    out = []
+    for r in range(nrec):
+        for f in range(nfilter):
+            for mf in misfits:
+                for p in params:
+                    out.append({"rec": r, "filter": f, "misfit": mf, "param": p})
+    return out
+
+
+def make_problem(name, **over):
+    """Named workloads (see module docstring); keyword arguments override fields."""
+    if name == "c0":
+        kw = dict(nelem_target=2000, nt=1024, dt=1.0, receivers_deg=[30.0],
+                  filters=[("Gabor", (40.0, 0.5, 0.0, 0.0)), ("Gabor", (30.0, 0.5, 0.0, 0.0))],
+                  kernels=_kernels(1, 2, ["CC"], ["vp"]), ntet=500, tet_edge=400e3,
+                  npoints_per_step=32, max_iter=8)
+    elif name == "c1":
+        kw = dict(nelem_target=20000, nt=2048, dt=1.0,
+                  receivers_deg=[10.0 * (i + 1) for i in range(10)],
+                  filters=[("Gabor", (p, 0.5, 0.0, 0.0)) for p in (20.0, 30.0, 40.0, 60.0)],
+                  kernels=_kernels(10, 4, ["CC", "AM"], ["vp", "vs"]), ntet=50000,
+                  tet_edge=300e3, npoints_per_step=16, max_iter=64,
+                  allowed_error=1e-30, allowed_relative_error=1e-3)
+    elif name == "tiny":
+        kw = dict(nelem_target=450, nt=256, dt=4.0, receivers_deg=[30.0, 60.0],
+                  filters=[("Gabor", (40.0, 0.5, 0.0, 0.0)), ("Gabor", (60.0, 0.5, 0.0, 0.0))],
+                  kernels=_kernels(2, 2, ["CC", "AM"], ["vp", "vs"]), ntet=24,
+                  tet_edge=500e3, npoints_per_step=8, max_iter=6, unmasked_frac=0.75)
+    else:
+        raise ValueError(name)
+    kw.update(over)
+    return Problem(name, **kw)

@@ -1,0 +1,97 @@
+/*
+ * mck_oracle.h - FP64 CPU restatement of MC Kernel's per-point kernel evaluation and
+ * Monte Carlo integration path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP implementation:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ * The product (mc_kernel_amd) never links, loads or calls it.
+ *
+ * Every function follows one reference routine, cited file:line in mck_oracle.c,
+ * restated operation by operation (same evaluation order, no FMA contraction:
+ * built with -ffp-contract=off) so that element indices and Halton points are
+ * bit-identical to the reference's.
+ *
+ * Parity pins (tests/test_oracle_*.py): Halton known answers (test_halton.f90),
+ * GLL/GLJ nodes and derivative matrices (test_spectral_basis.f90), mapping round trips,
+ * polynomial strain closed forms (test_sem_derivatives.f90), FFT/taper/Gabor golden
+ * responses (tests/gaborresponse_*.txt), Butterworth golden responses, cumsum fixtures,
+ * MC known answers (test_mc_integration.f90), tet volume of tests/vertices.TEST, and the
+ * reference's own kdtree2.f90 (compiled into oracle/_ref) for the 12-nearest order.
+ */
+#ifndef MCK_ORACLE_H
+#define MCK_ORACLE_H
+
+#include <stdint.h>
+#include "../include/mck.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORA_PI 3.1415926535898   /* global_parameters.f90:40 (truncated pi) */
+
+/* spectral basis */
+void ora_gll_points(int n, double *x);
+void ora_glj_points(int n, double *x);
+void ora_lagrange_derivs(int n, const double *x, double *G);   /* G[i + j*(n+1)] = l_i'(x_j) */
+void ora_lagrange_basis(int n, const double *pts, double x, double *l);
+
+/* halton / geometry */
+void ora_halton(int ndim, const int64_t *seed, const int32_t *base, const int64_t *leap,
+                int64_t step, int npoints, double *r);       /* r[npoints][ndim] */
+void ora_tet_points(const double *v, int npoints, const double *coords, double *out);
+double ora_tet_volume(const double *v);                      /* v[4][3] */
+double ora_norm2(int n, const double *x);                    /* gfortran NORM2 */
+
+/* mapping */
+void ora_mapping(double xi, double eta, const double *nodes, int eltype, double *sz);
+void ora_inv_jacobian(double xi, double eta, const double *nodes, int eltype, double *ij);
+void ora_inv_mapping(double s, double z, const double *nodes, int eltype, double *xieta);
+int  ora_inside_element(double s, double z, const double *nodes, int eltype, double tol,
+                        double *xi, double *eta);
+
+/* strain at the 25 nodes of one element: u[var][jpol][ipol][nsamp] (merged block, dp) */
+void ora_strain_pole(int pole, const double *u, int nsamp, int npol, const double *nodes,
+                     int eltype, int axial, double *strain);  /* strain[6][jpol][ipol][nsamp] */
+void ora_straintrace_pole(int pole, const double *u, int nsamp, int npol,
+                          const double *nodes, int eltype, int axial, double *tr);
+
+/* rotations */
+void ora_rotate_frame(int n, const double *xyz, double lon, double colat,
+                      double *s, double *phi, double *z);
+
+/* FFT / filtering */
+int  ora_nextpow2(int n);
+void ora_rfft(int n, double dt, const double *x, double *X);     /* X[(n/2+1)][2]   */
+void ora_irfft(int n, double df, const double *X, double *x);
+void ora_taperandzeropad(int nin, int nout, int ntaper, int end_only, const double *x,
+                         double *out);
+void ora_cumsum_trapezoidal(int n, double dt, const double *x, double *out);
+void ora_filter_tf(const mck_filter_t *f, int nfreq, double df, double *tf); /* [nfreq][2] */
+
+/* kNN over element midpoints (fp32 squared distances, kdtree2.f90:1541-1552) */
+int  ora_nearest(int nelem, const float *mp_s, const float *mp_z, double s, double z,
+                 int nn, int32_t *idx, float *dist);   /* returns 1 if a tie affects order */
+
+/* whole-path context */
+typedef struct ora_ctx ora_ctx;
+ora_ctx *ora_create(const mck_setup_t *setup, char *err, int errlen);
+void     ora_destroy(ora_ctx *ctx);
+int      ora_locate(ora_ctx *ctx, int frame, int npts, const double *xyz, int32_t *elem,
+                    double *xi, double *eta, int32_t *ties);
+int      ora_load_strain(ora_ctx *ctx, int frame, int npts, const double *xyz,
+                         double *traces);      /* [npts][nd][nt] */
+int      ora_base_kernels(ora_ctx *ctx, int npts, const double *xyz, double *base);
+int      ora_slave_work(ora_ctx *ctx, int nelems, const double *vertices,
+                        const int32_t *conn, double *values, double *variance,
+                        int32_t *niterations, int64_t *halton_step,
+                        const int64_t *halton_offsets, int64_t *offsets_used,
+                        int64_t *evaluations);
+/* seismogram/kernel setup products (kernel.f90:181-385) for inspection */
+int      ora_kernel_info(ora_ctx *ctx, int k, double *normalization, int *ncut, int *ifirst);
+int      ora_strain_nd(ora_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,911 @@
+/*
+ * mck_oracle_path.c - FP64 CPU restatement of load_fw_points / load_bw_points, the
+ * kernel set-up (filters, seismogram windows) and slave_work's Monte Carlo loop.
+ * TEST INFRASTRUCTURE: the checker for the HIP implementation (see mck_oracle.h).
+ */
+#include "mck_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define DEG2RAD (ORA_PI / 180.0)
+
+typedef struct { double re, im; } cz;
+static cz zmul(cz a, cz b) { cz r; r.re = a.re * b.re - a.im * b.im; r.im = a.re * b.im + a.im * b.re; return r; }
+static cz zdiv(cz a, cz b)
+{
+    cz r;
+    if (fabs(b.re) >= fabs(b.im)) {
+        double ratio = b.im / b.re, den = b.re + b.im * ratio;
+        r.re = (a.re + a.im * ratio) / den; r.im = (a.im - a.re * ratio) / den;
+    } else {
+        double ratio = b.re / b.im, den = b.im + b.re * ratio;
+        r.re = (a.re * ratio + a.im) / den; r.im = (a.im * ratio - a.re) / den;
+    }
+    return r;
+}
+
+typedef struct {
+    int nd_needed;
+    int needs_bk[6];
+    int filter, misfit, param, rec;
+    double tw[2];
+    double normalization;
+    int ifirst, ncut, M;
+    double *seis_cut;      /* [ncut] */
+    double *seis_cut_fd;   /* [(M/2+1)][2] */
+} ora_kernel;
+
+typedef struct {
+    double lat, lon, colat, r[3];
+    int first, last;
+    int needs_bk[6];
+    double t_last_p, t_last_s;
+} ora_rec;
+
+struct ora_ctx {
+    mck_setup_t S;
+    int N, nomega;
+    double df;
+    int full, nd;
+    double gll[8], glj[8];
+    double *tf_fwd, *tf_bwd;   /* [nfilter][nomega][2] */
+    double *ts_fwd, *ts_bwd;   /* [nomega][2] time shift operators (deconv_stf == 0) */
+    ora_kernel *K;
+    ora_rec *R;
+    double src_lat, src_lon, src_colat, src_r[3];
+    int64_t halton_step;
+};
+
+static int param_needs(int param, int *nb)
+{
+    /* tabulate_kernels (kernel.f90:576-628); nb = lambda, mu, rho, a, b, c */
+    memset(nb, 0, 6 * sizeof(int));
+    switch (param) {
+    case MCK_PARAM_LAM: case MCK_PARAM_VP: nb[0] = 1; break;
+    case MCK_PARAM_VS: nb[0] = nb[1] = 1; break;
+    case MCK_PARAM_MU: nb[1] = 1; break;
+    case MCK_PARAM_VSH: nb[0] = nb[1] = nb[4] = nb[5] = 1; break;
+    case MCK_PARAM_VSV: nb[4] = 1; break;
+    case MCK_PARAM_VPH: nb[3] = nb[5] = 1; break;
+    case MCK_PARAM_VPV: case MCK_PARAM_ETA: nb[0] = nb[3] = nb[5] = 1; break;
+    default: return -1;
+    }
+    return 0;
+}
+
+/* apply time shift exp(2 pi i f tau) (filtering.f90:622-652) */
+static void make_timeshift(int nomega, double df, double tau, double *ts)
+{
+    int k;
+    for (k = 0; k < nomega; ++k) {
+        double th = 2 * ORA_PI * (df * k) * tau;
+        ts[2 * k] = cos(th);
+        ts[2 * k + 1] = sin(th);
+    }
+}
+
+/* add_stfs (filtering.f90:241-470) */
+static int add_stfs(ora_ctx *C, double *tff, double *tfb, char *err, int errlen)
+{
+    const mck_setup_t *S = &C->S;
+    int N = C->N, nw = C->nomega, k;
+    double *x = (double *)calloc(N, sizeof(double)), *pad = (double *)calloc(N, sizeof(double));
+    double *Fs = (double *)malloc(2 * nw * sizeof(double)), *Fe = (double *)malloc(2 * nw * sizeof(double));
+    double *f = (double *)malloc(nw * sizeof(double));
+    mck_filter_t lpf;
+    double *lp = (double *)malloc(2 * nw * sizeof(double));
+    double maxf = 0, maxb = 0, rf, rb;
+    for (k = 0; k < nw; ++k) f[k] = C->df * k;
+    ora_taperandzeropad(S->nt, N, 2, 0, S->stf_fwd, pad);
+    ora_rfft(N, S->dt, pad, Fs);
+    for (k = 0; k < S->nt; ++k) x[k] = S->stf_src ? S->stf_src[k] : (k == 0 ? 1.0 : 0.0);
+    ora_taperandzeropad(S->nt, N, 2, 1, x, pad);
+    ora_rfft(N, S->dt, pad, Fe);
+    lpf.filterclass = MCK_FILTER_BUTTER_LP;
+    lpf.frequencies[0] = 1.0 / (f[nw - 1] * 0.75);
+    lpf.frequencies[1] = 0; lpf.frequencies[2] = 12; lpf.frequencies[3] = 0;
+    ora_filter_tf(&lpf, nw, C->df, lp);
+    for (k = 0; k < nw; ++k) {
+        cz t = {tff[2 * k], tff[2 * k + 1]}, s = {Fs[2 * k], Fs[2 * k + 1]};
+        cz e = {Fe[2 * k], Fe[2 * k + 1]}, l = {lp[2 * k], lp[2 * k + 1]}, lc = {lp[2 * k], -lp[2 * k + 1]};
+        cz fw, bw, sh;
+        double th = 2 * ORA_PI * f[k] * S->stf_src_shift;
+        sh.re = cos(th); sh.im = sin(th);
+        l = zmul(l, lc);
+        fw = zdiv(t, s); fw.re *= S->fwd.amplitude; fw.im *= S->fwd.amplitude;
+        bw = zdiv(t, s); bw.re *= S->fwd.amplitude; bw.im *= S->fwd.amplitude;
+        fw = zmul(fw, e);
+        fw = zmul(fw, sh);
+        fw = zmul(fw, l);
+        bw = zmul(bw, l);
+        if (isnan(hypot(fw.re, fw.im))) { fw.re = 0; fw.im = 0; }
+        if (isnan(hypot(bw.re, bw.im))) { bw.re = 0; bw.im = 0; }
+        tff[2 * k] = fw.re; tff[2 * k + 1] = fw.im;
+        tfb[2 * k] = bw.re; tfb[2 * k + 1] = bw.im;
+        if (hypot(fw.re, fw.im) > maxf) maxf = hypot(fw.re, fw.im);
+        if (hypot(bw.re, bw.im) > maxb) maxb = hypot(bw.re, bw.im);
+    }
+    rf = hypot(tff[2 * (nw - 1)], tff[2 * (nw - 1) + 1]) / maxf;
+    rb = hypot(tfb[2 * (nw - 1)], tfb[2 * (nw - 1) + 1]) / maxb;
+    free(x); free(pad); free(Fs); free(Fe); free(f); free(lp);
+    if (rf > 1e-2 || rb > 1e-2) {
+        snprintf(err, errlen, "filter not vanishing at the highest frequency (%g, %g)", rf, rb);
+        return -1;
+    }
+    return 0;
+}
+
+/* integrate_parseval (kernel.f90:975-1020): Re[(ab_1 + ab_n + 2 sum ab_mid) df] */
+static double parseval(int M, double dt, const double *af, const double *bf)
+{
+    int nw = M / 2 + 1, k;
+    double df = 1. / (dt * M);
+    cz s = {0, 0}, a0, an, tot;
+    for (k = 1; k < nw - 1; ++k) {
+        cz a = {af[2 * k], af[2 * k + 1]}, b = {bf[2 * k], -bf[2 * k + 1]};
+        cz p = zmul(a, b);
+        s.re += p.re; s.im += p.im;
+    }
+    { cz a = {af[0], af[1]}, b = {bf[0], -bf[1]}; a0 = zmul(a, b); }
+    { cz a = {af[2 * (nw - 1)], af[2 * (nw - 1) + 1]}, b = {bf[2 * (nw - 1)], -bf[2 * (nw - 1) + 1]}; an = zmul(a, b); }
+    tot.re = a0.re + an.re + 2 * s.re;
+    tot.im = a0.im + an.im + 2 * s.im;
+    return tot.re * df;
+}
+
+/* cut_and_add_seismogram (kernel.f90:181-385) */
+static int kernel_init(ora_ctx *C, int k, char *err, int errlen)
+{
+    const mck_setup_t *S = &C->S;
+    ora_kernel *K = &C->K[k];
+    int N = C->N, nw = C->nomega, i, r = S->kernel_receiver[k];
+    const double *seis = S->seis + (size_t)r * S->nt;
+    double *disp = (double *)calloc(N, sizeof(double)), *velo = (double *)calloc(N, sizeof(double));
+    double *Xd = (double *)malloc(2 * nw * sizeof(double)), *Xv = (double *)malloc(2 * nw * sizeof(double));
+    double *d = (double *)malloc(N * sizeof(double)), *v = (double *)malloc(N * sizeof(double));
+    const double *tf = C->tf_fwd + (size_t)K->filter * 2 * nw;
+    double *pad, *cutfd, term;
+    ora_cumsum_trapezoidal(S->nt, S->dt, seis, disp);
+    for (i = 0; i < S->nt; ++i) velo[i] = seis[i];
+    ora_rfft(N, S->dt, disp, Xd);
+    ora_rfft(N, S->dt, velo, Xv);
+    for (i = 0; i < nw; ++i) {
+        cz a = {Xd[2 * i], Xd[2 * i + 1]}, b = {Xv[2 * i], Xv[2 * i + 1]}, t = {tf[2 * i], tf[2 * i + 1]};
+        if (!S->deconv_stf) {
+            cz sh = {C->ts_fwd[2 * i], C->ts_fwd[2 * i + 1]};
+            a = zmul(a, sh); b = zmul(b, sh);
+        }
+        a = zmul(a, t); b = zmul(b, t);
+        Xd[2 * i] = a.re; Xd[2 * i + 1] = a.im; Xv[2 * i] = b.re; Xv[2 * i + 1] = b.im;
+    }
+    ora_irfft(N, C->df, Xd, d);
+    ora_irfft(N, C->df, Xv, v);
+    /* cut_timewindow (kernel.f90:836-927) on t_i = (i-1) dt */
+    if (K->tw[0] < 0.0 || K->tw[1] > (N - 1) * S->dt || K->tw[1] - K->tw[0] <= 0) {
+        snprintf(err, errlen, "kernel %d: invalid time window", k);
+        free(disp); free(velo); free(Xd); free(Xv); free(d); free(v);
+        return -1;
+    }
+    K->ifirst = -1; K->ncut = 0;
+    for (i = 0; i < N; ++i) {
+        double t = i * S->dt;
+        if (t <= K->tw[1] && t >= K->tw[0]) { if (K->ifirst < 0) K->ifirst = i; K->ncut++; }
+        if (t > K->tw[1]) break;
+    }
+    if (K->ncut == 0) {
+        snprintf(err, errlen, "kernel %d: time window length was zero", k);
+        free(disp); free(velo); free(Xd); free(Xv); free(d); free(v);
+        return -1;
+    }
+    K->seis_cut = (double *)malloc(K->ncut * sizeof(double));
+    for (i = 0; i < K->ncut; ++i)
+        K->seis_cut[i] = (K->misfit == MCK_MISFIT_CC ? v : d)[K->ifirst + i];
+    K->M = 2 * ora_nextpow2(K->ncut);
+    pad = (double *)calloc(K->M, sizeof(double));
+    for (i = 0; i < K->ncut; ++i) pad[i] = K->seis_cut[i];
+    K->seis_cut_fd = (double *)malloc(2 * (K->M / 2 + 1) * sizeof(double));
+    ora_rfft(K->M, S->dt, pad, K->seis_cut_fd);
+    cutfd = (double *)malloc(2 * (K->M / 2 + 1) * sizeof(double));
+    ora_rfft(K->M, S->dt, pad, cutfd);
+    term = parseval(K->M, S->dt, cutfd, cutfd);
+    K->normalization = (term > 1e-100) ? 1.0 / term : 0.0;
+    free(pad); free(cutfd);
+    free(disp); free(velo); free(Xd); free(Xv); free(d); free(v);
+    return 0;
+}
+
+ora_ctx *ora_create(const mck_setup_t *S, char *err, int errlen)
+{
+    ora_ctx *C = (ora_ctx *)calloc(1, sizeof(ora_ctx));
+    int k, r, f;
+    C->S = *S;
+    C->N = 2 * ora_nextpow2(S->nt);
+    C->nomega = C->N / 2 + 1;
+    C->df = 1. / (S->dt * C->N);
+    ora_gll_points(4, C->gll);
+    ora_glj_points(4, C->glj);
+    if (S->fwd.npol != 4 || S->bwd.npol != 4) {
+        snprintf(err, errlen, "only npol = 4 is supported"); free(C); return NULL;
+    }
+    /* strain type (type_parameter.f90:477-486) */
+    C->full = 0;
+    for (k = 0; k < S->nkernel; ++k) {
+        int p = S->kernel_param[k];
+        if (p == MCK_PARAM_RHO) { snprintf(err, errlen, "rho kernels are not implemented (kernel.f90:554)"); free(C); return NULL; }
+        if (!(p == MCK_PARAM_LAM || p == MCK_PARAM_VP || p == MCK_PARAM_VPH || p == MCK_PARAM_VPV)) C->full = 1;
+    }
+    C->nd = C->full ? 6 : 1;
+    C->src_lat = S->src_lat * DEG2RAD;
+    C->src_lon = S->src_lon * DEG2RAD;
+    C->src_colat = (90 - S->src_lat) * DEG2RAD;
+    {
+        double rad = S->planet_radius - S->src_depth_km * 1e3;
+        C->src_r[0] = cos(C->src_lat) * cos(C->src_lon) * rad;
+        C->src_r[1] = cos(C->src_lat) * sin(C->src_lon) * rad;
+        C->src_r[2] = sin(C->src_lat) * rad;
+    }
+    C->R = (ora_rec *)calloc(S->nrec, sizeof(ora_rec));
+    for (r = 0; r < S->nrec; ++r) {
+        ora_rec *R = &C->R[r];
+        R->lat = S->rec_lat[r] * DEG2RAD;
+        R->lon = S->rec_lon[r] * DEG2RAD;
+        R->colat = (90 - S->rec_lat[r]) * DEG2RAD;
+        R->r[0] = cos(R->lat) * cos(R->lon) * S->planet_radius;
+        R->r[1] = cos(R->lat) * sin(R->lon) * S->planet_radius;
+        R->r[2] = sin(R->lat) * S->planet_radius;
+        R->first = -1; R->last = -2;
+    }
+    /* filters (filtering.f90:78-230) and STF deconvolution (add_stfs) */
+    C->tf_fwd = (double *)malloc((size_t)S->nfilter * 2 * C->nomega * sizeof(double));
+    C->tf_bwd = (double *)malloc((size_t)S->nfilter * 2 * C->nomega * sizeof(double));
+    for (f = 0; f < S->nfilter; ++f) {
+        double *a = C->tf_fwd + (size_t)f * 2 * C->nomega, *b = C->tf_bwd + (size_t)f * 2 * C->nomega;
+        ora_filter_tf(&S->filters[f], C->nomega, C->df, a);
+        memcpy(b, a, 2 * C->nomega * sizeof(double));
+        if (S->deconv_stf && add_stfs(C, a, b, err, errlen)) { ora_destroy(C); return NULL; }
+    }
+    C->ts_fwd = (double *)malloc(2 * C->nomega * sizeof(double));
+    C->ts_bwd = (double *)malloc(2 * C->nomega * sizeof(double));
+    make_timeshift(C->nomega, C->df, S->timeshift_fwd, C->ts_fwd);
+    make_timeshift(C->nomega, C->df, S->timeshift_bwd, C->ts_bwd);
+    /* kernels (type_parameter.f90:521-691) */
+    C->K = (ora_kernel *)calloc(S->nkernel, sizeof(ora_kernel));
+    for (k = 0; k < S->nkernel; ++k) {
+        ora_kernel *K = &C->K[k];
+        ora_rec *R;
+        int b;
+        K->rec = S->kernel_receiver[k];
+        K->filter = S->kernel_filter[k];
+        K->misfit = S->kernel_misfit[k];
+        K->param = S->kernel_param[k];
+        K->tw[0] = S->kernel_window[2 * k];
+        K->tw[1] = S->kernel_window[2 * k + 1];
+        if (param_needs(K->param, K->needs_bk)) { snprintf(err, errlen, "bad model parameter"); ora_destroy(C); return NULL; }
+        if (!C->full && (K->needs_bk[1] || K->needs_bk[3] || K->needs_bk[4] || K->needs_bk[5])) {
+            snprintf(err, errlen, "kernel %d needs the full strain tensor but only the trace is loaded", k);
+            ora_destroy(C); return NULL;
+        }
+        R = &C->R[K->rec];
+        if (R->first < 0) R->first = k;
+        R->last = k;
+        for (b = 0; b < 6; ++b) R->needs_bk[b] |= K->needs_bk[b];
+        if (K->param == MCK_PARAM_VP || K->param == MCK_PARAM_VPH || K->param == MCK_PARAM_VPV || K->param == MCK_PARAM_LAM)
+            R->t_last_p = fmax(R->t_last_p, K->tw[1]);
+        else
+            R->t_last_s = fmax(R->t_last_s, K->tw[1]);
+        if (kernel_init(C, k, err, errlen)) { ora_destroy(C); return NULL; }
+    }
+    C->halton_step = 0;
+    return C;
+}
+
+void ora_destroy(ora_ctx *C)
+{
+    int k;
+    if (!C) return;
+    if (C->K) for (k = 0; k < C->S.nkernel; ++k) { free(C->K[k].seis_cut); free(C->K[k].seis_cut_fd); }
+    free(C->K); free(C->R); free(C->tf_fwd); free(C->tf_bwd); free(C->ts_fwd); free(C->ts_bwd);
+    free(C);
+}
+
+int ora_strain_nd(ora_ctx *C) { return C->nd; }
+
+int ora_kernel_info(ora_ctx *C, int k, double *normalization, int *ncut, int *ifirst)
+{
+    *normalization = C->K[k].normalization;
+    *ncut = C->K[k].ncut;
+    *ifirst = C->K[k].ifirst;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------
+ * find_element (readfields.f90:1052-1134)
+ * ---------------------------------------------------------------------------------- */
+static int find_element(const mck_db_t *db, double s, double z, double *xi, double *eta,
+                        double *nodes, int *id_elem, int *tie)
+{
+    int32_t idx[12];
+    float dist[12];
+    int c, i;
+    *tie = ora_nearest(db->nelem, db->mp_s, db->mp_z, s, z, 12, idx, dist);
+    for (c = 0; c < 12; ++c) {
+        int e = idx[c];
+        for (i = 0; i < 4; ++i) {
+            int p = db->fem_mesh[4 * e + i];
+            nodes[i] = (double)db->mesh_s[p];
+            nodes[4 + i] = (double)db->mesh_z[p];
+        }
+        if (ora_inside_element(s, z, nodes, db->eltype[e], 1e-3, xi, eta)) {
+            *id_elem = e;
+            return 0;
+        }
+    }
+    return -1;
+}
+
+static void frame_of(ora_ctx *C, int frame, double *lon, double *colat)
+{
+    if (frame == 0) { *lon = C->src_lon; *colat = C->src_colat; }
+    else { *lon = C->R[frame - 1].lon; *colat = C->R[frame - 1].colat; }
+}
+
+int ora_locate(ora_ctx *C, int frame, int npts, const double *xyz, int32_t *elem,
+               double *xi, double *eta, int32_t *ties)
+{
+    const mck_db_t *db = frame == 0 ? &C->S.fwd : &C->S.bwd;
+    double lon, colat, R = C->S.planet_radius;
+    int p;
+    frame_of(C, frame, &lon, &colat);
+    for (p = 0; p < npts; ++p) {
+        double s, phi, z, d2, nodes[8];
+        int id, tie;
+        ora_rotate_frame(1, xyz + 3 * p, lon, colat, &s, &phi, &z);
+        d2 = s * s + z * z;
+        if (d2 > R * R) { s = s / sqrt(d2) * R; z = z / sqrt(d2) * R; }
+        if (find_element(db, s, z, &xi[p], &eta[p], nodes, &id, &tie)) return MCK_ENOTFOUND;
+        elem[p] = id + 1;
+        if (ties) ties[p] = tie;
+    }
+    return 0;
+}
+
+/* azim_factor (rotations.f90:89-131) */
+static double azim_factor(double phi, const double *mij, int isim, int ikind)
+{
+    switch (isim) {
+    case 1: return ikind == 1 ? mij[0] : 0.0;
+    case 2: return ikind == 1 ? 0.5 * (mij[1] + mij[2]) : 0.0;
+    case 3: return ikind == 1 ? mij[3] * cos(phi) + mij[4] * sin(phi)
+                              : -mij[3] * sin(phi) + mij[4] * cos(phi);
+    default: return ikind == 1 ? 0.5 * (mij[1] - mij[2]) * cos(2.0 * phi) + mij[5] * sin(2.0 * phi)
+                               : -0.5 * (mij[1] - mij[2]) * sin(2.0 * phi) + mij[5] * cos(2.0 * phi);
+    }
+}
+
+/* azim_factor_bw (rotations.f90:197-227) */
+static double azim_factor_bw(double phi, const double *fi, int isim, int ikind)
+{
+    if (isim == 1) return ikind == 1 ? fi[0] : 0.0;
+    return ikind == 1 ? fi[1] * cos(phi) + fi[2] * sin(phi) : -fi[1] * sin(phi) + fi[2] * cos(phi);
+}
+
+/* rotate_symm_tensor_voigt_src_to_xyz_2d (rotations.f90:382-419) */
+static void rot_src_to_xyz(double *t, int nt, double phi)
+{
+    double sp = sin(phi), cp = cos(phi);
+    int i;
+    for (i = 0; i < nt; ++i) {
+        double a1 = t[0 * nt + i], a2 = t[1 * nt + i], a3 = t[2 * nt + i];
+        double a4 = t[3 * nt + i], a5 = t[4 * nt + i], a6 = t[5 * nt + i];
+        t[0 * nt + i] = a1 * (cp * cp) + sp * (-2 * a6 * cp + a2 * sp);
+        t[1 * nt + i] = a2 * (cp * cp) + sp * (2 * a6 * cp + a1 * sp);
+        t[2 * nt + i] = a3;
+        t[3 * nt + i] = a4 * cp + a5 * sp;
+        t[4 * nt + i] = a5 * cp - a4 * sp;
+        t[5 * nt + i] = (a1 - a2) * cp * sp + a6 * (cp * cp - sp * sp);
+    }
+}
+
+/* rotate_symm_tensor_voigt_xyz_src_to_xyz_earth_2d (rotations.f90:555-612) */
+static void rot_xyz_src_to_earth(double *t, int nt, double phi, double theta)
+{
+    double sp = sin(phi), cp = cos(phi), st = sin(theta), ct = cos(theta);
+    int i;
+    for (i = 0; i < nt; ++i) {
+        double a1 = t[0 * nt + i], a2 = t[1 * nt + i], a3 = t[2 * nt + i];
+        double a4 = t[3 * nt + i], a5 = t[4 * nt + i], a6 = t[5 * nt + i];
+        t[0 * nt + i] = a1 * (cp * cp) * (ct * ct) - 2 * a6 * cp * ct * sp + a2 * (sp * sp)
+                      + 2 * a5 * (cp * cp) * ct * st - 2 * a4 * cp * sp * st
+                      + a3 * (cp * cp) * (st * st);
+        t[1 * nt + i] = a2 * (cp * cp)
+                      + sp * (2 * a6 * cp * ct + a1 * (ct * ct) * sp
+                              + st * (2 * a4 * cp + 2 * a5 * ct * sp + a3 * sp * st));
+        t[2 * nt + i] = a3 * (ct * ct) + st * (-2 * a5 * ct + a1 * st);
+        t[3 * nt + i] = ct * (a4 * cp + a5 * ct * sp + a3 * sp * st)
+                      - st * (a6 * cp + a1 * ct * sp + a5 * sp * st);
+        t[4 * nt + i] = ct * (a5 * cp * ct - a4 * sp + a3 * cp * st)
+                      - st * (a1 * cp * ct - a6 * sp + a5 * cp * st);
+        t[5 * nt + i] = cp * st * (a4 * cp + a5 * ct * sp + a3 * sp * st)
+                      - sp * (a2 * cp + a6 * ct * sp + a4 * sp * st)
+                      + cp * ct * (a6 * cp + a1 * ct * sp + a5 * sp * st);
+    }
+}
+
+/* lagrange_interpol_2D_td (spectral_basis.f90:181-314) of node values
+ * val[c][jpol][ipol][nt] -> out[c][nt] */
+static void interp_nodes(int nc, int nt, const double *val, const double *xp, const double *ep,
+                         double x1, double x2, double *out)
+{
+    double li[5], lj[5];
+    int c, i, j, t;
+    ora_lagrange_basis(4, xp, x1, li);
+    ora_lagrange_basis(4, ep, x2, lj);
+    for (c = 0; c < nc; ++c) {
+        double *o = out + (size_t)c * nt;
+        for (t = 0; t < nt; ++t) o[t] = 0.0;
+        for (i = 0; i <= 4; ++i)
+            for (j = 0; j <= 4; ++j) {
+                const double *v = val + (((size_t)c * 5 + j) * 5 + i) * nt;
+                for (t = 0; t < nt; ++t) o[t] = o[t] + v[t] * li[i] * lj[j];
+            }
+    }
+}
+
+/* Strain at (xi, eta) of one element for simulation `sim` (1-based within the DB):
+ * load_strain_point_merged_fwd/bwd (readfields.f90:2009-2403); out[nd][nt]. */
+static void strain_point(ora_ctx *C, const mck_db_t *db, int e, int pole, int var0,
+                         double xi, double eta, const double *nodes, double *out)
+{
+    int nt = C->S.nt, nd = C->nd, v, k, axial = db->axis[e];
+    size_t blk = 25 * (size_t)nt;
+    double *u = (double *)malloc(3 * blk * sizeof(double));
+    double *st = (double *)malloc(6 * blk * sizeof(double));
+    int nv = pole == 0 ? 2 : 3;
+    const float *src = db->snapshots + ((size_t)e * db->nvar + var0) * blk;
+    for (v = 0; v < nv; ++v)
+        for (k = 0; k < (int)blk; ++k) u[v * blk + k] = (double)src[v * blk + k];
+    if (C->full) ora_strain_pole(pole, u, nt, 4, nodes, db->eltype[e], axial, st);
+    else ora_straintrace_pole(pole, u, nt, 4, nodes, db->eltype[e], axial, st);
+    interp_nodes(nd, nt, st, axial ? C->glj : C->gll, C->gll, xi, eta, out);
+    if (C->full) {
+        for (k = 0; k < nt; ++k) { out[3 * nt + k] = -out[3 * nt + k]; out[5 * nt + k] = -out[5 * nt + k]; }
+    }
+    free(u); free(st);
+}
+
+/* load_fw_points (readfields.f90:1138-1308) / load_bw_points (readfields.f90:1312-1465);
+ * out[npts][nd][nt] */
+static int load_points(ora_ctx *C, int frame, int npts, const double *xyz, double *out)
+{
+    const mck_setup_t *S = &C->S;
+    const mck_db_t *db = frame == 0 ? &S->fwd : &S->bwd;
+    int nt = S->nt, nd = C->nd, p, c, k;
+    double lon, colat, R = S->planet_radius;
+    double *ut = (double *)malloc((size_t)nd * nt * sizeof(double));
+    frame_of(C, frame, &lon, &colat);
+    for (p = 0; p < npts; ++p) {
+        double s, phi, z, d2, xi, eta, nodes[8];
+        int e, tie;
+        double *o = out + (size_t)p * nd * nt;
+        ora_rotate_frame(1, xyz + 3 * p, lon, colat, &s, &phi, &z);
+        d2 = s * s + z * z;
+        if (d2 > R * R) { s = s / sqrt(d2) * R; z = z / sqrt(d2) * R; }
+        if (find_element(db, s, z, &xi, &eta, nodes, &e, &tie)) { free(ut); return MCK_ENOTFOUND; }
+        for (k = 0; k < nd * nt; ++k) o[k] = 0.0;
+        if (frame == 0) {
+            int isim;
+            static const int pole_of[4] = {0, 0, 1, 2}, var_of[4] = {0, 2, 4, 7};
+            for (isim = 1; isim <= 4; ++isim) {
+                double a1 = azim_factor(phi, S->src_mij, isim, 1), a2 = azim_factor(phi, S->src_mij, isim, 2);
+                strain_point(C, db, e, pole_of[isim - 1], var_of[isim - 1], xi, eta, nodes, ut);
+                for (k = 0; k < nd * nt; ++k) ut[k] = ut[k] / db->amplitude;
+                if (!C->full) {
+                    for (k = 0; k < nt; ++k) {
+                        double v = ut[k];
+                        if (v != v) v = 0.0;
+                        o[k] = o[k] + v * a1;
+                    }
+                } else {
+                    for (c = 0; c < 6; ++c) {
+                        double a = (c == 3 || c == 5) ? a2 : a1;
+                        for (k = 0; k < nt; ++k) o[c * nt + k] = o[c * nt + k] + ut[c * nt + k] * a;
+                    }
+                }
+            }
+            if (C->full) {
+                rot_src_to_xyz(o, nt, phi);
+                rot_xyz_src_to_earth(o, nt, C->src_lon, C->src_colat);
+            }
+        } else {
+            double az[6];
+            int isim;
+            if (S->component == 'Z') {
+                isim = 1;
+                for (c = 0; c < 6; ++c) az[c] = 1.0;
+            } else {
+                double fi[3] = {0, 0, 0}, a1, a2;
+                fi[S->component == 'R' ? 1 : 2] = 1.0;
+                isim = 2;
+                a1 = azim_factor_bw(phi, fi, 2, 1);
+                a2 = azim_factor_bw(phi, fi, 2, 2);
+                az[0] = a1; az[1] = a1; az[2] = a1; az[3] = a2; az[4] = a1 * 2; az[5] = a2 * 2;
+            }
+            strain_point(C, db, e, isim == 1 ? 0 : 1, isim == 1 ? 0 : 2, xi, eta, nodes, ut);
+            for (c = 0; c < nd; ++c)
+                for (k = 0; k < nt; ++k) o[c * nt + k] = ut[c * nt + k] * az[c];
+            for (k = 0; k < nd * nt; ++k) o[k] = o[k] / db->amplitude;
+            if (C->full) {
+                rot_src_to_xyz(o, nt, phi);
+                rot_xyz_src_to_earth(o, nt, C->R[frame - 1].lon, C->R[frame - 1].colat);
+            }
+        }
+    }
+    free(ut);
+    return 0;
+}
+
+int ora_load_strain(ora_ctx *C, int frame, int npts, const double *xyz, double *traces)
+{
+    return load_points(C, frame, npts, xyz, traces);
+}
+
+/* rfft(taperandzeropad(field, N, 10)) of every trace (slave.f90:525, 592) */
+static void fields_to_fd(ora_ctx *C, int ntr, const double *td, double *fd, const double *ts)
+{
+    int N = C->N, nw = C->nomega, i, k;
+    double *pad = (double *)malloc(N * sizeof(double));
+    for (i = 0; i < ntr; ++i) {
+        double *X = fd + (size_t)i * 2 * nw;
+        ora_taperandzeropad(C->S.nt, N, 10, 0, td + (size_t)i * C->S.nt, pad);
+        ora_rfft(N, C->S.dt, pad, X);
+        if (ts)
+            for (k = 0; k < nw; ++k) {
+                cz a = {X[2 * k], X[2 * k + 1]}, b = {ts[2 * k], ts[2 * k + 1]};
+                a = zmul(a, b);
+                X[2 * k] = a.re; X[2 * k + 1] = a.im;
+            }
+    }
+    free(pad);
+}
+
+/* calc_basekernel (kernel.f90:514-570); F, B [nd][nomega][2] -> out [nomega][2] */
+static void basekernel(int ibk, int full, int nd, int nw, const double *F, const double *B, double *out)
+{
+    int k, c;
+#define FF(c) (cz){F[((c) * nw + k) * 2], F[((c) * nw + k) * 2 + 1]}
+#define BB(c) (cz){B[((c) * nw + k) * 2], B[((c) * nw + k) * 2 + 1]}
+    for (k = 0; k < nw; ++k) {
+        cz r = {0, 0};
+        if (ibk == 0) {
+            if (!full) {
+                for (c = 0; c < nd; ++c) { cz p = zmul(FF(c), BB(c)); r.re += p.re; r.im += p.im; }
+                r.re = -r.re; r.im = -r.im;
+            } else {
+                cz fs = FF(0), bs = BB(0);
+                fs.re = fs.re + FF(1).re; fs.im = fs.im + FF(1).im;
+                fs.re = fs.re + FF(2).re; fs.im = fs.im + FF(2).im;
+                bs.re = bs.re + BB(1).re; bs.im = bs.im + BB(1).im;
+                bs.re = bs.re + BB(2).re; bs.im = bs.im + BB(2).im;
+                fs.re = -fs.re; fs.im = -fs.im;
+                r = zmul(fs, bs);
+            }
+        } else if (ibk == 1) {
+            for (c = 0; c < nd; ++c) { cz p = zmul(FF(c), BB(c)); r.re += p.re; r.im += p.im; }
+            r.re = -r.re * 2.0; r.im = -r.im * 2.0;
+        } else if (ibk == 3) {
+            cz a = {FF(1).re + FF(0).re, FF(1).im + FF(0).im}, b = {BB(1).re + BB(0).re, BB(1).im + BB(0).im};
+            r = zmul(a, b);
+        } else if (ibk == 4) {
+            cz p = zmul(BB(4), FF(4)), q = zmul(BB(3), FF(3));
+            r.re = (p.re + q.re) * 4.0; r.im = (p.im + q.im) * 4.0;
+        } else if (ibk == 5) {
+            cz a = {FF(0).re + FF(1).re, FF(0).im + FF(1).im};
+            cz p = zmul(a, BB(2)), q = zmul(zmul(BB(0), BB(1)), FF(2));
+            r.re = p.re + q.re; r.im = p.im + q.im;
+        }
+        out[2 * k] = r.re; out[2 * k + 1] = r.im;
+    }
+#undef FF
+#undef BB
+}
+
+/* apply_filter_2d + irfft + calc_misfit_kernel (kernel.f90:410-510, 946-956) */
+static double misfit_value(ora_ctx *C, int k, const double *conv_fd, double *work)
+{
+    ora_kernel *K = &C->K[k];
+    int N = C->N, nw = C->nomega, i;
+    const double *tf = C->tf_fwd + (size_t)K->filter * 2 * nw, *tb = C->tf_bwd + (size_t)K->filter * 2 * nw;
+    double *X = work, *x = work + 2 * nw, *pad, *af, val;
+    for (i = 0; i < nw; ++i) {
+        cz a = {conv_fd[2 * i], conv_fd[2 * i + 1]}, f = {tf[2 * i], tf[2 * i + 1]}, b = {tb[2 * i], tb[2 * i + 1]};
+        a = zmul(a, f);
+        a = zmul(a, b);
+        X[2 * i] = a.re; X[2 * i + 1] = a.im;
+    }
+    ora_irfft(N, C->df, X, x);
+    if (C->S.int_scheme == MCK_INT_TRAPEZOIDAL) {
+        double s = 0.0, f1, fn;
+        f1 = x[K->ifirst] * K->seis_cut[0];
+        fn = x[K->ifirst + K->ncut - 1] * K->seis_cut[K->ncut - 1];
+        for (i = 1; i < K->ncut - 1; ++i) s += x[K->ifirst + i] * K->seis_cut[i];
+        val = (f1 + fn + 2 * s) * C->S.dt * 0.5;
+    } else {
+        pad = (double *)calloc(K->M, sizeof(double));
+        af = (double *)malloc(2 * (K->M / 2 + 1) * sizeof(double));
+        for (i = 0; i < K->ncut; ++i) pad[i] = x[K->ifirst + i];
+        ora_rfft(K->M, C->S.dt, pad, af);
+        val = parseval(K->M, C->S.dt, af, K->seis_cut_fd);
+        free(pad); free(af);
+    }
+    return val * K->normalization * (-1.0);
+}
+
+/* model coefficients at radius r (interpolate_mesh.f90:98-109, background_model.f90:122-198) */
+typedef struct { float vp, vs, rho, phi, xi, eta, lam, mu, vsh, vsv, vph, vpv; } ora_model;
+static ora_model model_at(const mck_setup_t *S, double r)
+{
+    ora_model m;
+    int idx = (int)(r / (double)S->model_dr + 1);
+    if (idx < 1) idx = 1;
+    if (idx > S->model_ndepth) idx = S->model_ndepth;
+    idx -= 1;
+    m.vp = S->model_vp[idx]; m.vs = S->model_vs[idx]; m.rho = S->model_rho[idx];
+    m.phi = S->model_phi[idx]; m.xi = S->model_xi[idx]; m.eta = S->model_eta[idx];
+    m.lam = m.rho * (m.vp * m.vp - 2 * (m.vs * m.vs));
+    m.mu = m.rho * (m.vs * m.vs);
+    m.vsh = m.vs; m.vph = m.vp;
+    m.vsv = m.vsh / sqrtf(m.xi);
+    m.vpv = m.vph * sqrtf(m.phi);
+    return m;
+}
+
+/* calc_physical_kernels_scalar (kernel.f90:665-832); b = base kernels [6] */
+static double physical(int param, const double *b, const ora_model *m, int rel)
+{
+    double rho = m->rho;
+    switch (param) {
+    case MCK_PARAM_LAM: return rel ? b[0] * (double)m->lam : b[0];
+    case MCK_PARAM_MU:  return rel ? b[1] * (double)m->mu : b[1];
+    case MCK_PARAM_VP:  return rel ? 2.0 * rho * (double)(m->vp * m->vp) * b[0] : 2.0 * rho * (double)m->vp * b[0];
+    case MCK_PARAM_VS:  return (rel ? 2.0 * rho * (double)(m->vs * m->vs) : 2.0 * rho * (double)m->vs) * (b[1] - 2 * b[0]);
+    case MCK_PARAM_VSH: return (rel ? 2.0 * rho * (double)(m->vsh * m->vsh) : 2.0 * rho * (double)m->vsh)
+                               * (b[1] - 2 * b[0] - b[4] + 2 * b[5] * (double)(1 - m->eta));
+    case MCK_PARAM_VSV: return (rel ? 2.0 * rho * (double)(m->vsv * m->vsv) : 2.0 * rho * (double)m->vsv) * b[4];
+    case MCK_PARAM_VPH: return (rel ? 2.0 * rho * (double)(m->vph * m->vph) : 2.0 * rho * (double)m->vph)
+                               * (b[3] + b[5] * (double)m->eta);
+    case MCK_PARAM_VPV: return (rel ? 2.0 * rho * (double)(m->vpv * m->vpv) : 2.0 * rho * (double)m->vpv)
+                               * (b[0] - b[3] - b[5]);
+    case MCK_PARAM_ETA: return rel ? 2.0 * rho * (double)(m->vph * m->vph - m->vsh * m->vsh) * b[5] * (double)m->eta
+                                   : 2.0 * rho * (double)(m->vph * m->vph - m->vsh * m->vsh) * b[5];
+    default: return 0.0;
+    }
+}
+
+/* forward field of a batch of points: load_fw_points, cumsum_trapezoidal, rfft of the
+ * tapered field and the optional time shift (slave.f90:515-534); F[npts][nd][nomega][2] */
+static int batch_fwd(ora_ctx *C, int npts, const double *pts, double *F)
+{
+    const mck_setup_t *S = &C->S;
+    int nt = S->nt, nd = C->nd, p, rc;
+    double *fw = (double *)malloc((size_t)npts * nd * nt * sizeof(double));
+    double *tmp = (double *)malloc((size_t)nt * sizeof(double));
+    rc = load_points(C, 0, npts, pts, fw);
+    if (!rc) {
+        for (p = 0; p < npts * nd; ++p) {
+            ora_cumsum_trapezoidal(nt, S->dt, fw + (size_t)p * nt, tmp);
+            memcpy(fw + (size_t)p * nt, tmp, nt * sizeof(double));
+        }
+        fields_to_fd(C, npts * nd, fw, F, S->deconv_stf ? NULL : C->ts_fwd);
+    }
+    free(fw); free(tmp);
+    return rc;
+}
+
+/* one receiver of slave_work's receiver loop (slave.f90:586-640): load_bw_points, rfft,
+ * calc_basekernel per needed base kernel, apply_filter, irfft, calc_misfit_kernel for
+ * every kernel of the receiver that is not converged. */
+static int receiver_base(ora_ctx *C, int r, int npts, const double *pts, const double *F,
+                         const int *conv_flags, double *base, int64_t *evals, int32_t *niter)
+{
+    const mck_setup_t *S = &C->S;
+    ora_rec *R = &C->R[r];
+    int nt = S->nt, nd = C->nd, nw = C->nomega, p, b, k, rc;
+    double *bw = (double *)malloc((size_t)npts * nd * nt * sizeof(double));
+    double *B = (double *)malloc((size_t)npts * nd * nw * 2 * sizeof(double));
+    double *conv = (double *)malloc((size_t)nw * 2 * sizeof(double));
+    double *work = (double *)malloc(((size_t)2 * nw + C->N) * sizeof(double));
+    rc = load_points(C, 1 + r, npts, pts, bw);
+    if (rc) goto done;
+    fields_to_fd(C, npts * nd, bw, B, S->deconv_stf ? NULL : C->ts_bwd);
+    for (b = 0; b < 6; ++b) {
+        if (!R->needs_bk[b]) continue;
+        for (k = R->first; k <= R->last; ++k) {
+            if (!C->K[k].needs_bk[b]) continue;
+            if (conv_flags && conv_flags[k]) continue;
+            for (p = 0; p < npts; ++p) {
+                basekernel(b, C->full, nd, nw, F + (size_t)p * nd * nw * 2, B + (size_t)p * nd * nw * 2, conv);
+                base[((size_t)p * S->nkernel + k) * 6 + b] = misfit_value(C, k, conv, work);
+            }
+            if (niter) niter[k] += 1;
+            if (evals) {
+                int bb, first = 1;
+                for (bb = 0; bb < b; ++bb) if (C->K[k].needs_bk[bb]) first = 0;
+                if (first) *evals += npts;
+            }
+        }
+    }
+done:
+    free(bw); free(B); free(conv); free(work);
+    return rc;
+}
+
+int ora_base_kernels(ora_ctx *C, int npts, const double *xyz, double *base)
+{
+    int r, rc;
+    double *F = (double *)malloc((size_t)npts * C->nd * C->nomega * 2 * sizeof(double));
+    memset(base, 0, (size_t)npts * C->S.nkernel * 6 * sizeof(double));
+    rc = batch_fwd(C, npts, xyz, F);
+    for (r = 0; r < C->S.nrec && !rc; ++r) rc = receiver_base(C, r, npts, xyz, F, NULL, base, NULL, NULL);
+    free(F);
+    return rc;
+}
+
+/* dampen_field (readfields.f90:3184-3204) on base[npts][nkernel][6] */
+static void dampen(double *base, int npts, int nk, const double *pts, const double *c, double rmax)
+{
+    int p, i;
+    if (!(rmax > 0.0)) return;
+    for (p = 0; p < npts; ++p) {
+        double d[3] = {pts[3 * p] - c[0], pts[3 * p + 1] - c[1], pts[3 * p + 2] - c[2]};
+        double dist = ora_norm2(3, d);
+        if (dist < rmax)
+            for (i = 0; i < nk * 6; ++i) base[(size_t)p * nk * 6 + i] = base[(size_t)p * nk * 6 + i] * dist / rmax;
+    }
+}
+
+/* integrated_type (mc_integration.f90:69-187) */
+typedef struct {
+    int n;
+    double volume;
+    int64_t nmodels;
+    double *fsum, *f2sum, *integral, *variance, allowed_var, allowed_rel;
+    int *conv;
+} ora_mc;
+
+static void mc_init(ora_mc *m, int n, double volume, double aerr, double arel)
+{
+    int i;
+    m->n = n; m->volume = volume; m->nmodels = 0;
+    m->fsum = (double *)calloc(4 * (size_t)n, sizeof(double));
+    m->f2sum = m->fsum + n; m->integral = m->f2sum + n; m->variance = m->integral + n;
+    m->conv = (int *)calloc(n, sizeof(int));
+    m->allowed_var = aerr * aerr;
+    m->allowed_rel = arel;
+    for (i = 0; i < n; ++i) m->variance[i] = 1e100;
+}
+
+static int mc_check(ora_mc *m, int npts, const double *vals /* [npts][n] */)
+{
+    int i, p;
+    m->nmodels += npts;
+    for (i = 0; i < m->n; ++i) {
+        double s = 0.0, s2 = 0.0;
+        if (m->conv[i]) continue;
+        for (p = 0; p < npts; ++p) s += vals[(size_t)p * m->n + i];
+        for (p = 0; p < npts; ++p) s2 += vals[(size_t)p * m->n + i] * vals[(size_t)p * m->n + i];
+        m->fsum[i] = m->fsum[i] + s;
+        m->f2sum[i] = m->f2sum[i] + s2;
+        m->integral[i] = m->fsum[i] * m->volume / (double)m->nmodels;
+        if (m->nmodels > 2) {
+            m->variance[i] = m->volume * m->volume / (double)(m->nmodels - 1)
+                           * ((m->f2sum[i] / (double)m->nmodels)
+                              - (m->fsum[i] / (double)m->nmodels) * (m->fsum[i] / (double)m->nmodels));
+            if (m->variance[i] < m->allowed_var) m->conv[i] = 1;
+            else if (sqrt(m->variance[i]) < m->allowed_rel * fabs(m->integral[i])) m->conv[i] = 1;
+        } else {
+            m->variance[i] = m->integral[i] * 1e10;
+            m->conv[i] = 0;
+        }
+        if (m->integral[i] != m->integral[i] || m->variance[i] != m->variance[i]) return MCK_ENAN;
+    }
+    return 0;
+}
+
+static void mc_free(ora_mc *m) { free(m->fsum); free(m->conv); }
+
+/* travel_time_larger_Tmax (slave.f90:928-950) */
+static int travel_time_masked(ora_ctx *C, int r, const double *center)
+{
+    double d1[3], d2[3], dist, tp, ts;
+    int i;
+    for (i = 0; i < 3; ++i) { d1[i] = C->src_r[i] - center[i]; d2[i] = C->R[r].r[i] - center[i]; }
+    dist = ora_norm2(3, d1) + ora_norm2(3, d2);
+    tp = dist / (double)C->S.vp_max;
+    ts = dist / (double)C->S.vs_max;
+    return (tp > C->R[r].t_last_p && ts > C->R[r].t_last_s);
+}
+
+/* slave_work (slave.f90:273-810) for tetrahedral, volumetric elements */
+int ora_slave_work(ora_ctx *C, int nelems, const double *vertices, const int32_t *conn,
+                   double *values, double *variance, int32_t *niterations,
+                   int64_t *halton_step, const int64_t *halton_offsets,
+                   int64_t *offsets_used, int64_t *evaluations)
+{
+    const mck_setup_t *S = &C->S;
+    int nk = S->nkernel, P = S->npoints_per_step, e, r, k, i, rc = 0;
+    double *pts = (double *)malloc((size_t)P * 3 * sizeof(double));
+    double *hal = (double *)malloc((size_t)P * 3 * sizeof(double));
+    double *base = (double *)calloc((size_t)P * nk * 6, sizeof(double));
+    double *phys = (double *)malloc((size_t)P * nk * sizeof(double));
+    double *F = (double *)malloc((size_t)P * C->nd * C->nomega * 2 * sizeof(double));
+    int32_t *niter = (int32_t *)malloc(nk * sizeof(int32_t));
+    static const int64_t seed[3] = {20, 20, 20}, leap[3] = {2, 2, 2};
+    static const int32_t bases[3] = {2, 3, 5};
+    if (evaluations) *evaluations = 0;
+    C->halton_step = *halton_step;
+    for (e = 0; e < nelems; ++e) {
+        double v[12], center[3], volume;
+        ora_mc mc;
+        for (i = 0; i < 4; ++i) {
+            int vid = conn[e * 4 + i] - 1;
+            v[3 * i] = vertices[3 * vid]; v[3 * i + 1] = vertices[3 * vid + 1]; v[3 * i + 2] = vertices[3 * vid + 2];
+        }
+        for (i = 0; i < 3; ++i) center[i] = (v[i] + v[3 + i] + v[6 + i] + v[9 + i]) / 4.0;
+        volume = S->int_over_volume ? ora_tet_volume(v) : 1.0;
+        if (halton_offsets) C->halton_step = halton_offsets[e];
+        if (offsets_used) offsets_used[e] = C->halton_step;
+        mc_init(&mc, nk, volume, S->allowed_error * volume, S->allowed_relative_error);
+        for (k = 0; k < nk; ++k) niter[k] = 0;
+        for (;;) {
+            int all = 1, stop = 0;
+            for (k = 0; k < nk; ++k) if (!mc.conv[k]) all = 0;
+            if (all) break;
+            ora_halton(3, seed, bases, leap, C->halton_step, P, hal);
+            C->halton_step += P;
+            ora_tet_points(v, P, hal, pts);
+            for (k = 0; k < nk; ++k) if (niter[k] == S->max_iter) stop = 1;
+            if (stop) break;
+            rc = batch_fwd(C, P, pts, F);
+            if (rc) { mc_free(&mc); goto out; }
+            for (r = 0; r < S->nrec; ++r) {
+                ora_rec *R = &C->R[r];
+                int allconv = 1;
+                if (travel_time_masked(C, r, center)) {
+                    for (k = R->first; k <= R->last; ++k) {
+                        int b;
+                        for (i = 0; i < P; ++i) for (b = 0; b < 6; ++b) base[((size_t)i * nk + k) * 6 + b] = 0.0;
+                        niter[k] += 1;
+                    }
+                    continue;
+                }
+                for (k = R->first; k <= R->last; ++k) if (!mc.conv[k]) allconv = 0;
+                if (allconv) continue;
+                rc = receiver_base(C, r, P, pts, F, mc.conv, base, evaluations, niter);
+                if (rc) { mc_free(&mc); goto out; }
+                dampen(base, P, nk, pts, C->R[r].r, S->damp_radius);
+                dampen(base, P, nk, pts, C->src_r, S->damp_radius);
+            }
+            for (i = 0; i < P; ++i) {
+                double rr = ora_norm2(3, pts + 3 * i);
+                ora_model m = model_at(S, rr);
+                for (k = 0; k < nk; ++k)
+                    phys[(size_t)i * nk + k] = physical(S->kernel_param[k], base + ((size_t)i * nk + k) * 6, &m, S->relative_kernel);
+            }
+            rc = mc_check(&mc, P, phys);
+            if (rc) { mc_free(&mc); goto out; }
+        }
+        for (k = 0; k < nk; ++k) {
+            values[(size_t)e * nk + k] = mc.integral[k];
+            variance[(size_t)e * nk + k] = mc.variance[k];
+            niterations[(size_t)e * nk + k] = niter[k];
+        }
+        mc_free(&mc);
+    }
+out:
+    *halton_step = C->halton_step;
+    free(pts); free(hal); free(base); free(phys); free(F); free(niter);
+    return rc;
+}

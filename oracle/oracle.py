@@ -1,0 +1,253 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, never by the mc_kernel_amd package.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from mc_kernel_amd import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built")
+        L = C.CDLL(path)
+        f64p, f32p, i32p, i64p = abi.f64p, abi.f32p, abi.i32p, abi.i64p
+        L.ora_gll_points.argtypes = [C.c_int, f64p]
+        L.ora_glj_points.argtypes = [C.c_int, f64p]
+        L.ora_lagrange_derivs.argtypes = [C.c_int, f64p, f64p]
+        L.ora_halton.argtypes = [C.c_int, i64p, i32p, i64p, C.c_int64, C.c_int, f64p]
+        L.ora_tet_points.argtypes = [f64p, C.c_int, f64p, f64p]
+        L.ora_tet_volume.argtypes = [f64p]
+        L.ora_tet_volume.restype = C.c_double
+        L.ora_norm2.argtypes = [C.c_int, f64p]
+        L.ora_norm2.restype = C.c_double
+        L.ora_mapping.argtypes = [C.c_double, C.c_double, f64p, C.c_int, f64p]
+        L.ora_inv_jacobian.argtypes = [C.c_double, C.c_double, f64p, C.c_int, f64p]
+        L.ora_inv_mapping.argtypes = [C.c_double, C.c_double, f64p, C.c_int, f64p]
+        L.ora_inside_element.argtypes = [C.c_double, C.c_double, f64p, C.c_int, C.c_double,
+                                         f64p, f64p]
+        L.ora_strain_pole.argtypes = [C.c_int, f64p, C.c_int, C.c_int, f64p, C.c_int, C.c_int, f64p]
+        L.ora_straintrace_pole.argtypes = [C.c_int, f64p, C.c_int, C.c_int, f64p, C.c_int, C.c_int, f64p]
+        L.ora_rotate_frame.argtypes = [C.c_int, f64p, C.c_double, C.c_double, f64p, f64p, f64p]
+        L.ora_nextpow2.argtypes = [C.c_int]
+        L.ora_rfft.argtypes = [C.c_int, C.c_double, f64p, f64p]
+        L.ora_irfft.argtypes = [C.c_int, C.c_double, f64p, f64p]
+        L.ora_taperandzeropad.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, f64p, f64p]
+        L.ora_cumsum_trapezoidal.argtypes = [C.c_int, C.c_double, f64p, f64p]
+        L.ora_filter_tf.argtypes = [C.POINTER(abi.MckFilter), C.c_int, C.c_double, f64p]
+        L.ora_nearest.argtypes = [C.c_int, f32p, f32p, C.c_double, C.c_double, C.c_int, i32p, f32p]
+        L.ora_create.argtypes = [C.POINTER(abi.MckSetup), C.c_char_p, C.c_int]
+        L.ora_create.restype = C.c_void_p
+        L.ora_destroy.argtypes = [C.c_void_p]
+        L.ora_locate.argtypes = [C.c_void_p, C.c_int, C.c_int, f64p, i32p, f64p, f64p, i32p]
+        L.ora_load_strain.argtypes = [C.c_void_p, C.c_int, C.c_int, f64p, f64p]
+        L.ora_base_kernels.argtypes = [C.c_void_p, C.c_int, f64p, f64p]
+        L.ora_slave_work.argtypes = [C.c_void_p, C.c_int, f64p, i32p, f64p, f64p, i32p,
+                                     i64p, i64p, i64p, i64p]
+        L.ora_kernel_info.argtypes = [C.c_void_p, C.c_int, f64p, C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int)]
+        L.ora_strain_nd.argtypes = [C.c_void_p]
+        _LIB = L
+    return _LIB
+
+
+def P(a, t=C.c_double):
+    return abi.ptr(a, t)
+
+
+def gll_points(n):
+    x = np.zeros(n + 1)
+    lib().ora_gll_points(n, P(x))
+    return x
+
+
+def glj_points(n):
+    x = np.zeros(n + 1)
+    lib().ora_glj_points(n, P(x))
+    return x
+
+
+def lagrange_derivs(n, x):
+    G = np.zeros((n + 1) * (n + 1))
+    lib().ora_lagrange_derivs(n, P(np.ascontiguousarray(x, float)), P(G))
+    return G.reshape(n + 1, n + 1).T   # G[i, j] = l_i'(x_j)
+
+
+def halton(npoints, step=0, seed=(20, 20, 20), base=(2, 3, 5), leap=(2, 2, 2)):
+    nd = len(base)
+    r = np.zeros((npoints, nd))
+    s = np.array(seed, np.int64)
+    b = np.array(base, np.int32)
+    l = np.array(leap, np.int64)
+    lib().ora_halton(nd, P(s, C.c_int64), P(b, C.c_int32), P(l, C.c_int64), step, npoints, P(r))
+    return r
+
+
+def tet_points(v, coords):
+    v = np.ascontiguousarray(v, float)
+    coords = np.ascontiguousarray(coords, float)
+    out = np.zeros_like(coords)
+    lib().ora_tet_points(P(v), coords.shape[0], P(coords), P(out))
+    return out
+
+
+def tet_volume(v):
+    return lib().ora_tet_volume(P(np.ascontiguousarray(v, float)))
+
+
+def rfft(x, dt=1.0):
+    x = np.ascontiguousarray(x, float)
+    X = np.zeros((len(x) // 2 + 1) * 2)
+    lib().ora_rfft(len(x), dt, P(x), P(X))
+    return X[0::2] + 1j * X[1::2]
+
+
+def irfft(X, n, df):
+    Xa = np.zeros(2 * (n // 2 + 1))
+    Xa[0::2] = X.real
+    Xa[1::2] = X.imag
+    x = np.zeros(n)
+    lib().ora_irfft(n, df, P(Xa), P(x))
+    return x
+
+
+def taperandzeropad(x, nout, ntaper, end_only=False):
+    x = np.ascontiguousarray(x, float)
+    out = np.zeros(nout)
+    lib().ora_taperandzeropad(len(x), nout, ntaper, int(end_only), P(x), P(out))
+    return out
+
+
+def cumsum_trapezoidal(x, dt):
+    x = np.ascontiguousarray(x, float)
+    out = np.zeros_like(x)
+    lib().ora_cumsum_trapezoidal(len(x), dt, P(x), P(out))
+    return out
+
+
+def filter_tf(filterclass, freqs, nfreq, df):
+    f = abi.MckFilter()
+    f.filterclass = abi.FILTERS[filterclass]
+    for i in range(4):
+        f.frequencies[i] = freqs[i]
+    tf = np.zeros(2 * nfreq)
+    lib().ora_filter_tf(C.byref(f), nfreq, df, P(tf))
+    return tf[0::2] + 1j * tf[1::2]
+
+
+def mapping(xi, eta, nodes, eltype):
+    sz = np.zeros(2)
+    lib().ora_mapping(xi, eta, P(np.asfortranarray(nodes, float).ravel(order="F")), eltype, P(sz))
+    return sz
+
+
+def inv_mapping(s, z, nodes, eltype):
+    out = np.zeros(2)
+    lib().ora_inv_mapping(s, z, P(np.asfortranarray(nodes, float).ravel(order="F")), eltype, P(out))
+    return out
+
+
+def strain_pole(pole, u, nodes, eltype, axial, trace=False):
+    """u [nvar][jpol][ipol][nsamp] -> strain [6][jpol][ipol][nsamp] (or trace)."""
+    u = np.ascontiguousarray(u, float)
+    ns = u.shape[-1]
+    nd = np.asfortranarray(nodes, float).ravel(order="F")
+    if trace:
+        out = np.zeros((5, 5, ns))
+        lib().ora_straintrace_pole(pole, P(u), ns, 4, P(nd), eltype, int(axial), P(out))
+    else:
+        out = np.zeros((6, 5, 5, ns))
+        lib().ora_strain_pole(pole, P(u), ns, 4, P(nd), eltype, int(axial), P(out))
+    return out
+
+
+def nearest(mp_s, mp_z, s, z, nn=12):
+    idx = np.zeros(nn, np.int32)
+    dist = np.zeros(nn, np.float32)
+    tie = lib().ora_nearest(len(mp_s), P(mp_s, C.c_float), P(mp_z, C.c_float), s, z, nn,
+                            P(idx, C.c_int32), P(dist, C.c_float))
+    return idx, dist, tie
+
+
+class Oracle:
+    """Whole-path oracle for one MckSetup (the Problem must stay alive)."""
+
+    def __init__(self, setup):
+        err = C.create_string_buffer(512)
+        self.h = lib().ora_create(C.byref(setup), err, 512)
+        if not self.h:
+            raise RuntimeError("oracle setup failed: " + err.value.decode())
+        self.nk = setup.nkernel
+        self.nt = setup.nt
+        self.nd = lib().ora_strain_nd(self.h)
+        self.halton_step = 0
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ora_destroy(self.h)
+            self.h = None
+
+    def locate(self, frame, xyz):
+        xyz = np.ascontiguousarray(xyz, float)
+        n = xyz.shape[0]
+        elem = np.zeros(n, np.int32)
+        xi = np.zeros(n)
+        eta = np.zeros(n)
+        ties = np.zeros(n, np.int32)
+        rc = lib().ora_locate(self.h, frame, n, P(xyz), P(elem, C.c_int32), P(xi), P(eta),
+                              P(ties, C.c_int32))
+        if rc:
+            raise RuntimeError("oracle locate failed (%d)" % rc)
+        return elem, xi, eta, ties
+
+    def load_strain(self, frame, xyz):
+        xyz = np.ascontiguousarray(xyz, float)
+        out = np.zeros((xyz.shape[0], self.nd, self.nt))
+        rc = lib().ora_load_strain(self.h, frame, xyz.shape[0], P(xyz), P(out))
+        if rc:
+            raise RuntimeError("oracle load_strain failed (%d)" % rc)
+        return out
+
+    def base_kernels(self, xyz):
+        xyz = np.ascontiguousarray(xyz, float)
+        out = np.zeros((xyz.shape[0], self.nk, 6))
+        rc = lib().ora_base_kernels(self.h, xyz.shape[0], P(xyz), P(out))
+        if rc:
+            raise RuntimeError("oracle base_kernels failed (%d)" % rc)
+        return out
+
+    def kernel_info(self, k):
+        norm = C.c_double()
+        ncut, ifirst = C.c_int(), C.c_int()
+        lib().ora_kernel_info(self.h, k, C.byref(norm), C.byref(ncut), C.byref(ifirst))
+        return norm.value, ncut.value, ifirst.value
+
+    def slave_work(self, verts, conn, halton_offsets=None):
+        verts = np.ascontiguousarray(verts, float)
+        conn = np.ascontiguousarray(conn, np.int32)
+        ne = conn.shape[0]
+        vals = np.zeros((ne, self.nk))
+        var = np.zeros((ne, self.nk))
+        nit = np.zeros((ne, self.nk), np.int32)
+        step = np.array([self.halton_step], np.int64)
+        used = np.zeros(ne, np.int64)
+        ev = np.zeros(1, np.int64)
+        offs = None if halton_offsets is None else np.ascontiguousarray(halton_offsets, np.int64)
+        rc = lib().ora_slave_work(self.h, ne, P(verts), P(conn, C.c_int32), P(vals), P(var),
+                                  P(nit, C.c_int32), P(step, C.c_int64), P(offs, C.c_int64),
+                                  P(used, C.c_int64), P(ev, C.c_int64))
+        if rc:
+            raise RuntimeError("oracle slave_work failed (%d)" % rc)
+        self.halton_step = int(step[0])
+        return dict(values=vals, variance=var, niterations=nit, offsets=used,
+                    evaluations=int(ev[0]), halton_step=int(step[0]))
