@@ -1,0 +1,222 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X MC Kernel path: MC point-kernel evaluations per second.
+
+Workload c1 (one GPU): merged fwd/bwd databases of a 20,000-quad axisymmetric mesh with
+nt = 2048 (nfft 4096), 10 receivers at 10..100 deg, Gabor filters 20/30/40/60 s, CC and AM
+misfits for vp and vs (160 kernels), 50,000 tetrahedral voxels with Halton points,
+16 points per MC step, relative MC tolerance 1e-3.  The databases are generated directly
+in HBM.  A step is one task of --task voxels evaluated by mck_slave_work (the reference's
+work unit, slave.f90:273-810); the value is whole-job evaluations per second over the
+timed steps.  One evaluation is one (voxel, MC point, kernel) value actually computed
+(converged and travel-time-masked kernels excluded).
+
+Also reported: the roofline of the dominant kernel (algorithmic bytes per launch over its
+hipEvent time), the FP64 CPU oracle timed on a bounded sample on the host cores, and the
+plain one-point-at-a-time GPU form on a small sample.
+"""
+import argparse
+import ctypes as C
+import json
+import multiprocessing as mp
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MC point-kernel evals/sec at <=1e-5 rel err vs FP64; % of HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def workload(name):
+    from mc_kernel_amd import synth
+    return synth.make_problem(name, fill=False)
+
+
+def unmasked_voxels(p):
+    """Voxels with at least one receiver that is not travel-time masked."""
+    from mc_kernel_amd import synth
+    v = p.verts.reshape(p.ntet, 4, 3)
+    c = v.mean(axis=1)
+    src = synth.geo_xyz(p.src_lat, p.src_lon, synth.R_EARTH - p.src_depth * 1e3)
+    vp = float(p.model[0].max()); vs = float(p.model[1].max())
+    keep = np.zeros(p.ntet, bool)
+    for r in range(p.nrec):
+        rec = synth.geo_xyz(p.rec_lat[r], p.rec_lon[r], synth.R_EARTH)
+        d = np.linalg.norm(c - src, axis=1) + np.linalg.norm(c - rec, axis=1)
+        ks = p.kernel_receiver == r
+        params = p.kernel_param[ks]
+        tw = p.kernel_window[ks][:, 1]
+        tlp = tw[np.isin(params, [0, 1, 6, 7])].max(initial=0.0)
+        tls = tw[~np.isin(params, [0, 1, 6, 7])].max(initial=0.0)
+        keep |= ~((d / vp > tlp) & (d / vs > tls))
+    return np.nonzero(keep)[0]
+
+
+def _cpu_worker(args):
+    """One host core: the FP64 oracle on a bounded sample of voxels (lazy database)."""
+    name, voxels, max_iter = args
+    sys.path.insert(0, ROOT)
+    from mc_kernel_amd import synth
+    from mc_kernel_amd import abi
+    from oracle import oracle
+    p = synth.make_problem(name, fill=False, max_iter=max_iter)
+    m = p.mesh
+    fwd = np.empty((m.nelem, 10, 5, 5, p.nt), np.float32)
+    bwd = np.empty((m.nelem, 5, 5, 5, p.nt), np.float32)
+    p.setup.fwd.snapshots = abi.ptr(fwd, C.c_float)
+    p.setup.bwd.snapshots = abi.ptr(bwd, C.c_float)
+    lib = synth.synth_lib()
+    lib.synth_lazy_create.argtypes = [abi.i32p, C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                                      C.c_uint64, C.c_uint64]
+    lib.synth_lazy_create.restype = C.c_void_p
+    user = lib.synth_lazy_create(abi.ptr(m.sem, C.c_int32), 10, 5, p.nt, p.dt, 1001, 1002)
+    o = oracle.Oracle(p.setup)
+    o.set_lazy_fill(C.cast(lib.synth_fill_element, C.c_void_p), user)
+    conn = p.conn[voxels]
+    stride = (p.max_iter + 1) * p.npoints_per_step
+    offs = np.asarray(voxels, np.int64) * stride
+    o.slave_work(p.verts, conn, halton_offsets=offs)          # generates the touched elements
+    t0 = time.perf_counter()
+    r = o.slave_work(p.verts, conn, halton_offsets=offs)      # timed
+    return r["evaluations"], time.perf_counter() - t0
+
+
+def cpu_baseline(name, workers, per_worker, max_iter):
+    p = workload(name)
+    cand = unmasked_voxels(p)
+    need = workers * per_worker
+    sel = cand[np.linspace(0, len(cand) - 1, need).astype(int)] if len(cand) >= need else cand
+    chunks = [sel[i::workers] for i in range(workers)]
+    chunks = [c for c in chunks if len(c)]
+    ctx = mp.get_context("spawn")
+    t0 = time.perf_counter()
+    with ctx.Pool(len(chunks)) as pool:
+        res = pool.map(_cpu_worker, [(name, c, max_iter) for c in chunks])
+    wall = time.perf_counter() - t0
+    evals = sum(r[0] for r in res)
+    tmax = max(r[1] for r in res)
+    return {"value": evals / tmax, "unit": "evals/s", "cores": len(chunks), "kind": "port",
+            "sample": "%d unmasked voxels of %s, %d MC steps each (max_iter=%d), %d points/step; "
+                      "FP64 oracle, one process per core, timed region = slave_work only"
+                      % (len(sel), name, max_iter, max_iter, p.npoints_per_step),
+            "seconds": tmax, "evaluations": int(evals), "total_wall_s": wall}
+
+
+def gpu_db(p):
+    from mc_kernel_amd import abi
+    lib = C.CDLL(os.path.join(ROOT, "mc_kernel_amd", "libmck_synth_gpu.so"))
+    lib.synth_gpu_db.argtypes = [C.c_int, abi.i32p, C.c_int, C.c_int, C.c_double, C.c_uint64,
+                                 C.POINTER(C.c_void_p)]
+    m = p.mesh
+    out = []
+    for nvar, seed in ((10, 1001), (5, 1002)):
+        d = C.c_void_p()
+        rc = lib.synth_gpu_db(m.nelem, abi.ptr(m.sem, C.c_int32), nvar, p.nt, p.dt, seed, C.byref(d))
+        if rc:
+            raise RuntimeError("device database generation failed (%d)" % rc)
+        out.append(d)
+    p.setup.fwd.snapshots = C.cast(out[0], C.POINTER(C.c_float))
+    p.setup.bwd.snapshots = C.cast(out[1], C.POINTER(C.c_float))
+    return lib, out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c1")
+    ap.add_argument("--task", type=int, default=2048, help="voxels per step (task)")
+    ap.add_argument("--cpu-workers", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-voxels-per-worker", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-voxels", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
+              "(voxel-parallel multi-GPU sharding is described in DESIGN.md)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    # CPU baseline first, in fresh processes, before this process touches the GPU
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args.config, args.cpu_workers, args.cpu_voxels_per_worker, args.cpu_steps)
+        print("cpu baseline:", json.dumps(cpu), file=sys.stderr, flush=True)
+
+    import mc_kernel_amd as mk
+    p = workload(args.config)
+    lib, dbs = gpu_db(p)
+    k = mk.MCKernel(p.setup)
+    T = args.task
+    ntask = (p.ntet + T - 1) // T
+    steps = []
+    for i in range(args.warmup + args.steps):
+        t = i % ntask
+        conn = p.conn[t * T:(t + 1) * T]
+        t0 = time.perf_counter()
+        r = k.slave_work(p.verts, conn)
+        dt = time.perf_counter() - t0
+        if i >= args.warmup:
+            steps.append((dt, r["stats"]))
+        print("step %d: %.3f s, %d evals, %d MC steps" % (i, dt, r["stats"]["evaluations"],
+                                                          r["stats"]["mc_steps"]),
+              file=sys.stderr, flush=True)
+    tot_t = sum(s[0] for s in steps)
+    tot_e = sum(s[1]["evaluations"] for s in steps)
+    fwd_b = sum(s[1]["fwd_bytes"] for s in steps)
+    bwd_b = sum(s[1]["bwd_bytes"] for s in steps)
+    fwd_ms = sum(s[1]["strain_fwd_ms"] for s in steps)
+    bwd_ms = sum(s[1]["strain_bwd_ms"] for s in steps)
+    launches = sum(s[1]["strain_launches"] for s in steps) // 2
+    dev_ms = sum(s[1]["device_ms"] for s in steps)
+    if bwd_ms >= fwd_ms:
+        kern, byt, ms = "k_strain_fft<8> (bwd frames)", bwd_b, bwd_ms
+    else:
+        kern, byt, ms = "k_strain_fft<8> (fwd frame)", fwd_b, fwd_ms
+    achieved = (byt / launches) / (ms / launches * 1e-3) / 1e9 if ms > 0 else 0.0
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kern,
+            "bytes_per_launch": byt / max(launches, 1), "avg_launch_ms": ms / max(launches, 1)}
+
+    # plain one-point-at-a-time form on a small sample of the same workload
+    plain = None
+    if args.plain_voxels > 0:
+        sel = unmasked_voxels(p)[: args.plain_voxels]
+        conn = p.conn[sel]
+        k.halton_step = 0
+        t0 = time.perf_counter()
+        rp = k.slave_work(p.verts, conn, plain=True)
+        tp = time.perf_counter() - t0
+        k.halton_step = 0
+        t0 = time.perf_counter()
+        rb = k.slave_work(p.verts, conn)
+        tb = time.perf_counter() - t0
+        plain = {"voxels": int(len(sel)), "plain_evals_per_s": rp["stats"]["evaluations"] / tp,
+                 "batched_same_sample_evals_per_s": rb["stats"]["evaluations"] / tb,
+                 "identical_results": bool(np.array_equal(rp["values"], rb["values"]))}
+        print("plain sample:", json.dumps(plain), file=sys.stderr, flush=True)
+    k.close()
+
+    value = tot_e / tot_t
+    line = {
+        "metric": METRIC, "value": value, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * tot_t / len(steps), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 accumulation)",
+        "data": "synthetic (seeded AxiSEM-style merged DB generated in HBM)",
+        "config": {"workload": args.config + ": 20000-quad DB, nt=2048, 10 receivers, "
+                   "4 Gabor, CC/AM x vp/vs = 160 kernels, 50000 tets, 16 pts/step, "
+                   "max_iter 64, rel tol 1e-3", "task_voxels": T, "parallelism": "none (1 GPU)",
+                   "evaluations": int(tot_e), "device_ms": dev_ms},
+        "roofline": roof, "cpu_baseline": cpu, "plain_form": plain,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

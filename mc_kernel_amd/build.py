@@ -39,7 +39,18 @@ def build_synth(force=True):
 
 
 def hip_sources():
-    return [os.path.join(CSRC, f) for f in sorted(os.listdir(CSRC)) if f.endswith(".hip")]
+    return [os.path.join(CSRC, f) for f in sorted(os.listdir(CSRC))
+            if f.endswith(".hip") and not f.startswith("synth")]
+
+
+def build_synth_gpu(force=True):
+    """Device-side generator of synthetic databases (benchmark input, not the product)."""
+    src = os.path.join(CSRC, "synth_gpu.hip")
+    out = os.path.join(HERE, "libmck_synth_gpu.so")
+    if force or _stale(out, [src]):
+        hipcc = os.path.join(ROCM, "bin", "hipcc")
+        _run([hipcc] + HIP_FLAGS + ["-shared", src, "-o", out])
+    return out
 
 
 def build_hip(force=True):
@@ -94,6 +105,7 @@ def build_all(force=True):
     except subprocess.CalledProcessError as e:  # the reference build is optional
         print("oracle/_ref build failed:", e, file=sys.stderr)
     build_hip(force)
+    build_synth_gpu(force)
 
 
 if __name__ == "__main__":

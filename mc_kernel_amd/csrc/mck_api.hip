@@ -785,6 +785,8 @@ int init_impl(const mck_setup_t *s) {
     if ((rc = build_kernels(G))) return rc;
     HIPCHECK(G.evals.ensure(1));
     HIPCHECK(G.err.ensure(1));
+    HIPCHECK(G.elem_count.ensure(4));
+    HIPCHECK(G.elem_mark.ensure(((size_t)std::max(G.fwd.d.nelem, G.bwd.d.nelem) * G.R + 31) / 32 + 1));
     HIPCHECK(G.nwork.ensure(1));
     // points per chunk: spectra of (1 + R) frames within a memory budget
     size_t free_b = 0, total_b = 0;
@@ -882,6 +884,19 @@ int prepare_task(State &G, Task &T, int elem_type, int nvpe, int nelems, int nba
     HIPCHECK(hipMemsetAsync(G.steps.p, 0, ne * 4, st));
     HIPCHECK(hipMemsetAsync(G.evals.p, 0, 8, st));
     HIPCHECK(hipMemsetAsync(G.err.p, 0, 4, st));
+    HIPCHECK(hipMemsetAsync(G.elem_count.p, 0, 4 * 8, st));
+    return MCK_OK;
+}
+
+// algorithmic-byte accounting of one strain launch
+int mark_elements(State &G, bool fwd, int npairs, const Located *loc, int P, const uint8_t *active) {
+    const int R = fwd ? 0 : G.R;
+    const int nelem = fwd ? G.fwd.d.nelem : G.bwd.d.nelem;
+    const size_t words = ((size_t)nelem * std::max(R, 1) + 31) / 32 + 1;
+    HIPCHECK(hipMemsetAsync(G.elem_mark.p, 0, words * 4, G.stream));
+    hipLaunchKernelGGL(k_mark, dim3((npairs + 255) / 256), dim3(256), 0, G.stream, npairs, loc, R, P, active,
+                       nelem, G.elem_mark.p, G.elem_count.p + (fwd ? 0 : 2));
+    HIPCHECK(hipGetLastError());
     return MCK_OK;
 }
 
@@ -916,6 +931,8 @@ int eval_chunk(State &G, int P, int w0, int nw, std::vector<std::pair<hipEvent_t
                        G.bwd.d, G.frames.p, 1, R, np, G.xyz.p, G.S.planet_radius, act, P, R,
                        G.loc.p + (size_t)np, G.err.p);
     HIPCHECK(hipGetLastError());
+    if ((rc = mark_elements(G, true, np, G.loc.p, P, nullptr))) return rc;
+    if ((rc = mark_elements(G, false, np * R, G.loc.p + np, P, act))) return rc;
     hipEvent_t a, b;
     HIPCHECK(hipEventCreate(&a));
     HIPCHECK(hipEventCreate(&b));
@@ -989,8 +1006,9 @@ int finish_task(State &G, const Task &T, double *values, double *variance, int32
     const int ne = T.nelems, nk = G.nk;
     hipStream_t st = G.stream;
     std::vector<int> draws(ne), steps(ne);
-    unsigned long long ev = 0;
+    unsigned long long ev = 0, cnt[4] = {0, 0, 0, 0};
     int err = 0;
+    HIPCHECK(hipMemcpyAsync(cnt, G.elem_count.p, 4 * 8, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipMemcpyAsync(values, G.integral.p, (size_t)ne * nk * 8, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipMemcpyAsync(variance, G.variance.p, (size_t)ne * nk * 8, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipMemcpyAsync(niterations, G.niter.p, (size_t)ne * nk * 4, hipMemcpyDeviceToHost, st));
@@ -1000,6 +1018,14 @@ int finish_task(State &G, const Task &T, double *values, double *variance, int32
     HIPCHECK(hipMemcpyAsync(&err, G.err.p, 4, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     G.stats.evaluations = (int64_t)ev;
+    {
+        const double blk = 25.0 * G.pc.nt * sizeof(float);
+        const double spec = (double)G.pc.nd * G.pc.nomega * sizeof(float2);
+        G.stats.fwd_locations = (int64_t)cnt[1];
+        G.stats.bwd_locations = (int64_t)cnt[3];
+        G.stats.fwd_bytes = (int64_t)(cnt[0] * G.pc.nvar_fwd * blk + cnt[1] * spec);
+        G.stats.bwd_bytes = (int64_t)(cnt[2] * G.pc.nvar_bwd * blk + cnt[3] * spec);
+    }
     if (err & 1) return fail(MCK_ENOTFOUND, "element not found: a point is outside the 12 nearest elements");
     if (err & 2) return fail(MCK_ENAN, "NaN in a Monte Carlo integral");
     long long tot_steps = 0;

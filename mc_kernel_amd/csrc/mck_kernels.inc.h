@@ -924,4 +924,27 @@ __global__ void k_mc(int nwork, int P, int nk, int R, const int *work_voxel, con
     }
 }
 
+// Distinct elements gathered by one strain launch (per receiver for bwd frames): the
+// |F(B)| and sum_r |G_r(B)| of the algorithmic byte count.  counters: [0] distinct,
+// [1] pairs evaluated.
+__global__ void k_mark(int npairs, const Located *loc, int R, int P, const uint8_t *active,
+                       int nelem, unsigned *bitmap, unsigned long long *counters)
+{
+    int pair = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pair >= npairs) return;
+    int r = 0;
+    if (R > 0) {
+        int p = pair / R;
+        r = pair - p * R;
+        if (active && !active[(p / P) * R + r]) return;
+    }
+    int e = loc[pair].elem;
+    if (e < 0) return;
+    size_t key = (size_t)r * nelem + e;
+    unsigned bit = 1u << (key & 31);
+    unsigned old = atomicOr(&bitmap[key >> 5], bit);
+    if (!(old & bit)) atomicAdd(&counters[0], 1ull);
+    atomicAdd(&counters[1], 1ull);
+}
+
 }  // namespace mck

@@ -340,6 +340,34 @@ void synth_db_fill(int32_t nelem, const int32_t *sem, int32_t nvar, int32_t nt, 
     for (auto &x : th) x.join();
 }
 
+// on-demand filler of single elements (merged layout), for bounded CPU samples
+struct LazyDb {
+    const int32_t *sem;
+    int32_t nvar[2], nt;
+    double dt;
+    uint64_t seed[2];
+};
+
+void *synth_lazy_create(const int32_t *sem, int32_t nvar_fwd, int32_t nvar_bwd, int32_t nt,
+                        double dt, uint64_t seed_fwd, uint64_t seed_bwd) {
+    LazyDb *L = new LazyDb;
+    L->sem = sem; L->nvar[0] = nvar_fwd; L->nvar[1] = nvar_bwd; L->nt = nt; L->dt = dt;
+    L->seed[0] = seed_fwd; L->seed[1] = seed_bwd;
+    return L;
+}
+
+void synth_lazy_free(void *h) { delete (LazyDb *)h; }
+
+void synth_fill_element(int which, int e, float *dst, void *user) {
+    LazyDb *L = (LazyDb *)user;
+    const int nv = L->nvar[which], nt = L->nt;
+    for (int v = 0; v < nv; ++v)
+        for (int j = 0; j < 5; ++j)
+            for (int i = 0; i < 5; ++i)
+                synth_trace(L->seed[which], L->sem[25 * e + 5 * j + i], v, nt, L->dt,
+                            dst + (((size_t)v * 5 + j) * 5 + i) * nt);
+}
+
 // radial model table (create_interpolator, interpolate_mesh.f90:45-79): value of the
 // GLL point nearest (fp32 distance) to (0, (i-1) dz); the model at a point is given by
 // the smooth profile below.

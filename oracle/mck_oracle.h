@@ -90,6 +90,10 @@ int      ora_slave_work(ora_ctx *ctx, int nelems, const double *vertices,
 /* seismogram/kernel setup products (kernel.f90:181-385) for inspection */
 int      ora_kernel_info(ora_ctx *ctx, int k, double *normalization, int *ncut, int *ifirst);
 int      ora_strain_nd(ora_ctx *ctx);
+/* Optional on-demand generation of database elements (which: 0 fwd, 1 bwd): the
+ * snapshots arrays must then be writable; used for bounded CPU-baseline samples. */
+typedef void (*ora_fill_fn)(int which, int elem, float *dst, void *user);
+void     ora_set_lazy_fill(ora_ctx *ctx, ora_fill_fn fn, void *user);
 
 #ifdef __cplusplus
 }

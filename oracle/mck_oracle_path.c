@@ -57,7 +57,19 @@ struct ora_ctx {
     ora_rec *R;
     double src_lat, src_lon, src_colat, src_r[3];
     int64_t halton_step;
+    ora_fill_fn fill;
+    void *fill_user;
+    unsigned char *filled[2];
 };
+
+void ora_set_lazy_fill(ora_ctx *C, ora_fill_fn fn, void *user)
+{
+    C->fill = fn;
+    C->fill_user = user;
+    free(C->filled[0]); free(C->filled[1]);
+    C->filled[0] = (unsigned char *)calloc(C->S.fwd.nelem, 1);
+    C->filled[1] = (unsigned char *)calloc(C->S.bwd.nelem, 1);
+}
 
 static int param_needs(int param, int *nb)
 {
@@ -308,6 +320,7 @@ void ora_destroy(ora_ctx *C)
     if (!C) return;
     if (C->K) for (k = 0; k < C->S.nkernel; ++k) { free(C->K[k].seis_cut); free(C->K[k].seis_cut_fd); }
     free(C->K); free(C->R); free(C->tf_fwd); free(C->tf_bwd); free(C->ts_fwd); free(C->ts_bwd);
+    free(C->filled[0]); free(C->filled[1]);
     free(C);
 }
 
@@ -465,6 +478,13 @@ static void strain_point(ora_ctx *C, const mck_db_t *db, int e, int pole, int va
     double *st = (double *)malloc(6 * blk * sizeof(double));
     int nv = pole == 0 ? 2 : 3;
     const float *src = db->snapshots + ((size_t)e * db->nvar + var0) * blk;
+    if (C->fill) {
+        int which = (db == &C->S.fwd) ? 0 : 1;
+        if (!C->filled[which][e]) {
+            C->fill(which, e, (float *)(db->snapshots + (size_t)e * db->nvar * blk), C->fill_user);
+            C->filled[which][e] = 1;
+        }
+    }
     for (v = 0; v < nv; ++v)
         for (k = 0; k < (int)blk; ++k) u[v * blk + k] = (double)src[v * blk + k];
     if (C->full) ora_strain_pole(pole, u, nt, 4, nodes, db->eltype[e], axial, st);

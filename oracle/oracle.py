@@ -57,6 +57,7 @@ def lib():
         L.ora_kernel_info.argtypes = [C.c_void_p, C.c_int, f64p, C.POINTER(C.c_int),
                                       C.POINTER(C.c_int)]
         L.ora_strain_nd.argtypes = [C.c_void_p]
+        L.ora_set_lazy_fill.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         _LIB = L
     return _LIB
 
@@ -196,6 +197,10 @@ class Oracle:
         if getattr(self, "h", None):
             lib().ora_destroy(self.h)
             self.h = None
+
+    def set_lazy_fill(self, fn_ptr, user_ptr):
+        """Generate database elements on first use (fn_ptr: C function address)."""
+        lib().ora_set_lazy_fill(self.h, fn_ptr, user_ptr)
 
     def locate(self, frame, xyz):
         xyz = np.ascontiguousarray(xyz, float)
