@@ -130,7 +130,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c1")
-    ap.add_argument("--task", type=int, default=2048, help="voxels per step (task)")
+    ap.add_argument("--task", type=int, default=0, help="voxels per step (0 = whole job)")
     ap.add_argument("--cpu-workers", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--cpu-voxels-per-worker", type=int, default=1)
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -153,7 +153,7 @@ def main():
     p = workload(args.config)
     lib, dbs = gpu_db(p)
     k = mk.MCKernel(p.setup)
-    T = args.task
+    T = args.task if args.task > 0 else p.ntet
     ntask = (p.ntet + T - 1) // T
     steps = []
     for i in range(args.warmup + args.steps):

@@ -571,17 +571,28 @@ __global__ __launch_bounds__(kStrainThreads) void k_strain_fft(
 
     if (trace_out) {
         // raw strain traces (before cumsum): [pair][nd][nt]
-        for (int c = 0; c < nd; ++c)
 #pragma unroll
-            for (int q = 0; q < SPT; ++q)
-                if (t0 + q < nt) trace_out[((size_t)pair * nd + c) * nt + t0 + q] = acc[c][q];
+        for (int c = 0; c < 6; ++c)
+            if (c < nd)
+#pragma unroll
+                for (int q = 0; q < SPT; ++q)
+                    if (t0 + q < nt) trace_out[((size_t)pair * nd + c) * nt + t0 + q] = acc[c][q];
         return;
     }
     const int lane = tid & 63, wave = tid >> 6;
     for (int c = 0; c < nd; ++c) {
+        // component c of the accumulators, selected with static indices so that acc stays
+        // in registers
         float x[SPT];
 #pragma unroll
-        for (int q = 0; q < SPT; ++q) x[q] = (t0 + q < nt) ? acc[c][q] : 0.0f;
+        for (int q = 0; q < SPT; ++q) x[q] = 0.0f;
+#pragma unroll
+        for (int cc = 0; cc < 6; ++cc)
+            if (cc == c)
+#pragma unroll
+                for (int q = 0; q < SPT; ++q) x[q] = acc[cc][q];
+#pragma unroll
+        for (int q = 0; q < SPT; ++q) if (t0 + q >= nt) x[q] = 0.0f;
         if (fwd) {
             // cumsum_trapezoidal: c0 = x0, c1 = c0 + x1, ct = c(t-1) + x(t-1) + xt
             edge_x[tid + 1] = x[SPT - 1];
