@@ -90,6 +90,14 @@ int      ora_slave_work(ora_ctx *ctx, int nelems, const double *vertices,
 /* seismogram/kernel setup products (kernel.f90:181-385) for inspection */
 int      ora_kernel_info(ora_ctx *ctx, int k, double *normalization, int *ncut, int *ifirst);
 int      ora_strain_nd(ora_ctx *ctx);
+/* Voigt tensor rotations alone (test hook): t[6][nt] in place */
+void     ora_rotate_voigt(int which, double *t, int nt, double phi, double theta);
+/* integrate_parseval alone (test hook) */
+double   ora_integrate_parseval(int n, double dt, const double *f, const double *g);
+/* the Monte Carlo integrator alone (test hook): vals[nsteps][npts][n] */
+int      ora_mc_run(int n, double volume, double allowed_error, double allowed_rel, int nsteps,
+                    int npts, const double *vals, double *integral, double *variance,
+                    int32_t *conv);
 /* Optional on-demand generation of database elements (which: 0 fwd, 1 bwd): the
  * snapshots arrays must then be writable; used for bounded CPU-baseline samples. */
 typedef void (*ora_fill_fn)(int which, int elem, float *dst, void *user);

@@ -168,6 +168,22 @@ static double parseval(int M, double dt, const double *af, const double *bf)
     return tot.re * df;
 }
 
+/* Test hook: integrate_parseval of two time series of n samples, zero padded to
+ * M = 2 nextpow2(n) as the kernel's own FFT plan is (kernel.f90:975-1020). */
+double ora_integrate_parseval(int n, double dt, const double *f, const double *g)
+{
+    int M = 2 * ora_nextpow2(n), i;
+    double *pf = (double *)calloc(2 * (size_t)M, sizeof(double)), *pg = pf + M;
+    double *F = (double *)malloc(4 * (size_t)(M / 2 + 1) * sizeof(double)), *G = F + 2 * (M / 2 + 1);
+    double v;
+    for (i = 0; i < n; ++i) { pf[i] = f[i]; pg[i] = g[i]; }
+    ora_rfft(M, dt, pf, F);
+    ora_rfft(M, dt, pg, G);
+    v = parseval(M, dt, F, G);
+    free(pf); free(F);
+    return v;
+}
+
 /* cut_and_add_seismogram (kernel.f90:181-385) */
 static int kernel_init(ora_ctx *C, int k, char *err, int errlen)
 {
@@ -449,6 +465,14 @@ static void rot_xyz_src_to_earth(double *t, int nt, double phi, double theta)
 
 /* lagrange_interpol_2D_td (spectral_basis.f90:181-314) of node values
  * val[c][jpol][ipol][nt] -> out[c][nt] */
+/* Test hook: which 0 = src->xyz (azimuth phi), 1 = xyz_src->xyz_earth (phi, theta);
+ * t[6][nt] Voigt (xx, yy, zz, yz, xz, xy), rotated in place. */
+void ora_rotate_voigt(int which, double *t, int nt, double phi, double theta)
+{
+    if (which == 0) rot_src_to_xyz(t, nt, phi);
+    else rot_xyz_src_to_earth(t, nt, phi, theta);
+}
+
 static void interp_nodes(int nc, int nt, const double *val, const double *xp, const double *ep,
                          double x1, double x2, double *out)
 {
@@ -835,6 +859,20 @@ static int mc_check(ora_mc *m, int npts, const double *vals /* [npts][n] */)
 }
 
 static void mc_free(ora_mc *m) { free(m->fsum); free(m->conv); }
+
+/* Test hook: the integrator alone, fed nsteps batches of npts values of n functions
+ * (vals[nsteps][npts][n]); returns the final integral/variance/convergence flags. */
+int ora_mc_run(int n, double volume, double allowed_error, double allowed_rel, int nsteps,
+               int npts, const double *vals, double *integral, double *variance, int32_t *conv)
+{
+    ora_mc m;
+    int s, i, rc = 0;
+    mc_init(&m, n, volume, allowed_error, allowed_rel);
+    for (s = 0; s < nsteps && !rc; ++s) rc = mc_check(&m, npts, vals + (size_t)s * npts * n);
+    for (i = 0; i < n; ++i) { integral[i] = m.integral[i]; variance[i] = m.variance[i]; conv[i] = m.conv[i]; }
+    mc_free(&m);
+    return rc;
+}
 
 /* travel_time_larger_Tmax (slave.f90:928-950) */
 static int travel_time_masked(ora_ctx *C, int r, const double *center)

@@ -17,9 +17,8 @@ _LIB = None
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
-        if not os.path.exists(path):
-            raise RuntimeError("oracle/liboracle.so is not built")
+        from mc_kernel_amd import build
+        path = build.build_oracle(force=False)      # rebuilds only when sources are newer
         L = C.CDLL(path)
         f64p, f32p, i32p, i64p = abi.f64p, abi.f32p, abi.i32p, abi.i64p
         L.ora_gll_points.argtypes = [C.c_int, f64p]
@@ -58,6 +57,11 @@ def lib():
                                       C.POINTER(C.c_int)]
         L.ora_strain_nd.argtypes = [C.c_void_p]
         L.ora_set_lazy_fill.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ora_integrate_parseval.argtypes = [C.c_int, C.c_double, f64p, f64p]
+        L.ora_integrate_parseval.restype = C.c_double
+        L.ora_rotate_voigt.argtypes = [C.c_int, f64p, C.c_int, C.c_double, C.c_double]
+        L.ora_mc_run.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                 f64p, f64p, f64p, i32p]
         _LIB = L
     return _LIB
 
@@ -178,6 +182,34 @@ def nearest(mp_s, mp_z, s, z, nn=12):
     tie = lib().ora_nearest(len(mp_s), P(mp_s, C.c_float), P(mp_z, C.c_float), s, z, nn,
                             P(idx, C.c_int32), P(dist, C.c_float))
     return idx, dist, tie
+
+
+def integrate_parseval(f, g, dt):
+    f = np.ascontiguousarray(f, float)
+    g = np.ascontiguousarray(g, float)
+    return lib().ora_integrate_parseval(len(f), dt, P(f), P(g))
+
+
+def rotate_voigt(t, phi, theta=0.0, to_earth=False):
+    """t [6] or [6][nt] Voigt (xx, yy, zz, yz, xz, xy) -> rotated copy."""
+    a = np.array(t, float, order="C")
+    nt = 1 if a.ndim == 1 else a.shape[1]
+    lib().ora_rotate_voigt(int(to_earth), P(a), nt, phi, theta)
+    return a
+
+
+def mc_run(vals, volume, allowed_error=0.0, allowed_rel=0.0):
+    """Integrator alone: vals [nsteps][npts][n] -> (integral, variance, converged)."""
+    vals = np.ascontiguousarray(vals, float)
+    ns, npts, n = vals.shape
+    I = np.zeros(n)
+    V = np.zeros(n)
+    cv = np.zeros(n, np.int32)
+    rc = lib().ora_mc_run(n, volume, allowed_error, allowed_rel, ns, npts, P(vals), P(I), P(V),
+                          P(cv, C.c_int32))
+    if rc:
+        raise RuntimeError("oracle mc_run failed (%d)" % rc)
+    return I, V, cv.astype(bool)
 
 
 class Oracle:
