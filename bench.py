@@ -124,6 +124,20 @@ def gpu_db(p):
     return lib, out
 
 
+def measured_traffic(path, kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary: lines
+    'traffic <kernel tag>: <bytes per launch>' written from the FETCH_SIZE and WRITE_SIZE
+    runs, each scaled by the same counter's reading for a plain copy kernel."""
+    if not os.path.exists(path):
+        return None
+    tag = "bwd" if "bwd" in kernel else "fwd"
+    with open(path) as f:
+        for ln in f:
+            if ln.startswith("traffic %s:" % tag):
+                return float(ln.split(":", 1)[1].split()[0])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,6 +150,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-voxels", type=int, default=16)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first run a plain copy kernel of this size (HBM counter calibration)")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "strain_fft_c1_traffic.md"),
+                    help="summary of the FETCH_SIZE/WRITE_SIZE counter runs for the roofline")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -152,6 +170,14 @@ def main():
     import mc_kernel_amd as mk
     p = workload(args.config)
     lib, dbs = gpu_db(p)
+    if args.copy_calibration_mib > 0:
+        ms = C.c_float()
+        nbytes = args.copy_calibration_mib << 20
+        lib.synth_gpu_copy_calibration.argtypes = [C.c_size_t, C.c_int, C.POINTER(C.c_float)]
+        if lib.synth_gpu_copy_calibration(nbytes, 5, C.byref(ms)):
+            raise RuntimeError("copy calibration failed")
+        print("copy calibration: %d bytes read + %d written per launch, %.3f ms, %.1f GB/s"
+              % (nbytes, nbytes, ms.value, 2 * nbytes / ms.value / 1e6), file=sys.stderr, flush=True)
     k = mk.MCKernel(p.setup)
     T = args.task if args.task > 0 else p.ntet
     ntask = (p.ntet + T - 1) // T
@@ -181,7 +207,8 @@ def main():
         kern, byt, ms = "k_strain_fft<8> (fwd frame)", fwd_b, fwd_ms
     achieved = (byt / launches) / (ms / launches * 1e-3) / 1e9 if ms > 0 else 0.0
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kern,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.traffic, kern),
+            "kernel": kern,
             "bytes_per_launch": byt / max(launches, 1), "avg_launch_ms": ms / max(launches, 1)}
 
     # plain one-point-at-a-time form on a small sample of the same workload

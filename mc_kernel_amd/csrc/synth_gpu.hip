@@ -50,9 +50,45 @@ __global__ void k_fill(int nelem, const int *sem, int nvar, int nt, float dt, ui
     }
 }
 
+// plain streaming copy: the HBM-traffic calibration for hardware counters (a launch
+// reads and writes exactly n4 * 16 bytes)
+__global__ void k_copy_calibration(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                                   size_t n4) {
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride)
+        dst[i] = src[i];
+}
+
 }  // namespace
 
 extern "C" {
+
+// Runs `reps` launches of a plain copy of `bytes` bytes (a multiple of 16) between two
+// fresh device buffers; returns the average launch time in ms through *ms.
+int synth_gpu_copy_calibration(size_t bytes, int reps, float *ms) {
+    float4 *a = nullptr, *b = nullptr;
+    size_t n4 = bytes / 16;
+    if (bytes % 16 || reps < 1) return 1;
+    if (hipMalloc((void **)&a, bytes) != hipSuccess) return 2;
+    if (hipMalloc((void **)&b, bytes) != hipSuccess) { (void)hipFree(a); return 2; }
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    hipLaunchKernelGGL(k_copy_calibration, dim3(4096), dim3(256), 0, 0, a, b, n4);
+    (void)hipEventRecord(e0, 0);
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(k_copy_calibration, dim3(4096), dim3(256), 0, 0, a, b, n4);
+    (void)hipEventRecord(e1, 0);
+    hipError_t err = hipEventSynchronize(e1);
+    float t = 0.0f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    *ms = t / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return err == hipSuccess ? 0 : 3;
+}
 
 // Allocates [nelem][nvar][5][5][nt] floats in device memory and fills them; sem is the
 // host array of GLL point ids [nelem][5][5].  Returns 0 on success.
