@@ -3,6 +3,7 @@
 // filters and per-kernel window spectra) and the step-synchronous batched evaluation
 // of slave_work (src/slave.f90:273-810) over all voxels of a task.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -309,7 +310,7 @@ struct DbDev {
     DevDb d{};
     float *u_owned = nullptr;      // device copy of the snapshots (null if caller-resident)
     DBuf<float> mp_s, mp_z;
-    DBuf<double> econ;
+    DBuf<double> econ, ejac;
     DBuf<int> eltype, axis, cell_start, cell_elems;
     std::vector<double> econ_host;
     std::vector<int> eltype_host, axis_host;
@@ -324,7 +325,7 @@ struct State {
     int R = 0, nk = 0, nframes = 0;
     DBuf<DevFrame> frames;
     std::vector<DevFrame> frames_host;
-    DBuf<float2> tw, tw2, H;
+    DBuf<float2> twq, H;
     DBuf<RecDesc> recs;
     DBuf<KernDesc> kerns;
     DBuf<float> model;
@@ -344,7 +345,9 @@ struct State {
     DBuf<double> xyz;
     DBuf<int> model_idx;
     DBuf<Located> loc;
-    DBuf<float2> Fspec, Bspec;
+    DBuf<float2> Fspec;                 // [point][nd_f][nomega] forward spectra
+    DBuf<int> keys_in, keys_out, vals_in, vals_out, pair_count;
+    DBuf<unsigned char> sort_tmp;
     DBuf<double> phys;
     DBuf<unsigned long long> evals;
     DBuf<int> err;
@@ -591,6 +594,33 @@ int build_kernels(State &G) {
     }
     for (int r = 0; r < G.R; ++r)
         if (recs[r].first_kernel < 0) return fail(MCK_EINVAL, "receiver without kernels");
+    // order each receiver's slots by window spectrum (one H read per bin and group)
+    for (int r = 0; r < G.R; ++r) {
+        RecDesc &rd = recs[r];
+        std::vector<int> perm(rd.nslot), inv(rd.nslot);
+        for (int q = 0; q < rd.nslot; ++q) perm[q] = q;
+        std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return rd.slot_h[a] < rd.slot_h[b]; });
+        RecDesc o = rd;
+        for (int q = 0; q < rd.nslot; ++q) {
+            rd.slot_h[q] = o.slot_h[perm[q]];
+            rd.slot_bk[q] = o.slot_bk[perm[q]];
+            inv[perm[q]] = q;
+        }
+        for (int k = rd.first_kernel; k < rd.first_kernel + rd.nkernel; ++k)
+            for (int b = 0; b < 6; ++b)
+                if (kerns[k].slot[b] >= 0) kerns[k].slot[b] = inv[kerns[k].slot[b]];
+        for (int q = 0; q < rd.nslot; ++q) rd.slot_lds[q] = G.pc.bk_lds[rd.slot_bk[q]];
+        rd.ngroup = 0;
+        for (int q = 0; q < rd.nslot;) {
+            int e = q + 1;
+            while (e < rd.nslot && e - q < kSlotGroup && rd.slot_h[e] == rd.slot_h[q]) ++e;
+            rd.grp_h[rd.ngroup] = rd.slot_h[q];
+            rd.grp_first[rd.ngroup] = q;
+            rd.grp_n[rd.ngroup] = e - q;
+            ++rd.ngroup;
+            q = e;
+        }
+    }
     std::vector<float2> Hh(Hs.size() * nw);
     for (size_t h = 0; h < Hs.size(); ++h)
         for (int q = 0; q < nw; ++q) Hh[h * nw + q] = make_float2((float)Hs[h][q].real(), (float)Hs[h][q].imag());
@@ -615,11 +645,12 @@ int build_kernels(State &G) {
 }
 
 
+// LDS of the forward kernel: two FFT buffers and the quarter twiddle table
 size_t strain_smem(int M) {
-    return ((sizeof(OpShared) + 15) / 16) * 16 + 2 * (size_t)M * sizeof(float2);
+    return ((sizeof(OpShared) + 15) / 16) * 16 + (2 * (size_t)M + M / 2) * sizeof(float2);
 }
 
-// samples per thread of the strain kernel for a trace length
+// samples per thread of the strain kernels for a trace length
 int spt_for(int nt) {
     int spt = (nt + kStrainThreads - 1) / kStrainThreads;
     int p = 1;
@@ -627,24 +658,95 @@ int spt_for(int nt) {
     return p;
 }
 
-hipError_t launch_strain(State &G, int fwd, int npairs, int P, const Located *loc, int loc_stride,
-                         const uint8_t *active, float2 *out, float *trace_out) {
+unsigned grid_x8(int n) { return 8u * (unsigned)((n + 7) / 8); }
+
+// forward spectra of npts points (order: element-sorted point ids, or null = identity)
+hipError_t launch_fwd(State &G, int npts, const int *order, const Located *loc, float2 *F) {
+    if (npts <= 0) return hipSuccess;
+    const size_t smem = strain_smem(G.pc.M);
+    const dim3 grid(grid_x8(npts)), block(kStrainThreads);
+    const int nd = G.pc.nd_f;
+#define FWD(SP, ND)                                                                               \
+    if (spt_for(G.pc.nt) == SP && nd == ND) {                                                     \
+        if (smem > 65536)                                                                         \
+            (void)hipFuncSetAttribute((const void *)&k_fwd_spectra<SP, ND>,                       \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);     \
+        hipLaunchKernelGGL((k_fwd_spectra<SP, ND>), grid, block, smem, G.stream, G.fwd.d, G.pc,   \
+                           G.frames.p, npts, order, loc, G.twq.p, F);                            \
+        return hipGetLastError();                                                                 \
+    }
+#define FWD_ND(SP) FWD(SP, 1) FWD(SP, 4) FWD(SP, 6)
+    FWD_ND(1) FWD_ND(2) FWD_ND(4) FWD_ND(8) FWD_ND(16)
+#undef FWD_ND
+#undef FWD
+    return hipErrorInvalidValue;
+}
+
+// LDS of the backward kernel: FFT buffers, reused for the base-kernel spectra, and the
+// quarter twiddle table
+size_t bwd_smem(int M, int nbk) {
+    return ((sizeof(OpShared) + 15) / 16) * 16
+         + (std::max<size_t>(2 * (size_t)M, (size_t)nbk * (M + 1)) + M / 2) * sizeof(float2);
+}
+
+// backward strain + evaluation of npairs (point, receiver) pairs q = p * R + r
+hipError_t launch_bwd(State &G, int npairs, const int *order, const Located *loc_f,
+                      const Located *loc_b, double *phys, double *base_out) {
     if (npairs <= 0) return hipSuccess;
+    const size_t smem = bwd_smem(G.pc.M, G.pc.nbk_lds);
+    const dim3 grid(grid_x8(npairs)), block(kStrainThreads);
+    const int nf = G.pc.nd_f, nb = G.pc.nd_b;
+#define BWD(SP, NF, NB)                                                                           \
+    if (spt_for(G.pc.nt) == SP && nf == NF && nb == NB) {                                         \
+        if (smem > 65536)                                                                         \
+            (void)hipFuncSetAttribute((const void *)&k_bwd_eval<SP, NF, NB>,                      \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);     \
+        hipLaunchKernelGGL((k_bwd_eval<SP, NF, NB>), grid, block, smem, G.stream, G.bwd.d, G.pc,  \
+                           G.frames.p, npairs, order, G.R, loc_f, loc_b, G.twq.p,                \
+                           G.Fspec.p, G.recs.p, G.kerns.p, G.H.p, G.model_idx.p, G.model.p,     \
+                           G.S.model_ndepth, G.S.relative_kernel, G.nk, phys, base_out);        \
+        return hipGetLastError();                                                                 \
+    }
+#define BWD_ND(SP) BWD(SP, 1, 1) BWD(SP, 4, 4) BWD(SP, 4, 6) BWD(SP, 6, 4) BWD(SP, 6, 6)
+    BWD_ND(1) BWD_ND(2) BWD_ND(4) BWD_ND(8) BWD_ND(16)
+#undef BWD_ND
+#undef BWD
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_traces(State &G, int fwd, int frame, int npts, const Located *loc, float *out) {
+    if (npts <= 0) return hipSuccess;
     const DbDev &D = fwd ? G.fwd : G.bwd;
-    size_t smem = strain_smem(G.pc.M);
-    dim3 grid(npairs), block(kStrainThreads);
     switch (spt_for(G.pc.nt)) {
-#define CASE(S)                                                                                  \
-    case S:                                                                                      \
-        hipLaunchKernelGGL((k_strain_fft<S>), grid, block, smem, G.stream, D.d, G.pc, G.frames.p, \
-                           fwd, npairs, G.R, P, loc, loc_stride, active, G.tw.p, G.tw2.p, out,   \
-                           trace_out);                                                           \
+#define CASE(SP)                                                                                  \
+    case SP:                                                                                      \
+        hipLaunchKernelGGL((k_strain_traces<SP>), dim3(npts), dim3(kStrainThreads), 0, G.stream,  \
+                           D.d, G.pc, G.frames.p, fwd, frame, npts, loc, out);                   \
         break;
         CASE(1) CASE(2) CASE(4) CASE(8) CASE(16)
 #undef CASE
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+// stable sort of n (element, id) pairs from keys_in/vals_in into keys_out/vals_out
+int sort_by_element(State &G, int n, int nelem) {
+    if (n <= 1) {
+        if (n == 1) {
+            HIPCHECK(hipMemcpyAsync(G.vals_out.p, G.vals_in.p, 4, hipMemcpyDeviceToDevice, G.stream));
+        }
+        return MCK_OK;
+    }
+    int bits = 1;
+    while ((1 << bits) <= nelem) ++bits;
+    size_t need = 0;
+    HIPCHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, need, G.keys_in.p, G.keys_out.p, G.vals_in.p,
+                                                G.vals_out.p, n, 0, bits, G.stream));
+    HIPCHECK(G.sort_tmp.ensure(need));
+    HIPCHECK(hipcub::DeviceRadixSort::SortPairs(G.sort_tmp.p, need, G.keys_in.p, G.keys_out.p, G.vals_in.p,
+                                                G.vals_out.p, n, 0, bits, G.stream));
+    return MCK_OK;
 }
 
 int setup_tables(State &G) {
@@ -684,21 +786,16 @@ int setup_tables(State &G) {
             G.rec_r[3 * r + 2] = std::sin(la) * S.planet_radius;
         }
     }
-    // twiddles
+    // quarter twiddle table w_N^r = exp(-2 pi i r / N), r < N/4, N = 2M (the kernels derive
+    // the other quadrants and the length-M twiddles w_M^t = w_N^(2t) from it)
     const int M = G.pc.M, N = 2 * M;
-    std::vector<float2> tw(M), tw2(M + 1);
-    for (int m = 0; m < M; ++m) {
-        double a = -2.0 * M_PI * m / M;
-        tw[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+    std::vector<float2> twq(N / 4);
+    for (int r = 0; r < N / 4; ++r) {
+        double a = -2.0 * M_PI * r / N;
+        twq[r] = make_float2((float)std::cos(a), (float)std::sin(a));
     }
-    for (int k = 0; k <= M; ++k) {
-        double a = -2.0 * M_PI * k / N;
-        tw2[k] = make_float2((float)std::cos(a), (float)std::sin(a));
-    }
-    HIPCHECK(G.tw.ensure(M));
-    HIPCHECK(G.tw2.ensure(M + 1));
-    HIPCHECK(hipMemcpy(G.tw.p, tw.data(), M * sizeof(float2), hipMemcpyHostToDevice));
-    HIPCHECK(hipMemcpy(G.tw2.p, tw2.data(), (M + 1) * sizeof(float2), hipMemcpyHostToDevice));
+    HIPCHECK(G.twq.ensure(N / 4));
+    HIPCHECK(hipMemcpy(G.twq.p, twq.data(), (N / 4) * sizeof(float2), hipMemcpyHostToDevice));
     // radial model table [6][ndepth]
     const int nd = S.model_ndepth;
     std::vector<float> mt((size_t)6 * nd);
@@ -713,12 +810,17 @@ int setup_tables(State &G) {
 }
 
 int ensure_chunk(State &G, int points) {
-    const int R = G.R, nd = G.pc.nd, nw = G.pc.nomega;
+    const int R = G.R, nw = G.pc.nomega;
+    const size_t npairs = (size_t)points * std::max(R, 1);
     HIPCHECK(G.xyz.ensure((size_t)points * 3));
     HIPCHECK(G.model_idx.ensure(points));
     HIPCHECK(G.loc.ensure((size_t)points * (1 + R)));
-    HIPCHECK(G.Fspec.ensure((size_t)points * nd * nw));
-    HIPCHECK(G.Bspec.ensure((size_t)points * R * nd * nw));
+    HIPCHECK(G.Fspec.ensure((size_t)points * G.pc.nd_f * nw));
+    HIPCHECK(G.keys_in.ensure(npairs));
+    HIPCHECK(G.keys_out.ensure(npairs));
+    HIPCHECK(G.vals_in.ensure(npairs));
+    HIPCHECK(G.vals_out.ensure(npairs));
+    HIPCHECK(G.pair_count.ensure(1));
     return MCK_OK;
 }
 
@@ -773,6 +875,27 @@ int init_impl(const mck_setup_t *s) {
         pc.bwd_isim = 2; pc.nvar_bwd = 3; pc.bwd_vars[0] = 2; pc.bwd_vars[1] = 3; pc.bwd_vars[2] = 4;
         pc.bwd_fi[0] = 0; pc.bwd_fi[1] = S.component == 'R' ? 1.0 : 0.0; pc.bwd_fi[2] = S.component == 'T' ? 1.0 : 0.0;
     }
+    // carried components: the trace, or the source-frame Voigt components with a non-zero
+    // azimuthal factor (monopole simulations have no zp and sp strain)
+    const bool fwd_mono = !act[2] && !act[3];
+    const bool bwd_mono = pc.bwd_isim == 1;
+    auto carry = [&](bool mono, int *nd, int *comps) {
+        if (!pc.full) { *nd = 1; comps[0] = 0; return; }
+        const int mc[4] = {0, 1, 2, 4};
+        *nd = mono ? 4 : 6;
+        for (int i = 0; i < *nd; ++i) comps[i] = mono ? mc[i] : i;
+    };
+    carry(fwd_mono, &pc.nd_f, pc.comps_f);
+    carry(bwd_mono, &pc.nd_b, pc.comps_b);
+    if (pc.nd_f == 1 && pc.nd_b != 1) return fail(MCK_EINVAL, "inconsistent strain types");
+    pc.need_bk = 0;
+    for (int k = 0; k < S.nkernel; ++k) {
+        int nb[6];
+        if (needs_of(S.kernel_param[k], nb) == 0)
+            for (int b = 0; b < 6; ++b) if (nb[b]) pc.need_bk |= 1 << b;
+    }
+    pc.nbk_lds = 0;
+    for (int b = 0; b < 6; ++b) pc.bk_lds[b] = (pc.need_bk >> b) & 1 ? pc.nbk_lds++ : -1;
     pc.ntaper = 10;
     for (int j = 0; j < 10; ++j) {
         float q = 3.3f * (float)(10 - j) / 10.0f;
@@ -782,18 +905,27 @@ int init_impl(const mck_setup_t *s) {
     if ((rc = upload_db(G.fwd, S.fwd, S.nt, "fwd"))) return rc;
     if ((rc = upload_db(G.bwd, S.bwd, S.nt, "bwd"))) return rc;
     if ((rc = setup_tables(G))) return rc;
+    for (DbDev *D : {&G.fwd, &G.bwd}) {
+        // inverse Jacobians and 1/s at the nodes of every element (needs the basis)
+        HIPCHECK(D->ejac.ensure((size_t)D->d.nelem * 125));
+        hipLaunchKernelGGL(k_elem_tables, dim3((D->d.nelem * 25 + 255) / 256), dim3(256), 0, G.stream, D->d,
+                           D->ejac.p);
+        HIPCHECK(hipGetLastError());
+        D->d.ejac = D->ejac.p;
+    }
+    HIPCHECK(hipStreamSynchronize(G.stream));
     if ((rc = build_kernels(G))) return rc;
     HIPCHECK(G.evals.ensure(1));
     HIPCHECK(G.err.ensure(1));
-    HIPCHECK(G.elem_count.ensure(4));
-    HIPCHECK(G.elem_mark.ensure(((size_t)std::max(G.fwd.d.nelem, G.bwd.d.nelem) * G.R + 31) / 32 + 1));
+    HIPCHECK(G.elem_count.ensure(8));
     HIPCHECK(G.nwork.ensure(1));
-    // points per chunk: spectra of (1 + R) frames within a memory budget
+    // points per chunk: forward spectra, locations and sort buffers within a memory budget
     size_t free_b = 0, total_b = 0;
     HIPCHECK(hipMemGetInfo(&free_b, &total_b));
-    size_t per_point = (size_t)(1 + G.R) * pc.nd * pc.nomega * sizeof(float2) + (size_t)(1 + G.R) * sizeof(Located) + 64;
-    size_t budget = std::min<size_t>(free_b / 3, (size_t)48 << 30);
-    G.chunk_points = (int)std::max<size_t>(1024, std::min<size_t>(budget / per_point, 1 << 20));
+    size_t per_point = (size_t)pc.nd_f * pc.nomega * sizeof(float2) + (size_t)(1 + G.R) * sizeof(Located)
+                     + (size_t)G.R * 16 + 64;
+    size_t budget = std::min<size_t>(free_b / 3, (size_t)64 << 30);
+    G.chunk_points = (int)std::max<size_t>(1024, std::min<size_t>(budget / per_point, 1 << 21));
     G.init = true;
     return MCK_OK;
 }
@@ -884,18 +1016,19 @@ int prepare_task(State &G, Task &T, int elem_type, int nvpe, int nelems, int nba
     HIPCHECK(hipMemsetAsync(G.steps.p, 0, ne * 4, st));
     HIPCHECK(hipMemsetAsync(G.evals.p, 0, 8, st));
     HIPCHECK(hipMemsetAsync(G.err.p, 0, 4, st));
-    HIPCHECK(hipMemsetAsync(G.elem_count.p, 0, 4 * 8, st));
+    HIPCHECK(hipMemsetAsync(G.elem_count.p, 0, 8 * 8, st));
     return MCK_OK;
 }
 
-// algorithmic-byte accounting of one strain launch
-int mark_elements(State &G, bool fwd, int npairs, const Located *loc, int P, const uint8_t *active) {
-    const int R = fwd ? 0 : G.R;
-    const int nelem = fwd ? G.fwd.d.nelem : G.bwd.d.nelem;
-    const size_t words = ((size_t)nelem * std::max(R, 1) + 31) / 32 + 1;
+// algorithmic-byte accounting of one launch: distinct elements (by_point 0) or points
+// (by_point 1) among the located pairs; counters at elem_count[slot], [slot + 1]
+int mark_keys(State &G, int npairs, const Located *loc, int R, int P, const uint8_t *active, int by_point,
+              size_t nkeys, int slot) {
+    const size_t words = (nkeys + 31) / 32 + 1;
+    HIPCHECK(G.elem_mark.ensure(words));
     HIPCHECK(hipMemsetAsync(G.elem_mark.p, 0, words * 4, G.stream));
     hipLaunchKernelGGL(k_mark, dim3((npairs + 255) / 256), dim3(256), 0, G.stream, npairs, loc, R, P, active,
-                       nelem, G.elem_mark.p, G.elem_count.p + (fwd ? 0 : 2));
+                       by_point, G.elem_mark.p, G.elem_count.p + slot);
     HIPCHECK(hipGetLastError());
     return MCK_OK;
 }
@@ -911,46 +1044,53 @@ VoxState vox_state(State &G) {
 // Evaluates the MC points of working voxels [w0, w0 + nw) of the current step; writes
 // physical kernel values phys[(w0 + w) * P + p][k].
 int eval_chunk(State &G, int P, int w0, int nw, std::vector<std::pair<hipEvent_t, hipEvent_t>> &evs) {
-    const int np = nw * P, R = G.R, nf = 1 + R;
+    const int np = nw * P, R = G.R;
     const int *wl = G.work_list.p + w0;
     const uint8_t *act = G.active.p + (size_t)w0 * R;
     hipStream_t st = G.stream;
     int rc;
     if ((rc = ensure_chunk(G, np))) return rc;
+    Located *loc_f = G.loc.p, *loc_b = G.loc.p + np;
     hipLaunchKernelGGL(k_points, dim3((np + 255) / 256), dim3(256), 0, st, nw, P, wl, G.hbase.p,
                        G.draws.p, G.tet.p, G.S.model_ndepth, G.S.model_dr, G.xyz.p, G.model_idx.p);
     HIPCHECK(hipGetLastError());
-    const long long nloc = (long long)np * nf;
     // fwd and bwd meshes may differ: locate frame 0 in the fwd mesh, frames 1.. in the bwd mesh
     hipLaunchKernelGGL(k_locate, dim3((unsigned)((np + 127) / 128)), dim3(128), 0, st, G.fwd.d,
-                       G.frames.p, 0, 1, np, G.xyz.p, G.S.planet_radius, nullptr, P, R,
-                       G.loc.p + (size_t)G.chunk_points * 0, G.err.p);
-    (void)nloc;
+                       G.frames.p, 0, 1, np, G.xyz.p, G.S.planet_radius, nullptr, P, R, loc_f, G.err.p);
     HIPCHECK(hipGetLastError());
     hipLaunchKernelGGL(k_locate, dim3((unsigned)(((long long)np * R + 127) / 128)), dim3(128), 0, st,
-                       G.bwd.d, G.frames.p, 1, R, np, G.xyz.p, G.S.planet_radius, act, P, R,
-                       G.loc.p + (size_t)np, G.err.p);
+                       G.bwd.d, G.frames.p, 1, R, np, G.xyz.p, G.S.planet_radius, act, P, R, loc_b, G.err.p);
     HIPCHECK(hipGetLastError());
-    if ((rc = mark_elements(G, true, np, G.loc.p, P, nullptr))) return rc;
-    if ((rc = mark_elements(G, false, np * R, G.loc.p + np, P, act))) return rc;
+    if ((rc = mark_keys(G, np, loc_f, 0, P, nullptr, 0, G.fwd.d.nelem, 0))) return rc;
+    if ((rc = mark_keys(G, np * R, loc_b, R, P, act, 0, G.bwd.d.nelem, 2))) return rc;
+    if ((rc = mark_keys(G, np * R, loc_b, R, P, act, 1, np, 4))) return rc;
+    // forward: points in element order
+    hipLaunchKernelGGL(k_keys_points, dim3((np + 255) / 256), dim3(256), 0, st, np, loc_f, G.fwd.d.nelem,
+                       G.keys_in.p, G.vals_in.p);
+    HIPCHECK(hipGetLastError());
+    if ((rc = sort_by_element(G, np, G.fwd.d.nelem))) return rc;
     hipEvent_t a, b;
     HIPCHECK(hipEventCreate(&a));
     HIPCHECK(hipEventCreate(&b));
     HIPCHECK(hipEventRecord(a, st));
-    HIPCHECK(launch_strain(G, 1, np, P, G.loc.p, 1, nullptr, G.Fspec.p, nullptr));
+    HIPCHECK(launch_fwd(G, np, G.vals_out.p, loc_f, G.Fspec.p));
     HIPCHECK(hipEventRecord(b, st));
     evs.push_back({a, b});
+    // backward: active located pairs in element order
+    HIPCHECK(hipMemsetAsync(G.pair_count.p, 0, 4, st));
+    hipLaunchKernelGGL(k_keys_pairs, dim3((unsigned)(((long long)np * R + 255) / 256)), dim3(256), 0, st,
+                       np * R, R, P, act, loc_b, G.keys_in.p, G.vals_in.p, G.pair_count.p);
+    HIPCHECK(hipGetLastError());
+    int npairs = 0;
+    HIPCHECK(hipMemcpyAsync(&npairs, G.pair_count.p, 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    if ((rc = sort_by_element(G, npairs, G.bwd.d.nelem))) return rc;
     HIPCHECK(hipEventCreate(&a));
     HIPCHECK(hipEventCreate(&b));
     HIPCHECK(hipEventRecord(a, st));
-    HIPCHECK(launch_strain(G, 0, np * R, P, G.loc.p + (size_t)np - 1, R, act, G.Bspec.p, nullptr));
+    HIPCHECK(launch_bwd(G, npairs, G.vals_out.p, loc_f, loc_b, G.phys.p + (size_t)w0 * P * G.nk, nullptr));
     HIPCHECK(hipEventRecord(b, st));
     evs.push_back({a, b});
-    hipLaunchKernelGGL(k_eval, dim3(np * R), dim3(kEvalThreads), 0, st, G.pc, R, np * R, P, G.Fspec.p,
-                       G.Bspec.p, act, G.recs.p, G.kerns.p, G.H.p, G.model_idx.p, G.model.p,
-                       G.S.model_ndepth, G.S.relative_kernel, G.nk,
-                       G.phys.p + (size_t)w0 * P * G.nk, nullptr);
-    HIPCHECK(hipGetLastError());
     G.stats.strain_launches += 2;
     return MCK_OK;
 }
@@ -1006,9 +1146,9 @@ int finish_task(State &G, const Task &T, double *values, double *variance, int32
     const int ne = T.nelems, nk = G.nk;
     hipStream_t st = G.stream;
     std::vector<int> draws(ne), steps(ne);
-    unsigned long long ev = 0, cnt[4] = {0, 0, 0, 0};
+    unsigned long long ev = 0, cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int err = 0;
-    HIPCHECK(hipMemcpyAsync(cnt, G.elem_count.p, 4 * 8, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(cnt, G.elem_count.p, 8 * 8, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipMemcpyAsync(values, G.integral.p, (size_t)ne * nk * 8, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipMemcpyAsync(variance, G.variance.p, (size_t)ne * nk * 8, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipMemcpyAsync(niterations, G.niter.p, (size_t)ne * nk * 4, hipMemcpyDeviceToHost, st));
@@ -1019,12 +1159,16 @@ int finish_task(State &G, const Task &T, double *values, double *variance, int32
     HIPCHECK(hipStreamSynchronize(st));
     G.stats.evaluations = (int64_t)ev;
     {
+        // algorithmic bytes: every distinct database element of a launch read once, the
+        // forward spectra written once (fwd) and read once per distinct point (bwd), the
+        // kernel values of every pair written once
         const double blk = 25.0 * G.pc.nt * sizeof(float);
-        const double spec = (double)G.pc.nd * G.pc.nomega * sizeof(float2);
+        const double spec = (double)G.pc.nd_f * G.pc.nomega * sizeof(float2);
+        const double kval = G.R ? (double)G.nk / G.R * sizeof(double) : 0.0;
         G.stats.fwd_locations = (int64_t)cnt[1];
         G.stats.bwd_locations = (int64_t)cnt[3];
         G.stats.fwd_bytes = (int64_t)(cnt[0] * G.pc.nvar_fwd * blk + cnt[1] * spec);
-        G.stats.bwd_bytes = (int64_t)(cnt[2] * G.pc.nvar_bwd * blk + cnt[3] * spec);
+        G.stats.bwd_bytes = (int64_t)(cnt[2] * G.pc.nvar_bwd * blk + cnt[4] * spec + cnt[3] * kval);
     }
     if (err & 1) return fail(MCK_ENOTFOUND, "element not found: a point is outside the 12 nearest elements");
     if (err & 2) return fail(MCK_ENAN, "NaN in a Monte Carlo integral");
@@ -1188,26 +1332,11 @@ int mck_load_strain_points(int32_t frame, int32_t npts, const double *xyz, float
     float *d_tr = nullptr;
     const size_t n = (size_t)npts * G.pc.nd * G.pc.nt;
     HIPCHECK(hipMalloc((void **)&d_tr, n * sizeof(float)));
-    // fwd: pairs = points with frame 0; bwd: pair = p * R + r with a per-point location
-    hipError_t e;
-    if (frame == 0) {
-        e = launch_strain(G, 1, npts, 1, G.loc.p, 1, nullptr, nullptr, d_tr);
-    } else {
-        // R = 1 view: pair p -> frame index 1; locations at loc[p] (stride 1, frame offset -1)
-        int R_save = G.R;
-        G.R = 1;
-        DevFrame fsave;
-        HIPCHECK(hipMemcpy(&fsave, G.frames.p + 1, sizeof(DevFrame), hipMemcpyDeviceToHost));
-        HIPCHECK(hipMemcpy(G.frames.p + 1, &G.frames_host[frame], sizeof(DevFrame), hipMemcpyHostToDevice));
-        e = launch_strain(G, 0, npts, 1, G.loc.p - 1, 1, nullptr, nullptr, d_tr);
-        G.R = R_save;
-        (void)hipStreamSynchronize(G.stream);
-        HIPCHECK(hipMemcpy(G.frames.p + 1, &fsave, sizeof(DevFrame), hipMemcpyHostToDevice));
-    }
-    if (e != hipSuccess) { (void)hipFree(d_tr); return fail(MCK_EHIP, hipGetErrorString(e)); }
-    HIPCHECK(hipMemcpyAsync(traces, d_tr, n * sizeof(float), hipMemcpyDeviceToHost, G.stream));
-    HIPCHECK(hipStreamSynchronize(G.stream));
+    hipError_t e = launch_traces(G, frame == 0, frame, npts, G.loc.p, d_tr);
+    if (e == hipSuccess) e = hipMemcpyAsync(traces, d_tr, n * sizeof(float), hipMemcpyDeviceToHost, G.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(G.stream);
     (void)hipFree(d_tr);
+    if (e != hipSuccess) return fail(MCK_EHIP, hipGetErrorString(e));
     int err = 0;
     HIPCHECK(hipMemcpy(&err, G.err.p, 4, hipMemcpyDeviceToHost));
     if (err & 1) return fail(MCK_ENOTFOUND, "element not found");
@@ -1231,25 +1360,24 @@ int mck_base_kernels(int32_t npts, const double *xyz, double *base) {
         midx[p] = id - 1;
     }
     HIPCHECK(hipMemcpyAsync(G.model_idx.p, midx.data(), npts * 4, hipMemcpyHostToDevice, G.stream));
+    Located *loc_f = G.loc.p, *loc_b = G.loc.p + npts;
     hipLaunchKernelGGL(k_locate, dim3((npts + 127) / 128), dim3(128), 0, G.stream, G.fwd.d, G.frames.p, 0, 1,
-                       npts, G.xyz.p, G.S.planet_radius, nullptr, 1, R, G.loc.p, G.err.p);
+                       npts, G.xyz.p, G.S.planet_radius, nullptr, 1, R, loc_f, G.err.p);
     hipLaunchKernelGGL(k_locate, dim3(((long long)npts * R + 127) / 128), dim3(128), 0, G.stream, G.bwd.d,
-                       G.frames.p, 1, R, npts, G.xyz.p, G.S.planet_radius, nullptr, 1, R, G.loc.p + npts, G.err.p);
+                       G.frames.p, 1, R, npts, G.xyz.p, G.S.planet_radius, nullptr, 1, R, loc_b, G.err.p);
     HIPCHECK(hipGetLastError());
-    HIPCHECK(launch_strain(G, 1, npts, 1, G.loc.p, 1, nullptr, G.Fspec.p, nullptr));
-    HIPCHECK(launch_strain(G, 0, npts * R, 1, G.loc.p + npts - 1, R, nullptr, G.Bspec.p, nullptr));
     double *d_base = nullptr, *d_phys = nullptr;
     HIPCHECK(hipMalloc((void **)&d_base, (size_t)npts * G.nk * 6 * 8));
     HIPCHECK(hipMalloc((void **)&d_phys, (size_t)npts * G.nk * 8));
-    HIPCHECK(hipMemsetAsync(d_base, 0, (size_t)npts * G.nk * 6 * 8, G.stream));
-    hipLaunchKernelGGL(k_eval, dim3(npts * R), dim3(kEvalThreads), 0, G.stream, G.pc, R, npts * R, 1, G.Fspec.p,
-                       G.Bspec.p, nullptr, G.recs.p, G.kerns.p, G.H.p, G.model_idx.p, G.model.p,
-                       G.S.model_ndepth, G.S.relative_kernel, G.nk, d_phys, d_base);
-    HIPCHECK(hipGetLastError());
-    HIPCHECK(hipMemcpyAsync(base, d_base, (size_t)npts * G.nk * 6 * 8, hipMemcpyDeviceToHost, G.stream));
-    HIPCHECK(hipStreamSynchronize(G.stream));
+    hipError_t e = hipMemsetAsync(d_base, 0, (size_t)npts * G.nk * 6 * 8, G.stream);
+    if (e == hipSuccess) e = launch_fwd(G, npts, nullptr, loc_f, G.Fspec.p);
+    if (e == hipSuccess) e = launch_bwd(G, npts * R, nullptr, loc_f, loc_b, d_phys, d_base);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(base, d_base, (size_t)npts * G.nk * 6 * 8, hipMemcpyDeviceToHost, G.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(G.stream);
     (void)hipFree(d_base);
     (void)hipFree(d_phys);
+    if (e != hipSuccess) return fail(MCK_EHIP, hipGetErrorString(e));
     int err = 0;
     HIPCHECK(hipMemcpy(&err, G.err.p, 4, hipMemcpyDeviceToHost));
     if (err & 1) return fail(MCK_ENOTFOUND, "element not found");
@@ -1268,10 +1396,10 @@ void mck_finalize(void) {
     for (DbDev *D : {&G.fwd, &G.bwd}) {
         if (D->u_owned) (void)hipFree(D->u_owned);
         D->u_owned = nullptr;
-        D->mp_s.release(); D->mp_z.release(); D->econ.release(); D->eltype.release();
+        D->mp_s.release(); D->mp_z.release(); D->econ.release(); D->ejac.release(); D->eltype.release();
         D->axis.release(); D->cell_start.release(); D->cell_elems.release();
     }
-    G.frames.release(); G.tw.release(); G.tw2.release(); G.H.release(); G.recs.release();
+    G.frames.release(); G.twq.release(); G.H.release(); G.recs.release();
     G.kerns.release(); G.model.release(); G.d_k_rec.release(); G.d_k_nbk.release();
     G.d_rec_first.release(); G.d_rec_count.release(); G.tet.release(); G.volume.release();
     G.allowed_var.release(); G.hbase.release(); G.masked.release(); G.live.release();
@@ -1279,8 +1407,9 @@ void mck_finalize(void) {
     G.fsum.release(); G.f2sum.release(); G.integral.release(); G.variance.release();
     G.conv.release(); G.done.release(); G.niter.release(); G.draws.release(); G.steps.release();
     G.nmodels.release(); G.xyz.release(); G.model_idx.release(); G.loc.release();
-    G.Fspec.release(); G.Bspec.release(); G.phys.release(); G.evals.release(); G.err.release();
-    G.elem_mark.release(); G.elem_count.release();
+    G.Fspec.release(); G.phys.release(); G.evals.release(); G.err.release();
+    G.elem_mark.release(); G.elem_count.release(); G.keys_in.release(); G.keys_out.release();
+    G.vals_in.release(); G.vals_out.release(); G.pair_count.release(); G.sort_tmp.release();
     G.init = false;
 }
 

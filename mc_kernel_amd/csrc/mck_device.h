@@ -26,6 +26,7 @@ struct DevDb {
     const float *u;           // [nelem][nvar][5][5][nt]
     const float *mp_s, *mp_z; // element midpoints
     const double *econ;       // [nelem][16] element constants (see host setup)
+    const double *ejac;       // [nelem][25][5] inverse Jacobian and 1/s at the nodes
     const int *eltype, *axis;
     // uniform grid over the midpoints for the exact 12-nearest search
     int gnx, gnz;
@@ -67,6 +68,15 @@ struct PathConf {
     double bwd_fi[3];
     float taper[10];          // Gaussian taper samples (fft.f90:586-593), fp32
     int ntaper;
+    // strain components carried through the FFT, per frame: the trace (nd 1), or the
+    // source-frame Voigt components with a non-zero azimuthal factor (4 for monopole
+    // sources ss, pp, zz, sz; 6 otherwise), rotated to the earth frame only in the
+    // frequency-domain contraction
+    int nd_f, nd_b;
+    int comps_f[6], comps_b[6];
+    int need_bk;              // bit b set: base kernel b is used by some kernel
+    int bk_lds[6];            // LDS array of base kernel b in the contraction (-1 = unused)
+    int nbk_lds;
 };
 
 // LDS layout of the per-point operator set-up
@@ -76,10 +86,24 @@ struct OpShared {
     double li[5], lj[5];
     double M[6][kMaxSim][6];// output comp x simulation x strain comp
     float W[6 * kMaxVar * 25];
+    float Q[6][6];          // earth Voigt component x carried component (source mode)
     int elem;
     int axial;
     int type;
 };
+
+// Per-pair coefficients of the frequency-domain contraction (backward kernel):
+// earth-frame strain = Q x carried components, for the fwd (Qf) and bwd (Qb) fields.
+struct EvalShared {
+    float Qf[6][6];
+    float Amu[6][6];        // Qf^T Qb: F.B = f^T Amu b
+    float uf[6], ub[6];     // trace rows (1,1,1,0,0,0) Q
+    double slotval[kMaxSlots];
+    double red[4][kMaxSlots];
+};
+
+constexpr int kSlotGroup = 6;        // slots sharing one window spectrum, summed together
+
 
 // receiver: distinct (window spectrum, base kernel) pairs ('slots') and its kernels
 struct RecDesc {
@@ -87,6 +111,10 @@ struct RecDesc {
     int first_kernel, nkernel;
     int slot_h[kMaxSlots];
     int slot_bk[kMaxSlots];    // 0 lambda, 1 mu, 3 a, 4 b, 5 c
+    int slot_lds[kMaxSlots];   // LDS array of the slot's base kernel in the contraction
+    // slots ordered by window spectrum, in groups sharing one spectrum (<= kSlotGroup)
+    int ngroup;
+    int grp_h[kMaxSlots], grp_first[kMaxSlots], grp_n[kMaxSlots];
 };
 
 // kernel: model parameter and the slot of every base kernel it needs (-1 = none)

@@ -8,14 +8,17 @@
 //                 midpoints with single-precision distances (kdtree2.f90:1541-1552) and
 //                 the Newton inverse mappings (finite_elem_mapping.f90) - bit-identical
 //                 element indices
-//  k_strain_fft   fused gather + strain + interpolation + azimuth + rotation
-//                 (sem_derivatives.f90, spectral_basis.f90:181-314, rotations.f90,
-//                 readfields.f90:1138-1465), cumsum (simple_routines.f90:314-341),
-//                 taper + zero pad (fft.f90:546-601) and LDS Stockham real FFT
-//                 (replacing FFTW in fft.f90:375-413)
-//  k_eval         fwd x bwd base kernels (kernel.f90:514-570) and the filtered Parseval
-//                 window integral (kernel.f90:410-510, 946-1020) in the frequency domain
-//                 with FP64 accumulation; physical kernels (kernel.f90:665-832)
+//  k_fwd_spectra  fused gather + strain + interpolation + azimuthal factors
+//                 (sem_derivatives.f90, spectral_basis.f90:181-314, readfields.f90:
+//                 1138-1465), cumsum (simple_routines.f90:314-341), taper + zero pad
+//                 (fft.f90:546-601) and LDS Stockham real FFT (replacing FFTW in
+//                 fft.f90:375-413) of the forward field, points in element order
+//  k_bwd_eval     the same for the backward field of every active (point, receiver) pair,
+//                 fused with the fwd x bwd base kernels (kernel.f90:514-570), the filtered
+//                 Parseval window integral (kernel.f90:410-510, 946-1020) in the frequency
+//                 domain with FP64 accumulation and the physical kernels (kernel.f90:
+//                 665-832)
+//  k_strain_traces  earth-frame strain time series of one frame (point-level API)
 //  k_mc           per-voxel MC sums, variance and convergence (mc_integration.f90:69-143)
 //                 and the iteration bookkeeping of slave_work (slave.f90:501-757)
 // Included once by mck_api.hip (single translation unit).
@@ -371,30 +374,48 @@ __device__ inline double azim_factor(double phi, const double *mij, int isim, in
 }
 
 
-// Builds the per-point weights W[c][v][jpol*5+ipol] such that the frame's output
-// component c at this point is sum_v sum_node W * u_v(node, t).
+// Per-element table of the inverse Jacobian (4) and 1/s at the 25 collocation points
+// (finite_elem_mapping.f90), [nelem][25][5]: computed once per database.
+__global__ void k_elem_tables(DevDb db, double *out)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= db.nelem * 25) return;
+    const int e = g / 25, node = g - (g / 25) * 25;
+    const int i = node % 5, j = node / 5;
+    const double *E = db.econ + (size_t)e * 16;
+    const double *xp = db.axis[e] ? c_basis.glj : c_basis.gll;
+    double ij[4], sz[2];
+    d_inv_jacobian(xp[i], c_basis.gll[j], E, db.eltype[e], ij);
+    d_mapping(xp[i], c_basis.gll[j], E, db.eltype[e], sz);
+    double *o = out + (size_t)g * 5;
+    o[0] = ij[0]; o[1] = ij[1]; o[2] = ij[2]; o[3] = ij[3];
+    o[4] = 1.0 / sz[0];
+}
+
+// Builds the per-point weights W[c][v][jpol*5+ipol] such that carried component c at
+// this point is sum_v sum_node W * u_v(node, t).
+//  earth mode (src_mode false): c are the earth-frame Voigt components (or the trace);
+//  source mode: c are the source-frame components comps[c] of the frame, with the
+//  azimuthal factors, the sign flip of components 4 and 6 and 1/amplitude applied;
+//  S.Q[i][c] is then the rotation of component c into earth-frame component i.
 __device__ void build_weights(OpShared &S, const DevDb &db, const PathConf &pc, const DevFrame &fr,
-                              const Located &L, bool fwd, int nvar, const int *vars)
+                              const Located &L, bool fwd, int nvar, const int *vars, int nd,
+                              const int *comps, bool src_mode)
 {
     const int tid = threadIdx.x;
     const int e = L.elem;
-    const double *E = db.econ + (size_t)e * 16;
     if (tid == 0) { S.elem = e; S.axial = db.axis[e]; S.type = db.eltype[e]; }
-    __syncthreads();
-    const bool axial = S.axial != 0;
-    const double *xp = axial ? c_basis.glj : c_basis.gll;
-    if (tid < 25) {
-        int i = tid % 5, j = tid / 5;
-        double ij[4], sz[2];
-        d_inv_jacobian(xp[i], c_basis.gll[j], E, S.type, ij);
-        d_mapping(xp[i], c_basis.gll[j], E, S.type, sz);
-        S.invj[tid][0] = ij[0]; S.invj[tid][1] = ij[1]; S.invj[tid][2] = ij[2]; S.invj[tid][3] = ij[3];
-        S.sinv[tid] = 1.0 / sz[0];
-    } else if (tid == 32 || tid == 33) {
+    if (tid < 125) {
+        // inverse Jacobian and 1/s at the 25 nodes (element table, see k_elem_tables)
+        const double v = db.ejac[(size_t)e * 125 + tid];
+        const int n = tid / 5, c = tid - (tid / 5) * 5;
+        if (c < 4) S.invj[n][c] = v; else S.sinv[n] = v;
+    } else if (tid == 160 || tid == 161) {
         // Lagrange basis at (xi, eta) (spectral_basis.f90:203-217)
-        const double *pts = tid == 32 ? xp : c_basis.gll;
-        double x = tid == 32 ? L.xi : L.eta;
-        double *l = tid == 32 ? S.li : S.lj;
+        const double *xp = db.axis[e] ? c_basis.glj : c_basis.gll;
+        const double *pts = tid == 160 ? xp : c_basis.gll;
+        double x = tid == 160 ? L.xi : L.eta;
+        double *l = tid == 160 ? S.li : S.lj;
         for (int i = 0; i < 5; ++i) {
             double v = 1.0;
             for (int m = 0; m < 5; ++m) {
@@ -403,11 +424,21 @@ __device__ void build_weights(OpShared &S, const DevDb &db, const PathConf &pc, 
             }
             l[i] = v;
         }
-    } else if (tid == 64) {
+    } else if (tid == 192) {
         // output transform: sign flip of comps 4, 6 (readfields.f90:2180-2184), azimuthal
         // factors, 1/amplitude, src -> xyz -> earth rotations (full strain only)
         double amp = db.amplitude;
         int nsim = fwd ? pc.nsim_fwd : 1;
+        double sp = sin(L.phi), cp = cos(L.phi);
+        if (pc.full && src_mode) {
+            for (int c = 0; c < nd; ++c) {
+                double t[6] = {0, 0, 0, 0, 0, 0};
+                t[comps[c]] = 1.0;
+                rot_src_to_xyz(t, sp, cp);
+                rot_xyz_to_earth(t, fr.sin_lon, fr.cos_lon, fr.sin_colat, fr.cos_colat);
+                for (int i = 0; i < 6; ++i) S.Q[i][c] = (float)t[i];
+            }
+        }
         for (int si = 0; si < nsim; ++si) {
             double az[6];
             if (fwd) {
@@ -423,8 +454,12 @@ __device__ void build_weights(OpShared &S, const DevDb &db, const PathConf &pc, 
             }
             if (!pc.full) {
                 for (int m = 0; m < 6; ++m) S.M[0][si][m] = (m == 0) ? az[0] / amp : 0.0;
+            } else if (src_mode) {
+                for (int c = 0; c < 6; ++c)
+                    for (int m = 0; m < 6; ++m)
+                        S.M[c][si][m] = (c < nd && m == comps[c])
+                                      ? ((m == 3 || m == 5) ? -1.0 : 1.0) * az[m] / amp : 0.0;
             } else {
-                double sp = sin(L.phi), cp = cos(L.phi);
                 for (int m = 0; m < 6; ++m) {
                     double t[6] = {0, 0, 0, 0, 0, 0};
                     t[m] = 1.0;
@@ -437,7 +472,7 @@ __device__ void build_weights(OpShared &S, const DevDb &db, const PathConf &pc, 
         }
     }
     __syncthreads();
-    const int nd = pc.nd;
+    const bool axial = S.axial != 0;
     const int total = nd * nvar * 25;
     const double *Gx = axial ? c_basis.G1 : c_basis.G2;
     for (int idx = tid; idx < total; idx += blockDim.x) {
@@ -496,210 +531,298 @@ __device__ void build_weights(OpShared &S, const DevDb &db, const PathConf &pc, 
     __syncthreads();
 }
 
-// ------------------------------------------------------------------------------------
-// k_strain_fft: one workgroup per (point, frame) pair.
-//  fwd: pairs are points; bwd: pairs are (point, receiver) = p * R + r.
-//  out: [pair][nd][nomega] float2 (spectrum times dt, FFTW r2c convention)
-// ------------------------------------------------------------------------------------
-template <int SPT>
-__global__ __launch_bounds__(kStrainThreads) void k_strain_fft(
-    DevDb db, PathConf pc, const DevFrame *frames, int fwd, int npairs, int R, int P,
-    const Located *loc, int loc_stride, const uint8_t *active, const float2 *tw,
-    const float2 *tw2, float2 *out, float *trace_out)
+// Workgroups are dispatched round-robin over the 8 XCDs; block b of a grid padded to a
+// multiple of 8 takes logical item (b % 8) * (grid / 8) + b / 8, so each XCD walks one
+// contiguous range of the element-sorted work and neighbours share its L2.
+__device__ inline int xcd_block(int n)
 {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    OpShared &S = *reinterpret_cast<OpShared *>(smem_raw);
-    float2 *bufA = reinterpret_cast<float2 *>(smem_raw + ((sizeof(OpShared) + 15) / 16) * 16);
-    float2 *bufB = bufA + pc.M;
-    __shared__ float wave_tot[kStrainThreads / 64];
-    __shared__ float edge_x[kStrainThreads + 1];
+    const int b = blockIdx.x, per = gridDim.x >> 3;
+    const int lb = (b & 7) * per + (b >> 3);
+    return lb < n ? lb : -1;
+}
 
-    const int pair = blockIdx.x;
-    if (pair >= npairs) return;
-    int p, f;
-    if (fwd) { p = pair; f = 0; }
-    else { p = pair / R; f = 1 + (pair - p * R); }
-    if (!fwd && active && !active[(p / P) * R + (f - 1)]) return;
-    const Located L = loc[(size_t)p * loc_stride + (fwd ? 0 : f)];
-    if (L.elem < 0) return;
-    const int nvar = fwd ? pc.nvar_fwd : pc.nvar_bwd;
-    const int *vars = fwd ? pc.fwd_vars : pc.bwd_vars;
-    build_weights(S, db, pc, frames[f], L, fwd != 0, nvar, vars);
-
-    const int tid = threadIdx.x;
-    const int nt = pc.nt, nd = pc.nd, M = pc.M;
-    const int t0 = tid * SPT;
-    float acc[6][SPT];
+// acc[c][q] = sum_v sum_node W[c][v][node] u_v(node, t0 + q): the strain (or trace) of
+// carried component c at this thread's SPT samples
+// loads rows n0..n0+NB-1 of one variable (this thread's samples)
+template <int SPT, int NB>
+__device__ inline void load_rows(const float *uv, int n0, int nt, int t0, bool full_vec, float (&x)[NB][SPT])
+{
 #pragma unroll
-    for (int c = 0; c < 6; ++c)
+    for (int b = 0; b < NB; ++b) {
+        const float *src = uv + (size_t)(n0 + b) * nt;
+        if (full_vec && (SPT % 4) == 0) {
 #pragma unroll
-        for (int q = 0; q < SPT; ++q) acc[c][q] = 0.0f;
-    const size_t blk = (size_t)25 * nt;
-    const float *ue = db.u + (size_t)S.elem * db.nvar * blk;
-    if (t0 < nt) {
-        const bool full_vec = (t0 + SPT <= nt) && ((nt & 3) == 0);
-        for (int v = 0; v < nvar; ++v) {
-            const float *uv = ue + (size_t)vars[v] * blk + t0;
-            for (int n = 0; n < 25; ++n) {
-                float x[SPT];
-                const float *src = uv + (size_t)n * nt;
-                if (full_vec && (SPT % 4) == 0) {
-#pragma unroll
-                    for (int q = 0; q < SPT; q += 4) {
-                        float4 t = *reinterpret_cast<const float4 *>(src + q);
-                        x[q] = t.x; x[q + 1] = t.y; x[q + 2] = t.z; x[q + 3] = t.w;
-                    }
-                } else {
-#pragma unroll
-                    for (int q = 0; q < SPT; ++q) x[q] = (t0 + q < nt) ? src[q] : 0.0f;
-                }
-                if (nd == 6) {
-#pragma unroll
-                    for (int c = 0; c < 6; ++c) {
-                        float w = S.W[(c * nvar + v) * 25 + n];
-#pragma unroll
-                        for (int q = 0; q < SPT; ++q) acc[c][q] = __builtin_fmaf(w, x[q], acc[c][q]);
-                    }
-                } else {
-                    float w = S.W[v * 25 + n];
-#pragma unroll
-                    for (int q = 0; q < SPT; ++q) acc[0][q] = __builtin_fmaf(w, x[q], acc[0][q]);
-                }
+            for (int q = 0; q < SPT; q += 4) {
+                float4 t = *reinterpret_cast<const float4 *>(src + q);
+                x[b][q] = t.x; x[b][q + 1] = t.y; x[b][q + 2] = t.z; x[b][q + 3] = t.w;
             }
+        } else {
+#pragma unroll
+            for (int q = 0; q < SPT; ++q) x[b][q] = (t0 + q < nt) ? src[q] : 0.0f;
         }
-    }
-
-    if (trace_out) {
-        // raw strain traces (before cumsum): [pair][nd][nt]
-#pragma unroll
-        for (int c = 0; c < 6; ++c)
-            if (c < nd)
-#pragma unroll
-                for (int q = 0; q < SPT; ++q)
-                    if (t0 + q < nt) trace_out[((size_t)pair * nd + c) * nt + t0 + q] = acc[c][q];
-        return;
-    }
-    const int lane = tid & 63, wave = tid >> 6;
-    for (int c = 0; c < nd; ++c) {
-        // component c of the accumulators, selected with static indices so that acc stays
-        // in registers
-        float x[SPT];
-#pragma unroll
-        for (int q = 0; q < SPT; ++q) x[q] = 0.0f;
-#pragma unroll
-        for (int cc = 0; cc < 6; ++cc)
-            if (cc == c)
-#pragma unroll
-                for (int q = 0; q < SPT; ++q) x[q] = acc[cc][q];
-#pragma unroll
-        for (int q = 0; q < SPT; ++q) if (t0 + q >= nt) x[q] = 0.0f;
-        if (fwd) {
-            // cumsum_trapezoidal: c0 = x0, c1 = c0 + x1, ct = c(t-1) + x(t-1) + xt
-            edge_x[tid + 1] = x[SPT - 1];
-            if (tid == 0) edge_x[0] = 0.0f;
-            __syncthreads();
-            float prev = edge_x[tid];
-            float d[SPT], run = 0.0f;
-#pragma unroll
-            for (int q = 0; q < SPT; ++q) {
-                int t = t0 + q;
-                float xm1 = q == 0 ? prev : x[q - 1];
-                d[q] = (t <= 1) ? x[q] : (xm1 + x[q]);
-                run += d[q];
-            }
-            // exclusive scan of the per-thread totals
-            float incl = run;
-            for (int o = 1; o < 64; o <<= 1) {
-                float y = __shfl_up(incl, o, 64);
-                if (lane >= o) incl += y;
-            }
-            if (lane == 63) wave_tot[wave] = incl;
-            __syncthreads();
-            float wbase = 0.0f;
-            for (int w2 = 0; w2 < wave; ++w2) wbase += wave_tot[w2];
-            float excl = wbase + incl - run;
-            float cs = excl;
-            const float h = (float)(pc.dt * 0.5);
-#pragma unroll
-            for (int q = 0; q < SPT; ++q) {
-                cs += d[q];
-                x[q] = (t0 + q < nt) ? cs * h : 0.0f;
-            }
-            __syncthreads();
-        }
-        // taper (10 samples at both ends) and packing into z[n] = x[2n] + i x[2n+1]
-#pragma unroll
-        for (int q = 0; q < SPT; ++q) {
-            int t = t0 + q;
-            if (t < pc.ntaper) x[q] *= pc.taper[t];
-            else if (t >= nt - pc.ntaper && t < nt) x[q] *= pc.taper[nt - 1 - t];
-        }
-        for (int n = tid; n < M; n += kStrainThreads) bufA[n] = make_float2(0.0f, 0.0f);
-        __syncthreads();
-        {
-            float *fa = reinterpret_cast<float *>(bufA);
-#pragma unroll
-            for (int q = 0; q < SPT; ++q)
-                if (t0 + q < nt) fa[t0 + q] = x[q];
-        }
-        __syncthreads();
-        // Stockham autosort FFT of length M, radix 4 (radix 2 first when log2 M is odd)
-        float2 *in = bufA, *ob = bufB;
-        for (int Ns = 1; Ns < M;) {
-            int Rr = ((M / Ns) % 4 == 0) ? 4 : 2;
-            int Mq = M / Rr;
-            int stride = M / (Ns * Rr);
-            for (int j = tid; j < Mq; j += kStrainThreads) {
-                int kk = j % Ns;
-                float2 v0 = in[j], v1 = in[j + Mq];
-                if (Rr == 2) {
-                    float2 w = tw[kk * stride];
-                    float2 t1 = make_float2(v1.x * w.x - v1.y * w.y, v1.x * w.y + v1.y * w.x);
-                    int o = (j / Ns) * Ns * 2 + kk;
-                    ob[o] = make_float2(v0.x + t1.x, v0.y + t1.y);
-                    ob[o + Ns] = make_float2(v0.x - t1.x, v0.y - t1.y);
-                } else {
-                    float2 v2 = in[j + 2 * Mq], v3 = in[j + 3 * Mq];
-                    float2 w1 = tw[kk * stride], w2 = tw[2 * kk * stride], w3 = tw[3 * kk * stride];
-                    float2 a1 = make_float2(v1.x * w1.x - v1.y * w1.y, v1.x * w1.y + v1.y * w1.x);
-                    float2 a2 = make_float2(v2.x * w2.x - v2.y * w2.y, v2.x * w2.y + v2.y * w2.x);
-                    float2 a3 = make_float2(v3.x * w3.x - v3.y * w3.y, v3.x * w3.y + v3.y * w3.x);
-                    float2 s02 = make_float2(v0.x + a2.x, v0.y + a2.y), d02 = make_float2(v0.x - a2.x, v0.y - a2.y);
-                    float2 s13 = make_float2(a1.x + a3.x, a1.y + a3.y), d13 = make_float2(a1.x - a3.x, a1.y - a3.y);
-                    int o = (j / Ns) * Ns * 4 + kk;
-                    ob[o] = make_float2(s02.x + s13.x, s02.y + s13.y);
-                    // y1 = d02 - i d13, y3 = d02 + i d13
-                    ob[o + Ns] = make_float2(d02.x + d13.y, d02.y - d13.x);
-                    ob[o + 2 * Ns] = make_float2(s02.x - s13.x, s02.y - s13.y);
-                    ob[o + 3 * Ns] = make_float2(d02.x - d13.y, d02.y + d13.x);
-                }
-            }
-            __syncthreads();
-            float2 *tmp = in; in = ob; ob = tmp;
-            Ns *= Rr;
-        }
-        // real-FFT post-processing: X_k = E_k + e^{-2 pi i k/N} O_k, k = 0..M; times dt
-        float2 *dst = out + ((size_t)pair * nd + c) * pc.nomega;
-        const float fdt = (float)pc.dt;
-        for (int k = tid; k <= M; k += kStrainThreads) {
-            float2 zk = in[k == M ? 0 : k];
-            float2 zm = in[(M - k) & (M - 1)];
-            float2 zc = make_float2(zm.x, -zm.y);
-            float2 E = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y + zc.y));
-            float2 dd = make_float2(zk.x - zc.x, zk.y - zc.y);
-            float2 O = make_float2(0.5f * dd.y, -0.5f * dd.x);
-            float2 w = tw2[k];
-            float2 X = make_float2(E.x + w.x * O.x - w.y * O.y, E.y + w.x * O.y + w.y * O.x);
-            dst[k] = make_float2(X.x * fdt, X.y * fdt);
-        }
-        __syncthreads();
     }
 }
 
-// ------------------------------------------------------------------------------------
-// k_eval: one workgroup per (point, receiver).  Slot s of receiver r is a pair
-// (H index, base kernel); value = sum_k Re(basekernel_k * H_k) in FP64.
-// ------------------------------------------------------------------------------------
+// FMAs of rows n0..n0+NB-1 of variable v into the carried components.  Monopole fields
+// (ND 4: ss, pp, zz, sz) come as (u_s, u_z) variable pairs; u_s feeds ss, pp and sz, u_z
+// feeds zz and sz - the other weights are exactly zero.
+template <int SPT, int ND, int NB>
+__device__ inline void fma_rows(const OpShared &S, int nvar, int v, int n0, const float (&x)[NB][SPT],
+                                float (&acc)[ND][SPT])
+{
+    if (ND == 4) {
+        if (v & 1) {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const float w2 = S.W[(2 * nvar + v) * 25 + n0 + b];
+                const float w3 = S.W[(3 * nvar + v) * 25 + n0 + b];
+#pragma unroll
+                for (int q = 0; q < SPT; ++q) {
+                    acc[2 % ND][q] = __builtin_fmaf(w2, x[b][q], acc[2 % ND][q]);
+                    acc[3 % ND][q] = __builtin_fmaf(w3, x[b][q], acc[3 % ND][q]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const float w0 = S.W[(0 * nvar + v) * 25 + n0 + b];
+                const float w1 = S.W[(1 * nvar + v) * 25 + n0 + b];
+                const float w3 = S.W[(3 * nvar + v) * 25 + n0 + b];
+#pragma unroll
+                for (int q = 0; q < SPT; ++q) {
+                    acc[0][q] = __builtin_fmaf(w0, x[b][q], acc[0][q]);
+                    acc[1 % ND][q] = __builtin_fmaf(w1, x[b][q], acc[1 % ND][q]);
+                    acc[3 % ND][q] = __builtin_fmaf(w3, x[b][q], acc[3 % ND][q]);
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int c = 0; c < ND; ++c) {
+                const float w = S.W[(c * nvar + v) * 25 + n0 + b];
+#pragma unroll
+                for (int q = 0; q < SPT; ++q) acc[c][q] = __builtin_fmaf(w, x[b][q], acc[c][q]);
+            }
+    }
+}
+
+// acc[c][q] = sum_v sum_node W[c][v][node] u_v(node, t0 + q): the strain (or trace) of
+// carried component c at this thread's SPT samples.  The 25 node rows of a variable are
+// read in 5 batches of 5; for the monopole components the loads of batch b + 1 are in
+// flight while batch b is summed.
+template <int SPT, int ND>
+__device__ inline void gather_strain(const OpShared &S, const float *ue, size_t blk, int nvar,
+                                     const int *vars, int t0, int nt, float (&acc)[ND][SPT])
+{
+#pragma unroll
+    for (int c = 0; c < ND; ++c)
+#pragma unroll
+        for (int q = 0; q < SPT; ++q) acc[c][q] = 0.0f;
+    if (t0 >= nt) return;
+    const bool full_vec = (t0 + SPT <= nt) && ((nt & 3) == 0);
+    constexpr int NB = 5;
+    if (ND != 4) {
+        // dense components: one batch in flight (register budget)
+        for (int v = 0; v < nvar; ++v) {
+            const float *uv = ue + (size_t)vars[v] * blk + t0;
+            for (int n0 = 0; n0 < 25; n0 += NB) {
+                float x[NB][SPT];
+                load_rows<SPT, NB>(uv, n0, nt, t0, full_vec, x);
+                fma_rows<SPT, ND, NB>(S, nvar, v, n0, x, acc);
+            }
+        }
+        return;
+    }
+    for (int v = 0; v < nvar; ++v) {
+        const float *uv = ue + (size_t)vars[v] * blk + t0;
+        float xa[NB][SPT], xb[NB][SPT];
+        load_rows<SPT, NB>(uv, 0, nt, t0, full_vec, xa);
+        load_rows<SPT, NB>(uv, 5, nt, t0, full_vec, xb);
+        fma_rows<SPT, ND, NB>(S, nvar, v, 0, xa, acc);
+        load_rows<SPT, NB>(uv, 10, nt, t0, full_vec, xa);
+        fma_rows<SPT, ND, NB>(S, nvar, v, 5, xb, acc);
+        load_rows<SPT, NB>(uv, 15, nt, t0, full_vec, xb);
+        fma_rows<SPT, ND, NB>(S, nvar, v, 10, xa, acc);
+        load_rows<SPT, NB>(uv, 20, nt, t0, full_vec, xa);
+        fma_rows<SPT, ND, NB>(S, nvar, v, 15, xb, acc);
+        fma_rows<SPT, ND, NB>(S, nvar, v, 20, xa, acc);
+    }
+}
+
+// component c of the accumulators, selected with static indices so that acc stays in
+// registers; samples beyond nt are zero
+template <int SPT, int ND>
+__device__ inline void select_comp(const float (&acc)[ND][SPT], int c, int t0, int nt, float (&x)[SPT])
+{
+#pragma unroll
+    for (int q = 0; q < SPT; ++q) x[q] = 0.0f;
+#pragma unroll
+    for (int cc = 0; cc < ND; ++cc)
+        if (cc == c)
+#pragma unroll
+            for (int q = 0; q < SPT; ++q) x[q] = acc[cc][q];
+#pragma unroll
+    for (int q = 0; q < SPT; ++q) if (t0 + q >= nt) x[q] = 0.0f;
+}
+
+// cumsum_trapezoidal (simple_routines.f90:314-341): c0 = x0, c1 = c0 + x1,
+// ct = c(t-1) + x(t-1) + xt, times dt/2; a block-wide scan of the per-thread totals
+template <int SPT>
+__device__ inline void cumsum_trapezoidal(float (&x)[SPT], int t0, int nt, float h, float *edge_x,
+                                          float *wave_tot)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    edge_x[tid + 1] = x[SPT - 1];
+    if (tid == 0) edge_x[0] = 0.0f;
+    __syncthreads();
+    float prev = edge_x[tid];
+    float d[SPT], run = 0.0f;
+#pragma unroll
+    for (int q = 0; q < SPT; ++q) {
+        int t = t0 + q;
+        float xm1 = q == 0 ? prev : x[q - 1];
+        d[q] = (t <= 1) ? x[q] : (xm1 + x[q]);
+        run += d[q];
+    }
+    float incl = run;
+    for (int o = 1; o < 64; o <<= 1) {
+        float y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) wave_tot[wave] = incl;
+    __syncthreads();
+    float wbase = 0.0f;
+    for (int w2 = 0; w2 < wave; ++w2) wbase += wave_tot[w2];
+    float cs = wbase + incl - run;
+#pragma unroll
+    for (int q = 0; q < SPT; ++q) {
+        cs += d[q];
+        x[q] = (t0 + q < nt) ? cs * h : 0.0f;
+    }
+    __syncthreads();
+}
+
+// taper factors of this thread's samples: ntaper samples at both ends of the nt-sample
+// trace (fft.f90:546-601), 1 elsewhere
+template <int SPT>
+__device__ inline void taper_factors(int t0, int nt, const PathConf &pc, float (&tap)[SPT])
+{
+#pragma unroll
+    for (int q = 0; q < SPT; ++q) {
+        const int t = t0 + q;
+        tap[q] = 1.0f;
+        if (t < pc.ntaper) tap[q] = pc.taper[t];
+        else if (t >= nt - pc.ntaper && t < nt) tap[q] = pc.taper[nt - 1 - t];
+    }
+}
+
+// w_N^i = exp(-2 pi i i / N) from the quarter table T[r] = w_N^r, r < N/4 (in LDS):
+// w_N^(q N/4 + r) = (-i)^q w_N^r
+__device__ inline float2 twiddle(const float2 *T, int i, int lq)
+{
+    const int q = (i >> lq) & 3;
+    const float2 w = T[i & ((1 << lq) - 1)];
+    return q == 0 ? w : q == 1 ? make_float2(w.y, -w.x) : q == 2 ? make_float2(-w.x, -w.y) : make_float2(-w.y, w.x);
+}
+
+// copies the quarter table (N/4 = M/2 entries) into LDS; the caller's barrier publishes it
+__device__ inline void load_twiddles(const float2 *twq, int M, float2 *Tq)
+{
+    for (int r = threadIdx.x; r < (M >> 1); r += kStrainThreads) Tq[r] = twq[r];
+}
+
+// Packs the zero-padded real sequence x (length 2M) as z[n] = x[2n] + i x[2n+1] and runs
+// the length-M Stockham autosort FFT in LDS (radix 4, after one radix-2 pass when log2 M
+// is odd).  z[n] vanishes for n >= M/2 (nt <= M), so the radix-2 pass reduces to
+// out[2n] = out[2n+1] = z[n] and is fused with the packing.  Returns the buffer holding
+// Z; ends with a barrier.
+template <int SPT>
+__device__ inline const float2 *lds_fft(const float (&x)[SPT], int t0, int nt, int M, float2 *bufA,
+                                        float2 *bufB, const float2 *Tq)
+{
+    const int tid = threadIdx.x;
+    const int lM = 31 - __clz(M);
+    float2 *in = bufA, *ob = bufB;
+    int Ns = 1;
+    if (lM & 1) {
+        // thread holds samples t0..t0+SPT-1 = z[t0/2 .. (t0+SPT)/2 - 1]
+        if (SPT >= 2) {
+#pragma unroll
+            for (int q = 0; q < SPT; q += 2) {
+                const int n = (t0 + q) >> 1;
+                if (n < (M >> 1)) {
+                    const float2 z = make_float2(t0 + q < nt ? x[q] : 0.0f, t0 + q + 1 < nt ? x[q + 1] : 0.0f);
+                    reinterpret_cast<float4 *>(in)[n] = make_float4(z.x, z.y, z.x, z.y);
+                }
+            }
+        } else {
+            // one sample per thread: pair up through LDS as floats
+            float *fa = reinterpret_cast<float *>(ob);
+            for (int n = tid; n < M; n += kStrainThreads) ob[n] = make_float2(0.0f, 0.0f);
+            __syncthreads();
+            if (t0 < nt) fa[t0] = x[0];
+            __syncthreads();
+            for (int n = tid; n < (M >> 1); n += kStrainThreads) {
+                const float2 z = ob[n];
+                in[2 * n] = z;
+                in[2 * n + 1] = z;
+            }
+        }
+        Ns = 2;
+    } else {
+        for (int n = tid; n < M; n += kStrainThreads) in[n] = make_float2(0.0f, 0.0f);
+        __syncthreads();
+        float *fa = reinterpret_cast<float *>(in);
+#pragma unroll
+        for (int q = 0; q < SPT; ++q)
+            if (t0 + q < nt) fa[t0 + q] = x[q];
+    }
+    __syncthreads();
+    const int Mq = M >> 2;
+    for (int lNs = Ns == 1 ? 0 : 1; Ns < M; Ns <<= 2, lNs += 2) {
+        const int lstride = lM - lNs - 2;            // M / (4 Ns)
+#pragma unroll 1
+        for (int j = tid; j < Mq; j += kStrainThreads) {
+            const int kk = j & (Ns - 1);
+            const float2 v0 = in[j], v1 = in[j + Mq], v2 = in[j + 2 * Mq], v3 = in[j + 3 * Mq];
+            // w_M^t = w_N^(2t), N = 2M
+            const int ti = kk << (lstride + 1);
+            const float2 w1 = twiddle(Tq, ti, lM - 1), w2 = twiddle(Tq, 2 * ti, lM - 1),
+                         w3 = twiddle(Tq, 3 * ti, lM - 1);
+            const float2 a1 = make_float2(v1.x * w1.x - v1.y * w1.y, v1.x * w1.y + v1.y * w1.x);
+            const float2 a2 = make_float2(v2.x * w2.x - v2.y * w2.y, v2.x * w2.y + v2.y * w2.x);
+            const float2 a3 = make_float2(v3.x * w3.x - v3.y * w3.y, v3.x * w3.y + v3.y * w3.x);
+            const float2 s02 = make_float2(v0.x + a2.x, v0.y + a2.y), d02 = make_float2(v0.x - a2.x, v0.y - a2.y);
+            const float2 s13 = make_float2(a1.x + a3.x, a1.y + a3.y), d13 = make_float2(a1.x - a3.x, a1.y - a3.y);
+            const int o = ((j >> lNs) << (lNs + 2)) + kk;
+            ob[o] = make_float2(s02.x + s13.x, s02.y + s13.y);
+            // y1 = d02 - i d13, y3 = d02 + i d13
+            ob[o + Ns] = make_float2(d02.x + d13.y, d02.y - d13.x);
+            ob[o + 2 * Ns] = make_float2(s02.x - s13.x, s02.y - s13.y);
+            ob[o + 3 * Ns] = make_float2(d02.x - d13.y, d02.y + d13.x);
+        }
+        __syncthreads();
+        float2 *tmp = in; in = ob; ob = tmp;
+    }
+    return in;
+}
+
+// real-FFT post-processing: X_k = E_k + e^{-2 pi i k/N} O_k (k = 0..M), times dt (the
+// FFTW r2c convention of fft.f90:375-413 scaled as rfft_type%rfft does)
+__device__ inline float2 rfft_bin(const float2 *Z, int k, int M, const float2 *Tq, float dt)
+{
+    float2 zk = Z[k == M ? 0 : k];
+    float2 zm = Z[(M - k) & (M - 1)];
+    float2 zc = make_float2(zm.x, -zm.y);
+    float2 E = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y + zc.y));
+    float2 dd = make_float2(zk.x - zc.x, zk.y - zc.y);
+    float2 O = make_float2(0.5f * dd.y, -0.5f * dd.x);
+    float2 w = twiddle(Tq, k, 30 - __clz(M));     // w_N^k, log2(N/4) = log2(M) - 1
+    float2 X = make_float2(E.x + w.x * O.x - w.y * O.y, E.y + w.x * O.y + w.y * O.x);
+    return make_float2(X.x * dt, X.y * dt);
+}
 
 __device__ inline float2 cmulf(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 
@@ -709,86 +832,351 @@ __device__ inline double wave_sum(double v)
     return v;
 }
 
-__global__ __launch_bounds__(kEvalThreads) void k_eval(
-    PathConf pc, int R, int npairs, int P, const float2 *F, const float2 *B, const uint8_t *active,
-    const RecDesc *recs, const KernDesc *kern, const float2 *H, const int *model_idx,
-    const float *model /* [6][ndepth] */, int ndepth, int relative, int nk, double *phys,
-    double *base_out)
+// one DPP step on both halves of a double; rows outside ROWMASK receive +0
+template <int CTRL, int ROWMASK>
+__device__ inline double dpp_d(double v)
 {
-    const int pair = blockIdx.x;
-    if (pair >= npairs) return;
-    const int p = pair / R, r = pair - (pair / R) * R;
-    if (active && !active[(p / P) * R + r]) return;
-    const RecDesc &rd = recs[r];
-    const int tid = threadIdx.x, nd = pc.nd, nw = pc.nomega;
-    const float2 *Fp = F + (size_t)p * nd * nw;
-    const float2 *Bp = B + (size_t)pair * nd * nw;
-    __shared__ double red[kEvalThreads / 64][16];
-    __shared__ double slotval[kMaxSlots];
-    for (int s0 = 0; s0 < rd.nslot; s0 += 16) {
-        const int ns = min(16, rd.nslot - s0);
-        double acc[16];
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWMASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWMASK, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// wave64 sum through DPP moves (quad permutes, row mirrors, row broadcasts); the total
+// is exact in lane 63 only
+__device__ inline double wave_sum_lane63(double v)
+{
+    v += dpp_d<0xB1, 0xF>(v);    // quad_perm [1,0,3,2]
+    v += dpp_d<0x4E, 0xF>(v);    // quad_perm [2,3,0,1]
+    v += dpp_d<0x141, 0xF>(v);   // row_half_mirror
+    v += dpp_d<0x140, 0xF>(v);   // row_mirror: every lane holds its row's sum
+    v += dpp_d<0x142, 0xA>(v);   // row_bcast:15 into rows 1 and 3
+    v += dpp_d<0x143, 0xC>(v);   // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
+// ------------------------------------------------------------------------------------
+// k_strain_traces: one workgroup per point; earth-frame strain (or trace) time series
+// [point][nd][nt] of one frame (load_fw_points / load_bw_points, readfields.f90:689-1063).
+// ------------------------------------------------------------------------------------
+template <int SPT>
+__global__ __launch_bounds__(kStrainThreads) void k_strain_traces(
+    DevDb db, PathConf pc, const DevFrame *frames, int fwd, int frame, int npts, const Located *loc,
+    float *trace_out)
+{
+    __shared__ OpShared S;
+    const int p = blockIdx.x;
+    if (p >= npts) return;
+    const Located L = loc[p];
+    if (L.elem < 0) return;
+    const int nvar = fwd ? pc.nvar_fwd : pc.nvar_bwd;
+    const int *vars = fwd ? pc.fwd_vars : pc.bwd_vars;
+    build_weights(S, db, pc, frames[frame], L, fwd != 0, nvar, vars, pc.nd, nullptr, false);
+    const int nt = pc.nt, nd = pc.nd, t0 = threadIdx.x * SPT;
+    const size_t blk = (size_t)25 * nt;
+    float acc[6][SPT];
+    gather_strain<SPT, 6>(S, db.u + (size_t)S.elem * db.nvar * blk, blk, nvar, vars, t0, nt, acc);
 #pragma unroll
-        for (int s = 0; s < 16; ++s) acc[s] = 0.0;
-        for (int k = tid; k < nw; k += kEvalThreads) {
-            float2 bk[6];
-            if (nd == 1) {
-                float2 f0 = Fp[k], b0 = Bp[k];
-                float2 pr = cmulf(f0, b0);
-                bk[0] = make_float2(-pr.x, -pr.y);
-                bk[1] = make_float2(-2.0f * pr.x, -2.0f * pr.y);
-                bk[3] = bk[4] = bk[5] = make_float2(0.f, 0.f);
-            } else {
-                float2 f[6], b[6];
+    for (int c = 0; c < 6; ++c)
+        if (c < nd)
 #pragma unroll
-                for (int c = 0; c < 6; ++c) { f[c] = Fp[(size_t)c * nw + k]; b[c] = Bp[(size_t)c * nw + k]; }
-                float2 fs = make_float2(f[0].x + f[1].x + f[2].x, f[0].y + f[1].y + f[2].y);
-                float2 bs = make_float2(b[0].x + b[1].x + b[2].x, b[0].y + b[1].y + b[2].y);
-                float2 l = cmulf(fs, bs);
-                bk[0] = make_float2(-l.x, -l.y);
-                float2 m = make_float2(0.f, 0.f);
+            for (int q = 0; q < SPT; ++q)
+                if (t0 + q < nt) trace_out[((size_t)p * nd + c) * nt + t0 + q] = acc[c][q];
+}
+
+// ------------------------------------------------------------------------------------
+// k_fwd_spectra: one workgroup per point of the forward frame, points in element order.
+// Fused gather + strain + interpolation + azimuthal factors (sem_derivatives.f90,
+// spectral_basis.f90:181-314, readfields.f90:1138-1465), cumsum (simple_routines.f90:
+// 314-341), taper + zero pad (fft.f90:546-601) and LDS real FFT (fft.f90:375-413).
+// F: [point][ND][nomega] spectra of the carried source-frame components.
+// ------------------------------------------------------------------------------------
+template <int SPT, int ND>
+__global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && ND <= 4) ? 3 : (SPT <= 8 ? 2 : 1)) void k_fwd_spectra(
+    DevDb db, PathConf pc, const DevFrame *frames, int npts, const int *order, const Located *loc,
+    const float2 *twq, float2 *F)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    OpShared &S = *reinterpret_cast<OpShared *>(smem_raw);
+    float2 *bufA = reinterpret_cast<float2 *>(smem_raw + ((sizeof(OpShared) + 15) / 16) * 16);
+    float2 *bufB = bufA + pc.M;
+    float2 *Tq = bufA + 2 * pc.M;
+    __shared__ float wave_tot[kStrainThreads / 64];
+    __shared__ float edge_x[kStrainThreads + 1];
+
+    const int lb = xcd_block(npts);
+    if (lb < 0) return;
+    const int p = order ? order[lb] : lb;
+    const Located L = loc[p];
+    if (L.elem < 0) return;
+    load_twiddles(twq, pc.M, Tq);
+    build_weights(S, db, pc, frames[0], L, true, pc.nvar_fwd, pc.fwd_vars, ND, pc.comps_f, true);
+
+    const int tid = threadIdx.x, nt = pc.nt, M = pc.M, t0 = tid * SPT;
+    const size_t blk = (size_t)25 * nt;
+    float acc[ND][SPT];
+    gather_strain<SPT, ND>(S, db.u + (size_t)S.elem * db.nvar * blk, blk, pc.nvar_fwd, pc.fwd_vars, t0, nt, acc);
+    const float fdt = (float)pc.dt, h = (float)(pc.dt * 0.5);
+    float tap[SPT];
+    taper_factors(t0, nt, pc, tap);
 #pragma unroll
-                for (int c = 0; c < 6; ++c) { float2 q = cmulf(f[c], b[c]); m.x += q.x; m.y += q.y; }
-                bk[1] = make_float2(-2.0f * m.x, -2.0f * m.y);
-                bk[3] = cmulf(make_float2(f[1].x + f[0].x, f[1].y + f[0].y), make_float2(b[1].x + b[0].x, b[1].y + b[0].y));
-                float2 q5 = cmulf(b[4], f[4]), q4 = cmulf(b[3], f[3]);
-                bk[4] = make_float2(4.0f * (q5.x + q4.x), 4.0f * (q5.y + q4.y));
-                float2 c1 = cmulf(make_float2(f[0].x + f[1].x, f[0].y + f[1].y), b[2]);
-                float2 c2 = cmulf(cmulf(b[0], b[1]), f[2]);
-                bk[5] = make_float2(c1.x + c2.x, c1.y + c2.y);
-            }
-            bk[2] = make_float2(0.f, 0.f);
-            for (int s = 0; s < ns; ++s) {
-                float2 h = H[(size_t)rd.slot_h[s0 + s] * nw + k];
-                float2 v = bk[rd.slot_bk[s0 + s]];
-                acc[s] += (double)v.x * (double)h.x - (double)v.y * (double)h.y;
-            }
-        }
-        const int lane = tid & 63, wave = tid >> 6;
-        for (int s = 0; s < ns; ++s) {
-            double t = wave_sum(acc[s]);
-            if (lane == 0) red[wave][s] = t;
-        }
-        __syncthreads();
-        if (tid < ns) {
-            double t = 0.0;
-            for (int w = 0; w < kEvalThreads / 64; ++w) t += red[w][tid];
-            slotval[s0 + tid] = t;
-        }
+    for (int c = 0; c < ND; ++c) {
+        float x[SPT];
+        select_comp(acc, c, t0, nt, x);
+        cumsum_trapezoidal(x, t0, nt, h, edge_x, wave_tot);
+#pragma unroll
+        for (int q = 0; q < SPT; ++q) x[q] *= tap[q];
+        const float2 *Z = lds_fft(x, t0, nt, M, bufA, bufB, Tq);
+        float2 *dst = F + ((size_t)p * ND + c) * pc.nomega;
+        for (int k = tid; k <= M; k += kStrainThreads) dst[k] = rfft_bin(Z, k, M, Tq, fdt);
         __syncthreads();
     }
+}
+
+// ------------------------------------------------------------------------------------
+// k_bwd_eval: one workgroup per active (point, receiver) pair, pairs in element order.
+// The backward strain spectra (as k_fwd_spectra, without cumsum) stay in registers and
+// are contracted at once with the forward spectra of the point: base kernels
+// (kernel.f90:514-570) in the earth frame through the per-pair rotations, and the filtered
+// Parseval window integrals (kernel.f90:410-510, 946-1020) as sum_k Re(bk_k H_k) with
+// FP64 accumulation; then the physical kernels (kernel.f90:665-832).
+//  phys: [point][nk] values of the receiver's kernels; base_out (optional): [point][nk][6]
+// ------------------------------------------------------------------------------------
+template <int SPT, int NDF, int NDB>
+__global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && NDB <= 4) ? 3 : (SPT <= 8 ? 2 : 1)) void k_bwd_eval(
+    DevDb db, PathConf pc, const DevFrame *frames, int npairs, const int *order, int R,
+    const Located *loc_f, const Located *loc_b, const float2 *twq,
+    const float2 *F, const RecDesc *recs, const KernDesc *kern, const float2 *H,
+    const int *model_idx, const float *model /* [6][ndepth] */, int ndepth, int relative, int nk,
+    double *phys, double *base_out)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    OpShared &S = *reinterpret_cast<OpShared *>(smem_raw);
+    float2 *bufA = reinterpret_cast<float2 *>(smem_raw + ((sizeof(OpShared) + 15) / 16) * 16);
+    float2 *bufB = bufA + pc.M;
+    float2 *Tq = bufA + max(2 * pc.M, pc.nbk_lds * (pc.M + 1));
+    __shared__ EvalShared E;
+
+    const int lb = xcd_block(npairs);
+    if (lb < 0) return;
+    const int q = order ? order[lb] : lb;
+    const int p = q / R, r = q - (q / R) * R;
+    const Located L = loc_b[q];
+    if (L.elem < 0) return;
+    load_twiddles(twq, pc.M, Tq);
+    build_weights(S, db, pc, frames[1 + r], L, false, pc.nvar_bwd, pc.bwd_vars, NDB, pc.comps_b, true);
+
+    const int tid = threadIdx.x;
+    if (NDF > 1) {
+        // rotation of the point's forward components (frame 0), F.B and trace coefficients
+        if (tid == 0) {
+            const Located Lf = loc_f[p];
+            const DevFrame f0 = frames[0];
+            const double sp = sin(Lf.phi), cp = cos(Lf.phi);
+            for (int c = 0; c < NDF; ++c) {
+                double t[6] = {0, 0, 0, 0, 0, 0};
+                t[pc.comps_f[c]] = 1.0;
+                rot_src_to_xyz(t, sp, cp);
+                rot_xyz_to_earth(t, f0.sin_lon, f0.cos_lon, f0.sin_colat, f0.cos_colat);
+                for (int i = 0; i < 6; ++i) E.Qf[i][c] = (float)t[i];
+            }
+        }
+        __syncthreads();
+        if (tid < NDF * NDB) {
+            const int m = tid / NDB, n = tid - (tid / NDB) * NDB;
+            float a = 0.0f;
+            for (int i = 0; i < 6; ++i) a += E.Qf[i][m] * S.Q[i][n];
+            E.Amu[m][n] = a;
+        } else if (tid >= 64 && tid < 64 + NDF) {
+            const int m = tid - 64;
+            E.uf[m] = E.Qf[0][m] + E.Qf[1][m] + E.Qf[2][m];
+        } else if (tid >= 128 && tid < 128 + NDB) {
+            const int n = tid - 128;
+            E.ub[n] = S.Q[0][n] + S.Q[1][n] + S.Q[2][n];
+        }
+        // (made visible by the barriers of the FFTs below)
+    }
+
+    const int nt = pc.nt, M = pc.M, nw = pc.nomega, t0 = tid * SPT;
+    const size_t blk = (size_t)25 * nt;
+    float2 spec[NDB][SPT], specM[NDB];
+    {
+        float acc[NDB][SPT];
+        gather_strain<SPT, NDB>(S, db.u + (size_t)S.elem * db.nvar * blk, blk, pc.nvar_bwd, pc.bwd_vars, t0, nt, acc);
+        const float fdt = (float)pc.dt;
+        float tap[SPT];
+        taper_factors(t0, nt, pc, tap);
+        for (int c = 0; c < NDB; ++c) {
+            float x[SPT];
+            select_comp(acc, c, t0, nt, x);
+#pragma unroll
+            for (int q = 0; q < SPT; ++q) x[q] *= tap[q];
+            const float2 *Z = lds_fft(x, t0, nt, M, bufA, bufB, Tq);
+#pragma unroll
+            for (int j = 0; j < SPT; ++j) {
+                const int k = tid + kStrainThreads * j;
+                const float2 X = k < M ? rfft_bin(Z, k, M, Tq, fdt) : make_float2(0.0f, 0.0f);
+#pragma unroll
+                for (int cc = 0; cc < NDB; ++cc) if (cc == c) spec[cc][j] = X;
+            }
+            const float2 XM = rfft_bin(Z, M, M, Tq, fdt);
+#pragma unroll
+            for (int cc = 0; cc < NDB; ++cc) if (cc == c) specM[cc] = XM;
+            __syncthreads();
+        }
+    }
+
+    // frequency-domain contraction.  Pass A: the base kernels bk_b(k) of every bin from the
+    // forward spectra (HBM) and the backward spectra (registers) into LDS (over the FFT
+    // buffers); thread tid owns k = tid + 256 j, k = M goes to thread M % 256.
+    float2 *bkL = bufA;                                  // [pc.nbk_lds][M + 1]
+    const float2 *Fp = F + (size_t)p * NDF * nw;
+    const int need = pc.need_bk;
+    const int kM_owner = M & (kStrainThreads - 1);
+    constexpr int NJ = SPT + 1;                          // bins per thread (the last: k = M)
+    constexpr int JB = NJ < 3 ? NJ : 3;                  // bins whose loads are issued together
+#pragma unroll
+    for (int j0 = 0; j0 < NJ; j0 += JB) {
+        float2 fv[JB][NDF];
+#pragma unroll
+        for (int jj = 0; jj < JB; ++jj) {
+            const int j = j0 + jj;
+            const int k = j < SPT ? tid + kStrainThreads * j : M;
+            const bool ok = j < SPT ? (k < M) : (j == SPT && tid == kM_owner);
+#pragma unroll
+            for (int c = 0; c < NDF; ++c)
+                fv[jj][c] = (j < NJ && ok) ? Fp[(size_t)c * nw + k] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int jj = 0; jj < JB; ++jj) {
+            const int j = j0 + jj;
+            if (j >= NJ) continue;
+            const int k = j < SPT ? tid + kStrainThreads * j : M;
+            if (j < SPT ? (k >= M) : (tid != kM_owner)) continue;
+            const float2 *f = fv[jj];
+            float2 b[NDB];
+#pragma unroll
+            for (int c = 0; c < NDB; ++c) b[c] = j < SPT ? spec[c][j < SPT ? j : 0] : specM[c];
+            float2 bk0, bk1 = make_float2(0.f, 0.f), bk3 = bk1, bk4 = bk1, bk5 = bk1;
+            if (NDF == 1) {
+                const float2 pr = cmulf(f[0], b[0]);
+                bk0 = make_float2(-pr.x, -pr.y);
+                bk1 = make_float2(-2.0f * pr.x, -2.0f * pr.y);
+            } else {
+                float2 trf = make_float2(0.f, 0.f), trb = trf;
+#pragma unroll
+                for (int m = 0; m < NDF; ++m) { trf.x += E.uf[m] * f[m].x; trf.y += E.uf[m] * f[m].y; }
+#pragma unroll
+                for (int n = 0; n < NDB; ++n) { trb.x += E.ub[n] * b[n].x; trb.y += E.ub[n] * b[n].y; }
+                const float2 l = cmulf(trf, trb);
+                bk0 = make_float2(-l.x, -l.y);
+                if (need & 2) {
+                    float2 mu = make_float2(0.f, 0.f);
+#pragma unroll
+                    for (int m = 0; m < NDF; ++m) {
+                        float2 g = make_float2(0.f, 0.f);
+#pragma unroll
+                        for (int n = 0; n < NDB; ++n) { g.x += E.Amu[m][n] * b[n].x; g.y += E.Amu[m][n] * b[n].y; }
+                        const float2 t = cmulf(f[m], g);
+                        mu.x += t.x; mu.y += t.y;
+                    }
+                    bk1 = make_float2(-2.0f * mu.x, -2.0f * mu.y);
+                }
+                if (need & (8 | 16 | 32)) {
+                    float2 fe[6], be[6];
+#pragma unroll
+                    for (int i = 0; i < 6; ++i) {
+                        fe[i] = make_float2(0.f, 0.f);
+                        be[i] = make_float2(0.f, 0.f);
+#pragma unroll
+                        for (int m = 0; m < NDF; ++m) { fe[i].x += E.Qf[i][m] * f[m].x; fe[i].y += E.Qf[i][m] * f[m].y; }
+#pragma unroll
+                        for (int n = 0; n < NDB; ++n) { be[i].x += S.Q[i][n] * b[n].x; be[i].y += S.Q[i][n] * b[n].y; }
+                    }
+                    bk3 = cmulf(make_float2(fe[1].x + fe[0].x, fe[1].y + fe[0].y),
+                                make_float2(be[1].x + be[0].x, be[1].y + be[0].y));
+                    const float2 q5 = cmulf(be[4], fe[4]), q4 = cmulf(be[3], fe[3]);
+                    bk4 = make_float2(4.0f * (q5.x + q4.x), 4.0f * (q5.y + q4.y));
+                    // k_c exactly as kernel.f90:564-567 writes it
+                    const float2 c1 = cmulf(make_float2(fe[0].x + fe[1].x, fe[0].y + fe[1].y), be[2]);
+                    const float2 c2 = cmulf(cmulf(be[0], be[1]), fe[2]);
+                    bk5 = make_float2(c1.x + c2.x, c1.y + c2.y);
+                }
+            }
+            const int stride = M + 1;
+            if (pc.bk_lds[0] >= 0) bkL[pc.bk_lds[0] * stride + k] = bk0;
+            if (pc.bk_lds[1] >= 0) bkL[pc.bk_lds[1] * stride + k] = bk1;
+            if (pc.bk_lds[3] >= 0) bkL[pc.bk_lds[3] * stride + k] = bk3;
+            if (pc.bk_lds[4] >= 0) bkL[pc.bk_lds[4] * stride + k] = bk4;
+            if (pc.bk_lds[5] >= 0) bkL[pc.bk_lds[5] * stride + k] = bk5;
+        }
+    }
+    __syncthreads();
+    // Pass B: sum_k Re(bk_b(k) H_h(k)) in FP64 for every slot (h, b) of the receiver; the
+    // slots are ordered by h so that each window spectrum is read once per bin
+    const RecDesc &rd = recs[r];
+    const int lane = tid & 63, wave = tid >> 6;
+    // window spectra of the next group are loaded while the current one is summed
+    float2 hv[NJ], hn[NJ];
+    auto load_h = [&](int hh, float2 (&h)[NJ]) {
+        const float2 *Hh = H + (size_t)hh * nw;
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const int k = j < SPT ? tid + kStrainThreads * j : M;
+            const bool ok = j < SPT ? (k < M) : (tid == kM_owner);
+            h[j] = ok ? Hh[k] : make_float2(0.f, 0.f);
+        }
+    };
+    if (rd.ngroup > 0) load_h(rd.grp_h[0], hv);
+    for (int g = 0; g < rd.ngroup; ++g) {
+        const int s0 = rd.grp_first[g], ng = rd.grp_n[g];
+        if (g + 1 < rd.ngroup) load_h(rd.grp_h[g + 1], hn);
+        int off[kSlotGroup];
+#pragma unroll
+        for (int t = 0; t < kSlotGroup; ++t) off[t] = t < ng ? rd.slot_lds[s0 + t] * (M + 1) : 0;
+        double a2[kSlotGroup];
+#pragma unroll
+        for (int t = 0; t < kSlotGroup; ++t) a2[t] = 0.0;
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const int k = j < SPT ? tid + kStrainThreads * j : M;
+            if (j < SPT ? (k >= M) : (tid != kM_owner)) continue;
+#pragma unroll
+            for (int t = 0; t < kSlotGroup; ++t)
+                if (t < ng) {
+                    const float2 v = bkL[off[t] + k];
+                    // Re(v h) rounded once in fp32 (the inputs carry fp32 rounding anyway),
+                    // accumulated in fp64
+                    a2[t] += (double)__builtin_fmaf(v.x, hv[j].x, -(v.y * hv[j].y));
+                }
+        }
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) hv[j] = hn[j];
+#pragma unroll
+        for (int t = 0; t < kSlotGroup; ++t)
+            if (t < ng) {
+                const double v = wave_sum_lane63(a2[t]);
+                if (lane == 63) E.red[wave][s0 + t] = v;
+            }
+    }
+    __syncthreads();
+    if (tid < rd.nslot) {
+        double t = 0.0;
+        for (int w = 0; w < kStrainThreads / 64; ++w) t += E.red[w][tid];
+        E.slotval[tid] = t;
+    }
+    __syncthreads();
     // physical kernels (kernel.f90:665-832) with the model at the point (fp32 values)
     const int mi = model_idx[p];
     const float vp = model[0 * ndepth + mi], vs = model[1 * ndepth + mi], rho_f = model[2 * ndepth + mi];
     const float phi_m = model[3 * ndepth + mi], xi_m = model[4 * ndepth + mi], eta_m = model[5 * ndepth + mi];
     const double rho = rho_f;
-    for (int kk = tid; kk < rd.nkernel; kk += kEvalThreads) {
-        int k = rd.first_kernel + kk;
+    for (int kk = tid; kk < rd.nkernel; kk += kStrainThreads) {
+        const int k = rd.first_kernel + kk;
         const KernDesc &kd = kern[k];
         double b[6];
-        for (int q = 0; q < 6; ++q) b[q] = kd.slot[q] >= 0 ? slotval[kd.slot[q]] : 0.0;
+        for (int qq = 0; qq < 6; ++qq) b[qq] = kd.slot[qq] >= 0 ? E.slotval[kd.slot[qq]] : 0.0;
         if (base_out) {
-            for (int q = 0; q < 6; ++q) base_out[((size_t)p * nk + k) * 6 + q] = b[q];
+            for (int qq = 0; qq < 6; ++qq) base_out[((size_t)p * nk + k) * 6 + qq] = b[qq];
             continue;
         }
         double v = 0.0;
@@ -809,6 +1197,31 @@ __global__ __launch_bounds__(kEvalThreads) void k_eval(
         }
         phys[(size_t)p * nk + k] = v;
     }
+}
+
+// sort keys: element of every point (fwd) / of every active pair with a located element
+// (bwd, compacted through an atomic counter); keys past the last element sort last
+__global__ void k_keys_points(int n, const Located *loc, int nelem, int *keys, int *vals)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int e = loc[i].elem;
+    keys[i] = e < 0 ? nelem : e;
+    vals[i] = i;
+}
+
+__global__ void k_keys_pairs(int npairs, int R, int P, const uint8_t *active, const Located *loc,
+                             int *keys, int *vals, int *count)
+{
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= npairs) return;
+    int p = q / R, r = q - (q / R) * R;
+    if (active && !active[(p / P) * R + r]) return;
+    int e = loc[q].elem;
+    if (e < 0) return;
+    int pos = atomicAdd(count, 1);
+    keys[pos] = e;
+    vals[pos] = q;
 }
 
 // ------------------------------------------------------------------------------------
@@ -935,23 +1348,23 @@ __global__ void k_mc(int nwork, int P, int nk, int R, const int *work_voxel, con
     }
 }
 
-// Distinct elements gathered by one strain launch (per receiver for bwd frames): the
-// |F(B)| and sum_r |G_r(B)| of the algorithmic byte count.  counters: [0] distinct,
-// [1] pairs evaluated.
+// Algorithmic-byte accounting of one launch: distinct keys among the located (active)
+// pairs, key = element (by_point 0) or point (by_point 1).  counters: [0] distinct,
+// [1] pairs.
 __global__ void k_mark(int npairs, const Located *loc, int R, int P, const uint8_t *active,
-                       int nelem, unsigned *bitmap, unsigned long long *counters)
+                       int by_point, unsigned *bitmap, unsigned long long *counters)
 {
     int pair = blockIdx.x * blockDim.x + threadIdx.x;
     if (pair >= npairs) return;
-    int r = 0;
+    int p = pair;
     if (R > 0) {
-        int p = pair / R;
-        r = pair - p * R;
+        p = pair / R;
+        int r = pair - p * R;
         if (active && !active[(p / P) * R + r]) return;
     }
     int e = loc[pair].elem;
     if (e < 0) return;
-    size_t key = (size_t)r * nelem + e;
+    size_t key = by_point ? (size_t)p : (size_t)e;
     unsigned bit = 1u << (key & 31);
     unsigned old = atomicOr(&bitmap[key >> 5], bit);
     if (!(old & bit)) atomicAdd(&counters[0], 1ull);
