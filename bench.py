@@ -126,14 +126,13 @@ def gpu_db(p):
 
 def measured_traffic(path, kernel):
     """HBM bytes per launch of `kernel` from the committed counter summary: lines
-    'traffic <kernel tag>: <bytes per launch>' written from the FETCH_SIZE and WRITE_SIZE
+    'traffic <kernel>: <bytes per launch>' written from the FETCH_SIZE and WRITE_SIZE
     runs, each scaled by the same counter's reading for a plain copy kernel."""
     if not os.path.exists(path):
         return None
-    tag = "bwd" if "bwd" in kernel else "fwd"
     with open(path) as f:
         for ln in f:
-            if ln.startswith("traffic %s:" % tag):
+            if ln.startswith("traffic %s:" % kernel):
                 return float(ln.split(":", 1)[1].split()[0])
     return None
 
@@ -146,13 +145,13 @@ def main():
     ap.add_argument("--config", default="c1")
     ap.add_argument("--task", type=int, default=0, help="voxels per step (0 = whole job)")
     ap.add_argument("--cpu-workers", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--cpu-voxels-per-worker", type=int, default=1)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-voxels-per-worker", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-voxels", type=int, default=16)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run a plain copy kernel of this size (HBM counter calibration)")
-    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "strain_fft_c1_traffic.md"),
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "c1_hbm_traffic.md"),
                     help="summary of the FETCH_SIZE/WRITE_SIZE counter runs for the roofline")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -202,9 +201,9 @@ def main():
     launches = sum(s[1]["strain_launches"] for s in steps) // 2
     dev_ms = sum(s[1]["device_ms"] for s in steps)
     if bwd_ms >= fwd_ms:
-        kern, byt, ms = "k_strain_fft<8> (bwd frames)", bwd_b, bwd_ms
+        kern, byt, ms = "k_bwd_eval", bwd_b, bwd_ms
     else:
-        kern, byt, ms = "k_strain_fft<8> (fwd frame)", fwd_b, fwd_ms
+        kern, byt, ms = "k_fwd_spectra", fwd_b, fwd_ms
     achieved = (byt / launches) / (ms / launches * 1e-3) / 1e9 if ms > 0 else 0.0
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.traffic, kern),

@@ -83,3 +83,33 @@ def test_product_does_not_import_oracle():
                 text = open(os.path.join(dirpath, fn), errors="replace").read()
                 assert "import oracle" not in text and "from oracle" not in text, fn
                 assert "liboracle" not in text or fn == "build.py", fn
+
+
+FLANG = shutil.which("flang") or shutil.which("amdflang")
+
+
+@pytest.mark.skipif(FLANG is None or not os.path.exists(LIB), reason="flang or libmck.so missing")
+def test_fortran_binding_layout(tmp_path):
+    """The iso_c_binding module (mc_kernel_amd/fortran/mck_gpu.f90) has the C layout of
+    include/mck.h and links against the library (no GPU call)."""
+    mod = os.path.join(ROOT, "mc_kernel_amd", "fortran", "mck_gpu.f90")
+    prog = os.path.join(ROOT, "tests", "fortran", "layout_check.f90")
+    subprocess.run([FLANG, "-c", mod, "-o", str(tmp_path / "mck_gpu.o"), "-module-dir", str(tmp_path)],
+                   check=True, cwd=tmp_path)
+    exe = tmp_path / "layout_check"
+    subprocess.run([FLANG, prog, str(tmp_path / "mck_gpu.o"), "-I", str(tmp_path),
+                    "-L" + os.path.dirname(LIB), "-lmck", "-Wl,-rpath," + os.path.dirname(LIB),
+                    "-o", str(exe)], check=True, cwd=tmp_path)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    got = {}
+    for ln in out.splitlines():
+        parts = ln.split()
+        if parts[0] == "last_error":
+            continue
+        got[(parts[0], parts[1])] = int(parts[2])
+    for cname, py in STRUCTS.items():
+        if (cname, "size") in got:
+            assert got[(cname, "size")] == C.sizeof(py), cname
+    for (cname, field), v in got.items():
+        if field != "size":
+            assert v == getattr(STRUCTS[cname], field).offset, (cname, field)
