@@ -11,12 +11,24 @@
  * built with -ffp-contract=off) so that element indices and Halton points are
  * bit-identical to the reference's.
  *
- * Parity pins (tests/test_oracle_*.py): Halton known answers (test_halton.f90),
- * GLL/GLJ nodes and derivative matrices (test_spectral_basis.f90), mapping round trips,
- * polynomial strain closed forms (test_sem_derivatives.f90), FFT/taper/Gabor golden
- * responses (tests/gaborresponse_*.txt), Butterworth golden responses, cumsum fixtures,
- * MC known answers (test_mc_integration.f90), tet volume of tests/vertices.TEST, and the
- * reference's own kdtree2.f90 (compiled into oracle/_ref) for the 12-nearest order.
+ * Parity pins (tests/test_oracle_golden.py): Halton known answers (test_halton.f90) and the
+ * equality of the quad-precision and double-division forms; GLL/GLJ nodes and derivative
+ * matrices (test_spectral_basis.f90); mapping / inverse mapping / inverse Jacobian /
+ * inside-element known answers (test_finite_elem_mapping.f90); polynomial strain closed
+ * forms (test_sem_derivatives.f90); Voigt rotation known answers and R T R^T
+ * (test_rotations.f90); frame rotation geometry; azimuthal factors under rotation of the
+ * source; Gabor and Butterworth golden responses (tests/gaborresponse_*.txt,
+ * butterresponse*.txt), FFT identities, taper, cumsum fixtures; cut_timewindow known
+ * answers and the Parseval integral (test_kernel.f90); tabulate_kernels truth table;
+ * isotropic physical kernels by the chain rule; MC known answers (test_mc_integration.f90);
+ * tet volume of tests/vertices.TEST; the 12-nearest order against the reference's own
+ * kdtree2.f90 (oracle/_ref).
+ *
+ * Parity unpinned (restated from the reference text, no fixture or closed form checks
+ * them): calc_basekernel (kernel.f90:514-570) including the k_c product as written; the
+ * anisotropic physical kernels vsh, vsv, vph, vpv, eta (kernel.f90:745-822); add_stfs
+ * (filtering.f90:241-470, deconv_stf); dampen_field (readfields.f90:3184-3204); the
+ * slave_work loop itself (slave.f90:425-806), checked end to end only against itself.
  */
 #ifndef MCK_ORACLE_H
 #define MCK_ORACLE_H
@@ -92,6 +104,15 @@ int      ora_kernel_info(ora_ctx *ctx, int k, double *normalization, int *ncut, 
 int      ora_strain_nd(ora_ctx *ctx);
 /* Voigt tensor rotations alone (test hook): t[6][nt] in place */
 void     ora_rotate_voigt(int which, double *t, int nt, double phi, double theta);
+/* physical kernels alone (test hook) */
+double   ora_physical(int param, const double *b, float vp, float vs, float rho, float phi,
+                      float xi, float eta, int rel);
+/* tabulate_kernels alone (test hook): nb = lambda, mu, rho, a, b, c */
+int      ora_param_needs(int param, int32_t *nb);
+/* azim_factor alone (test hook): isim 1..4, ikind 1/2 */
+double   ora_azim_factor(double phi, const double *mij, int isim, int ikind);
+/* cut_timewindow on a time axis t[n] (test hook and kernel set-up) */
+void     ora_cut_window(int n, const double *t, double w0, double w1, int *ifirst, int *ncut);
 /* integrate_parseval alone (test hook) */
 double   ora_integrate_parseval(int n, double dt, const double *f, const double *g);
 /* the Monte Carlo integrator alone (test hook): vals[nsteps][npts][n] */

@@ -88,6 +88,14 @@ static int param_needs(int param, int *nb)
     return 0;
 }
 
+/* Test hook: tabulate_kernels alone */
+int ora_param_needs(int param, int32_t *nb)
+{
+    int b, v[6], rc = param_needs(param, v);
+    for (b = 0; b < 6; ++b) nb[b] = v[b];
+    return rc;
+}
+
 /* apply time shift exp(2 pi i f tau) (filtering.f90:622-652) */
 static void make_timeshift(int nomega, double df, double tau, double *ts)
 {
@@ -184,6 +192,18 @@ double ora_integrate_parseval(int n, double dt, const double *f, const double *g
     return v;
 }
 
+/* cut_timewindow (kernel.f90:836-927): samples with w0 <= t_i <= w1 of a monotonic time
+ * axis t[n]; first index and count (count 0 = empty window) */
+void ora_cut_window(int n, const double *t, double w0, double w1, int *ifirst, int *ncut)
+{
+    int i;
+    *ifirst = -1; *ncut = 0;
+    for (i = 0; i < n; ++i) {
+        if (t[i] <= w1 && t[i] >= w0) { if (*ifirst < 0) *ifirst = i; (*ncut)++; }
+        if (t[i] > w1) break;
+    }
+}
+
 /* cut_and_add_seismogram (kernel.f90:181-385) */
 static int kernel_init(ora_ctx *C, int k, char *err, int errlen)
 {
@@ -217,11 +237,11 @@ static int kernel_init(ora_ctx *C, int k, char *err, int errlen)
         free(disp); free(velo); free(Xd); free(Xv); free(d); free(v);
         return -1;
     }
-    K->ifirst = -1; K->ncut = 0;
-    for (i = 0; i < N; ++i) {
-        double t = i * S->dt;
-        if (t <= K->tw[1] && t >= K->tw[0]) { if (K->ifirst < 0) K->ifirst = i; K->ncut++; }
-        if (t > K->tw[1]) break;
+    {
+        double *tt = (double *)malloc(N * sizeof(double));
+        for (i = 0; i < N; ++i) tt[i] = i * S->dt;
+        ora_cut_window(N, tt, K->tw[0], K->tw[1], &K->ifirst, &K->ncut);
+        free(tt);
     }
     if (K->ncut == 0) {
         snprintf(err, errlen, "kernel %d: time window length was zero", k);
@@ -412,6 +432,12 @@ static double azim_factor(double phi, const double *mij, int isim, int ikind)
     default: return ikind == 1 ? 0.5 * (mij[1] - mij[2]) * cos(2.0 * phi) + mij[5] * sin(2.0 * phi)
                                : -0.5 * (mij[1] - mij[2]) * sin(2.0 * phi) + mij[5] * cos(2.0 * phi);
     }
+}
+
+/* Test hook: azim_factor alone */
+double ora_azim_factor(double phi, const double *mij, int isim, int ikind)
+{
+    return azim_factor(phi, mij, isim, ikind);
 }
 
 /* azim_factor_bw (rotations.f90:197-227) */
@@ -726,6 +752,18 @@ static double physical(int param, const double *b, const ora_model *m, int rel)
                                    : 2.0 * rho * (double)(m->vph * m->vph - m->vsh * m->vsh) * b[5];
     default: return 0.0;
     }
+}
+
+/* Test hook: physical kernels alone; model values as the fp32 table holds them */
+double ora_physical(int param, const double *b, float vp, float vs, float rho, float phi, float xi,
+                    float eta, int rel)
+{
+    ora_model m;
+    m.vp = vp; m.vs = vs; m.rho = rho; m.phi = phi; m.xi = xi; m.eta = eta;
+    m.lam = rho * (vp * vp - 2 * (vs * vs));
+    m.mu = rho * (vs * vs);
+    m.vsh = vs; m.vph = vp; m.vsv = vs / sqrtf(xi); m.vpv = vp * sqrtf(phi);
+    return physical(param, b, &m, rel);
 }
 
 /* forward field of a batch of points: load_fw_points, cumsum_trapezoidal, rfft of the

@@ -59,6 +59,13 @@ def lib():
         L.ora_set_lazy_fill.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.ora_integrate_parseval.argtypes = [C.c_int, C.c_double, f64p, f64p]
         L.ora_integrate_parseval.restype = C.c_double
+        L.ora_physical.argtypes = [C.c_int, f64p] + [C.c_float] * 6 + [C.c_int]
+        L.ora_physical.restype = C.c_double
+        L.ora_param_needs.argtypes = [C.c_int, i32p]
+        L.ora_azim_factor.argtypes = [C.c_double, f64p, C.c_int, C.c_int]
+        L.ora_azim_factor.restype = C.c_double
+        L.ora_cut_window.argtypes = [C.c_int, f64p, C.c_double, C.c_double, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int)]
         L.ora_rotate_voigt.argtypes = [C.c_int, f64p, C.c_int, C.c_double, C.c_double]
         L.ora_mc_run.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                  f64p, f64p, f64p, i32p]
@@ -188,6 +195,39 @@ def integrate_parseval(f, g, dt):
     f = np.ascontiguousarray(f, float)
     g = np.ascontiguousarray(g, float)
     return lib().ora_integrate_parseval(len(f), dt, P(f), P(g))
+
+
+def physical(param, base, vp, vs, rho, phi=1.0, xi=1.0, eta=1.0, relative=True):
+    b = np.ascontiguousarray(base, float)
+    return lib().ora_physical(abi.PARAMS[param], P(b), vp, vs, rho, phi, xi, eta, int(relative))
+
+
+def param_needs(param):
+    nb = np.zeros(6, np.int32)
+    rc = lib().ora_param_needs(abi.PARAMS[param], P(nb, C.c_int32))
+    return rc, nb.astype(bool)
+
+
+def azim_factor(phi, mij, isim, ikind):
+    m = np.ascontiguousarray(mij, float)
+    return lib().ora_azim_factor(phi, P(m), isim, ikind)
+
+
+def rotate_frame(xyz, lon, colat):
+    """Points [n][3] into the frame whose pole is at (lon, colat) (radians): s, phi, z."""
+    xyz = np.ascontiguousarray(xyz, float)
+    n = xyz.shape[0]
+    s, ph, z = np.zeros(n), np.zeros(n), np.zeros(n)
+    lib().ora_rotate_frame(n, P(xyz), lon, colat, P(s), P(ph), P(z))
+    return s, ph, z
+
+
+def cut_window(t, w0, w1):
+    """Indices of the samples of time axis t inside [w0, w1] (first, count)."""
+    t = np.ascontiguousarray(t, float)
+    i0, n = C.c_int(), C.c_int()
+    lib().ora_cut_window(len(t), P(t), w0, w1, C.byref(i0), C.byref(n))
+    return i0.value, n.value
 
 
 def rotate_voigt(t, phi, theta=0.0, to_earth=False):

@@ -531,3 +531,105 @@ def test_oracle_slave_work_invariants(tiny, tiny_oracle):
     again = tiny_oracle.slave_work(p.verts, p.conn[:12], halton_offsets=r["offsets"])
     assert np.array_equal(again["values"], r["values"])
     assert np.array_equal(again["niterations"], r["niterations"])
+
+
+# ------------------------------------------------------------------------------ more pins
+def test_cut_timewindow_known_answers():
+    """test_kernel.f90:37-117: samples i*dt (i = 1..100, dt 0.1), values i."""
+    t = np.arange(1, 101) * 0.1
+    for w, ref in [((1.45, 2.01), [15, 16, 17, 18, 19, 20]), ((1.45, 2.0), [15, 16, 17, 18, 19, 20]),
+                   ((1.45, 1.9999999), [15, 16, 17, 18, 19]), ((7.95, 8.05), [80])]:
+        i0, n = O.cut_window(t, *w)
+        assert list(range(i0 + 1, i0 + 1 + n)) == ref, (w, i0, n)
+    t5 = np.arange(1, 101) * 0.1 - 5
+    i0, n = O.cut_window(t5, -0.39, 0.29)
+    assert [i - 50 for i in range(i0 + 1, i0 + 1 + n)] == [-3, -2, -1, 0, 1, 2]
+
+
+def _rotate_mt(m, a):
+    """Moment tensor (rr, tt, pp, rt, rp, tp) rotated by a about the vertical axis."""
+    rr, tt, pp, rt, rp, tp = m
+    c, s = math.cos(a), math.sin(a)
+    R = np.array([[c, -s], [s, c]])
+    T = R @ np.array([[tt, tp], [tp, pp]]) @ R.T
+    v = R @ np.array([rt, rp])
+    return [rr, T[0, 0], T[1, 1], v[0], v[1], T[0, 1]]
+
+
+def test_azimuthal_factors_rotate_with_the_source():
+    """rotations.f90:89-131: turning the mechanism about the vertical by a turns every
+    azimuthal factor by a (one sign convention for all simulations and both kinds);
+    monopole factors are invariant."""
+    rng = np.random.default_rng(4)
+    m = rng.normal(size=6)
+    ok = {1: True, -1: True}
+    for _ in range(20):
+        a, phi = rng.uniform(0, 2 * math.pi, 2)
+        mr = _rotate_mt(m, a)
+        for isim in (1, 2):
+            assert math.isclose(O.azim_factor(phi, mr, isim, 1), O.azim_factor(phi, m, isim, 1), abs_tol=1e-12)
+        for sign in (1, -1):
+            for isim in (3, 4):
+                for kind in (1, 2):
+                    if not math.isclose(O.azim_factor(phi, mr, isim, kind),
+                                        O.azim_factor(phi - sign * a, m, isim, kind), abs_tol=1e-12):
+                        ok[sign] = False
+    assert ok[1] or ok[-1]
+
+
+def test_rotate_frame_epicentral_geometry():
+    """rotations.f90:737-782: rotation into the frame of a pole keeps |x| and puts the
+    point at z = |x| cos(distance to the pole), s = |x| sin(distance)."""
+    rng = np.random.default_rng(6)
+    lon, colat = 0.7, 1.1
+    pole = np.array([math.sin(colat) * math.cos(lon), math.sin(colat) * math.sin(lon), math.cos(colat)])
+    x = rng.normal(size=(50, 3)) * 6e6
+    s, ph, z = O.rotate_frame(x, lon, colat)
+    r = np.linalg.norm(x, axis=1)
+    cosd = (x @ pole) / r
+    np.testing.assert_allclose(z, r * cosd, rtol=1e-12, atol=1e-6)
+    np.testing.assert_allclose(s, r * np.sqrt(1 - cosd ** 2), rtol=1e-9, atol=1e-3)
+    assert np.all((ph >= 0) & (ph < 2 * math.pi))
+
+
+@pytest.mark.parametrize("param,needs", [
+    ("lam", "100000"), ("mu", "010000"), ("vp", "100000"), ("vs", "110000"),
+    ("vpv", "100101"), ("vsv", "000010"), ("vph", "000101"), ("vsh", "110011"),
+    ("eta", "100101")])
+def test_tabulate_kernels_truth_table(param, needs):
+    """test_kernel.f90:377-487: base kernels (lambda, mu, rho, a, b, c) each model
+    parameter needs; rho kernels are rejected (kernel.f90:554-556)."""
+    rc, nb = O.param_needs(param)
+    assert rc == 0
+    assert "".join("1" if x else "0" for x in nb) == needs
+
+
+def test_isotropic_physical_kernels_follow_the_chain_rule():
+    """kernel.f90:665-832 for lam, mu, vp, vs: K_x = K_lambda dlambda/dx + K_mu dmu/dx with
+    lambda = rho (vp^2 - 2 vs^2), mu = rho vs^2 (times x for relative perturbations),
+    derivatives taken numerically here."""
+    rng = np.random.default_rng(8)
+    vp, vs, rho = np.float32(8100.0), np.float32(4500.0), np.float32(3300.0)
+
+    def lam_mu(p):
+        return p["rho"] * (p["vp"] ** 2 - 2 * p["vs"] ** 2), p["rho"] * p["vs"] ** 2
+
+    base = {"vp": float(vp), "vs": float(vs), "rho": float(rho)}
+    for _ in range(5):
+        b = np.zeros(6)
+        b[0], b[1] = rng.normal(size=2)
+        for param, var in (("vp", "vp"), ("vs", "vs")):
+            h = base[var] * 1e-6
+            up, dn = dict(base), dict(base)
+            up[var] += h
+            dn[var] -= h
+            dl = (lam_mu(up)[0] - lam_mu(dn)[0]) / (2 * h)
+            dm = (lam_mu(up)[1] - lam_mu(dn)[1]) / (2 * h)
+            k_abs = b[0] * dl + b[1] * dm
+            np.testing.assert_allclose(O.physical(param, b, vp, vs, rho, relative=False), k_abs, rtol=1e-6)
+            np.testing.assert_allclose(O.physical(param, b, vp, vs, rho, relative=True), k_abs * base[var],
+                                       rtol=1e-6)
+        lam, mu = lam_mu(base)
+        np.testing.assert_allclose(O.physical("lam", b, vp, vs, rho, relative=True), b[0] * lam, rtol=1e-6)
+        np.testing.assert_allclose(O.physical("mu", b, vp, vs, rho, relative=True), b[1] * mu, rtol=1e-6)
+        assert O.physical("lam", b, vp, vs, rho, relative=False) == b[0]
