@@ -331,6 +331,7 @@ struct State {
     DBuf<float> model;
     std::vector<int> k_rec, k_nbk, rec_first, rec_count;
     DBuf<int> d_k_rec, d_k_nbk, d_rec_first, d_rec_count;
+    DBuf<double> d_rec_r;                    // [R][3] receiver positions (m)
     std::vector<double> t_last_p, t_last_s, rec_r;  // rec_r [R][3]
     double src_r[3];
     // per-task and per-chunk buffers
@@ -691,7 +692,7 @@ size_t bwd_smem(int M, int nbk) {
 
 // backward strain + evaluation of npairs (point, receiver) pairs q = p * R + r
 hipError_t launch_bwd(State &G, int npairs, const int *order, const Located *loc_f,
-                      const Located *loc_b, double *phys, double *base_out) {
+                      const Located *loc_b, double *phys, double *base_out, const uint8_t *active, int P) {
     if (npairs <= 0) return hipSuccess;
     const size_t smem = bwd_smem(G.pc.M, G.pc.nbk_lds);
     const dim3 grid(grid_x8(npairs)), block(kStrainThreads);
@@ -704,7 +705,8 @@ hipError_t launch_bwd(State &G, int npairs, const int *order, const Located *loc
         hipLaunchKernelGGL((k_bwd_eval<SP, NF, NB>), grid, block, smem, G.stream, G.bwd.d, G.pc,  \
                            G.frames.p, npairs, order, G.R, loc_f, loc_b, G.twq.p,                \
                            G.Fspec.p, G.recs.p, G.kerns.p, G.H.p, G.model_idx.p, G.model.p,     \
-                           G.S.model_ndepth, G.S.relative_kernel, G.nk, phys, base_out);        \
+                           G.S.model_ndepth, G.S.relative_kernel, G.nk, phys, base_out,         \
+                           G.xyz.p, active, P, G.d_rec_r.p);                                     \
         return hipGetLastError();                                                                 \
     }
 #define BWD_ND(SP) BWD(SP, 1, 1) BWD(SP, 4, 4) BWD(SP, 4, 6) BWD(SP, 6, 4) BWD(SP, 6, 6)
@@ -785,6 +787,10 @@ int setup_tables(State &G) {
             G.rec_r[3 * r + 1] = std::cos(la) * std::sin(lo) * S.planet_radius;
             G.rec_r[3 * r + 2] = std::sin(la) * S.planet_radius;
         }
+        HIPCHECK(G.d_rec_r.ensure(3 * G.R));
+        HIPCHECK(hipMemcpy(G.d_rec_r.p, G.rec_r.data(), 3 * G.R * sizeof(double), hipMemcpyHostToDevice));
+        for (int i = 0; i < 3; ++i) G.pc.src_r[i] = G.src_r[i];
+        G.pc.damp_radius = S.damp_radius;
     }
     // quarter twiddle table w_N^r = exp(-2 pi i r / N), r < N/4, N = 2M (the kernels derive
     // the other quadrants and the length-M twiddles w_M^t = w_N^(2t) from it)
@@ -833,8 +839,6 @@ int init_impl(const mck_setup_t *s) {
     if (S.nrec <= 0 || S.nkernel <= 0) return fail(MCK_EINVAL, "no receivers or kernels");
     if (S.fwd.nvar != 10 || S.bwd.nvar != 5) return fail(MCK_EINVAL, "merged databases need 10 (fwd) / 5 (bwd) variables");
     if (S.quasirandom != 1) return fail(MCK_EUNSUPPORTED, "only quasi-random (Halton) points are supported");
-    if (S.mask_src_rec) return fail(MCK_EUNSUPPORTED, "MASK_SOURCE_RECEIVER is not supported");
-    if (S.damp_radius > 0) return fail(MCK_EUNSUPPORTED, "DAMP_RADIUS_SOURCE_RECEIVER > 0 is not supported");
     if (S.component != 'Z' && S.component != 'R' && S.component != 'T')
         return fail(MCK_EINVAL, "component must be Z, R or T");
     if (!G.stream) HIPCHECK(hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking));
@@ -930,6 +934,43 @@ int init_impl(const mck_setup_t *s) {
     return MCK_OK;
 }
 
+// point_in_tetrahedron (polytopes.f90:660-715): barycentric coordinates as ratios of
+// quad-precision 4x4 determinants (cofactor expansion along the first row,
+// determinant_4_qp polytopes.f90:351-380); inside unless one is below -1e-9.
+__float128 det4q(__float128 m[4][4]) {
+#define A(r, c) m[(r) - 1][(c) - 1]
+    return A(1,1) * ( A(2,2) * ( A(3,3) * A(4,4) - A(3,4) * A(4,3) )
+                    - A(2,3) * ( A(3,2) * A(4,4) - A(3,4) * A(4,2) )
+                    + A(2,4) * ( A(3,2) * A(4,3) - A(3,3) * A(4,2) ) )
+         - A(1,2) * ( A(2,1) * ( A(3,3) * A(4,4) - A(3,4) * A(4,3) )
+                    - A(2,3) * ( A(3,1) * A(4,4) - A(3,4) * A(4,1) )
+                    + A(2,4) * ( A(3,1) * A(4,3) - A(3,3) * A(4,1) ) )
+         + A(1,3) * ( A(2,1) * ( A(3,2) * A(4,4) - A(3,4) * A(4,2) )
+                    - A(2,2) * ( A(3,1) * A(4,4) - A(3,4) * A(4,1) )
+                    + A(2,4) * ( A(3,1) * A(4,2) - A(3,2) * A(4,1) ) )
+         - A(1,4) * ( A(2,1) * ( A(3,2) * A(4,3) - A(3,3) * A(4,2) )
+                    - A(2,2) * ( A(3,1) * A(4,3) - A(3,3) * A(4,1) )
+                    + A(2,3) * ( A(3,1) * A(4,2) - A(3,2) * A(4,1) ) );
+#undef A
+}
+
+bool point_in_tet(const double *v, const double *p) {
+    __float128 m0[4][4], m[4][4];
+    for (int c = 0; c < 4; ++c) {
+        for (int r = 0; r < 3; ++r) m0[r][c] = v[3 * c + r];
+        m0[3][c] = 1;
+    }
+    const __float128 d0 = det4q(m0);
+    // epsilon of the quad-precision kind
+    if ((d0 < 0 ? -d0 : d0) < (__float128)1.92592994438723585305597794258492732e-34Q) return false;
+    for (int i = 0; i < 4; ++i) {
+        for (int r = 0; r < 4; ++r) for (int c = 0; c < 4; ++c) m[r][c] = m0[r][c];
+        for (int r = 0; r < 3; ++r) m[r][i] = p[r];
+        if (det4q(m) / d0 < (__float128)(-1e-9)) return false;
+    }
+    return true;
+}
+
 struct Task {
     int nelems = 0, P = 0;
     std::vector<double> tet;     // [ne][4][3]
@@ -962,6 +1003,8 @@ int prepare_task(State &G, Task &T, int elem_type, int nvpe, int nelems, int nba
             for (int c = 0; c < 3; ++c) T.tet[(size_t)e * 12 + 3 * i + c] = vertices[3 * vid + c];
         }
         const double *v = &T.tet[(size_t)e * 12];
+        // element containing the source: no integration, zero results (slave.f90:466-482)
+        if (S.mask_src_rec && point_in_tet(v, G.src_r)) { T.live[e] = 0; continue; }
         double vol = S.int_over_volume ? tet_volume(v) : 1.0;
         T.vol[e] = vol;
         double ae = S.allowed_error * vol;
@@ -974,6 +1017,8 @@ int prepare_task(State &G, Task &T, int elem_type, int nvpe, int nelems, int nba
                      + norm2_3(G.rec_r[3 * r] - c[0], G.rec_r[3 * r + 1] - c[1], G.rec_r[3 * r + 2] - c[2]);
             double tp = d / (double)S.vp_max, ts = d / (double)S.vs_max;
             T.masked[(size_t)e * G.R + r] = (tp > G.t_last_p[r] && ts > G.t_last_s[r]) ? 1 : 0;
+            // receiver inside the element (slave.f90:539-545): same treatment
+            if (S.mask_src_rec && point_in_tet(v, &G.rec_r[3 * r])) T.masked[(size_t)e * G.R + r] = 1;
         }
         T.hbase[e] = halton_offsets ? halton_offsets[e] : (*halton_step + (int64_t)e * stride);
     }
@@ -1088,7 +1133,7 @@ int eval_chunk(State &G, int P, int w0, int nw, std::vector<std::pair<hipEvent_t
     HIPCHECK(hipEventCreate(&a));
     HIPCHECK(hipEventCreate(&b));
     HIPCHECK(hipEventRecord(a, st));
-    HIPCHECK(launch_bwd(G, npairs, G.vals_out.p, loc_f, loc_b, G.phys.p + (size_t)w0 * P * G.nk, nullptr));
+    HIPCHECK(launch_bwd(G, npairs, G.vals_out.p, loc_f, loc_b, G.phys.p + (size_t)w0 * P * G.nk, nullptr, act, P));
     HIPCHECK(hipEventRecord(b, st));
     evs.push_back({a, b});
     G.stats.strain_launches += 2;
@@ -1371,7 +1416,7 @@ int mck_base_kernels(int32_t npts, const double *xyz, double *base) {
     HIPCHECK(hipMalloc((void **)&d_phys, (size_t)npts * G.nk * 8));
     hipError_t e = hipMemsetAsync(d_base, 0, (size_t)npts * G.nk * 6 * 8, G.stream);
     if (e == hipSuccess) e = launch_fwd(G, npts, nullptr, loc_f, G.Fspec.p);
-    if (e == hipSuccess) e = launch_bwd(G, npts * R, nullptr, loc_f, loc_b, d_phys, d_base);
+    if (e == hipSuccess) e = launch_bwd(G, npts * R, nullptr, loc_f, loc_b, d_phys, d_base, nullptr, 1);
     if (e == hipSuccess)
         e = hipMemcpyAsync(base, d_base, (size_t)npts * G.nk * 6 * 8, hipMemcpyDeviceToHost, G.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(G.stream);
@@ -1401,7 +1446,7 @@ void mck_finalize(void) {
     }
     G.frames.release(); G.twq.release(); G.H.release(); G.recs.release();
     G.kerns.release(); G.model.release(); G.d_k_rec.release(); G.d_k_nbk.release();
-    G.d_rec_first.release(); G.d_rec_count.release(); G.tet.release(); G.volume.release();
+    G.d_rec_first.release(); G.d_rec_count.release(); G.d_rec_r.release(); G.tet.release(); G.volume.release();
     G.allowed_var.release(); G.hbase.release(); G.masked.release(); G.live.release();
     G.working.release(); G.active.release(); G.work_list.release(); G.nwork.release();
     G.fsum.release(); G.f2sum.release(); G.integral.release(); G.variance.release();

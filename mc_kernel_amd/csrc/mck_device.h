@@ -77,6 +77,9 @@ struct PathConf {
     int need_bk;              // bit b set: base kernel b is used by some kernel
     int bk_lds[6];            // LDS array of base kernel b in the contraction (-1 = unused)
     int nbk_lds;
+    // dampen_field (readfields.f90:3184-3204): radius (<= 0 disables), source position
+    double damp_radius;
+    double src_r[3];
 };
 
 // LDS layout of the per-point operator set-up

@@ -947,7 +947,8 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && NDB <= 4) ? 3 : (SPT <
     const Located *loc_f, const Located *loc_b, const float2 *twq,
     const float2 *F, const RecDesc *recs, const KernDesc *kern, const float2 *H,
     const int *model_idx, const float *model /* [6][ndepth] */, int ndepth, int relative, int nk,
-    double *phys, double *base_out)
+    double *phys, double *base_out, const double *xyz, const uint8_t *active, int P,
+    const double *rec_xyz)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     OpShared &S = *reinterpret_cast<OpShared *>(smem_raw);
@@ -1170,11 +1171,27 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && NDB <= 4) ? 3 : (SPT <
     const float vp = model[0 * ndepth + mi], vs = model[1 * ndepth + mi], rho_f = model[2 * ndepth + mi];
     const float phi_m = model[3 * ndepth + mi], xi_m = model[4 * ndepth + mi], eta_m = model[5 * ndepth + mi];
     const double rho = rho_f;
+    // dampen_field after every processed receiver (slave.f90:701-706): the base kernels of
+    // receiver r are scaled by the receiver and source factors of r and of every later
+    // receiver processed in this step (readfields.f90:3184-3204)
+    double damp = 1.0;
+    if (pc.damp_radius > 0.0 && active) {
+        const double x0 = xyz[3 * (size_t)p], x1 = xyz[3 * (size_t)p + 1], x2 = xyz[3 * (size_t)p + 2];
+        const double ds = sqrt((x0 - pc.src_r[0]) * (x0 - pc.src_r[0]) + (x1 - pc.src_r[1]) * (x1 - pc.src_r[1])
+                               + (x2 - pc.src_r[2]) * (x2 - pc.src_r[2]));
+        const double fs = ds < pc.damp_radius ? ds / pc.damp_radius : 1.0;
+        for (int r2 = r; r2 < R; ++r2) {
+            if (!active[(size_t)(p / P) * R + r2]) continue;
+            const double *c = rec_xyz + 3 * r2;
+            const double dr = sqrt((x0 - c[0]) * (x0 - c[0]) + (x1 - c[1]) * (x1 - c[1]) + (x2 - c[2]) * (x2 - c[2]));
+            damp = damp * (dr < pc.damp_radius ? dr / pc.damp_radius : 1.0) * fs;
+        }
+    }
     for (int kk = tid; kk < rd.nkernel; kk += kStrainThreads) {
         const int k = rd.first_kernel + kk;
         const KernDesc &kd = kern[k];
         double b[6];
-        for (int qq = 0; qq < 6; ++qq) b[qq] = kd.slot[qq] >= 0 ? E.slotval[kd.slot[qq]] : 0.0;
+        for (int qq = 0; qq < 6; ++qq) b[qq] = kd.slot[qq] >= 0 ? E.slotval[kd.slot[qq]] * damp : 0.0;
         if (base_out) {
             for (int qq = 0; qq < 6; ++qq) base_out[((size_t)p * nk + k) * 6 + qq] = b[qq];
             continue;

@@ -53,6 +53,7 @@ void ora_halton(int ndim, const int64_t *seed, const int32_t *base, const int64_
                 int64_t step, int npoints, double *r);       /* r[npoints][ndim] */
 void ora_tet_points(const double *v, int npoints, const double *coords, double *out);
 double ora_tet_volume(const double *v);                      /* v[4][3] */
+int    ora_point_in_tet(const double *v, const double *p);       /* v[4][3], p[3] */
 double ora_norm2(int n, const double *x);                    /* gfortran NORM2 */
 
 /* mapping */

@@ -199,6 +199,48 @@ double ora_tet_volume(const double *v)
     }
 }
 
+/* determinant_4_qp (polytopes.f90:351-380): cofactor expansion along the first row, in
+ * quad precision; m[r][c] = a(r+1, c+1) */
+static __float128 det4q(__float128 m[4][4])
+{
+#define A(r, c) m[(r) - 1][(c) - 1]
+    return A(1,1) * ( A(2,2) * ( A(3,3) * A(4,4) - A(3,4) * A(4,3) )
+                    - A(2,3) * ( A(3,2) * A(4,4) - A(3,4) * A(4,2) )
+                    + A(2,4) * ( A(3,2) * A(4,3) - A(3,3) * A(4,2) ) )
+         - A(1,2) * ( A(2,1) * ( A(3,3) * A(4,4) - A(3,4) * A(4,3) )
+                    - A(2,3) * ( A(3,1) * A(4,4) - A(3,4) * A(4,1) )
+                    + A(2,4) * ( A(3,1) * A(4,3) - A(3,3) * A(4,1) ) )
+         + A(1,3) * ( A(2,1) * ( A(3,2) * A(4,4) - A(3,4) * A(4,2) )
+                    - A(2,2) * ( A(3,1) * A(4,4) - A(3,4) * A(4,1) )
+                    + A(2,4) * ( A(3,1) * A(4,2) - A(3,2) * A(4,1) ) )
+         - A(1,4) * ( A(2,1) * ( A(3,2) * A(4,3) - A(3,3) * A(4,2) )
+                    - A(2,2) * ( A(3,1) * A(4,3) - A(3,3) * A(4,1) )
+                    + A(2,3) * ( A(3,1) * A(4,2) - A(3,2) * A(4,1) ) );
+#undef A
+}
+
+/* point_in_tetrahedron (polytopes.f90:660-715): barycentric coordinates as ratios of
+ * quad-precision 4x4 determinants; inside unless one is below -1e-9.  v[4][3], p[3]. */
+int ora_point_in_tet(const double *v, const double *p)
+{
+    __float128 m0[4][4], m[4][4], d0;
+    int r, c, i;
+    for (c = 0; c < 4; ++c) {
+        for (r = 0; r < 3; ++r) m0[r][c] = v[3 * c + r];
+        m0[3][c] = 1;
+    }
+    d0 = det4q(m0);
+    if ((d0 < 0 ? -d0 : d0) < (__float128)1.92592994438723585305597794258492732e-34Q) return 0;
+    for (i = 0; i < 4; ++i) {
+        __float128 di;
+        for (r = 0; r < 4; ++r) for (c = 0; c < 4; ++c) m[r][c] = m0[r][c];
+        for (r = 0; r < 3; ++r) m[r][i] = p[r];
+        di = det4q(m) / d0;
+        if (di < (__float128)(-1e-9)) return 0;
+    }
+    return 1;
+}
+
 /* gfortran's inline NORM2: scaled sum of squares (BLAS dnrm2 style). */
 double ora_norm2(int n, const double *x)
 {

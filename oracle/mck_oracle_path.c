@@ -953,6 +953,15 @@ int ora_slave_work(ora_ctx *C, int nelems, const double *vertices, const int32_t
         volume = S->int_over_volume ? ora_tet_volume(v) : 1.0;
         if (halton_offsets) C->halton_step = halton_offsets[e];
         if (offsets_used) offsets_used[e] = C->halton_step;
+        /* element containing the source: no integration (slave.f90:466-482) */
+        if (S->mask_src_rec && ora_point_in_tet(v, C->src_r)) {
+            for (k = 0; k < nk; ++k) {
+                values[(size_t)e * nk + k] = 0.0;
+                variance[(size_t)e * nk + k] = 0.0;
+                niterations[(size_t)e * nk + k] = 0;
+            }
+            continue;
+        }
         mc_init(&mc, nk, volume, S->allowed_error * volume, S->allowed_relative_error);
         for (k = 0; k < nk; ++k) niter[k] = 0;
         for (;;) {
@@ -969,7 +978,9 @@ int ora_slave_work(ora_ctx *C, int nelems, const double *vertices, const int32_t
             for (r = 0; r < S->nrec; ++r) {
                 ora_rec *R = &C->R[r];
                 int allconv = 1;
-                if (travel_time_masked(C, r, center)) {
+                /* receiver inside the element (slave.f90:539-545), then the travel-time
+                 * mask (slave.f90:546-557): zero kernels, one iteration each */
+                if ((S->mask_src_rec && ora_point_in_tet(v, C->R[r].r)) || travel_time_masked(C, r, center)) {
                     for (k = R->first; k <= R->last; ++k) {
                         int b;
                         for (i = 0; i < P; ++i) for (b = 0; b < 6; ++b) base[((size_t)i * nk + k) * 6 + b] = 0.0;

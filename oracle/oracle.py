@@ -28,6 +28,7 @@ def lib():
         L.ora_tet_points.argtypes = [f64p, C.c_int, f64p, f64p]
         L.ora_tet_volume.argtypes = [f64p]
         L.ora_tet_volume.restype = C.c_double
+        L.ora_point_in_tet.argtypes = [f64p, f64p]
         L.ora_norm2.argtypes = [C.c_int, f64p]
         L.ora_norm2.restype = C.c_double
         L.ora_mapping.argtypes = [C.c_double, C.c_double, f64p, C.c_int, f64p]
@@ -115,6 +116,12 @@ def tet_points(v, coords):
 
 def tet_volume(v):
     return lib().ora_tet_volume(P(np.ascontiguousarray(v, float)))
+
+
+def point_in_tet(v, q):
+    v = np.ascontiguousarray(v, float)
+    q = np.ascontiguousarray(q, float)
+    return bool(lib().ora_point_in_tet(P(v), P(q)))
 
 
 def rfft(x, dt=1.0):

@@ -33,6 +33,41 @@ def case(request):
     k.close()
 
 
+def _tet_around(c, a=60e3):
+    c = np.asarray(c, float)
+    return c + a * np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]], float)
+
+
+def test_masks_and_damping():
+    """MASK_SOURCE_RECEIVER (slave.f90:466-482, 539-545) and DAMP_RADIUS_SOURCE_RECEIVER
+    (dampen_field after every processed receiver, slave.f90:701-706): tets around the
+    source and a receiver are added to the task."""
+    import mc_kernel_amd as mk
+    from oracle import oracle
+    p = synth.make_problem("tiny", ntet=8)
+    p.setup.mask_src_rec = 1
+    p.setup.damp_radius = 2500e3
+    src = synth.geo_xyz(p.src_lat, p.src_lon, synth.R_EARTH - p.src_depth * 1e3)
+    rec = synth.geo_xyz(p.rec_lat[0], p.rec_lon[0], synth.R_EARTH - 30e3)
+    verts = np.concatenate([np.asarray(p.verts, float).reshape(-1, 3), _tet_around(src), _tet_around(rec)])
+    n0 = np.asarray(p.verts).reshape(-1, 3).shape[0]
+    conn = np.concatenate([np.asarray(p.conn, np.int32),
+                           np.arange(n0 + 1, n0 + 5, dtype=np.int32)[None],
+                           np.arange(n0 + 5, n0 + 9, dtype=np.int32)[None]])
+    o = oracle.Oracle(p.setup)
+    ref = o.slave_work(verts, conn)
+    assert not np.any(ref["values"][-2]) and not np.any(ref["niterations"][-2])   # source tet
+    k = mk.MCKernel(p.setup)
+    try:
+        res = k.slave_work(verts, conn, halton_offsets=ref["offsets"])
+    finally:
+        k.close()
+    assert np.array_equal(res["niterations"], ref["niterations"])
+    assert res["stats"]["evaluations"] == ref["evaluations"]
+    for j in range(p.nkernel):
+        assert _rel(res["values"][:, j], ref["values"][:, j]) < TOL, j
+
+
 def _rel(g, r):
     n = np.linalg.norm(r)
     return np.linalg.norm(g - r) / n if n else np.linalg.norm(g)

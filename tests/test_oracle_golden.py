@@ -633,3 +633,18 @@ def test_isotropic_physical_kernels_follow_the_chain_rule():
         np.testing.assert_allclose(O.physical("lam", b, vp, vs, rho, relative=True), b[0] * lam, rtol=1e-6)
         np.testing.assert_allclose(O.physical("mu", b, vp, vs, rho, relative=True), b[1] * mu, rtol=1e-6)
         assert O.physical("lam", b, vp, vs, rho, relative=False) == b[0]
+
+
+def test_point_in_tetrahedron():
+    """polytopes.f90:660-715 (test_polytopes.f90:414-434): Halton points of a tet are
+    inside; barycentric coordinates below -1e-9 are outside; degenerate tets contain
+    nothing."""
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float)
+    for q, inside in [([0.1, 0.1, 0.1], True), ([0, 0, 0], True), ([1, 0, 0], True),
+                      ([0.5, 0.5, 0.1], False), ([-1e-12, 0.2, 0.2], True), ([-1e-6, 0.2, 0.2], False)]:
+        assert O.point_in_tet(v, q) == inside, q
+    w = np.array([[0, 0, 0], [4e5, 1e5, 0], [1e5, 3e5, 2e5], [-1e5, 2e5, 5e5]], float) + 5e6
+    pts = O.tet_points(w, O.halton(500, seed=(20, 20, 20), base=(2, 3, 5), leap=(2, 2, 2)))
+    assert all(O.point_in_tet(w, q) for q in pts)
+    flat = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]], float)
+    assert not O.point_in_tet(flat, [0.2, 0.2, 0.0])
