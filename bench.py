@@ -107,7 +107,8 @@ def cpu_baseline(name, workers, per_worker, max_iter):
 
 
 def gpu_db(p):
-    from mc_kernel_amd import abi
+    from mc_kernel_amd import abi, bind_hip_runtime
+    bind_hip_runtime()
     lib = C.CDLL(os.path.join(ROOT, "mc_kernel_amd", "libmck_synth_gpu.so"))
     lib.synth_gpu_db.argtypes = [C.c_int, abi.i32p, C.c_int, C.c_int, C.c_double, C.c_uint64,
                                  C.POINTER(C.c_void_p)]
@@ -137,6 +138,59 @@ def measured_traffic(path, kernel):
     return None
 
 
+def bench_autograd(args):
+    """c3: the stage of the c1 workload (nt 2048, 10 receivers, 4 Gabor filters, 160
+    kernels) at --autograd-points points with their strain traces; forward and the
+    hand-written adjoint (FFTs recomputed), timed with events on torch's stream."""
+    import torch
+    import mc_kernel_amd as mk
+    from mc_kernel_amd import autograd as ad
+    from mc_kernel_amd import synth
+    p = synth.make_problem("c1", nelem_target=2000, ntet=200)
+    k = mk.MCKernel(p.setup)
+    nd, nt, nw, nf, nr, nk = ad.dims()
+    P = args.autograd_points
+    x = p.sample_points(P, seed=31)
+    fw = torch.from_numpy(k.load_strain(0, x)).cuda()
+    bw = torch.empty((P, nr, nd, nt), dtype=torch.float32, device="cuda")
+    for r in range(nr):
+        bw[:, r] = torch.from_numpy(k.load_strain(1 + r, x)).cuda()
+    tf = ad.transfer_functions()
+    fw.requires_grad_(True); bw.requires_grad_(True); tf.requires_grad_(True)
+    w = torch.rand((P, nk, 6), dtype=torch.float64, device="cuda")
+    fwd_ms, bwd_ms = [], []
+    for i in range(args.warmup + args.steps):
+        fw.grad = bw.grad = tf.grad = None
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record()
+        base = ad.filter_convolve_window(tf, fw, bw)
+        e1.record()
+        (base * w).sum().backward()
+        e2.record()
+        torch.cuda.synchronize()
+        if i >= args.warmup:
+            fwd_ms.append(e0.elapsed_time(e1))
+            bwd_ms.append(e1.elapsed_time(e2))
+        print("autograd step %d: forward %.2f ms, backward %.2f ms" % (i, e0.elapsed_time(e1), e1.elapsed_time(e2)),
+              file=sys.stderr, flush=True)
+    k.close()
+    tf_b, fw_b, bw_b = 8 * nf * nw, 4 * P * nd * nt, 4 * P * nr * nd * nt
+    base_b = 8 * P * nk * 6
+    f_ms, b_ms = float(np.mean(fwd_ms)), float(np.mean(bwd_ms))
+    line = {
+        "metric": "c3 filter-convolve-window-integrate stage: forward and adjoint time",
+        "value": f_ms + b_ms, "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": f_ms + b_ms, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (f64 accumulation)", "data": "strain traces of a synthetic DB at random points",
+        "config": {"workload": "c3: c1 stage (nt 2048, 10 receivers, 4 Gabor, 160 kernels), "
+                               "%d points, gradients w.r.t. G, fwd and bwd traces" % P},
+        "forward_ms": f_ms, "backward_ms": b_ms,
+        "forward_algorithmic_GBs": (tf_b + fw_b + bw_b + base_b) / (f_ms * 1e-3) / 1e9,
+        "backward_algorithmic_GBs": (2 * (tf_b + fw_b + bw_b) + 2 * base_b) / (b_ms * 1e-3) / 1e9,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,12 +207,20 @@ def main():
                     help="first run a plain copy kernel of this size (HBM counter calibration)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "c1_hbm_traffic.md"),
                     help="summary of the FETCH_SIZE/WRITE_SIZE counter runs for the roofline")
+    ap.add_argument("--autograd", action="store_true",
+                    help="time the differentiable filter-convolve-window-integrate stage at "
+                         "the c3 size instead (forward and adjoint, one JSON line)")
+    ap.add_argument("--autograd-points", type=int, default=10000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
               "(voxel-parallel multi-GPU sharding is described in DESIGN.md)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+
+    if args.autograd:
+        bench_autograd(args)
+        return
 
     # CPU baseline first, in fresh processes, before this process touches the GPU
     cpu = None

@@ -246,6 +246,37 @@ int mck_load_strain_points(int32_t frame, int32_t npts, const double *xyz, float
  */
 int mck_base_kernels(int32_t npts, const double *xyz, double *base);
 
+/*
+ * The filter-convolve-window-integrate stage of the receiver loop (slave.f90:515-534,
+ * 586-640: rfft of the tapered traces, calc_basekernel kernel.f90:514-570,
+ * apply_filter_2d kernel.f90:946-956, irfft and calc_misfit_kernel kernel.f90:410-510) as a
+ * differentiable function, for a gradient check by automatic differentiation.  The
+ * reference has no such interface: it is the entry of the PyTorch binding.  Inputs:
+ *  tf         [nfilter][nomega][2] float: the filter responses G (filtering.f90:78-230);
+ *             the stage applies TF_fwd TF_bwd = G^2 x (STF deconvolution factors, time
+ *             shifts) as the set-up defines them
+ *  fw         [npts][nd][nt] float: earth-frame fwd strain traces before
+ *             cumsum_trapezoidal (as mck_load_strain_points frame 0 returns them)
+ *  bw         [npts][nrec][nd][nt] float: bwd strain traces of every receiver
+ * The filtered seismogram windows and their normalisations (kernel.f90:181-385) are the
+ * set-up's and enter as constants.  All arrays are DEVICE memory; work is queued on
+ * `stream` (a hipStream_t, NULL = default stream) and complete on return.  Scratch
+ * buffers are the library's: calls must not overlap.
+ */
+/* nd (1 trace, 6 full strain), nt, nomega = N/2 + 1, nfilter, nrec, nkernel */
+int mck_ad_dims(int32_t *nd, int32_t *nt, int32_t *nomega, int32_t *nfilter, int32_t *nrec,
+                int32_t *nkernel);
+/* the set-up's filter responses G, host [nfilter][nomega][2] */
+int mck_ad_transfer(float *tf);
+/* base [npts][nkernel][6] double, as mck_base_kernels */
+int mck_ad_forward(int32_t npts, const float *tf, const float *fw, const float *bw, double *base,
+                   void *stream);
+/* Adjoint for the weights gbase [npts][nkernel][6] of the base kernels: gtf (d/dRe G +
+ * i d/dIm G of sum gbase base), gfw, gbw, shaped as tf, fw, bw.  The FFTs are
+ * recomputed from the traces. */
+int mck_ad_backward(int32_t npts, const float *tf, const float *fw, const float *bw,
+                    const double *gbase, float *gtf, float *gfw, float *gbw, void *stream);
+
 /* Statistics of the last mck_slave_work / mck_slave_work_plain call. */
 int mck_get_stats(mck_stats_t *stats);
 

@@ -12,7 +12,7 @@ import numpy as np
 
 from . import abi
 
-__all__ = ["MCKernel", "MckError", "load_library", "abi"]
+__all__ = ["MCKernel", "MckError", "load_library", "bind_hip_runtime", "abi"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
@@ -28,6 +28,21 @@ def library_path():
     return os.path.join(_HERE, "libmck.so")
 
 
+def bind_hip_runtime():
+    """Makes the HIP libraries of this package share PyTorch's HIP runtime.
+
+    PyTorch-ROCm ships its own libamdhip64 / libhsa-runtime64 with the same sonames as
+    ROCm's.  A process that holds device tensors of torch and device memory of libmck.so
+    must run one runtime: once torch is imported, the dynamic loader resolves libmck.so's
+    libamdhip64.so.7 to the copy torch already loaded.  Loading libmck.so first would
+    map ROCm's copy beside torch's, and torch's device initialisation then fails.
+    Call before loading any HIP library of the package (load_library does)."""
+    try:
+        import torch  # noqa: F401  (loads torch's HIP runtime)
+    except ImportError:
+        pass
+
+
 def load_library():
     """Loads libmck.so (built in-tree by __graft_entry__.build())."""
     global _LIB
@@ -36,6 +51,7 @@ def load_library():
         if not os.path.exists(path):
             raise RuntimeError("libmck.so is not built (run __graft_entry__.build()); "
                                "there is no CPU fallback")
+        bind_hip_runtime()
         _LIB = abi.declare(C.CDLL(path))
     return _LIB
 

@@ -85,8 +85,8 @@ def ptr(arr, ctype):
 # exported symbols of libmck.so, in the order of include/mck.h
 EXPORTS = [
     "mck_init", "mck_slave_work", "mck_slave_work_plain", "mck_locate_points",
-    "mck_load_strain_points", "mck_base_kernels", "mck_get_stats", "mck_finalize",
-    "mck_last_error",
+    "mck_load_strain_points", "mck_base_kernels", "mck_ad_dims", "mck_ad_transfer",
+    "mck_ad_forward", "mck_ad_backward", "mck_get_stats", "mck_finalize", "mck_last_error",
 ]
 
 
@@ -106,6 +106,15 @@ def declare(lib):
     lib.mck_load_strain_points.restype = C.c_int
     lib.mck_base_kernels.argtypes = [C.c_int32, f64p, f64p]
     lib.mck_base_kernels.restype = C.c_int
+    lib.mck_ad_dims.argtypes = [i32p] * 6
+    lib.mck_ad_dims.restype = C.c_int
+    lib.mck_ad_transfer.argtypes = [f32p]
+    lib.mck_ad_transfer.restype = C.c_int
+    # device pointers and the stream travel as addresses
+    lib.mck_ad_forward.argtypes = [C.c_int32] + [C.c_void_p] * 5
+    lib.mck_ad_forward.restype = C.c_int
+    lib.mck_ad_backward.argtypes = [C.c_int32] + [C.c_void_p] * 8
+    lib.mck_ad_backward.restype = C.c_int
     lib.mck_get_stats.argtypes = [C.POINTER(MckStats)]
     lib.mck_get_stats.restype = C.c_int
     lib.mck_finalize.argtypes = []

@@ -11,6 +11,7 @@
 #include <complex>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <map>
 #include <string>
 #include <tuple>
@@ -20,6 +21,7 @@
 #include "mck_device.h"
 
 #include "mck_kernels.inc.h"
+#include "mck_ad.inc.h"
 
 using namespace mck;
 using cd = std::complex<double>;
@@ -357,6 +359,14 @@ struct State {
     int chunk_points = 0;
     hipStream_t stream = nullptr;
     mck_stats_t stats{};
+    // differentiable filter-convolve-window-integrate stage: window spectra without their
+    // filter (W_h), the filter of each window spectrum, the filter responses G of the
+    // set-up, and scratch of the forward and adjoint launches
+    int nH = 0;
+    std::vector<float> ad_tf_host;           // [nfilter][nomega][2]
+    DBuf<float2> ad_W, ad_Hg, ad_Fs, ad_gF, ad_U;
+    DBuf<double2> ad_part;
+    DBuf<int> ad_hfilt;
 };
 State g;
 
@@ -457,9 +467,14 @@ int build_kernels(State &G) {
     std::vector<double> f(nw);
     for (int k = 0; k < nw; ++k) f[k] = df * k;
     std::vector<std::vector<cd>> tff(S.nfilter), tfb(S.nfilter);
+    G.ad_tf_host.assign((size_t)S.nfilter * nw * 2, 0.0f);
     for (int i = 0; i < S.nfilter; ++i) {
         tff[i] = filter_tf(S.filters[i], f);
         tfb[i] = tff[i];
+        for (int k = 0; k < nw; ++k) {
+            G.ad_tf_host[((size_t)i * nw + k) * 2] = (float)tff[i][k].real();
+            G.ad_tf_host[((size_t)i * nw + k) * 2 + 1] = (float)tff[i][k].imag();
+        }
         if (S.deconv_stf) {
             // add_stfs (filtering.f90:241-470)
             if (!S.stf_fwd) return fail(MCK_EINVAL, "deconv_stf needs stf_fwd");
@@ -485,6 +500,25 @@ int build_kernels(State &G) {
                 return fail(MCK_EINVAL, "filter is not vanishing fast enough for high frequencies");
         }
     }
+    // the factors of TF_fwd TF_bwd besides the filter response G (G^2 rest_f rest_b):
+    // add_stfs of a unit response, non-finite values (0/0 bins) set to 0
+    std::vector<cd> rest(nw, 1.0);
+    if (S.deconv_stf) {
+        auto Fs = rfft(taper_zeropad(S.stf_fwd, nt, N, 2, false), N, dt);
+        std::vector<double> e(nt, 0.0);
+        for (int t = 0; t < nt; ++t) e[t] = S.stf_src ? S.stf_src[t] : (t == 0 ? 1.0 : 0.0);
+        auto Fe = rfft(taper_zeropad(e.data(), nt, N, 2, true), N, dt);
+        auto lp = butter_lp(f, 1.0 / (f[nw - 1] * 0.75), 12);
+        for (int k = 0; k < nw; ++k) {
+            cd l = lp[k] * std::conj(lp[k]);
+            cd a = cd(1.0) / Fs[k] * S.fwd.amplitude;
+            double th = 2 * kPi * f[k] * S.stf_src_shift;
+            cd rf = a * Fe[k] * cd(std::cos(th), std::sin(th)) * l, rb = a * l;
+            if (!std::isfinite(std::abs(rf))) rf = 0.0;
+            if (!std::isfinite(std::abs(rb))) rb = 0.0;
+            rest[k] = rf * rb;
+        }
+    }
     std::vector<cd> ts(nw, 1.0);
     if (!S.deconv_stf)
         for (int k = 0; k < nw; ++k) {
@@ -493,7 +527,8 @@ int build_kernels(State &G) {
         }
     // seismograms per receiver (cut_and_add_seismogram, kernel.f90:181-385)
     std::map<std::tuple<int, int, int, int>, int> hidx;  // (rec, filter, misfit, window) -> H
-    std::vector<std::vector<cd>> Hs;
+    std::vector<std::vector<cd>> Hs, Ws;
+    std::vector<int> hfilt;
     std::vector<RecDesc> recs(G.R);
     std::vector<KernDesc> kerns(G.nk);
     for (int r = 0; r < G.R; ++r) { recs[r].nslot = 0; recs[r].first_kernel = -1; recs[r].nkernel = 0; }
@@ -561,7 +596,7 @@ int build_kernels(State &G) {
                 }
                 w[n] = dt * om * xs[ifirst + n];
             }
-            std::vector<cd> Hk(nw);
+            std::vector<cd> Hk(nw), Wk(nw);
             for (int q = 0; q < nw; ++q) {
                 cd Sk = 0.0;
                 for (int n = 0; n < ncut; ++n) {
@@ -571,9 +606,12 @@ int build_kernels(State &G) {
                 }
                 double ck = (q == 0 || q == nw - 1) ? 1.0 : 2.0;
                 Hk[q] = -norm * df * ck * Sk * tff[fi][q] * tfb[fi][q] * ts[q];
+                Wk[q] = -norm * df * ck * Sk * rest[q] * ts[q];
             }
             h = (int)Hs.size();
             Hs.push_back(Hk);
+            Ws.push_back(Wk);
+            hfilt.push_back(fi);
             hidx[key] = h;
         }
         KernDesc &kd = kerns[k];
@@ -627,6 +665,13 @@ int build_kernels(State &G) {
         for (int q = 0; q < nw; ++q) Hh[h * nw + q] = make_float2((float)Hs[h][q].real(), (float)Hs[h][q].imag());
     HIPCHECK(G.H.ensure(Hh.size()));
     HIPCHECK(hipMemcpy(G.H.p, Hh.data(), Hh.size() * sizeof(float2), hipMemcpyHostToDevice));
+    G.nH = (int)Hs.size();
+    for (size_t h = 0; h < Ws.size(); ++h)
+        for (int q = 0; q < nw; ++q) Hh[h * nw + q] = make_float2((float)Ws[h][q].real(), (float)Ws[h][q].imag());
+    HIPCHECK(G.ad_W.ensure(Hh.size()));
+    HIPCHECK(hipMemcpy(G.ad_W.p, Hh.data(), Hh.size() * sizeof(float2), hipMemcpyHostToDevice));
+    HIPCHECK(G.ad_hfilt.ensure(hfilt.size()));
+    HIPCHECK(hipMemcpy(G.ad_hfilt.p, hfilt.data(), hfilt.size() * sizeof(int), hipMemcpyHostToDevice));
     HIPCHECK(G.recs.ensure(G.R));
     HIPCHECK(hipMemcpy(G.recs.p, recs.data(), G.R * sizeof(RecDesc), hipMemcpyHostToDevice));
     HIPCHECK(G.kerns.ensure(G.nk));
@@ -1312,6 +1357,92 @@ int locate_frame(State &G, int frame, int npts) {
     return MCK_OK;
 }
 
+// ---- the differentiable filter-convolve-window-integrate stage ------------------------
+size_t ad_smem(int M) { return (2 * (size_t)M + M / 2) * sizeof(float2); }
+
+bool device_ptr(const void *p) {
+    hipPointerAttribute_t a;
+    if (!p || hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice;
+}
+
+int ad_check(const State &G, int npts, std::initializer_list<const void *> ptrs) {
+    if (!G.init) return fail(MCK_ENOTINIT, "mck_init has not been called");
+    if (npts < 0) return fail(MCK_EINVAL, "negative point count");
+    if (npts == 0) return MCK_OK;
+    for (const void *p : ptrs)
+        if (!device_ptr(p)) return fail(MCK_EINVAL, "arguments must be device memory");
+    return MCK_OK;
+}
+
+// W_h G_f^2 of every window spectrum and the forward spectra of the traces
+hipError_t ad_prologue(State &G, int npts, const float2 *tf, const float *fw, hipStream_t st) {
+    const int nw = G.pc.nomega, nd = G.pc.nd, M = G.pc.M;
+    hipError_t e = G.ad_Hg.ensure((size_t)G.nH * nw);
+    if (e == hipSuccess) e = G.ad_Fs.ensure((size_t)npts * nd * nw);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_ad_hg, dim3((G.nH * nw + 255) / 256), dim3(256), 0, st, G.nH, nw, G.ad_W.p,
+                       G.ad_hfilt.p, tf, G.ad_Hg.p);
+    const size_t smem = ad_smem(M);
+    switch (spt_for(G.pc.nt)) {
+#define CASE(SP)                                                                                  \
+    case SP:                                                                                      \
+        if (smem > 65536)                                                                         \
+            (void)hipFuncSetAttribute((const void *)&k_ad_fspec<SP>,                              \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);     \
+        hipLaunchKernelGGL((k_ad_fspec<SP>), dim3(npts), dim3(kStrainThreads), smem, st, G.pc,    \
+                           npts, nd, fw, G.twq.p, G.ad_Fs.p);                                    \
+        break;
+        CASE(1) CASE(2) CASE(4) CASE(8) CASE(16)
+#undef CASE
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t ad_launch_eval(State &G, int npts, const float *bw, double *base, hipStream_t st) {
+    const size_t smem = ad_smem(G.pc.M);
+    const int npairs = npts * G.R;
+#define EVAL(SP, ND)                                                                              \
+    if (spt_for(G.pc.nt) == SP && G.pc.nd == ND) {                                                \
+        if (smem > 65536)                                                                         \
+            (void)hipFuncSetAttribute((const void *)&k_ad_eval<SP, ND>,                           \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);     \
+        hipLaunchKernelGGL((k_ad_eval<SP, ND>), dim3(npairs), dim3(kStrainThreads), smem, st,     \
+                           G.pc, npairs, G.R, bw, G.twq.p, G.ad_Fs.p, G.recs.p, G.kerns.p,       \
+                           G.ad_Hg.p, G.nk, base);                                                \
+        return hipGetLastError();                                                                 \
+    }
+    EVAL(1, 1) EVAL(2, 1) EVAL(4, 1) EVAL(8, 1) EVAL(16, 1)
+    EVAL(1, 6) EVAL(2, 6) EVAL(4, 6) EVAL(8, 6) EVAL(16, 6)
+#undef EVAL
+    return hipErrorInvalidValue;
+}
+
+hipError_t ad_launch_grad(State &G, int npts, const float *bw, const double *gbase, float *gbw,
+                          float *gfw, hipStream_t st) {
+    const size_t smem = ad_smem(G.pc.M);
+    const int nF = G.S.nfilter;
+#define GRAD(SP, ND)                                                                              \
+    if (spt_for(G.pc.nt) == SP && G.pc.nd == ND) {                                                \
+        if (smem > 65536)                                                                         \
+            (void)hipFuncSetAttribute((const void *)&k_ad_grad<SP, ND>,                           \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);     \
+        hipLaunchKernelGGL((k_ad_grad<SP, ND>), dim3(npts), dim3(kStrainThreads), smem, st,       \
+                           G.pc, npts, G.R, bw, G.twq.p, G.ad_Fs.p, G.recs.p, G.kerns.p,         \
+                           G.ad_Hg.p, G.ad_W.p, G.ad_hfilt.p, nF, G.nk, gbase, G.ad_gF.p,        \
+                           G.ad_U.p, gbw, gfw);                                                   \
+        return hipGetLastError();                                                                 \
+    }
+    GRAD(1, 1) GRAD(2, 1) GRAD(4, 1) GRAD(8, 1) GRAD(16, 1)
+    GRAD(1, 6) GRAD(2, 6) GRAD(4, 6) GRAD(8, 6) GRAD(16, 6)
+#undef GRAD
+    return hipErrorInvalidValue;
+}
+
 }  // namespace
 
 // =====================================================================================
@@ -1429,6 +1560,73 @@ int mck_base_kernels(int32_t npts, const double *xyz, double *base) {
     return MCK_OK;
 }
 
+int mck_ad_dims(int32_t *nd, int32_t *nt, int32_t *nomega, int32_t *nfilter, int32_t *nrec,
+                int32_t *nkernel) {
+    const State &G = g;
+    if (!G.init) return fail(MCK_ENOTINIT, "mck_init has not been called");
+    if (nd) *nd = G.pc.nd;
+    if (nt) *nt = G.pc.nt;
+    if (nomega) *nomega = G.pc.nomega;
+    if (nfilter) *nfilter = G.S.nfilter;
+    if (nrec) *nrec = G.R;
+    if (nkernel) *nkernel = G.nk;
+    return MCK_OK;
+}
+
+int mck_ad_transfer(float *tf) {
+    const State &G = g;
+    if (!G.init) return fail(MCK_ENOTINIT, "mck_init has not been called");
+    if (!tf) return fail(MCK_EINVAL, "null output");
+    std::memcpy(tf, G.ad_tf_host.data(), G.ad_tf_host.size() * sizeof(float));
+    return MCK_OK;
+}
+
+int mck_ad_forward(int32_t npts, const float *tf, const float *fw, const float *bw, double *base,
+                   void *stream) {
+    State &G = g;
+    int rc = ad_check(G, npts, {tf, fw, bw, base});
+    if (rc || npts == 0) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e = ad_prologue(G, npts, reinterpret_cast<const float2 *>(tf), fw, st);
+    if (e == hipSuccess) e = ad_launch_eval(G, npts, bw, base, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return fail(MCK_EHIP, std::string("mck_ad_forward: ") + hipGetErrorString(e));
+    return MCK_OK;
+}
+
+int mck_ad_backward(int32_t npts, const float *tf, const float *fw, const float *bw,
+                    const double *gbase, float *gtf, float *gfw, float *gbw, void *stream) {
+    State &G = g;
+    int rc = ad_check(G, npts, {tf, fw, bw, gbase, gtf, gfw, gbw});
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const int nw = G.pc.nomega, nF = G.S.nfilter, n = nF * nw;
+    if (npts == 0) {
+        // no points: the response gradient is zero
+        if (!device_ptr(gtf)) return fail(MCK_EINVAL, "arguments must be device memory");
+        hipError_t e = hipMemsetAsync(gtf, 0, (size_t)n * sizeof(float2), st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return fail(MCK_EHIP, hipGetErrorString(e));
+        return MCK_OK;
+    }
+    const int chunk = 256, nchunk = (npts + chunk - 1) / chunk;
+    hipError_t e = G.ad_gF.ensure((size_t)npts * G.pc.nd * nw);
+    if (e == hipSuccess) e = G.ad_U.ensure((size_t)npts * n);
+    if (e == hipSuccess) e = G.ad_part.ensure((size_t)nchunk * n);
+    if (e == hipSuccess) e = ad_prologue(G, npts, reinterpret_cast<const float2 *>(tf), fw, st);
+    if (e == hipSuccess) e = ad_launch_grad(G, npts, bw, gbase, gbw, gfw, st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_ad_upart, dim3((n + 255) / 256, nchunk), dim3(256), 0, st, n, npts, chunk,
+                           G.ad_U.p, G.ad_part.p);
+        hipLaunchKernelGGL(k_ad_tfgrad, dim3((n + 255) / 256), dim3(256), 0, st, n, nchunk, G.ad_part.p,
+                           reinterpret_cast<const float2 *>(tf), reinterpret_cast<float2 *>(gtf));
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return fail(MCK_EHIP, std::string("mck_ad_backward: ") + hipGetErrorString(e));
+    return MCK_OK;
+}
+
 int mck_get_stats(mck_stats_t *stats) {
     if (!stats) return fail(MCK_EINVAL, "null stats");
     *stats = g.stats;
@@ -1455,6 +1653,8 @@ void mck_finalize(void) {
     G.Fspec.release(); G.phys.release(); G.evals.release(); G.err.release();
     G.elem_mark.release(); G.elem_count.release(); G.keys_in.release(); G.keys_out.release();
     G.vals_in.release(); G.vals_out.release(); G.pair_count.release(); G.sort_tmp.release();
+    G.ad_W.release(); G.ad_Hg.release(); G.ad_Fs.release(); G.ad_gF.release(); G.ad_U.release();
+    G.ad_part.release(); G.ad_hfilt.release();
     G.init = false;
 }
 

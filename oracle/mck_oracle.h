@@ -22,7 +22,9 @@
  * answers and the Parseval integral (test_kernel.f90); tabulate_kernels truth table;
  * isotropic physical kernels by the chain rule; MC known answers (test_mc_integration.f90);
  * tet volume of tests/vertices.TEST; the 12-nearest order against the reference's own
- * kdtree2.f90 (oracle/_ref).
+ * kdtree2.f90 (oracle/_ref); the differentiable filter-convolve-window-integrate stage:
+ * its forward equals ora_base_kernels bit for bit, its adjoint equals FP64 central
+ * differences of the forward (exact for a function linear/quadratic in every input).
  *
  * Parity unpinned (restated from the reference text, no fixture or closed form checks
  * them): calc_basekernel (kernel.f90:514-570) including the k_c product as written; the
@@ -120,6 +122,17 @@ double   ora_integrate_parseval(int n, double dt, const double *f, const double 
 int      ora_mc_run(int n, double volume, double allowed_error, double allowed_rel, int nsteps,
                     int npts, const double *vals, double *integral, double *variance,
                     int32_t *conv);
+/* The filter-convolve-window-integrate stage as a function of the filter responses
+ * G [nfilter][nomega][2] and the strain traces fw [npts][nd][nt] (before
+ * cumsum_trapezoidal), bw [npts][nrec][nd][nt]: base [npts][nkernel][6] as
+ * ora_base_kernels, and its FP64 adjoint for the weights gbase [npts][nkernel][6]:
+ * gG as d/dRe + i d/dIm, gfw, gbw.  ora_ad_transfer returns the set-up's own G. */
+void     ora_ad_transfer(ora_ctx *ctx, double *G);
+int      ora_ad_forward(ora_ctx *ctx, int npts, const double *G, const double *fw,
+                        const double *bw, double *base);
+int      ora_ad_backward(ora_ctx *ctx, int npts, const double *G, const double *fw,
+                         const double *bw, const double *gbase, double *gG, double *gfw,
+                         double *gbw);
 /* Optional on-demand generation of database elements (which: 0 fwd, 1 bwd): the
  * snapshots arrays must then be writable; used for bounded CPU-baseline samples. */
 typedef void (*ora_fill_fn)(int which, int elem, float *dst, void *user);

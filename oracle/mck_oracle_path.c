@@ -52,6 +52,8 @@ struct ora_ctx {
     int full, nd;
     double gll[8], glj[8];
     double *tf_fwd, *tf_bwd;   /* [nfilter][nomega][2] */
+    double *tf_g;              /* [nfilter][nomega][2] filter responses alone */
+    double *rest_f, *rest_b;   /* [nomega][2] TF_fwd = G rest_f, TF_bwd = G rest_b */
     double *ts_fwd, *ts_bwd;   /* [nomega][2] time shift operators (deconv_stf == 0) */
     ora_kernel *K;
     ora_rec *R;
@@ -309,11 +311,25 @@ ora_ctx *ora_create(const mck_setup_t *S, char *err, int errlen)
     /* filters (filtering.f90:78-230) and STF deconvolution (add_stfs) */
     C->tf_fwd = (double *)malloc((size_t)S->nfilter * 2 * C->nomega * sizeof(double));
     C->tf_bwd = (double *)malloc((size_t)S->nfilter * 2 * C->nomega * sizeof(double));
+    C->tf_g = (double *)malloc((size_t)S->nfilter * 2 * C->nomega * sizeof(double));
     for (f = 0; f < S->nfilter; ++f) {
         double *a = C->tf_fwd + (size_t)f * 2 * C->nomega, *b = C->tf_bwd + (size_t)f * 2 * C->nomega;
         ora_filter_tf(&S->filters[f], C->nomega, C->df, a);
+        memcpy(C->tf_g + (size_t)f * 2 * C->nomega, a, 2 * C->nomega * sizeof(double));
         memcpy(b, a, 2 * C->nomega * sizeof(double));
         if (S->deconv_stf && add_stfs(C, a, b, err, errlen)) { ora_destroy(C); return NULL; }
+    }
+    /* the deconvolution factors alone: add_stfs of a unit response (its high-frequency
+     * check does not apply to them) */
+    C->rest_f = (double *)malloc(2 * C->nomega * sizeof(double));
+    C->rest_b = (double *)malloc(2 * C->nomega * sizeof(double));
+    for (k = 0; k < C->nomega; ++k) {
+        C->rest_f[2 * k] = C->rest_b[2 * k] = 1.0;
+        C->rest_f[2 * k + 1] = C->rest_b[2 * k + 1] = 0.0;
+    }
+    if (S->deconv_stf) {
+        char e2[256];
+        add_stfs(C, C->rest_f, C->rest_b, e2, sizeof e2);
     }
     C->ts_fwd = (double *)malloc(2 * C->nomega * sizeof(double));
     C->ts_bwd = (double *)malloc(2 * C->nomega * sizeof(double));
@@ -356,6 +372,7 @@ void ora_destroy(ora_ctx *C)
     if (!C) return;
     if (C->K) for (k = 0; k < C->S.nkernel; ++k) { free(C->K[k].seis_cut); free(C->K[k].seis_cut_fd); }
     free(C->K); free(C->R); free(C->tf_fwd); free(C->tf_bwd); free(C->ts_fwd); free(C->ts_bwd);
+    free(C->tf_g); free(C->rest_f); free(C->rest_b);
     free(C->filled[0]); free(C->filled[1]);
     free(C);
 }
@@ -682,12 +699,13 @@ static void basekernel(int ibk, int full, int nd, int nw, const double *F, const
 #undef BB
 }
 
-/* apply_filter_2d + irfft + calc_misfit_kernel (kernel.f90:410-510, 946-956) */
-static double misfit_value(ora_ctx *C, int k, const double *conv_fd, double *work)
+/* apply_filter_2d + irfft + calc_misfit_kernel (kernel.f90:410-510, 946-956) with the
+ * transfer functions tf (forward) and tb (backward), [nomega][2] */
+static double misfit_value_tf(ora_ctx *C, int k, const double *conv_fd, const double *tf,
+                              const double *tb, double *work)
 {
     ora_kernel *K = &C->K[k];
     int N = C->N, nw = C->nomega, i;
-    const double *tf = C->tf_fwd + (size_t)K->filter * 2 * nw, *tb = C->tf_bwd + (size_t)K->filter * 2 * nw;
     double *X = work, *x = work + 2 * nw, *pad, *af, val;
     for (i = 0; i < nw; ++i) {
         cz a = {conv_fd[2 * i], conv_fd[2 * i + 1]}, f = {tf[2 * i], tf[2 * i + 1]}, b = {tb[2 * i], tb[2 * i + 1]};
@@ -711,6 +729,12 @@ static double misfit_value(ora_ctx *C, int k, const double *conv_fd, double *wor
         free(pad); free(af);
     }
     return val * K->normalization * (-1.0);
+}
+
+static double misfit_value(ora_ctx *C, int k, const double *conv_fd, double *work)
+{
+    int nw = C->nomega, f = C->K[k].filter;
+    return misfit_value_tf(C, k, conv_fd, C->tf_fwd + (size_t)f * 2 * nw, C->tf_bwd + (size_t)f * 2 * nw, work);
 }
 
 /* model coefficients at radius r (interpolate_mesh.f90:98-109, background_model.f90:122-198) */
@@ -1015,4 +1039,242 @@ out:
     *halton_step = C->halton_step;
     free(pts); free(hal); free(base); free(phys); free(F); free(niter);
     return rc;
+}
+
+/* ------------------------------------------------------------------------------------
+ * The filter-convolve-window-integrate stage as a function of the filter responses G
+ * and the strain traces, and its adjoint.  The stage has no counterpart in the reference;
+ * its forward is the receiver-loop body of slave_work (slave.f90:515-534, 586-640:
+ * cumsum_trapezoidal and rfft of the tapered fwd traces, rfft of the tapered bwd traces,
+ * calc_basekernel kernel.f90:514-570, apply_filter_2d kernel.f90:946-956, irfft,
+ * calc_misfit_kernel kernel.f90:410-510) with TF_fwd = G rest_f, TF_bwd = G rest_b, where
+ * G is a filter's response (filtering.f90:78-230) and rest_* the deconvolution factors of
+ * add_stfs (filtering.f90:241-470; 1 without deconvolution).  The filtered seismogram
+ * windows and normalisations are those of the set-up (kernel.f90:181-385), constants of
+ * the stage.  The adjoint runs the same steps backwards in FP64.
+ * ---------------------------------------------------------------------------------- */
+
+/* G of every filter [nfilter][nomega][2] */
+void ora_ad_transfer(ora_ctx *C, double *G)
+{
+    memcpy(G, C->tf_g, (size_t)C->S.nfilter * 2 * C->nomega * sizeof(double));
+}
+
+static void ad_tf(ora_ctx *C, const double *G, int f, double *ta, double *tb)
+{
+    int k, nw = C->nomega;
+    for (k = 0; k < nw; ++k) {
+        cz g = {G[((size_t)f * nw + k) * 2], G[((size_t)f * nw + k) * 2 + 1]};
+        cz a = {C->rest_f[2 * k], C->rest_f[2 * k + 1]}, b = {C->rest_b[2 * k], C->rest_b[2 * k + 1]};
+        a = zmul(g, a); b = zmul(g, b);
+        ta[2 * k] = a.re; ta[2 * k + 1] = a.im;
+        tb[2 * k] = b.re; tb[2 * k + 1] = b.im;
+    }
+}
+
+/* F [npts][nd][nomega][2] of fw [npts][nd][nt], B [npts][nrec][nd][nomega][2] of bw */
+static void ad_spectra(ora_ctx *C, int npts, const double *fw, const double *bw, double *F, double *B)
+{
+    const mck_setup_t *S = &C->S;
+    int nt = S->nt, nd = C->nd, i;
+    double *tmp = (double *)malloc((size_t)npts * nd * nt * sizeof(double));
+    for (i = 0; i < npts * nd; ++i) ora_cumsum_trapezoidal(nt, S->dt, fw + (size_t)i * nt, tmp + (size_t)i * nt);
+    fields_to_fd(C, npts * nd, tmp, F, S->deconv_stf ? NULL : C->ts_fwd);
+    fields_to_fd(C, npts * S->nrec * nd, bw, B, S->deconv_stf ? NULL : C->ts_bwd);
+    free(tmp);
+}
+
+int ora_ad_forward(ora_ctx *C, int npts, const double *G, const double *fw, const double *bw,
+                   double *base)
+{
+    const mck_setup_t *S = &C->S;
+    int nd = C->nd, nw = C->nomega, nk = S->nkernel, R = S->nrec, p, r, b, k;
+    size_t fs = (size_t)nd * nw * 2;
+    double *F = (double *)malloc((size_t)npts * fs * sizeof(double));
+    double *B = (double *)malloc((size_t)npts * R * fs * sizeof(double));
+    double *conv = (double *)malloc((size_t)nw * 2 * sizeof(double));
+    double *ta = (double *)malloc((size_t)nw * 4 * sizeof(double)), *tb = ta + 2 * nw;
+    double *work = (double *)malloc(((size_t)2 * nw + C->N) * sizeof(double));
+    memset(base, 0, (size_t)npts * nk * 6 * sizeof(double));
+    ad_spectra(C, npts, fw, bw, F, B);
+    for (p = 0; p < npts; ++p)
+        for (r = 0; r < R; ++r)
+            for (b = 0; b < 6; ++b) {
+                if (!C->R[r].needs_bk[b]) continue;
+                for (k = C->R[r].first; k <= C->R[r].last; ++k) {
+                    if (!C->K[k].needs_bk[b]) continue;
+                    basekernel(b, C->full, nd, nw, F + p * fs, B + ((size_t)p * R + r) * fs, conv);
+                    ad_tf(C, G, C->K[k].filter, ta, tb);
+                    base[((size_t)p * nk + k) * 6 + b] = misfit_value_tf(C, k, conv, ta, tb, work);
+                }
+            }
+    free(F); free(B); free(conv); free(ta); free(work);
+    return 0;
+}
+
+/* gradient of one calc_basekernel product (holomorphic in F and B): gF_c += g conj(dbk/dF_c),
+ * gB_c += g conj(dbk/dB_c) at bin q; F, B, gF, gB [nd][nomega][2] */
+static void basekernel_adjoint(int ibk, int full, int nd, int nw, int q, const double *F,
+                               const double *B, cz g, double *gF, double *gB)
+{
+    cz dF[6], dB[6];
+    int c;
+#define FF(c) (cz){F[((c) * nw + q) * 2], F[((c) * nw + q) * 2 + 1]}
+#define BB(c) (cz){B[((c) * nw + q) * 2], B[((c) * nw + q) * 2 + 1]}
+    for (c = 0; c < 6; ++c) { dF[c].re = dF[c].im = dB[c].re = dB[c].im = 0.0; }
+    if ((ibk == 0 && !full) || ibk == 1) {
+        double s = ibk == 0 ? -1.0 : -2.0;
+        for (c = 0; c < nd; ++c) {
+            dF[c].re = s * BB(c).re; dF[c].im = s * BB(c).im;
+            dB[c].re = s * FF(c).re; dB[c].im = s * FF(c).im;
+        }
+    } else if (ibk == 0) {
+        cz fs = {-(FF(0).re + FF(1).re + FF(2).re), -(FF(0).im + FF(1).im + FF(2).im)};
+        cz bs = {-(BB(0).re + BB(1).re + BB(2).re), -(BB(0).im + BB(1).im + BB(2).im)};
+        for (c = 0; c < 3; ++c) { dF[c] = bs; dB[c] = fs; }
+    } else if (ibk == 3) {
+        cz fs = {FF(0).re + FF(1).re, FF(0).im + FF(1).im}, bs = {BB(0).re + BB(1).re, BB(0).im + BB(1).im};
+        dF[0] = dF[1] = bs; dB[0] = dB[1] = fs;
+    } else if (ibk == 4) {
+        for (c = 3; c < 5; ++c) {
+            dF[c].re = 4.0 * BB(c).re; dF[c].im = 4.0 * BB(c).im;
+            dB[c].re = 4.0 * FF(c).re; dB[c].im = 4.0 * FF(c).im;
+        }
+    } else if (ibk == 5) {
+        /* (F0 + F1) B2 + (B0 B1) F2 */
+        dF[0] = dF[1] = BB(2);
+        dF[2] = zmul(BB(0), BB(1));
+        dB[2].re = FF(0).re + FF(1).re; dB[2].im = FF(0).im + FF(1).im;
+        dB[0] = zmul(BB(1), FF(2));
+        dB[1] = zmul(BB(0), FF(2));
+    }
+    for (c = 0; c < nd; ++c) {
+        cz a = {dF[c].re, -dF[c].im}, b = {dB[c].re, -dB[c].im};
+        a = zmul(g, a); b = zmul(g, b);
+        gF[(c * nw + q) * 2] += a.re; gF[(c * nw + q) * 2 + 1] += a.im;
+        gB[(c * nw + q) * 2] += b.re; gB[(c * nw + q) * 2 + 1] += b.im;
+    }
+#undef FF
+#undef BB
+}
+
+/* adjoint of x -> rfft(x) (length N, scaled by dt) for real x, restricted to t < nt:
+ * xbar[t] = dt sum_q Re(g_q exp(+2 pi i q t / N)), g [nomega][2] gradient w.r.t. X */
+static void rfft_adjoint(ora_ctx *C, const double *g, int nt, double *xbar)
+{
+    int N = C->N, nw = C->nomega, k;
+    double *X = (double *)malloc((size_t)nw * 2 * sizeof(double)), *y = (double *)malloc(N * sizeof(double));
+    for (k = 0; k < nw; ++k) {
+        double h = (k == 0 || k == nw - 1) ? 1.0 : 0.5;
+        X[2 * k] = g[2 * k] * h;
+        X[2 * k + 1] = g[2 * k + 1] * h;
+    }
+    ora_irfft(N, 1.0, X, y);
+    for (k = 0; k < nt; ++k) xbar[k] = y[k] * C->S.dt;
+    free(X); free(y);
+}
+
+int ora_ad_backward(ora_ctx *C, int npts, const double *G, const double *fw, const double *bw,
+                    const double *gbase, double *gG, double *gfw, double *gbw)
+{
+    const mck_setup_t *S = &C->S;
+    int nd = C->nd, nw = C->nomega, nk = S->nkernel, R = S->nrec, N = C->N, nt = S->nt;
+    int p, r, b, k, q, c, n, t;
+    size_t fs = (size_t)nd * nw * 2;
+    double *F = (double *)malloc((size_t)npts * fs * sizeof(double));
+    double *B = (double *)malloc((size_t)npts * R * fs * sizeof(double));
+    double *gF = (double *)calloc(fs, sizeof(double)), *gB = (double *)calloc(fs, sizeof(double));
+    double *conv = (double *)malloc((size_t)nw * 2 * sizeof(double));
+    double *ta = (double *)malloc((size_t)nw * 4 * sizeof(double)), *tb = ta + 2 * nw;
+    double *xbar = (double *)malloc(N * sizeof(double)), *Xb = (double *)malloc((size_t)nw * 2 * sizeof(double));
+    double *g1 = (double *)malloc((size_t)nw * 2 * sizeof(double));
+    double *win = (double *)malloc(N * sizeof(double)), *one = (double *)malloc(nt * sizeof(double));
+    double *tr = (double *)malloc(nt * sizeof(double)), *gcs = (double *)malloc(nt * sizeof(double));
+    for (t = 0; t < nt; ++t) one[t] = 1.0;
+    ora_taperandzeropad(nt, N, 10, 0, one, win);        /* the taper of fields_to_fd */
+    memset(gG, 0, (size_t)S->nfilter * nw * 2 * sizeof(double));
+    ad_spectra(C, npts, fw, bw, F, B);
+    for (p = 0; p < npts; ++p) {
+        memset(gF, 0, fs * sizeof(double));
+        for (r = 0; r < R; ++r) {
+            const double *Fp = F + p * fs, *Bp = B + ((size_t)p * R + r) * fs;
+            memset(gB, 0, fs * sizeof(double));
+            for (b = 0; b < 6; ++b) {
+                if (!C->R[r].needs_bk[b]) continue;
+                for (k = C->R[r].first; k <= C->R[r].last; ++k) {
+                    ora_kernel *K = &C->K[k];
+                    double gv = gbase[((size_t)p * nk + k) * 6 + b];
+                    const double *rf = C->rest_f, *rb = C->rest_b;
+                    double *gGf = gG + (size_t)K->filter * nw * 2;
+                    if (!K->needs_bk[b] || gv == 0.0) continue;
+                    basekernel(b, C->full, nd, nw, Fp, Bp, conv);
+                    ad_tf(C, G, K->filter, ta, tb);
+                    /* value = -norm dt sum_n w_n x(ifirst + n) s(n): the Parseval integral of
+                     * the zero-padded window (integrate_parseval_b_complex) and the
+                     * trapezoidal rule (kernel.f90:482-491) are both this weighted sum */
+                    memset(xbar, 0, N * sizeof(double));
+                    for (n = 0; n < K->ncut; ++n) {
+                        double w = 1.0;
+                        if (S->int_scheme == MCK_INT_TRAPEZOIDAL) {
+                            w = (n > 0 && n < K->ncut - 1) ? 1.0 : 0.0;
+                            if (n == 0) w += 0.5;
+                            if (n == K->ncut - 1) w += 0.5;
+                        }
+                        xbar[K->ifirst + n] = -gv * K->normalization * S->dt * w * K->seis_cut[n];
+                    }
+                    /* x = df c2r(X): dL/dX_q = df c_q sum_t xbar_t e^{-2 pi i q t/N}, the
+                     * imaginary parts at DC and Nyquist are discarded by the c2r */
+                    ora_rfft(N, 1.0, xbar, Xb);
+                    for (q = 0; q < nw; ++q) {
+                        double cq = (q == 0 || q == nw - 1) ? 1.0 : 2.0;
+                        cz gX = {C->df * cq * Xb[2 * q], C->df * cq * Xb[2 * q + 1]};
+                        cz cv = {conv[2 * q], conv[2 * q + 1]};
+                        cz a = {ta[2 * q], ta[2 * q + 1]}, bb = {tb[2 * q], tb[2 * q + 1]};
+                        cz ab, cb, ca, gc, gta, gtb, t1, t2;
+                        if (q == 0 || q == nw - 1) gX.im = 0.0;
+                        ab = zmul(a, bb); ab.im = -ab.im;
+                        cb = zmul(cv, bb); cb.im = -cb.im;
+                        ca = zmul(cv, a); ca.im = -ca.im;
+                        gc = zmul(gX, ab);
+                        gta = zmul(gX, cb);
+                        gtb = zmul(gX, ca);
+                        t1 = zmul(gta, (cz){rf[2 * q], -rf[2 * q + 1]});
+                        t2 = zmul(gtb, (cz){rb[2 * q], -rb[2 * q + 1]});
+                        gGf[2 * q] += t1.re + t2.re;
+                        gGf[2 * q + 1] += t1.im + t2.im;
+                        basekernel_adjoint(b, C->full, nd, nw, q, Fp, Bp, gc, gF, gB);
+                    }
+                }
+            }
+            /* B = rfft(taper bw) ts_bwd */
+            for (c = 0; c < nd; ++c) {
+                for (q = 0; q < nw; ++q) {
+                    cz g = {gB[(c * nw + q) * 2], gB[(c * nw + q) * 2 + 1]};
+                    if (!S->deconv_stf) g = zmul(g, (cz){C->ts_bwd[2 * q], -C->ts_bwd[2 * q + 1]});
+                    g1[2 * q] = g.re; g1[2 * q + 1] = g.im;
+                }
+                rfft_adjoint(C, g1, nt, tr);
+                for (t = 0; t < nt; ++t) gbw[(((size_t)p * R + r) * nd + c) * nt + t] = tr[t] * win[t];
+            }
+        }
+        /* F = rfft(taper cumsum_trapezoidal(fw)) ts_fwd */
+        for (c = 0; c < nd; ++c) {
+            double *o = gfw + ((size_t)p * nd + c) * nt;
+            for (q = 0; q < nw; ++q) {
+                cz g = {gF[(c * nw + q) * 2], gF[(c * nw + q) * 2 + 1]};
+                if (!S->deconv_stf) g = zmul(g, (cz){C->ts_fwd[2 * q], -C->ts_fwd[2 * q + 1]});
+                g1[2 * q] = g.re; g1[2 * q + 1] = g.im;
+            }
+            rfft_adjoint(C, g1, nt, tr);
+            /* cumsum_trapezoidal backwards: out_i = h c_i, c_0 = x_0, c_1 = c_0 + x_1,
+             * c_i = c_{i-1} + x_{i-1} + x_i */
+            for (t = 0; t < nt; ++t) { gcs[t] = tr[t] * win[t] * S->dt * 0.5; o[t] = 0.0; }
+            for (t = nt - 1; t >= 2; --t) { gcs[t - 1] += gcs[t]; o[t - 1] += gcs[t]; o[t] += gcs[t]; }
+            if (nt > 1) { gcs[0] += gcs[1]; o[1] += gcs[1]; }
+            o[0] += gcs[0];
+        }
+    }
+    free(F); free(B); free(gF); free(gB); free(conv); free(ta); free(xbar); free(Xb); free(g1);
+    free(win); free(one); free(tr); free(gcs);
+    return 0;
 }

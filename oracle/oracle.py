@@ -68,6 +68,9 @@ def lib():
         L.ora_cut_window.argtypes = [C.c_int, f64p, C.c_double, C.c_double, C.POINTER(C.c_int),
                                      C.POINTER(C.c_int)]
         L.ora_rotate_voigt.argtypes = [C.c_int, f64p, C.c_int, C.c_double, C.c_double]
+        L.ora_ad_transfer.argtypes = [C.c_void_p, f64p]
+        L.ora_ad_forward.argtypes = [C.c_void_p, C.c_int, f64p, f64p, f64p, f64p]
+        L.ora_ad_backward.argtypes = [C.c_void_p, C.c_int, f64p, f64p, f64p, f64p, f64p, f64p, f64p]
         L.ora_mc_run.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                  f64p, f64p, f64p, i32p]
         _LIB = L
@@ -259,6 +262,10 @@ def mc_run(vals, volume, allowed_error=0.0, allowed_rel=0.0):
     return I, V, cv.astype(bool)
 
 
+def ora_nomega(nt):
+    return lib().ora_nextpow2(nt) + 1
+
+
 class Oracle:
     """Whole-path oracle for one MckSetup (the Problem must stay alive)."""
 
@@ -269,6 +276,9 @@ class Oracle:
             raise RuntimeError("oracle setup failed: " + err.value.decode())
         self.nk = setup.nkernel
         self.nt = setup.nt
+        self.nrec = setup.nrec
+        self.nfilter = setup.nfilter
+        self.nomega = ora_nomega(setup.nt)
         self.nd = lib().ora_strain_nd(self.h)
         self.halton_step = 0
 
@@ -335,3 +345,40 @@ class Oracle:
         self.halton_step = int(step[0])
         return dict(values=vals, variance=var, niterations=nit, offsets=used,
                     evaluations=int(ev[0]), halton_step=int(step[0]))
+
+    # the filter-convolve-window-integrate stage (G complex [nfilter][nomega], fw
+    # [npts][nd][nt], bw [npts][nrec][nd][nt]) and its FP64 adjoint
+    def ad_transfer(self):
+        G = np.zeros((self.nfilter, self.nomega, 2))
+        lib().ora_ad_transfer(self.h, P(G))
+        return G[..., 0] + 1j * G[..., 1]
+
+    def _ad_inputs(self, G, fw, bw):
+        G = np.ascontiguousarray(np.stack([np.real(G), np.imag(G)], -1), float)
+        fw = np.ascontiguousarray(fw, float)
+        bw = np.ascontiguousarray(bw, float)
+        assert G.shape == (self.nfilter, self.nomega, 2)
+        assert fw.shape[1:] == (self.nd, self.nt) and bw.shape[1:] == (self.nrec, self.nd, self.nt)
+        assert bw.shape[0] == fw.shape[0]
+        return G, fw, bw
+
+    def ad_forward(self, G, fw, bw):
+        G, fw, bw = self._ad_inputs(G, fw, bw)
+        out = np.zeros((fw.shape[0], self.nk, 6))
+        rc = lib().ora_ad_forward(self.h, fw.shape[0], P(G), P(fw), P(bw), P(out))
+        if rc:
+            raise RuntimeError("oracle ad_forward failed (%d)" % rc)
+        return out
+
+    def ad_backward(self, G, fw, bw, gbase):
+        G, fw, bw = self._ad_inputs(G, fw, bw)
+        gbase = np.ascontiguousarray(gbase, float)
+        assert gbase.shape == (fw.shape[0], self.nk, 6)
+        gG = np.zeros_like(G)
+        gfw = np.zeros_like(fw)
+        gbw = np.zeros_like(bw)
+        rc = lib().ora_ad_backward(self.h, fw.shape[0], P(G), P(fw), P(bw), P(gbase), P(gG),
+                                   P(gfw), P(gbw))
+        if rc:
+            raise RuntimeError("oracle ad_backward failed (%d)" % rc)
+        return gG[..., 0] + 1j * gG[..., 1], gfw, gbw
