@@ -184,7 +184,9 @@ int mck_init(const mck_setup_t *setup);
  *  elem_type       4 = tetrahedron (inversion_mesh.f90:766-805 ielem_type); others rejected
  *  nvpe            vertices per element (4)
  *  nelems          elements in this task
- *  nbasis          basis functions per element (1 = volumetric)
+ *  nbasis          basis functions per element: 1 = volumetric, 4 = linear functions on
+ *                  the tetrahedron's vertices (inversion_mesh.f90:530-622, one Monte Carlo
+ *                  integrator per basis function, slave.f90:421, 489-497, 723-744)
  *  vertices        [nvertices][3] metres (Fortran vertices(3, nvertices))
  *  conn            [nelems][nvpe] 1-based vertex ids (Fortran connectivity(nvpe, nelems));
  *                  elements whose first id is <= 0 are skipped (empty slots of a last task)

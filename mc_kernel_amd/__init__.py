@@ -81,19 +81,22 @@ class MCKernel:
             self.lib.mck_finalize()
             self.lib = None
 
-    def slave_work(self, verts, conn, halton_offsets=None, plain=False):
-        """mck_slave_work for a tetrahedral task; returns a dict of numpy arrays."""
+    def slave_work(self, verts, conn, halton_offsets=None, plain=False, nbasis=1):
+        """mck_slave_work for a tetrahedral task; returns a dict of numpy arrays.
+        values/variance are (ne, nk) for volumetric elements (nbasis 1) and
+        (ne, 4, nk) for basis functions on the vertices (nbasis 4)."""
         verts = np.ascontiguousarray(verts, np.float64)
         conn = np.ascontiguousarray(conn, np.int32)
         ne = conn.shape[0]
-        vals = np.zeros((ne, self.nk))
-        var = np.zeros((ne, self.nk))
+        shape = (ne, self.nk) if nbasis == 1 else (ne, nbasis, self.nk)
+        vals = np.zeros(shape)
+        var = np.zeros(shape)
         nit = np.zeros((ne, self.nk), np.int32)
         tm = np.zeros(ne)
         step = np.array([self.halton_step], np.int64)
         offs = None if halton_offsets is None else np.ascontiguousarray(halton_offsets, np.int64)
         fn = self.lib.mck_slave_work_plain if plain else self.lib.mck_slave_work
-        _check(fn(4, 4, ne, 1, verts.shape[0], abi.ptr(verts, C.c_double),
+        _check(fn(4, 4, ne, nbasis, verts.shape[0], abi.ptr(verts, C.c_double),
                   abi.ptr(conn, C.c_int32), abi.ptr(vals, C.c_double), abi.ptr(var, C.c_double),
                   abi.ptr(nit, C.c_int32), abi.ptr(tm, C.c_double), abi.ptr(step, C.c_int64),
                   abi.ptr(offs, C.c_int64)))

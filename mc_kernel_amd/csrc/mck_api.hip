@@ -338,6 +338,8 @@ struct State {
     double src_r[3];
     // per-task and per-chunk buffers
     DBuf<double> tet, volume, allowed_var;
+    int nbasis = 1;                          // basis functions per element of the task
+    DBuf<double> abinv, bweights;            // [nvox][4][4], [step points][4] (nbasis 4)
     DBuf<int64_t> hbase;
     DBuf<uint8_t> masked, live, working, active;
     DBuf<int> work_list, nwork;
@@ -1016,8 +1018,35 @@ bool point_in_tet(const double *v, const double *p) {
     return true;
 }
 
+// init_weight_tet_mesh (inversion_mesh.f90:1086-1116): inverse of the matrix with columns
+// (v_j - v_1, 1), j = 1..4, by cofactors; A[4 i + j] = inv(i, j), so that the barycentric
+// coordinate i of x is sum_j inv(i, j) (x - v_1, 1)_j
+void tet_barycentric_inverse(const double *v, double *A) {
+    double m[4][4];
+    for (int c = 0; c < 4; ++c) {
+        for (int r = 0; r < 3; ++r) m[r][c] = v[3 * c + r] - v[r];
+        m[3][c] = 1.0;
+    }
+    auto minor3 = [&](int r0, int c0) {
+        double t[9];
+        int n = 0;
+        for (int r = 0; r < 4; ++r)
+            for (int c = 0; c < 4; ++c)
+                if (r != r0 && c != c0) t[n++] = m[r][c];
+        return t[0] * (t[4] * t[8] - t[5] * t[7]) - t[1] * (t[3] * t[8] - t[5] * t[6])
+             + t[2] * (t[3] * t[7] - t[4] * t[6]);
+    };
+    double cof[4][4], det = 0.0;
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) cof[r][c] = (((r + c) & 1) ? -1.0 : 1.0) * minor3(r, c);
+    for (int c = 0; c < 4; ++c) det += m[0][c] * cof[0][c];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) A[4 * i + j] = cof[j][i] / det;
+}
+
 struct Task {
-    int nelems = 0, P = 0;
+    int nelems = 0, P = 0, nbasis = 1;
+    std::vector<double> abinv;   // [ne][4][4] (nbasis 4)
     std::vector<double> tet;     // [ne][4][3]
     std::vector<double> vol, allowed_var;
     std::vector<uint8_t> masked, live;
@@ -1029,9 +1058,12 @@ int prepare_task(State &G, Task &T, int elem_type, int nvpe, int nelems, int nba
                  const int64_t *halton_offsets) {
     const mck_setup_t &S = G.S;
     if (elem_type != 4 || nvpe != 4) return fail(MCK_EUNSUPPORTED, "only tetrahedral inversion meshes are supported");
-    if (nbasis != 1) return fail(MCK_EUNSUPPORTED, "only volumetric basis functions are supported");
+    if (nbasis != 1 && nbasis != 4)
+        return fail(MCK_EINVAL, "tetrahedra have 1 (volumetric) or 4 (on-vertex) basis functions");
     if (nelems <= 0) return fail(MCK_EINVAL, "empty task");
     T.nelems = nelems;
+    T.nbasis = nbasis;
+    if (nbasis > 1) T.abinv.assign((size_t)nelems * 16, 0.0);
     T.P = S.npoints_per_step;
     T.tet.assign((size_t)nelems * 12, 0.0);
     T.vol.assign(nelems, 1.0);
@@ -1048,6 +1080,7 @@ int prepare_task(State &G, Task &T, int elem_type, int nvpe, int nelems, int nba
             for (int c = 0; c < 3; ++c) T.tet[(size_t)e * 12 + 3 * i + c] = vertices[3 * vid + c];
         }
         const double *v = &T.tet[(size_t)e * 12];
+        if (nbasis > 1) tet_barycentric_inverse(v, &T.abinv[(size_t)e * 16]);
         // element containing the source: no integration, zero results (slave.f90:466-482)
         if (S.mask_src_rec && point_in_tet(v, G.src_r)) { T.live[e] = 0; continue; }
         double vol = S.int_over_volume ? tet_volume(v) : 1.0;
@@ -1067,8 +1100,13 @@ int prepare_task(State &G, Task &T, int elem_type, int nvpe, int nelems, int nba
         }
         T.hbase[e] = halton_offsets ? halton_offsets[e] : (*halton_step + (int64_t)e * stride);
     }
-    const int ne = nelems, nk = G.nk;
+    const int ne = nelems, nk = G.nk, nbk = nelems * nbasis;
+    G.nbasis = nbasis;
     HIPCHECK(G.tet.ensure((size_t)ne * 12));
+    if (nbasis > 1) {
+        HIPCHECK(G.abinv.ensure((size_t)ne * 16));
+        HIPCHECK(hipMemcpyAsync(G.abinv.p, T.abinv.data(), (size_t)ne * 16 * 8, hipMemcpyHostToDevice, G.stream));
+    }
     HIPCHECK(G.volume.ensure(ne));
     HIPCHECK(G.allowed_var.ensure(ne));
     HIPCHECK(G.hbase.ensure(ne));
@@ -1077,11 +1115,11 @@ int prepare_task(State &G, Task &T, int elem_type, int nvpe, int nelems, int nba
     HIPCHECK(G.working.ensure(ne));
     HIPCHECK(G.work_list.ensure(ne));
     HIPCHECK(G.active.ensure((size_t)ne * G.R));
-    HIPCHECK(G.fsum.ensure((size_t)ne * nk));
-    HIPCHECK(G.f2sum.ensure((size_t)ne * nk));
-    HIPCHECK(G.integral.ensure((size_t)ne * nk));
-    HIPCHECK(G.variance.ensure((size_t)ne * nk));
-    HIPCHECK(G.conv.ensure((size_t)ne * nk));
+    HIPCHECK(G.fsum.ensure((size_t)nbk * nk));
+    HIPCHECK(G.f2sum.ensure((size_t)nbk * nk));
+    HIPCHECK(G.integral.ensure((size_t)nbk * nk));
+    HIPCHECK(G.variance.ensure((size_t)nbk * nk));
+    HIPCHECK(G.conv.ensure((size_t)nbk * nk));
     HIPCHECK(G.niter.ensure((size_t)ne * nk));
     HIPCHECK(G.nmodels.ensure(ne));
     HIPCHECK(G.draws.ensure(ne));
@@ -1094,11 +1132,11 @@ int prepare_task(State &G, Task &T, int elem_type, int nvpe, int nelems, int nba
     HIPCHECK(hipMemcpyAsync(G.hbase.p, T.hbase.data(), ne * 8, hipMemcpyHostToDevice, st));
     HIPCHECK(hipMemcpyAsync(G.masked.p, T.masked.data(), T.masked.size(), hipMemcpyHostToDevice, st));
     HIPCHECK(hipMemcpyAsync(G.live.p, T.live.data(), ne, hipMemcpyHostToDevice, st));
-    HIPCHECK(hipMemsetAsync(G.fsum.p, 0, (size_t)ne * nk * 8, st));
-    HIPCHECK(hipMemsetAsync(G.f2sum.p, 0, (size_t)ne * nk * 8, st));
-    HIPCHECK(hipMemsetAsync(G.integral.p, 0, (size_t)ne * nk * 8, st));
-    HIPCHECK(hipMemsetAsync(G.variance.p, 0, (size_t)ne * nk * 8, st));
-    HIPCHECK(hipMemsetAsync(G.conv.p, 0, (size_t)ne * nk, st));
+    HIPCHECK(hipMemsetAsync(G.fsum.p, 0, (size_t)nbk * nk * 8, st));
+    HIPCHECK(hipMemsetAsync(G.f2sum.p, 0, (size_t)nbk * nk * 8, st));
+    HIPCHECK(hipMemsetAsync(G.integral.p, 0, (size_t)nbk * nk * 8, st));
+    HIPCHECK(hipMemsetAsync(G.variance.p, 0, (size_t)nbk * nk * 8, st));
+    HIPCHECK(hipMemsetAsync(G.conv.p, 0, (size_t)nbk * nk, st));
     HIPCHECK(hipMemsetAsync(G.niter.p, 0, (size_t)ne * nk * 4, st));
     HIPCHECK(hipMemsetAsync(G.nmodels.p, 0, ne * 8, st));
     HIPCHECK(hipMemsetAsync(G.draws.p, 0, ne * 4, st));
@@ -1125,6 +1163,7 @@ int mark_keys(State &G, int npairs, const Located *loc, int R, int P, const uint
 
 VoxState vox_state(State &G) {
     VoxState v;
+    v.nbasis = G.nbasis;
     v.fsum = G.fsum.p; v.f2sum = G.f2sum.p; v.integral = G.integral.p; v.variance = G.variance.p;
     v.conv = G.conv.p; v.niter = G.niter.p; v.nmodels = G.nmodels.p; v.draws = G.draws.p;
     v.done = G.done.p; v.steps = G.steps.p;
@@ -1142,7 +1181,9 @@ int eval_chunk(State &G, int P, int w0, int nw, std::vector<std::pair<hipEvent_t
     if ((rc = ensure_chunk(G, np))) return rc;
     Located *loc_f = G.loc.p, *loc_b = G.loc.p + np;
     hipLaunchKernelGGL(k_points, dim3((np + 255) / 256), dim3(256), 0, st, nw, P, wl, G.hbase.p,
-                       G.draws.p, G.tet.p, G.S.model_ndepth, G.S.model_dr, G.xyz.p, G.model_idx.p);
+                       G.draws.p, G.tet.p, G.S.model_ndepth, G.S.model_dr, G.xyz.p, G.model_idx.p,
+                       G.nbasis, G.abinv.p, G.nbasis > 1 ? G.bweights.p + (size_t)w0 * P * 4 : nullptr,
+                       G.err.p);
     HIPCHECK(hipGetLastError());
     // fwd and bwd meshes may differ: locate frame 0 in the fwd mesh, frames 1.. in the bwd mesh
     hipLaunchKernelGGL(k_locate, dim3((unsigned)((np + 127) / 128)), dim3(128), 0, st, G.fwd.d,
@@ -1218,12 +1259,13 @@ int mc_step(State &G, const Task &T, int *nwork_out, std::vector<std::pair<hipEv
                        G.work_list.p, G.masked.p, G.d_rec_first.p, G.d_rec_count.p, vs, G.active.p);
     HIPCHECK(hipGetLastError());
     HIPCHECK(G.phys.ensure((size_t)nwork * P * nk));
+    if (G.nbasis > 1) HIPCHECK(G.bweights.ensure((size_t)nwork * P * 4));
     const int wc = std::max(1, G.chunk_points / P);
     int rc;
     for (int w0 = 0; w0 < nwork; w0 += wc)
         if ((rc = eval_chunk(G, P, w0, std::min(wc, nwork - w0), evs))) return rc;
     hipLaunchKernelGGL(k_mc, dim3(nwork), dim3(256), 0, st, nwork, P, nk, R, G.work_list.p, G.phys.p,
-                       G.masked.p, G.active.p, G.d_k_rec.p, G.d_k_nbk.p, G.volume.p, G.allowed_var.p,
+                       G.nbasis > 1 ? G.bweights.p : nullptr, G.masked.p, G.active.p, G.d_k_rec.p, G.d_k_nbk.p, G.volume.p, G.allowed_var.p,
                        G.S.allowed_relative_error, vs, G.evals.p, G.err.p);
     HIPCHECK(hipGetLastError());
     G.stats.points += (int64_t)nwork * P;
@@ -1233,14 +1275,14 @@ int mc_step(State &G, const Task &T, int *nwork_out, std::vector<std::pair<hipEv
 
 int finish_task(State &G, const Task &T, double *values, double *variance, int32_t *niterations,
                 double *ctime, int64_t *halton_step, const int64_t *halton_offsets, double dev_ms) {
-    const int ne = T.nelems, nk = G.nk;
+    const int ne = T.nelems, nk = G.nk, nb = T.nbasis;
     hipStream_t st = G.stream;
     std::vector<int> draws(ne), steps(ne);
     unsigned long long ev = 0, cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int err = 0;
     HIPCHECK(hipMemcpyAsync(cnt, G.elem_count.p, 8 * 8, hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipMemcpyAsync(values, G.integral.p, (size_t)ne * nk * 8, hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipMemcpyAsync(variance, G.variance.p, (size_t)ne * nk * 8, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(values, G.integral.p, (size_t)ne * nb * nk * 8, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(variance, G.variance.p, (size_t)ne * nb * nk * 8, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipMemcpyAsync(niterations, G.niter.p, (size_t)ne * nk * 4, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipMemcpyAsync(draws.data(), G.draws.p, ne * 4, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipMemcpyAsync(steps.data(), G.steps.p, ne * 4, hipMemcpyDeviceToHost, st));
@@ -1262,11 +1304,13 @@ int finish_task(State &G, const Task &T, double *values, double *variance, int32
     }
     if (err & 1) return fail(MCK_ENOTFOUND, "element not found: a point is outside the 12 nearest elements");
     if (err & 2) return fail(MCK_ENAN, "NaN in a Monte Carlo integral");
+    if (err & 4) return fail(MCK_EINVAL, "basis-function weight outside [0, 1]: point outside its element");
     long long tot_steps = 0;
     for (int e = 0; e < ne; ++e) tot_steps += steps[e];
     for (int e = 0; e < ne; ++e) {
         if (!T.live[e]) {
-            for (int k = 0; k < nk; ++k) { values[(size_t)e * nk + k] = 0; variance[(size_t)e * nk + k] = 0; niterations[(size_t)e * nk + k] = 0; }
+            for (int k = 0; k < nb * nk; ++k) { values[(size_t)e * nb * nk + k] = 0; variance[(size_t)e * nb * nk + k] = 0; }
+            for (int k = 0; k < nk; ++k) niterations[(size_t)e * nk + k] = 0;
             if (ctime) ctime[e] = 0.0;
             continue;
         }
@@ -1651,6 +1695,7 @@ void mck_finalize(void) {
     G.conv.release(); G.done.release(); G.niter.release(); G.draws.release(); G.steps.release();
     G.nmodels.release(); G.xyz.release(); G.model_idx.release(); G.loc.release();
     G.Fspec.release(); G.phys.release(); G.evals.release(); G.err.release();
+    G.abinv.release(); G.bweights.release();
     G.elem_mark.release(); G.elem_count.release(); G.keys_in.release(); G.keys_out.release();
     G.vals_in.release(); G.vals_out.release(); G.pair_count.release(); G.sort_tmp.release();
     G.ad_W.release(); G.ad_Hg.release(); G.ad_Fs.release(); G.ad_gF.release(); G.ad_U.release();

@@ -126,10 +126,12 @@ struct KernDesc {
     int slot[6];               // slot of each base kernel the parameter needs (-1 = none)
 };
 
-// Monte Carlo state of the voxels of one task
+// Monte Carlo state of the voxels of one task: one integrator per basis function
+// (1 volumetric, 4 on the vertices of a tetrahedron)
 struct VoxState {
-    double *fsum, *f2sum, *integral, *variance;   // [nvox][nk]
-    uint8_t *conv;                                // [nvox][nk]
+    int nbasis;
+    double *fsum, *f2sum, *integral, *variance;   // [nvox][nbasis][nk]
+    uint8_t *conv;                                // [nvox][nbasis][nk]
     int *niter;                                   // [nvox][nk]
     int64_t *nmodels;                             // [nvox]
     int *draws;                                   // [nvox] Halton draws so far

@@ -167,9 +167,12 @@ __device__ inline double d_norm2(double a, double b, double c)
 // ------------------------------------------------------------------------------------
 // k_points: one thread per MC point of the working voxels of this chunk
 // ------------------------------------------------------------------------------------
+// With nbasis 4 it also evaluates the vertex basis functions at the point (weights,
+// inversion_mesh.f90:530-622): bweights[point][4] from abinv [nvox][4][4].
 __global__ void k_points(int nwork, int P, const int *work_voxel, const int64_t *halton_base,
                          const int *voxel_draws, const double *tet /* [nvox][4][3] */,
-                         int ndepth, float dr, double *xyz, int *model_idx)
+                         int ndepth, float dr, double *xyz, int *model_idx, int nbasis,
+                         const double *abinv, double *bweights, int *err)
 {
     int gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= nwork * P) return;
@@ -220,6 +223,17 @@ __global__ void k_points(int nwork, int P, const int *work_voxel, const int64_t 
     int id = (int)(r / (double)dr + 1);
     id = id < 1 ? 1 : (id > ndepth ? ndepth : id);
     model_idx[gid] = id - 1;
+    if (nbasis > 1) {
+        const double *A = abinv + (size_t)v * 16;
+        const double dx = x[0] - V[0], dy = x[1] - V[1], dz = x[2] - V[2];
+        bool bad = false;
+        for (int b = 0; b < 4; ++b) {
+            const double w = A[4 * b] * dx + A[4 * b + 1] * dy + A[4 * b + 2] * dz + A[4 * b + 3] * 1.0;
+            bweights[(size_t)gid * 4 + b] = w;
+            bad |= (w < -1e-9 || w > 1.0 + 1e-9);
+        }
+        if (bad) atomicOr(err, 4);
+    }
 }
 
 // ------------------------------------------------------------------------------------
@@ -1255,10 +1269,10 @@ __global__ void k_step_begin(int nvox, int nk, int max_iter, VoxState st, const 
     working[v] = 0;
     if (st.done[v] || !live[v]) return;
     bool all = true, anymax = false;
-    for (int k = 0; k < nk; ++k) {
-        if (!st.conv[(size_t)v * nk + k]) all = false;
+    for (int i = 0; i < st.nbasis * nk; ++i)
+        if (!st.conv[(size_t)v * st.nbasis * nk + i]) all = false;
+    for (int k = 0; k < nk; ++k)
         if (st.niter[(size_t)v * nk + k] == max_iter) anymax = true;
-    }
     if (all) { st.done[v] = 1; return; }
     st.draws[v] += 1;
     if (anymax) { st.done[v] = 1; return; }
@@ -1287,7 +1301,8 @@ __global__ __launch_bounds__(1024) void k_compact(int n, const uint8_t *flag, in
     if (tid == 1023) *count = part[1023];
 }
 
-// receivers evaluated this step: not masked and not all converged (slave.f90:539-584)
+// receivers evaluated this step: not masked and not all converged in every basis
+// function (slave.f90:539-584, allallconverged mc_integration.f90:322-340)
 __global__ void k_receiver_active(int nwork, int R, int nk, const int *work_voxel,
                                   const uint8_t *masked, const int *rec_first, const int *rec_count,
                                   VoxState st, uint8_t *active)
@@ -1298,14 +1313,19 @@ __global__ void k_receiver_active(int nwork, int R, int nk, const int *work_voxe
     int v = work_voxel[w];
     uint8_t a = 0;
     if (!masked[(size_t)v * R + r]) {
-        for (int k = rec_first[r]; k < rec_first[r] + rec_count[r]; ++k)
-            if (!st.conv[(size_t)v * nk + k]) { a = 1; break; }
+        for (int b = 0; b < st.nbasis && !a; ++b)
+            for (int k = rec_first[r]; k < rec_first[r] + rec_count[r]; ++k)
+                if (!st.conv[((size_t)v * st.nbasis + b) * nk + k]) { a = 1; break; }
     }
     active[gid] = a;
 }
 
-// check_montecarlo_integral (mc_integration.f90:69-143) and niterations bookkeeping
+// check_montecarlo_integral (mc_integration.f90:69-143) of every basis function and the
+// niterations bookkeeping.  A kernel is evaluated while it is not converged in every
+// basis function (allisconverged, mc_integration.f90:343-359); basis function b sees the
+// physical kernel of the weighted base kernels, phys x w_b (slave.f90:723-744).
 __global__ void k_mc(int nwork, int P, int nk, int R, const int *work_voxel, const double *phys,
+                     const double *bweights /* [nwork * P][4], nbasis 4 only */,
                      const uint8_t *masked, const uint8_t *active, const int *k_rec,
                      const int *k_nbk, const double *volume, const double *allowed_var, double allowed_rel, VoxState st,
                      unsigned long long *evals, int *err)
@@ -1313,6 +1333,7 @@ __global__ void k_mc(int nwork, int P, int nk, int R, const int *work_voxel, con
     const int w = blockIdx.x;
     if (w >= nwork) return;
     const int v = work_voxel[w];
+    const int nb = st.nbasis;
     const int64_t n_old = st.nmodels[v];
     const int64_t n = n_old + P;
     const double V = volume[v];
@@ -1321,34 +1342,44 @@ __global__ void k_mc(int nwork, int P, int nk, int R, const int *work_voxel, con
         const size_t vk = (size_t)v * nk + k;
         const int r = k_rec[k];
         const bool msk = masked[(size_t)v * R + r] != 0;
-        const bool was_conv = st.conv[vk] != 0;
+        bool was_conv = true;
+        for (int b = 0; b < nb; ++b)
+            if (!st.conv[((size_t)v * nb + b) * nk + k]) was_conv = false;
         if (msk) st.niter[vk] += 1;
         else if (active[(size_t)w * R + r] && !was_conv) { st.niter[vk] += k_nbk[k]; my_evals += P; }
         if (was_conv) continue;
-        double s = 0.0, s2 = 0.0;
-        if (!msk) {
-            for (int p = 0; p < P; ++p) s += phys[((size_t)w * P + p) * nk + k];
-            for (int p = 0; p < P; ++p) {
-                double x = phys[((size_t)w * P + p) * nk + k];
-                s2 += x * x;
+        for (int b = 0; b < nb; ++b) {
+            const size_t vbk = ((size_t)v * nb + b) * nk + k;
+            if (st.conv[vbk]) continue;
+            double s = 0.0, s2 = 0.0;
+            if (!msk) {
+                for (int p = 0; p < P; ++p) {
+                    const double wt = nb > 1 ? bweights[((size_t)w * P + p) * 4 + b] : 1.0;
+                    s += phys[((size_t)w * P + p) * nk + k] * wt;
+                }
+                for (int p = 0; p < P; ++p) {
+                    const double wt = nb > 1 ? bweights[((size_t)w * P + p) * 4 + b] : 1.0;
+                    double x = phys[((size_t)w * P + p) * nk + k] * wt;
+                    s2 += x * x;
+                }
             }
+            double fs = st.fsum[vbk] + s, f2 = st.f2sum[vbk] + s2;
+            st.fsum[vbk] = fs;
+            st.f2sum[vbk] = f2;
+            double integral = fs * V / (double)n, variance;
+            bool conv = false;
+            if (n > 2) {
+                variance = V * V / (double)(n - 1) * ((f2 / (double)n) - (fs / (double)n) * (fs / (double)n));
+                if (variance < allowed_var[v]) conv = true;
+                else if (sqrt(variance) < allowed_rel * fabs(integral)) conv = true;
+            } else {
+                variance = integral * 1e10;
+            }
+            st.integral[vbk] = integral;
+            st.variance[vbk] = variance;
+            st.conv[vbk] = conv ? 1 : 0;
+            if (integral != integral || variance != variance) atomicOr(err, 2);
         }
-        double fs = st.fsum[vk] + s, f2 = st.f2sum[vk] + s2;
-        st.fsum[vk] = fs;
-        st.f2sum[vk] = f2;
-        double integral = fs * V / (double)n, variance;
-        bool conv = false;
-        if (n > 2) {
-            variance = V * V / (double)(n - 1) * ((f2 / (double)n) - (fs / (double)n) * (fs / (double)n));
-            if (variance < allowed_var[v]) conv = true;
-            else if (sqrt(variance) < allowed_rel * fabs(integral)) conv = true;
-        } else {
-            variance = integral * 1e10;
-        }
-        st.integral[vk] = integral;
-        st.variance[vk] = variance;
-        st.conv[vk] = conv ? 1 : 0;
-        if (integral != integral || variance != variance) atomicOr(err, 2);
     }
     // block sum of evaluations
     __shared__ unsigned long long red[16];

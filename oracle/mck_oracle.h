@@ -55,6 +55,7 @@ void ora_halton(int ndim, const int64_t *seed, const int32_t *base, const int64_
                 int64_t step, int npoints, double *r);       /* r[npoints][ndim] */
 void ora_tet_points(const double *v, int npoints, const double *coords, double *out);
 double ora_tet_volume(const double *v);                      /* v[4][3] */
+int    ora_tet_weights(const double *v, int npts, const double *pts, double *w); /* w[npts][4] */
 int    ora_point_in_tet(const double *v, const double *p);       /* v[4][3], p[3] */
 double ora_norm2(int n, const double *x);                    /* gfortran NORM2 */
 
@@ -97,7 +98,7 @@ int      ora_locate(ora_ctx *ctx, int frame, int npts, const double *xyz, int32_
 int      ora_load_strain(ora_ctx *ctx, int frame, int npts, const double *xyz,
                          double *traces);      /* [npts][nd][nt] */
 int      ora_base_kernels(ora_ctx *ctx, int npts, const double *xyz, double *base);
-int      ora_slave_work(ora_ctx *ctx, int nelems, const double *vertices,
+int      ora_slave_work(ora_ctx *ctx, int nelems, int nbasis, const double *vertices,
                         const int32_t *conn, double *values, double *variance,
                         int32_t *niterations, int64_t *halton_step,
                         const int64_t *halton_offsets, int64_t *offsets_used,

@@ -948,49 +948,119 @@ static int travel_time_masked(ora_ctx *C, int r, const double *center)
     return (tp > C->R[r].t_last_p && ts > C->R[r].t_last_s);
 }
 
-/* slave_work (slave.f90:273-810) for tetrahedral, volumetric elements */
-int ora_slave_work(ora_ctx *C, int nelems, const double *vertices, const int32_t *conn,
+/* init_weight_tet_mesh (inversion_mesh.f90:1086-1116): the inverse of
+ * ab = [[v_j - v_1], [1]] (columns j = 1..4), by the adjugate; A[4 i + j] = inv(ab)(i, j) */
+static double det3(double a, double b, double c, double d, double e, double f, double g, double h, double i)
+{
+    return a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
+}
+
+static void tet_abinv(const double *v, double *A)
+{
+    double ab[4][4], cof[4][4], det = 0.0;
+    int r, c, i, j;
+    for (c = 0; c < 4; ++c) {
+        for (r = 0; r < 3; ++r) ab[r][c] = v[3 * c + r] - v[r];
+        ab[3][c] = 1.0;
+    }
+    for (r = 0; r < 4; ++r)
+        for (c = 0; c < 4; ++c) {
+            double m[9];
+            int n = 0;
+            for (i = 0; i < 4; ++i)
+                for (j = 0; j < 4; ++j)
+                    if (i != r && j != c) m[n++] = ab[i][j];
+            cof[r][c] = ((r + c) & 1 ? -1.0 : 1.0) * det3(m[0], m[1], m[2], m[3], m[4], m[5], m[6], m[7], m[8]);
+        }
+    for (c = 0; c < 4; ++c) det += ab[0][c] * cof[0][c];
+    for (r = 0; r < 4; ++r)
+        for (c = 0; c < 4; ++c) A[4 * r + c] = cof[c][r] / det;
+}
+
+/* weights (inversion_mesh.f90:530-622): basis function ivertex of a tetrahedron at the
+ * points, sum_j abinv(ivertex, j) (dx, dy, dz, 1)_j with d = point - vertex 1 */
+static int tet_weights(const double *v, const double *A, int ivertex, int npts, const double *pts, double *w)
+{
+    int p, bad = 0;
+    for (p = 0; p < npts; ++p) {
+        double dx = pts[3 * p] - v[0], dy = pts[3 * p + 1] - v[1], dz = pts[3 * p + 2] - v[2];
+        w[p] = A[4 * ivertex] * dx + A[4 * ivertex + 1] * dy + A[4 * ivertex + 2] * dz + A[4 * ivertex + 3] * 1;
+        if (w[p] < -1e-9 || w[p] > 1.0 + 1e-9) bad = 1;
+    }
+    return bad;
+}
+
+/* Test hook: the four vertex basis functions of a tetrahedron v[4][3] at pts[npts][3]:
+ * w[npts][4]; returns 1 if a weight is outside [-1e-9, 1 + 1e-9] */
+int ora_tet_weights(const double *v, int npts, const double *pts, double *w)
+{
+    double A[16], *t = (double *)malloc((size_t)npts * sizeof(double));
+    int b, p, bad = 0;
+    tet_abinv(v, A);
+    for (b = 0; b < 4; ++b) {
+        bad |= tet_weights(v, A, b, npts, pts, t);
+        for (p = 0; p < npts; ++p) w[4 * p + b] = t[p];
+    }
+    free(t);
+    return bad;
+}
+
+/* slave_work (slave.f90:273-810) for tetrahedral elements, volumetric (nbasis 1) or
+ * on-vertex (nbasis 4) basis functions: one integrator per basis function, values
+ * [nelems][nbasis][nkernel] */
+int ora_slave_work(ora_ctx *C, int nelems, int nbasis, const double *vertices, const int32_t *conn,
                    double *values, double *variance, int32_t *niterations,
                    int64_t *halton_step, const int64_t *halton_offsets,
                    int64_t *offsets_used, int64_t *evaluations)
 {
     const mck_setup_t *S = &C->S;
-    int nk = S->nkernel, P = S->npoints_per_step, e, r, k, i, rc = 0;
+    int nk = S->nkernel, P = S->npoints_per_step, e, r, k, i, b, rc = 0;
     double *pts = (double *)malloc((size_t)P * 3 * sizeof(double));
     double *hal = (double *)malloc((size_t)P * 3 * sizeof(double));
     double *base = (double *)calloc((size_t)P * nk * 6, sizeof(double));
+    double *wbase = (double *)malloc((size_t)P * nk * 6 * sizeof(double));
     double *phys = (double *)malloc((size_t)P * nk * sizeof(double));
+    double *wts = (double *)malloc((size_t)P * sizeof(double));
     double *F = (double *)malloc((size_t)P * C->nd * C->nomega * 2 * sizeof(double));
     int32_t *niter = (int32_t *)malloc(nk * sizeof(int32_t));
+    int *conv_all = (int *)malloc(nk * sizeof(int));
+    ora_mc mcs[4];
     static const int64_t seed[3] = {20, 20, 20}, leap[3] = {2, 2, 2};
     static const int32_t bases[3] = {2, 3, 5};
+    if (nbasis != 1 && nbasis != 4) { rc = MCK_EINVAL; goto out; }
     if (evaluations) *evaluations = 0;
     C->halton_step = *halton_step;
     for (e = 0; e < nelems; ++e) {
-        double v[12], center[3], volume;
-        ora_mc mc;
+        double v[12], center[3], volume, A[16];
         for (i = 0; i < 4; ++i) {
             int vid = conn[e * 4 + i] - 1;
             v[3 * i] = vertices[3 * vid]; v[3 * i + 1] = vertices[3 * vid + 1]; v[3 * i + 2] = vertices[3 * vid + 2];
         }
         for (i = 0; i < 3; ++i) center[i] = (v[i] + v[3 + i] + v[6 + i] + v[9 + i]) / 4.0;
         volume = S->int_over_volume ? ora_tet_volume(v) : 1.0;
+        if (nbasis > 1) tet_abinv(v, A);
         if (halton_offsets) C->halton_step = halton_offsets[e];
         if (offsets_used) offsets_used[e] = C->halton_step;
         /* element containing the source: no integration (slave.f90:466-482) */
         if (S->mask_src_rec && ora_point_in_tet(v, C->src_r)) {
-            for (k = 0; k < nk; ++k) {
-                values[(size_t)e * nk + k] = 0.0;
-                variance[(size_t)e * nk + k] = 0.0;
-                niterations[(size_t)e * nk + k] = 0;
+            for (k = 0; k < nk * nbasis; ++k) {
+                values[(size_t)e * nk * nbasis + k] = 0.0;
+                variance[(size_t)e * nk * nbasis + k] = 0.0;
             }
+            for (k = 0; k < nk; ++k) niterations[(size_t)e * nk + k] = 0;
             continue;
         }
-        mc_init(&mc, nk, volume, S->allowed_error * volume, S->allowed_relative_error);
+        for (b = 0; b < nbasis; ++b)
+            mc_init(&mcs[b], nk, volume, S->allowed_error * volume, S->allowed_relative_error);
         for (k = 0; k < nk; ++k) niter[k] = 0;
         for (;;) {
             int all = 1, stop = 0;
-            for (k = 0; k < nk; ++k) if (!mc.conv[k]) all = 0;
+            /* allallconverged / allisconverged (mc_integration.f90:322-359) */
+            for (k = 0; k < nk; ++k) {
+                conv_all[k] = 1;
+                for (b = 0; b < nbasis; ++b) if (!mcs[b].conv[k]) conv_all[k] = 0;
+                if (!conv_all[k]) all = 0;
+            }
             if (all) break;
             ora_halton(3, seed, bases, leap, C->halton_step, P, hal);
             C->halton_step += P;
@@ -998,7 +1068,7 @@ int ora_slave_work(ora_ctx *C, int nelems, const double *vertices, const int32_t
             for (k = 0; k < nk; ++k) if (niter[k] == S->max_iter) stop = 1;
             if (stop) break;
             rc = batch_fwd(C, P, pts, F);
-            if (rc) { mc_free(&mc); goto out; }
+            if (rc) { for (b = 0; b < nbasis; ++b) mc_free(&mcs[b]); goto out; }
             for (r = 0; r < S->nrec; ++r) {
                 ora_rec *R = &C->R[r];
                 int allconv = 1;
@@ -1012,32 +1082,46 @@ int ora_slave_work(ora_ctx *C, int nelems, const double *vertices, const int32_t
                     }
                     continue;
                 }
-                for (k = R->first; k <= R->last; ++k) if (!mc.conv[k]) allconv = 0;
+                for (k = R->first; k <= R->last; ++k) if (!conv_all[k]) allconv = 0;
                 if (allconv) continue;
-                rc = receiver_base(C, r, P, pts, F, mc.conv, base, evaluations, niter);
-                if (rc) { mc_free(&mc); goto out; }
+                rc = receiver_base(C, r, P, pts, F, conv_all, base, evaluations, niter);
+                if (rc) { for (b = 0; b < nbasis; ++b) mc_free(&mcs[b]); goto out; }
                 dampen(base, P, nk, pts, C->R[r].r, S->damp_radius);
                 dampen(base, P, nk, pts, C->src_r, S->damp_radius);
             }
-            for (i = 0; i < P; ++i) {
-                double rr = ora_norm2(3, pts + 3 * i);
-                ora_model m = model_at(S, rr);
-                for (k = 0; k < nk; ++k)
-                    phys[(size_t)i * nk + k] = physical(S->kernel_param[k], base + ((size_t)i * nk + k) * 6, &m, S->relative_kernel);
+            /* per basis function: weights, mult3d_1d, physical kernels, MC check
+             * (slave.f90:723-744) */
+            for (b = 0; b < nbasis; ++b) {
+                if (nbasis == 1) {
+                    for (i = 0; i < P; ++i) wts[i] = 1.0;
+                } else if (tet_weights(v, A, b, P, pts, wts)) {
+                    rc = MCK_EINVAL;                  /* weight outside [0, 1]: pabort */
+                    for (i = 0; i < nbasis; ++i) mc_free(&mcs[i]);
+                    goto out;
+                }
+                for (i = 0; i < P; ++i)
+                    for (k = 0; k < nk * 6; ++k) wbase[(size_t)i * nk * 6 + k] = base[(size_t)i * nk * 6 + k] * wts[i];
+                for (i = 0; i < P; ++i) {
+                    double rr = ora_norm2(3, pts + 3 * i);
+                    ora_model m = model_at(S, rr);
+                    for (k = 0; k < nk; ++k)
+                        phys[(size_t)i * nk + k] = physical(S->kernel_param[k], wbase + ((size_t)i * nk + k) * 6, &m, S->relative_kernel);
+                }
+                rc = mc_check(&mcs[b], P, phys);
+                if (rc) { for (i = 0; i < nbasis; ++i) mc_free(&mcs[i]); goto out; }
             }
-            rc = mc_check(&mc, P, phys);
-            if (rc) { mc_free(&mc); goto out; }
         }
-        for (k = 0; k < nk; ++k) {
-            values[(size_t)e * nk + k] = mc.integral[k];
-            variance[(size_t)e * nk + k] = mc.variance[k];
-            niterations[(size_t)e * nk + k] = niter[k];
-        }
-        mc_free(&mc);
+        for (b = 0; b < nbasis; ++b)
+            for (k = 0; k < nk; ++k) {
+                values[((size_t)e * nbasis + b) * nk + k] = mcs[b].integral[k];
+                variance[((size_t)e * nbasis + b) * nk + k] = mcs[b].variance[k];
+            }
+        for (k = 0; k < nk; ++k) niterations[(size_t)e * nk + k] = niter[k];
+        for (b = 0; b < nbasis; ++b) mc_free(&mcs[b]);
     }
 out:
     *halton_step = C->halton_step;
-    free(pts); free(hal); free(base); free(phys); free(F); free(niter);
+    free(pts); free(hal); free(base); free(wbase); free(phys); free(wts); free(F); free(niter); free(conv_all);
     return rc;
 }
 

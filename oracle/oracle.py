@@ -52,8 +52,9 @@ def lib():
         L.ora_locate.argtypes = [C.c_void_p, C.c_int, C.c_int, f64p, i32p, f64p, f64p, i32p]
         L.ora_load_strain.argtypes = [C.c_void_p, C.c_int, C.c_int, f64p, f64p]
         L.ora_base_kernels.argtypes = [C.c_void_p, C.c_int, f64p, f64p]
-        L.ora_slave_work.argtypes = [C.c_void_p, C.c_int, f64p, i32p, f64p, f64p, i32p,
+        L.ora_slave_work.argtypes = [C.c_void_p, C.c_int, C.c_int, f64p, i32p, f64p, f64p, i32p,
                                      i64p, i64p, i64p, i64p]
+        L.ora_tet_weights.argtypes = [f64p, C.c_int, f64p, f64p]
         L.ora_kernel_info.argtypes = [C.c_void_p, C.c_int, f64p, C.POINTER(C.c_int),
                                       C.POINTER(C.c_int)]
         L.ora_strain_nd.argtypes = [C.c_void_p]
@@ -119,6 +120,15 @@ def tet_points(v, coords):
 
 def tet_volume(v):
     return lib().ora_tet_volume(P(np.ascontiguousarray(v, float)))
+
+
+def tet_weights(v, pts):
+    """The four vertex basis functions of tetrahedron v[4][3] at pts[n][3]: (w[n][4], bad)"""
+    v = np.ascontiguousarray(v, float)
+    pts = np.ascontiguousarray(pts, float)
+    w = np.zeros((pts.shape[0], 4))
+    bad = lib().ora_tet_weights(P(v), pts.shape[0], P(pts), P(w))
+    return w, bool(bad)
 
 
 def point_in_tet(v, q):
@@ -326,18 +336,20 @@ class Oracle:
         lib().ora_kernel_info(self.h, k, C.byref(norm), C.byref(ncut), C.byref(ifirst))
         return norm.value, ncut.value, ifirst.value
 
-    def slave_work(self, verts, conn, halton_offsets=None):
+    def slave_work(self, verts, conn, halton_offsets=None, nbasis=1):
+        """values/variance (ne, nk) for volumetric elements, (ne, nbasis, nk) otherwise"""
         verts = np.ascontiguousarray(verts, float)
         conn = np.ascontiguousarray(conn, np.int32)
         ne = conn.shape[0]
-        vals = np.zeros((ne, self.nk))
-        var = np.zeros((ne, self.nk))
+        shape = (ne, self.nk) if nbasis == 1 else (ne, nbasis, self.nk)
+        vals = np.zeros(shape)
+        var = np.zeros(shape)
         nit = np.zeros((ne, self.nk), np.int32)
         step = np.array([self.halton_step], np.int64)
         used = np.zeros(ne, np.int64)
         ev = np.zeros(1, np.int64)
         offs = None if halton_offsets is None else np.ascontiguousarray(halton_offsets, np.int64)
-        rc = lib().ora_slave_work(self.h, ne, P(verts), P(conn, C.c_int32), P(vals), P(var),
+        rc = lib().ora_slave_work(self.h, ne, nbasis, P(verts), P(conn, C.c_int32), P(vals), P(var),
                                   P(nit, C.c_int32), P(step, C.c_int64), P(offs, C.c_int64),
                                   P(used, C.c_int64), P(ev, C.c_int64))
         if rc:

@@ -88,3 +88,29 @@ def test_plain_equals_batched(tiny, gpu):
     assert np.array_equal(a["niterations"], b["niterations"])
     assert np.array_equal(a["values"], b["values"])
     assert np.array_equal(a["variance"], b["variance"])
+
+
+def test_slave_work_vertex_basis(tiny, gpu):
+    """Basis functions on the tetrahedra's vertices (nbasis 4): one integrator per basis
+    function on the same points, values (ne, 4, nk)."""
+    from oracle import oracle
+    o = oracle.Oracle(tiny.setup)
+    ref = o.slave_work(tiny.verts, tiny.conn, nbasis=4)
+    res = gpu.slave_work(tiny.verts, tiny.conn, halton_offsets=ref["offsets"], nbasis=4)
+    assert res["values"].shape == ref["values"].shape == (tiny.ntet, 4, tiny.nkernel)
+    assert np.array_equal(res["niterations"], ref["niterations"])
+    assert res["stats"]["evaluations"] == ref["evaluations"]
+    for k in range(tiny.nkernel):
+        r = ref["values"][:, :, k]
+        nrm = np.linalg.norm(r)
+        if nrm == 0:
+            assert not np.any(res["values"][:, :, k])
+            continue
+        err = np.linalg.norm(res["values"][:, :, k] - r) / nrm
+        assert err < TOL, (k, err)
+    # the plain (reference-order) form agrees bit for bit with the batched one
+    gpu.halton_step = 0
+    a = gpu.slave_work(tiny.verts, tiny.conn, nbasis=4)
+    gpu.halton_step = 0
+    b = gpu.slave_work(tiny.verts, tiny.conn, plain=True, nbasis=4)
+    assert np.array_equal(a["values"], b["values"]) and np.array_equal(a["niterations"], b["niterations"])
