@@ -148,7 +148,10 @@ typedef struct {
     double  allowed_relative_error; /* ALLOWED_RELATIVE_ERROR                            */
     int32_t int_over_volume;   /* !NO_INT_OVER_VOLUME                                    */
     int32_t relative_kernel;   /* !KERNEL_FOR_ABSOLUTE_PERTURBATIONS                     */
-    int32_t quasirandom;       /* !USE_PSEUDORANDOM_NUMBERS (only 1 is supported)        */
+    int32_t quasirandom;       /* !USE_PSEUDORANDOM_NUMBERS: 1 Halton points in         */
+                               /* tetrahedra; 0 the library's pseudo-random stream (the  */
+                               /* reference's random_number is clock-seeded); voxels     */
+                               /* always take pseudo-random points                       */
     int32_t int_scheme;        /* MCK_INT_*                                              */
     int32_t deconv_stf;        /* !NO_DECONVOLVE_STF                                     */
     double  damp_radius;       /* DAMP_RADIUS_SOURCE_RECEIVER (m); <= 0 disables         */
@@ -181,12 +184,14 @@ int mck_init(const mck_setup_t *setup);
 
 /*
  * Replaces slave_work (src/slave.f90:273-810) for one task of inversion elements.
- *  elem_type       4 = tetrahedron (inversion_mesh.f90:766-805 ielem_type); others rejected
- *  nvpe            vertices per element (4)
+ *  elem_type       4 = tetrahedron, 5 = voxel (block in radius, colatitude and longitude
+ *                  given by its 8 corners, random_voxel.f90); others rejected
+ *  nvpe            vertices per element (4 or 8)
  *  nelems          elements in this task
  *  nbasis          basis functions per element: 1 = volumetric, 4 = linear functions on
  *                  the tetrahedron's vertices (inversion_mesh.f90:530-622, one Monte Carlo
- *                  integrator per basis function, slave.f90:421, 489-497, 723-744)
+ *                  integrator per basis function, slave.f90:421, 489-497, 723-744); voxels
+ *                  take 1 or 8 (every weight 1, as weights() gives them)
  *  vertices        [nvertices][3] metres (Fortran vertices(3, nvertices))
  *  conn            [nelems][nvpe] 1-based vertex ids (Fortran connectivity(nvpe, nelems));
  *                  elements whose first id is <= 0 are skipped (empty slots of a last task)

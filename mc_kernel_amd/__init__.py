@@ -82,12 +82,13 @@ class MCKernel:
             self.lib = None
 
     def slave_work(self, verts, conn, halton_offsets=None, plain=False, nbasis=1):
-        """mck_slave_work for a tetrahedral task; returns a dict of numpy arrays.
-        values/variance are (ne, nk) for volumetric elements (nbasis 1) and
-        (ne, 4, nk) for basis functions on the vertices (nbasis 4)."""
+        """mck_slave_work for a task of tetrahedra (conn (ne, 4)) or voxels (conn (ne, 8));
+        returns a dict of numpy arrays.  values/variance are (ne, nk) for volumetric
+        elements (nbasis 1) and (ne, nbasis, nk) for basis functions on the vertices."""
         verts = np.ascontiguousarray(verts, np.float64)
         conn = np.ascontiguousarray(conn, np.int32)
-        ne = conn.shape[0]
+        ne, nvpe = conn.shape
+        etype = 4 if nvpe == 4 else 5
         shape = (ne, self.nk) if nbasis == 1 else (ne, nbasis, self.nk)
         vals = np.zeros(shape)
         var = np.zeros(shape)
@@ -96,7 +97,7 @@ class MCKernel:
         step = np.array([self.halton_step], np.int64)
         offs = None if halton_offsets is None else np.ascontiguousarray(halton_offsets, np.int64)
         fn = self.lib.mck_slave_work_plain if plain else self.lib.mck_slave_work
-        _check(fn(4, 4, ne, nbasis, verts.shape[0], abi.ptr(verts, C.c_double),
+        _check(fn(etype, nvpe, ne, nbasis, verts.shape[0], abi.ptr(verts, C.c_double),
                   abi.ptr(conn, C.c_int32), abi.ptr(vals, C.c_double), abi.ptr(var, C.c_double),
                   abi.ptr(nit, C.c_int32), abi.ptr(tm, C.c_double), abi.ptr(step, C.c_int64),
                   abi.ptr(offs, C.c_int64)))

@@ -339,6 +339,7 @@ struct State {
     // per-task and per-chunk buffers
     DBuf<double> tet, volume, allowed_var;
     int nbasis = 1;                          // basis functions per element of the task
+    int vox = 0;                             // elements of the task are voxels
     DBuf<double> abinv, bweights;            // [nvox][4][4], [step points][4] (nbasis 4)
     DBuf<int64_t> hbase;
     DBuf<uint8_t> masked, live, working, active;
@@ -885,7 +886,6 @@ int init_impl(const mck_setup_t *s) {
     if (S.nt < 16 || S.nt > 4096) return fail(MCK_EUNSUPPORTED, "nt must be in [16, 4096]");
     if (S.nrec <= 0 || S.nkernel <= 0) return fail(MCK_EINVAL, "no receivers or kernels");
     if (S.fwd.nvar != 10 || S.bwd.nvar != 5) return fail(MCK_EINVAL, "merged databases need 10 (fwd) / 5 (bwd) variables");
-    if (S.quasirandom != 1) return fail(MCK_EUNSUPPORTED, "only quasi-random (Halton) points are supported");
     if (S.component != 'Z' && S.component != 'R' && S.component != 'T')
         return fail(MCK_EINVAL, "component must be Z, R or T");
     if (!G.stream) HIPCHECK(hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking));
@@ -1044,10 +1044,48 @@ void tet_barycentric_inverse(const double *v, double *A) {
         for (int j = 0; j < 4; ++j) A[4 * i + j] = cof[j][i] / det;
 }
 
+// Voxels (random_voxel.f90): spherical range r_min, r_max, colat_min, colat_max,
+// lon_min, lon_max of the 8 corners (cartesian_to_spherical_range, :166-209)
+void vox_range(const double *v, double *g) {
+    for (int i = 0; i < 8; ++i) {
+        const double x = v[3 * i], y = v[3 * i + 1], z = v[3 * i + 2];
+        const double r = std::sqrt(x * x + y * y + z * z), c = std::acos(z / r), l = std::atan2(y, x);
+        if (i == 0 || r < g[0]) g[0] = r;
+        if (i == 0 || r > g[1]) g[1] = r;
+        if (i == 0 || c < g[2]) g[2] = c;
+        if (i == 0 || c > g[3]) g[3] = c;
+        if (i == 0 || l < g[4]) g[4] = l;
+        if (i == 0 || l > g[5]) g[5] = l;
+    }
+}
+
+// get_volume_vox (random_voxel.f90:91-120)
+double vox_volume(const double *g) {
+    const double lat1 = kPi / 2.0 - g[2], lat2 = kPi / 2.0 - g[3];
+    const double shell = (4.0 / 3.0) * kPi * (g[1] * g[1] * g[1] - g[0] * g[0] * g[0]);
+    return shell * (std::fabs(g[4] - g[5]) * std::fabs(std::sin(lat1) - std::sin(lat2)) / (4.0 * kPi));
+}
+
+// get_center_vox (random_voxel.f90:125-145) as written: the longitude of the centre is
+// the mid colatitude
+void vox_center(const double *g, double *c) {
+    const double r = (g[1] - g[0]) / 2.0 + g[0], th = (g[3] - g[2]) / 2.0 + g[2], ph = th;
+    c[0] = r * std::cos(ph) * std::sin(th);
+    c[1] = r * std::sin(ph) * std::sin(th);
+    c[2] = r * std::cos(th);
+}
+
+// point_in_voxel (random_voxel.f90:232-259)
+bool point_in_vox(const double *g, const double *p) {
+    const double r = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    const double c = std::acos(p[2] / r), l = std::atan2(p[1], p[0]);
+    return r >= g[0] && r <= g[1] && c >= g[2] && c <= g[3] && l >= g[4] && l <= g[5];
+}
+
 struct Task {
-    int nelems = 0, P = 0, nbasis = 1;
+    int nelems = 0, P = 0, nbasis = 1, vox = 0;
     std::vector<double> abinv;   // [ne][4][4] (nbasis 4)
-    std::vector<double> tet;     // [ne][4][3]
+    std::vector<double> tet;     // [ne][12]: tetrahedron vertices, or a voxel's range
     std::vector<double> vol, allowed_var;
     std::vector<uint8_t> masked, live;
     std::vector<int64_t> hbase;
@@ -1057,13 +1095,18 @@ int prepare_task(State &G, Task &T, int elem_type, int nvpe, int nelems, int nba
                  const double *vertices, const int32_t *conn, const int64_t *halton_step,
                  const int64_t *halton_offsets) {
     const mck_setup_t &S = G.S;
-    if (elem_type != 4 || nvpe != 4) return fail(MCK_EUNSUPPORTED, "only tetrahedral inversion meshes are supported");
-    if (nbasis != 1 && nbasis != 4)
+    const bool vox = elem_type == 5;
+    if (!((elem_type == 4 && nvpe == 4) || (vox && nvpe == 8)))
+        return fail(MCK_EUNSUPPORTED, "inversion elements must be tetrahedra (type 4) or voxels (type 5)");
+    if (!vox && nbasis != 1 && nbasis != 4)
         return fail(MCK_EINVAL, "tetrahedra have 1 (volumetric) or 4 (on-vertex) basis functions");
+    if (vox && nbasis != 1 && nbasis != 8)
+        return fail(MCK_EINVAL, "voxels have 1 (volumetric) or 8 (on-vertex) basis functions");
     if (nelems <= 0) return fail(MCK_EINVAL, "empty task");
     T.nelems = nelems;
     T.nbasis = nbasis;
-    if (nbasis > 1) T.abinv.assign((size_t)nelems * 16, 0.0);
+    T.vox = vox ? 1 : 0;
+    if (nbasis > 1 && !vox) T.abinv.assign((size_t)nelems * 16, 0.0);
     T.P = S.npoints_per_step;
     T.tet.assign((size_t)nelems * 12, 0.0);
     T.vol.assign(nelems, 1.0);
@@ -1073,22 +1116,33 @@ int prepare_task(State &G, Task &T, int elem_type, int nvpe, int nelems, int nba
     T.hbase.assign(nelems, 0);
     const int64_t stride = (int64_t)(S.max_iter + 1) * T.P;
     for (int e = 0; e < nelems; ++e) {
-        if (conn[4 * e] <= 0) { T.live[e] = 0; continue; }
-        for (int i = 0; i < 4; ++i) {
-            int vid = conn[4 * e + i] - 1;
+        if (conn[nvpe * e] <= 0) { T.live[e] = 0; continue; }
+        double ev[24];
+        for (int i = 0; i < nvpe; ++i) {
+            int vid = conn[nvpe * e + i] - 1;
             if (vid < 0 || vid >= nvert) return fail(MCK_EINVAL, "connectivity out of range");
-            for (int c = 0; c < 3; ++c) T.tet[(size_t)e * 12 + 3 * i + c] = vertices[3 * vid + c];
+            for (int c = 0; c < 3; ++c) ev[3 * i + c] = vertices[3 * vid + c];
         }
-        const double *v = &T.tet[(size_t)e * 12];
-        if (nbasis > 1) tet_barycentric_inverse(v, &T.abinv[(size_t)e * 16]);
-        // element containing the source: no integration, zero results (slave.f90:466-482)
-        if (S.mask_src_rec && point_in_tet(v, G.src_r)) { T.live[e] = 0; continue; }
-        double vol = S.int_over_volume ? tet_volume(v) : 1.0;
+        double *v = &T.tet[(size_t)e * 12];
+        double c[3], vol;
+        if (vox) {
+            vox_range(ev, v);
+            if (!(v[0] > 0.0) || v[1] <= v[0] || v[3] <= v[2] || v[5] <= v[4])
+                return fail(MCK_EINVAL, "degenerate voxel");
+            // element containing the source: no integration, zero results (slave.f90:466-482)
+            if (S.mask_src_rec && point_in_vox(v, G.src_r)) { T.live[e] = 0; continue; }
+            vol = S.int_over_volume ? vox_volume(v) : 1.0;
+            vox_center(v, c);
+        } else {
+            std::copy(ev, ev + 12, v);
+            if (nbasis > 1) tet_barycentric_inverse(v, &T.abinv[(size_t)e * 16]);
+            if (S.mask_src_rec && point_in_tet(v, G.src_r)) { T.live[e] = 0; continue; }
+            vol = S.int_over_volume ? tet_volume(v) : 1.0;
+            for (int i = 0; i < 3; ++i) c[i] = (v[i] + v[3 + i] + v[6 + i] + v[9 + i]) / 4.0;
+        }
         T.vol[e] = vol;
         double ae = S.allowed_error * vol;
         T.allowed_var[e] = ae * ae;
-        double c[3];
-        for (int i = 0; i < 3; ++i) c[i] = (v[i] + v[3 + i] + v[6 + i] + v[9 + i]) / 4.0;
         // travel_time_larger_Tmax (slave.f90:928-950)
         for (int r = 0; r < G.R; ++r) {
             double d = norm2_3(G.src_r[0] - c[0], G.src_r[1] - c[1], G.src_r[2] - c[2])
@@ -1096,14 +1150,16 @@ int prepare_task(State &G, Task &T, int elem_type, int nvpe, int nelems, int nba
             double tp = d / (double)S.vp_max, ts = d / (double)S.vs_max;
             T.masked[(size_t)e * G.R + r] = (tp > G.t_last_p[r] && ts > G.t_last_s[r]) ? 1 : 0;
             // receiver inside the element (slave.f90:539-545): same treatment
-            if (S.mask_src_rec && point_in_tet(v, &G.rec_r[3 * r])) T.masked[(size_t)e * G.R + r] = 1;
+            if (S.mask_src_rec && (vox ? point_in_vox(v, &G.rec_r[3 * r]) : point_in_tet(v, &G.rec_r[3 * r])))
+                T.masked[(size_t)e * G.R + r] = 1;
         }
         T.hbase[e] = halton_offsets ? halton_offsets[e] : (*halton_step + (int64_t)e * stride);
     }
     const int ne = nelems, nk = G.nk, nbk = nelems * nbasis;
     G.nbasis = nbasis;
+    G.vox = T.vox;
     HIPCHECK(G.tet.ensure((size_t)ne * 12));
-    if (nbasis > 1) {
+    if (nbasis > 1 && !vox) {
         HIPCHECK(G.abinv.ensure((size_t)ne * 16));
         HIPCHECK(hipMemcpyAsync(G.abinv.p, T.abinv.data(), (size_t)ne * 16 * 8, hipMemcpyHostToDevice, G.stream));
     }
@@ -1180,10 +1236,11 @@ int eval_chunk(State &G, int P, int w0, int nw, std::vector<std::pair<hipEvent_t
     int rc;
     if ((rc = ensure_chunk(G, np))) return rc;
     Located *loc_f = G.loc.p, *loc_b = G.loc.p + np;
+    const bool tet_basis = G.nbasis > 1 && !G.vox;
     hipLaunchKernelGGL(k_points, dim3((np + 255) / 256), dim3(256), 0, st, nw, P, wl, G.hbase.p,
-                       G.draws.p, G.tet.p, G.S.model_ndepth, G.S.model_dr, G.xyz.p, G.model_idx.p,
-                       G.nbasis, G.abinv.p, G.nbasis > 1 ? G.bweights.p + (size_t)w0 * P * 4 : nullptr,
-                       G.err.p);
+                       G.draws.p, G.tet.p, G.vox, G.S.quasirandom, G.S.model_ndepth, G.S.model_dr, G.xyz.p,
+                       G.model_idx.p, G.nbasis, G.abinv.p,
+                       tet_basis ? G.bweights.p + (size_t)w0 * P * 4 : nullptr, G.err.p);
     HIPCHECK(hipGetLastError());
     // fwd and bwd meshes may differ: locate frame 0 in the fwd mesh, frames 1.. in the bwd mesh
     hipLaunchKernelGGL(k_locate, dim3((unsigned)((np + 127) / 128)), dim3(128), 0, st, G.fwd.d,
@@ -1259,13 +1316,13 @@ int mc_step(State &G, const Task &T, int *nwork_out, std::vector<std::pair<hipEv
                        G.work_list.p, G.masked.p, G.d_rec_first.p, G.d_rec_count.p, vs, G.active.p);
     HIPCHECK(hipGetLastError());
     HIPCHECK(G.phys.ensure((size_t)nwork * P * nk));
-    if (G.nbasis > 1) HIPCHECK(G.bweights.ensure((size_t)nwork * P * 4));
+    if (G.nbasis > 1 && !G.vox) HIPCHECK(G.bweights.ensure((size_t)nwork * P * 4));
     const int wc = std::max(1, G.chunk_points / P);
     int rc;
     for (int w0 = 0; w0 < nwork; w0 += wc)
         if ((rc = eval_chunk(G, P, w0, std::min(wc, nwork - w0), evs))) return rc;
     hipLaunchKernelGGL(k_mc, dim3(nwork), dim3(256), 0, st, nwork, P, nk, R, G.work_list.p, G.phys.p,
-                       G.nbasis > 1 ? G.bweights.p : nullptr, G.masked.p, G.active.p, G.d_k_rec.p, G.d_k_nbk.p, G.volume.p, G.allowed_var.p,
+                       G.nbasis > 1 && !G.vox ? G.bweights.p : nullptr, G.masked.p, G.active.p, G.d_k_rec.p, G.d_k_nbk.p, G.volume.p, G.allowed_var.p,
                        G.S.allowed_relative_error, vs, G.evals.p, G.err.p);
     HIPCHECK(hipGetLastError());
     G.stats.points += (int64_t)nwork * P;

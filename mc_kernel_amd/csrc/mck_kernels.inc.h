@@ -167,10 +167,27 @@ __device__ inline double d_norm2(double a, double b, double c)
 // ------------------------------------------------------------------------------------
 // k_points: one thread per MC point of the working voxels of this chunk
 // ------------------------------------------------------------------------------------
+// Pseudo-random stream standing in for Fortran's random_number (which the reference
+// seeds from the clock): value d of draw idx is the SplitMix64 output of the counter
+// 0x5EED0000 + (3 idx + d + 1) * 0x9E3779B97F4A7C15 (Steele, Lea and Flood 2014), top 53
+// bits scaled to [0, 1).  Draw positions are assigned as the Halton positions are.
+__device__ inline double d_prng(int64_t idx, int d)
+{
+    uint64_t z = 0x5EED0000ULL + ((uint64_t)idx * 3u + (uint64_t)d + 1u) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z = z ^ (z >> 31);
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
 // With nbasis 4 it also evaluates the vertex basis functions at the point (weights,
 // inversion_mesh.f90:530-622): bweights[point][4] from abinv [nvox][4][4].
+// geom: tetrahedra [nvox][4][3] vertices; voxels [nvox][12] whose first 6 values are the
+// spherical range r_min, r_max, colat_min, colat_max, lon_min, lon_max.
+// Tetrahedra take Halton (quasi) or pseudo-random numbers, voxels pseudo-random ones
+// (generate_random_points, inversion_mesh.f90:626-667).
 __global__ void k_points(int nwork, int P, const int *work_voxel, const int64_t *halton_base,
-                         const int *voxel_draws, const double *tet /* [nvox][4][3] */,
+                         const int *voxel_draws, const double *geom, int vox, int quasi,
                          int ndepth, float dr, double *xyz, int *model_idx, int nbasis,
                          const double *abinv, double *bweights, int *err)
 {
@@ -181,40 +198,54 @@ __global__ void k_points(int nwork, int P, const int *work_voxel, const int64_t 
     // the current draw of voxel v is its (voxel_draws[v] - 1)-th
     int64_t idx = halton_base[v] + (int64_t)(voxel_draws[v] - 1) * P + p;
     double c[4];
-    const int base[3] = {2, 3, 5};
-    for (int d = 0; d < 3; ++d) {
-        int64_t s2 = 20 + idx * 2;
-        double bk = (double)base[d];
-        double acc = 0.0;
-        while (s2 != 0) {
-            int64_t digit = s2 % base[d];
-            acc = acc + (double)digit / bk;   // = dp(digit * qp(1/b^k)), see halton tests
-            bk = bk * (double)base[d];
-            s2 = s2 / base[d];
+    if (quasi && !vox) {
+        const int base[3] = {2, 3, 5};
+        for (int d = 0; d < 3; ++d) {
+            int64_t s2 = 20 + idx * 2;
+            double bk = (double)base[d];
+            double acc = 0.0;
+            while (s2 != 0) {
+                int64_t digit = s2 % base[d];
+                acc = acc + (double)digit / bk;   // = dp(digit * qp(1/b^k)), see halton tests
+                bk = bk * (double)base[d];
+                s2 = s2 / base[d];
+            }
+            c[d] = acc;
         }
-        c[d] = acc;
+    } else {
+        for (int d = 0; d < 3; ++d) c[d] = d_prng(idx, d);
     }
-    // Rocchini-Cignoni fold (polytopes.f90:88-110)
-    if (c[0] + c[1] > 1.0) { c[0] = 1.0 - c[0]; c[1] = 1.0 - c[1]; }
-    if (c[1] + c[2] > 1.0) {
-        double t = c[2];
-        c[2] = 1.0 - c[0] - c[1];
-        c[1] = 1.0 - t;
-    } else if (c[0] + c[1] + c[2] > 1.0) {
-        double t = c[2];
-        c[2] = c[0] + c[1] + c[2] - 1.0;
-        c[0] = 1.0 - c[1] - t;
-    }
-    c[3] = 1.0 - (c[0] + c[1] + c[2]);
-    const double *V = tet + (size_t)v * 12;
+    const double *V = geom + (size_t)v * 12;
     double x[3];
-    for (int i = 0; i < 3; ++i) {
-        double a = 0.0;
-        a = a + V[0 * 3 + i] * c[0];
-        a = a + V[1 * 3 + i] * c[1];
-        a = a + V[2 * 3 + i] * c[2];
-        a = a + V[3 * 3 + i] * c[3];
-        x[i] = a;
+    if (vox) {
+        // generate_random_points_vox (random_voxel.f90:45-86)
+        const double r = pow(c[0] * (V[1] * V[1] * V[1] - V[0] * V[0] * V[0]) + V[0] * V[0] * V[0], 1.0 / 3.0);
+        const double th = acos(cos(V[2]) + (cos(V[3]) - cos(V[2])) * c[1]);
+        const double ph = V[4] + fabs(V[5] - V[4]) * c[2];
+        x[0] = r * cos(ph) * sin(th);
+        x[1] = r * sin(ph) * sin(th);
+        x[2] = r * cos(th);
+    } else {
+        // Rocchini-Cignoni fold (polytopes.f90:88-110)
+        if (c[0] + c[1] > 1.0) { c[0] = 1.0 - c[0]; c[1] = 1.0 - c[1]; }
+        if (c[1] + c[2] > 1.0) {
+            double t = c[2];
+            c[2] = 1.0 - c[0] - c[1];
+            c[1] = 1.0 - t;
+        } else if (c[0] + c[1] + c[2] > 1.0) {
+            double t = c[2];
+            c[2] = c[0] + c[1] + c[2] - 1.0;
+            c[0] = 1.0 - c[1] - t;
+        }
+        c[3] = 1.0 - (c[0] + c[1] + c[2]);
+        for (int i = 0; i < 3; ++i) {
+            double a = 0.0;
+            a = a + V[0 * 3 + i] * c[0];
+            a = a + V[1 * 3 + i] * c[1];
+            a = a + V[2 * 3 + i] * c[2];
+            a = a + V[3 * 3 + i] * c[3];
+            x[i] = a;
+        }
     }
     xyz[3 * (size_t)gid + 0] = x[0];
     xyz[3 * (size_t)gid + 1] = x[1];
@@ -223,7 +254,7 @@ __global__ void k_points(int nwork, int P, const int *work_voxel, const int64_t 
     int id = (int)(r / (double)dr + 1);
     id = id < 1 ? 1 : (id > ndepth ? ndepth : id);
     model_idx[gid] = id - 1;
-    if (nbasis > 1) {
+    if (nbasis > 1 && !vox) {
         const double *A = abinv + (size_t)v * 16;
         const double dx = x[0] - V[0], dy = x[1] - V[1], dz = x[2] - V[2];
         bool bad = false;
@@ -1325,7 +1356,7 @@ __global__ void k_receiver_active(int nwork, int R, int nk, const int *work_voxe
 // basis function (allisconverged, mc_integration.f90:343-359); basis function b sees the
 // physical kernel of the weighted base kernels, phys x w_b (slave.f90:723-744).
 __global__ void k_mc(int nwork, int P, int nk, int R, const int *work_voxel, const double *phys,
-                     const double *bweights /* [nwork * P][4], nbasis 4 only */,
+                     const double *bweights /* [nwork * P][4] of tetrahedra, nbasis 4; null = 1 */,
                      const uint8_t *masked, const uint8_t *active, const int *k_rec,
                      const int *k_nbk, const double *volume, const double *allowed_var, double allowed_rel, VoxState st,
                      unsigned long long *evals, int *err)
@@ -1354,11 +1385,11 @@ __global__ void k_mc(int nwork, int P, int nk, int R, const int *work_voxel, con
             double s = 0.0, s2 = 0.0;
             if (!msk) {
                 for (int p = 0; p < P; ++p) {
-                    const double wt = nb > 1 ? bweights[((size_t)w * P + p) * 4 + b] : 1.0;
+                    const double wt = bweights ? bweights[((size_t)w * P + p) * 4 + b] : 1.0;
                     s += phys[((size_t)w * P + p) * nk + k] * wt;
                 }
                 for (int p = 0; p < P; ++p) {
-                    const double wt = nb > 1 ? bweights[((size_t)w * P + p) * 4 + b] : 1.0;
+                    const double wt = bweights ? bweights[((size_t)w * P + p) * 4 + b] : 1.0;
                     double x = phys[((size_t)w * P + p) * nk + k] * wt;
                     s2 += x * x;
                 }

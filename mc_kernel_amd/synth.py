@@ -322,6 +322,23 @@ class Problem:
         s.mask_src_rec = 0
         return s
 
+    def voxels(self, nvox, seed=7, dlat=4.0, dlon=4.0, dr=300e3):
+        """Voxel inversion elements (blocks in radius, colatitude and longitude given by
+        their 8 corners, random_voxel.f90): nvox blocks of dlat x dlon degrees and dr
+        metres scattered between the source and the first receiver; (verts, conn) with
+        conn (nvox, 8) 1-based."""
+        rng = np.random.default_rng(seed)
+        verts, conn = [], []
+        for i in range(nvox):
+            f = rng.uniform(0.0, 1.0)
+            lat = self.src_lat + f * (self.rec_lat[0] - self.src_lat) + rng.uniform(-5, 5)
+            lon = self.src_lon + f * (self.rec_lon[0] - self.src_lon) + rng.uniform(-5, 5)
+            r0 = rng.uniform(R_EARTH - 2800e3, R_EARTH - 100e3 - dr)
+            for (a, b, c) in np.ndindex(2, 2, 2):
+                verts.append(geo_xyz(lat + a * dlat, lon + b * dlon, r0 + c * dr))
+            conn.append(np.arange(8 * i + 1, 8 * i + 9))
+        return np.ascontiguousarray(np.array(verts, np.float64)), np.ascontiguousarray(np.array(conn, np.int32))
+
     def sample_points(self, npts, seed=3):
         """Uniform points inside the first tetrahedra (for point-level parity tests)."""
         rng = np.random.default_rng(seed)

@@ -56,6 +56,15 @@ void ora_halton(int ndim, const int64_t *seed, const int32_t *base, const int64_
 void ora_tet_points(const double *v, int npoints, const double *coords, double *out);
 double ora_tet_volume(const double *v);                      /* v[4][3] */
 int    ora_tet_weights(const double *v, int npts, const double *pts, double *w); /* w[npts][4] */
+/* pseudo-random stream (stand-in for random_number) and voxels v[8][3] */
+uint64_t ora_splitmix64(uint64_t z);
+double ora_prng(int64_t idx, int d);
+void   ora_prng_points(int64_t step, int npoints, double *r);   /* r[npoints][3] */
+void   ora_vox_range(const double *v, double *range);           /* r, colat, lon min/max */
+void   ora_vox_points(const double *v, int npoints, const double *coords, double *out);
+double ora_vox_volume(const double *v);
+void   ora_vox_center(const double *v, double *c);
+int    ora_point_in_vox(const double *v, const double *p);
 int    ora_point_in_tet(const double *v, const double *p);       /* v[4][3], p[3] */
 double ora_norm2(int n, const double *x);                    /* gfortran NORM2 */
 
@@ -98,7 +107,8 @@ int      ora_locate(ora_ctx *ctx, int frame, int npts, const double *xyz, int32_
 int      ora_load_strain(ora_ctx *ctx, int frame, int npts, const double *xyz,
                          double *traces);      /* [npts][nd][nt] */
 int      ora_base_kernels(ora_ctx *ctx, int npts, const double *xyz, double *base);
-int      ora_slave_work(ora_ctx *ctx, int nelems, int nbasis, const double *vertices,
+int      ora_slave_work(ora_ctx *ctx, int elem_type, int nvpe, int nelems, int nbasis,
+                        const double *vertices,
                         const int32_t *conn, double *values, double *variance,
                         int32_t *niterations, int64_t *halton_step,
                         const int64_t *halton_offsets, int64_t *offsets_used,

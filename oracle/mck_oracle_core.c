@@ -168,6 +168,111 @@ void ora_tet_points(const double *v, int npoints, const double *coords, double *
     }
 }
 
+/* ------------------------------------------------------------------------------------
+ * Pseudo-random numbers.  The reference draws them from Fortran's random_number, seeded
+ * by the run time library (clock / OS entropy), so its streams cannot be reproduced.
+ * This build uses a counter-based generator instead: value d of draw idx of a stream is
+ * the SplitMix64 output (Steele, Lea and Flood 2014) of the counter
+ * 0x5EED_0000 + (3 idx + d + 1) * 0x9E3779B97F4A7C15, its top 53 bits scaled to [0, 1).
+ * Stream positions are assigned exactly as the Halton positions are.
+ * ---------------------------------------------------------------------------------- */
+/* the SplitMix64 output function of a state z */
+uint64_t ora_splitmix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+double ora_prng(int64_t idx, int d)
+{
+    uint64_t z = ora_splitmix64(0x5EED0000ULL + ((uint64_t)idx * 3u + (uint64_t)d + 1u) * 0x9E3779B97F4A7C15ULL);
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+void ora_prng_points(int64_t step, int npoints, double *r)
+{
+    int j, d;
+    for (j = 0; j < npoints; ++j)
+        for (d = 0; d < 3; ++d) r[3 * j + d] = ora_prng(step + j, d);
+}
+
+/* ------------------------------------------------------------------------------------
+ * Voxels: blocks bounded by two radii, two colatitudes and two longitudes, given by their
+ * 8 corners (src/random_voxel.f90).  range[6] = r_min, r_max, colat_min, colat_max,
+ * lon_min, lon_max of the corners (cartesian_to_spherical_range and cartesian_to_spherical_points, :166-209).
+ * ---------------------------------------------------------------------------------- */
+void ora_vox_range(const double *v, double *range)
+{
+    int i;
+    for (i = 0; i < 8; ++i) {
+        double x = v[3 * i], y = v[3 * i + 1], z = v[3 * i + 2];
+        double r = sqrt(x * x + y * y + z * z), c = acos(z / r), l = atan2(y, x);
+        if (i == 0 || r < range[0]) range[0] = r;
+        if (i == 0 || r > range[1]) range[1] = r;
+        if (i == 0 || c < range[2]) range[2] = c;
+        if (i == 0 || c > range[3]) range[3] = c;
+        if (i == 0 || l < range[4]) range[4] = l;
+        if (i == 0 || l > range[5]) range[5] = l;
+    }
+}
+
+static void sph_to_xyz(double r, double colat, double lon, double *u)
+{
+    /* spherical_to_cartesian_point (random_voxel.f90:214-227) */
+    u[0] = r * cos(lon) * sin(colat);
+    u[1] = r * sin(lon) * sin(colat);
+    u[2] = r * cos(colat);
+}
+
+/* generate_random_points_vox (random_voxel.f90:45-86): r uniform in r^3, cos(colat)
+ * uniform, longitude uniform; coords[npoints][3] uniform numbers */
+void ora_vox_points(const double *v, int npoints, const double *coords, double *out)
+{
+    double g[6];
+    int i;
+    ora_vox_range(v, g);
+    for (i = 0; i < npoints; ++i) {
+        double r = pow(coords[3 * i] * (g[1] * g[1] * g[1] - g[0] * g[0] * g[0]) + g[0] * g[0] * g[0], 1.0 / 3.0);
+        double c = acos(cos(g[2]) + (cos(g[3]) - cos(g[2])) * coords[3 * i + 1]);
+        double l = g[4] + fabs(g[5] - g[4]) * coords[3 * i + 2];
+        sph_to_xyz(r, c, l, out + 3 * i);
+    }
+}
+
+/* get_volume_vox (random_voxel.f90:91-120): shell volume times the area fraction of the
+ * latitude-longitude quadrilateral */
+double ora_vox_volume(const double *v)
+{
+    double g[6], shell, frac, lat1, lat2;
+    ora_vox_range(v, g);
+    lat1 = ORA_PI / 2.0 - g[2];
+    lat2 = ORA_PI / 2.0 - g[3];
+    shell = (4.0 / 3.0) * ORA_PI * (g[1] * g[1] * g[1] - g[0] * g[0] * g[0]);
+    frac = fabs(g[4] - g[5]) * fabs(sin(lat1) - sin(lat2)) / (4.0 * ORA_PI);
+    return shell * frac;
+}
+
+/* get_center_vox (random_voxel.f90:125-145), as written: the third spherical coordinate
+ * (longitude) of the centre is taken from the colatitude range */
+void ora_vox_center(const double *v, double *c)
+{
+    double g[6];
+    ora_vox_range(v, g);
+    sph_to_xyz((g[1] - g[0]) / 2.0 + g[0], (g[3] - g[2]) / 2.0 + g[2], (g[3] - g[2]) / 2.0 + g[2], c);
+}
+
+/* point_in_voxel (random_voxel.f90:232-259): inclusive range test */
+int ora_point_in_vox(const double *v, const double *p)
+{
+    double g[6], r, c, l;
+    ora_vox_range(v, g);
+    r = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    c = acos(p[2] / r);
+    l = atan2(p[1], p[0]);
+    return r >= g[0] && r <= g[1] && c >= g[2] && c <= g[3] && l >= g[4] && l <= g[5];
+}
+
 /* get_volume_tet (polytopes.f90:495-506) with determinant_4 in quad precision
  * (polytopes.f90:320-347). */
 double ora_tet_volume(const double *v)

@@ -1005,14 +1005,19 @@ int ora_tet_weights(const double *v, int npts, const double *pts, double *w)
     return bad;
 }
 
-/* slave_work (slave.f90:273-810) for tetrahedral elements, volumetric (nbasis 1) or
- * on-vertex (nbasis 4) basis functions: one integrator per basis function, values
- * [nelems][nbasis][nkernel] */
-int ora_slave_work(ora_ctx *C, int nelems, int nbasis, const double *vertices, const int32_t *conn,
+/* slave_work (slave.f90:273-810) for tetrahedra (elem_type 4, nvpe 4; volumetric or
+ * on-vertex basis functions, nbasis 1 or 4) and voxels (elem_type 5, nvpe 8; nbasis 1 or
+ * 8, weight 1 for every basis function as weights() gives non-tetrahedra,
+ * inversion_mesh.f90:614-617): one integrator per basis function, values
+ * [nelems][nbasis][nkernel].  Tetrahedra take Halton points (quasirandom) or the
+ * pseudo-random stream; voxels always the pseudo-random stream (inversion_mesh.f90:662). */
+int ora_slave_work(ora_ctx *C, int elem_type, int nvpe, int nelems, int nbasis,
+                   const double *vertices, const int32_t *conn,
                    double *values, double *variance, int32_t *niterations,
                    int64_t *halton_step, const int64_t *halton_offsets,
                    int64_t *offsets_used, int64_t *evaluations)
 {
+    const int vox = elem_type == 5;
     const mck_setup_t *S = &C->S;
     int nk = S->nkernel, P = S->npoints_per_step, e, r, k, i, b, rc = 0;
     double *pts = (double *)malloc((size_t)P * 3 * sizeof(double));
@@ -1024,25 +1029,31 @@ int ora_slave_work(ora_ctx *C, int nelems, int nbasis, const double *vertices, c
     double *F = (double *)malloc((size_t)P * C->nd * C->nomega * 2 * sizeof(double));
     int32_t *niter = (int32_t *)malloc(nk * sizeof(int32_t));
     int *conv_all = (int *)malloc(nk * sizeof(int));
-    ora_mc mcs[4];
+    ora_mc mcs[8];                     /* one integrator per basis function (<= 8) */
     static const int64_t seed[3] = {20, 20, 20}, leap[3] = {2, 2, 2};
     static const int32_t bases[3] = {2, 3, 5};
-    if (nbasis != 1 && nbasis != 4) { rc = MCK_EINVAL; goto out; }
+    if (!((elem_type == 4 && nvpe == 4 && (nbasis == 1 || nbasis == 4))
+          || (vox && nvpe == 8 && (nbasis == 1 || nbasis == 8)))) { rc = MCK_EINVAL; goto out; }
     if (evaluations) *evaluations = 0;
     C->halton_step = *halton_step;
     for (e = 0; e < nelems; ++e) {
-        double v[12], center[3], volume, A[16];
-        for (i = 0; i < 4; ++i) {
-            int vid = conn[e * 4 + i] - 1;
+        double v[24], center[3], volume, A[16];
+        for (i = 0; i < nvpe; ++i) {
+            int vid = conn[e * nvpe + i] - 1;
             v[3 * i] = vertices[3 * vid]; v[3 * i + 1] = vertices[3 * vid + 1]; v[3 * i + 2] = vertices[3 * vid + 2];
         }
-        for (i = 0; i < 3; ++i) center[i] = (v[i] + v[3 + i] + v[6 + i] + v[9 + i]) / 4.0;
-        volume = S->int_over_volume ? ora_tet_volume(v) : 1.0;
-        if (nbasis > 1) tet_abinv(v, A);
+        if (vox) {
+            ora_vox_center(v, center);
+            volume = S->int_over_volume ? ora_vox_volume(v) : 1.0;
+        } else {
+            for (i = 0; i < 3; ++i) center[i] = (v[i] + v[3 + i] + v[6 + i] + v[9 + i]) / 4.0;
+            volume = S->int_over_volume ? ora_tet_volume(v) : 1.0;
+            if (nbasis > 1) tet_abinv(v, A);
+        }
         if (halton_offsets) C->halton_step = halton_offsets[e];
         if (offsets_used) offsets_used[e] = C->halton_step;
         /* element containing the source: no integration (slave.f90:466-482) */
-        if (S->mask_src_rec && ora_point_in_tet(v, C->src_r)) {
+        if (S->mask_src_rec && (vox ? ora_point_in_vox(v, C->src_r) : ora_point_in_tet(v, C->src_r))) {
             for (k = 0; k < nk * nbasis; ++k) {
                 values[(size_t)e * nk * nbasis + k] = 0.0;
                 variance[(size_t)e * nk * nbasis + k] = 0.0;
@@ -1062,9 +1073,11 @@ int ora_slave_work(ora_ctx *C, int nelems, int nbasis, const double *vertices, c
                 if (!conv_all[k]) all = 0;
             }
             if (all) break;
-            ora_halton(3, seed, bases, leap, C->halton_step, P, hal);
+            if (S->quasirandom && !vox) ora_halton(3, seed, bases, leap, C->halton_step, P, hal);
+            else ora_prng_points(C->halton_step, P, hal);
             C->halton_step += P;
-            ora_tet_points(v, P, hal, pts);
+            if (vox) ora_vox_points(v, P, hal, pts);
+            else ora_tet_points(v, P, hal, pts);
             for (k = 0; k < nk; ++k) if (niter[k] == S->max_iter) stop = 1;
             if (stop) break;
             rc = batch_fwd(C, P, pts, F);
@@ -1074,7 +1087,8 @@ int ora_slave_work(ora_ctx *C, int nelems, int nbasis, const double *vertices, c
                 int allconv = 1;
                 /* receiver inside the element (slave.f90:539-545), then the travel-time
                  * mask (slave.f90:546-557): zero kernels, one iteration each */
-                if ((S->mask_src_rec && ora_point_in_tet(v, C->R[r].r)) || travel_time_masked(C, r, center)) {
+                if ((S->mask_src_rec && (vox ? ora_point_in_vox(v, C->R[r].r) : ora_point_in_tet(v, C->R[r].r)))
+                    || travel_time_masked(C, r, center)) {
                     for (k = R->first; k <= R->last; ++k) {
                         int b;
                         for (i = 0; i < P; ++i) for (b = 0; b < 6; ++b) base[((size_t)i * nk + k) * 6 + b] = 0.0;
@@ -1092,7 +1106,7 @@ int ora_slave_work(ora_ctx *C, int nelems, int nbasis, const double *vertices, c
             /* per basis function: weights, mult3d_1d, physical kernels, MC check
              * (slave.f90:723-744) */
             for (b = 0; b < nbasis; ++b) {
-                if (nbasis == 1) {
+                if (nbasis == 1 || vox) {
                     for (i = 0; i < P; ++i) wts[i] = 1.0;
                 } else if (tet_weights(v, A, b, P, pts, wts)) {
                     rc = MCK_EINVAL;                  /* weight outside [0, 1]: pabort */

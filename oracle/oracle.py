@@ -52,8 +52,19 @@ def lib():
         L.ora_locate.argtypes = [C.c_void_p, C.c_int, C.c_int, f64p, i32p, f64p, f64p, i32p]
         L.ora_load_strain.argtypes = [C.c_void_p, C.c_int, C.c_int, f64p, f64p]
         L.ora_base_kernels.argtypes = [C.c_void_p, C.c_int, f64p, f64p]
-        L.ora_slave_work.argtypes = [C.c_void_p, C.c_int, C.c_int, f64p, i32p, f64p, f64p, i32p,
-                                     i64p, i64p, i64p, i64p]
+        L.ora_slave_work.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, f64p, i32p, f64p,
+                                     f64p, i32p, i64p, i64p, i64p, i64p]
+        L.ora_splitmix64.argtypes = [C.c_uint64]
+        L.ora_splitmix64.restype = C.c_uint64
+        L.ora_prng.argtypes = [C.c_int64, C.c_int]
+        L.ora_prng.restype = C.c_double
+        L.ora_prng_points.argtypes = [C.c_int64, C.c_int, f64p]
+        L.ora_vox_range.argtypes = [f64p, f64p]
+        L.ora_vox_points.argtypes = [f64p, C.c_int, f64p, f64p]
+        L.ora_vox_volume.argtypes = [f64p]
+        L.ora_vox_volume.restype = C.c_double
+        L.ora_vox_center.argtypes = [f64p, f64p]
+        L.ora_point_in_vox.argtypes = [f64p, f64p]
         L.ora_tet_weights.argtypes = [f64p, C.c_int, f64p, f64p]
         L.ora_kernel_info.argtypes = [C.c_void_p, C.c_int, f64p, C.POINTER(C.c_int),
                                       C.POINTER(C.c_int)]
@@ -129,6 +140,39 @@ def tet_weights(v, pts):
     w = np.zeros((pts.shape[0], 4))
     bad = lib().ora_tet_weights(P(v), pts.shape[0], P(pts), P(w))
     return w, bool(bad)
+
+
+def prng_points(step, n):
+    r = np.zeros((n, 3))
+    lib().ora_prng_points(step, n, P(r))
+    return r
+
+
+def vox_range(v):
+    g = np.zeros(6)
+    lib().ora_vox_range(P(np.ascontiguousarray(v, float)), P(g))
+    return g
+
+
+def vox_points(v, coords):
+    coords = np.ascontiguousarray(coords, float)
+    out = np.zeros_like(coords)
+    lib().ora_vox_points(P(np.ascontiguousarray(v, float)), coords.shape[0], P(coords), P(out))
+    return out
+
+
+def vox_volume(v):
+    return lib().ora_vox_volume(P(np.ascontiguousarray(v, float)))
+
+
+def vox_center(v):
+    c = np.zeros(3)
+    lib().ora_vox_center(P(np.ascontiguousarray(v, float)), P(c))
+    return c
+
+
+def point_in_vox(v, q):
+    return bool(lib().ora_point_in_vox(P(np.ascontiguousarray(v, float)), P(np.ascontiguousarray(q, float))))
 
 
 def point_in_tet(v, q):
@@ -337,10 +381,12 @@ class Oracle:
         return norm.value, ncut.value, ifirst.value
 
     def slave_work(self, verts, conn, halton_offsets=None, nbasis=1):
-        """values/variance (ne, nk) for volumetric elements, (ne, nbasis, nk) otherwise"""
+        """values/variance (ne, nk) for volumetric elements, (ne, nbasis, nk) otherwise;
+        conn (ne, 4) tetrahedra or (ne, 8) voxels"""
         verts = np.ascontiguousarray(verts, float)
         conn = np.ascontiguousarray(conn, np.int32)
-        ne = conn.shape[0]
+        ne, nvpe = conn.shape
+        etype = 4 if nvpe == 4 else 5
         shape = (ne, self.nk) if nbasis == 1 else (ne, nbasis, self.nk)
         vals = np.zeros(shape)
         var = np.zeros(shape)
@@ -349,7 +395,7 @@ class Oracle:
         used = np.zeros(ne, np.int64)
         ev = np.zeros(1, np.int64)
         offs = None if halton_offsets is None else np.ascontiguousarray(halton_offsets, np.int64)
-        rc = lib().ora_slave_work(self.h, ne, nbasis, P(verts), P(conn, C.c_int32), P(vals), P(var),
+        rc = lib().ora_slave_work(self.h, etype, nvpe, ne, nbasis, P(verts), P(conn, C.c_int32), P(vals), P(var),
                                   P(nit, C.c_int32), P(step, C.c_int64), P(offs, C.c_int64),
                                   P(used, C.c_int64), P(ev, C.c_int64))
         if rc:

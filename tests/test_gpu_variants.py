@@ -94,3 +94,49 @@ def test_slave_work_variant(case):
     assert res["stats"]["evaluations"] == ref["evaluations"], name
     for j in range(p.nkernel):
         assert _rel(res["values"][:, j], ref["values"][:, j]) < TOL, (name, j)
+
+
+@pytest.mark.parametrize("nbasis", [1, 8])
+def test_voxel_elements(nbasis):
+    """Voxel inversion elements (random_voxel.f90): pseudo-random points in spherical
+    blocks, block volume and (as written) centre for the travel-time mask; pinned to the
+    oracle's stream positions.  With 8 basis functions every weight is 1."""
+    import mc_kernel_amd as mk
+    from oracle import oracle
+    p = synth.make_problem("tiny", ntet=2, allowed_error=1e-30, allowed_relative_error=0.05)
+    verts, conn = p.voxels(8, seed=11)
+    o = oracle.Oracle(p.setup)
+    ref = o.slave_work(verts, conn, nbasis=nbasis)
+    assert ref["niterations"].max() > 2          # several MC steps
+    k = mk.MCKernel(p.setup)
+    try:
+        res = k.slave_work(verts, conn, halton_offsets=ref["offsets"], nbasis=nbasis)
+    finally:
+        k.close()
+    assert res["values"].shape == ref["values"].shape
+    assert np.array_equal(res["niterations"], ref["niterations"])
+    v, r = res["values"].reshape(len(conn), -1), ref["values"].reshape(len(conn), -1)
+    assert np.abs(r).max() > 0
+    err = np.linalg.norm(v - r) / np.linalg.norm(r)
+    assert err < TOL, err
+    if nbasis == 8:
+        assert np.array_equal(res["values"][:, 0], res["values"][:, 7])
+
+
+def test_pseudorandom_tetrahedra():
+    """USE_PSEUDORANDOM_NUMBERS: the pseudo-random stream in tetrahedra, pinned to the
+    oracle's stream positions."""
+    import mc_kernel_amd as mk
+    from oracle import oracle
+    p = synth.make_problem("tiny", ntet=8)
+    p.setup.quasirandom = 0
+    o = oracle.Oracle(p.setup)
+    ref = o.slave_work(p.verts, p.conn)
+    k = mk.MCKernel(p.setup)
+    try:
+        res = k.slave_work(p.verts, p.conn, halton_offsets=ref["offsets"])
+    finally:
+        k.close()
+    assert np.array_equal(res["niterations"], ref["niterations"])
+    err = np.linalg.norm(res["values"] - ref["values"]) / np.linalg.norm(ref["values"])
+    assert err < TOL, err
