@@ -663,6 +663,8 @@ int build_kernels(State &G) {
             q = e;
         }
     }
+    G.pc.max_slots = 1;
+    for (int r = 0; r < G.R; ++r) G.pc.max_slots = std::max(G.pc.max_slots, recs[r].nslot);
     std::vector<float2> Hh(Hs.size() * nw);
     for (size_t h = 0; h < Hs.size(); ++h)
         for (int q = 0; q < nw; ++q) Hh[h * nw + q] = make_float2((float)Hs[h][q].real(), (float)Hs[h][q].imag());
@@ -731,18 +733,19 @@ hipError_t launch_fwd(State &G, int npts, const int *order, const Located *loc, 
     return hipErrorInvalidValue;
 }
 
-// LDS of the backward kernel: FFT buffers, reused for the base-kernel spectra, and the
-// quarter twiddle table
-size_t bwd_smem(int M, int nbk) {
+// LDS of the backward kernel: FFT buffers, reused for the base-kernel spectra, the
+// quarter twiddle table and the per-slot sums (4 waves + total) of the largest receiver
+size_t bwd_smem(int M, int nbk, int max_slots) {
     return ((sizeof(OpShared) + 15) / 16) * 16
-         + (std::max<size_t>(2 * (size_t)M, (size_t)nbk * (M + 1)) + M / 2) * sizeof(float2);
+         + (std::max<size_t>(2 * (size_t)M, (size_t)nbk * (M + 1)) + M / 2) * sizeof(float2)
+         + (size_t)5 * max_slots * sizeof(double);
 }
 
 // backward strain + evaluation of npairs (point, receiver) pairs q = p * R + r
 hipError_t launch_bwd(State &G, int npairs, const int *order, const Located *loc_f,
                       const Located *loc_b, double *phys, double *base_out, const uint8_t *active, int P) {
     if (npairs <= 0) return hipSuccess;
-    const size_t smem = bwd_smem(G.pc.M, G.pc.nbk_lds);
+    const size_t smem = bwd_smem(G.pc.M, G.pc.nbk_lds, G.pc.max_slots);
     const dim3 grid(grid_x8(npairs)), block(kStrainThreads);
     const int nf = G.pc.nd_f, nb = G.pc.nd_b;
 #define BWD(SP, NF, NB)                                                                           \

@@ -11,7 +11,8 @@ constexpr int kMaxKernelsPerRec = 256;
 constexpr int kNN = 12;          // nearest midpoints tested (readfields.f90:1067)
 constexpr int kStrainThreads = 256;
 constexpr int kEvalThreads = 256;
-constexpr int kMaxSlots = 64;    // distinct (window-filter, base kernel) pairs per receiver
+constexpr int kMaxSlots = 256;   // distinct (window-filter, base kernel) pairs per receiver
+static_assert(kMaxSlots <= kStrainThreads, "one thread per slot in the final slot reduction");
 
 // GLL/GLJ collocation points and Lagrange derivative matrices for npol = 4
 // (calc_gradient_terms, readfields.f90:3084-3150); G[i + 5 j] = l_i'(x_j).
@@ -77,6 +78,7 @@ struct PathConf {
     int need_bk;              // bit b set: base kernel b is used by some kernel
     int bk_lds[6];            // LDS array of base kernel b in the contraction (-1 = unused)
     int nbk_lds;
+    int max_slots;            // largest slot count of a receiver
     // dampen_field (readfields.f90:3184-3204): radius (<= 0 disables), source position
     double damp_radius;
     double src_r[3];
@@ -97,12 +99,11 @@ struct OpShared {
 
 // Per-pair coefficients of the frequency-domain contraction (backward kernel):
 // earth-frame strain = Q x carried components, for the fwd (Qf) and bwd (Qb) fields.
+// (The per-slot sums live in dynamic LDS, sized by the largest receiver.)
 struct EvalShared {
     float Qf[6][6];
     float Amu[6][6];        // Qf^T Qb: F.B = f^T Amu b
     float uf[6], ub[6];     // trace rows (1,1,1,0,0,0) Q
-    double slotval[kMaxSlots];
-    double red[4][kMaxSlots];
 };
 
 constexpr int kSlotGroup = 6;        // slots sharing one window spectrum, summed together

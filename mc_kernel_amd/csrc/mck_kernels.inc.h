@@ -1000,6 +1000,8 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && NDB <= 4) ? 3 : (SPT <
     float2 *bufA = reinterpret_cast<float2 *>(smem_raw + ((sizeof(OpShared) + 15) / 16) * 16);
     float2 *bufB = bufA + pc.M;
     float2 *Tq = bufA + max(2 * pc.M, pc.nbk_lds * (pc.M + 1));
+    double *slot_red = reinterpret_cast<double *>(Tq + (pc.M >> 1));   // [4][max_slots]
+    double *slotval = slot_red + 4 * pc.max_slots;                      // [max_slots]
     __shared__ EvalShared E;
 
     const int lb = xcd_block(npairs);
@@ -1201,14 +1203,14 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && NDB <= 4) ? 3 : (SPT <
         for (int t = 0; t < kSlotGroup; ++t)
             if (t < ng) {
                 const double v = wave_sum_lane63(a2[t]);
-                if (lane == 63) E.red[wave][s0 + t] = v;
+                if (lane == 63) slot_red[wave * pc.max_slots + s0 + t] = v;
             }
     }
     __syncthreads();
     if (tid < rd.nslot) {
         double t = 0.0;
-        for (int w = 0; w < kStrainThreads / 64; ++w) t += E.red[w][tid];
-        E.slotval[tid] = t;
+        for (int w = 0; w < kStrainThreads / 64; ++w) t += slot_red[w * pc.max_slots + tid];
+        slotval[tid] = t;
     }
     __syncthreads();
     // physical kernels (kernel.f90:665-832) with the model at the point (fp32 values)
@@ -1236,7 +1238,7 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && NDB <= 4) ? 3 : (SPT <
         const int k = rd.first_kernel + kk;
         const KernDesc &kd = kern[k];
         double b[6];
-        for (int qq = 0; qq < 6; ++qq) b[qq] = kd.slot[qq] >= 0 ? E.slotval[kd.slot[qq]] * damp : 0.0;
+        for (int qq = 0; qq < 6; ++qq) b[qq] = kd.slot[qq] >= 0 ? slotval[kd.slot[qq]] * damp : 0.0;
         if (base_out) {
             for (int qq = 0; qq < 6; ++qq) base_out[((size_t)p * nk + k) * 6 + qq] = b[qq];
             continue;

@@ -8,6 +8,9 @@ The workloads follow the benchmark configurations of this path:
   c1    20,000-quad DB, nt=2048, 10 receivers at 10..100 deg, Gabor 20/30/40/60 s,
         CC and AM for vp and vs (160 kernels), 50,000 tetrahedra, relative MC
         tolerance 1e-3, 16 points per step, MAXIMUM_ITERATIONS 64 (<= 1,024 points).
+  c2    c1's database shape with 50 receivers at 2..100 deg, Gabor 10/20/30/40/60 s and a
+        Butterworth band pass 80-8 s, CC and AM for vp and vs (1200 kernels), 200,000
+        tetrahedra, 16 points per step, up to 256 steps.
   tiny  a few-hundred-quad DB with nt=256, dt=4 s for CPU tests.
 """
 import ctypes as C
@@ -371,6 +374,16 @@ def make_problem(name, **over):
                   filters=[("Gabor", (p, 0.5, 0.0, 0.0)) for p in (20.0, 30.0, 40.0, 60.0)],
                   kernels=_kernels(10, 4, ["CC", "AM"], ["vp", "vs"]), ntet=50000,
                   tet_edge=300e3, npoints_per_step=16, max_iter=64,
+                  allowed_error=1e-30, allowed_relative_error=1e-3)
+    elif name == "c2":
+        # 50 receivers x 6 filters x CC/AM x vp/vs = 1200 kernels (the reference's rho
+        # kernels of this configuration are rejected, as the reference aborts on them)
+        kw = dict(nelem_target=20000, nt=2048, dt=1.0,
+                  receivers_deg=[2.0 * (i + 1) for i in range(50)],
+                  filters=[("Gabor", (p, 0.5, 0.0, 0.0)) for p in (10.0, 20.0, 30.0, 40.0, 60.0)]
+                          + [("Butterw_BP", (80.0, 8.0, 4.0, 0.0))],
+                  kernels=_kernels(50, 6, ["CC", "AM"], ["vp", "vs"]), ntet=200000,
+                  tet_edge=300e3, npoints_per_step=16, max_iter=256,
                   allowed_error=1e-30, allowed_relative_error=1e-3)
     elif name == "tiny":
         kw = dict(nelem_target=450, nt=256, dt=4.0, receivers_deg=[30.0, 60.0],

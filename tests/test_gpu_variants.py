@@ -140,3 +140,30 @@ def test_pseudorandom_tetrahedra():
     assert np.array_equal(res["niterations"], ref["niterations"])
     err = np.linalg.norm(res["values"] - ref["values"]) / np.linalg.norm(ref["values"])
     assert err < TOL, err
+
+
+def test_c2_kernel_set():
+    """The c2 kernel set: 50 receivers x (5 Gabor + Butterworth band pass) x CC/AM x vp/vs
+    = 1200 kernels, 72 (window spectrum, base kernel) slots per receiver; reduced
+    database and task so that the oracle runs in seconds."""
+    import mc_kernel_amd as mk
+    from oracle import oracle
+    p = synth.make_problem("c2", nelem_target=2000, nt=1024, ntet=2, npoints_per_step=8, max_iter=2,
+                           unmasked_frac=1.0)
+    assert p.nkernel == 1200
+    o = oracle.Oracle(p.setup)
+    x = p.sample_points(4, seed=9)
+    ref_b = o.base_kernels(x)
+    ref = o.slave_work(p.verts, p.conn)
+    k = mk.MCKernel(p.setup)
+    try:
+        g_b = k.base_kernels(x)
+        res = k.slave_work(p.verts, p.conn, halton_offsets=ref["offsets"])
+    finally:
+        k.close()
+    err_b = np.linalg.norm(g_b - ref_b) / np.linalg.norm(ref_b)
+    assert err_b < TOL, err_b
+    assert np.array_equal(res["niterations"], ref["niterations"])
+    assert res["stats"]["evaluations"] == ref["evaluations"]
+    err = np.linalg.norm(res["values"] - ref["values"]) / np.linalg.norm(ref["values"])
+    assert err < TOL, err
