@@ -128,6 +128,7 @@ __device__ inline void ad_basekernels_adjoint(const float2 (&f)[ND], const float
 // holding the result; ends with a barrier.
 __device__ inline float2 *lds_cfft(float2 *in, float2 *ob, int M, const float2 *Tq)
 {
+#pragma clang fp contract(fast)
     const int tid = threadIdx.x;
     const int lM = 31 - __clz(M);
     int Ns = 1, lNs = 0;
@@ -177,6 +178,7 @@ template <int SPT>
 __device__ inline void rfft_adjoint(const float2 (&g)[SPT + 1], int t0, int M, float dt, float2 *bufA,
                                     float2 *bufB, float2 *zM, const float2 *Tq, float (&y)[SPT])
 {
+#pragma clang fp contract(fast)
     const int tid = threadIdx.x, lM = 31 - __clz(M);
 #pragma unroll
     for (int j = 0; j <= SPT; ++j) {

@@ -763,6 +763,11 @@ __device__ inline void taper_factors(int t0, int nt, const PathConf &pc, float (
     }
 }
 
+// The FFTs and the frequency-domain products allow FMA contraction (fp contract fast):
+// they are FP32 work with a 1e-5 tolerance.  The translation unit is built with
+// -ffp-contract=off so that the point-location and Halton arithmetic match the
+// reference operation for operation.
+
 // w_N^i = exp(-2 pi i i / N) from the quarter table T[r] = w_N^r, r < N/4 (in LDS):
 // w_N^(q N/4 + r) = (-i)^q w_N^r
 __device__ inline float2 twiddle(const float2 *T, int i, int lq)
@@ -787,6 +792,7 @@ template <int SPT>
 __device__ inline const float2 *lds_fft(const float (&x)[SPT], int t0, int nt, int M, float2 *bufA,
                                         float2 *bufB, const float2 *Tq)
 {
+#pragma clang fp contract(fast)
     const int tid = threadIdx.x;
     const int lM = 31 - __clz(M);
     float2 *in = bufA, *ob = bufB;
@@ -858,6 +864,7 @@ __device__ inline const float2 *lds_fft(const float (&x)[SPT], int t0, int nt, i
 // FFTW r2c convention of fft.f90:375-413 scaled as rfft_type%rfft does)
 __device__ inline float2 rfft_bin(const float2 *Z, int k, int M, const float2 *Tq, float dt)
 {
+#pragma clang fp contract(fast)
     float2 zk = Z[k == M ? 0 : k];
     float2 zm = Z[(M - k) & (M - 1)];
     float2 zc = make_float2(zm.x, -zm.y);
@@ -869,7 +876,11 @@ __device__ inline float2 rfft_bin(const float2 *Z, int k, int M, const float2 *T
     return make_float2(X.x * dt, X.y * dt);
 }
 
-__device__ inline float2 cmulf(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ inline float2 cmulf(float2 a, float2 b)
+{
+#pragma clang fp contract(fast)
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
 
 __device__ inline double wave_sum(double v)
 {
@@ -1084,6 +1095,7 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && NDB <= 4) ? 3 : (SPT <
     constexpr int JB = NJ < 3 ? NJ : 3;                  // bins whose loads are issued together
 #pragma unroll
     for (int j0 = 0; j0 < NJ; j0 += JB) {
+#pragma clang fp contract(fast)
         float2 fv[JB][NDF];
 #pragma unroll
         for (int jj = 0; jj < JB; ++jj) {
