@@ -698,7 +698,7 @@ int build_kernels(State &G) {
 
 // LDS of the forward kernel: two FFT buffers and the quarter twiddle table
 size_t strain_smem(int M) {
-    return ((sizeof(OpShared) + 15) / 16) * 16 + (2 * (size_t)M + M / 2) * sizeof(float2);
+    return ((sizeof(OpShared) + 15) / 16) * 16 + (2 * (size_t)M + tq_len(M)) * sizeof(float2);
 }
 
 // samples per thread of the strain kernels for a trace length
@@ -737,7 +737,7 @@ hipError_t launch_fwd(State &G, int npts, const int *order, const Located *loc, 
 // quarter twiddle table and the per-slot sums (4 waves + total) of the largest receiver
 size_t bwd_smem(int M, int nbk, int max_slots) {
     return ((sizeof(OpShared) + 15) / 16) * 16
-         + (std::max<size_t>(2 * (size_t)M, (size_t)nbk * (M + 1)) + M / 2) * sizeof(float2)
+         + (std::max<size_t>(2 * (size_t)M, (size_t)nbk * (M + 1)) + tq_len(M)) * sizeof(float2)
          + (size_t)5 * max_slots * sizeof(double);
 }
 
@@ -1462,7 +1462,7 @@ int locate_frame(State &G, int frame, int npts) {
 }
 
 // ---- the differentiable filter-convolve-window-integrate stage ------------------------
-size_t ad_smem(int M) { return (2 * (size_t)M + M / 2) * sizeof(float2); }
+size_t ad_smem(int M) { return (2 * (size_t)M + tq_len(M)) * sizeof(float2); }
 
 bool device_ptr(const void *p) {
     hipPointerAttribute_t a;

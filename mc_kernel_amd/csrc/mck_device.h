@@ -14,6 +14,11 @@ constexpr int kEvalThreads = 256;
 constexpr int kMaxSlots = 256;   // distinct (window-filter, base kernel) pairs per receiver
 static_assert(kMaxSlots <= kStrainThreads, "one thread per slot in the final slot reduction");
 
+// LDS entries of the quarter twiddle table of an M-point complex FFT: M/2 values, one
+// padding slot after every 16 so that strided lookups spread over the LDS banks
+__host__ __device__ constexpr int tq_len(int M) { return (M >> 1) + (M >> 5) + 1; }
+__host__ __device__ constexpr int tq_slot(int r) { return r + (r >> 4); }
+
 // GLL/GLJ collocation points and Lagrange derivative matrices for npol = 4
 // (calc_gradient_terms, readfields.f90:3084-3150); G[i + 5 j] = l_i'(x_j).
 struct Basis {

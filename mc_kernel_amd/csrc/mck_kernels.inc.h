@@ -768,19 +768,26 @@ __device__ inline void taper_factors(int t0, int nt, const PathConf &pc, float (
 // -ffp-contract=off so that the point-location and Halton arithmetic match the
 // reference operation for operation.
 
+__device__ inline float2 cmulf(float2 a, float2 b)
+{
+#pragma clang fp contract(fast)
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
 // w_N^i = exp(-2 pi i i / N) from the quarter table T[r] = w_N^r, r < N/4 (in LDS):
 // w_N^(q N/4 + r) = (-i)^q w_N^r
 __device__ inline float2 twiddle(const float2 *T, int i, int lq)
 {
     const int q = (i >> lq) & 3;
-    const float2 w = T[i & ((1 << lq) - 1)];
+    const float2 w = T[tq_slot(i & ((1 << lq) - 1))];
     return q == 0 ? w : q == 1 ? make_float2(w.y, -w.x) : q == 2 ? make_float2(-w.x, -w.y) : make_float2(-w.y, w.x);
 }
 
-// copies the quarter table (N/4 = M/2 entries) into LDS; the caller's barrier publishes it
+// copies the quarter table (N/4 = M/2 entries) into LDS at the padded slots tq_slot(r);
+// the caller's barrier publishes it
 __device__ inline void load_twiddles(const float2 *twq, int M, float2 *Tq)
 {
-    for (int r = threadIdx.x; r < (M >> 1); r += kStrainThreads) Tq[r] = twq[r];
+    for (int r = threadIdx.x; r < (M >> 1); r += kStrainThreads) Tq[tq_slot(r)] = twq[r];
 }
 
 // Packs the zero-padded real sequence x (length 2M) as z[n] = x[2n] + i x[2n+1] and runs
@@ -840,8 +847,8 @@ __device__ inline const float2 *lds_fft(const float (&x)[SPT], int t0, int nt, i
             const float2 v0 = in[j], v1 = in[j + Mq], v2 = in[j + 2 * Mq], v3 = in[j + 3 * Mq];
             // w_M^t = w_N^(2t), N = 2M
             const int ti = kk << (lstride + 1);
-            const float2 w1 = twiddle(Tq, ti, lM - 1), w2 = twiddle(Tq, 2 * ti, lM - 1),
-                         w3 = twiddle(Tq, 3 * ti, lM - 1);
+            // w2 = w1^2, w3 = w1 w2 (one table lookup per butterfly)
+            const float2 w1 = twiddle(Tq, ti, lM - 1), w2 = cmulf(w1, w1), w3 = cmulf(w1, w2);
             const float2 a1 = make_float2(v1.x * w1.x - v1.y * w1.y, v1.x * w1.y + v1.y * w1.x);
             const float2 a2 = make_float2(v2.x * w2.x - v2.y * w2.y, v2.x * w2.y + v2.y * w2.x);
             const float2 a3 = make_float2(v3.x * w3.x - v3.y * w3.y, v3.x * w3.y + v3.y * w3.x);
@@ -876,11 +883,6 @@ __device__ inline float2 rfft_bin(const float2 *Z, int k, int M, const float2 *T
     return make_float2(X.x * dt, X.y * dt);
 }
 
-__device__ inline float2 cmulf(float2 a, float2 b)
-{
-#pragma clang fp contract(fast)
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
 
 __device__ inline double wave_sum(double v)
 {
@@ -1011,7 +1013,7 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && NDB <= 4) ? 3 : (SPT <
     float2 *bufA = reinterpret_cast<float2 *>(smem_raw + ((sizeof(OpShared) + 15) / 16) * 16);
     float2 *bufB = bufA + pc.M;
     float2 *Tq = bufA + max(2 * pc.M, pc.nbk_lds * (pc.M + 1));
-    double *slot_red = reinterpret_cast<double *>(Tq + (pc.M >> 1));   // [4][max_slots]
+    double *slot_red = reinterpret_cast<double *>(Tq + tq_len(pc.M));   // [4][max_slots]
     double *slotval = slot_red + 4 * pc.max_slots;                      // [max_slots]
     __shared__ EvalShared E;
 
