@@ -151,8 +151,8 @@ __device__ inline float2 *lds_cfft(float2 *in, float2 *ob, int M, const float2 *
             const int kk = j & (Ns - 1);
             const float2 v0 = in[j], v1 = in[j + Mq], v2 = in[j + 2 * Mq], v3 = in[j + 3 * Mq];
             const int ti = kk << (lstride + 1);
-            // w2 = w1^2, w3 = w1 w2 (one table lookup per butterfly)
-            const float2 w1 = twiddle(Tq, ti, lM - 1), w2 = cmulf(w1, w1), w3 = cmulf(w1, w2);
+            // ti < N/4: w1 lies in the table's quadrant; w2 = w1^2, w3 = w1 w2
+            const float2 w1 = Tq[tq_slot(ti)], w2 = cmulf(w1, w1), w3 = cmulf(w1, w2);
             const float2 a1 = cmulf(v1, w1), a2 = cmulf(v2, w2), a3 = cmulf(v3, w3);
             const float2 s02 = make_float2(v0.x + a2.x, v0.y + a2.y), d02 = make_float2(v0.x - a2.x, v0.y - a2.y);
             const float2 s13 = make_float2(a1.x + a3.x, a1.y + a3.y), d13 = make_float2(a1.x - a3.x, a1.y - a3.y);

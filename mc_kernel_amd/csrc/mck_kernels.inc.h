@@ -847,8 +847,8 @@ __device__ inline const float2 *lds_fft(const float (&x)[SPT], int t0, int nt, i
             const float2 v0 = in[j], v1 = in[j + Mq], v2 = in[j + 2 * Mq], v3 = in[j + 3 * Mq];
             // w_M^t = w_N^(2t), N = 2M
             const int ti = kk << (lstride + 1);
-            // w2 = w1^2, w3 = w1 w2 (one table lookup per butterfly)
-            const float2 w1 = twiddle(Tq, ti, lM - 1), w2 = cmulf(w1, w1), w3 = cmulf(w1, w2);
+            // ti < N/4: w1 lies in the table's quadrant; w2 = w1^2, w3 = w1 w2
+            const float2 w1 = Tq[tq_slot(ti)], w2 = cmulf(w1, w1), w3 = cmulf(w1, w2);
             const float2 a1 = make_float2(v1.x * w1.x - v1.y * w1.y, v1.x * w1.y + v1.y * w1.x);
             const float2 a2 = make_float2(v2.x * w2.x - v2.y * w2.y, v2.x * w2.y + v2.y * w2.x);
             const float2 a3 = make_float2(v3.x * w3.x - v3.y * w3.y, v3.x * w3.y + v3.y * w3.x);
