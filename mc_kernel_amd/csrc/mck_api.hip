@@ -328,8 +328,9 @@ struct State {
     DBuf<DevFrame> frames;
     std::vector<DevFrame> frames_host;
     DBuf<float2> twq, H;
-    DBuf<RecDesc> recs;
-    DBuf<KernDesc> kerns;
+    DBuf<RecDesc> recs, recs_base;           // slot layouts: physical kernels / base kernels
+    DBuf<KernDesc> kerns, kerns_base;
+    PathConf pc_base{};                      // pc with the base-kernel layout
     DBuf<float> model;
     std::vector<int> k_rec, k_nbk, rec_first, rec_count;
     DBuf<int> d_k_rec, d_k_nbk, d_rec_first, d_rec_count;
@@ -533,8 +534,8 @@ int build_kernels(State &G) {
     std::vector<std::vector<cd>> Hs, Ws;
     std::vector<int> hfilt;
     std::vector<RecDesc> recs(G.R);
-    std::vector<KernDesc> kerns(G.nk);
-    for (int r = 0; r < G.R; ++r) { recs[r].nslot = 0; recs[r].first_kernel = -1; recs[r].nkernel = 0; }
+    std::vector<int> h_of(G.nk, 0);
+    for (int r = 0; r < G.R; ++r) { recs[r] = RecDesc{}; recs[r].nslot = 0; recs[r].first_kernel = -1; recs[r].nkernel = 0; }
     G.k_rec.assign(G.nk, 0);
     G.k_nbk.assign(G.nk, 0);
     G.t_last_p.assign(G.R, 0.0);
@@ -617,54 +618,96 @@ int build_kernels(State &G) {
             hfilt.push_back(fi);
             hidx[key] = h;
         }
-        KernDesc &kd = kerns[k];
-        kd.param = pa;
-        for (int b = 0; b < 6; ++b) {
-            kd.slot[b] = -1;
-            if (!nb[b]) continue;
-            int s = -1;
-            for (int q = 0; q < rd.nslot; ++q)
-                if (rd.slot_h[q] == h && rd.slot_bk[q] == b) s = q;
-            if (s < 0) {
-                if (rd.nslot >= kMaxSlots) return fail(MCK_EUNSUPPORTED, "too many distinct windows per receiver");
-                s = rd.nslot++;
-                rd.slot_h[s] = h;
-                rd.slot_bk[s] = b;
-            }
-            kd.slot[b] = s;
-        }
+        h_of[k] = h;
     }
     for (int r = 0; r < G.R; ++r)
         if (recs[r].first_kernel < 0) return fail(MCK_EINVAL, "receiver without kernels");
-    // order each receiver's slots by window spectrum (one H read per bin and group)
-    for (int r = 0; r < G.R; ++r) {
-        RecDesc &rd = recs[r];
-        std::vector<int> perm(rd.nslot), inv(rd.nslot);
-        for (int q = 0; q < rd.nslot; ++q) perm[q] = q;
-        std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return rd.slot_h[a] < rd.slot_h[b]; });
-        RecDesc o = rd;
-        for (int q = 0; q < rd.nslot; ++q) {
-            rd.slot_h[q] = o.slot_h[perm[q]];
-            rd.slot_bk[q] = o.slot_bk[perm[q]];
-            inv[perm[q]] = q;
+    // Two slot layouts: the base kernels themselves (mck_base_kernels and the
+    // differentiable stage) and, for the physical kernels, the model-independent
+    // combinations each parameter reads (one slot where the reference combines two)
+    auto layout = [&](bool combo, std::vector<RecDesc> &lr, std::vector<KernDesc> &lk, PathConf &pc) -> int {
+        lr = recs;
+        lk.assign(G.nk, KernDesc{});
+        int stored = 0;
+        for (int k = 0; k < G.nk; ++k) {
+            int nb[kNumBk] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            if (needs_of(S.kernel_param[k], nb)) return fail(MCK_EINVAL, "unknown model parameter");
+            if (combo) {
+                switch (S.kernel_param[k]) {
+                case MCK_PARAM_VS: nb[0] = nb[1] = 0; nb[6] = 1; break;                 // mu - 2 lambda
+                case MCK_PARAM_VPV: nb[0] = nb[3] = nb[5] = 0; nb[7] = 1; break;       // lambda - a - c
+                case MCK_PARAM_VSH: nb[0] = nb[1] = nb[4] = 0; nb[8] = 1; break;       // + 2 (1 - eta) c
+                default: break;
+                }
+            }
+            RecDesc &rd = lr[S.kernel_receiver[k]];
+            KernDesc &kd = lk[k];
+            kd.param = S.kernel_param[k];
+            for (int b = 0; b < kNumBk; ++b) {
+                kd.slot[b] = -1;
+                if (!nb[b]) continue;
+                stored |= 1 << b;
+                int s = -1;
+                for (int q = 0; q < rd.nslot; ++q)
+                    if (rd.slot_h[q] == h_of[k] && rd.slot_bk[q] == b) s = q;
+                if (s < 0) {
+                    if (rd.nslot >= kMaxSlots) return fail(MCK_EUNSUPPORTED, "too many distinct windows per receiver");
+                    s = rd.nslot++;
+                    rd.slot_h[s] = h_of[k];
+                    rd.slot_bk[s] = b;
+                }
+                kd.slot[b] = s;
+            }
         }
-        for (int k = rd.first_kernel; k < rd.first_kernel + rd.nkernel; ++k)
-            for (int b = 0; b < 6; ++b)
-                if (kerns[k].slot[b] >= 0) kerns[k].slot[b] = inv[kerns[k].slot[b]];
-        for (int q = 0; q < rd.nslot; ++q) rd.slot_lds[q] = G.pc.bk_lds[rd.slot_bk[q]];
-        rd.ngroup = 0;
-        for (int q = 0; q < rd.nslot;) {
-            int e = q + 1;
-            while (e < rd.nslot && e - q < kSlotGroup && rd.slot_h[e] == rd.slot_h[q]) ++e;
-            rd.grp_h[rd.ngroup] = rd.slot_h[q];
-            rd.grp_first[rd.ngroup] = q;
-            rd.grp_n[rd.ngroup] = e - q;
-            ++rd.ngroup;
-            q = e;
+        // base kernels formed per bin: the stored ones and the parts of the combinations
+        int need = stored & 0x3f;
+        if (stored & (1 << 6)) need |= 1 | 2;
+        if (stored & (1 << 7)) need |= 1 | 8 | 32;
+        if (stored & (1 << 8)) need |= 1 | 2 | 16;
+        pc.need_bk = need;
+        pc.nbk_lds = 0;
+        for (int b = 0; b < kNumBk; ++b) pc.bk_lds[b] = (stored >> b) & 1 ? pc.nbk_lds++ : -1;
+        // order each receiver's slots by window spectrum (one H read per bin and group)
+        for (int r = 0; r < G.R; ++r) {
+            RecDesc &rd = lr[r];
+            std::vector<int> perm(rd.nslot), inv(rd.nslot);
+            for (int q = 0; q < rd.nslot; ++q) perm[q] = q;
+            std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return rd.slot_h[a] < rd.slot_h[b]; });
+            RecDesc o = rd;
+            for (int q = 0; q < rd.nslot; ++q) {
+                rd.slot_h[q] = o.slot_h[perm[q]];
+                rd.slot_bk[q] = o.slot_bk[perm[q]];
+                inv[perm[q]] = q;
+            }
+            for (int k = rd.first_kernel; k < rd.first_kernel + rd.nkernel; ++k)
+                for (int b = 0; b < kNumBk; ++b)
+                    if (lk[k].slot[b] >= 0) lk[k].slot[b] = inv[lk[k].slot[b]];
+            for (int q = 0; q < rd.nslot; ++q) rd.slot_lds[q] = pc.bk_lds[rd.slot_bk[q]];
+            rd.ngroup = 0;
+            for (int q = 0; q < rd.nslot;) {
+                int e = q + 1;
+                while (e < rd.nslot && e - q < kSlotGroup && rd.slot_h[e] == rd.slot_h[q]) ++e;
+                rd.grp_h[rd.ngroup] = rd.slot_h[q];
+                rd.grp_first[rd.ngroup] = q;
+                rd.grp_n[rd.ngroup] = e - q;
+                ++rd.ngroup;
+                q = e;
+            }
         }
+        pc.max_slots = 1;
+        for (int r = 0; r < G.R; ++r) pc.max_slots = std::max(pc.max_slots, lr[r].nslot);
+        return MCK_OK;
+    };
+    std::vector<RecDesc> recs_base;
+    std::vector<KernDesc> kerns, kerns_base;
+    G.pc_base = G.pc;
+    int lrc;
+    if ((lrc = layout(false, recs_base, kerns_base, G.pc_base))) return lrc;
+    {
+        std::vector<RecDesc> rc2;
+        if ((lrc = layout(true, rc2, kerns, G.pc))) return lrc;
+        recs = rc2;
     }
-    G.pc.max_slots = 1;
-    for (int r = 0; r < G.R; ++r) G.pc.max_slots = std::max(G.pc.max_slots, recs[r].nslot);
     std::vector<float2> Hh(Hs.size() * nw);
     for (size_t h = 0; h < Hs.size(); ++h)
         for (int q = 0; q < nw; ++q) Hh[h * nw + q] = make_float2((float)Hs[h][q].real(), (float)Hs[h][q].imag());
@@ -681,6 +724,10 @@ int build_kernels(State &G) {
     HIPCHECK(hipMemcpy(G.recs.p, recs.data(), G.R * sizeof(RecDesc), hipMemcpyHostToDevice));
     HIPCHECK(G.kerns.ensure(G.nk));
     HIPCHECK(hipMemcpy(G.kerns.p, kerns.data(), G.nk * sizeof(KernDesc), hipMemcpyHostToDevice));
+    HIPCHECK(G.recs_base.ensure(G.R));
+    HIPCHECK(hipMemcpy(G.recs_base.p, recs_base.data(), G.R * sizeof(RecDesc), hipMemcpyHostToDevice));
+    HIPCHECK(G.kerns_base.ensure(G.nk));
+    HIPCHECK(hipMemcpy(G.kerns_base.p, kerns_base.data(), G.nk * sizeof(KernDesc), hipMemcpyHostToDevice));
     G.rec_first.resize(G.R);
     G.rec_count.resize(G.R);
     for (int r = 0; r < G.R; ++r) { G.rec_first[r] = recs[r].first_kernel; G.rec_count[r] = recs[r].nkernel; }
@@ -745,17 +792,22 @@ size_t bwd_smem(int M, int nbk, int max_slots) {
 hipError_t launch_bwd(State &G, int npairs, const int *order, const Located *loc_f,
                       const Located *loc_b, double *phys, double *base_out, const uint8_t *active, int P) {
     if (npairs <= 0) return hipSuccess;
-    const size_t smem = bwd_smem(G.pc.M, G.pc.nbk_lds, G.pc.max_slots);
+    // base kernels come from the base-kernel slot layout, physical kernels from the
+    // combined one
+    const PathConf &pc = base_out ? G.pc_base : G.pc;
+    const RecDesc *recs = base_out ? G.recs_base.p : G.recs.p;
+    const KernDesc *kerns = base_out ? G.kerns_base.p : G.kerns.p;
+    const size_t smem = bwd_smem(pc.M, pc.nbk_lds, pc.max_slots);
     const dim3 grid(grid_x8(npairs)), block(kStrainThreads);
-    const int nf = G.pc.nd_f, nb = G.pc.nd_b;
+    const int nf = pc.nd_f, nb = pc.nd_b;
 #define BWD(SP, NF, NB)                                                                           \
-    if (spt_for(G.pc.nt) == SP && nf == NF && nb == NB) {                                         \
+    if (spt_for(pc.nt) == SP && nf == NF && nb == NB) {                                           \
         if (smem > 65536)                                                                         \
             (void)hipFuncSetAttribute((const void *)&k_bwd_eval<SP, NF, NB>,                      \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);     \
-        hipLaunchKernelGGL((k_bwd_eval<SP, NF, NB>), grid, block, smem, G.stream, G.bwd.d, G.pc,  \
+        hipLaunchKernelGGL((k_bwd_eval<SP, NF, NB>), grid, block, smem, G.stream, G.bwd.d, pc,    \
                            G.frames.p, npairs, order, G.R, loc_f, loc_b, G.twq.p,                \
-                           G.Fspec.p, G.recs.p, G.kerns.p, G.H.p, G.model_idx.p, G.model.p,     \
+                           G.Fspec.p, recs, kerns, G.H.p, G.model_idx.p, G.model.p,              \
                            G.S.model_ndepth, G.S.relative_kernel, G.nk, phys, base_out,         \
                            G.xyz.p, active, P, G.d_rec_r.p);                                     \
         return hipGetLastError();                                                                 \
@@ -942,14 +994,6 @@ int init_impl(const mck_setup_t *s) {
     carry(fwd_mono, &pc.nd_f, pc.comps_f);
     carry(bwd_mono, &pc.nd_b, pc.comps_b);
     if (pc.nd_f == 1 && pc.nd_b != 1) return fail(MCK_EINVAL, "inconsistent strain types");
-    pc.need_bk = 0;
-    for (int k = 0; k < S.nkernel; ++k) {
-        int nb[6];
-        if (needs_of(S.kernel_param[k], nb) == 0)
-            for (int b = 0; b < 6; ++b) if (nb[b]) pc.need_bk |= 1 << b;
-    }
-    pc.nbk_lds = 0;
-    for (int b = 0; b < 6; ++b) pc.bk_lds[b] = (pc.need_bk >> b) & 1 ? pc.nbk_lds++ : -1;
     pc.ntaper = 10;
     for (int j = 0; j < 10; ++j) {
         float q = 3.3f * (float)(10 - j) / 10.0f;
@@ -1484,20 +1528,20 @@ int ad_check(const State &G, int npts, std::initializer_list<const void *> ptrs)
 
 // W_h G_f^2 of every window spectrum and the forward spectra of the traces
 hipError_t ad_prologue(State &G, int npts, const float2 *tf, const float *fw, hipStream_t st) {
-    const int nw = G.pc.nomega, nd = G.pc.nd, M = G.pc.M;
+    const int nw = G.pc_base.nomega, nd = G.pc_base.nd, M = G.pc_base.M;
     hipError_t e = G.ad_Hg.ensure((size_t)G.nH * nw);
     if (e == hipSuccess) e = G.ad_Fs.ensure((size_t)npts * nd * nw);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_ad_hg, dim3((G.nH * nw + 255) / 256), dim3(256), 0, st, G.nH, nw, G.ad_W.p,
                        G.ad_hfilt.p, tf, G.ad_Hg.p);
     const size_t smem = ad_smem(M);
-    switch (spt_for(G.pc.nt)) {
+    switch (spt_for(G.pc_base.nt)) {
 #define CASE(SP)                                                                                  \
     case SP:                                                                                      \
         if (smem > 65536)                                                                         \
             (void)hipFuncSetAttribute((const void *)&k_ad_fspec<SP>,                              \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);     \
-        hipLaunchKernelGGL((k_ad_fspec<SP>), dim3(npts), dim3(kStrainThreads), smem, st, G.pc,    \
+        hipLaunchKernelGGL((k_ad_fspec<SP>), dim3(npts), dim3(kStrainThreads), smem, st, G.pc_base,    \
                            npts, nd, fw, G.twq.p, G.ad_Fs.p);                                    \
         break;
         CASE(1) CASE(2) CASE(4) CASE(8) CASE(16)
@@ -1508,15 +1552,15 @@ hipError_t ad_prologue(State &G, int npts, const float2 *tf, const float *fw, hi
 }
 
 hipError_t ad_launch_eval(State &G, int npts, const float *bw, double *base, hipStream_t st) {
-    const size_t smem = ad_smem(G.pc.M);
+    const size_t smem = ad_smem(G.pc_base.M);
     const int npairs = npts * G.R;
 #define EVAL(SP, ND)                                                                              \
-    if (spt_for(G.pc.nt) == SP && G.pc.nd == ND) {                                                \
+    if (spt_for(G.pc_base.nt) == SP && G.pc_base.nd == ND) {                                                \
         if (smem > 65536)                                                                         \
             (void)hipFuncSetAttribute((const void *)&k_ad_eval<SP, ND>,                           \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);     \
         hipLaunchKernelGGL((k_ad_eval<SP, ND>), dim3(npairs), dim3(kStrainThreads), smem, st,     \
-                           G.pc, npairs, G.R, bw, G.twq.p, G.ad_Fs.p, G.recs.p, G.kerns.p,       \
+                           G.pc_base, npairs, G.R, bw, G.twq.p, G.ad_Fs.p, G.recs_base.p, G.kerns_base.p,       \
                            G.ad_Hg.p, G.nk, base);                                                \
         return hipGetLastError();                                                                 \
     }
@@ -1528,15 +1572,15 @@ hipError_t ad_launch_eval(State &G, int npts, const float *bw, double *base, hip
 
 hipError_t ad_launch_grad(State &G, int npts, const float *bw, const double *gbase, float *gbw,
                           float *gfw, hipStream_t st) {
-    const size_t smem = ad_smem(G.pc.M);
+    const size_t smem = ad_smem(G.pc_base.M);
     const int nF = G.S.nfilter;
 #define GRAD(SP, ND)                                                                              \
-    if (spt_for(G.pc.nt) == SP && G.pc.nd == ND) {                                                \
+    if (spt_for(G.pc_base.nt) == SP && G.pc_base.nd == ND) {                                                \
         if (smem > 65536)                                                                         \
             (void)hipFuncSetAttribute((const void *)&k_ad_grad<SP, ND>,                           \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);     \
         hipLaunchKernelGGL((k_ad_grad<SP, ND>), dim3(npts), dim3(kStrainThreads), smem, st,       \
-                           G.pc, npts, G.R, bw, G.twq.p, G.ad_Fs.p, G.recs.p, G.kerns.p,         \
+                           G.pc_base, npts, G.R, bw, G.twq.p, G.ad_Fs.p, G.recs_base.p, G.kerns_base.p,         \
                            G.ad_Hg.p, G.ad_W.p, G.ad_hfilt.p, nF, G.nk, gbase, G.ad_gF.p,        \
                            G.ad_U.p, gbw, gfw);                                                   \
         return hipGetLastError();                                                                 \

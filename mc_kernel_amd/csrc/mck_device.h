@@ -12,6 +12,11 @@ constexpr int kNN = 12;          // nearest midpoints tested (readfields.f90:106
 constexpr int kStrainThreads = 256;
 constexpr int kEvalThreads = 256;
 constexpr int kMaxSlots = 256;   // distinct (window-filter, base kernel) pairs per receiver
+// base kernels of the frequency-domain contraction: lambda, mu, rho (unused), a, b, c
+// (calc_basekernel, kernel.f90:514-570) and the model-independent combinations that the
+// physical kernels use (kernel.f90:665-832): 6 = mu - 2 lambda (vs), 7 = lambda - a - c
+// (vpv), 8 = mu - 2 lambda - b (vsh, whose other term is c)
+constexpr int kNumBk = 9;
 static_assert(kMaxSlots <= kStrainThreads, "one thread per slot in the final slot reduction");
 
 // LDS entries of the quarter twiddle table of an M-point complex FFT: M/2 values, one
@@ -80,8 +85,8 @@ struct PathConf {
     // frequency-domain contraction
     int nd_f, nd_b;
     int comps_f[6], comps_b[6];
-    int need_bk;              // bit b set: base kernel b is used by some kernel
-    int bk_lds[6];            // LDS array of base kernel b in the contraction (-1 = unused)
+    int need_bk;              // bit b set: base kernel b (< 6) must be formed per bin
+    int bk_lds[kNumBk];       // LDS array of (combined) base kernel b (-1 = not stored)
     int nbk_lds;
     int max_slots;            // largest slot count of a receiver
     // dampen_field (readfields.f90:3184-3204): radius (<= 0 disables), source position
@@ -119,7 +124,7 @@ struct RecDesc {
     int nslot;
     int first_kernel, nkernel;
     int slot_h[kMaxSlots];
-    int slot_bk[kMaxSlots];    // 0 lambda, 1 mu, 3 a, 4 b, 5 c
+    int slot_bk[kMaxSlots];    // base kernel (or combination) of the slot, < kNumBk
     int slot_lds[kMaxSlots];   // LDS array of the slot's base kernel in the contraction
     // slots ordered by window spectrum, in groups sharing one spectrum (<= kSlotGroup)
     int ngroup;
@@ -129,7 +134,7 @@ struct RecDesc {
 // kernel: model parameter and the slot of every base kernel it needs (-1 = none)
 struct KernDesc {
     int param;                 // MCK_PARAM_*
-    int slot[6];               // slot of each base kernel the parameter needs (-1 = none)
+    int slot[kNumBk];          // slot of each (combined) base kernel it reads (-1 = none)
 };
 
 // Monte Carlo state of the voxels of one task: one integrator per basis function

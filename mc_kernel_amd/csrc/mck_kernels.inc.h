@@ -1170,6 +1170,13 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && NDB <= 4) ? 3 : (SPT <
             if (pc.bk_lds[3] >= 0) bkL[pc.bk_lds[3] * stride + k] = bk3;
             if (pc.bk_lds[4] >= 0) bkL[pc.bk_lds[4] * stride + k] = bk4;
             if (pc.bk_lds[5] >= 0) bkL[pc.bk_lds[5] * stride + k] = bk5;
+            // combinations read by single physical kernels
+            if (pc.bk_lds[6] >= 0)
+                bkL[pc.bk_lds[6] * stride + k] = make_float2(bk1.x - 2.0f * bk0.x, bk1.y - 2.0f * bk0.y);
+            if (pc.bk_lds[7] >= 0)
+                bkL[pc.bk_lds[7] * stride + k] = make_float2(bk0.x - bk3.x - bk5.x, bk0.y - bk3.y - bk5.y);
+            if (pc.bk_lds[8] >= 0)
+                bkL[pc.bk_lds[8] * stride + k] = make_float2(bk1.x - 2.0f * bk0.x - bk4.x, bk1.y - 2.0f * bk0.y - bk4.y);
         }
     }
     __syncthreads();
@@ -1251,24 +1258,26 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && NDB <= 4) ? 3 : (SPT <
     for (int kk = tid; kk < rd.nkernel; kk += kStrainThreads) {
         const int k = rd.first_kernel + kk;
         const KernDesc &kd = kern[k];
-        double b[6];
-        for (int qq = 0; qq < 6; ++qq) b[qq] = kd.slot[qq] >= 0 ? slotval[kd.slot[qq]] * damp : 0.0;
+        double b[kNumBk];
+        for (int qq = 0; qq < kNumBk; ++qq) b[qq] = kd.slot[qq] >= 0 ? slotval[kd.slot[qq]] * damp : 0.0;
         if (base_out) {
             for (int qq = 0; qq < 6; ++qq) base_out[((size_t)p * nk + k) * 6 + qq] = b[qq];
             continue;
         }
+        // (the receiver's slots use the combinations: b[6] = mu - 2 lambda, b[7] = lambda
+        // - a - c, b[8] = mu - 2 lambda - b)
         double v = 0.0;
         float vsh = vs, vph = vp, vsv = vsh / sqrtf(xi_m), vpv = vph * sqrtf(phi_m);
         switch (kd.param) {
         case 0: v = relative ? b[0] * (double)(rho_f * (vp * vp - 2 * (vs * vs))) : b[0]; break;
         case 3: v = relative ? b[1] * (double)(rho_f * (vs * vs)) : b[1]; break;
         case 1: v = relative ? 2.0 * rho * (double)(vp * vp) * b[0] : 2.0 * rho * (double)vp * b[0]; break;
-        case 2: v = (relative ? 2.0 * rho * (double)(vs * vs) : 2.0 * rho * (double)vs) * (b[1] - 2 * b[0]); break;
+        case 2: v = (relative ? 2.0 * rho * (double)(vs * vs) : 2.0 * rho * (double)vs) * b[6]; break;
         case 4: v = (relative ? 2.0 * rho * (double)(vsh * vsh) : 2.0 * rho * (double)vsh)
-                    * (b[1] - 2 * b[0] - b[4] + 2 * b[5] * (double)(1 - eta_m)); break;
+                    * (b[8] + 2 * b[5] * (double)(1 - eta_m)); break;
         case 5: v = (relative ? 2.0 * rho * (double)(vsv * vsv) : 2.0 * rho * (double)vsv) * b[4]; break;
         case 6: v = (relative ? 2.0 * rho * (double)(vph * vph) : 2.0 * rho * (double)vph) * (b[3] + b[5] * (double)eta_m); break;
-        case 7: v = (relative ? 2.0 * rho * (double)(vpv * vpv) : 2.0 * rho * (double)vpv) * (b[0] - b[3] - b[5]); break;
+        case 7: v = (relative ? 2.0 * rho * (double)(vpv * vpv) : 2.0 * rho * (double)vpv) * b[7]; break;
         case 8: v = relative ? 2.0 * rho * (double)(vph * vph - vsh * vsh) * b[5] * (double)eta_m
                              : 2.0 * rho * (double)(vph * vph - vsh * vsh) * b[5]; break;
         default: v = 0.0;
