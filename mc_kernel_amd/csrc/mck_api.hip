@@ -10,6 +10,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <initializer_list>
 #include <map>
@@ -1024,6 +1025,12 @@ int init_impl(const mck_setup_t *s) {
                      + (size_t)G.R * 16 + 64;
     size_t budget = std::min<size_t>(free_b / 3, (size_t)64 << 30);
     G.chunk_points = (int)std::max<size_t>(1024, std::min<size_t>(budget / per_point, 1 << 21));
+    // tuning knob: points per evaluation launch (smaller launches keep the forward
+    // spectra of a launch in the last-level cache)
+    if (const char *e = std::getenv("MCK_CHUNK_POINTS")) {
+        const long v = std::atol(e);
+        if (v > 0) G.chunk_points = (int)std::min<long>(v, G.chunk_points);
+    }
     G.init = true;
     return MCK_OK;
 }
