@@ -359,7 +359,6 @@ struct State {
     DBuf<double> phys;
     DBuf<unsigned long long> evals;
     DBuf<int> err;
-    DBuf<unsigned int> elem_mark;
     DBuf<unsigned long long> elem_count;
     int chunk_points = 0;
     hipStream_t stream = nullptr;
@@ -841,6 +840,7 @@ int sort_by_element(State &G, int n, int nelem) {
     if (n <= 1) {
         if (n == 1) {
             HIPCHECK(hipMemcpyAsync(G.vals_out.p, G.vals_in.p, 4, hipMemcpyDeviceToDevice, G.stream));
+            HIPCHECK(hipMemcpyAsync(G.keys_out.p, G.keys_in.p, 4, hipMemcpyDeviceToDevice, G.stream));
         }
         return MCK_OK;
     }
@@ -1258,19 +1258,6 @@ int prepare_task(State &G, Task &T, int elem_type, int nvpe, int nelems, int nba
     return MCK_OK;
 }
 
-// algorithmic-byte accounting of one launch: distinct elements (by_point 0) or points
-// (by_point 1) among the located pairs; counters at elem_count[slot], [slot + 1]
-int mark_keys(State &G, int npairs, const Located *loc, int R, int P, const uint8_t *active, int by_point,
-              size_t nkeys, int slot) {
-    const size_t words = (nkeys + 31) / 32 + 1;
-    HIPCHECK(G.elem_mark.ensure(words));
-    HIPCHECK(hipMemsetAsync(G.elem_mark.p, 0, words * 4, G.stream));
-    hipLaunchKernelGGL(k_mark, dim3((npairs + 255) / 256), dim3(256), 0, G.stream, npairs, loc, R, P, active,
-                       by_point, G.elem_mark.p, G.elem_count.p + slot);
-    HIPCHECK(hipGetLastError());
-    return MCK_OK;
-}
-
 VoxState vox_state(State &G) {
     VoxState v;
     v.nbasis = G.nbasis;
@@ -1303,14 +1290,17 @@ int eval_chunk(State &G, int P, int w0, int nw, std::vector<std::pair<hipEvent_t
     hipLaunchKernelGGL(k_locate, dim3((unsigned)(((long long)np * R + 127) / 128)), dim3(128), 0, st,
                        G.bwd.d, G.frames.p, 1, R, np, G.xyz.p, G.S.planet_radius, act, P, R, loc_b, G.err.p);
     HIPCHECK(hipGetLastError());
-    if ((rc = mark_keys(G, np, loc_f, 0, P, nullptr, 0, G.fwd.d.nelem, 0))) return rc;
-    if ((rc = mark_keys(G, np * R, loc_b, R, P, act, 0, G.bwd.d.nelem, 2))) return rc;
-    if ((rc = mark_keys(G, np * R, loc_b, R, P, act, 1, np, 4))) return rc;
+    hipLaunchKernelGGL(k_count_points, dim3((nw + 255) / 256), dim3(256), 0, st, nw, R, P, act,
+                       G.elem_count.p + 4);
+    HIPCHECK(hipGetLastError());
     // forward: points in element order
     hipLaunchKernelGGL(k_keys_points, dim3((np + 255) / 256), dim3(256), 0, st, np, loc_f, G.fwd.d.nelem,
                        G.keys_in.p, G.vals_in.p);
     HIPCHECK(hipGetLastError());
     if ((rc = sort_by_element(G, np, G.fwd.d.nelem))) return rc;
+    hipLaunchKernelGGL(k_count_runs, dim3((np + 255) / 256), dim3(256), 0, st, np, G.keys_out.p,
+                       G.fwd.d.nelem, G.elem_count.p + 0);
+    HIPCHECK(hipGetLastError());
     hipEvent_t a, b;
     HIPCHECK(hipEventCreate(&a));
     HIPCHECK(hipEventCreate(&b));
@@ -1327,6 +1317,11 @@ int eval_chunk(State &G, int P, int w0, int nw, std::vector<std::pair<hipEvent_t
     HIPCHECK(hipMemcpyAsync(&npairs, G.pair_count.p, 4, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     if ((rc = sort_by_element(G, npairs, G.bwd.d.nelem))) return rc;
+    if (npairs > 0) {
+        hipLaunchKernelGGL(k_count_runs, dim3((npairs + 255) / 256), dim3(256), 0, st, npairs, G.keys_out.p,
+                           G.bwd.d.nelem, G.elem_count.p + 2);
+        HIPCHECK(hipGetLastError());
+    }
     HIPCHECK(hipEventCreate(&a));
     HIPCHECK(hipEventCreate(&b));
     HIPCHECK(hipEventRecord(a, st));
@@ -1807,7 +1802,7 @@ void mck_finalize(void) {
     G.nmodels.release(); G.xyz.release(); G.model_idx.release(); G.loc.release();
     G.Fspec.release(); G.phys.release(); G.evals.release(); G.err.release();
     G.abinv.release(); G.bweights.release();
-    G.elem_mark.release(); G.elem_count.release(); G.keys_in.release(); G.keys_out.release();
+    G.elem_count.release(); G.keys_in.release(); G.keys_out.release();
     G.vals_in.release(); G.vals_out.release(); G.pair_count.release(); G.sort_tmp.release();
     G.ad_W.release(); G.ad_Hg.release(); G.ad_Fs.release(); G.ad_gF.release(); G.ad_U.release();
     G.ad_part.release(); G.ad_hfilt.release();

@@ -1311,6 +1311,32 @@ __global__ void k_keys_pairs(int npairs, int R, int P, const uint8_t *active, co
     vals[pos] = q;
 }
 
+// Algorithmic-byte accounting from the element-sorted keys of a launch: counters[0] +=
+// distinct keys below nkeys (elements read), counters[1] += keys below nkeys (items)
+__global__ void k_count_runs(int n, const int *keys, int nkeys, unsigned long long *counters)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = i < n ? keys[i] : nkeys;
+    const bool valid = k < nkeys;
+    const bool start = valid && (i == 0 || keys[i - 1] != k);
+    const unsigned long long nv = __popcll(__ballot(valid)), ns = __popcll(__ballot(start));
+    if ((threadIdx.x & 63) == 0) {
+        if (ns) atomicAdd(counters, ns);
+        if (nv) atomicAdd(counters + 1, nv);
+    }
+}
+
+// points of the launch with at least one evaluated receiver: counter += P per voxel
+__global__ void k_count_points(int nwork, int R, int P, const uint8_t *active, unsigned long long *counter)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    bool any = false;
+    if (w < nwork)
+        for (int r = 0; r < R; ++r) any |= active[(size_t)w * R + r] != 0;
+    const unsigned long long n = __popcll(__ballot(any));
+    if ((threadIdx.x & 63) == 0 && n) atomicAdd(counter, n * (unsigned long long)P);
+}
+
 // ------------------------------------------------------------------------------------
 // Monte Carlo bookkeeping
 // ------------------------------------------------------------------------------------
@@ -1452,27 +1478,5 @@ __global__ void k_mc(int nwork, int P, int nk, int R, const int *work_voxel, con
     }
 }
 
-// Algorithmic-byte accounting of one launch: distinct keys among the located (active)
-// pairs, key = element (by_point 0) or point (by_point 1).  counters: [0] distinct,
-// [1] pairs.
-__global__ void k_mark(int npairs, const Located *loc, int R, int P, const uint8_t *active,
-                       int by_point, unsigned *bitmap, unsigned long long *counters)
-{
-    int pair = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pair >= npairs) return;
-    int p = pair;
-    if (R > 0) {
-        p = pair / R;
-        int r = pair - p * R;
-        if (active && !active[(p / P) * R + r]) return;
-    }
-    int e = loc[pair].elem;
-    if (e < 0) return;
-    size_t key = by_point ? (size_t)p : (size_t)e;
-    unsigned bit = 1u << (key & 31);
-    unsigned old = atomicOr(&bitmap[key >> 5], bit);
-    if (!(old & bit)) atomicAdd(&counters[0], 1ull);
-    atomicAdd(&counters[1], 1ull);
-}
 
 }  // namespace mck
