@@ -744,8 +744,8 @@ int build_kernels(State &G) {
 
 
 // LDS of the forward kernel: two FFT buffers and the quarter twiddle table
-size_t strain_smem(int M) {
-    return ((sizeof(OpShared) + 15) / 16) * 16 + (2 * (size_t)M + tq_len(M)) * sizeof(float2);
+size_t strain_smem(const PathConf &pc) {
+    return (size_t)pc.op_bytes_f + (2 * (size_t)pc.M + tq_len(pc.M)) * sizeof(float2);
 }
 
 // samples per thread of the strain kernels for a trace length
@@ -761,7 +761,7 @@ unsigned grid_x8(int n) { return 8u * (unsigned)((n + 7) / 8); }
 // forward spectra of npts points (order: element-sorted point ids, or null = identity)
 hipError_t launch_fwd(State &G, int npts, const int *order, const Located *loc, float2 *F) {
     if (npts <= 0) return hipSuccess;
-    const size_t smem = strain_smem(G.pc.M);
+    const size_t smem = strain_smem(G.pc);
     const dim3 grid(grid_x8(npts)), block(kStrainThreads);
     const int nd = G.pc.nd_f;
 #define FWD(SP, ND)                                                                               \
@@ -782,8 +782,9 @@ hipError_t launch_fwd(State &G, int npts, const int *order, const Located *loc, 
 
 // LDS of the backward kernel: FFT buffers, reused for the base-kernel spectra, the
 // quarter twiddle table and the per-slot sums (4 waves + total) of the largest receiver
-size_t bwd_smem(int M, int nbk, int max_slots) {
-    return ((sizeof(OpShared) + 15) / 16) * 16
+size_t bwd_smem(const PathConf &pc) {
+    const int M = pc.M, nbk = pc.nbk_lds, max_slots = pc.max_slots;
+    return (size_t)pc.op_bytes_b
          + (std::max<size_t>(2 * (size_t)M, (size_t)nbk * (M + 1)) + tq_len(M)) * sizeof(float2)
          + (size_t)5 * max_slots * sizeof(double);
 }
@@ -797,7 +798,7 @@ hipError_t launch_bwd(State &G, int npairs, const int *order, const Located *loc
     const PathConf &pc = base_out ? G.pc_base : G.pc;
     const RecDesc *recs = base_out ? G.recs_base.p : G.recs.p;
     const KernDesc *kerns = base_out ? G.kerns_base.p : G.kerns.p;
-    const size_t smem = bwd_smem(pc.M, pc.nbk_lds, pc.max_slots);
+    const size_t smem = bwd_smem(pc);
     const dim3 grid(grid_x8(npairs)), block(kStrainThreads);
     const int nf = pc.nd_f, nb = pc.nd_b;
 #define BWD(SP, NF, NB)                                                                           \
@@ -994,6 +995,8 @@ int init_impl(const mck_setup_t *s) {
     };
     carry(fwd_mono, &pc.nd_f, pc.comps_f);
     carry(bwd_mono, &pc.nd_b, pc.comps_b);
+    pc.op_bytes_f = (int)op_bytes(pc.nd_f * pc.nvar_fwd * 25);
+    pc.op_bytes_b = (int)op_bytes(pc.nd_b * pc.nvar_bwd * 25);
     if (pc.nd_f == 1 && pc.nd_b != 1) return fail(MCK_EINVAL, "inconsistent strain types");
     pc.ntaper = 10;
     for (int j = 0; j < 10; ++j) {

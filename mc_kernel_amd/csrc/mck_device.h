@@ -1,6 +1,7 @@
 // mck_device.h - device-side data layout shared by the HIP kernels and the host driver.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 namespace mck {
@@ -89,6 +90,7 @@ struct PathConf {
     int bk_lds[kNumBk];       // LDS array of (combined) base kernel b (-1 = not stored)
     int nbk_lds;
     int max_slots;            // largest slot count of a receiver
+    int op_bytes_f, op_bytes_b;   // LDS of the per-point operator set-up, fwd / bwd
     // dampen_field (readfields.f90:3184-3204): radius (<= 0 disables), source position
     double damp_radius;
     double src_r[3];
@@ -100,12 +102,18 @@ struct OpShared {
     double sinv[25];        // 1 / s at the nodes
     double li[5], lj[5];
     double M[6][kMaxSim][6];// output comp x simulation x strain comp
-    float W[6 * kMaxVar * 25];
     float Q[6][6];          // earth Voigt component x carried component (source mode)
     int elem;
     int axial;
     int type;
+    float W[6 * kMaxVar * 25];  // last member: a kernel's LDS holds only nd x nvar x 25
 };
+
+// LDS bytes of an OpShared whose weight table holds nw entries (16-byte aligned)
+__host__ __device__ constexpr size_t op_bytes(int nw)
+{
+    return ((offsetof(OpShared, W) + 4 * (size_t)nw + 15) / 16) * 16;
+}
 
 // Per-pair coefficients of the frequency-domain contraction (backward kernel):
 // earth-frame strain = Q x carried components, for the fwd (Qf) and bwd (Qb) fields.

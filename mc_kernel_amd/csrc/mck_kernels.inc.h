@@ -955,7 +955,7 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && ND <= 4) ? 3 : (SPT <=
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     OpShared &S = *reinterpret_cast<OpShared *>(smem_raw);
-    float2 *bufA = reinterpret_cast<float2 *>(smem_raw + ((sizeof(OpShared) + 15) / 16) * 16);
+    float2 *bufA = reinterpret_cast<float2 *>(smem_raw + pc.op_bytes_f);
     float2 *bufB = bufA + pc.M;
     float2 *Tq = bufA + 2 * pc.M;
     __shared__ float wave_tot[kStrainThreads / 64];
@@ -1010,7 +1010,7 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && NDB <= 4) ? 3 : (SPT <
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     OpShared &S = *reinterpret_cast<OpShared *>(smem_raw);
-    float2 *bufA = reinterpret_cast<float2 *>(smem_raw + ((sizeof(OpShared) + 15) / 16) * 16);
+    float2 *bufA = reinterpret_cast<float2 *>(smem_raw + pc.op_bytes_b);
     float2 *bufB = bufA + pc.M;
     float2 *Tq = bufA + max(2 * pc.M, pc.nbk_lds * (pc.M + 1));
     double *slot_red = reinterpret_cast<double *>(Tq + tq_len(pc.M));   // [4][max_slots]
