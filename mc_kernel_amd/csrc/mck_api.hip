@@ -354,6 +354,9 @@ struct State {
     DBuf<int> model_idx;
     DBuf<Located> loc;
     DBuf<float2> Fspec;                 // [point][nd_f][nomega] forward spectra
+    // operator set-up (k_op_setup): gather weights and rotations of the points (by point
+    // id) and of the pairs (by position in the pair order)
+    DBuf<float> opWf, opQf, opWb, opQb;
     DBuf<int> keys_in, keys_out, vals_in, vals_out, pair_count;
     DBuf<unsigned char> sort_tmp;
     DBuf<double> phys;
@@ -758,7 +761,19 @@ int spt_for(int nt) {
 
 unsigned grid_x8(int n) { return 8u * (unsigned)((n + 7) / 8); }
 
-// forward spectra of npts points (order: element-sorted point ids, or null = identity)
+// operator set-up of n points (fwd) or pairs (bwd) in launch order (null = identity)
+hipError_t launch_setup(State &G, bool fwd, int n, const int *order, const Located *loc) {
+    if (n <= 0) return hipSuccess;
+    const DbDev &D = fwd ? G.fwd : G.bwd;
+    const size_t smem = (size_t)kSetupWaves * op_bytes(0);
+    hipLaunchKernelGGL(k_op_setup, dim3((unsigned)((n + kSetupWaves - 1) / kSetupWaves)), dim3(64 * kSetupWaves),
+                       smem, G.stream, D.d, G.pc, G.frames.p, n, order, G.R, loc, fwd ? 1 : 0,
+                       fwd ? G.opWf.p : G.opWb.p, fwd ? G.opQf.p : G.opQb.p);
+    return hipGetLastError();
+}
+
+// forward spectra of npts points (order: element-sorted point ids, or null = identity);
+// the operator set-up of the points (launch_setup) precedes it
 hipError_t launch_fwd(State &G, int npts, const int *order, const Located *loc, float2 *F) {
     if (npts <= 0) return hipSuccess;
     const size_t smem = strain_smem(G.pc);
@@ -770,7 +785,7 @@ hipError_t launch_fwd(State &G, int npts, const int *order, const Located *loc, 
             (void)hipFuncSetAttribute((const void *)&k_fwd_spectra<SP, ND>,                       \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);     \
         hipLaunchKernelGGL((k_fwd_spectra<SP, ND>), grid, block, smem, G.stream, G.fwd.d, G.pc,   \
-                           G.frames.p, npts, order, loc, G.twq.p, F);                            \
+                           npts, order, loc, G.opWf.p, G.twq.p, F);                              \
         return hipGetLastError();                                                                 \
     }
 #define FWD_ND(SP) FWD(SP, 1) FWD(SP, 4) FWD(SP, 6)
@@ -789,9 +804,10 @@ size_t bwd_smem(const PathConf &pc) {
          + (size_t)5 * max_slots * sizeof(double);
 }
 
-// backward strain + evaluation of npairs (point, receiver) pairs q = p * R + r
-hipError_t launch_bwd(State &G, int npairs, const int *order, const Located *loc_f,
-                      const Located *loc_b, double *phys, double *base_out, const uint8_t *active, int P) {
+// backward strain + evaluation of npairs (point, receiver) pairs q = p * R + r; the
+// operator set-up of the pairs (launch_setup, same order) precedes it
+hipError_t launch_bwd(State &G, int npairs, const int *order, const Located *loc_b, double *phys,
+                      double *base_out, const uint8_t *active, int P) {
     if (npairs <= 0) return hipSuccess;
     // base kernels come from the base-kernel slot layout, physical kernels from the
     // combined one
@@ -807,7 +823,7 @@ hipError_t launch_bwd(State &G, int npairs, const int *order, const Located *loc
             (void)hipFuncSetAttribute((const void *)&k_bwd_eval<SP, NF, NB>,                      \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);     \
         hipLaunchKernelGGL((k_bwd_eval<SP, NF, NB>), grid, block, smem, G.stream, G.bwd.d, pc,    \
-                           G.frames.p, npairs, order, G.R, loc_f, loc_b, G.twq.p,                \
+                           npairs, order, G.R, loc_b, G.opWb.p, G.opQb.p, G.opQf.p, G.twq.p,     \
                            G.Fspec.p, recs, kerns, G.H.p, G.model_idx.p, G.model.p,              \
                            G.S.model_ndepth, G.S.relative_kernel, G.nk, phys, base_out,         \
                            G.xyz.p, active, P, G.d_rec_r.p);                                     \
@@ -927,6 +943,10 @@ int ensure_chunk(State &G, int points) {
     HIPCHECK(G.model_idx.ensure(points));
     HIPCHECK(G.loc.ensure((size_t)points * (1 + R)));
     HIPCHECK(G.Fspec.ensure((size_t)points * G.pc.nd_f * nw));
+    HIPCHECK(G.opWf.ensure((size_t)points * G.pc.nd_f * G.pc.nvar_fwd * 25));
+    HIPCHECK(G.opQf.ensure((size_t)points * 36));
+    HIPCHECK(G.opWb.ensure(npairs * G.pc.nd_b * G.pc.nvar_bwd * 25));
+    HIPCHECK(G.opQb.ensure(npairs * 36));
     HIPCHECK(G.keys_in.ensure(npairs));
     HIPCHECK(G.keys_out.ensure(npairs));
     HIPCHECK(G.vals_in.ensure(npairs));
@@ -1025,7 +1045,9 @@ int init_impl(const mck_setup_t *s) {
     size_t free_b = 0, total_b = 0;
     HIPCHECK(hipMemGetInfo(&free_b, &total_b));
     size_t per_point = (size_t)pc.nd_f * pc.nomega * sizeof(float2) + (size_t)(1 + G.R) * sizeof(Located)
-                     + (size_t)G.R * 16 + 64;
+                     + (size_t)G.R * 16 + 64
+                     + ((size_t)pc.nd_f * pc.nvar_fwd * 25 + 36) * sizeof(float)
+                     + (size_t)G.R * ((size_t)pc.nd_b * pc.nvar_bwd * 25 + 36) * sizeof(float);
     size_t budget = std::min<size_t>(free_b / 3, (size_t)64 << 30);
     G.chunk_points = (int)std::max<size_t>(1024, std::min<size_t>(budget / per_point, 1 << 21));
     // tuning knob: points per evaluation launch (smaller launches keep the forward
@@ -1304,6 +1326,7 @@ int eval_chunk(State &G, int P, int w0, int nw, std::vector<std::pair<hipEvent_t
     hipLaunchKernelGGL(k_count_runs, dim3((np + 255) / 256), dim3(256), 0, st, np, G.keys_out.p,
                        G.fwd.d.nelem, G.elem_count.p + 0);
     HIPCHECK(hipGetLastError());
+    HIPCHECK(launch_setup(G, true, np, G.vals_out.p, loc_f));
     hipEvent_t a, b;
     HIPCHECK(hipEventCreate(&a));
     HIPCHECK(hipEventCreate(&b));
@@ -1325,10 +1348,11 @@ int eval_chunk(State &G, int P, int w0, int nw, std::vector<std::pair<hipEvent_t
                            G.bwd.d.nelem, G.elem_count.p + 2);
         HIPCHECK(hipGetLastError());
     }
+    HIPCHECK(launch_setup(G, false, npairs, G.vals_out.p, loc_b));
     HIPCHECK(hipEventCreate(&a));
     HIPCHECK(hipEventCreate(&b));
     HIPCHECK(hipEventRecord(a, st));
-    HIPCHECK(launch_bwd(G, npairs, G.vals_out.p, loc_f, loc_b, G.phys.p + (size_t)w0 * P * G.nk, nullptr, act, P));
+    HIPCHECK(launch_bwd(G, npairs, G.vals_out.p, loc_b, G.phys.p + (size_t)w0 * P * G.nk, nullptr, act, P));
     HIPCHECK(hipEventRecord(b, st));
     evs.push_back({a, b});
     G.stats.strain_launches += 2;
@@ -1699,8 +1723,10 @@ int mck_base_kernels(int32_t npts, const double *xyz, double *base) {
     HIPCHECK(hipMalloc((void **)&d_base, (size_t)npts * G.nk * 6 * 8));
     HIPCHECK(hipMalloc((void **)&d_phys, (size_t)npts * G.nk * 8));
     hipError_t e = hipMemsetAsync(d_base, 0, (size_t)npts * G.nk * 6 * 8, G.stream);
+    if (e == hipSuccess) e = launch_setup(G, true, npts, nullptr, loc_f);
     if (e == hipSuccess) e = launch_fwd(G, npts, nullptr, loc_f, G.Fspec.p);
-    if (e == hipSuccess) e = launch_bwd(G, npts * R, nullptr, loc_f, loc_b, d_phys, d_base, nullptr, 1);
+    if (e == hipSuccess) e = launch_setup(G, false, npts * R, nullptr, loc_b);
+    if (e == hipSuccess) e = launch_bwd(G, npts * R, nullptr, loc_b, d_phys, d_base, nullptr, 1);
     if (e == hipSuccess)
         e = hipMemcpyAsync(base, d_base, (size_t)npts * G.nk * 6 * 8, hipMemcpyDeviceToHost, G.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(G.stream);

@@ -443,24 +443,28 @@ __global__ void k_elem_tables(DevDb db, double *out)
 //  source mode: c are the source-frame components comps[c] of the frame, with the
 //  azimuthal factors, the sign flip of components 4 and 6 and 1/amplitude applied;
 //  S.Q[i][c] is then the rotation of component c into earth-frame component i.
-__device__ void build_weights(OpShared &S, const DevDb &db, const PathConf &pc, const DevFrame &fr,
-                              const Located &L, bool fwd, int nvar, const int *vars, int nd,
-                              const int *comps, bool src_mode)
+// The set-up runs in two parts separated by a barrier: op_setup_parts (element tables,
+// Lagrange bases, output transform) and op_weights.  nthr threads cooperate (tid < nthr);
+// role_li, role_lj and role_xf name the threads that form the two Lagrange bases and the
+// output transform.
+__device__ void op_setup_parts(OpShared &S, const DevDb &db, const PathConf &pc, const DevFrame &fr,
+                               const Located &L, bool fwd, int nd, const int *comps, bool src_mode,
+                               int tid, int nthr, int role_li, int role_lj, int role_xf)
 {
-    const int tid = threadIdx.x;
     const int e = L.elem;
     if (tid == 0) { S.elem = e; S.axial = db.axis[e]; S.type = db.eltype[e]; }
-    if (tid < 125) {
+    for (int i = tid; i < 125; i += nthr) {
         // inverse Jacobian and 1/s at the 25 nodes (element table, see k_elem_tables)
-        const double v = db.ejac[(size_t)e * 125 + tid];
-        const int n = tid / 5, c = tid - (tid / 5) * 5;
+        const double v = db.ejac[(size_t)e * 125 + i];
+        const int n = i / 5, c = i - (i / 5) * 5;
         if (c < 4) S.invj[n][c] = v; else S.sinv[n] = v;
-    } else if (tid == 160 || tid == 161) {
+    }
+    if (tid == role_li || tid == role_lj) {
         // Lagrange basis at (xi, eta) (spectral_basis.f90:203-217)
         const double *xp = db.axis[e] ? c_basis.glj : c_basis.gll;
-        const double *pts = tid == 160 ? xp : c_basis.gll;
-        double x = tid == 160 ? L.xi : L.eta;
-        double *l = tid == 160 ? S.li : S.lj;
+        const double *pts = tid == role_li ? xp : c_basis.gll;
+        double x = tid == role_li ? L.xi : L.eta;
+        double *l = tid == role_li ? S.li : S.lj;
         for (int i = 0; i < 5; ++i) {
             double v = 1.0;
             for (int m = 0; m < 5; ++m) {
@@ -469,7 +473,8 @@ __device__ void build_weights(OpShared &S, const DevDb &db, const PathConf &pc, 
             }
             l[i] = v;
         }
-    } else if (tid == 192) {
+    }
+    if (tid == role_xf) {
         // output transform: sign flip of comps 4, 6 (readfields.f90:2180-2184), azimuthal
         // factors, 1/amplitude, src -> xyz -> earth rotations (full strain only)
         double amp = db.amplitude;
@@ -516,11 +521,16 @@ __device__ void build_weights(OpShared &S, const DevDb &db, const PathConf &pc, 
             }
         }
     }
-    __syncthreads();
+}
+
+// W[(c * nvar + v) * 25 + node] from the tables of op_setup_parts (W: LDS or global)
+__device__ void op_weights(const OpShared &S, const PathConf &pc, bool fwd, int nvar, const int *vars,
+                           int nd, int tid, int nthr, float *W)
+{
     const bool axial = S.axial != 0;
     const int total = nd * nvar * 25;
     const double *Gx = axial ? c_basis.G1 : c_basis.G2;
-    for (int idx = tid; idx < total; idx += blockDim.x) {
+    for (int idx = tid; idx < total; idx += nthr) {
         int n = idx % 25, v = (idx / 25) % nvar, c = idx / (25 * nvar);
         int k = n % 5, l = n / 5;
         int var = vars[v];
@@ -571,9 +581,60 @@ __device__ void build_weights(OpShared &S, const DevDb &db, const PathConf &pc, 
             w += peta * c_basis.G2[l + 5 * j];
         }
         if (!(axial && k == 0)) w += af * S.li[k] * S.lj[l] * S.sinv[k + 5 * l];
-        S.W[(c * nvar + v) * 25 + n] = (float)w;
+        W[(c * nvar + v) * 25 + n] = (float)w;
     }
+}
+
+__device__ void build_weights(OpShared &S, const DevDb &db, const PathConf &pc, const DevFrame &fr,
+                              const Located &L, bool fwd, int nvar, const int *vars, int nd,
+                              const int *comps, bool src_mode)
+{
+    op_setup_parts(S, db, pc, fr, L, fwd, nd, comps, src_mode, threadIdx.x, blockDim.x, 160, 161, 192);
     __syncthreads();
+    op_weights(S, pc, fwd, nvar, vars, nd, threadIdx.x, blockDim.x, S.W);
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------------------------
+// k_op_setup: the operator set-up of the spectra kernels, one wave per item, so that its
+// serial FP64 chains (sine and cosine of the azimuth, frame rotations, Lagrange bases)
+// run at full occupancy instead of inside the FFT workgroups.
+//  fwd: items are points (frame 0), outputs indexed by point id;
+//  bwd: items are (point, receiver) pairs q = p * R + r (frame 1 + r), outputs indexed
+//       by the pair's position in the launch order.
+// W: [item][nd][nvar][25] gather weights (as build_weights in source mode); Q: [item][6][6]
+// rotation of the carried components into the earth frame (columns >= nd and the trace
+// mode: zero).
+// ------------------------------------------------------------------------------------
+constexpr int kSetupWaves = 4;
+
+__global__ __launch_bounds__(64 * kSetupWaves) void k_op_setup(DevDb db, PathConf pc, const DevFrame *frames,
+                                                               int n, const int *order, int R, const Located *loc,
+                                                               int fwd, float *Wout, float *Qout)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    OpShared &S = *reinterpret_cast<OpShared *>(smem_raw + wave * op_bytes(0));
+    const int lb = blockIdx.x * kSetupWaves + wave;
+    const int q = lb < n ? (order ? order[lb] : lb) : 0;
+    Located L;
+    L.elem = -1;
+    if (lb < n) L = loc[q];
+    const bool live = L.elem >= 0;
+    const int nvar = fwd ? pc.nvar_fwd : pc.nvar_bwd;
+    const int *vars = fwd ? pc.fwd_vars : pc.bwd_vars;
+    const int nd = fwd ? pc.nd_f : pc.nd_b;
+    const int *comps = fwd ? pc.comps_f : pc.comps_b;
+    const int frame = fwd ? 0 : 1 + (q - (q / R) * R);
+    if (live) op_setup_parts(S, db, pc, frames[frame], L, fwd != 0, nd, comps, true, lane, 64, 61, 62, 63);
+    __syncthreads();
+    if (!live) return;
+    const size_t item = fwd ? (size_t)q : (size_t)lb;
+    op_weights(S, pc, fwd != 0, nvar, vars, nd, lane, 64, Wout + item * (size_t)(nd * nvar * 25));
+    if (lane < 36) {
+        const int i = lane / 6, c = lane - (lane / 6) * 6;
+        Qout[item * 36 + lane] = (pc.full && c < nd) ? S.Q[i][c] : 0.0f;
+    }
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs; block b of a grid padded to a
@@ -950,7 +1011,7 @@ __global__ __launch_bounds__(kStrainThreads) void k_strain_traces(
 // ------------------------------------------------------------------------------------
 template <int SPT, int ND>
 __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && ND <= 4) ? 3 : (SPT <= 8 ? 2 : 1)) void k_fwd_spectra(
-    DevDb db, PathConf pc, const DevFrame *frames, int npts, const int *order, const Located *loc,
+    DevDb db, PathConf pc, int npts, const int *order, const Located *loc, const float *Wf,
     const float2 *twq, float2 *F)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -967,12 +1028,18 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && ND <= 4) ? 3 : (SPT <=
     const Located L = loc[p];
     if (L.elem < 0) return;
     load_twiddles(twq, pc.M, Tq);
-    build_weights(S, db, pc, frames[0], L, true, pc.nvar_fwd, pc.fwd_vars, ND, pc.comps_f, true);
+    {
+        // gather weights of the point (k_op_setup)
+        const int nw = ND * pc.nvar_fwd * 25;
+        const float *src = Wf + (size_t)p * nw;
+        for (int i = threadIdx.x; i < nw; i += kStrainThreads) S.W[i] = src[i];
+    }
+    __syncthreads();
 
     const int tid = threadIdx.x, nt = pc.nt, M = pc.M, t0 = tid * SPT;
     const size_t blk = (size_t)25 * nt;
     float acc[ND][SPT];
-    gather_strain<SPT, ND>(S, db.u + (size_t)S.elem * db.nvar * blk, blk, pc.nvar_fwd, pc.fwd_vars, t0, nt, acc);
+    gather_strain<SPT, ND>(S, db.u + (size_t)L.elem * db.nvar * blk, blk, pc.nvar_fwd, pc.fwd_vars, t0, nt, acc);
     const float fdt = (float)pc.dt, h = (float)(pc.dt * 0.5);
     float tap[SPT];
     taper_factors(t0, nt, pc, tap);
@@ -1001,8 +1068,8 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && ND <= 4) ? 3 : (SPT <=
 // ------------------------------------------------------------------------------------
 template <int SPT, int NDF, int NDB>
 __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && NDB <= 4) ? 3 : (SPT <= 8 ? 2 : 1)) void k_bwd_eval(
-    DevDb db, PathConf pc, const DevFrame *frames, int npairs, const int *order, int R,
-    const Located *loc_f, const Located *loc_b, const float2 *twq,
+    DevDb db, PathConf pc, int npairs, const int *order, int R,
+    const Located *loc_b, const float *Wb, const float *Qb, const float *Qf, const float2 *twq,
     const float2 *F, const RecDesc *recs, const KernDesc *kern, const float2 *H,
     const int *model_idx, const float *model /* [6][ndepth] */, int ndepth, int relative, int nk,
     double *phys, double *base_out, const double *xyz, const uint8_t *active, int P,
@@ -1024,24 +1091,19 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && NDB <= 4) ? 3 : (SPT <
     const Located L = loc_b[q];
     if (L.elem < 0) return;
     load_twiddles(twq, pc.M, Tq);
-    build_weights(S, db, pc, frames[1 + r], L, false, pc.nvar_bwd, pc.bwd_vars, NDB, pc.comps_b, true);
-
     const int tid = threadIdx.x;
+    {
+        // gather weights and rotation of the pair, rotation of the point's forward
+        // components (frame 0) (k_op_setup)
+        const int nw = NDB * pc.nvar_bwd * 25;
+        const float *src = Wb + (size_t)lb * nw;
+        for (int i = tid; i < nw; i += kStrainThreads) S.W[i] = src[i];
+        if (tid < 36) (&S.Q[0][0])[tid] = Qb[(size_t)lb * 36 + tid];
+        else if (NDF > 1 && tid >= 64 && tid < 100) (&E.Qf[0][0])[tid - 64] = Qf[(size_t)p * 36 + tid - 64];
+    }
+    __syncthreads();
     if (NDF > 1) {
-        // rotation of the point's forward components (frame 0), F.B and trace coefficients
-        if (tid == 0) {
-            const Located Lf = loc_f[p];
-            const DevFrame f0 = frames[0];
-            const double sp = sin(Lf.phi), cp = cos(Lf.phi);
-            for (int c = 0; c < NDF; ++c) {
-                double t[6] = {0, 0, 0, 0, 0, 0};
-                t[pc.comps_f[c]] = 1.0;
-                rot_src_to_xyz(t, sp, cp);
-                rot_xyz_to_earth(t, f0.sin_lon, f0.cos_lon, f0.sin_colat, f0.cos_colat);
-                for (int i = 0; i < 6; ++i) E.Qf[i][c] = (float)t[i];
-            }
-        }
-        __syncthreads();
+        // F.B and trace coefficients
         if (tid < NDF * NDB) {
             const int m = tid / NDB, n = tid - (tid / NDB) * NDB;
             float a = 0.0f;
@@ -1062,7 +1124,7 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && NDB <= 4) ? 3 : (SPT <
     float2 spec[NDB][SPT], specM[NDB];
     {
         float acc[NDB][SPT];
-        gather_strain<SPT, NDB>(S, db.u + (size_t)S.elem * db.nvar * blk, blk, pc.nvar_bwd, pc.bwd_vars, t0, nt, acc);
+        gather_strain<SPT, NDB>(S, db.u + (size_t)L.elem * db.nvar * blk, blk, pc.nvar_bwd, pc.bwd_vars, t0, nt, acc);
         const float fdt = (float)pc.dt;
         float tap[SPT];
         taper_factors(t0, nt, pc, tap);
