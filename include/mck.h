@@ -156,6 +156,17 @@ typedef struct {
     int32_t deconv_stf;        /* !NO_DECONVOLVE_STF                                     */
     double  damp_radius;       /* DAMP_RADIUS_SOURCE_RECEIVER (m); <= 0 disables         */
     int32_t mask_src_rec;      /* MASK_SOURCE_RECEIVER                                   */
+
+    /* Databases larger than device memory (no reference counterpart: readfields.f90
+     * reads hyperslabs from NetCDF on demand).  0: each database's snapshots are copied
+     * to device memory once (used in place when `snapshots` already is device memory),
+     * unless the two together exceed 3/4 of the free device memory, in which case they
+     * are paged with pages of 1/64 of it.  > 0: snapshots in host memory are paged: they
+     * stay where they are (the caller keeps them valid and unchanged until mck_finalize)
+     * and every launch copies the element pages its points touch, db_page_elems
+     * elements each, through two device buffers per database.  Results are identical
+     * to the resident mode.                                                             */
+    int32_t db_page_elems;
 } mck_setup_t;
 
 /* Work-unit statistics of the last mck_slave_work call (device timings via hipEvents). */
@@ -172,6 +183,7 @@ typedef struct {
     double  strain_fwd_ms;     /* summed time of the fwd strain-gather/FFT kernel        */
     double  strain_bwd_ms;     /* summed time of the bwd strain-gather/FFT kernel        */
     double  wall_ms;           /* host wall time of the whole call                       */
+    int64_t paged_bytes;       /* snapshot bytes copied host -> device by paging         */
 } mck_stats_t;
 
 /*

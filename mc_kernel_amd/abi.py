@@ -61,6 +61,7 @@ class MckSetup(C.Structure):
         ("int_over_volume", C.c_int32), ("relative_kernel", C.c_int32),
         ("quasirandom", C.c_int32), ("int_scheme", C.c_int32), ("deconv_stf", C.c_int32),
         ("damp_radius", C.c_double), ("mask_src_rec", C.c_int32),
+        ("db_page_elems", C.c_int32),
     ]
 
 
@@ -72,6 +73,7 @@ class MckStats(C.Structure):
         ("mc_steps", C.c_int32), ("strain_launches", C.c_int32),
         ("device_ms", C.c_double), ("strain_fwd_ms", C.c_double),
         ("strain_bwd_ms", C.c_double), ("wall_ms", C.c_double),
+        ("paged_bytes", C.c_int64),
     ]
 
 

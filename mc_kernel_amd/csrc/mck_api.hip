@@ -312,6 +312,23 @@ double tet_volume(const double *v) {
 struct DbDev {
     DevDb d{};
     float *u_owned = nullptr;      // device copy of the snapshots (null if caller-resident)
+    // paged snapshots (databases larger than device memory): the caller's host copy is
+    // streamed through two device buffers of page_elems elements
+    bool paged = false;
+    int page_elems = 0, npages = 0;
+    size_t elem_floats = 0;        // floats per element: nvar x 25 x nt
+    const float *host_u = nullptr;
+    float *slot[2] = {nullptr, nullptr};
+    hipEvent_t copied[2] = {nullptr, nullptr}, freed[2] = {nullptr, nullptr};
+    void release_pages() {
+        for (int i = 0; i < 2; ++i) {
+            if (slot[i]) (void)hipFree(slot[i]);
+            if (copied[i]) (void)hipEventDestroy(copied[i]);
+            if (freed[i]) (void)hipEventDestroy(freed[i]);
+            slot[i] = nullptr; copied[i] = freed[i] = nullptr;
+        }
+        paged = false; page_elems = npages = 0; host_u = nullptr;
+    }
     DBuf<float> mp_s, mp_z;
     DBuf<double> econ, ejac;
     DBuf<int> eltype, axis, cell_start, cell_elems;
@@ -357,6 +374,8 @@ struct State {
     // operator set-up (k_op_setup): gather weights and rotations of the points (by point
     // id) and of the pairs (by position in the pair order)
     DBuf<float> opWf, opQf, opWb, opQb;
+    DBuf<int> page_bounds;              // [npages + 1] first item of every database page
+    hipStream_t copy_stream = nullptr;  // page copies of paged databases
     DBuf<int> keys_in, keys_out, vals_in, vals_out, pair_count;
     DBuf<unsigned char> sort_tmp;
     DBuf<double> phys;
@@ -377,7 +396,8 @@ struct State {
 };
 State g;
 
-int upload_db(DbDev &D, const mck_db_t &db, int nt, const char *name) {
+// page_elems > 0: keep the snapshots in host memory and page them (see run_paged)
+int upload_db(DbDev &D, const mck_db_t &db, int nt, const char *name, int page_elems, hipStream_t st) {
     if (db.npol != 4) return fail(MCK_EINVAL, std::string(name) + ": only npol = 4 is supported");
     if (db.nelem <= 0 || !db.snapshots || !db.mp_s || !db.fem_mesh)
         return fail(MCK_EINVAL, std::string(name) + ": incomplete mesh");
@@ -425,22 +445,39 @@ int upload_db(DbDev &D, const mck_db_t &db, int nt, const char *name) {
     HIPCHECK(hipMemcpy(D.axis.p, db.axis, ne * 4, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(D.cell_start.p, cnt.data(), cnt.size() * 4, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(D.cell_elems.p, elems.data(), ne * 4, hipMemcpyHostToDevice));
-    // snapshots: use in place when they already live in device memory
-    const size_t nbytes = (size_t)ne * db.nvar * 25 * nt * sizeof(float);
+    // snapshots: use in place when they already live in device memory, page them from
+    // host memory when asked to, else copy them once
+    D.elem_floats = (size_t)db.nvar * 25 * nt;
+    const size_t nbytes = (size_t)ne * D.elem_floats * sizeof(float);
     hipPointerAttribute_t attr;
     bool on_device = false;
     if (hipPointerGetAttributes(&attr, db.snapshots) == hipSuccess)
         on_device = (attr.type == hipMemoryTypeDevice);
     (void)hipGetLastError();
     const float *u = db.snapshots;
-    if (!on_device) {
-        if (D.u_owned) { (void)hipFree(D.u_owned); D.u_owned = nullptr; }
+    if (D.u_owned) { (void)hipFree(D.u_owned); D.u_owned = nullptr; }
+    D.release_pages();
+    if (!on_device && page_elems > 0 && page_elems < ne) {
+        D.paged = true;
+        D.page_elems = page_elems;
+        D.npages = (ne + page_elems - 1) / page_elems;
+        D.host_u = db.snapshots;
+        for (int i = 0; i < 2; ++i) {
+            HIPCHECK(hipMalloc((void **)&D.slot[i], (size_t)page_elems * D.elem_floats * sizeof(float)));
+            HIPCHECK(hipEventCreateWithFlags(&D.copied[i], hipEventDisableTiming));
+            HIPCHECK(hipEventCreateWithFlags(&D.freed[i], hipEventDisableTiming));
+            HIPCHECK(hipEventRecord(D.freed[i], st));
+        }
+        u = D.slot[0];
+    } else if (!on_device) {
         HIPCHECK(hipMalloc((void **)&D.u_owned, nbytes));
         HIPCHECK(hipMemcpy(D.u_owned, db.snapshots, nbytes, hipMemcpyHostToDevice));
         u = D.u_owned;
     }
     D.d.nelem = ne; D.d.nvar = db.nvar; D.d.nt = nt;
     D.d.u = u;
+    D.d.u_first = 0;
+    D.d.u_last = D.paged ? std::min(ne, D.page_elems) : ne;
     D.d.mp_s = D.mp_s.p; D.d.mp_z = D.mp_z.p;
     D.d.econ = D.econ.p; D.d.eltype = D.eltype.p; D.d.axis = D.axis.p;
     D.d.gnx = gnx; D.d.gnz = gnz; D.d.gs0 = smin; D.d.gz0 = zmin; D.d.gh = h;
@@ -772,9 +809,9 @@ hipError_t launch_setup(State &G, bool fwd, int n, const int *order, const Locat
     return hipGetLastError();
 }
 
-// forward spectra of npts points (order: element-sorted point ids, or null = identity);
-// the operator set-up of the points (launch_setup) precedes it
-hipError_t launch_fwd(State &G, int npts, const int *order, const Located *loc, float2 *F) {
+// forward spectra of npts points (order: element-sorted point ids, or null = identity)
+// whose elements db holds; the operator set-up of the points (launch_setup) precedes it
+hipError_t launch_fwd(State &G, const DevDb &db, int npts, const int *order, const Located *loc, float2 *F) {
     if (npts <= 0) return hipSuccess;
     const size_t smem = strain_smem(G.pc);
     const dim3 grid(grid_x8(npts)), block(kStrainThreads);
@@ -784,7 +821,7 @@ hipError_t launch_fwd(State &G, int npts, const int *order, const Located *loc, 
         if (smem > 65536)                                                                         \
             (void)hipFuncSetAttribute((const void *)&k_fwd_spectra<SP, ND>,                       \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);     \
-        hipLaunchKernelGGL((k_fwd_spectra<SP, ND>), grid, block, smem, G.stream, G.fwd.d, G.pc,   \
+        hipLaunchKernelGGL((k_fwd_spectra<SP, ND>), grid, block, smem, G.stream, db, G.pc,        \
                            npts, order, loc, G.opWf.p, G.twq.p, F);                              \
         return hipGetLastError();                                                                 \
     }
@@ -804,10 +841,12 @@ size_t bwd_smem(const PathConf &pc) {
          + (size_t)5 * max_slots * sizeof(double);
 }
 
-// backward strain + evaluation of npairs (point, receiver) pairs q = p * R + r; the
-// operator set-up of the pairs (launch_setup, same order) precedes it
-hipError_t launch_bwd(State &G, int npairs, const int *order, const Located *loc_b, double *phys,
-                      double *base_out, const uint8_t *active, int P) {
+// backward strain + evaluation of npairs (point, receiver) pairs q = p * R + r whose
+// elements db holds; the operator set-up of the pairs (launch_setup) precedes it and
+// Wb, Qb point at the set-up of the first pair of this launch
+hipError_t launch_bwd(State &G, const DevDb &db, int npairs, const int *order, const Located *loc_b,
+                      const float *Wb, const float *Qb, double *phys, double *base_out,
+                      const uint8_t *active, int P) {
     if (npairs <= 0) return hipSuccess;
     // base kernels come from the base-kernel slot layout, physical kernels from the
     // combined one
@@ -822,8 +861,8 @@ hipError_t launch_bwd(State &G, int npairs, const int *order, const Located *loc
         if (smem > 65536)                                                                         \
             (void)hipFuncSetAttribute((const void *)&k_bwd_eval<SP, NF, NB>,                      \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);     \
-        hipLaunchKernelGGL((k_bwd_eval<SP, NF, NB>), grid, block, smem, G.stream, G.bwd.d, pc,    \
-                           npairs, order, G.R, loc_b, G.opWb.p, G.opQb.p, G.opQf.p, G.twq.p,     \
+        hipLaunchKernelGGL((k_bwd_eval<SP, NF, NB>), grid, block, smem, G.stream, db, pc,         \
+                           npairs, order, G.R, loc_b, Wb, Qb, G.opQf.p, G.twq.p,                 \
                            G.Fspec.p, recs, kerns, G.H.p, G.model_idx.p, G.model.p,              \
                            G.S.model_ndepth, G.S.relative_kernel, G.nk, phys, base_out,         \
                            G.xyz.p, active, P, G.d_rec_r.p);                                     \
@@ -836,20 +875,69 @@ hipError_t launch_bwd(State &G, int npairs, const int *order, const Located *loc
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_traces(State &G, int fwd, int frame, int npts, const Located *loc, float *out) {
+hipError_t launch_traces(State &G, const DevDb &db, int fwd, int frame, int npts, const Located *loc,
+                         float *out) {
     if (npts <= 0) return hipSuccess;
-    const DbDev &D = fwd ? G.fwd : G.bwd;
     switch (spt_for(G.pc.nt)) {
 #define CASE(SP)                                                                                  \
     case SP:                                                                                      \
         hipLaunchKernelGGL((k_strain_traces<SP>), dim3(npts), dim3(kStrainThreads), 0, G.stream,  \
-                           D.d, G.pc, G.frames.p, fwd, frame, npts, loc, out);                   \
+                           db, G.pc, G.frames.p, fwd, frame, npts, loc, out);                    \
         break;
         CASE(1) CASE(2) CASE(4) CASE(8) CASE(16)
 #undef CASE
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+// Runs launch(db, i0, i1) over the items [i0, i1) of a launch whose elements the device
+// view db holds: once over [0, n) when database D is resident; otherwise page by page,
+// each page's snapshots copied from the host into one of two device buffers on the copy
+// stream (the copy of the next page overlaps the kernels of this one; a buffer is
+// refilled only after the kernels that read it).  sorted_keys: element id of every item
+// in launch order, ascending (pages then cover disjoint item ranges and untouched pages
+// are not copied); null: every page covers all items and the kernels skip the items
+// outside it.
+template <class Launch>
+int run_paged(State &G, DbDev &D, int n, const int *sorted_keys, Launch &&launch) {
+    if (n <= 0) return MCK_OK;
+    if (!D.paged) {
+        HIPCHECK(launch(D.d, 0, n));
+        return MCK_OK;
+    }
+    std::vector<int> bounds(D.npages + 1, n);
+    bounds[0] = 0;
+    if (sorted_keys) {
+        HIPCHECK(G.page_bounds.ensure(D.npages + 1));
+        hipLaunchKernelGGL(k_page_bounds, dim3((D.npages + 256) / 256), dim3(256), 0, G.stream, n, sorted_keys,
+                           D.page_elems, D.npages, G.page_bounds.p);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(bounds.data(), G.page_bounds.p, (D.npages + 1) * sizeof(int),
+                                hipMemcpyDeviceToHost, G.stream));
+        HIPCHECK(hipStreamSynchronize(G.stream));
+    }
+    int used = 0;
+    for (int b = 0; b < D.npages; ++b) {
+        const int i0 = sorted_keys ? bounds[b] : 0, i1 = sorted_keys ? bounds[b + 1] : n;
+        if (i1 <= i0) continue;
+        const int k = used++ & 1;
+        const int e0 = b * D.page_elems, e1 = std::min(D.d.nelem, e0 + D.page_elems);
+        HIPCHECK(hipStreamWaitEvent(G.copy_stream, D.freed[k], 0));
+        HIPCHECK(hipMemcpyAsync(D.slot[k], D.host_u + (size_t)e0 * D.elem_floats,
+                                (size_t)(e1 - e0) * D.elem_floats * sizeof(float), hipMemcpyHostToDevice,
+                                G.copy_stream));
+        HIPCHECK(hipEventRecord(D.copied[k], G.copy_stream));
+        HIPCHECK(hipStreamWaitEvent(G.stream, D.copied[k], 0));
+        DevDb d = D.d;
+        d.u = D.slot[k];
+        d.u_first = e0;
+        d.u_last = e1;
+        HIPCHECK(launch(d, i0, i1));
+        HIPCHECK(hipEventRecord(D.freed[k], G.stream));
+        G.stats.paged_bytes += (int64_t)((size_t)(e1 - e0) * D.elem_floats * sizeof(float));
+    }
+    return MCK_OK;
 }
 
 // stable sort of n (element, id) pairs from keys_in/vals_in into keys_out/vals_out
@@ -1024,8 +1112,23 @@ int init_impl(const mck_setup_t *s) {
         pc.taper[j] = expf(-(q * q));
     }
     int rc;
-    if ((rc = upload_db(G.fwd, S.fwd, S.nt, "fwd"))) return rc;
-    if ((rc = upload_db(G.bwd, S.bwd, S.nt, "bwd"))) return rc;
+    if (S.db_page_elems < 0) return fail(MCK_EINVAL, "db_page_elems must be >= 0");
+    if (!G.copy_stream) HIPCHECK(hipStreamCreateWithFlags(&G.copy_stream, hipStreamNonBlocking));
+    {
+        // paging: as asked, or by itself when the two databases' snapshots exceed 3/4 of
+        // the free device memory (two pages of each then take 1/16 of it)
+        size_t free_b = 0, total_b = 0;
+        HIPCHECK(hipMemGetInfo(&free_b, &total_b));
+        const size_t ef = (size_t)S.fwd.nvar * 25 * S.nt * 4, eb = (size_t)S.bwd.nvar * 25 * S.nt * 4;
+        const size_t need = (size_t)std::max(S.fwd.nelem, 0) * ef + (size_t)std::max(S.bwd.nelem, 0) * eb;
+        int pf = S.db_page_elems, pb = S.db_page_elems;
+        if (pf == 0 && need > free_b / 4 * 3) {
+            pf = (int)std::max<size_t>(1, std::min<size_t>(free_b / 64 / ef, INT32_MAX));
+            pb = (int)std::max<size_t>(1, std::min<size_t>(free_b / 64 / eb, INT32_MAX));
+        }
+        if ((rc = upload_db(G.fwd, S.fwd, S.nt, "fwd", pf, G.stream))) return rc;
+        if ((rc = upload_db(G.bwd, S.bwd, S.nt, "bwd", pb, G.stream))) return rc;
+    }
     if ((rc = setup_tables(G))) return rc;
     for (DbDev *D : {&G.fwd, &G.bwd}) {
         // inverse Jacobians and 1/s at the nodes of every element (needs the basis)
@@ -1331,7 +1434,10 @@ int eval_chunk(State &G, int P, int w0, int nw, std::vector<std::pair<hipEvent_t
     HIPCHECK(hipEventCreate(&a));
     HIPCHECK(hipEventCreate(&b));
     HIPCHECK(hipEventRecord(a, st));
-    HIPCHECK(launch_fwd(G, np, G.vals_out.p, loc_f, G.Fspec.p));
+    if ((rc = run_paged(G, G.fwd, np, G.keys_out.p, [&](const DevDb &d, int i0, int i1) {
+             return launch_fwd(G, d, i1 - i0, G.vals_out.p + i0, loc_f, G.Fspec.p);
+         })))
+        return rc;
     HIPCHECK(hipEventRecord(b, st));
     evs.push_back({a, b});
     // backward: active located pairs in element order
@@ -1352,7 +1458,15 @@ int eval_chunk(State &G, int P, int w0, int nw, std::vector<std::pair<hipEvent_t
     HIPCHECK(hipEventCreate(&a));
     HIPCHECK(hipEventCreate(&b));
     HIPCHECK(hipEventRecord(a, st));
-    HIPCHECK(launch_bwd(G, npairs, G.vals_out.p, loc_b, G.phys.p + (size_t)w0 * P * G.nk, nullptr, act, P));
+    {
+        const size_t nwb = (size_t)G.pc.nd_b * G.pc.nvar_bwd * 25;
+        double *phys = G.phys.p + (size_t)w0 * P * G.nk;
+        if ((rc = run_paged(G, G.bwd, npairs, G.keys_out.p, [&](const DevDb &d, int i0, int i1) {
+                 return launch_bwd(G, d, i1 - i0, G.vals_out.p + i0, loc_b, G.opWb.p + (size_t)i0 * nwb,
+                                   G.opQb.p + (size_t)i0 * 36, phys, nullptr, act, P);
+             })))
+            return rc;
+    }
     HIPCHECK(hipEventRecord(b, st));
     evs.push_back({a, b});
     G.stats.strain_launches += 2;
@@ -1685,10 +1799,14 @@ int mck_load_strain_points(int32_t frame, int32_t npts, const double *xyz, float
     float *d_tr = nullptr;
     const size_t n = (size_t)npts * G.pc.nd * G.pc.nt;
     HIPCHECK(hipMalloc((void **)&d_tr, n * sizeof(float)));
-    hipError_t e = launch_traces(G, frame == 0, frame, npts, G.loc.p, d_tr);
-    if (e == hipSuccess) e = hipMemcpyAsync(traces, d_tr, n * sizeof(float), hipMemcpyDeviceToHost, G.stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(G.stream);
+    rc = run_paged(G, frame == 0 ? G.fwd : G.bwd, npts, nullptr, [&](const DevDb &d, int, int) {
+        return launch_traces(G, d, frame == 0, frame, npts, G.loc.p, d_tr);
+    });
+    hipError_t e = hipSuccess;
+    if (rc == MCK_OK) e = hipMemcpyAsync(traces, d_tr, n * sizeof(float), hipMemcpyDeviceToHost, G.stream);
+    if (rc == MCK_OK && e == hipSuccess) e = hipStreamSynchronize(G.stream);
     (void)hipFree(d_tr);
+    if (rc) return rc;
     if (e != hipSuccess) return fail(MCK_EHIP, hipGetErrorString(e));
     int err = 0;
     HIPCHECK(hipMemcpy(&err, G.err.p, 4, hipMemcpyDeviceToHost));
@@ -1724,14 +1842,21 @@ int mck_base_kernels(int32_t npts, const double *xyz, double *base) {
     HIPCHECK(hipMalloc((void **)&d_phys, (size_t)npts * G.nk * 8));
     hipError_t e = hipMemsetAsync(d_base, 0, (size_t)npts * G.nk * 6 * 8, G.stream);
     if (e == hipSuccess) e = launch_setup(G, true, npts, nullptr, loc_f);
-    if (e == hipSuccess) e = launch_fwd(G, npts, nullptr, loc_f, G.Fspec.p);
-    if (e == hipSuccess) e = launch_setup(G, false, npts * R, nullptr, loc_b);
-    if (e == hipSuccess) e = launch_bwd(G, npts * R, nullptr, loc_b, d_phys, d_base, nullptr, 1);
     if (e == hipSuccess)
+        rc = run_paged(G, G.fwd, npts, nullptr, [&](const DevDb &d, int, int) {
+            return launch_fwd(G, d, npts, nullptr, loc_f, G.Fspec.p);
+        });
+    if (e == hipSuccess && rc == MCK_OK) e = launch_setup(G, false, npts * R, nullptr, loc_b);
+    if (e == hipSuccess && rc == MCK_OK)
+        rc = run_paged(G, G.bwd, npts * R, nullptr, [&](const DevDb &d, int, int) {
+            return launch_bwd(G, d, npts * R, nullptr, loc_b, G.opWb.p, G.opQb.p, d_phys, d_base, nullptr, 1);
+        });
+    if (e == hipSuccess && rc == MCK_OK)
         e = hipMemcpyAsync(base, d_base, (size_t)npts * G.nk * 6 * 8, hipMemcpyDeviceToHost, G.stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(G.stream);
+    if (e == hipSuccess && rc == MCK_OK) e = hipStreamSynchronize(G.stream);
     (void)hipFree(d_base);
     (void)hipFree(d_phys);
+    if (rc) return rc;
     if (e != hipSuccess) return fail(MCK_EHIP, hipGetErrorString(e));
     int err = 0;
     HIPCHECK(hipMemcpy(&err, G.err.p, 4, hipMemcpyDeviceToHost));
@@ -1818,6 +1943,7 @@ void mck_finalize(void) {
     for (DbDev *D : {&G.fwd, &G.bwd}) {
         if (D->u_owned) (void)hipFree(D->u_owned);
         D->u_owned = nullptr;
+        D->release_pages();
         D->mp_s.release(); D->mp_z.release(); D->econ.release(); D->ejac.release(); D->eltype.release();
         D->axis.release(); D->cell_start.release(); D->cell_elems.release();
     }
@@ -1830,6 +1956,7 @@ void mck_finalize(void) {
     G.conv.release(); G.done.release(); G.niter.release(); G.draws.release(); G.steps.release();
     G.nmodels.release(); G.xyz.release(); G.model_idx.release(); G.loc.release();
     G.Fspec.release(); G.phys.release(); G.evals.release(); G.err.release();
+    G.opWf.release(); G.opQf.release(); G.opWb.release(); G.opQb.release(); G.page_bounds.release();
     G.abinv.release(); G.bweights.release();
     G.elem_count.release(); G.keys_in.release(); G.keys_out.release();
     G.vals_in.release(); G.vals_out.release(); G.pair_count.release(); G.sort_tmp.release();

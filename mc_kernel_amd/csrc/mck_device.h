@@ -35,7 +35,8 @@ struct Basis {
 // One merged database in HBM.
 struct DevDb {
     int nelem, nvar, nt;
-    const float *u;           // [nelem][nvar][5][5][nt]
+    const float *u;           // [u_last - u_first][nvar][5][5][nt]: elements u_first ..
+    int u_first, u_last;      // u_last - 1 (all elements when resident, one page when paged)
     const float *mp_s, *mp_z; // element midpoints
     const double *econ;       // [nelem][16] element constants (see host setup)
     const double *ejac;       // [nelem][25][5] inverse Jacobian and 1/s at the nodes

@@ -445,11 +445,12 @@ __global__ void k_elem_tables(DevDb db, double *out)
 //  S.Q[i][c] is then the rotation of component c into earth-frame component i.
 // The set-up runs in two parts separated by a barrier: op_setup_parts (element tables,
 // Lagrange bases, output transform) and op_weights.  nthr threads cooperate (tid < nthr);
-// role_li, role_lj and role_xf name the threads that form the two Lagrange bases and the
-// output transform.
+// threads role_l .. role_l + 9 form the ten Lagrange basis values, role_xf .. role_xf +
+// nd - 1 the rotation columns and role_m .. role_m + nsim - 1 the output transform of one
+// simulation each (each value is computed exactly as by a single thread).
 __device__ void op_setup_parts(OpShared &S, const DevDb &db, const PathConf &pc, const DevFrame &fr,
                                const Located &L, bool fwd, int nd, const int *comps, bool src_mode,
-                               int tid, int nthr, int role_li, int role_lj, int role_xf)
+                               int tid, int nthr, int role_l, int role_xf, int role_m)
 {
     const int e = L.elem;
     if (tid == 0) { S.elem = e; S.axial = db.axis[e]; S.type = db.eltype[e]; }
@@ -459,65 +460,64 @@ __device__ void op_setup_parts(OpShared &S, const DevDb &db, const PathConf &pc,
         const int n = i / 5, c = i - (i / 5) * 5;
         if (c < 4) S.invj[n][c] = v; else S.sinv[n] = v;
     }
-    if (tid == role_li || tid == role_lj) {
-        // Lagrange basis at (xi, eta) (spectral_basis.f90:203-217)
-        const double *xp = db.axis[e] ? c_basis.glj : c_basis.gll;
-        const double *pts = tid == role_li ? xp : c_basis.gll;
-        double x = tid == role_li ? L.xi : L.eta;
-        double *l = tid == role_li ? S.li : S.lj;
-        for (int i = 0; i < 5; ++i) {
-            double v = 1.0;
-            for (int m = 0; m < 5; ++m) {
-                if (m == i) continue;
-                v = v * (x - pts[m]) / (pts[i] - pts[m]);
-            }
-            l[i] = v;
+    if (tid >= role_l && tid < role_l + 10) {
+        // Lagrange basis at xi (GLJ on the axis) or eta (spectral_basis.f90:203-217)
+        const int which = tid - role_l;
+        const bool is_xi = which < 5;
+        const int i = is_xi ? which : which - 5;
+        const double *pts = is_xi ? (db.axis[e] ? c_basis.glj : c_basis.gll) : c_basis.gll;
+        const double x = is_xi ? L.xi : L.eta;
+        double v = 1.0;
+        for (int m = 0; m < 5; ++m) {
+            if (m == i) continue;
+            v = v * (x - pts[m]) / (pts[i] - pts[m]);
         }
+        if (is_xi) S.li[i] = v; else S.lj[i] = v;
     }
-    if (tid == role_xf) {
-        // output transform: sign flip of comps 4, 6 (readfields.f90:2180-2184), azimuthal
-        // factors, 1/amplitude, src -> xyz -> earth rotations (full strain only)
-        double amp = db.amplitude;
-        int nsim = fwd ? pc.nsim_fwd : 1;
-        double sp = sin(L.phi), cp = cos(L.phi);
-        if (pc.full && src_mode) {
-            for (int c = 0; c < nd; ++c) {
+    if (pc.full && src_mode && tid >= role_xf && tid < role_xf + nd) {
+        // rotation of carried component c: src -> xyz -> earth (full strain only)
+        const int c = tid - role_xf;
+        const double sp = sin(L.phi), cp = cos(L.phi);
+        double t[6] = {0, 0, 0, 0, 0, 0};
+        t[comps[c]] = 1.0;
+        rot_src_to_xyz(t, sp, cp);
+        rot_xyz_to_earth(t, fr.sin_lon, fr.cos_lon, fr.sin_colat, fr.cos_colat);
+        for (int i = 0; i < 6; ++i) S.Q[i][c] = (float)t[i];
+    }
+    const int nsim = fwd ? pc.nsim_fwd : 1;
+    if (tid >= role_m && tid < role_m + nsim) {
+        // output transform of simulation si: sign flip of comps 4, 6 (readfields.f90:
+        // 2180-2184), azimuthal factors, 1/amplitude (and, in earth mode, the rotations)
+        const int si = tid - role_m;
+        const double amp = db.amplitude;
+        double az[6];
+        if (fwd) {
+            int isim = pc.fwd_sim[si];
+            double a1 = azim_factor(L.phi, pc.mij, isim, 1), a2 = azim_factor(L.phi, pc.mij, isim, 2);
+            az[0] = a1; az[1] = a1; az[2] = a1; az[3] = a2; az[4] = a1; az[5] = a2;
+        } else if (pc.bwd_isim == 1) {
+            for (int m = 0; m < 6; ++m) az[m] = 1.0;
+        } else {
+            double a1 = pc.bwd_fi[1] * cos(L.phi) + pc.bwd_fi[2] * sin(L.phi);
+            double a2 = -pc.bwd_fi[1] * sin(L.phi) + pc.bwd_fi[2] * cos(L.phi);
+            az[0] = a1; az[1] = a1; az[2] = a1; az[3] = a2; az[4] = a1 * 2; az[5] = a2 * 2;
+        }
+        if (!pc.full) {
+            for (int m = 0; m < 6; ++m) S.M[0][si][m] = (m == 0) ? az[0] / amp : 0.0;
+        } else if (src_mode) {
+            for (int c = 0; c < 6; ++c)
+                for (int m = 0; m < 6; ++m)
+                    S.M[c][si][m] = (c < nd && m == comps[c])
+                                  ? ((m == 3 || m == 5) ? -1.0 : 1.0) * az[m] / amp : 0.0;
+        } else {
+            const double sp = sin(L.phi), cp = cos(L.phi);
+            for (int m = 0; m < 6; ++m) {
                 double t[6] = {0, 0, 0, 0, 0, 0};
-                t[comps[c]] = 1.0;
+                t[m] = 1.0;
                 rot_src_to_xyz(t, sp, cp);
                 rot_xyz_to_earth(t, fr.sin_lon, fr.cos_lon, fr.sin_colat, fr.cos_colat);
-                for (int i = 0; i < 6; ++i) S.Q[i][c] = (float)t[i];
-            }
-        }
-        for (int si = 0; si < nsim; ++si) {
-            double az[6];
-            if (fwd) {
-                int isim = pc.fwd_sim[si];
-                double a1 = azim_factor(L.phi, pc.mij, isim, 1), a2 = azim_factor(L.phi, pc.mij, isim, 2);
-                az[0] = a1; az[1] = a1; az[2] = a1; az[3] = a2; az[4] = a1; az[5] = a2;
-            } else if (pc.bwd_isim == 1) {
-                for (int m = 0; m < 6; ++m) az[m] = 1.0;
-            } else {
-                double a1 = pc.bwd_fi[1] * cos(L.phi) + pc.bwd_fi[2] * sin(L.phi);
-                double a2 = -pc.bwd_fi[1] * sin(L.phi) + pc.bwd_fi[2] * cos(L.phi);
-                az[0] = a1; az[1] = a1; az[2] = a1; az[3] = a2; az[4] = a1 * 2; az[5] = a2 * 2;
-            }
-            if (!pc.full) {
-                for (int m = 0; m < 6; ++m) S.M[0][si][m] = (m == 0) ? az[0] / amp : 0.0;
-            } else if (src_mode) {
-                for (int c = 0; c < 6; ++c)
-                    for (int m = 0; m < 6; ++m)
-                        S.M[c][si][m] = (c < nd && m == comps[c])
-                                      ? ((m == 3 || m == 5) ? -1.0 : 1.0) * az[m] / amp : 0.0;
-            } else {
-                for (int m = 0; m < 6; ++m) {
-                    double t[6] = {0, 0, 0, 0, 0, 0};
-                    t[m] = 1.0;
-                    rot_src_to_xyz(t, sp, cp);
-                    rot_xyz_to_earth(t, fr.sin_lon, fr.cos_lon, fr.sin_colat, fr.cos_colat);
-                    double f = ((m == 3 || m == 5) ? -1.0 : 1.0) * az[m] / amp;
-                    for (int c = 0; c < 6; ++c) S.M[c][si][m] = t[c] * f;
-                }
+                double f = ((m == 3 || m == 5) ? -1.0 : 1.0) * az[m] / amp;
+                for (int c = 0; c < 6; ++c) S.M[c][si][m] = t[c] * f;
             }
         }
     }
@@ -589,7 +589,7 @@ __device__ void build_weights(OpShared &S, const DevDb &db, const PathConf &pc, 
                               const Located &L, bool fwd, int nvar, const int *vars, int nd,
                               const int *comps, bool src_mode)
 {
-    op_setup_parts(S, db, pc, fr, L, fwd, nd, comps, src_mode, threadIdx.x, blockDim.x, 160, 161, 192);
+    op_setup_parts(S, db, pc, fr, L, fwd, nd, comps, src_mode, threadIdx.x, blockDim.x, 160, 192, 224);
     __syncthreads();
     op_weights(S, pc, fwd, nvar, vars, nd, threadIdx.x, blockDim.x, S.W);
     __syncthreads();
@@ -626,7 +626,7 @@ __global__ __launch_bounds__(64 * kSetupWaves) void k_op_setup(DevDb db, PathCon
     const int nd = fwd ? pc.nd_f : pc.nd_b;
     const int *comps = fwd ? pc.comps_f : pc.comps_b;
     const int frame = fwd ? 0 : 1 + (q - (q / R) * R);
-    if (live) op_setup_parts(S, db, pc, frames[frame], L, fwd != 0, nd, comps, true, lane, 64, 61, 62, 63);
+    if (live) op_setup_parts(S, db, pc, frames[frame], L, fwd != 0, nd, comps, true, lane, 64, 40, 50, 56);
     __syncthreads();
     if (!live) return;
     const size_t item = fwd ? (size_t)q : (size_t)lb;
@@ -986,14 +986,14 @@ __global__ __launch_bounds__(kStrainThreads) void k_strain_traces(
     const int p = blockIdx.x;
     if (p >= npts) return;
     const Located L = loc[p];
-    if (L.elem < 0) return;
+    if (L.elem < db.u_first || L.elem >= db.u_last) return;   // not located / not in this page
     const int nvar = fwd ? pc.nvar_fwd : pc.nvar_bwd;
     const int *vars = fwd ? pc.fwd_vars : pc.bwd_vars;
     build_weights(S, db, pc, frames[frame], L, fwd != 0, nvar, vars, pc.nd, nullptr, false);
     const int nt = pc.nt, nd = pc.nd, t0 = threadIdx.x * SPT;
     const size_t blk = (size_t)25 * nt;
     float acc[6][SPT];
-    gather_strain<SPT, 6>(S, db.u + (size_t)S.elem * db.nvar * blk, blk, nvar, vars, t0, nt, acc);
+    gather_strain<SPT, 6>(S, db.u + (size_t)(L.elem - db.u_first) * db.nvar * blk, blk, nvar, vars, t0, nt, acc);
 #pragma unroll
     for (int c = 0; c < 6; ++c)
         if (c < nd)
@@ -1026,7 +1026,7 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && ND <= 4) ? 3 : (SPT <=
     if (lb < 0) return;
     const int p = order ? order[lb] : lb;
     const Located L = loc[p];
-    if (L.elem < 0) return;
+    if (L.elem < db.u_first || L.elem >= db.u_last) return;   // not located / not in this page
     load_twiddles(twq, pc.M, Tq);
     {
         // gather weights of the point (k_op_setup)
@@ -1039,7 +1039,7 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && ND <= 4) ? 3 : (SPT <=
     const int tid = threadIdx.x, nt = pc.nt, M = pc.M, t0 = tid * SPT;
     const size_t blk = (size_t)25 * nt;
     float acc[ND][SPT];
-    gather_strain<SPT, ND>(S, db.u + (size_t)L.elem * db.nvar * blk, blk, pc.nvar_fwd, pc.fwd_vars, t0, nt, acc);
+    gather_strain<SPT, ND>(S, db.u + (size_t)(L.elem - db.u_first) * db.nvar * blk, blk, pc.nvar_fwd, pc.fwd_vars, t0, nt, acc);
     const float fdt = (float)pc.dt, h = (float)(pc.dt * 0.5);
     float tap[SPT];
     taper_factors(t0, nt, pc, tap);
@@ -1089,7 +1089,7 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && NDB <= 4) ? 3 : (SPT <
     const int q = order ? order[lb] : lb;
     const int p = q / R, r = q - (q / R) * R;
     const Located L = loc_b[q];
-    if (L.elem < 0) return;
+    if (L.elem < db.u_first || L.elem >= db.u_last) return;   // not located / not in this page
     load_twiddles(twq, pc.M, Tq);
     const int tid = threadIdx.x;
     {
@@ -1124,7 +1124,7 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && NDB <= 4) ? 3 : (SPT <
     float2 spec[NDB][SPT], specM[NDB];
     {
         float acc[NDB][SPT];
-        gather_strain<SPT, NDB>(S, db.u + (size_t)L.elem * db.nvar * blk, blk, pc.nvar_bwd, pc.bwd_vars, t0, nt, acc);
+        gather_strain<SPT, NDB>(S, db.u + (size_t)(L.elem - db.u_first) * db.nvar * blk, blk, pc.nvar_bwd, pc.bwd_vars, t0, nt, acc);
         const float fdt = (float)pc.dt;
         float tap[SPT];
         taper_factors(t0, nt, pc, tap);
@@ -1350,6 +1350,21 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && NDB <= 4) ? 3 : (SPT <
 
 // sort keys: element of every point (fwd) / of every active pair with a located element
 // (bwd, compacted through an atomic counter); keys past the last element sort last
+// first item of every database page in element-sorted keys: out[b] = number of keys
+// below b * page_elems, b = 0 .. npages (keys of unlocated items, nelem, sort last)
+__global__ void k_page_bounds(int n, const int *keys, int page_elems, int npages, int *out)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > npages) return;
+    const long long e = (long long)b * page_elems;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = lo + ((hi - lo) >> 1);
+        if (keys[mid] < e) lo = mid + 1; else hi = mid;
+    }
+    out[b] = lo;
+}
+
 __global__ void k_keys_points(int n, const Located *loc, int nelem, int *keys, int *vals)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -62,12 +62,14 @@ module mck_gpu
      integer(c_int32_t) :: int_over_volume, relative_kernel, quasirandom, int_scheme, deconv_stf
      real(c_double) :: damp_radius
      integer(c_int32_t) :: mask_src_rec
+     integer(c_int32_t) :: db_page_elems
   end type mck_setup_t
 
   type, bind(c), public :: mck_stats_t
      integer(c_int64_t) :: evaluations, points, fwd_locations, bwd_locations, fwd_bytes, bwd_bytes
      integer(c_int32_t) :: mc_steps, strain_launches
      real(c_double) :: device_ms, strain_fwd_ms, strain_bwd_ms, wall_ms
+     integer(c_int64_t) :: paged_bytes
   end type mck_stats_t
 
   public :: mck_init, mck_slave_work, mck_slave_work_plain, mck_locate_points, &

@@ -1,0 +1,69 @@
+"""Databases paged from host memory (mck_setup_t.db_page_elems > 0: the snapshots stay in
+host memory and every launch copies the element pages its points touch through two
+device buffers per database) give the same results as resident databases, bit for bit:
+
+  * slave_work (element-sorted launches: each page covers one range of the sorted points
+    and pairs, untouched pages are not copied), for a monopole (explosion, Z) and a
+    general moment tensor with an R receiver (all four forward simulations);
+  * base kernels and strain traces (unsorted launches: every page covers all points and
+    the kernels skip the points outside it);
+  * a page larger than the database falls back to the resident copy.
+"""
+import numpy as np
+import pytest
+
+from mc_kernel_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+DC = [0.3e20, 1.0e20, -1.3e20, 0.5e20, -0.4e20, 0.7e20]
+CASES = {"explosion_Z": {}, "general_mij_R": dict(mij=DC, component="R")}
+PAGE = 97          # elements per page: about 20 pages of the tiny databases
+
+
+def _with_kernel(page, kw, fn):
+    import mc_kernel_amd as mk
+    p = synth.make_problem("tiny", ntet=12, **kw)
+    p.setup.db_page_elems = page
+    k = mk.MCKernel(p.setup)
+    try:
+        return p, fn(p, k)
+    finally:
+        k.close()
+
+
+def _task(p, k):
+    res = k.slave_work(p.verts, p.conn)
+    x = p.sample_points(40, seed=9)
+    traces = [k.load_strain(f, x) for f in (0, 1)]
+    return res, k.base_kernels(x), traces
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_paged_equals_resident(case):
+    print("paging case", case, flush=True)
+    p, (ref, base_ref, tr_ref) = _with_kernel(0, CASES[case], _task)
+    nelem = p.setup.fwd.nelem
+    assert nelem > 4 * PAGE
+    _, (got, base_got, tr_got) = _with_kernel(PAGE, CASES[case], _task)
+    assert np.abs(ref["values"]).max() > 0
+    for key in ("values", "variance", "niterations"):
+        np.testing.assert_array_equal(got[key], ref[key], err_msg=key)
+    assert got["halton_step"] == ref["halton_step"]
+    np.testing.assert_array_equal(base_got, base_ref)
+    for a, b in zip(tr_got, tr_ref):
+        np.testing.assert_array_equal(a, b)
+    # the task's launches copied pages, and never more than every page once per launch
+    st, st0 = got["stats"], ref["stats"]
+    assert st0["paged_bytes"] == 0
+    assert st["paged_bytes"] > 0
+    db_bytes = 4 * 25 * p.setup.nt * (p.setup.fwd.nvar * nelem + p.setup.bwd.nvar * p.setup.bwd.nelem)
+    assert st["paged_bytes"] <= st["mc_steps"] * db_bytes
+    assert st["evaluations"] == st0["evaluations"]
+
+
+def test_page_larger_than_database_is_resident():
+    p, ref = _with_kernel(0, {}, lambda p, k: k.slave_work(p.verts, p.conn))
+    _, got = _with_kernel(10 ** 9, {}, lambda p, k: k.slave_work(p.verts, p.conn))
+    np.testing.assert_array_equal(got["values"], ref["values"])
+    assert got["stats"]["paged_bytes"] == 0
