@@ -125,6 +125,33 @@ def gpu_db(p):
     return lib, out
 
 
+def page_db_to_host(p, lib, dbs, pinned):
+    """Moves the generated databases to host memory (page-locked or pageable), for the
+    library to page them (mck_setup_t.db_page_elems); returns the host buffers to keep."""
+    m = p.mesh
+    keep = []
+    lib.synth_gpu_download.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+    lib.synth_gpu_free.argtypes = [C.c_void_p]
+    lib.synth_gpu_host_alloc.argtypes = [C.c_size_t, C.POINTER(C.c_void_p)]
+    for i, (nvar, db) in enumerate(((10, p.setup.fwd), (5, p.setup.bwd))):
+        n = m.nelem * nvar * 25 * p.nt
+        if pinned:
+            h = C.c_void_p()
+            if lib.synth_gpu_host_alloc(n * 4, C.byref(h)):
+                raise RuntimeError("page-locked host allocation failed")
+            ptr = h.value
+            keep.append(h)
+        else:
+            arr = np.empty(n, np.float32)
+            ptr = arr.ctypes.data
+            keep.append(arr)
+        if lib.synth_gpu_download(dbs[i], ptr, n):
+            raise RuntimeError("database download failed")
+        lib.synth_gpu_free(dbs[i])
+        db.snapshots = C.cast(C.c_void_p(ptr), C.POINTER(C.c_float))
+    return keep
+
+
 def measured_traffic(path, kernel):
     """HBM bytes per launch of `kernel` from the committed counter summary: lines
     'traffic <kernel>: <bytes per launch>' written from the FETCH_SIZE and WRITE_SIZE
@@ -211,6 +238,12 @@ def main():
                     help="time the differentiable filter-convolve-window-integrate stage at "
                          "the c3 size instead (forward and adjoint, one JSON line)")
     ap.add_argument("--autograd-points", type=int, default=10000)
+    ap.add_argument("--db-page-elems", type=int, default=0,
+                    help="keep the databases in host memory and page them, N elements per "
+                         "page (0: resident in HBM, the default workload)")
+    ap.add_argument("--db-host-memory", choices=["pinned", "pageable"], default="pinned")
+    ap.add_argument("--db-device-pages", type=int, default=0,
+                    help="device page buffers per database when paging (0: 2)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -239,6 +272,11 @@ def main():
             raise RuntimeError("copy calibration failed")
         print("copy calibration: %d bytes read + %d written per launch, %.3f ms, %.1f GB/s"
               % (nbytes, nbytes, ms.value, 2 * nbytes / ms.value / 1e6), file=sys.stderr, flush=True)
+    host_dbs = None
+    if args.db_page_elems > 0:
+        host_dbs = page_db_to_host(p, lib, dbs, args.db_host_memory == "pinned")
+        p.setup.db_page_elems = args.db_page_elems
+        p.setup.db_device_pages = args.db_device_pages
     k = mk.MCKernel(p.setup)
     T = args.task if args.task > 0 else p.ntet
     ntask = (p.ntet + T - 1) // T
@@ -251,8 +289,8 @@ def main():
         dt = time.perf_counter() - t0
         if i >= args.warmup:
             steps.append((dt, r["stats"]))
-        print("step %d: %.3f s, %d evals, %d MC steps" % (i, dt, r["stats"]["evaluations"],
-                                                          r["stats"]["mc_steps"]),
+        print("step %d: %.3f s, %d evals, %d MC steps, %.1f GB paged" % (
+              i, dt, r["stats"]["evaluations"], r["stats"]["mc_steps"], r["stats"]["paged_bytes"] / 1e9),
               file=sys.stderr, flush=True)
     tot_t = sum(s[0] for s in steps)
     tot_e = sum(s[1]["evaluations"] for s in steps)
@@ -303,6 +341,14 @@ def main():
                    "evaluations": int(tot_e), "device_ms": dev_ms},
         "roofline": roof, "cpu_baseline": cpu, "plain_form": plain,
     }
+    if host_dbs is not None:
+        paged = sum(s[1]["paged_bytes"] for s in steps)
+        line["config"]["databases"] = ("paged from %s host memory, %d elements per page, "
+                                       "%d device page buffers per database"
+                                       % (args.db_host_memory, args.db_page_elems,
+                                          max(2, args.db_device_pages)))
+        line["paging"] = {"bytes": paged, "host_to_device_GBs": paged / tot_t / 1e9}
+        line["data"] = "synthetic (seeded AxiSEM-style merged DB generated in HBM, moved to host memory)"
     print(json.dumps(line), flush=True)
 
 

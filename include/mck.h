@@ -161,12 +161,16 @@ typedef struct {
      * reads hyperslabs from NetCDF on demand).  0: each database's snapshots are copied
      * to device memory once (used in place when `snapshots` already is device memory),
      * unless the two together exceed 3/4 of the free device memory, in which case they
-     * are paged with pages of 1/64 of it.  > 0: snapshots in host memory are paged: they
-     * stay where they are (the caller keeps them valid and unchanged until mck_finalize)
-     * and every launch copies the element pages its points touch, db_page_elems
-     * elements each, through two device buffers per database.  Results are identical
-     * to the resident mode.                                                             */
+     * are paged with pages of 1/64 of it and buffers filling half of it.  > 0: snapshots
+     * in host memory are paged: they stay where they are (the caller keeps them valid
+     * and unchanged until mck_finalize) and every launch copies the element pages its
+     * points touch, db_page_elems elements each, into device page buffers.  Results
+     * are identical to the resident mode.                                               */
     int32_t db_page_elems;
+    /* Device page buffers per database when paging (0 or < 2: 2).  The first
+     * db_device_pages - 2 pages keep a buffer of their own once loaded; the others
+     * rotate through the last two buffers (copies overlap the kernels).                 */
+    int32_t db_device_pages;
 } mck_setup_t;
 
 /* Work-unit statistics of the last mck_slave_work call (device timings via hipEvents). */

@@ -61,7 +61,7 @@ class MckSetup(C.Structure):
         ("int_over_volume", C.c_int32), ("relative_kernel", C.c_int32),
         ("quasirandom", C.c_int32), ("int_scheme", C.c_int32), ("deconv_stf", C.c_int32),
         ("damp_radius", C.c_double), ("mask_src_rec", C.c_int32),
-        ("db_page_elems", C.c_int32),
+        ("db_page_elems", C.c_int32), ("db_device_pages", C.c_int32),
     ]
 
 

@@ -313,21 +313,24 @@ struct DbDev {
     DevDb d{};
     float *u_owned = nullptr;      // device copy of the snapshots (null if caller-resident)
     // paged snapshots (databases larger than device memory): the caller's host copy is
-    // streamed through two device buffers of page_elems elements
+    // paged through nslot device buffers of page_elems elements.  Pages below nslot - 2
+    // keep a buffer of their own once loaded; the others rotate through the last two.
     bool paged = false;
-    int page_elems = 0, npages = 0;
+    int page_elems = 0, npages = 0, nslot = 0;
     size_t elem_floats = 0;        // floats per element: nvar x 25 x nt
     const float *host_u = nullptr;
-    float *slot[2] = {nullptr, nullptr};
-    hipEvent_t copied[2] = {nullptr, nullptr}, freed[2] = {nullptr, nullptr};
+    std::vector<float *> slot;
+    std::vector<int> slot_page;    // page held by each buffer (-1: none)
+    std::vector<hipEvent_t> copied, freed;
+    int rot = 0;                   // next rotating buffer
     void release_pages() {
-        for (int i = 0; i < 2; ++i) {
+        for (size_t i = 0; i < slot.size(); ++i) {
             if (slot[i]) (void)hipFree(slot[i]);
             if (copied[i]) (void)hipEventDestroy(copied[i]);
             if (freed[i]) (void)hipEventDestroy(freed[i]);
-            slot[i] = nullptr; copied[i] = freed[i] = nullptr;
         }
-        paged = false; page_elems = npages = 0; host_u = nullptr;
+        slot.clear(); slot_page.clear(); copied.clear(); freed.clear();
+        paged = false; page_elems = npages = nslot = rot = 0; host_u = nullptr;
     }
     DBuf<float> mp_s, mp_z;
     DBuf<double> econ, ejac;
@@ -396,8 +399,10 @@ struct State {
 };
 State g;
 
-// page_elems > 0: keep the snapshots in host memory and page them (see run_paged)
-int upload_db(DbDev &D, const mck_db_t &db, int nt, const char *name, int page_elems, hipStream_t st) {
+// page_elems > 0: keep the snapshots in host memory and page them through nslot (>= 2)
+// device buffers (see run_paged)
+int upload_db(DbDev &D, const mck_db_t &db, int nt, const char *name, int page_elems, int nslot,
+              hipStream_t st) {
     if (db.npol != 4) return fail(MCK_EINVAL, std::string(name) + ": only npol = 4 is supported");
     if (db.nelem <= 0 || !db.snapshots || !db.mp_s || !db.fem_mesh)
         return fail(MCK_EINVAL, std::string(name) + ": incomplete mesh");
@@ -461,8 +466,13 @@ int upload_db(DbDev &D, const mck_db_t &db, int nt, const char *name, int page_e
         D.paged = true;
         D.page_elems = page_elems;
         D.npages = (ne + page_elems - 1) / page_elems;
+        D.nslot = std::max(2, std::min(nslot, D.npages));
         D.host_u = db.snapshots;
-        for (int i = 0; i < 2; ++i) {
+        D.slot.assign(D.nslot, nullptr);
+        D.slot_page.assign(D.nslot, -1);
+        D.copied.assign(D.nslot, nullptr);
+        D.freed.assign(D.nslot, nullptr);
+        for (int i = 0; i < D.nslot; ++i) {
             HIPCHECK(hipMalloc((void **)&D.slot[i], (size_t)page_elems * D.elem_floats * sizeof(float)));
             HIPCHECK(hipEventCreateWithFlags(&D.copied[i], hipEventDisableTiming));
             HIPCHECK(hipEventCreateWithFlags(&D.freed[i], hipEventDisableTiming));
@@ -892,13 +902,13 @@ hipError_t launch_traces(State &G, const DevDb &db, int fwd, int frame, int npts
 }
 
 // Runs launch(db, i0, i1) over the items [i0, i1) of a launch whose elements the device
-// view db holds: once over [0, n) when database D is resident; otherwise page by page,
-// each page's snapshots copied from the host into one of two device buffers on the copy
-// stream (the copy of the next page overlaps the kernels of this one; a buffer is
-// refilled only after the kernels that read it).  sorted_keys: element id of every item
-// in launch order, ascending (pages then cover disjoint item ranges and untouched pages
-// are not copied); null: every page covers all items and the kernels skip the items
-// outside it.
+// view db holds: once over [0, n) when database D is resident; otherwise page by page.
+// A page below nslot - 2 is copied into its own buffer on first use and stays; any other
+// page is copied into one of the two rotating buffers (on the copy stream: the copy of
+// the next page overlaps the kernels of this one, and a buffer is refilled only after
+// the kernels that read it).  sorted_keys: element id of every item in launch order,
+// ascending (pages then cover disjoint item ranges and untouched pages are not copied);
+// null: every page covers all items and the kernels skip the items outside it.
 template <class Launch>
 int run_paged(State &G, DbDev &D, int n, const int *sorted_keys, Launch &&launch) {
     if (n <= 0) return MCK_OK;
@@ -917,17 +927,26 @@ int run_paged(State &G, DbDev &D, int n, const int *sorted_keys, Launch &&launch
                                 hipMemcpyDeviceToHost, G.stream));
         HIPCHECK(hipStreamSynchronize(G.stream));
     }
-    int used = 0;
+    const int nfixed = D.nslot - 2;
     for (int b = 0; b < D.npages; ++b) {
         const int i0 = sorted_keys ? bounds[b] : 0, i1 = sorted_keys ? bounds[b + 1] : n;
         if (i1 <= i0) continue;
-        const int k = used++ & 1;
+        int k = b < nfixed ? b : -1;
+        if (k < 0) {
+            // a rotating buffer: the one that already holds the page, else the next one
+            for (int j = nfixed; j < D.nslot; ++j) if (D.slot_page[j] == b) k = j;
+            if (k < 0) { k = nfixed + D.rot; D.rot ^= 1; }
+        }
         const int e0 = b * D.page_elems, e1 = std::min(D.d.nelem, e0 + D.page_elems);
-        HIPCHECK(hipStreamWaitEvent(G.copy_stream, D.freed[k], 0));
-        HIPCHECK(hipMemcpyAsync(D.slot[k], D.host_u + (size_t)e0 * D.elem_floats,
-                                (size_t)(e1 - e0) * D.elem_floats * sizeof(float), hipMemcpyHostToDevice,
-                                G.copy_stream));
-        HIPCHECK(hipEventRecord(D.copied[k], G.copy_stream));
+        if (D.slot_page[k] != b) {
+            HIPCHECK(hipStreamWaitEvent(G.copy_stream, D.freed[k], 0));
+            HIPCHECK(hipMemcpyAsync(D.slot[k], D.host_u + (size_t)e0 * D.elem_floats,
+                                    (size_t)(e1 - e0) * D.elem_floats * sizeof(float), hipMemcpyHostToDevice,
+                                    G.copy_stream));
+            HIPCHECK(hipEventRecord(D.copied[k], G.copy_stream));
+            D.slot_page[k] = b;
+            G.stats.paged_bytes += (int64_t)((size_t)(e1 - e0) * D.elem_floats * sizeof(float));
+        }
         HIPCHECK(hipStreamWaitEvent(G.stream, D.copied[k], 0));
         DevDb d = D.d;
         d.u = D.slot[k];
@@ -935,7 +954,6 @@ int run_paged(State &G, DbDev &D, int n, const int *sorted_keys, Launch &&launch
         d.u_last = e1;
         HIPCHECK(launch(d, i0, i1));
         HIPCHECK(hipEventRecord(D.freed[k], G.stream));
-        G.stats.paged_bytes += (int64_t)((size_t)(e1 - e0) * D.elem_floats * sizeof(float));
     }
     return MCK_OK;
 }
@@ -1112,7 +1130,8 @@ int init_impl(const mck_setup_t *s) {
         pc.taper[j] = expf(-(q * q));
     }
     int rc;
-    if (S.db_page_elems < 0) return fail(MCK_EINVAL, "db_page_elems must be >= 0");
+    if (S.db_page_elems < 0 || S.db_device_pages < 0)
+        return fail(MCK_EINVAL, "db_page_elems and db_device_pages must be >= 0");
     if (!G.copy_stream) HIPCHECK(hipStreamCreateWithFlags(&G.copy_stream, hipStreamNonBlocking));
     {
         // paging: as asked, or by itself when the two databases' snapshots exceed 3/4 of
@@ -1121,13 +1140,18 @@ int init_impl(const mck_setup_t *s) {
         HIPCHECK(hipMemGetInfo(&free_b, &total_b));
         const size_t ef = (size_t)S.fwd.nvar * 25 * S.nt * 4, eb = (size_t)S.bwd.nvar * 25 * S.nt * 4;
         const size_t need = (size_t)std::max(S.fwd.nelem, 0) * ef + (size_t)std::max(S.bwd.nelem, 0) * eb;
-        int pf = S.db_page_elems, pb = S.db_page_elems;
+        int pf = S.db_page_elems, pb = S.db_page_elems, nsf = S.db_device_pages, nsb = S.db_device_pages;
         if (pf == 0 && need > free_b / 4 * 3) {
+            // automatic paging: pages of 1/64 of the free memory, buffers filling half of it
+            // in proportion to the two databases' sizes
             pf = (int)std::max<size_t>(1, std::min<size_t>(free_b / 64 / ef, INT32_MAX));
             pb = (int)std::max<size_t>(1, std::min<size_t>(free_b / 64 / eb, INT32_MAX));
+            const double share = (double)S.fwd.nelem * ef / (double)need;
+            nsf = (int)std::max(2.0, share * 32);
+            nsb = (int)std::max(2.0, (1.0 - share) * 32);
         }
-        if ((rc = upload_db(G.fwd, S.fwd, S.nt, "fwd", pf, G.stream))) return rc;
-        if ((rc = upload_db(G.bwd, S.bwd, S.nt, "bwd", pb, G.stream))) return rc;
+        if ((rc = upload_db(G.fwd, S.fwd, S.nt, "fwd", pf, nsf, G.stream))) return rc;
+        if ((rc = upload_db(G.bwd, S.bwd, S.nt, "bwd", pb, nsb, G.stream))) return rc;
     }
     if ((rc = setup_tables(G))) return rc;
     for (DbDev *D : {&G.fwd, &G.bwd}) {

@@ -113,6 +113,13 @@ int synth_gpu_db(int nelem, const int *sem_host, int nvar, int nt, double dt, ui
 
 int synth_gpu_free(float *d) { return hipFree(d) == hipSuccess ? 0 : 1; }
 
+// page-locked host memory for a database that the library pages from the host
+int synth_gpu_host_alloc(size_t bytes, float **p) {
+    return hipHostMalloc((void **)p, bytes, hipHostMallocDefault) == hipSuccess ? 0 : 1;
+}
+
+int synth_gpu_host_free(float *p) { return hipHostFree(p) == hipSuccess ? 0 : 1; }
+
 // copies a device database to host memory (for the CPU baseline)
 int synth_gpu_download(const float *d, float *host, size_t nfloat) {
     return hipMemcpy(host, d, nfloat * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess ? 0 : 1;

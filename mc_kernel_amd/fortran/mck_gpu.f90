@@ -62,7 +62,7 @@ module mck_gpu
      integer(c_int32_t) :: int_over_volume, relative_kernel, quasirandom, int_scheme, deconv_stf
      real(c_double) :: damp_radius
      integer(c_int32_t) :: mask_src_rec
-     integer(c_int32_t) :: db_page_elems
+     integer(c_int32_t) :: db_page_elems, db_device_pages
   end type mck_setup_t
 
   type, bind(c), public :: mck_stats_t

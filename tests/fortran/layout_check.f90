@@ -20,6 +20,7 @@ program layout_check
   write(*, '(a, 1x, i0)') 'mck_setup_t damp_radius', off(c_loc(s%damp_radius), c_loc(s))
   write(*, '(a, 1x, i0)') 'mck_setup_t mask_src_rec', off(c_loc(s%mask_src_rec), c_loc(s))
   write(*, '(a, 1x, i0)') 'mck_setup_t db_page_elems', off(c_loc(s%db_page_elems), c_loc(s))
+  write(*, '(a, 1x, i0)') 'mck_setup_t db_device_pages', off(c_loc(s%db_device_pages), c_loc(s))
   write(*, '(a, 1x, i0)') 'mck_stats_t size', c_sizeof(st)
   write(*, '(a, 1x, i0)') 'mck_stats_t device_ms', off(c_loc(st%device_ms), c_loc(st))
   write(*, '(a, 1x, i0)') 'mck_stats_t paged_bytes', off(c_loc(st%paged_bytes), c_loc(st))

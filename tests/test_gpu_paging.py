@@ -1,6 +1,7 @@
 """Databases paged from host memory (mck_setup_t.db_page_elems > 0: the snapshots stay in
-host memory and every launch copies the element pages its points touch through two
-device buffers per database) give the same results as resident databases, bit for bit:
+host memory and every launch copies the element pages its points touch into device page
+buffers: db_device_pages - 2 pages keep a buffer of their own, the others rotate through
+two) give the same results as resident databases, bit for bit:
 
   * slave_work (element-sorted launches: each page covers one range of the sorted points
     and pairs, untouched pages are not copied), for a monopole (explosion, Z) and a
@@ -18,13 +19,14 @@ pytestmark = pytest.mark.gpu
 
 DC = [0.3e20, 1.0e20, -1.3e20, 0.5e20, -0.4e20, 0.7e20]
 CASES = {"explosion_Z": {}, "general_mij_R": dict(mij=DC, component="R")}
-PAGE = 97          # elements per page: about 20 pages of the tiny databases
+PAGE = 37          # elements per page: 13 pages of the tiny databases (462 elements)
 
 
-def _with_kernel(page, kw, fn):
+def _with_kernel(page, kw, fn, device_pages=0):
     import mc_kernel_amd as mk
     p = synth.make_problem("tiny", ntet=12, **kw)
     p.setup.db_page_elems = page
+    p.setup.db_device_pages = device_pages
     k = mk.MCKernel(p.setup)
     try:
         return p, fn(p, k)
@@ -39,13 +41,14 @@ def _task(p, k):
     return res, k.base_kernels(x), traces
 
 
+@pytest.mark.parametrize("device_pages", [2, 7])
 @pytest.mark.parametrize("case", sorted(CASES))
-def test_paged_equals_resident(case):
-    print("paging case", case, flush=True)
+def test_paged_equals_resident(case, device_pages):
+    print("paging case", case, device_pages, flush=True)
     p, (ref, base_ref, tr_ref) = _with_kernel(0, CASES[case], _task)
     nelem = p.setup.fwd.nelem
     assert nelem > 4 * PAGE
-    _, (got, base_got, tr_got) = _with_kernel(PAGE, CASES[case], _task)
+    _, (got, base_got, tr_got) = _with_kernel(PAGE, CASES[case], _task, device_pages)
     assert np.abs(ref["values"]).max() > 0
     for key in ("values", "variance", "niterations"):
         np.testing.assert_array_equal(got[key], ref[key], err_msg=key)
@@ -67,3 +70,25 @@ def test_page_larger_than_database_is_resident():
     _, got = _with_kernel(10 ** 9, {}, lambda p, k: k.slave_work(p.verts, p.conn))
     np.testing.assert_array_equal(got["values"], ref["values"])
     assert got["stats"]["paged_bytes"] == 0
+
+
+def test_fixed_page_buffers_are_copied_once():
+    """Unsorted launches (base kernels) visit every page in order.  With 2 buffers every
+    call copies every page of both databases; with 7, pages 0-4 keep their buffers and
+    pages 5-12 rotate through the other two, so a second call copies exactly 5 pages per
+    database less."""
+    def two_calls(p, k):
+        x = p.sample_points(40, seed=9)
+        k.base_kernels(x)
+        s1 = k.stats()["paged_bytes"]
+        b = k.base_kernels(x)
+        return b, k.stats()["paged_bytes"] - s1
+    p, (b2, d2) = _with_kernel(PAGE, {}, two_calls, 2)
+    _, (b7, d7) = _with_kernel(PAGE, {}, two_calls, 7)
+    np.testing.assert_array_equal(b7, b2)
+    f, b = p.setup.fwd, p.setup.bwd
+    assert f.nelem > 12 * PAGE and b.nelem > 12 * PAGE
+    elem_bytes = lambda db: db.nvar * 25 * p.setup.nt * 4
+    full = f.nelem * elem_bytes(f) + b.nelem * elem_bytes(b)
+    assert d2 == full
+    assert d7 == full - 5 * PAGE * (elem_bytes(f) + elem_bytes(b))
