@@ -97,6 +97,28 @@ def build_ref(force=True):
     return out
 
 
+HDF5_PREFIX = os.environ.get("MCK_HDF5_PREFIX", "/opt/conda")
+
+
+def build_dbio(force=True):
+    """libmck_dbio.so: merged database files through HDF5 (host code, g++); skipped when
+    the HDF5 headers and library are absent."""
+    src = os.path.join(CSRC, "mck_dbio.cpp")
+    out = os.path.join(HERE, "libmck_dbio.so")
+    inc, lib = os.path.join(HDF5_PREFIX, "include"), os.path.join(HDF5_PREFIX, "lib")
+    if not (os.path.exists(os.path.join(inc, "hdf5.h")) and os.path.exists(os.path.join(lib, "libhdf5.so"))):
+        print("libmck_dbio.so skipped: no HDF5 under %s" % HDF5_PREFIX, file=sys.stderr)
+        return None
+    hdrs = [os.path.join(ROOT, "include", "mck_dbio.h"), os.path.join(ROOT, "include", "mck.h")]
+    if force or _stale(out, [src] + hdrs):
+        # libhdf5 by path with an rpath for it; libstdc++ linked statically so that the
+        # older C++ runtime beside libhdf5 is never picked up in its place
+        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", os.path.join(ROOT, "include"),
+              "-I", inc, src, "-o", out, os.path.join(lib, "libhdf5.so"), "-Wl,-rpath," + lib,
+              "-static-libstdc++", "-static-libgcc", "-Wl,--no-undefined"])
+    return out
+
+
 def build_all(force=True):
     build_synth(force)
     build_oracle(force)
@@ -106,6 +128,7 @@ def build_all(force=True):
         print("oracle/_ref build failed:", e, file=sys.stderr)
     build_hip(force)
     build_synth_gpu(force)
+    build_dbio(force)
 
 
 if __name__ == "__main__":

@@ -8,7 +8,9 @@ two) give the same results as resident databases, bit for bit:
     general moment tensor with an R receiver (all four forward simulations);
   * base kernels and strain traces (unsorted launches: every page covers all points and
     the kernels skip the points outside it);
-  * a page larger than the database falls back to the resident copy.
+  * a page larger than the database falls back to the resident copy;
+  * databases opened from merged database files (include/mck_dbio.h), snapshots
+    memory-mapped, resident or paged from the mapping.
 """
 import numpy as np
 import pytest
@@ -92,3 +94,39 @@ def test_fixed_page_buffers_are_copied_once():
     full = f.nelem * elem_bytes(f) + b.nelem * elem_bytes(b)
     assert d2 == full
     assert d7 == full - 5 * PAGE * (elem_bytes(f) + elem_bytes(b))
+
+
+@pytest.mark.parametrize("page", [0, PAGE])
+def test_slave_work_from_database_files(tmp_path, page):
+    """Databases written as merged database files and opened with mck_db_open (snapshots
+    memory-mapped from the file) give the same slave_work as the in-memory databases,
+    resident and paged from the mapping.  The file stores the source magnitude as a
+    32-bit real (as the reference reads it), so the in-memory set-up uses that value."""
+    import mc_kernel_amd as mk
+    from mc_kernel_amd import dbfile
+    p = synth.make_problem("tiny", ntet=12)
+    amp = float(np.float32(1.0e20))
+    p.setup.fwd.amplitude = amp
+    p.setup.bwd.amplitude = amp
+    k = mk.MCKernel(p.setup)
+    try:
+        ref = k.slave_work(p.verts, p.conn)
+    finally:
+        k.close()
+    paths = {w: str(tmp_path / ("%s.nc4" % w)) for w in ("fwd", "bwd")}
+    for w, path in paths.items():
+        dbfile.write_problem_db(path, p, w)
+    with dbfile.DbFile(paths["fwd"], 10) as ff, dbfile.DbFile(paths["bwd"], 5) as fb:
+        assert ff.mapped and fb.mapped
+        p.setup.fwd = ff.db
+        p.setup.bwd = fb.db
+        p.setup.db_page_elems = page
+        k = mk.MCKernel(p.setup)
+        try:
+            got = k.slave_work(p.verts, p.conn)
+        finally:
+            k.close()
+    assert np.abs(ref["values"]).max() > 0
+    for key in ("values", "variance", "niterations"):
+        np.testing.assert_array_equal(got[key], ref[key], err_msg=key)
+    assert (got["stats"]["paged_bytes"] > 0) == (page > 0)
