@@ -123,7 +123,9 @@ typedef struct {
     const double *rec_lon;     /* [nrec] degrees                                         */
     char    component;         /* 'Z', 'R' or 'T'                                        */
     const double *seis;        /* [nrec][nt] velocity seismogram of each receiver        */
-                               /* (sem_data%seis, readfields.f90:1004-1048)              */
+                               /* (sem_data%seis, readfields.f90:1004-1048); NULL: mck_init */
+                               /* computes them by reciprocity from the bwd database     */
+                               /* (load_fw_points_rdbm, readfields.f90:1469-1604)         */
 
     /* filters and kernels (type_parameter.f90:521-691)                                  */
     int32_t nfilter;
@@ -268,6 +270,15 @@ int mck_load_strain_points(int32_t frame, int32_t npts, const double *xyz, float
  *             kernel's model parameter does not need are 0
  */
 int mck_base_kernels(int32_t npts, const double *xyz, double *base);
+
+/*
+ * The seismograms the kernels' windows use, seis[nrec][nt]: a copy of mck_setup_t.seis,
+ * or, when that was NULL, the ones mck_init computed by reciprocity from the backward
+ * database (sem_data%load_seismogram, readfields.f90:1004-1048, 1469-1604: the source
+ * point located in each receiver frame, the moment tensor rotated into that frame and
+ * contracted with the backward strain there).
+ */
+int mck_seismograms(double *seis);
 
 /*
  * The filter-convolve-window-integrate stage of the receiver loop (slave.f90:515-534,

@@ -129,6 +129,12 @@ class MCKernel:
                                                abi.ptr(out, C.c_float)))
         return out
 
+    def seismograms(self):
+        """[nrec][nt] seismograms the kernels use (the set-up's, or by reciprocity)."""
+        out = np.zeros((self.R, self.nt))
+        _check(self.lib.mck_seismograms(abi.ptr(out, C.c_double)))
+        return out
+
     def base_kernels(self, xyz):
         xyz = np.ascontiguousarray(xyz, np.float64)
         out = np.zeros((xyz.shape[0], self.nk, 6))

@@ -378,6 +378,7 @@ struct State {
     // id) and of the pairs (by position in the pair order)
     DBuf<float> opWf, opQf, opWb, opQb;
     DBuf<int> page_bounds;              // [npages + 1] first item of every database page
+    std::vector<double> seis_host;      // [nrec][nt] seismograms by reciprocity (seis NULL)
     hipStream_t copy_stream = nullptr;  // page copies of paged databases
     DBuf<int> keys_in, keys_out, vals_in, vals_out, pair_count;
     DBuf<unsigned char> sort_tmp;
@@ -958,6 +959,116 @@ int run_paged(State &G, DbDev &D, int n, const int *sorted_keys, Launch &&launch
     return MCK_OK;
 }
 
+// Voigt rotations of the moment tensor for the reciprocal seismograms, host FP64, the
+// same expressions as the reference:
+// rotate_symm_tensor_voigt_xyz_src_to_xyz_earth_1d (rotations.f90:555-612)
+void h_rot_xyz_src_to_earth(double *t, double phi, double theta) {
+    const double sp = std::sin(phi), cp = std::cos(phi), st = std::sin(theta), ct = std::cos(theta);
+    const double a1 = t[0], a2 = t[1], a3 = t[2], a4 = t[3], a5 = t[4], a6 = t[5];
+    t[0] = a1 * (cp * cp) * (ct * ct) - 2 * a6 * cp * ct * sp + a2 * (sp * sp)
+         + 2 * a5 * (cp * cp) * ct * st - 2 * a4 * cp * sp * st + a3 * (cp * cp) * (st * st);
+    t[1] = a2 * (cp * cp) + sp * (2 * a6 * cp * ct + a1 * (ct * ct) * sp
+                                  + st * (2 * a4 * cp + 2 * a5 * ct * sp + a3 * sp * st));
+    t[2] = a3 * (ct * ct) + st * (-2 * a5 * ct + a1 * st);
+    t[3] = ct * (a4 * cp + a5 * ct * sp + a3 * sp * st) - st * (a6 * cp + a1 * ct * sp + a5 * sp * st);
+    t[4] = ct * (a5 * cp * ct - a4 * sp + a3 * cp * st) - st * (a1 * cp * ct - a6 * sp + a5 * cp * st);
+    t[5] = cp * st * (a4 * cp + a5 * ct * sp + a3 * sp * st) - sp * (a2 * cp + a6 * ct * sp + a4 * sp * st)
+         + cp * ct * (a6 * cp + a1 * ct * sp + a5 * sp * st);
+}
+
+// rotate_symm_tensor_voigt_xyz_earth_to_xyz_src_1d (rotations.f90:615-671)
+void h_rot_earth_to_xyz_src(double *t, double phi, double theta) {
+    const double sp = std::sin(phi), cp = std::cos(phi), st = std::sin(theta), ct = std::cos(theta);
+    const double a1 = t[0], a2 = t[1], a3 = t[2], a4 = t[3], a5 = t[4], a6 = t[5];
+    t[0] = a1 * (cp * cp) * (ct * ct) + 2 * a6 * cp * (ct * ct) * sp + a2 * (ct * ct) * (sp * sp)
+         - 2 * a5 * cp * ct * st - 2 * a4 * ct * sp * st + a3 * (st * st);
+    t[1] = a2 * (cp * cp) + sp * (-2 * a6 * cp + a1 * sp);
+    t[2] = a3 * (ct * ct) + st * (2 * a5 * cp * ct + 2 * a4 * ct * sp + a1 * (cp * cp) * st
+                                  + 2 * a6 * cp * sp * st + a2 * (sp * sp) * st);
+    t[3] = cp * (a4 * ct + a6 * cp * st + a2 * sp * st) - sp * (a5 * ct + a1 * cp * st + a6 * sp * st);
+    t[4] = ct * sp * (a4 * ct + a6 * cp * st + a2 * sp * st) - st * (a3 * ct + a5 * cp * st + a4 * sp * st)
+         + cp * ct * (a5 * ct + a1 * cp * st + a6 * sp * st);
+    t[5] = cp * (a6 * cp * ct + a2 * ct * sp - a4 * st) - sp * (a1 * cp * ct + a6 * ct * sp - a5 * st);
+}
+
+// rotate_symm_tensor_voigt_xyz_to_src_1d (rotations.f90:423-455)
+void h_rot_xyz_to_src(double *t, double phi) {
+    const double sp = std::sin(phi), cp = std::cos(phi);
+    const double a1 = t[0], a2 = t[1], a3 = t[2], a4 = t[3], a5 = t[4], a6 = t[5];
+    t[0] = a1 * (cp * cp) + sp * (2 * a6 * cp + a2 * sp);
+    t[1] = a2 * (cp * cp) + sp * (-2 * a6 * cp + a1 * sp);
+    t[2] = a3;
+    t[3] = a4 * cp - a5 * sp;
+    t[4] = a5 * cp + a4 * sp;
+    t[5] = cp * (a6 * cp + a2 * sp) - sp * (a1 * cp + a6 * sp);
+}
+
+// load_seismogram (readfields.f90:1004-1048) / load_fw_points_rdbm (readfields.f90:
+// 1469-1604): the seismogram of every receiver by reciprocity into G.seis_host.  The
+// source point is located in each receiver frame in the fwd mesh (k_locate, as
+// find_element on fwdmesh); the moment tensor is rotated source xyz -> earth ->
+// receiver xyz -> receiver (s, phi, z) and divided by the bwd amplitude on the host;
+// k_seis contracts it with the bwd strain at that element (FP64).
+int reciprocity_seismograms(State &G) {
+    const mck_setup_t &S = G.S;
+    const int R = G.R, nt = S.nt;
+    if (S.fwd.nelem != S.bwd.nelem)
+        return fail(MCK_EINVAL, "seismograms by reciprocity need the same mesh in both databases");
+    double *d_xyz = nullptr, *d_mb = nullptr, *d_seis = nullptr;
+    Located *d_loc = nullptr;
+    std::vector<Located> loc(R);
+    std::vector<double> mb((size_t)R * 6);
+    G.seis_host.assign((size_t)R * nt, 0.0);
+    int rc = MCK_OK;
+    hipError_t e = hipMalloc((void **)&d_xyz, 3 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&d_loc, (size_t)R * sizeof(Located));
+    if (e == hipSuccess) e = hipMalloc((void **)&d_mb, (size_t)R * 6 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&d_seis, (size_t)R * nt * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpy(d_xyz, G.src_r, 3 * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemsetAsync(G.err.p, 0, 4, G.stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_locate, dim3((R + 127) / 128), dim3(128), 0, G.stream, G.fwd.d, G.frames.p, 1, R, 1,
+                           d_xyz, S.planet_radius, nullptr, 1, R, d_loc, G.err.p);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(loc.data(), d_loc, (size_t)R * sizeof(Located), hipMemcpyDeviceToHost, G.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(G.stream);
+    int err = 0;
+    if (e == hipSuccess) e = hipMemcpy(&err, G.err.p, 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && (err & 1)) rc = fail(MCK_ENOTFOUND, "source point not found in the mesh for the reciprocal seismograms");
+    if (e == hipSuccess && rc == MCK_OK) {
+        const double *m = S.src_mij;
+        const double slon = S.src_lon * kDeg2Rad, scolat = (90 - S.src_lat) * kDeg2Rad;
+        for (int r = 0; r < R; ++r) {
+            // Voigt (tt, pp, rr, rp, rt, tp) of the CMT order (rr, tt, pp, rt, rp, tp), source.f90:90-95
+            double *t = &mb[(size_t)r * 6];
+            t[0] = m[1]; t[1] = m[2]; t[2] = m[0]; t[3] = m[4]; t[4] = m[3]; t[5] = m[5];
+            h_rot_xyz_src_to_earth(t, slon, scolat);
+            h_rot_earth_to_xyz_src(t, S.rec_lon[r] * kDeg2Rad, (90 - S.rec_lat[r]) * kDeg2Rad);
+            h_rot_xyz_to_src(t, loc[r].phi);
+            for (int i = 0; i < 6; ++i) t[i] = t[i] / S.bwd.amplitude;
+        }
+        e = hipMemcpy(d_mb, mb.data(), mb.size() * sizeof(double), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemsetAsync(d_seis, 0, (size_t)R * nt * sizeof(double), G.stream);
+    }
+    if (e == hipSuccess && rc == MCK_OK) {
+        PathConf pcs = G.pc;
+        pcs.full = 1;
+        const int comp = S.component == 'Z' ? 0 : (S.component == 'R' ? 1 : 2);
+        rc = run_paged(G, G.bwd, R, nullptr, [&](const DevDb &d, int, int) {
+            hipLaunchKernelGGL(k_seis, dim3(R), dim3(kStrainThreads), 0, G.stream, d, pcs, d_loc, d_mb, comp, d_seis);
+            return hipGetLastError();
+        });
+    }
+    if (e == hipSuccess && rc == MCK_OK)
+        e = hipMemcpyAsync(G.seis_host.data(), d_seis, (size_t)R * nt * sizeof(double), hipMemcpyDeviceToHost, G.stream);
+    if (e == hipSuccess && rc == MCK_OK) e = hipStreamSynchronize(G.stream);
+    (void)hipFree(d_xyz); (void)hipFree(d_loc); (void)hipFree(d_mb); (void)hipFree(d_seis);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(MCK_EHIP, std::string("reciprocal seismograms: ") + hipGetErrorString(e));
+    return MCK_OK;
+}
+
 // stable sort of n (element, id) pairs from keys_in/vals_in into keys_out/vals_out
 int sort_by_element(State &G, int n, int nelem) {
     if (n <= 1) {
@@ -1163,6 +1274,14 @@ int init_impl(const mck_setup_t *s) {
         D->d.ejac = D->ejac.p;
     }
     HIPCHECK(hipStreamSynchronize(G.stream));
+    // seismograms: a copy of the caller's, or by reciprocity from the bwd database
+    if (S.seis) {
+        G.seis_host.assign(S.seis, S.seis + (size_t)G.R * S.nt);
+    } else {
+        HIPCHECK(G.err.ensure(1));
+        if ((rc = reciprocity_seismograms(G))) return rc;
+    }
+    G.S.seis = G.seis_host.data();
     if ((rc = build_kernels(G))) return rc;
     HIPCHECK(G.evals.ensure(1));
     HIPCHECK(G.err.ensure(1));
@@ -1952,6 +2071,14 @@ int mck_ad_backward(int32_t npts, const float *tf, const float *fw, const float 
     }
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return fail(MCK_EHIP, std::string("mck_ad_backward: ") + hipGetErrorString(e));
+    return MCK_OK;
+}
+
+int mck_seismograms(double *seis) {
+    State &G = g;
+    if (!G.init) return fail(MCK_ENOTINIT, "mck_init has not been called");
+    if (!seis) return fail(MCK_EINVAL, "null seis");
+    std::memcpy(seis, G.seis_host.data(), G.seis_host.size() * sizeof(double));
     return MCK_OK;
 }
 

@@ -523,9 +523,11 @@ __device__ void op_setup_parts(OpShared &S, const DevDb &db, const PathConf &pc,
     }
 }
 
-// W[(c * nvar + v) * 25 + node] from the tables of op_setup_parts (W: LDS or global)
+// W[(c * nvar + v) * 25 + node] from the tables of op_setup_parts (W: LDS or global,
+// f32 for the spectra kernels, f64 for the seismograms)
+template <class T>
 __device__ void op_weights(const OpShared &S, const PathConf &pc, bool fwd, int nvar, const int *vars,
-                           int nd, int tid, int nthr, float *W)
+                           int nd, int tid, int nthr, T *W)
 {
     const bool axial = S.axial != 0;
     const int total = nd * nvar * 25;
@@ -581,7 +583,7 @@ __device__ void op_weights(const OpShared &S, const PathConf &pc, bool fwd, int 
             w += peta * c_basis.G2[l + 5 * j];
         }
         if (!(axial && k == 0)) w += af * S.li[k] * S.lj[l] * S.sinv[k + 5 * l];
-        W[(c * nvar + v) * 25 + n] = (float)w;
+        W[(c * nvar + v) * 25 + n] = (T)w;
     }
 }
 
@@ -593,6 +595,70 @@ __device__ void build_weights(OpShared &S, const DevDb &db, const PathConf &pc, 
     __syncthreads();
     op_weights(S, pc, fwd, nvar, vars, nd, threadIdx.x, blockDim.x, S.W);
     __syncthreads();
+}
+
+// ------------------------------------------------------------------------------------
+// k_seis: the seismogram of receiver blockIdx.x by reciprocity (load_fw_points_rdbm,
+// readfields.f90:1469-1604): the full strain of simulation bwd_isim at the source point
+// located in the receiver's frame (comps 4 and 6 negated as load_strain_point_merged_bwd
+// does, readfields.f90:2390-2393; no azimuthal factor, no amplitude), from FP64 weights
+// and FP64 sums, contracted with the rotated moment tensor mb[r][6] (already divided by
+// the amplitude) in the reference's order of operations (readfields.f90:1562-1597).
+// comp: 0 Z, 1 R, 2 T; pc.full must be 1 (full strain tables).  Receivers whose element
+// lies outside db's page are skipped.
+// ------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kStrainThreads) void k_seis(DevDb db, PathConf pc, const Located *loc,
+                                                       const double *mb, int comp, double *seis)
+{
+    __shared__ OpShared S;
+    __shared__ double Wd[6 * 3 * 25];
+    const int r = blockIdx.x, tid = threadIdx.x;
+    const Located L = loc[r];
+    if (L.elem < db.u_first || L.elem >= db.u_last) return;
+    DevFrame none;
+    none.cos_lon = 1.0; none.sin_lon = 0.0; none.cos_colat = 1.0; none.sin_colat = 0.0;
+    // element tables and Lagrange bases only (no output-transform roles)
+    op_setup_parts(S, db, pc, none, L, false, 6, nullptr, false, tid, kStrainThreads, 160, -64, -64);
+    if (tid < 36) {
+        const int c = tid / 6, m = tid - (tid / 6) * 6;
+        S.M[c][0][m] = c == m ? ((m == 3 || m == 5) ? -1.0 : 1.0) : 0.0;
+    }
+    __syncthreads();
+    const int nvar = pc.nvar_bwd;
+    op_weights(S, pc, false, nvar, pc.bwd_vars, 6, tid, kStrainThreads, Wd);
+    __syncthreads();
+    const int nt = pc.nt;
+    const size_t blk = (size_t)25 * nt;
+    const float *ue = db.u + (size_t)(L.elem - db.u_first) * db.nvar * blk;
+    const double *m = mb + 6 * r;
+    for (int t = tid; t < nt; t += kStrainThreads) {
+        double e[6] = {0, 0, 0, 0, 0, 0};
+        for (int v = 0; v < nvar; ++v) {
+            const float *uv = ue + (size_t)pc.bwd_vars[v] * blk + t;
+            for (int n = 0; n < 25; ++n) {
+                const double x = (double)uv[(size_t)n * nt];
+#pragma unroll
+                for (int c = 0; c < 6; ++c) e[c] += Wd[(c * nvar + v) * 25 + n] * x;
+            }
+        }
+        double a = 0.0;
+        if (comp == 0) {
+            a = a + m[0] * e[0];
+            a = a + m[1] * e[1];
+            a = a + m[2] * e[2];
+            a = a + 2 * m[4] * e[4];
+        } else if (comp == 1) {
+            a = a + m[0] * e[0];
+            a = a + m[1] * e[1];
+            a = a + m[2] * e[2];
+            a = a + m[4] * e[4] * 2;
+            a = -a;
+        } else {
+            a = a + m[3] * e[3] * 2;
+            a = a + m[5] * e[5] * 2;
+        }
+        seis[(size_t)r * nt + t] = a;
+    }
 }
 
 // ------------------------------------------------------------------------------------

@@ -73,7 +73,7 @@ module mck_gpu
   end type mck_stats_t
 
   public :: mck_init, mck_slave_work, mck_slave_work_plain, mck_locate_points, &
-            mck_load_strain_points, mck_base_kernels, mck_get_stats, mck_finalize, &
+            mck_load_strain_points, mck_base_kernels, mck_seismograms, mck_get_stats, mck_finalize, &
             mck_last_error, mck_error_message, mck_check
 
   interface
@@ -134,6 +134,13 @@ module mck_gpu
        real(c_double), intent(in) :: xyz(*)
        real(c_double), intent(out) :: base(*)
      end function mck_base_kernels
+
+     ! seis(nt, nrec): the seismograms the kernels use (mck_setup_t%seis or, when that was
+     ! c_null_ptr, the ones computed by reciprocity at mck_init)
+     integer(c_int) function mck_seismograms(seis) bind(c, name='mck_seismograms')
+       import :: c_int, c_double
+       real(c_double), intent(out) :: seis(*)
+     end function mck_seismograms
 
      integer(c_int) function mck_get_stats(stats) bind(c, name='mck_get_stats')
        import :: c_int, mck_stats_t

@@ -107,6 +107,13 @@ int      ora_locate(ora_ctx *ctx, int frame, int npts, const double *xyz, int32_
 int      ora_load_strain(ora_ctx *ctx, int frame, int npts, const double *xyz,
                          double *traces);      /* [npts][nd][nt] */
 int      ora_base_kernels(ora_ctx *ctx, int npts, const double *xyz, double *base);
+/* Seismogram of every receiver by reciprocity from the bwd database, seis[nrec][nt]
+ * (load_seismogram / load_fw_points_rdbm, readfields.f90:1004-1048, 1469-1604); used by
+ * ora_create when the set-up's seis is NULL. */
+int      ora_seismograms(ora_ctx *ctx, double *seis);
+/* Test hook: which 2 = xyz_earth -> xyz_src (rotations.f90:615-671), 3 = xyz -> src
+ * (rotations.f90:423-455); t[6] Voigt, in place. */
+void     ora_rotate_voigt_inv(int which, double *t, double phi, double theta);
 int      ora_slave_work(ora_ctx *ctx, int elem_type, int nvpe, int nelems, int nbasis,
                         const double *vertices,
                         const int32_t *conn, double *values, double *variance,

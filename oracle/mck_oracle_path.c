@@ -58,6 +58,7 @@ struct ora_ctx {
     ora_kernel *K;
     ora_rec *R;
     double src_lat, src_lon, src_colat, src_r[3];
+    double *seis_own;          /* [nrec][nt] seismograms by reciprocity when S.seis is NULL */
     int64_t halton_step;
     ora_fill_fn fill;
     void *fill_user;
@@ -335,6 +336,16 @@ ora_ctx *ora_create(const mck_setup_t *S, char *err, int errlen)
     C->ts_bwd = (double *)malloc(2 * C->nomega * sizeof(double));
     make_timeshift(C->nomega, C->df, S->timeshift_fwd, C->ts_fwd);
     make_timeshift(C->nomega, C->df, S->timeshift_bwd, C->ts_bwd);
+    /* seismograms: the caller's, or by reciprocity from the bwd database (load_seismogram,
+     * slave.f90:94-173 calls it before the kernels are set up) */
+    if (!S->seis) {
+        C->seis_own = (double *)malloc((size_t)S->nrec * S->nt * sizeof(double));
+        if (ora_seismograms(C, C->seis_own)) {
+            snprintf(err, errlen, "source point not found in the mesh for the reciprocal seismograms");
+            ora_destroy(C); return NULL;
+        }
+        C->S.seis = C->seis_own;
+    }
     /* kernels (type_parameter.f90:521-691) */
     C->K = (ora_kernel *)calloc(S->nkernel, sizeof(ora_kernel));
     for (k = 0; k < S->nkernel; ++k) {
@@ -371,6 +382,7 @@ void ora_destroy(ora_ctx *C)
     int k;
     if (!C) return;
     if (C->K) for (k = 0; k < C->S.nkernel; ++k) { free(C->K[k].seis_cut); free(C->K[k].seis_cut_fd); }
+    free(C->seis_own);
     free(C->K); free(C->R); free(C->tf_fwd); free(C->tf_bwd); free(C->ts_fwd); free(C->ts_bwd);
     free(C->tf_g); free(C->rest_f); free(C->rest_b);
     free(C->filled[0]); free(C->filled[1]);
@@ -637,6 +649,111 @@ static int load_points(ora_ctx *C, int frame, int npts, const double *xyz, doubl
 int ora_load_strain(ora_ctx *C, int frame, int npts, const double *xyz, double *traces)
 {
     return load_points(C, frame, npts, xyz, traces);
+}
+
+/* rotate_symm_tensor_voigt_xyz_earth_to_xyz_src_1d (rotations.f90:615-671) */
+static void rot_earth_to_xyz_src(double *t, double phi, double theta)
+{
+    double sp = sin(phi), cp = cos(phi), st = sin(theta), ct = cos(theta);
+    double a1 = t[0], a2 = t[1], a3 = t[2], a4 = t[3], a5 = t[4], a6 = t[5];
+    t[0] = a1 * (cp * cp) * (ct * ct) + 2 * a6 * cp * (ct * ct) * sp + a2 * (ct * ct) * (sp * sp)
+         - 2 * a5 * cp * ct * st - 2 * a4 * ct * sp * st + a3 * (st * st);
+    t[1] = a2 * (cp * cp) + sp * (-2 * a6 * cp + a1 * sp);
+    t[2] = a3 * (ct * ct) + st * (2 * a5 * cp * ct + 2 * a4 * ct * sp + a1 * (cp * cp) * st
+                                  + 2 * a6 * cp * sp * st + a2 * (sp * sp) * st);
+    t[3] = cp * (a4 * ct + a6 * cp * st + a2 * sp * st) - sp * (a5 * ct + a1 * cp * st + a6 * sp * st);
+    t[4] = ct * sp * (a4 * ct + a6 * cp * st + a2 * sp * st) - st * (a3 * ct + a5 * cp * st + a4 * sp * st)
+         + cp * ct * (a5 * ct + a1 * cp * st + a6 * sp * st);
+    t[5] = cp * (a6 * cp * ct + a2 * ct * sp - a4 * st) - sp * (a1 * cp * ct + a6 * ct * sp - a5 * st);
+}
+
+/* rotate_symm_tensor_voigt_xyz_to_src_1d (rotations.f90:423-455) */
+static void rot_xyz_to_src(double *t, double phi)
+{
+    double sp = sin(phi), cp = cos(phi);
+    double a1 = t[0], a2 = t[1], a3 = t[2], a4 = t[3], a5 = t[4], a6 = t[5];
+    t[0] = a1 * (cp * cp) + sp * (2 * a6 * cp + a2 * sp);
+    t[1] = a2 * (cp * cp) + sp * (-2 * a6 * cp + a1 * sp);
+    t[2] = a3;
+    t[3] = a4 * cp - a5 * sp;
+    t[4] = a5 * cp + a4 * sp;
+    t[5] = cp * (a6 * cp + a2 * sp) - sp * (a1 * cp + a6 * sp);
+}
+
+/* Test hook: 2 = xyz_earth -> xyz_src (phi, theta), 3 = xyz -> src (phi); t[6] */
+void ora_rotate_voigt_inv(int which, double *t, double phi, double theta)
+{
+    if (which == 2) rot_earth_to_xyz_src(t, phi, theta);
+    else rot_xyz_to_src(t, phi);
+}
+
+/* The rotated moment tensor of receiver r's reciprocal source, load_fw_points_rdbm
+ * (readfields.f90:1514-1530): source Voigt (tt, pp, rr, rp, rt, tp) (source.f90:90-95)
+ * src xyz -> earth xyz -> receiver xyz -> receiver (s, phi, z), / bwd amplitude. */
+static void seis_moment(ora_ctx *C, int r, double phi, double *mb)
+{
+    const double *m = C->S.src_mij;
+    int i;
+    mb[0] = m[1]; mb[1] = m[2]; mb[2] = m[0]; mb[3] = m[4]; mb[4] = m[3]; mb[5] = m[5];
+    rot_xyz_src_to_earth(mb, 1, C->src_lon, C->src_colat);
+    rot_earth_to_xyz_src(mb, C->R[r].lon, C->R[r].colat);
+    rot_xyz_to_src(mb, phi);
+    for (i = 0; i < 6; ++i) mb[i] = mb[i] / C->S.bwd.amplitude;
+}
+
+/* load_seismogram / load_fw_points_rdbm (readfields.f90:1004-1048, 1469-1604): the
+ * seismogram of every receiver by reciprocity, seis[nrec][nt].  The source point is
+ * located in the receiver frame in the fwd mesh (without the surface clamp) and the full
+ * strain of simulation 1 (Z) or 2 (R, T) of the bwd database is read at that element
+ * (load_strain_point_merged_bwd, readfields.f90:2209-2403, comps 4 and 6 negated), then
+ * contracted with the rotated moment tensor in the reference's order of operations. */
+int ora_seismograms(ora_ctx *C, double *seis)
+{
+    const mck_setup_t *S = &C->S;
+    const mck_db_t *fm = &S->fwd, *bd = &S->bwd;
+    const int nt = S->nt, isim = S->component == 'Z' ? 1 : 2;
+    const int pole = isim == 1 ? 0 : 1, var0 = isim == 1 ? 0 : 2, nv = isim == 1 ? 2 : 3;
+    const size_t blk = 25 * (size_t)nt;
+    double *u = (double *)malloc(3 * blk * sizeof(double));
+    double *st = (double *)malloc(6 * blk * sizeof(double));
+    double *ut = (double *)malloc(6 * (size_t)nt * sizeof(double));
+    int r, k, v, rc = 0;
+    if (fm->nelem != bd->nelem) { rc = MCK_EINVAL; goto done; }
+    for (r = 0; r < S->nrec; ++r) {
+        double s, phi, z, xi, eta, nodes[8], mb[6];
+        int e, tie, i;
+        double *o = seis + (size_t)r * nt;
+        const float *src;
+        ora_rotate_frame(1, C->src_r, C->R[r].lon, C->R[r].colat, &s, &phi, &z);
+        if (find_element(fm, s, z, &xi, &eta, nodes, &e, &tie)) { rc = MCK_ENOTFOUND; goto done; }
+        seis_moment(C, r, phi, mb);
+        src = bd->snapshots + ((size_t)e * bd->nvar + var0) * blk;
+        for (v = 0; v < nv; ++v)
+            for (k = 0; k < (int)blk; ++k) u[v * blk + k] = (double)src[v * blk + k];
+        ora_strain_pole(pole, u, nt, 4, nodes, fm->eltype[e], fm->axis[e], st);
+        interp_nodes(6, nt, st, fm->axis[e] ? C->glj : C->gll, C->gll, xi, eta, ut);
+        for (k = 0; k < nt; ++k) { ut[3 * nt + k] = -ut[3 * nt + k]; ut[5 * nt + k] = -ut[5 * nt + k]; }
+        for (k = 0; k < nt; ++k) {
+            double a = 0.0;
+            if (S->component == 'Z') {
+                for (i = 0; i < 3; ++i) a = a + mb[i] * ut[i * nt + k];
+                a = a + 2 * mb[4] * ut[4 * nt + k];
+            } else if (S->component == 'R') {
+                a = a + mb[0] * ut[0 * nt + k];
+                a = a + mb[1] * ut[1 * nt + k];
+                a = a + mb[2] * ut[2 * nt + k];
+                a = a + mb[4] * ut[4 * nt + k] * 2;
+                a = -a;
+            } else {
+                a = a + mb[3] * ut[3 * nt + k] * 2;
+                a = a + mb[5] * ut[5 * nt + k] * 2;
+            }
+            o[k] = a;
+        }
+    }
+done:
+    free(u); free(st); free(ut);
+    return rc;
 }
 
 /* rfft(taperandzeropad(field, N, 10)) of every trace (slave.f90:525, 592) */

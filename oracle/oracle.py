@@ -80,6 +80,8 @@ def lib():
         L.ora_cut_window.argtypes = [C.c_int, f64p, C.c_double, C.c_double, C.POINTER(C.c_int),
                                      C.POINTER(C.c_int)]
         L.ora_rotate_voigt.argtypes = [C.c_int, f64p, C.c_int, C.c_double, C.c_double]
+        L.ora_rotate_voigt_inv.argtypes = [C.c_int, f64p, C.c_double, C.c_double]
+        L.ora_seismograms.argtypes = [C.c_void_p, f64p]
         L.ora_ad_transfer.argtypes = [C.c_void_p, f64p]
         L.ora_ad_forward.argtypes = [C.c_void_p, C.c_int, f64p, f64p, f64p, f64p]
         L.ora_ad_backward.argtypes = [C.c_void_p, C.c_int, f64p, f64p, f64p, f64p, f64p, f64p, f64p]
@@ -302,6 +304,13 @@ def rotate_voigt(t, phi, theta=0.0, to_earth=False):
     return a
 
 
+def rotate_voigt_inv(t, phi, theta=0.0, earth_to_xyz_src=True):
+    """t [6] Voigt: xyz_earth -> xyz_src (phi, theta) or xyz -> src (phi); rotated copy."""
+    a = np.array(t, float, order="C")
+    lib().ora_rotate_voigt_inv(2 if earth_to_xyz_src else 3, P(a), phi, theta)
+    return a
+
+
 def mc_run(vals, volume, allowed_error=0.0, allowed_rel=0.0):
     """Integrator alone: vals [nsteps][npts][n] -> (integral, variance, converged)."""
     vals = np.ascontiguousarray(vals, float)
@@ -364,6 +373,14 @@ class Oracle:
         rc = lib().ora_load_strain(self.h, frame, xyz.shape[0], P(xyz), P(out))
         if rc:
             raise RuntimeError("oracle load_strain failed (%d)" % rc)
+        return out
+
+    def seismograms(self):
+        """[nrec][nt] seismograms by reciprocity from the bwd database."""
+        out = np.zeros((self.nrec, self.nt))
+        rc = lib().ora_seismograms(self.h, P(out))
+        if rc:
+            raise RuntimeError("oracle seismograms failed (%d)" % rc)
         return out
 
     def base_kernels(self, xyz):
