@@ -281,6 +281,27 @@ int mck_base_kernels(int32_t npts, const double *xyz, double *base);
 int mck_seismograms(double *seis);
 
 /*
+ * The model integrals of a task, integrate_1d_model (slave.f90:815-869, computed by the
+ * reference for every element when INT_OVER_BACKGROUND_MODEL is set, slave.f90:450-455,
+ * and returned as slave_result%model, slave.f90:475, 781): per element and basis
+ * function the Monte Carlo integral (volume 1, allowed error 1e-2, relative 1e-2) of the
+ * 12 background-model parameters vp vs rho vph vpv vsh vsv eta phi xi lam mu
+ * (background_model.f90:34-37) times the basis function, 100 points per step (Halton in
+ * tetrahedra whatever quasirandom says, the library's pseudo-random stream in voxels),
+ * at most 9999 steps.  Elements and vertices as mck_slave_work.
+ *  model           out [nelems][nbasis][12]   (Fortran model(12, nbasis, nelems))
+ *  nsteps          out [nelems] MC steps taken (nullable)
+ *  halton_step     in/out; with halton_offsets == NULL element e's points start at
+ *                  *halton_step + e * 999900 (9999 steps of 100) and *halton_step
+ *                  advances past the task
+ *  halton_offsets  optional [nelems] stream position of each element's first point
+ */
+int mck_integrate_model(int32_t elem_type, int32_t nvpe, int32_t nelems, int32_t nbasis,
+                        int32_t nvertices, const double *vertices, const int32_t *conn,
+                        double *model, int32_t *nsteps, int64_t *halton_step,
+                        const int64_t *halton_offsets);
+
+/*
  * The filter-convolve-window-integrate stage of the receiver loop (slave.f90:515-534,
  * 586-640: rfft of the tapered traces, calc_basekernel kernel.f90:514-570,
  * apply_filter_2d kernel.f90:946-956, irfft and calc_misfit_kernel kernel.f90:410-510) as a

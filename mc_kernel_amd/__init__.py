@@ -129,6 +129,22 @@ class MCKernel:
                                                abi.ptr(out, C.c_float)))
         return out
 
+    def integrate_model(self, verts, conn, nbasis=1, halton_offsets=None):
+        """mck_integrate_model: (model [ne][nbasis][12], nsteps [ne]); advances halton_step."""
+        verts = np.ascontiguousarray(verts, np.float64).reshape(-1, 3)
+        conn = np.ascontiguousarray(conn, np.int32)
+        ne, nvpe = conn.shape
+        model = np.zeros((ne, nbasis, 12))
+        nsteps = np.zeros(ne, np.int32)
+        step = np.array([self.halton_step], np.int64)
+        offs = None if halton_offsets is None else np.ascontiguousarray(halton_offsets, np.int64)
+        _check(self.lib.mck_integrate_model(4 if nvpe == 4 else 5, nvpe, ne, nbasis, verts.shape[0],
+                                            abi.ptr(verts, C.c_double), abi.ptr(conn, C.c_int32),
+                                            abi.ptr(model, C.c_double), abi.ptr(nsteps, C.c_int32),
+                                            abi.ptr(step, C.c_int64), abi.ptr(offs, C.c_int64)))
+        self.halton_step = int(step[0])
+        return model, nsteps
+
     def seismograms(self):
         """[nrec][nt] seismograms the kernels use (the set-up's, or by reciprocity)."""
         out = np.zeros((self.R, self.nt))

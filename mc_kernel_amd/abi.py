@@ -87,7 +87,7 @@ def ptr(arr, ctype):
 # exported symbols of libmck.so, in the order of include/mck.h
 EXPORTS = [
     "mck_init", "mck_slave_work", "mck_slave_work_plain", "mck_locate_points",
-    "mck_load_strain_points", "mck_base_kernels", "mck_seismograms", "mck_ad_dims", "mck_ad_transfer",
+    "mck_load_strain_points", "mck_base_kernels", "mck_seismograms", "mck_integrate_model", "mck_ad_dims", "mck_ad_transfer",
     "mck_ad_forward", "mck_ad_backward", "mck_get_stats", "mck_finalize", "mck_last_error",
 ]
 
@@ -98,6 +98,9 @@ def declare(lib):
     lib.mck_init.restype = C.c_int
     lib.mck_seismograms.argtypes = [f64p]
     lib.mck_seismograms.restype = C.c_int
+    lib.mck_integrate_model.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, f64p, i32p,
+                                        f64p, i32p, i64p, i64p]
+    lib.mck_integrate_model.restype = C.c_int
     work = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, f64p, i32p,
             f64p, f64p, i32p, f64p, i64p, i64p]
     lib.mck_slave_work.argtypes = work

@@ -2074,6 +2074,79 @@ int mck_ad_backward(int32_t npts, const float *tf, const float *fw, const float 
     return MCK_OK;
 }
 
+int mck_integrate_model(int32_t elem_type, int32_t nvpe, int32_t nelems, int32_t nbasis, int32_t nvertices,
+                        const double *vertices, const int32_t *conn, double *model, int32_t *nsteps,
+                        int64_t *halton_step, const int64_t *halton_offsets) {
+    State &G = g;
+    if (!G.init) return fail(MCK_ENOTINIT, "mck_init has not been called");
+    const bool vox = elem_type == 5;
+    if (!((elem_type == 4 && nvpe == 4) || (vox && nvpe == 8)))
+        return fail(MCK_EUNSUPPORTED, "inversion elements must be tetrahedra (type 4) or voxels (type 5)");
+    if ((!vox && nbasis != 1 && nbasis != 4) || (vox && nbasis != 1 && nbasis != 8))
+        return fail(MCK_EINVAL, "invalid number of basis functions");
+    if (nelems <= 0 || !vertices || !conn || !model || !halton_step) return fail(MCK_EINVAL, "invalid argument");
+    constexpr int64_t kStride = (int64_t)kModelMaxStep * kModelPts;
+    std::vector<double> geom((size_t)nelems * 12, 0.0), abinv;
+    std::vector<uint8_t> live(nelems, 1);
+    std::vector<int64_t> base(nelems, 0);
+    if (nbasis > 1 && !vox) abinv.assign((size_t)nelems * 16, 0.0);
+    for (int e = 0; e < nelems; ++e) {
+        base[e] = halton_offsets ? halton_offsets[e] : *halton_step + (int64_t)e * kStride;
+        if (conn[nvpe * e] <= 0) { live[e] = 0; continue; }
+        double ev[24];
+        for (int i = 0; i < nvpe; ++i) {
+            const int vid = conn[nvpe * e + i] - 1;
+            if (vid < 0 || vid >= nvertices) return fail(MCK_EINVAL, "connectivity out of range");
+            for (int c = 0; c < 3; ++c) ev[3 * i + c] = vertices[3 * vid + c];
+        }
+        double *v = &geom[(size_t)e * 12];
+        if (vox) {
+            vox_range(ev, v);
+            if (!(v[0] > 0.0) || v[1] <= v[0] || v[3] <= v[2] || v[5] <= v[4])
+                return fail(MCK_EINVAL, "degenerate voxel");
+        } else {
+            std::copy(ev, ev + 12, v);
+            if (nbasis > 1) tet_barycentric_inverse(v, &abinv[(size_t)e * 16]);
+        }
+    }
+    double *d_geom = nullptr, *d_abinv = nullptr, *d_out = nullptr;
+    int64_t *d_base = nullptr;
+    uint8_t *d_live = nullptr;
+    int *d_nsteps = nullptr;
+    const size_t nout = (size_t)nelems * nbasis * kModelPar;
+    std::vector<int> ns(nelems, 0);
+    hipError_t e = hipMalloc((void **)&d_geom, geom.size() * 8);
+    if (e == hipSuccess && !abinv.empty()) e = hipMalloc((void **)&d_abinv, abinv.size() * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&d_out, nout * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&d_base, (size_t)nelems * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&d_live, (size_t)nelems);
+    if (e == hipSuccess) e = hipMalloc((void **)&d_nsteps, (size_t)nelems * 4);
+    if (e == hipSuccess) e = hipMemcpy(d_geom, geom.data(), geom.size() * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess && d_abinv) e = hipMemcpy(d_abinv, abinv.data(), abinv.size() * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_base, base.data(), (size_t)nelems * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_live, live.data(), (size_t)nelems, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemsetAsync(G.err.p, 0, 4, G.stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_model_integral, dim3(nelems), dim3(128), 0, G.stream, nelems, d_geom, vox ? 1 : 0,
+                           nbasis, d_abinv, d_base, d_live, G.model.p, G.S.model_ndepth, G.S.model_dr, d_out,
+                           d_nsteps, G.err.p);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(model, d_out, nout * 8, hipMemcpyDeviceToHost, G.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(ns.data(), d_nsteps, (size_t)nelems * 4, hipMemcpyDeviceToHost, G.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(G.stream);
+    int err = 0;
+    if (e == hipSuccess) e = hipMemcpy(&err, G.err.p, 4, hipMemcpyDeviceToHost);
+    (void)hipFree(d_geom); (void)hipFree(d_abinv); (void)hipFree(d_out);
+    (void)hipFree(d_base); (void)hipFree(d_live); (void)hipFree(d_nsteps);
+    if (e != hipSuccess) return fail(MCK_EHIP, std::string("mck_integrate_model: ") + hipGetErrorString(e));
+    if (err & 4) return fail(MCK_EINVAL, "basis function weight outside [0, 1] (point outside its element)");
+    if (err & 8) return fail(MCK_ENAN, "NaN in a model integral (mc_integration.f90:113-130)");
+    if (nsteps) std::copy(ns.begin(), ns.end(), nsteps);
+    if (!halton_offsets) *halton_step += (int64_t)nelems * kStride;
+    return MCK_OK;
+}
+
 int mck_seismograms(double *seis) {
     State &G = g;
     if (!G.init) return fail(MCK_ENOTINIT, "mck_init has not been called");

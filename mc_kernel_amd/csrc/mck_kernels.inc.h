@@ -179,24 +179,12 @@ __device__ inline double d_prng(int64_t idx, int d)
     z = z ^ (z >> 31);
     return (double)(z >> 11) * (1.0 / 9007199254740992.0);
 }
-
-// With nbasis 4 it also evaluates the vertex basis functions at the point (weights,
-// inversion_mesh.f90:530-622): bweights[point][4] from abinv [nvox][4][4].
-// geom: tetrahedra [nvox][4][3] vertices; voxels [nvox][12] whose first 6 values are the
-// spherical range r_min, r_max, colat_min, colat_max, lon_min, lon_max.
-// Tetrahedra take Halton (quasi) or pseudo-random numbers, voxels pseudo-random ones
-// (generate_random_points, inversion_mesh.f90:626-667).
-__global__ void k_points(int nwork, int P, const int *work_voxel, const int64_t *halton_base,
-                         const int *voxel_draws, const double *geom, int vox, int quasi,
-                         int ndepth, float dr, double *xyz, int *model_idx, int nbasis,
-                         const double *abinv, double *bweights, int *err)
+// Point `idx` of an element's stream mapped into its geometry V (a tetrahedron's 4
+// vertices, or a voxel's range): Halton (quasi, tetrahedra) or the pseudo-random
+// stream in [0, 1)^3, then generate_random_points_vox (random_voxel.f90:45-86) or the
+// Rocchini-Cignoni fold (polytopes.f90:88-110).
+__device__ inline void draw_point(int64_t idx, const double *V, int vox, int quasi, double *x)
 {
-    int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= nwork * P) return;
-    int w = gid / P, p = gid - w * P;
-    int v = work_voxel[w];
-    // the current draw of voxel v is its (voxel_draws[v] - 1)-th
-    int64_t idx = halton_base[v] + (int64_t)(voxel_draws[v] - 1) * P + p;
     double c[4];
     if (quasi && !vox) {
         const int base[3] = {2, 3, 5};
@@ -215,8 +203,6 @@ __global__ void k_points(int nwork, int P, const int *work_voxel, const int64_t 
     } else {
         for (int d = 0; d < 3; ++d) c[d] = d_prng(idx, d);
     }
-    const double *V = geom + (size_t)v * 12;
-    double x[3];
     if (vox) {
         // generate_random_points_vox (random_voxel.f90:45-86)
         const double r = pow(c[0] * (V[1] * V[1] * V[1] - V[0] * V[0] * V[0]) + V[0] * V[0] * V[0], 1.0 / 3.0);
@@ -247,6 +233,37 @@ __global__ void k_points(int nwork, int P, const int *work_voxel, const int64_t 
             x[i] = a;
         }
     }
+}
+
+// linear basis function b of tetrahedron V at x from its barycentric inverse A
+// (inversion_mesh.f90:554-563)
+__device__ inline double tet_bary(const double *A, const double *V, const double *x, int b)
+{
+    const double dx = x[0] - V[0], dy = x[1] - V[1], dz = x[2] - V[2];
+    return A[4 * b] * dx + A[4 * b + 1] * dy + A[4 * b + 2] * dz + A[4 * b + 3] * 1.0;
+}
+
+
+// With nbasis 4 it also evaluates the vertex basis functions at the point (weights,
+// inversion_mesh.f90:530-622): bweights[point][4] from abinv [nvox][4][4].
+// geom: tetrahedra [nvox][4][3] vertices; voxels [nvox][12] whose first 6 values are the
+// spherical range r_min, r_max, colat_min, colat_max, lon_min, lon_max.
+// Tetrahedra take Halton (quasi) or pseudo-random numbers, voxels pseudo-random ones
+// (generate_random_points, inversion_mesh.f90:626-667).
+__global__ void k_points(int nwork, int P, const int *work_voxel, const int64_t *halton_base,
+                         const int *voxel_draws, const double *geom, int vox, int quasi,
+                         int ndepth, float dr, double *xyz, int *model_idx, int nbasis,
+                         const double *abinv, double *bweights, int *err)
+{
+    int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= nwork * P) return;
+    int w = gid / P, p = gid - w * P;
+    int v = work_voxel[w];
+    // the current draw of voxel v is its (voxel_draws[v] - 1)-th
+    int64_t idx = halton_base[v] + (int64_t)(voxel_draws[v] - 1) * P + p;
+    const double *V = geom + (size_t)v * 12;
+    double x[3];
+    draw_point(idx, V, vox, quasi, x);
     xyz[3 * (size_t)gid + 0] = x[0];
     xyz[3 * (size_t)gid + 1] = x[1];
     xyz[3 * (size_t)gid + 2] = x[2];
@@ -256,15 +273,117 @@ __global__ void k_points(int nwork, int P, const int *work_voxel, const int64_t 
     model_idx[gid] = id - 1;
     if (nbasis > 1 && !vox) {
         const double *A = abinv + (size_t)v * 16;
-        const double dx = x[0] - V[0], dy = x[1] - V[1], dz = x[2] - V[2];
         bool bad = false;
         for (int b = 0; b < 4; ++b) {
-            const double w = A[4 * b] * dx + A[4 * b + 1] * dy + A[4 * b + 2] * dz + A[4 * b + 3] * 1.0;
+            const double w = tet_bary(A, V, x, b);
             bweights[(size_t)gid * 4 + b] = w;
             bad |= (w < -1e-9 || w > 1.0 + 1e-9);
         }
         if (bad) atomicOr(err, 4);
     }
+}
+
+// ------------------------------------------------------------------------------------
+// k_model_integral: integrate_1d_model (slave.f90:815-869), one workgroup per element
+// running its whole Monte Carlo loop: 100 points per step (Halton in tetrahedra whatever
+// quasirandom says, the pseudo-random stream in voxels), the 12 background-model
+// parameters of the radial table (combine(), background_model.f90:122-198, fp32; order
+// vp vs rho vph vpv vsh vsv eta phi xi lam mu) times the basis function's weight, one
+// integrator per (basis function, parameter) with volume 1 and errors 1e-2
+// (check_montecarlo_integral, mc_integration.f90:69-143: the 100-point sums in point
+// order, as the reference's sum()), until every integrator has converged or after 9999
+// steps.  base[e]: stream position of the element's first point.  out[e][nbasis][12],
+// nsteps[e]; err bit 4: a basis weight outside [0, 1], bit 8: NaN.
+// ------------------------------------------------------------------------------------
+constexpr int kModelPts = 100, kModelPar = 12, kModelMaxStep = 9999;
+
+__global__ __launch_bounds__(128) void k_model_integral(int nelem, const double *geom, int vox, int nbasis,
+                                                        const double *abinv, const int64_t *base,
+                                                        const uint8_t *live, const float *model, int ndepth,
+                                                        float dr, double *out, int *nsteps, int *err)
+{
+    __shared__ double val[kModelPts][kModelPar];
+    __shared__ double wt[kModelPts][8];
+    __shared__ int nconv;
+    const int e = blockIdx.x, tid = threadIdx.x;
+    if (e >= nelem) return;
+    const int nint = nbasis * kModelPar;
+    const bool owner = tid < nint;
+    const int b = tid / kModelPar, k = tid - (tid / kModelPar) * kModelPar;
+    if (!live[e]) {
+        if (owner) out[((size_t)e * nbasis + b) * kModelPar + k] = 0.0;
+        if (tid == 0) nsteps[e] = 0;
+        return;
+    }
+    const double *V = geom + (size_t)e * 12;
+    double fsum = 0.0, f2sum = 0.0, integ = 0.0;
+    int64_t nmodels = 0;
+    bool conv = false;
+    int istep = 0;
+    for (;;) {
+        // allallconverged (mc_integration.f90:322-340)
+        if (tid == 0) nconv = 0;
+        __syncthreads();
+        if (owner && conv) atomicAdd(&nconv, 1);
+        __syncthreads();
+        if (nconv == nint) break;
+        if (tid < kModelPts) {
+            double x[3];
+            draw_point(base[e] + (int64_t)istep * kModelPts + tid, V, vox, 1, x);
+            const double r = d_norm2(x[0], x[1], x[2]);
+            int id = (int)(r / (double)dr + 1);
+            id = (id < 1 ? 1 : (id > ndepth ? ndepth : id)) - 1;
+            const float vp = model[0 * ndepth + id], vs = model[1 * ndepth + id], rho = model[2 * ndepth + id];
+            const float phi = model[3 * ndepth + id], xi = model[4 * ndepth + id], eta = model[5 * ndepth + id];
+            const float lam = rho * (vp * vp - 2 * (vs * vs)), mu = rho * (vs * vs);
+            const float vsh = vs, vph = vp, vsv = vsh / sqrtf(xi), vpv = vph * sqrtf(phi);
+            const float c[kModelPar] = {vp, vs, rho, vph, vpv, vsh, vsv, eta, phi, xi, lam, mu};
+#pragma unroll
+            for (int j = 0; j < kModelPar; ++j) val[tid][j] = (double)c[j];
+            bool bad = false;
+            for (int bb = 0; bb < nbasis; ++bb) {
+                double w = 1.0;
+                if (nbasis > 1 && !vox) {
+                    w = tet_bary(abinv + (size_t)e * 16, V, x, bb);
+                    bad |= (w < -1e-9 || w > 1.0 + 1e-9);
+                }
+                wt[tid][bb] = w;
+            }
+            if (bad) atomicOr(err, 4);
+        }
+        __syncthreads();
+        if (owner) {
+            nmodels += kModelPts;
+            if (!conv) {
+                double s1 = 0.0, s2 = 0.0;
+#pragma unroll 4
+                for (int i = 0; i < kModelPts; ++i) s1 += val[i][k] * wt[i][b];
+#pragma unroll 4
+                for (int i = 0; i < kModelPts; ++i) {
+                    const double f = val[i][k] * wt[i][b];
+                    s2 += f * f;
+                }
+                fsum = fsum + s1;
+                f2sum = f2sum + s2;
+                integ = fsum * 1.0 / (double)nmodels;
+                double var;
+                if (nmodels > 2) {
+                    var = 1.0 * 1.0 / (double)(nmodels - 1)
+                        * ((f2sum / (double)nmodels) - (fsum / (double)nmodels) * (fsum / (double)nmodels));
+                    if (var < 1e-2 * 1e-2) conv = true;
+                    else if (sqrt(var) < 1e-2 * fabs(integ)) conv = true;
+                } else {
+                    var = integ * 1e10;
+                }
+                if (integ != integ || var != var) atomicOr(err, 8);
+            }
+        }
+        __syncthreads();
+        ++istep;
+        if (istep >= kModelMaxStep) break;
+    }
+    if (owner) out[((size_t)e * nbasis + b) * kModelPar + k] = integ;
+    if (tid == 0) nsteps[e] = istep;
 }
 
 // ------------------------------------------------------------------------------------

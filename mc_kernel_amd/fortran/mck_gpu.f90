@@ -73,7 +73,8 @@ module mck_gpu
   end type mck_stats_t
 
   public :: mck_init, mck_slave_work, mck_slave_work_plain, mck_locate_points, &
-            mck_load_strain_points, mck_base_kernels, mck_seismograms, mck_get_stats, mck_finalize, &
+            mck_load_strain_points, mck_base_kernels, mck_seismograms, mck_integrate_model, &
+            mck_get_stats, mck_finalize, &
             mck_last_error, mck_error_message, mck_check
 
   interface
@@ -141,6 +142,20 @@ module mck_gpu
        import :: c_int, c_double
        real(c_double), intent(out) :: seis(*)
      end function mck_seismograms
+
+     ! model(12, nbasis, nelems): integrate_1d_model for the task (slave.f90:815-869)
+     integer(c_int) function mck_integrate_model(elem_type, nvpe, nelems, nbasis, nvertices, &
+                                                 vertices, conn, model, nsteps, halton_step, &
+                                                 halton_offsets) bind(c, name='mck_integrate_model')
+       import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+       integer(c_int32_t), value :: elem_type, nvpe, nelems, nbasis, nvertices
+       real(c_double), intent(in) :: vertices(*)
+       integer(c_int32_t), intent(in) :: conn(*)
+       real(c_double), intent(out) :: model(*)
+       integer(c_int32_t), intent(out) :: nsteps(*)
+       integer(c_int64_t), intent(inout) :: halton_step
+       type(c_ptr), value :: halton_offsets
+     end function mck_integrate_model
 
      integer(c_int) function mck_get_stats(stats) bind(c, name='mck_get_stats')
        import :: c_int, mck_stats_t

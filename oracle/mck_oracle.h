@@ -111,6 +111,11 @@ int      ora_base_kernels(ora_ctx *ctx, int npts, const double *xyz, double *bas
  * (load_seismogram / load_fw_points_rdbm, readfields.f90:1004-1048, 1469-1604); used by
  * ora_create when the set-up's seis is NULL. */
 int      ora_seismograms(ora_ctx *ctx, double *seis);
+/* integrate_1d_model (slave.f90:815-869) for a task: model[nelems][nbasis][12] MC
+ * integrals of the background-model parameters, nsteps[nelems] (nullable) */
+int      ora_integrate_model(ora_ctx *ctx, int elem_type, int nvpe, int nelems, int nbasis,
+                             const double *vertices, const int32_t *conn, double *model,
+                             int32_t *nsteps, int64_t *halton_step, const int64_t *halton_offsets);
 /* Test hook: which 2 = xyz_earth -> xyz_src (rotations.f90:615-671), 3 = xyz -> src
  * (rotations.f90:423-455); t[6] Voigt, in place. */
 void     ora_rotate_voigt_inv(int which, double *t, double phi, double theta);

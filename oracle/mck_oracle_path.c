@@ -1269,6 +1269,87 @@ out:
  * the stage.  The adjoint runs the same steps backwards in FP64.
  * ---------------------------------------------------------------------------------- */
 
+/* integrate_1d_model (slave.f90:815-869): per inversion element and basis function the
+ * Monte Carlo integral (volume 1, allowed error 1e-2, relative 1e-2,
+ * initialize_montecarlo, mc_integration.f90:147-187) of the 12 background-model
+ * parameters in the order of background_model.f90:34-37 (vp vs rho vph vpv vsh vsv eta
+ * phi xi lam mu; combine(), background_model.f90:122-198, in fp32), each times the basis
+ * function's weight (weight(), background_model.f90:233-261), over 100 points per step:
+ * Halton points in tetrahedra whatever quasirandom says (generate_random_points(ielement,
+ * nptperstep_model, .true.)), pseudo-random points in voxels; the loop stops when every
+ * integrator has converged or after 9999 steps.  Stream positions: halton_offsets[e], or
+ * *halton_step + e * 9999 * 100 (then *halton_step advances past the task).
+ * model[nelems][nbasis][12]; nsteps[nelems] (nullable).  Elements whose first vertex id is
+ * <= 0 (empty slots of a last task) give zeros. */
+int ora_integrate_model(ora_ctx *C, int elem_type, int nvpe, int nelems, int nbasis,
+                        const double *vertices, const int32_t *conn, double *model,
+                        int32_t *nsteps, int64_t *halton_step, const int64_t *halton_offsets)
+{
+    enum { NP = 100, NF = 12, MAXSTEP = 9999 };
+    const int vox = elem_type == 5;
+    const mck_setup_t *S = &C->S;
+    static const int64_t seed[3] = {20, 20, 20}, leap[3] = {2, 2, 2};
+    static const int32_t bases[3] = {2, 3, 5};
+    double pts[NP * 3], hal[NP * 3], wts[NP], vals[NP * NF];
+    ora_mc mcs[8];
+    int e, i, k, b, rc = 0;
+    if (!((elem_type == 4 && nvpe == 4 && (nbasis == 1 || nbasis == 4))
+          || (vox && nvpe == 8 && (nbasis == 1 || nbasis == 8)))) return MCK_EINVAL;
+    for (e = 0; e < nelems; ++e) {
+        double v[24], A[16];
+        int64_t pos = halton_offsets ? halton_offsets[e] : *halton_step + (int64_t)e * MAXSTEP * NP;
+        int istep = 0, all;
+        if (conn[e * nvpe] <= 0) {
+            for (k = 0; k < nbasis * NF; ++k) model[(size_t)e * nbasis * NF + k] = 0.0;
+            if (nsteps) nsteps[e] = 0;
+            continue;
+        }
+        for (i = 0; i < nvpe; ++i) {
+            int vid = conn[e * nvpe + i] - 1;
+            v[3 * i] = vertices[3 * vid]; v[3 * i + 1] = vertices[3 * vid + 1]; v[3 * i + 2] = vertices[3 * vid + 2];
+        }
+        if (!vox && nbasis > 1) tet_abinv(v, A);
+        for (b = 0; b < nbasis; ++b) mc_init(&mcs[b], NF, 1.0, 1e-2, 1e-2);
+        for (;;) {
+            /* allallconverged (mc_integration.f90:322-340) */
+            all = 1;
+            for (b = 0; b < nbasis; ++b) for (k = 0; k < NF; ++k) if (!mcs[b].conv[k]) all = 0;
+            if (all) break;
+            if (!vox) ora_halton(3, seed, bases, leap, pos, NP, hal);
+            else ora_prng_points(pos, NP, hal);
+            pos += NP;
+            if (vox) ora_vox_points(v, NP, hal, pts);
+            else ora_tet_points(v, NP, hal, pts);
+            for (b = 0; b < nbasis && !rc; ++b) {
+                if (nbasis == 1 || vox) {
+                    for (i = 0; i < NP; ++i) wts[i] = 1.0;
+                } else if (tet_weights(v, A, b, NP, pts, wts)) {
+                    rc = MCK_EINVAL;
+                    break;
+                }
+                for (i = 0; i < NP; ++i) {
+                    ora_model m = model_at(S, ora_norm2(3, pts + 3 * i));
+                    const float c[NF] = {m.vp, m.vs, m.rho, m.vph, m.vpv, m.vsh, m.vsv, m.eta, m.phi, m.xi,
+                                         m.lam, m.mu};
+                    for (k = 0; k < NF; ++k) vals[i * NF + k] = (double)c[k] * wts[i];
+                }
+                rc = mc_check(&mcs[b], NP, vals);
+            }
+            if (rc) break;
+            istep += 1;
+            if (istep >= MAXSTEP) break;
+        }
+        for (b = 0; b < nbasis; ++b) {
+            for (k = 0; k < NF; ++k) model[((size_t)e * nbasis + b) * NF + k] = mcs[b].integral[k];
+            mc_free(&mcs[b]);
+        }
+        if (nsteps) nsteps[e] = istep;
+        if (rc) return rc;
+    }
+    if (!halton_offsets) *halton_step += (int64_t)nelems * MAXSTEP * NP;
+    return 0;
+}
+
 /* G of every filter [nfilter][nomega][2] */
 void ora_ad_transfer(ora_ctx *C, double *G)
 {

@@ -82,6 +82,8 @@ def lib():
         L.ora_rotate_voigt.argtypes = [C.c_int, f64p, C.c_int, C.c_double, C.c_double]
         L.ora_rotate_voigt_inv.argtypes = [C.c_int, f64p, C.c_double, C.c_double]
         L.ora_seismograms.argtypes = [C.c_void_p, f64p]
+        L.ora_integrate_model.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, f64p, i32p,
+                                          f64p, i32p, i64p, i64p]
         L.ora_ad_transfer.argtypes = [C.c_void_p, f64p]
         L.ora_ad_forward.argtypes = [C.c_void_p, C.c_int, f64p, f64p, f64p, f64p]
         L.ora_ad_backward.argtypes = [C.c_void_p, C.c_int, f64p, f64p, f64p, f64p, f64p, f64p, f64p]
@@ -374,6 +376,22 @@ class Oracle:
         if rc:
             raise RuntimeError("oracle load_strain failed (%d)" % rc)
         return out
+
+    def integrate_model(self, verts, conn, nbasis=1, halton_step=0, halton_offsets=None):
+        """integrate_1d_model for a task: (model [ne][nbasis][12], nsteps [ne], new step)."""
+        verts = np.ascontiguousarray(verts, float).reshape(-1, 3)
+        conn = np.ascontiguousarray(conn, np.int32)
+        ne, nvpe = conn.shape
+        model = np.zeros((ne, nbasis, 12))
+        nsteps = np.zeros(ne, np.int32)
+        step = np.array([halton_step], np.int64)
+        offs = None if halton_offsets is None else np.ascontiguousarray(halton_offsets, np.int64)
+        rc = lib().ora_integrate_model(self.h, 4 if nvpe == 4 else 5, nvpe, ne, nbasis, P(verts),
+                                       P(conn, C.c_int32), P(model), P(nsteps, C.c_int32),
+                                       P(step, C.c_int64), None if offs is None else P(offs, C.c_int64))
+        if rc:
+            raise RuntimeError("oracle integrate_model failed (%d)" % rc)
+        return model, nsteps, int(step[0])
 
     def seismograms(self):
         """[nrec][nt] seismograms by reciprocity from the bwd database."""
