@@ -179,7 +179,8 @@ typedef struct {
 typedef struct {
     int64_t evaluations;       /* computed (voxel, MC point, kernel) values              */
     int64_t points;            /* MC points drawn                                        */
-    int64_t fwd_locations;     /* point locations in the fwd mesh                        */
+    int64_t fwd_locations;     /* points located in the fwd mesh with an active receiver */
+                               /* (their forward spectra are computed)                   */
     int64_t bwd_locations;     /* point locations in the bwd mesh (sum over receivers)   */
     int64_t fwd_bytes;         /* algorithmic DB bytes of the strain gathers (fwd)       */
     int64_t bwd_bytes;         /* algorithmic DB bytes of the strain gathers (bwd)       */

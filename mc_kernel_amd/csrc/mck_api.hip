@@ -810,19 +810,20 @@ int spt_for(int nt) {
 unsigned grid_x8(int n) { return 8u * (unsigned)((n + 7) / 8); }
 
 // operator set-up of n points (fwd) or pairs (bwd) in launch order (null = identity)
-hipError_t launch_setup(State &G, bool fwd, int n, const int *order, const Located *loc) {
+hipError_t launch_setup(State &G, bool fwd, int n, const int *order, const int *keys, const Located *loc) {
     if (n <= 0) return hipSuccess;
     const DbDev &D = fwd ? G.fwd : G.bwd;
     const size_t smem = (size_t)kSetupWaves * op_bytes(0);
     hipLaunchKernelGGL(k_op_setup, dim3((unsigned)((n + kSetupWaves - 1) / kSetupWaves)), dim3(64 * kSetupWaves),
-                       smem, G.stream, D.d, G.pc, G.frames.p, n, order, G.R, loc, fwd ? 1 : 0,
+                       smem, G.stream, D.d, G.pc, G.frames.p, n, order, keys, G.R, loc, fwd ? 1 : 0,
                        fwd ? G.opWf.p : G.opWb.p, fwd ? G.opQf.p : G.opQb.p);
     return hipGetLastError();
 }
 
 // forward spectra of npts points (order: element-sorted point ids, or null = identity)
 // whose elements db holds; the operator set-up of the points (launch_setup) precedes it
-hipError_t launch_fwd(State &G, const DevDb &db, int npts, const int *order, const Located *loc, float2 *F) {
+hipError_t launch_fwd(State &G, const DevDb &db, int npts, const int *order, const int *keys, const Located *loc,
+                      float2 *F) {
     if (npts <= 0) return hipSuccess;
     const size_t smem = strain_smem(G.pc);
     const dim3 grid(grid_x8(npts)), block(kStrainThreads);
@@ -833,7 +834,7 @@ hipError_t launch_fwd(State &G, const DevDb &db, int npts, const int *order, con
             (void)hipFuncSetAttribute((const void *)&k_fwd_spectra<SP, ND>,                       \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);     \
         hipLaunchKernelGGL((k_fwd_spectra<SP, ND>), grid, block, smem, G.stream, db, G.pc,        \
-                           npts, order, loc, G.opWf.p, G.twq.p, F);                              \
+                           npts, order, keys, loc, G.opWf.p, G.twq.p, F);                        \
         return hipGetLastError();                                                                 \
     }
 #define FWD_ND(SP) FWD(SP, 1) FWD(SP, 4) FWD(SP, 6)
@@ -1566,19 +1567,19 @@ int eval_chunk(State &G, int P, int w0, int nw, std::vector<std::pair<hipEvent_t
     HIPCHECK(hipGetLastError());
     // forward: points in element order
     hipLaunchKernelGGL(k_keys_points, dim3((np + 255) / 256), dim3(256), 0, st, np, loc_f, G.fwd.d.nelem,
-                       G.keys_in.p, G.vals_in.p);
+                       act, P, R, G.keys_in.p, G.vals_in.p);
     HIPCHECK(hipGetLastError());
     if ((rc = sort_by_element(G, np, G.fwd.d.nelem))) return rc;
     hipLaunchKernelGGL(k_count_runs, dim3((np + 255) / 256), dim3(256), 0, st, np, G.keys_out.p,
                        G.fwd.d.nelem, G.elem_count.p + 0);
     HIPCHECK(hipGetLastError());
-    HIPCHECK(launch_setup(G, true, np, G.vals_out.p, loc_f));
+    HIPCHECK(launch_setup(G, true, np, G.vals_out.p, G.keys_out.p, loc_f));
     hipEvent_t a, b;
     HIPCHECK(hipEventCreate(&a));
     HIPCHECK(hipEventCreate(&b));
     HIPCHECK(hipEventRecord(a, st));
     if ((rc = run_paged(G, G.fwd, np, G.keys_out.p, [&](const DevDb &d, int i0, int i1) {
-             return launch_fwd(G, d, i1 - i0, G.vals_out.p + i0, loc_f, G.Fspec.p);
+             return launch_fwd(G, d, i1 - i0, G.vals_out.p + i0, G.keys_out.p + i0, loc_f, G.Fspec.p);
          })))
         return rc;
     HIPCHECK(hipEventRecord(b, st));
@@ -1597,7 +1598,7 @@ int eval_chunk(State &G, int P, int w0, int nw, std::vector<std::pair<hipEvent_t
                            G.bwd.d.nelem, G.elem_count.p + 2);
         HIPCHECK(hipGetLastError());
     }
-    HIPCHECK(launch_setup(G, false, npairs, G.vals_out.p, loc_b));
+    HIPCHECK(launch_setup(G, false, npairs, G.vals_out.p, nullptr, loc_b));
     HIPCHECK(hipEventCreate(&a));
     HIPCHECK(hipEventCreate(&b));
     HIPCHECK(hipEventRecord(a, st));
@@ -1984,12 +1985,12 @@ int mck_base_kernels(int32_t npts, const double *xyz, double *base) {
     HIPCHECK(hipMalloc((void **)&d_base, (size_t)npts * G.nk * 6 * 8));
     HIPCHECK(hipMalloc((void **)&d_phys, (size_t)npts * G.nk * 8));
     hipError_t e = hipMemsetAsync(d_base, 0, (size_t)npts * G.nk * 6 * 8, G.stream);
-    if (e == hipSuccess) e = launch_setup(G, true, npts, nullptr, loc_f);
+    if (e == hipSuccess) e = launch_setup(G, true, npts, nullptr, nullptr, loc_f);
     if (e == hipSuccess)
         rc = run_paged(G, G.fwd, npts, nullptr, [&](const DevDb &d, int, int) {
-            return launch_fwd(G, d, npts, nullptr, loc_f, G.Fspec.p);
+            return launch_fwd(G, d, npts, nullptr, nullptr, loc_f, G.Fspec.p);
         });
-    if (e == hipSuccess && rc == MCK_OK) e = launch_setup(G, false, npts * R, nullptr, loc_b);
+    if (e == hipSuccess && rc == MCK_OK) e = launch_setup(G, false, npts * R, nullptr, nullptr, loc_b);
     if (e == hipSuccess && rc == MCK_OK)
         rc = run_paged(G, G.bwd, npts * R, nullptr, [&](const DevDb &d, int, int) {
             return launch_bwd(G, d, npts * R, nullptr, loc_b, G.opWb.p, G.opQb.p, d_phys, d_base, nullptr, 1);

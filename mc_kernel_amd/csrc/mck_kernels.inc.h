@@ -794,8 +794,8 @@ __global__ __launch_bounds__(kStrainThreads) void k_seis(DevDb db, PathConf pc, 
 constexpr int kSetupWaves = 4;
 
 __global__ __launch_bounds__(64 * kSetupWaves) void k_op_setup(DevDb db, PathConf pc, const DevFrame *frames,
-                                                               int n, const int *order, int R, const Located *loc,
-                                                               int fwd, float *Wout, float *Qout)
+                                                               int n, const int *order, const int *keys, int R,
+                                                               const Located *loc, int fwd, float *Wout, float *Qout)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -804,7 +804,7 @@ __global__ __launch_bounds__(64 * kSetupWaves) void k_op_setup(DevDb db, PathCon
     const int q = lb < n ? (order ? order[lb] : lb) : 0;
     Located L;
     L.elem = -1;
-    if (lb < n) L = loc[q];
+    if (lb < n && !(keys && keys[lb] >= db.nelem)) L = loc[q];
     const bool live = L.elem >= 0;
     const int nvar = fwd ? pc.nvar_fwd : pc.nvar_bwd;
     const int *vars = fwd ? pc.fwd_vars : pc.bwd_vars;
@@ -1196,7 +1196,7 @@ __global__ __launch_bounds__(kStrainThreads) void k_strain_traces(
 // ------------------------------------------------------------------------------------
 template <int SPT, int ND>
 __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && ND <= 4) ? 3 : (SPT <= 8 ? 2 : 1)) void k_fwd_spectra(
-    DevDb db, PathConf pc, int npts, const int *order, const Located *loc, const float *Wf,
+    DevDb db, PathConf pc, int npts, const int *order, const int *keys, const Located *loc, const float *Wf,
     const float2 *twq, float2 *F)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1209,6 +1209,7 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && ND <= 4) ? 3 : (SPT <=
 
     const int lb = xcd_block(npts);
     if (lb < 0) return;
+    if (keys && keys[lb] >= db.nelem) return;                  // spectra nobody reads
     const int p = order ? order[lb] : lb;
     const Located L = loc[p];
     if (L.elem < db.u_first || L.elem >= db.u_last) return;   // not located / not in this page
@@ -1550,12 +1551,18 @@ __global__ void k_page_bounds(int n, const int *keys, int page_elems, int npages
     out[b] = lo;
 }
 
-__global__ void k_keys_points(int n, const Located *loc, int nelem, int *keys, int *vals)
+// (points of a voxel without an active receiver - all travel-time masked or converged -
+// get the key nelem too: their forward spectra would never be read)
+__global__ void k_keys_points(int n, const Located *loc, int nelem, const uint8_t *active, int P, int R,
+                              int *keys, int *vals)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int e = loc[i].elem;
-    keys[i] = e < 0 ? nelem : e;
+    bool any = active == nullptr;
+    if (active)
+        for (int r = 0; r < R; ++r) any |= active[(size_t)(i / P) * R + r] != 0;
+    keys[i] = (e < 0 || !any) ? nelem : e;
     vals[i] = i;
 }
 
