@@ -378,6 +378,7 @@ struct State {
     // id) and of the pairs (by position in the pair order)
     DBuf<float> opWf, opQf, opWb, opQb;
     DBuf<int> page_bounds;              // [npages + 1] first item of every database page
+    DBuf<int> nlive;                    // [2]: 0, points of a launch with an active receiver
     std::vector<double> seis_host;      // [nrec][nt] seismograms by reciprocity (seis NULL)
     hipStream_t copy_stream = nullptr;  // page copies of paged databases
     DBuf<int> keys_in, keys_out, vals_in, vals_out, pair_count;
@@ -822,8 +823,8 @@ hipError_t launch_setup(State &G, bool fwd, int n, const int *order, const int *
 
 // forward spectra of npts points (order: element-sorted point ids, or null = identity)
 // whose elements db holds; the operator set-up of the points (launch_setup) precedes it
-hipError_t launch_fwd(State &G, const DevDb &db, int npts, const int *order, const int *keys, const Located *loc,
-                      float2 *F) {
+hipError_t launch_fwd(State &G, const DevDb &db, int npts, const int *nlive, const int *order, const int *keys,
+                      const Located *loc, float2 *F) {
     if (npts <= 0) return hipSuccess;
     const size_t smem = strain_smem(G.pc);
     const dim3 grid(grid_x8(npts)), block(kStrainThreads);
@@ -834,7 +835,7 @@ hipError_t launch_fwd(State &G, const DevDb &db, int npts, const int *order, con
             (void)hipFuncSetAttribute((const void *)&k_fwd_spectra<SP, ND>,                       \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);     \
         hipLaunchKernelGGL((k_fwd_spectra<SP, ND>), grid, block, smem, G.stream, db, G.pc,        \
-                           npts, order, keys, loc, G.opWf.p, G.twq.p, F);                        \
+                           npts, nlive, order, keys, loc, G.opWf.p, G.twq.p, F);                 \
         return hipGetLastError();                                                                 \
     }
 #define FWD_ND(SP) FWD(SP, 1) FWD(SP, 4) FWD(SP, 6)
@@ -1574,12 +1575,18 @@ int eval_chunk(State &G, int P, int w0, int nw, std::vector<std::pair<hipEvent_t
                        G.fwd.d.nelem, G.elem_count.p + 0);
     HIPCHECK(hipGetLastError());
     HIPCHECK(launch_setup(G, true, np, G.vals_out.p, G.keys_out.p, loc_f));
+    // points with an active receiver come first in the sorted order: their count
+    HIPCHECK(G.nlive.ensure(2));
+    hipLaunchKernelGGL(k_page_bounds, dim3(1), dim3(64), 0, st, np, G.keys_out.p, G.fwd.d.nelem, 1, G.nlive.p);
+    HIPCHECK(hipGetLastError());
     hipEvent_t a, b;
     HIPCHECK(hipEventCreate(&a));
     HIPCHECK(hipEventCreate(&b));
     HIPCHECK(hipEventRecord(a, st));
     if ((rc = run_paged(G, G.fwd, np, G.keys_out.p, [&](const DevDb &d, int i0, int i1) {
-             return launch_fwd(G, d, i1 - i0, G.vals_out.p + i0, G.keys_out.p + i0, loc_f, G.Fspec.p);
+             // (a paged launch covers one page's range, which holds no inactive point)
+             return launch_fwd(G, d, i1 - i0, G.fwd.paged ? nullptr : G.nlive.p + 1, G.vals_out.p + i0,
+                               G.keys_out.p + i0, loc_f, G.Fspec.p);
          })))
         return rc;
     HIPCHECK(hipEventRecord(b, st));
@@ -1988,7 +1995,7 @@ int mck_base_kernels(int32_t npts, const double *xyz, double *base) {
     if (e == hipSuccess) e = launch_setup(G, true, npts, nullptr, nullptr, loc_f);
     if (e == hipSuccess)
         rc = run_paged(G, G.fwd, npts, nullptr, [&](const DevDb &d, int, int) {
-            return launch_fwd(G, d, npts, nullptr, nullptr, loc_f, G.Fspec.p);
+            return launch_fwd(G, d, npts, nullptr, nullptr, nullptr, loc_f, G.Fspec.p);
         });
     if (e == hipSuccess && rc == MCK_OK) e = launch_setup(G, false, npts * R, nullptr, nullptr, loc_b);
     if (e == hipSuccess && rc == MCK_OK)
@@ -2182,6 +2189,7 @@ void mck_finalize(void) {
     G.nmodels.release(); G.xyz.release(); G.model_idx.release(); G.loc.release();
     G.Fspec.release(); G.phys.release(); G.evals.release(); G.err.release();
     G.opWf.release(); G.opQf.release(); G.opWb.release(); G.opQb.release(); G.page_bounds.release();
+    G.nlive.release();
     G.abinv.release(); G.bweights.release();
     G.elem_count.release(); G.keys_in.release(); G.keys_out.release();
     G.vals_in.release(); G.vals_out.release(); G.pair_count.release(); G.sort_tmp.release();

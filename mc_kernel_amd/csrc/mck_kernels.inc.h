@@ -832,6 +832,16 @@ __device__ inline int xcd_block(int n)
     return lb < n ? lb : -1;
 }
 
+// the same over the first n items of a grid that may be larger: every XCD takes one
+// contiguous eighth of those n (blocks past them return -1)
+__device__ inline int xcd_block_n(int n)
+{
+    const int b = blockIdx.x, per = (n + 7) >> 3, i = b >> 3;
+    if (i >= per) return -1;
+    const int lb = (b & 7) * per + i;
+    return lb < n ? lb : -1;
+}
+
 // acc[c][q] = sum_v sum_node W[c][v][node] u_v(node, t0 + q): the strain (or trace) of
 // carried component c at this thread's SPT samples
 // loads rows n0..n0+NB-1 of one variable (this thread's samples)
@@ -1196,8 +1206,8 @@ __global__ __launch_bounds__(kStrainThreads) void k_strain_traces(
 // ------------------------------------------------------------------------------------
 template <int SPT, int ND>
 __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && ND <= 4) ? 3 : (SPT <= 8 ? 2 : 1)) void k_fwd_spectra(
-    DevDb db, PathConf pc, int npts, const int *order, const int *keys, const Located *loc, const float *Wf,
-    const float2 *twq, float2 *F)
+    DevDb db, PathConf pc, int npts, const int *nlive, const int *order, const int *keys, const Located *loc,
+    const float *Wf, const float2 *twq, float2 *F)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     OpShared &S = *reinterpret_cast<OpShared *>(smem_raw);
@@ -1207,7 +1217,9 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && ND <= 4) ? 3 : (SPT <=
     __shared__ float wave_tot[kStrainThreads / 64];
     __shared__ float edge_x[kStrainThreads + 1];
 
-    const int lb = xcd_block(npts);
+    // nlive (device, nullable): the sorted items before the first one keyed past the last
+    // element; the XCD ranges cover only those
+    const int lb = xcd_block_n(nlive ? min(*nlive, npts) : npts);
     if (lb < 0) return;
     if (keys && keys[lb] >= db.nelem) return;                  // spectra nobody reads
     const int p = order ? order[lb] : lb;
