@@ -397,125 +397,188 @@ __global__ __launch_bounds__(kStrainThreads, SPT <= 8 ? 2 : 1) void k_ad_eval(
     }
 }
 
-// Backward: one workgroup per point, receivers in turn.  gbase [npts][nk][6] weights the
-// base kernels; results: gbw [npts][R][ND][nt], gfw [npts][ND][nt], and per point the
-// filter-response partial sums Up [npts][nF][nw] = sum_{r, s} g_s W_h bk_b, reduced over
-// points by k_ad_upart / k_ad_tfgrad.  gFs [npts][ND][nw] is the point's scratch for the
-// forward-spectrum gradient (each bin read and written by its owning thread only).
-// (full strain from 4 samples per thread holds more than 256 VGPRs: one wave per SIMD)
+// Backward in four stages, each with few registers, over the pairs q = p R + r of a
+// chunk of points (intermediates in global memory):
+//   k_ad_bstage  per pair: B = rfft(taper bw) -> Bs [q][ND][nw]
+//   k_ad_cstage  per pair: per bin the adjoint of the contraction: slot weights
+//                gw = sum of dL/dbase over the receiver's kernels, Y_b = sum gw H, U_f +=
+//                W_h sum gw bk_b -> Ups [q][nF][nw]; holomorphic derivatives of bk ->
+//                conj(dL/dB) over Bs, conj(dL/dF) -> dFs [q][ND][nw]
+//   k_ad_bgrad   per pair: adjoint real FFT of dL/dB and the taper -> dL/dbw
+//   k_ad_fgrad   per point: dL/dF and U summed over the receivers in order, adjoint real
+//                FFT, taper and cumsum_trapezoidal adjoint -> dL/dfw; U -> Up [p][nF][nw]
+template <int SPT>
+__global__ __launch_bounds__(kStrainThreads) void k_ad_bstage(PathConf pc, int npairs, int nd, const float *bw,
+                                                            const float2 *twq, float2 *Bs)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2 *bufA = reinterpret_cast<float2 *>(smem_raw);
+    float2 *bufB = bufA + pc.M;
+    float2 *Tq = bufA + 2 * pc.M;
+    const int q = blockIdx.x;
+    if (q >= npairs) return;
+    load_twiddles(twq, pc.M, Tq);
+    const int tid = threadIdx.x, nt = pc.nt, M = pc.M, nw = pc.nomega, t0 = tid * SPT;
+    const float fdt = (float)pc.dt;
+    float tap[SPT];
+    taper_factors(t0, nt, pc, tap);
+    for (int c = 0; c < nd; ++c) {
+        const float *src = bw + ((size_t)q * nd + c) * nt;
+        float x[SPT];
+#pragma unroll
+        for (int i = 0; i < SPT; ++i) x[i] = t0 + i < nt ? src[t0 + i] * tap[i] : 0.0f;
+        const float2 *Z = lds_fft(x, t0, nt, M, bufA, bufB, Tq);
+        float2 *dst = Bs + ((size_t)q * nd + c) * nw;
+        for (int k = tid; k <= M; k += kStrainThreads) dst[k] = rfft_bin(Z, k, M, Tq, fdt);
+        __syncthreads();
+    }
+}
+
 template <int SPT, int ND>
-__global__ __launch_bounds__(kStrainThreads, (SPT >= 16 || (ND == 6 && SPT >= 4)) ? 1 : 2) void k_ad_grad(
-    PathConf pc, int npts, int R, const float *bw, const float2 *twq, const float2 *Fs,
-    const RecDesc *recs, const KernDesc *kern, const float2 *Hg, const float2 *W, const int *hfilt,
-    int nF, int nk, const double *gbase, float2 *gFs, float2 *Up, float *gbw, float *gfw)
+__global__ __launch_bounds__(kStrainThreads) void k_ad_cstage(
+    PathConf pc, int npairs, int R, const float2 *Fs, float2 *Bs, const RecDesc *recs, const KernDesc *kern,
+    const float2 *Hg, const float2 *W, const int *hfilt, int nF, int nk, const double *gbase, float2 *dFs,
+    float2 *Ups)
+{
+    __shared__ float gw[kMaxSlots][6];      // [group][base kernel] slot weight of the receiver
+    const int q = blockIdx.x;
+    if (q >= npairs) return;
+    const int p = q / R, r = q - (q / R) * R;
+    const int tid = threadIdx.x, M = pc.M, nw = pc.nomega;
+    const RecDesc &rd = recs[r];
+    for (int i = tid; i < rd.ngroup * 6; i += kStrainThreads) gw[i / 6][i - (i / 6) * 6] = 0.0f;
+    __syncthreads();
+    if (tid < rd.ngroup) {
+        for (int t = 0; t < rd.grp_n[tid]; ++t) {
+            const int s = rd.grp_first[tid] + t;
+            double gs = 0.0;
+            for (int kk = 0; kk < rd.nkernel; ++kk) {
+                const int k = rd.first_kernel + kk;
+                for (int b = 0; b < 6; ++b)
+                    if (kern[k].slot[b] == s) gs += gbase[((size_t)p * nk + k) * 6 + b];
+            }
+            gw[tid][rd.slot_bk[s]] = (float)gs;
+        }
+    }
+    __syncthreads();
+    float2 *U = Ups + (size_t)q * nF * nw;
+    float2 *B = Bs + (size_t)q * ND * nw;
+    float2 *dF = dFs + (size_t)q * ND * nw;
+    const float2 *Fp = Fs + (size_t)p * ND * nw;
+#pragma unroll 1
+    for (int j = 0; j <= SPT; ++j) {
+        int k;
+        if (!ad_bin<SPT>(j, M, k)) continue;
+        for (int f = 0; f < nF; ++f) U[(size_t)f * nw + k] = make_float2(0.f, 0.f);
+        float2 f[ND], b[ND], gF[ND], gB[ND];
+#pragma unroll
+        for (int c = 0; c < ND; ++c) { f[c] = Fp[(size_t)c * nw + k]; b[c] = B[(size_t)c * nw + k]; }
+        const Bk6 bk = ad_basekernels<ND>(f, b, pc.need_bk);
+        Bk6 Y = bk_zero();
+        for (int g = 0; g < rd.ngroup; ++g) {
+            const int h = rd.grp_h[g];
+            const float2 hg = Hg[(size_t)h * nw + k], w = W[(size_t)h * nw + k];
+            float2 Z = make_float2(0.f, 0.f);
+            auto term = [&](auto bsel) {
+                constexpr int Bi = decltype(bsel)::value;
+                const float gs = gw[g][Bi];
+                bk_at<Bi>(Y) = cadd(bk_at<Bi>(Y), cscale(gs, hg));
+                Z = cadd(Z, cscale(gs, bk_at<Bi>(bk)));
+            };
+            term(std::integral_constant<int, 0>{});
+            term(std::integral_constant<int, 1>{});
+            term(std::integral_constant<int, 3>{});
+            term(std::integral_constant<int, 4>{});
+            term(std::integral_constant<int, 5>{});
+            float2 *u = U + (size_t)hfilt[h] * nw + k;
+            *u = cadd(*u, cmulf(w, Z));
+        }
+        ad_basekernels_adjoint<ND>(f, b, Y, gF, gB);
+#pragma unroll
+        for (int c = 0; c < ND; ++c) {
+            B[(size_t)c * nw + k] = cconj(gB[c]);
+            dF[(size_t)c * nw + k] = cconj(gF[c]);
+        }
+    }
+}
+
+template <int SPT>
+__global__ __launch_bounds__(kStrainThreads) void k_ad_bgrad(PathConf pc, int npairs, int nd, const float2 *Bs,
+                                                           const float2 *twq, float *gbw)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float2 *bufA = reinterpret_cast<float2 *>(smem_raw);
     float2 *bufB = bufA + pc.M;
     float2 *Tq = bufA + 2 * pc.M;
     __shared__ float2 zM;
-    __shared__ float gw[kMaxSlots][6];      // [group][base kernel] slot weight of the receiver
+    const int q = blockIdx.x;
+    if (q >= npairs) return;
+    load_twiddles(twq, pc.M, Tq);
+    const int tid = threadIdx.x, nt = pc.nt, M = pc.M, nw = pc.nomega, t0 = tid * SPT;
+    float tap[SPT];
+    taper_factors(t0, nt, pc, tap);
+    for (int c = 0; c < nd; ++c) {
+        const float2 *src = Bs + ((size_t)q * nd + c) * nw;
+        float2 g[SPT + 1];
+#pragma unroll
+        for (int j = 0; j <= SPT; ++j) {
+            int k;
+            g[j] = ad_bin<SPT>(j, M, k) ? src[k] : make_float2(0.f, 0.f);
+        }
+        float y[SPT];
+        rfft_adjoint<SPT>(g, t0, M, (float)pc.dt, bufA, bufB, &zM, Tq, y);
+        float *dst = gbw + ((size_t)q * nd + c) * nt;
+#pragma unroll
+        for (int i = 0; i < SPT; ++i)
+            if (t0 + i < nt) dst[t0 + i] = y[i] * tap[i];
+    }
+}
+
+template <int SPT>
+__global__ __launch_bounds__(kStrainThreads) void k_ad_fgrad(PathConf pc, int npts, int R, int nd, int nF,
+                                                           const float2 *dFs, const float2 *Ups,
+                                                           const float2 *twq, float2 *Up, float *gfw)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2 *bufA = reinterpret_cast<float2 *>(smem_raw);
+    float2 *bufB = bufA + pc.M;
+    float2 *Tq = bufA + 2 * pc.M;
+    __shared__ float2 zM;
     __shared__ float wave_tot[kStrainThreads / 64];
     __shared__ float edge_x[kStrainThreads + 1];
     const int p = blockIdx.x;
     if (p >= npts) return;
     load_twiddles(twq, pc.M, Tq);
     const int tid = threadIdx.x, nt = pc.nt, M = pc.M, nw = pc.nomega, t0 = tid * SPT;
+    // U of the point: the receivers' parts in order
+    for (int i = tid; i < nF * nw; i += kStrainThreads) {
+        float2 a = make_float2(0.f, 0.f);
+        for (int r = 0; r < R; ++r) a = cadd(a, Ups[((size_t)p * R + r) * nF * nw + i]);
+        Up[(size_t)p * nF * nw + i] = a;
+    }
     float tap[SPT];
     taper_factors(t0, nt, pc, tap);
-    float2 *gF = gFs + (size_t)p * ND * nw;
-    float2 *U = Up + (size_t)p * nF * nw;
-#pragma unroll
-    for (int j = 0; j <= SPT; ++j) {
-        int k;
-        if (!ad_bin<SPT>(j, M, k)) continue;
-#pragma unroll
-        for (int c = 0; c < ND; ++c) gF[(size_t)c * nw + k] = make_float2(0.f, 0.f);
-        for (int f = 0; f < nF; ++f) U[(size_t)f * nw + k] = make_float2(0.f, 0.f);
-    }
-    for (int r = 0; r < R; ++r) {
-        const RecDesc &rd = recs[r];
-        __syncthreads();                    // the previous receiver's slot weights are read
-        for (int i = tid; i < rd.ngroup * 6; i += kStrainThreads) gw[i / 6][i - (i / 6) * 6] = 0.0f;
-        __syncthreads();
-        if (tid < rd.ngroup) {
-            for (int t = 0; t < rd.grp_n[tid]; ++t) {
-                const int s = rd.grp_first[tid] + t;
-                double gs = 0.0;
-                for (int kk = 0; kk < rd.nkernel; ++kk) {
-                    const int k = rd.first_kernel + kk;
-                    for (int b = 0; b < 6; ++b)
-                        if (kern[k].slot[b] == s) gs += gbase[((size_t)p * nk + k) * 6 + b];
-                }
-                gw[tid][rd.slot_bk[s]] = (float)gs;
-            }
-        }
-        const size_t qp = (size_t)p * R + r;
-        float2 spec[ND][SPT + 1];
-        ad_bspec<SPT, ND>(pc, bw + qp * ND * nt, tap, bufA, bufB, Tq, spec);   // (its barriers publish gw)
-#pragma unroll
-        for (int j = 0; j <= SPT; ++j) {
-            int k;
-            if (!ad_bin<SPT>(j, M, k)) continue;
-            float2 f[ND], b[ND], dF[ND], dB[ND];
-#pragma unroll
-            for (int c = 0; c < ND; ++c) { f[c] = Fs[((size_t)p * ND + c) * nw + k]; b[c] = spec[c][j]; }
-            const Bk6 bk = ad_basekernels<ND>(f, b, pc.need_bk);
-            Bk6 Y = bk_zero();
-            for (int g = 0; g < rd.ngroup; ++g) {
-                const int h = rd.grp_h[g];
-                const float2 hg = Hg[(size_t)h * nw + k], w = W[(size_t)h * nw + k];
-                float2 Z = make_float2(0.f, 0.f);
-                auto term = [&](auto bsel) {
-                    constexpr int B = decltype(bsel)::value;
-                    const float gs = gw[g][B];
-                    bk_at<B>(Y) = cadd(bk_at<B>(Y), cscale(gs, hg));
-                    Z = cadd(Z, cscale(gs, bk_at<B>(bk)));
-                };
-                term(std::integral_constant<int, 0>{});
-                term(std::integral_constant<int, 1>{});
-                term(std::integral_constant<int, 3>{});
-                term(std::integral_constant<int, 4>{});
-                term(std::integral_constant<int, 5>{});
-                float2 *u = U + (size_t)hfilt[h] * nw + k;
-                *u = cadd(*u, cmulf(w, Z));
-            }
-            ad_basekernels_adjoint<ND>(f, b, Y, dF, dB);
-#pragma unroll
-            for (int c = 0; c < ND; ++c) {
-                spec[c][j] = cconj(dB[c]);
-                gF[(size_t)c * nw + k] = cadd(gF[(size_t)c * nw + k], cconj(dF[c]));
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < ND; ++c) {
-            float y[SPT];
-            rfft_adjoint<SPT>(spec[c], t0, M, (float)pc.dt, bufA, bufB, &zM, Tq, y);
-            float *dst = gbw + (qp * ND + c) * nt;
-#pragma unroll
-            for (int q = 0; q < SPT; ++q)
-                if (t0 + q < nt) dst[t0 + q] = y[q] * tap[q];
-        }
-    }
-    // forward traces: through the taper and cumsum_trapezoidal
     const float h = (float)(pc.dt * 0.5);
 #pragma unroll 1
-    for (int c = 0; c < ND; ++c) {
+    for (int c = 0; c < nd; ++c) {
         float2 g[SPT + 1];
 #pragma unroll
         for (int j = 0; j <= SPT; ++j) {
             int k;
-            g[j] = ad_bin<SPT>(j, M, k) ? gF[(size_t)c * nw + k] : make_float2(0.f, 0.f);
+            float2 a = make_float2(0.f, 0.f);
+            if (ad_bin<SPT>(j, M, k))
+                for (int r = 0; r < R; ++r) a = cadd(a, dFs[(((size_t)p * R + r) * nd + c) * nw + k]);
+            g[j] = a;
         }
         float y[SPT];
         rfft_adjoint<SPT>(g, t0, M, (float)pc.dt, bufA, bufB, &zM, Tq, y);
 #pragma unroll
-        for (int q = 0; q < SPT; ++q) y[q] *= tap[q];
+        for (int i = 0; i < SPT; ++i) y[i] *= tap[i];
         cumsum_trapezoidal_adjoint<SPT>(y, t0, nt, h, edge_x, wave_tot);
-        float *dst = gfw + ((size_t)p * ND + c) * nt;
+        float *dst = gfw + ((size_t)p * nd + c) * nt;
 #pragma unroll
-        for (int q = 0; q < SPT; ++q)
-            if (t0 + q < nt) dst[t0 + q] = y[q];
+        for (int i = 0; i < SPT; ++i)
+            if (t0 + i < nt) dst[t0 + i] = y[i];
     }
 }
 

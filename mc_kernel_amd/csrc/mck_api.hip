@@ -395,7 +395,8 @@ struct State {
     // set-up, and scratch of the forward and adjoint launches
     int nH = 0;
     std::vector<float> ad_tf_host;           // [nfilter][nomega][2]
-    DBuf<float2> ad_W, ad_Hg, ad_Fs, ad_gF, ad_U;
+    DBuf<float2> ad_W, ad_Hg, ad_Fs, ad_U;
+    DBuf<float2> ad_Bs, ad_dFs, ad_Ups;      // backward stage buffers of a chunk of points
     DBuf<double2> ad_part;
     DBuf<int> ad_hfilt;
 };
@@ -1864,25 +1865,57 @@ hipError_t ad_launch_eval(State &G, int npts, const float *bw, double *base, hip
     return hipErrorInvalidValue;
 }
 
+// The backward stages (k_ad_bstage, k_ad_cstage, k_ad_bgrad, k_ad_fgrad) over chunks of
+// points whose stage buffers (B and dL/dF spectra and U parts of every pair) stay within
+// about 8 GB.
 hipError_t ad_launch_grad(State &G, int npts, const float *bw, const double *gbase, float *gbw,
                           float *gfw, hipStream_t st) {
-    const size_t smem = ad_smem(G.pc_base.M);
-    const int nF = G.S.nfilter;
-#define GRAD(SP, ND)                                                                              \
-    if (spt_for(G.pc_base.nt) == SP && G.pc_base.nd == ND) {                                                \
-        if (smem > 65536)                                                                         \
-            (void)hipFuncSetAttribute((const void *)&k_ad_grad<SP, ND>,                           \
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);     \
-        hipLaunchKernelGGL((k_ad_grad<SP, ND>), dim3(npts), dim3(kStrainThreads), smem, st,       \
-                           G.pc_base, npts, G.R, bw, G.twq.p, G.ad_Fs.p, G.recs_base.p, G.kerns_base.p,         \
-                           G.ad_Hg.p, G.ad_W.p, G.ad_hfilt.p, nF, G.nk, gbase, G.ad_gF.p,        \
-                           G.ad_U.p, gbw, gfw);                                                   \
-        return hipGetLastError();                                                                 \
+    const PathConf &pc = G.pc_base;
+    const int R = G.R, nd = pc.nd, nw = pc.nomega, nt = pc.nt, nF = G.S.nfilter, nk = G.nk;
+    const size_t smem = ad_smem(pc.M);
+    const size_t per_pt = (size_t)R * (2 * (size_t)nd + nF) * nw * sizeof(float2);
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)npts, ((size_t)8 << 30) / per_pt));
+    hipError_t e = G.ad_Bs.ensure((size_t)chunk * R * nd * nw);
+    if (e == hipSuccess) e = G.ad_dFs.ensure((size_t)chunk * R * nd * nw);
+    if (e == hipSuccess) e = G.ad_Ups.ensure((size_t)chunk * R * nF * nw);
+    if (e != hipSuccess) return e;
+    const int spt = spt_for(nt);
+    for (int p0 = 0; p0 < npts && e == hipSuccess; p0 += chunk) {
+        const int np = std::min(chunk, npts - p0), nq = np * R;
+        const float *bwc = bw + (size_t)p0 * R * nd * nt;
+        const double *gbc = gbase + (size_t)p0 * nk * 6;
+        const float2 *Fsc = G.ad_Fs.p + (size_t)p0 * nd * nw;
+        float2 *Upc = G.ad_U.p + (size_t)p0 * nF * nw;
+        float *gbwc = gbw + (size_t)p0 * R * nd * nt, *gfwc = gfw + (size_t)p0 * nd * nt;
+        bool done = false;
+#define STAGES(SP, ND)                                                                                    \
+        if (!done && spt == SP && nd == ND) {                                                             \
+            if (smem > 65536) {                                                                           \
+                (void)hipFuncSetAttribute((const void *)&k_ad_bstage<SP>,                                 \
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);         \
+                (void)hipFuncSetAttribute((const void *)&k_ad_bgrad<SP>,                                  \
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);         \
+                (void)hipFuncSetAttribute((const void *)&k_ad_fgrad<SP>,                                  \
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);         \
+            }                                                                                             \
+            hipLaunchKernelGGL((k_ad_bstage<SP>), dim3(nq), dim3(kStrainThreads), smem, st, pc, nq, nd,   \
+                               bwc, G.twq.p, G.ad_Bs.p);                                                  \
+            hipLaunchKernelGGL((k_ad_cstage<SP, ND>), dim3(nq), dim3(kStrainThreads), 0, st, pc, nq, R,   \
+                               Fsc, G.ad_Bs.p, G.recs_base.p, G.kerns_base.p, G.ad_Hg.p, G.ad_W.p,        \
+                               G.ad_hfilt.p, nF, nk, gbc, G.ad_dFs.p, G.ad_Ups.p);                       \
+            hipLaunchKernelGGL((k_ad_bgrad<SP>), dim3(nq), dim3(kStrainThreads), smem, st, pc, nq, nd,    \
+                               G.ad_Bs.p, G.twq.p, gbwc);                                                 \
+            hipLaunchKernelGGL((k_ad_fgrad<SP>), dim3(np), dim3(kStrainThreads), smem, st, pc, np, R, nd, \
+                               nF, G.ad_dFs.p, G.ad_Ups.p, G.twq.p, Upc, gfwc);                           \
+            e = hipGetLastError();                                                                        \
+            done = true;                                                                                  \
+        }
+        STAGES(1, 1) STAGES(2, 1) STAGES(4, 1) STAGES(8, 1) STAGES(16, 1)
+        STAGES(1, 6) STAGES(2, 6) STAGES(4, 6) STAGES(8, 6) STAGES(16, 6)
+#undef STAGES
+        if (!done) return hipErrorInvalidValue;
     }
-    GRAD(1, 1) GRAD(2, 1) GRAD(4, 1) GRAD(8, 1) GRAD(16, 1)
-    GRAD(1, 6) GRAD(2, 6) GRAD(4, 6) GRAD(8, 6) GRAD(16, 6)
-#undef GRAD
-    return hipErrorInvalidValue;
+    return e;
 }
 
 }  // namespace
@@ -2065,8 +2098,7 @@ int mck_ad_backward(int32_t npts, const float *tf, const float *fw, const float 
         return MCK_OK;
     }
     const int chunk = 256, nchunk = (npts + chunk - 1) / chunk;
-    hipError_t e = G.ad_gF.ensure((size_t)npts * G.pc.nd * nw);
-    if (e == hipSuccess) e = G.ad_U.ensure((size_t)npts * n);
+    hipError_t e = G.ad_U.ensure((size_t)npts * n);
     if (e == hipSuccess) e = G.ad_part.ensure((size_t)nchunk * n);
     if (e == hipSuccess) e = ad_prologue(G, npts, reinterpret_cast<const float2 *>(tf), fw, st);
     if (e == hipSuccess) e = ad_launch_grad(G, npts, bw, gbase, gbw, gfw, st);
@@ -2193,7 +2225,8 @@ void mck_finalize(void) {
     G.abinv.release(); G.bweights.release();
     G.elem_count.release(); G.keys_in.release(); G.keys_out.release();
     G.vals_in.release(); G.vals_out.release(); G.pair_count.release(); G.sort_tmp.release();
-    G.ad_W.release(); G.ad_Hg.release(); G.ad_Fs.release(); G.ad_gF.release(); G.ad_U.release();
+    G.ad_W.release(); G.ad_Hg.release(); G.ad_Fs.release(); G.ad_U.release();
+    G.ad_Bs.release(); G.ad_dFs.release(); G.ad_Ups.release();
     G.ad_part.release(); G.ad_hfilt.release();
     G.init = false;
 }
