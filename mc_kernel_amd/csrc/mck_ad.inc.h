@@ -434,6 +434,8 @@ __global__ __launch_bounds__(kStrainThreads) void k_ad_bstage(PathConf pc, int n
     }
 }
 
+constexpr int kAdRegFilters = 8;
+
 template <int SPT, int ND>
 __global__ __launch_bounds__(kStrainThreads) void k_ad_cstage(
     PathConf pc, int npairs, int R, const float2 *Fs, float2 *Bs, const RecDesc *recs, const KernDesc *kern,
@@ -465,11 +467,18 @@ __global__ __launch_bounds__(kStrainThreads) void k_ad_cstage(
     float2 *B = Bs + (size_t)q * ND * nw;
     float2 *dF = dFs + (size_t)q * ND * nw;
     const float2 *Fp = Fs + (size_t)p * ND * nw;
+    // U parts of up to kAdRegFilters filters accumulate in registers (static indices),
+    // more filters in the pair's U buffer
+    const bool ureg = nF <= kAdRegFilters;
 #pragma unroll 1
     for (int j = 0; j <= SPT; ++j) {
         int k;
         if (!ad_bin<SPT>(j, M, k)) continue;
-        for (int f = 0; f < nF; ++f) U[(size_t)f * nw + k] = make_float2(0.f, 0.f);
+        float2 ua[kAdRegFilters];
+#pragma unroll
+        for (int f = 0; f < kAdRegFilters; ++f) ua[f] = make_float2(0.f, 0.f);
+        if (!ureg)
+            for (int f = 0; f < nF; ++f) U[(size_t)f * nw + k] = make_float2(0.f, 0.f);
         float2 f[ND], b[ND], gF[ND], gB[ND];
 #pragma unroll
         for (int c = 0; c < ND; ++c) { f[c] = Fp[(size_t)c * nw + k]; b[c] = B[(size_t)c * nw + k]; }
@@ -490,9 +499,21 @@ __global__ __launch_bounds__(kStrainThreads) void k_ad_cstage(
             term(std::integral_constant<int, 3>{});
             term(std::integral_constant<int, 4>{});
             term(std::integral_constant<int, 5>{});
-            float2 *u = U + (size_t)hfilt[h] * nw + k;
-            *u = cadd(*u, cmulf(w, Z));
+            const float2 wz = cmulf(w, Z);
+            const int fh = hfilt[h];
+            if (ureg) {
+#pragma unroll
+                for (int ff = 0; ff < kAdRegFilters; ++ff)
+                    if (ff == fh) ua[ff] = cadd(ua[ff], wz);
+            } else {
+                float2 *u = U + (size_t)fh * nw + k;
+                *u = cadd(*u, wz);
+            }
         }
+        if (ureg)
+#pragma unroll
+            for (int ff = 0; ff < kAdRegFilters; ++ff)
+                if (ff < nF) U[(size_t)ff * nw + k] = ua[ff];
         ad_basekernels_adjoint<ND>(f, b, Y, gF, gB);
 #pragma unroll
         for (int c = 0; c < ND; ++c) {
