@@ -90,3 +90,33 @@ def test_rejects_malformed(written, tmp_path):
     bad.write_bytes(b"CDF\x01" + b"\0" * 64)          # a NetCDF-3 header
     with pytest.raises(dbfile.DbIoError, match="cannot open"):
         dbfile.DbFile(str(bad))
+
+
+@built
+def test_fortran_binding_layout(tmp_path):
+    """The iso_c_binding module mc_kernel_amd/fortran/mck_dbio.f90 has the C layout of
+    mck_dbfile_t and links against libmck_dbio.so (a missing file fails with a message)."""
+    import shutil
+    import subprocess
+    flang = shutil.which("flang") or shutil.which("amdflang")
+    if flang is None or not os.path.exists(os.path.join(ROOT, "mc_kernel_amd", "libmck.so")):
+        pytest.skip("flang or libmck.so missing")
+    fdir = os.path.join(ROOT, "mc_kernel_amd", "fortran")
+    for mod in ("mck_gpu.f90", "mck_dbio.f90"):
+        subprocess.run([flang, "-c", os.path.join(fdir, mod), "-o", str(tmp_path / (mod[:-4] + ".o")),
+                        "-module-dir", str(tmp_path)], check=True, cwd=tmp_path)
+    exe = tmp_path / "dbio_layout_check"
+    libdir = os.path.dirname(dbfile.library_path())
+    subprocess.run([flang, os.path.join(ROOT, "tests", "fortran", "dbio_layout_check.f90"),
+                    str(tmp_path / "mck_gpu.o"), str(tmp_path / "mck_dbio.o"), "-I", str(tmp_path),
+                    "-L" + libdir, "-lmck_dbio", "-lmck", "-Wl,-rpath," + libdir, "-o", str(exe)],
+                   check=True, cwd=tmp_path)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True, cwd=tmp_path).stdout
+    got = {}
+    for ln in out.splitlines():
+        parts = ln.split(None, 2)
+        got[parts[0]] = parts[1:]
+    assert int(got["size"][0]) == C.sizeof(dbfile.MckDbFile)
+    for field in ("nt", "dt", "file_version", "stf_dump", "mesh_eta", "mapped", "priv"):
+        assert int(got[field][0]) == getattr(dbfile.MckDbFile, field).offset, field
+    assert int(got["open"][0]) != 0 and "cannot open" in got["open"][1]
