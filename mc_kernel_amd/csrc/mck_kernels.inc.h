@@ -1439,9 +1439,10 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && NDB <= 4) ? 3 : (SPT <
                 bkL[pc.bk_lds[8] * stride + k] = make_float2(bk1.x - 2.0f * bk0.x - bk4.x, bk1.y - 2.0f * bk0.y - bk4.y);
         }
     }
-    __syncthreads();
     // Pass B: sum_k Re(bk_b(k) H_h(k)) in FP64 for every slot (h, b) of the receiver; the
-    // slots are ordered by h so that each window spectrum is read once per bin
+    // slots are ordered by h so that each window spectrum is read once per bin.  A thread
+    // reads only the bins it wrote in pass A (the same k = tid + 256 j, and k = M on the
+    // same thread), so no barrier separates the passes.
     const RecDesc &rd = recs[r];
     const int lane = tid & 63, wave = tid >> 6;
     // window spectra of the next group are loaded while the current one is summed
