@@ -333,6 +333,68 @@ int mck_ad_forward(int32_t npts, const float *tf, const float *fw, const float *
 int mck_ad_backward(int32_t npts, const float *tf, const float *fw, const float *bw,
                     const double *gbase, float *gtf, float *gfw, float *gbw, void *stream);
 
+/*
+ * Result assembly: the master's accumulators K_x, Var, Bg_Model, niterations,
+ * computation_time and element_proc (master_queue.f90:253-323 extract_receive_buffer,
+ * the arrays allocated by init_queue, master_queue.f90:60-219), held in device memory.
+ * They belong to the set-up of mck_init (nkernel columns) and are freed by mck_init and
+ * mck_finalize.
+ *  int_type     0 = 'onvertices' (a basis function's result goes to its vertex,
+ *               connectivity(ibasis, ielement)), 1 = 'volumetric' (to the element)
+ *  npoints      inversion-mesh vertices (onvertices) or elements (volumetric)
+ *  nelements    inversion-mesh elements
+ *  nbasis       basis functions per element (the tasks' nbasis)
+ *  nmodel       0, or 12 to accumulate the background-model integrals
+ *               (INT_OVER_BACKGROUND_MODEL, mck_integrate_model)
+ *  connectivity [nelements][nbasis] 1-based vertex of each basis function (Fortran
+ *               connectivity(nbasis, nelements)); NULL in volumetric mode
+ */
+int mck_results_init(int32_t int_type, int32_t npoints, int32_t nelements, int32_t nbasis,
+                     int32_t nmodel, const int32_t *connectivity);
+
+/*
+ * Adds one task's results, replacing extract_receive_buffer (master_queue.f90:253-323):
+ * K_x(ipoint,:) += kernel_values(:, ibasis, iel), Var likewise, Bg_Model with the model
+ * integrals; niterations(ielement,:), computation_time(ielement) and element_proc(ielement)
+ * = rank are assigned.  Contributions to one point are added in task order (element,
+ * then basis function), as the reference's loop adds them.
+ *  elems           [nelems] 1-based inversion-mesh element of each task slot; -1 = empty
+ *  kernel_values,  as mck_slave_work returns them ([nelems][nbasis][nkernel],
+ *  kernel_variance [nelems][nbasis][nkernel], [nelems][nkernel], [nelems]), host or
+ *  niterations,    device memory.  All four NULL: the results of the last mck_slave_work
+ *  computation_time  call, still in device memory (no host round trip); nelems and
+ *                  nbasis must be that call's.  computation_time alone may be NULL (0 s).
+ *  model           [nelems][nbasis][12] model integrals when nmodel = 12 (host or device;
+ *                  NULL = those of the last mck_integrate_model call, kept in device
+ *                  memory); ignored when nmodel = 0
+ */
+int mck_results_add(int32_t nelems, const int32_t *elems, const double *kernel_values,
+                    const double *kernel_variance, const int32_t *niterations,
+                    const double *computation_time, const double *model, int32_t rank);
+
+/* The accumulators themselves (each nullable), FP64 point-major as the device holds them:
+ * kernel, variance [npoints][nkernel], model [npoints][nmodel], niterations
+ * [nelements][nkernel], computation_time, element_proc [nelements]. */
+int mck_results_get(double *kernel, double *variance, double *model, int32_t *niterations,
+                    double *computation_time, int32_t *element_proc);
+
+/* Replaces the accumulators (each nullable = unchanged), laid out as mck_results_get
+ * returns them: resuming from intermediate results (read_intermediate,
+ * master_queue.f90:615-659) or merging the partial sums of several devices. */
+int mck_results_load(const double *kernel, const double *variance, const double *model,
+                     const int32_t *niterations, const double *computation_time,
+                     const int32_t *element_proc);
+
+#define MCK_RES_KERNEL    0   /* real(K_x, sp)                        [nkernel][npoints]   */
+#define MCK_RES_VARIANCE  1   /* real(Var, sp)                        [nkernel][npoints]   */
+#define MCK_RES_ERROR     2   /* sqrt(Var/|K_x|) where |K_x| > 1e-30, else 1000 (finalize,
+                                 master_queue.f90:383-387)            [nkernel][npoints]   */
+#define MCK_RES_MODEL     3   /* real(Bg_Model, sp)                   [nmodel][npoints]    */
+#define MCK_RES_NITER     4   /* real(niterations, sp)                [nkernel][nelements] */
+/* One output variable of finalize (master_queue.f90:327-581) in single precision, laid
+ * out as the output files hold it: entry-major, i.e. Fortran (npoints, nentries). */
+int mck_results_export(int32_t which, float *out);
+
 /* Statistics of the last mck_slave_work / mck_slave_work_plain call. */
 int mck_get_stats(mck_stats_t *stats);
 

@@ -145,6 +145,11 @@ class MCKernel:
         self.halton_step = int(step[0])
         return model, nsteps
 
+    def results(self, int_type, npoints, nelements, nbasis=1, nmodel=0, connectivity=None):
+        """mck_results_init: the master's accumulators in device memory (results.Results)."""
+        from .results import Results
+        return Results(self, int_type, npoints, nelements, nbasis, nmodel, connectivity)
+
     def seismograms(self):
         """[nrec][nt] seismograms the kernels use (the set-up's, or by reciprocity)."""
         out = np.zeros((self.R, self.nt))

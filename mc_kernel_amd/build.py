@@ -76,7 +76,7 @@ def build_hip(force=True):
 
 def build_oracle(force=True):
     odir = os.path.join(ROOT, "oracle")
-    srcs = [os.path.join(odir, f) for f in ("mck_oracle_core.c", "mck_oracle_path.c")]
+    srcs = [os.path.join(odir, f) for f in ("mck_oracle_core.c", "mck_oracle_path.c", "mck_oracle_results.c")]
     out = os.path.join(odir, "liboracle.so")
     if force or _stale(out, srcs + [os.path.join(odir, "mck_oracle.h")]):
         _run(["gcc", "-O2", "-std=gnu11", "-fPIC", "-shared", "-ffp-contract=off",
@@ -109,7 +109,7 @@ def build_dbio(force=True):
     if not (os.path.exists(os.path.join(inc, "hdf5.h")) and os.path.exists(os.path.join(lib, "libhdf5.so"))):
         print("libmck_dbio.so skipped: no HDF5 under %s" % HDF5_PREFIX, file=sys.stderr)
         return None
-    hdrs = [os.path.join(ROOT, "include", "mck_dbio.h"), os.path.join(ROOT, "include", "mck.h")]
+    hdrs = [os.path.join(ROOT, "include", h) for h in ("mck_dbio.h", "mck_output.h", "mck.h")]
     if force or _stale(out, [src] + hdrs):
         # libhdf5 by path with an rpath for it; libstdc++ linked statically so that the
         # older C++ runtime beside libhdf5 is never picked up in its place

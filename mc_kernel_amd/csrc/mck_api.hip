@@ -23,6 +23,7 @@
 
 #include "mck_kernels.inc.h"
 #include "mck_ad.inc.h"
+#include "mck_results.inc.h"
 
 using namespace mck;
 using cd = std::complex<double>;
@@ -399,6 +400,29 @@ struct State {
     DBuf<float2> ad_Bs, ad_dFs, ad_Ups;      // backward stage buffers of a chunk of points
     DBuf<double2> ad_part;
     DBuf<int> ad_hfilt;
+    // results of the last mck_slave_work (device: integral, variance, niter) and
+    // mck_integrate_model (device: last_model), for mck_results_add without a host copy
+    int last_nelems = 0, last_nbasis = 0, last_model_nelems = 0, last_model_nbasis = 0;
+    std::vector<double> last_ctime;
+    DBuf<double> last_model;
+    // the master's accumulators (mck_results_*)
+    struct Results {
+        bool init = false;
+        int int_type = 0, npoints = 0, nelements = 0, nbasis = 1, nmodel = 0;
+        std::vector<int> conn;               // [nelements][nbasis] 1-based vertices
+        DBuf<double> K, V, M, ctime;         // [npoints][nk], [npoints][nk], [npoints][nmodel]
+        DBuf<int> niter, proc;               // [nelements][nk], [nelements]
+        DBuf<int> seg_start, seg_point, slots, elem_idx;
+        DBuf<double> stage;                  // host inputs staged in device memory
+        DBuf<int> stage_i;
+        void release() {
+            K.release(); V.release(); M.release(); ctime.release(); niter.release(); proc.release();
+            seg_start.release(); seg_point.release(); slots.release(); elem_idx.release();
+            stage.release(); stage_i.release();
+            conn.clear();
+            init = false;
+        }
+    } res;
 };
 State g;
 
@@ -1706,15 +1730,18 @@ int finish_task(State &G, const Task &T, double *values, double *variance, int32
     if (err & 4) return fail(MCK_EINVAL, "basis-function weight outside [0, 1]: point outside its element");
     long long tot_steps = 0;
     for (int e = 0; e < ne; ++e) tot_steps += steps[e];
+    G.last_ctime.assign(ne, 0.0);
     for (int e = 0; e < ne; ++e) {
         if (!T.live[e]) {
             for (int k = 0; k < nb * nk; ++k) { values[(size_t)e * nb * nk + k] = 0; variance[(size_t)e * nb * nk + k] = 0; }
             for (int k = 0; k < nk; ++k) niterations[(size_t)e * nk + k] = 0;
-            if (ctime) ctime[e] = 0.0;
             continue;
         }
-        if (ctime) ctime[e] = tot_steps ? dev_ms * 1e-3 * steps[e] / (double)tot_steps : 0.0;
+        G.last_ctime[e] = tot_steps ? dev_ms * 1e-3 * steps[e] / (double)tot_steps : 0.0;
     }
+    if (ctime) std::copy(G.last_ctime.begin(), G.last_ctime.end(), ctime);
+    G.last_nelems = ne;
+    G.last_nbasis = nb;
     // the Halton stream position after the task
     if (halton_offsets) {
         int last = -1;
@@ -1736,6 +1763,7 @@ int slave_work_impl(bool plain, int32_t elem_type, int32_t nvpe, int32_t nelems,
         return fail(MCK_EINVAL, "null argument");
     auto t0 = std::chrono::steady_clock::now();
     G.stats = mck_stats_t{};
+    G.last_nelems = G.last_nbasis = 0;
     Task T;
     int rc;
     if ((rc = prepare_task(G, T, elem_type, nvpe, nelems, nbasis, nvertices, vertices, conn, halton_step,
@@ -2149,7 +2177,7 @@ int mck_integrate_model(int32_t elem_type, int32_t nvpe, int32_t nelems, int32_t
             if (nbasis > 1) tet_barycentric_inverse(v, &abinv[(size_t)e * 16]);
         }
     }
-    double *d_geom = nullptr, *d_abinv = nullptr, *d_out = nullptr;
+    double *d_geom = nullptr, *d_abinv = nullptr;
     int64_t *d_base = nullptr;
     uint8_t *d_live = nullptr;
     int *d_nsteps = nullptr;
@@ -2157,7 +2185,9 @@ int mck_integrate_model(int32_t elem_type, int32_t nvpe, int32_t nelems, int32_t
     std::vector<int> ns(nelems, 0);
     hipError_t e = hipMalloc((void **)&d_geom, geom.size() * 8);
     if (e == hipSuccess && !abinv.empty()) e = hipMalloc((void **)&d_abinv, abinv.size() * 8);
-    if (e == hipSuccess) e = hipMalloc((void **)&d_out, nout * 8);
+    G.last_model_nelems = G.last_model_nbasis = 0;
+    if (e == hipSuccess) e = G.last_model.ensure(nout);
+    double *d_out = G.last_model.p;
     if (e == hipSuccess) e = hipMalloc((void **)&d_base, (size_t)nelems * 8);
     if (e == hipSuccess) e = hipMalloc((void **)&d_live, (size_t)nelems);
     if (e == hipSuccess) e = hipMalloc((void **)&d_nsteps, (size_t)nelems * 4);
@@ -2177,13 +2207,15 @@ int mck_integrate_model(int32_t elem_type, int32_t nvpe, int32_t nelems, int32_t
     if (e == hipSuccess) e = hipStreamSynchronize(G.stream);
     int err = 0;
     if (e == hipSuccess) e = hipMemcpy(&err, G.err.p, 4, hipMemcpyDeviceToHost);
-    (void)hipFree(d_geom); (void)hipFree(d_abinv); (void)hipFree(d_out);
+    (void)hipFree(d_geom); (void)hipFree(d_abinv);
     (void)hipFree(d_base); (void)hipFree(d_live); (void)hipFree(d_nsteps);
     if (e != hipSuccess) return fail(MCK_EHIP, std::string("mck_integrate_model: ") + hipGetErrorString(e));
     if (err & 4) return fail(MCK_EINVAL, "basis function weight outside [0, 1] (point outside its element)");
     if (err & 8) return fail(MCK_ENAN, "NaN in a model integral (mc_integration.f90:113-130)");
     if (nsteps) std::copy(ns.begin(), ns.end(), nsteps);
     if (!halton_offsets) *halton_step += (int64_t)nelems * kStride;
+    G.last_model_nelems = nelems;
+    G.last_model_nbasis = nbasis;
     return MCK_OK;
 }
 
@@ -2192,6 +2224,235 @@ int mck_seismograms(double *seis) {
     if (!G.init) return fail(MCK_ENOTINIT, "mck_init has not been called");
     if (!seis) return fail(MCK_EINVAL, "null seis");
     std::memcpy(seis, G.seis_host.data(), G.seis_host.size() * sizeof(double));
+    return MCK_OK;
+}
+
+int mck_results_init(int32_t int_type, int32_t npoints, int32_t nelements, int32_t nbasis, int32_t nmodel,
+                     const int32_t *connectivity) {
+    State &G = g;
+    auto &Rs = G.res;
+    if (!G.init) return fail(MCK_ENOTINIT, "mck_init has not been called");
+    if (int_type != 0 && int_type != 1) return fail(MCK_EINVAL, "int_type must be 0 (onvertices) or 1 (volumetric)");
+    if (npoints <= 0 || nelements <= 0 || nbasis <= 0 || (nmodel != 0 && nmodel != kModelPar))
+        return fail(MCK_EINVAL, "invalid result dimensions");
+    if (int_type == 1 && (npoints != nelements || nbasis != 1))
+        return fail(MCK_EINVAL, "volumetric results: one basis function per element and npoints = nelements");
+    if (int_type == 0 && !connectivity) return fail(MCK_EINVAL, "onvertices results need the connectivity");
+    Rs.release();
+    if (int_type == 0) {
+        Rs.conn.assign(connectivity, connectivity + (size_t)nelements * nbasis);
+        for (int v : Rs.conn)
+            if (v < 1 || v > npoints) return fail(MCK_EINVAL, "connectivity out of range");
+    }
+    Rs.int_type = int_type;
+    Rs.npoints = npoints;
+    Rs.nelements = nelements;
+    Rs.nbasis = nbasis;
+    Rs.nmodel = nmodel;
+    const size_t nk = (size_t)G.nk;
+    HIPCHECK(Rs.K.ensure((size_t)npoints * nk));
+    HIPCHECK(Rs.V.ensure((size_t)npoints * nk));
+    HIPCHECK(Rs.M.ensure((size_t)npoints * std::max(nmodel, 1)));
+    HIPCHECK(Rs.niter.ensure((size_t)nelements * nk));
+    HIPCHECK(Rs.ctime.ensure(nelements));
+    HIPCHECK(Rs.proc.ensure(nelements));
+    HIPCHECK(hipMemsetAsync(Rs.K.p, 0, Rs.K.n * 8, G.stream));
+    HIPCHECK(hipMemsetAsync(Rs.V.p, 0, Rs.V.n * 8, G.stream));
+    HIPCHECK(hipMemsetAsync(Rs.M.p, 0, Rs.M.n * 8, G.stream));
+    HIPCHECK(hipMemsetAsync(Rs.niter.p, 0, Rs.niter.n * 4, G.stream));
+    HIPCHECK(hipMemsetAsync(Rs.ctime.p, 0, Rs.ctime.n * 8, G.stream));
+    HIPCHECK(hipMemsetAsync(Rs.proc.p, 0, Rs.proc.n * 4, G.stream));
+    HIPCHECK(hipStreamSynchronize(G.stream));
+    Rs.init = true;
+    return MCK_OK;
+}
+
+int mck_results_add(int32_t nelems, const int32_t *elems, const double *kernel_values,
+                    const double *kernel_variance, const int32_t *niterations, const double *computation_time,
+                    const double *model, int32_t rank) {
+    State &G = g;
+    auto &Rs = G.res;
+    if (!G.init) return fail(MCK_ENOTINIT, "mck_init has not been called");
+    if (!Rs.init) return fail(MCK_ENOTINIT, "mck_results_init has not been called");
+    if (nelems < 0 || (nelems > 0 && !elems)) return fail(MCK_EINVAL, "invalid task elements");
+    if (nelems == 0) return MCK_OK;
+    const int nb = Rs.nbasis, nk = G.nk;
+    const bool last = !kernel_values && !kernel_variance && !niterations && !computation_time;
+    if (!last && (!kernel_values || !kernel_variance || !niterations))
+        return fail(MCK_EINVAL, "kernel_values, kernel_variance and niterations: all given or all NULL");
+    if (last && (G.last_nelems != nelems || G.last_nbasis != nb))
+        return fail(MCK_EINVAL, "no results of a mck_slave_work call with these nelems and nbasis in device memory");
+    const bool with_model = Rs.nmodel > 0;
+    if (with_model && !model && (G.last_model_nelems != nelems || G.last_model_nbasis != nb))
+        return fail(MCK_EINVAL, "no model integrals of a mck_integrate_model call with these nelems and nbasis");
+    // the task's contributions grouped by point, each group in task order
+    std::vector<int> eidx(nelems, -1), pt, slot;
+    pt.reserve((size_t)nelems * nb);
+    slot.reserve((size_t)nelems * nb);
+    for (int iel = 0; iel < nelems; ++iel) {
+        const int e = elems[iel];
+        if (e == -1) continue;
+        if (e < 1 || e > Rs.nelements) return fail(MCK_EINVAL, "element id out of range");
+        eidx[iel] = e - 1;
+        for (int ib = 0; ib < nb; ++ib) {
+            pt.push_back(Rs.int_type == 0 ? Rs.conn[(size_t)(e - 1) * nb + ib] - 1 : e - 1);
+            slot.push_back(iel * nb + ib);
+        }
+    }
+    std::vector<int> order(pt.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return pt[a] < pt[b]; });
+    std::vector<int> seg_start, seg_point, slots(order.size());
+    for (size_t j = 0; j < order.size(); ++j) {
+        if (j == 0 || pt[order[j]] != pt[order[j - 1]]) {
+            seg_start.push_back((int)j);
+            seg_point.push_back(pt[order[j]]);
+        }
+        slots[j] = slot[order[j]];
+    }
+    const int nseg = (int)seg_point.size();
+    seg_start.push_back((int)order.size());
+    hipStream_t st = G.stream;
+    HIPCHECK(Rs.seg_start.ensure(seg_start.size()));
+    HIPCHECK(Rs.seg_point.ensure(std::max(nseg, 1)));
+    HIPCHECK(Rs.slots.ensure(std::max<size_t>(slots.size(), 1)));
+    HIPCHECK(Rs.elem_idx.ensure(nelems));
+    HIPCHECK(hipMemcpyAsync(Rs.seg_start.p, seg_start.data(), seg_start.size() * 4, hipMemcpyHostToDevice, st));
+    if (nseg) {
+        HIPCHECK(hipMemcpyAsync(Rs.seg_point.p, seg_point.data(), (size_t)nseg * 4, hipMemcpyHostToDevice, st));
+        HIPCHECK(hipMemcpyAsync(Rs.slots.p, slots.data(), slots.size() * 4, hipMemcpyHostToDevice, st));
+    }
+    HIPCHECK(hipMemcpyAsync(Rs.elem_idx.p, eidx.data(), (size_t)nelems * 4, hipMemcpyHostToDevice, st));
+    // inputs: in device memory already, or staged
+    const size_t nv = (size_t)nelems * nb * nk, nm = with_model ? (size_t)nelems * nb * kModelPar : 0;
+    const double *dv = G.integral.p, *dvar = G.variance.p, *dm = G.last_model.p, *dct = nullptr;
+    const int *dni = G.niter.p;
+    const std::vector<double> &ct_last = G.last_ctime;
+    {
+        size_t need = 0;
+        const bool sv = !last && !device_ptr(kernel_values), svar = !last && !device_ptr(kernel_variance);
+        const bool sm = with_model && model && !device_ptr(model);
+        const bool sct = last || (computation_time && !device_ptr(computation_time));
+        need = (sv ? nv : 0) + (svar ? nv : 0) + (sm ? nm : 0) + (sct ? nelems : 0);
+        HIPCHECK(Rs.stage.ensure(std::max<size_t>(need, 1)));
+        double *q = Rs.stage.p;
+        auto put = [&](const double *src, size_t n) -> const double * {
+            double *d = q;
+            q += n;
+            return hipMemcpyAsync(d, src, n * 8, hipMemcpyHostToDevice, st) == hipSuccess ? d : nullptr;
+        };
+        if (!last) {
+            dv = sv ? put(kernel_values, nv) : kernel_values;
+            dvar = svar ? put(kernel_variance, nv) : kernel_variance;
+            if (!dv || !dvar) return fail(MCK_EHIP, "mck_results_add: staging copy failed");
+            if (device_ptr(niterations)) {
+                dni = niterations;
+            } else {
+                HIPCHECK(Rs.stage_i.ensure((size_t)nelems * nk));
+                HIPCHECK(hipMemcpyAsync(Rs.stage_i.p, niterations, (size_t)nelems * nk * 4, hipMemcpyHostToDevice, st));
+                dni = Rs.stage_i.p;
+            }
+        }
+        if (with_model && model) {
+            dm = sm ? put(model, nm) : model;
+            if (!dm) return fail(MCK_EHIP, "mck_results_add: staging copy failed");
+        }
+        if (sct) {
+            dct = put(last ? ct_last.data() : computation_time, nelems);
+            if (!dct) return fail(MCK_EHIP, "mck_results_add: staging copy failed");
+        } else if (computation_time) {
+            dct = computation_time;
+        }
+    }
+    if (nseg) {
+        const long long nt1 = (long long)nseg * nk;
+        hipLaunchKernelGGL(k_results_add, dim3((unsigned)((nt1 + 255) / 256)), dim3(256), 0, st, nseg, nk,
+                           Rs.seg_start.p, Rs.seg_point.p, Rs.slots.p, dv, Rs.K.p);
+        hipLaunchKernelGGL(k_results_add, dim3((unsigned)((nt1 + 255) / 256)), dim3(256), 0, st, nseg, nk,
+                           Rs.seg_start.p, Rs.seg_point.p, Rs.slots.p, dvar, Rs.V.p);
+        if (with_model) {
+            const long long nt2 = (long long)nseg * kModelPar;
+            hipLaunchKernelGGL(k_results_add, dim3((unsigned)((nt2 + 255) / 256)), dim3(256), 0, st, nseg,
+                               kModelPar, Rs.seg_start.p, Rs.seg_point.p, Rs.slots.p, dm, Rs.M.p);
+        }
+    }
+    const long long nt3 = (long long)nelems * nk;
+    hipLaunchKernelGGL(k_results_elem, dim3((unsigned)((nt3 + 255) / 256)), dim3(256), 0, st, nelems, nk,
+                       Rs.elem_idx.p, dni, dct, rank, Rs.niter.p, Rs.ctime.p, Rs.proc.p);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(st));
+    return MCK_OK;
+}
+
+int mck_results_get(double *kernel, double *variance, double *model, int32_t *niterations,
+                    double *computation_time, int32_t *element_proc) {
+    State &G = g;
+    auto &Rs = G.res;
+    if (!G.init) return fail(MCK_ENOTINIT, "mck_init has not been called");
+    if (!Rs.init) return fail(MCK_ENOTINIT, "mck_results_init has not been called");
+    const size_t np = Rs.npoints, ne = Rs.nelements, nk = G.nk;
+    if (kernel) HIPCHECK(hipMemcpy(kernel, Rs.K.p, np * nk * 8, hipMemcpyDeviceToHost));
+    if (variance) HIPCHECK(hipMemcpy(variance, Rs.V.p, np * nk * 8, hipMemcpyDeviceToHost));
+    if (model && Rs.nmodel) HIPCHECK(hipMemcpy(model, Rs.M.p, np * Rs.nmodel * 8, hipMemcpyDeviceToHost));
+    if (niterations) HIPCHECK(hipMemcpy(niterations, Rs.niter.p, ne * nk * 4, hipMemcpyDeviceToHost));
+    if (computation_time) HIPCHECK(hipMemcpy(computation_time, Rs.ctime.p, ne * 8, hipMemcpyDeviceToHost));
+    if (element_proc) HIPCHECK(hipMemcpy(element_proc, Rs.proc.p, ne * 4, hipMemcpyDeviceToHost));
+    return MCK_OK;
+}
+
+int mck_results_load(const double *kernel, const double *variance, const double *model,
+                     const int32_t *niterations, const double *computation_time, const int32_t *element_proc) {
+    State &G = g;
+    auto &Rs = G.res;
+    if (!G.init) return fail(MCK_ENOTINIT, "mck_init has not been called");
+    if (!Rs.init) return fail(MCK_ENOTINIT, "mck_results_init has not been called");
+    const size_t np = Rs.npoints, ne = Rs.nelements, nk = G.nk;
+    if (kernel) HIPCHECK(hipMemcpy(Rs.K.p, kernel, np * nk * 8, hipMemcpyDefault));
+    if (variance) HIPCHECK(hipMemcpy(Rs.V.p, variance, np * nk * 8, hipMemcpyDefault));
+    if (model && Rs.nmodel) HIPCHECK(hipMemcpy(Rs.M.p, model, np * Rs.nmodel * 8, hipMemcpyDefault));
+    if (niterations) HIPCHECK(hipMemcpy(Rs.niter.p, niterations, ne * nk * 4, hipMemcpyDefault));
+    if (computation_time) HIPCHECK(hipMemcpy(Rs.ctime.p, computation_time, ne * 8, hipMemcpyDefault));
+    if (element_proc) HIPCHECK(hipMemcpy(Rs.proc.p, element_proc, ne * 4, hipMemcpyDefault));
+    return MCK_OK;
+}
+
+int mck_results_export(int32_t which, float *out) {
+    State &G = g;
+    auto &Rs = G.res;
+    if (!G.init) return fail(MCK_ENOTINIT, "mck_init has not been called");
+    if (!Rs.init) return fail(MCK_ENOTINIT, "mck_results_init has not been called");
+    if (!out) return fail(MCK_EINVAL, "null output");
+    int np = Rs.npoints, ncol = G.nk;
+    const void *src = nullptr;
+    switch (which) {
+    case MCK_RES_KERNEL: src = Rs.K.p; break;
+    case MCK_RES_VARIANCE: src = Rs.V.p; break;
+    case MCK_RES_ERROR: src = Rs.V.p; break;
+    case MCK_RES_MODEL:
+        if (!Rs.nmodel) return fail(MCK_EINVAL, "no model integrals accumulated (nmodel = 0)");
+        src = Rs.M.p;
+        ncol = Rs.nmodel;
+        break;
+    case MCK_RES_NITER: src = Rs.niter.p; np = Rs.nelements; break;
+    default: return fail(MCK_EINVAL, "unknown result variable");
+    }
+    const size_t n = (size_t)np * ncol;
+    const bool dev_out = device_ptr(out);
+    float *d = out;
+    if (!dev_out) {
+        HIPCHECK(Rs.stage.ensure((n * 4 + 7) / 8));
+        d = (float *)Rs.stage.p;
+    }
+    const dim3 grid((np + 31) / 32, (ncol + 31) / 32);
+    if (which == MCK_RES_ERROR)
+        hipLaunchKernelGGL(k_results_out<1>, grid, dim3(256), 0, G.stream, np, ncol, src, Rs.K.p, d);
+    else if (which == MCK_RES_NITER)
+        hipLaunchKernelGGL(k_results_out<2>, grid, dim3(256), 0, G.stream, np, ncol, src, nullptr, d);
+    else
+        hipLaunchKernelGGL(k_results_out<0>, grid, dim3(256), 0, G.stream, np, ncol, src, nullptr, d);
+    HIPCHECK(hipGetLastError());
+    if (!dev_out) HIPCHECK(hipMemcpyAsync(out, d, n * 4, hipMemcpyDeviceToHost, G.stream));
+    HIPCHECK(hipStreamSynchronize(G.stream));
     return MCK_OK;
 }
 
@@ -2228,6 +2489,9 @@ void mck_finalize(void) {
     G.ad_W.release(); G.ad_Hg.release(); G.ad_Fs.release(); G.ad_U.release();
     G.ad_Bs.release(); G.ad_dFs.release(); G.ad_Ups.release();
     G.ad_part.release(); G.ad_hfilt.release();
+    G.last_model.release();
+    G.last_nelems = G.last_nbasis = G.last_model_nelems = G.last_model_nbasis = 0;
+    G.res.release();
     G.init = false;
 }
 

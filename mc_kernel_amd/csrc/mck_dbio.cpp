@@ -1,4 +1,5 @@
-// mck_dbio.cpp - merged AxiSEM database files through HDF5 (see include/mck_dbio.h).
+// mck_dbio.cpp - merged AxiSEM database files through HDF5 (include/mck_dbio.h) and the
+// output files of a kernel run (include/mck_output.h).
 //
 // Readers follow readfields.f90: the root attributes of open_file_read_varids_and_
 // attributes (readfields.f90:499-658), the Mesh group of read_meshes (readfields.f90:
@@ -6,6 +7,7 @@
 // (readfields.f90:2673-2768); the variable layout is the one repack_database.py:151-187
 // writes.  Host code only (g++), no device calls.
 #include "mck_dbio.h"
+#include "mck_output.h"
 
 #include <hdf5.h>
 
@@ -15,7 +17,9 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -359,6 +363,255 @@ int mck_db_write(const char *path, const mck_dbfile_t *f) {
         if ((rc = write_var(mesh.id, v.name, v.ft, v.mt, v.d, v.p))) return rc;
     }
     if (H5Fflush(file.id, H5F_SCOPE_GLOBAL) < 0) return fail(MCK_EINVAL, "cannot flush the file");
+    return MCK_OK;
+}
+
+}  // extern "C"
+
+// =====================================================================================
+// Output files (include/mck_output.h)
+// =====================================================================================
+namespace {
+
+const char *xdmf_topology(const char *t) {
+    // inversion_mesh.f90:2058-2073
+    if (!strcmp(t, "tri")) return "Triangle";
+    if (!strcmp(t, "tet")) return "Tetrahedron";
+    if (!strcmp(t, "quad")) return "Quadrilateral";
+    if (!strcmp(t, "hex") || !strcmp(t, "vox")) return "Hexahedron";
+    return nullptr;
+}
+
+// Fortran E15.8 of a single-precision value: "0.dddddddd" mantissa and a two-digit
+// exponent, right-aligned in 15 characters
+std::string fortran_e15_8(float v) {
+    char buf[64];
+    if (v == 0.0f) {
+        snprintf(buf, sizeof buf, "%15s", "0.00000000E+00");
+        return buf;
+    }
+    char d[32];
+    snprintf(d, sizeof d, "%.7E", (double)std::fabs(v));   // d.dddddddE+xx
+    const int ex = atoi(d + 10) + 1;
+    char m[16];
+    snprintf(m, sizeof m, "0.%c%.7s", d[0], d + 2);
+    char e[16];
+    snprintf(e, sizeof e, "E%c%02d", ex < 0 ? '-' : '+', std::abs(ex));
+    snprintf(buf, sizeof buf, "%15s", (std::string(v < 0 ? "-" : "") + m + e).c_str());
+    return buf;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mck_write_xdmf(const char *filename, const char *element_type, int32_t nvertices, const double *vertices,
+                   int32_t nelements, int32_t nvpe, const int32_t *connectivity, const int32_t *block_id,
+                   int32_t nvars, const mck_outvar_t *vars) {
+    if (!filename || !element_type || !vertices || !connectivity || nvertices <= 0 || nelements <= 0 || nvpe <= 0 ||
+        nvars < 0 || (nvars && !vars))
+        return fail(MCK_EINVAL, "invalid argument");
+    const char *topo = xdmf_topology(element_type);
+    if (!topo) return fail(MCK_EINVAL, std::string("xdmf output for element type '") + element_type + "' not implemented");
+    for (int i = 0; i < nvars; ++i)
+        if (!vars[i].name || !vars[i].values || vars[i].nentries <= 0)
+            return fail(MCK_EINVAL, "incomplete output variable");
+    const std::string base(filename), nc = base + ".nc";
+    const size_t slash = base.rfind('/');
+    const std::string nc_rel = (slash == std::string::npos ? base : base.substr(slash + 1)) + ".nc";
+    {
+        Quiet quiet;
+        Hid file(H5Fcreate(nc.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), H5Fclose);
+        if (file.id < 0) return fail(MCK_EINVAL, "cannot create '" + nc + "'");
+        const hsize_t V = (hsize_t)nvertices, E = (hsize_t)nelements, NV = (hsize_t)nvpe;
+        std::vector<float> pts((size_t)nvertices * 3);
+        for (size_t i = 0; i < pts.size(); ++i) pts[i] = (float)vertices[i];
+        std::vector<int32_t> grid((size_t)nelements * nvpe), block(nelements, 1);
+        for (size_t i = 0; i < grid.size(); ++i) {
+            if (connectivity[i] < 1 || connectivity[i] > nvertices) return fail(MCK_EINVAL, "connectivity out of range");
+            grid[i] = connectivity[i] - 1;
+        }
+        if (block_id) std::copy(block_id, block_id + nelements, block.begin());
+        int rc;
+        if ((rc = write_var(file.id, "points", H5T_IEEE_F32LE, H5T_NATIVE_FLOAT, {V, 3}, pts.data()))) return rc;
+        if ((rc = write_var(file.id, "grid", H5T_STD_I32LE, H5T_NATIVE_INT32, {E, NV}, grid.data()))) return rc;
+        if ((rc = write_var(file.id, "block", H5T_STD_I32LE, H5T_NATIVE_INT32, {E}, block.data()))) return rc;
+        for (int cells = 0; cells < 2; ++cells) {
+            bool any = false;
+            for (int i = 0; i < nvars; ++i) any |= (vars[i].on_cells != 0) == (cells != 0);
+            if (!any) continue;
+            const char *gname = cells ? "cell_data" : "node_data";
+            Hid g(H5Gcreate2(file.id, gname, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), H5Gclose);
+            if (g.id < 0) return fail(MCK_EINVAL, std::string("cannot create group '") + gname + "'");
+            for (int i = 0; i < nvars; ++i) {
+                if ((vars[i].on_cells != 0) != (cells != 0)) continue;
+                if ((rc = write_var(g.id, vars[i].name, H5T_IEEE_F32LE, H5T_NATIVE_FLOAT,
+                                    {(hsize_t)vars[i].nentries, cells ? E : V}, vars[i].values)))
+                    return rc;
+            }
+        }
+        if (H5Fflush(file.id, H5F_SCOPE_GLOBAL) < 0) return fail(MCK_EINVAL, "cannot flush '" + nc + "'");
+    }
+    // the XDMF description (formats 733, 7341, 73421, 73422, 7343, 736 of
+    // inversion_mesh.f90:2163-2232)
+    FILE *x = fopen((base + ".xdmf").c_str(), "w");
+    if (!x) return fail(MCK_EINVAL, "cannot create '" + base + ".xdmf'");
+    fprintf(x,
+            "<?xml version=\"1.0\" ?>\n"
+            "<!DOCTYPE Xdmf SYSTEM \"Xdmf.dtd\" []>\n"
+            "<Xdmf xmlns:xi=\"http://www.w3.org/2003/XInclude\" Version=\"2.2\">\n"
+            "<Domain>\n\n"
+            "<DataItem Name=\"grid\" Dimensions=\"%10d%3d\" NumberType=\"Int\" Format=\"hdf\">\n"
+            "  %s:/grid\n</DataItem>\n\n"
+            "<DataItem Name=\"points\" Dimensions=\"%10d 3\" NumberType=\"Float\" Format=\"hdf\">\n"
+            "  %s:/points\n</DataItem>\n\n"
+            "<DataItem Name=\"block\" Dimensions=\"%10d\" NumberType=\"Int\" Format=\"hdf\">\n"
+            "  %s:/block\n</DataItem>\n\n"
+            "<Grid Name=\"CellsTime\" GridType=\"Collection\" CollectionType=\"Temporal\">\n\n",
+            nelements, nvpe, nc_rel.c_str(), nvertices, nc_rel.c_str(), nelements, nc_rel.c_str());
+    fprintf(x,
+            "    <Grid Name=\"grid\" GridType=\"Uniform\">\n"
+            "        <Time Value=\"%8.2f\" />\n"
+            "        <Topology TopologyType=\"%s\" NumberOfElements=\"%10d\">\n"
+            "            <DataItem Reference=\"XML\">\n"
+            "                /Xdmf/Domain/DataItem[@Name=\"grid\"]\n"
+            "            </DataItem>\n"
+            "        </Topology>\n"
+            "        <Geometry GeometryType=\"XYZ\">\n"
+            "            <DataItem Reference=\"XML\">\n"
+            "                /Xdmf/Domain/DataItem[@Name=\"points\"]\n"
+            "            </DataItem>\n"
+            "        </Geometry>\n"
+            "        <Attribute Name=\"blockid\" AttributeType=\"Scalar\" Center=\"Cell\">\n"
+            "            <DataItem Reference=\"XML\">\n"
+            "                /Xdmf/Domain/DataItem[@Name=\"block\"]\n"
+            "            </DataItem>\n"
+            "        </Attribute>\n",
+            0.0, topo, nelements);
+    // node variables first, then cell variables (inversion_mesh.f90:2092-2118)
+    for (int cells = 0; cells < 2; ++cells)
+        for (int i = 0; i < nvars; ++i) {
+            const mck_outvar_t &v = vars[i];
+            if ((v.on_cells != 0) != (cells != 0)) continue;
+            const int n = cells ? nelements : nvertices;
+            for (int j = 0; j < v.nentries; ++j)
+                fprintf(x,
+                        "        <Attribute Name=\"%s\" AttributeType=\"Scalar\" Center=\"%s\">\n"
+                        "            <DataItem ItemType=\"HyperSlab\" Dimensions=\"%10d\" Type=\"HyperSlab\">\n"
+                        "                <DataItem Dimensions=\"3 2\" Format=\"XML\">\n"
+                        "                    %10d          0 \n"
+                        "                             1          1 \n"
+                        "                             1 %10d\n"
+                        "                </DataItem>\n"
+                        "                <DataItem Dimensions=\"%10d%10d\" NumberType=\"Float\" Format=\"hdf\">\n"
+                        "                   %s:/%s/%s\n"
+                        "                </DataItem>\n"
+                        "            </DataItem>\n"
+                        "        </Attribute>\n",
+                        v.entry_names ? v.entry_names[j] : v.name, cells ? "Cell" : "Node", n, j, n, v.nentries, n,
+                        nc_rel.c_str(), cells ? "cell_data" : "node_data", v.name);
+        }
+    fprintf(x, "    </Grid>\n\n</Grid>\n</Domain>\n</Xdmf>\n");
+    if (fclose(x) != 0) return fail(MCK_EINVAL, "cannot write '" + base + ".xdmf'");
+    return MCK_OK;
+}
+
+int mck_write_csr(const char *filename, int32_t npoints, int32_t nkernel, const float *values, double threshold,
+                  int32_t ascii) {
+    if (!filename || npoints <= 0 || nkernel <= 0 || !values) return fail(MCK_EINVAL, "invalid argument");
+    const std::string base(filename);
+    FILE *fv = fopen((base + ".val").c_str(), ascii ? "w" : "wb");
+    FILE *fi = fopen((base + ".ind").c_str(), ascii ? "w" : "wb");
+    FILE *fp = fopen((base + ".pnt").c_str(), "w");
+    if (!fv || !fi || !fp) {
+        if (fv) fclose(fv);
+        if (fi) fclose(fi);
+        if (fp) fclose(fp);
+        return fail(MCK_EINVAL, "cannot create the files '" + base + ".val/.ind/.pnt'");
+    }
+    // inversion_mesh.f90:1886-2022: kernels outer, points inner, |value| > threshold
+    int32_t n = 0;
+    bool ok = true;
+    for (int k = 0; k < nkernel; ++k) {
+        for (int32_t p = 0; p < npoints; ++p) {
+            const float v = values[(size_t)k * npoints + p];
+            if (!(std::fabs((double)v) > threshold)) continue;
+            ++n;
+            const int32_t idx = p + 1;
+            if (ascii) {
+                ok &= fprintf(fv, " %s", fortran_e15_8(v).c_str()) > 0;
+                ok &= fprintf(fi, " %15d", idx) > 0;
+            } else {
+                ok &= fwrite(&v, 4, 1, fv) == 1;
+                ok &= fwrite(&idx, 4, 1, fi) == 1;
+            }
+        }
+        if (ascii) {
+            ok &= fputc('\n', fv) != EOF;
+            ok &= fputc('\n', fi) != EOF;
+        }
+        // list-directed integer: a blank and the value in 11 columns
+        ok &= fprintf(fp, "%12d\n", n) > 0;
+    }
+    ok &= fclose(fv) == 0;
+    ok &= fclose(fi) == 0;
+    ok &= fclose(fp) == 0;
+    return ok ? MCK_OK : fail(MCK_EINVAL, "cannot write the files '" + base + ".val/.ind/.pnt'");
+}
+
+int mck_write_intermediate(const char *path, int32_t npoints, int32_t nelements, int32_t nkernel, int32_t nmodel,
+                           int32_t nhet, const float *kernel, const float *variance, const float *bg_model,
+                           const float *het_model, const int32_t *niterations, const int32_t *element_proc) {
+    if (!path || npoints <= 0 || nelements <= 0 || nkernel <= 0 || nmodel < 0 || nhet < 0 || !kernel ||
+        !variance || !niterations || !element_proc || (nmodel && !bg_model) || (nhet && !het_model))
+        return fail(MCK_EINVAL, "invalid argument");
+    Quiet quiet;
+    Hid file(H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), H5Fclose);
+    if (file.id < 0) return fail(MCK_EINVAL, std::string("cannot create '") + path + "'");
+    const hsize_t P = (hsize_t)npoints, E = (hsize_t)nelements, K = (hsize_t)nkernel;
+    int rc;
+    if ((rc = write_var(file.id, "K_x", H5T_IEEE_F32LE, H5T_NATIVE_FLOAT, {K, P}, kernel))) return rc;
+    if ((rc = write_var(file.id, "Var", H5T_IEEE_F32LE, H5T_NATIVE_FLOAT, {K, P}, variance))) return rc;
+    if (nmodel && (rc = write_var(file.id, "Bg_Model", H5T_IEEE_F32LE, H5T_NATIVE_FLOAT, {(hsize_t)nmodel, P},
+                                  bg_model)))
+        return rc;
+    if (nhet && (rc = write_var(file.id, "Het_Model", H5T_IEEE_F32LE, H5T_NATIVE_FLOAT, {(hsize_t)nhet, P},
+                                het_model)))
+        return rc;
+    if ((rc = write_var(file.id, "niterations", H5T_STD_I32LE, H5T_NATIVE_INT32, {K, E}, niterations))) return rc;
+    if ((rc = write_var(file.id, "element_proc", H5T_STD_I32LE, H5T_NATIVE_INT32, {E}, element_proc))) return rc;
+    if (H5Fflush(file.id, H5F_SCOPE_GLOBAL) < 0) return fail(MCK_EINVAL, "cannot flush the file");
+    return MCK_OK;
+}
+
+int mck_read_intermediate(const char *path, int32_t npoints, int32_t nelements, int32_t nkernel, int32_t nmodel,
+                          int32_t nhet, float *kernel, float *variance, float *bg_model, float *het_model,
+                          int32_t *niterations, int32_t *element_proc) {
+    if (!path || npoints <= 0 || nelements <= 0 || nkernel <= 0 || nmodel < 0 || nhet < 0)
+        return fail(MCK_EINVAL, "invalid argument");
+    Quiet quiet;
+    Hid file(H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT), H5Fclose);
+    if (file.id < 0) return fail(MCK_EINVAL, std::string("cannot open '") + path + "'");
+    const hsize_t P = (hsize_t)npoints, E = (hsize_t)nelements, K = (hsize_t)nkernel;
+    std::vector<float> f;
+    std::vector<int32_t> iv;
+    int rc;
+    struct FV { const char *name; std::vector<hsize_t> d; float *out; };
+    for (const FV &v : {FV{"K_x", {K, P}, kernel}, FV{"Var", {K, P}, variance},
+                        FV{"Bg_Model", {(hsize_t)nmodel, P}, nmodel ? bg_model : nullptr},
+                        FV{"Het_Model", {(hsize_t)nhet, P}, nhet ? het_model : nullptr}}) {
+        if (!v.out) continue;
+        if ((rc = read_var(file.id, v.name, H5T_NATIVE_FLOAT, v.d, f))) return rc;
+        std::copy(f.begin(), f.end(), v.out);
+    }
+    if (niterations) {
+        if ((rc = read_var(file.id, "niterations", H5T_NATIVE_INT32, {K, E}, iv))) return rc;
+        std::copy(iv.begin(), iv.end(), niterations);
+    }
+    if (element_proc) {
+        if ((rc = read_var(file.id, "element_proc", H5T_NATIVE_INT32, {E}, iv))) return rc;
+        std::copy(iv.begin(), iv.end(), element_proc);
+    }
     return MCK_OK;
 }
 

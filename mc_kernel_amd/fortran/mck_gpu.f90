@@ -18,6 +18,8 @@ module mck_gpu
        MCK_PARAM_VS = 2, MCK_PARAM_MU = 3, MCK_PARAM_VSH = 4, MCK_PARAM_VSV = 5, &
        MCK_PARAM_VPH = 6, MCK_PARAM_VPV = 7, MCK_PARAM_ETA = 8, MCK_PARAM_RHO = 9
   integer(c_int32_t), parameter, public :: MCK_MISFIT_CC = 0, MCK_MISFIT_AM = 1
+  integer(c_int32_t), parameter, public :: MCK_RES_KERNEL = 0, MCK_RES_VARIANCE = 1, &
+       MCK_RES_ERROR = 2, MCK_RES_MODEL = 3, MCK_RES_NITER = 4
   integer(c_int32_t), parameter, public :: MCK_INT_PARSEVAL = 0, MCK_INT_TRAPEZOIDAL = 1
   integer(c_int32_t), parameter, public :: MCK_FILTER_IDENT = 0, MCK_FILTER_GABOR = 1, &
        MCK_FILTER_BUTTER_LP = 2, MCK_FILTER_BUTTER_HP = 3, MCK_FILTER_BUTTER_BP = 4, &
@@ -74,6 +76,8 @@ module mck_gpu
 
   public :: mck_init, mck_slave_work, mck_slave_work_plain, mck_locate_points, &
             mck_load_strain_points, mck_base_kernels, mck_seismograms, mck_integrate_model, &
+            mck_results_init, mck_results_add, mck_results_get, mck_results_load, &
+            mck_results_export, &
             mck_get_stats, mck_finalize, &
             mck_last_error, mck_error_message, mck_check
 
@@ -156,6 +160,50 @@ module mck_gpu
        integer(c_int64_t), intent(inout) :: halton_step
        type(c_ptr), value :: halton_offsets
      end function mck_integrate_model
+
+     ! the master's accumulators in device memory (extract_receive_buffer,
+     ! master_queue.f90:253-323); connectivity(nbasis, nelements), c_null_ptr when volumetric
+     integer(c_int) function mck_results_init(int_type, npoints, nelements, nbasis, nmodel, &
+                                              connectivity) bind(c, name='mck_results_init')
+       import :: c_int, c_int32_t, c_ptr
+       integer(c_int32_t), value :: int_type, npoints, nelements, nbasis, nmodel
+       type(c_ptr), value :: connectivity
+     end function mck_results_init
+
+     ! elems(nelems) global element ids (-1 = empty slot); the result arrays as
+     ! mck_slave_work returns them (host or device addresses), all c_null_ptr = the
+     ! last mck_slave_work's results in device memory
+     integer(c_int) function mck_results_add(nelems, elems, kernel_values, kernel_variance, &
+                                             niterations, computation_time, model, rank) &
+                                             bind(c, name='mck_results_add')
+       import :: c_int, c_int32_t, c_ptr
+       integer(c_int32_t), value :: nelems, rank
+       integer(c_int32_t), intent(in) :: elems(*)
+       type(c_ptr), value :: kernel_values, kernel_variance, niterations, computation_time, model
+     end function mck_results_add
+
+     ! K_x, Var (nkernel, npoints), Bg_Model (nmodel, npoints), niterations
+     ! (nkernel, nelements), computation_time, element_proc (nelements): c_null_ptr skips
+     integer(c_int) function mck_results_get(kernel, variance, model, niterations, &
+                                             computation_time, element_proc) &
+                                             bind(c, name='mck_results_get')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: kernel, variance, model, niterations, computation_time, element_proc
+     end function mck_results_get
+
+     integer(c_int) function mck_results_load(kernel, variance, model, niterations, &
+                                              computation_time, element_proc) &
+                                              bind(c, name='mck_results_load')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: kernel, variance, model, niterations, computation_time, element_proc
+     end function mck_results_load
+
+     ! finalize's single-precision variables, out(npoints, nentries) as the files hold them
+     integer(c_int) function mck_results_export(which, out) bind(c, name='mck_results_export')
+       import :: c_int, c_int32_t, c_float
+       integer(c_int32_t), value :: which
+       real(c_float), intent(out) :: out(*)
+     end function mck_results_export
 
      integer(c_int) function mck_get_stats(stats) bind(c, name='mck_get_stats')
        import :: c_int, mck_stats_t

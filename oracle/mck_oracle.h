@@ -116,6 +116,18 @@ int      ora_seismograms(ora_ctx *ctx, double *seis);
 int      ora_integrate_model(ora_ctx *ctx, int elem_type, int nvpe, int nelems, int nbasis,
                              const double *vertices, const int32_t *conn, double *model,
                              int32_t *nsteps, int64_t *halton_step, const int64_t *halton_offsets);
+/* extract_receive_buffer (master_queue.f90:253-323) for one task, serial sums into
+ * K, V [npoints][nk], M [npoints][nmodel], niter_g [nelements][nk], ctime_g, proc
+ * [nelements]; returns nonzero for an element or point id out of range */
+int      ora_results_add(int int_type, int nbasis, int nk, int nmodel, int npoints, int nelements,
+                         const int32_t *conn, int nelems, const int32_t *elems, const double *values,
+                         const double *variance, const int32_t *niter, const double *ctime,
+                         const double *model, int rank, double *K, double *V, double *M,
+                         int32_t *niter_g, double *ctime_g, int32_t *proc);
+/* finalize's sp conversions (MCK_RES_*): a [np][ncol] -> out [ncol][np]; kx for the error,
+ * ia for niterations */
+void     ora_results_export(int which, int np, int ncol, const double *a, const double *kx,
+                            const int32_t *ia, float *out);
 /* Test hook: which 2 = xyz_earth -> xyz_src (rotations.f90:615-671), 3 = xyz -> src
  * (rotations.f90:423-455); t[6] Voigt, in place. */
 void     ora_rotate_voigt_inv(int which, double *t, double phi, double theta);

@@ -84,6 +84,10 @@ def lib():
         L.ora_seismograms.argtypes = [C.c_void_p, f64p]
         L.ora_integrate_model.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, f64p, i32p,
                                           f64p, i32p, i64p, i64p]
+        L.ora_results_add.argtypes = [C.c_int] * 6 + [i32p, C.c_int, i32p, f64p, f64p, i32p, f64p, f64p,
+                                                       C.c_int, f64p, f64p, f64p, i32p, f64p, i32p]
+        L.ora_results_export.argtypes = [C.c_int, C.c_int, C.c_int, f64p, f64p, i32p, f32p]
+        L.ora_results_export.restype = None
         L.ora_ad_transfer.argtypes = [C.c_void_p, f64p]
         L.ora_ad_forward.argtypes = [C.c_void_p, C.c_int, f64p, f64p, f64p, f64p]
         L.ora_ad_backward.argtypes = [C.c_void_p, C.c_int, f64p, f64p, f64p, f64p, f64p, f64p, f64p]
@@ -475,3 +479,48 @@ class Oracle:
         if rc:
             raise RuntimeError("oracle ad_backward failed (%d)" % rc)
         return gG[..., 0] + 1j * gG[..., 1], gfw, gbw
+
+
+class Results:
+    """The master's accumulators on the CPU (ora_results_add / ora_results_export),
+    the checker of mck_results_*: K, V [npoints][nk], M [npoints][nmodel], niter
+    [nelements][nk], ctime, proc [nelements]."""
+
+    def __init__(self, int_type, npoints, nelements, nbasis, nk, nmodel=0, connectivity=None):
+        self.int_type, self.np, self.ne, self.nb, self.nk, self.nm = int_type, npoints, nelements, nbasis, nk, nmodel
+        self.conn = None if connectivity is None else np.ascontiguousarray(connectivity, np.int32)
+        self.K = np.zeros((npoints, nk))
+        self.V = np.zeros((npoints, nk))
+        self.M = np.zeros((npoints, max(nmodel, 1)))
+        self.niter = np.zeros((nelements, nk), np.int32)
+        self.ctime = np.zeros(nelements)
+        self.proc = np.zeros(nelements, np.int32)
+
+    def add(self, elems, values, variance, niterations, computation_time=None, model=None, rank=1):
+        elems = np.ascontiguousarray(elems, np.int32)
+        values = np.ascontiguousarray(values, float)
+        variance = np.ascontiguousarray(variance, float)
+        niterations = np.ascontiguousarray(niterations, np.int32)
+        ct = None if computation_time is None else np.ascontiguousarray(computation_time, float)
+        md = None if model is None else np.ascontiguousarray(model, float)
+        rc = lib().ora_results_add(self.int_type, self.nb, self.nk, self.nm, self.np, self.ne,
+                                   None if self.conn is None else P(self.conn, C.c_int32), len(elems),
+                                   P(elems, C.c_int32), P(values), P(variance), P(niterations, C.c_int32),
+                                   None if ct is None else P(ct), None if md is None else P(md), rank,
+                                   P(self.K), P(self.V), P(self.M), P(self.niter, C.c_int32), P(self.ctime),
+                                   P(self.proc, C.c_int32))
+        if rc:
+            raise RuntimeError("oracle results_add: id out of range")
+
+    def export(self, which):
+        from mc_kernel_amd import abi as A
+        if which == A.MCK_RES_NITER:
+            a, ncol, n = None, self.nk, self.ne
+        elif which == A.MCK_RES_MODEL:
+            a, ncol, n = self.M, self.nm, self.np
+        else:
+            a, ncol, n = (self.K if which == A.MCK_RES_KERNEL else self.V), self.nk, self.np
+        out = np.zeros((ncol, n), np.float32)
+        lib().ora_results_export(which, n, ncol, None if a is None else P(np.ascontiguousarray(a)),
+                                 P(self.K), P(self.niter, C.c_int32), P(out, C.c_float))
+        return out

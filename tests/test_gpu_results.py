@@ -1,0 +1,181 @@
+"""Result assembly on the GPU (mck_results_*, extract_receive_buffer master_queue.f90:
+253-323 and finalize's conversions master_queue.f90:327-581) against the oracle (pinned
+in tests/test_oracle_results.py), bit for bit: both add each point's contributions to
+its FP64 accumulator one at a time in task order.
+
+  * several random tasks with shared vertices, empty slots and recomputed elements,
+    4 basis functions, model integrals: accumulators and all five exported variables;
+  * volumetric mode;
+  * the results of the last slave_work / integrate_model taken from device memory give
+    the same accumulators as the same results passed from the host; device tensors as
+    inputs;
+  * load / get round trip (resume), argument checks;
+  * finalize's files from the device accumulators of a slave_work task, and a resume
+    from the intermediate results written from them.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from mc_kernel_amd import abi, synth
+
+pytestmark = pytest.mark.gpu
+
+EXPORTS = [abi.MCK_RES_KERNEL, abi.MCK_RES_VARIANCE, abi.MCK_RES_ERROR, abi.MCK_RES_MODEL, abi.MCK_RES_NITER]
+
+
+def _kernel():
+    import mc_kernel_amd as mk
+    p = synth.make_problem("tiny", ntet=4)
+    return mk.MCKernel(p.setup), p
+
+
+def _tasks(rng, ne, nk, nb, ntask=5, size=40):
+    out = []
+    for _ in range(ntask):
+        t = (rng.permutation(ne)[:size] + 1).astype(np.int32)
+        t[rng.random(size) < 0.1] = -1
+        vals = rng.standard_normal((size, nb, nk)) * 10.0 ** rng.integers(-3, 3, (size, nb, 1))
+        var = np.abs(rng.standard_normal((size, nb, nk)))
+        nit = rng.integers(1, 64, (size, nk)).astype(np.int32)
+        ct = rng.random(size)
+        model = rng.standard_normal((size, nb, 12))
+        out.append((t, vals, var, nit, ct, model, int(rng.integers(1, 9))))
+    return out
+
+
+def _compare(res, ref):
+    got = res.get()
+    np.testing.assert_array_equal(got["kernel"], ref.K)
+    np.testing.assert_array_equal(got["variance"], ref.V)
+    if ref.nm:
+        np.testing.assert_array_equal(got["model"], ref.M)
+    np.testing.assert_array_equal(got["niterations"], ref.niter)
+    np.testing.assert_array_equal(got["computation_time"], ref.ctime)
+    np.testing.assert_array_equal(got["element_proc"], ref.proc)
+    for w in EXPORTS:
+        if w == abi.MCK_RES_MODEL and not ref.nm:
+            continue
+        np.testing.assert_array_equal(res.export(w), ref.export(w), err_msg=str(w))
+
+
+@pytest.mark.parametrize("int_type", [0, 1])
+def test_random_tasks_match_oracle(int_type):
+    from oracle import oracle
+    print("results case int_type", int_type, flush=True)
+    k, p = _kernel()
+    try:
+        rng = np.random.default_rng(11 + int_type)
+        nv, ne, nk = 300, 500, k.nk
+        nb = 4 if int_type == 0 else 1
+        npt = nv if int_type == 0 else ne
+        conn = np.array([rng.choice(nv, 4, replace=False) + 1 for _ in range(ne)], np.int32)
+        c = conn if int_type == 0 else None
+        res = k.results(int_type, npt, ne, nb, 12, c)
+        ref = oracle.Results(int_type, npt, ne, nb, nk, 12, c)
+        for t, vals, var, nit, ct, model, rank in _tasks(rng, ne, nk, nb):
+            res.add(t, vals, var, nit, ct, model, rank)
+            ref.add(t, vals, var, nit, ct, model, rank)
+        _compare(res, ref)
+    finally:
+        k.close()
+
+
+def test_device_resident_results():
+    k, p = _kernel()
+    try:
+        out = k.slave_work(p.verts, p.conn, nbasis=4)
+        model, _ = k.integrate_model(p.verts, p.conn, nbasis=4)
+        ne = len(p.conn)
+        nv = np.asarray(p.verts).reshape(-1, 3).shape[0]
+        elems = np.arange(1, ne + 1, dtype=np.int32)
+        a = k.results(0, nv, ne, 4, 12, p.conn)
+        a.add(elems, rank=3)                              # last slave_work + integrate_model
+        ga = a.get()
+        b = k.results(0, nv, ne, 4, 12, p.conn)
+        b.add(elems, out["values"], out["variance"], out["niterations"], out["computation_time"], model, rank=3)
+        gb = b.get()
+        for key in ga:
+            np.testing.assert_array_equal(ga[key], gb[key], err_msg=key)
+        assert np.abs(ga["kernel"]).max() > 0
+        # device tensors as inputs
+        import torch
+        dev = [torch.from_numpy(np.ascontiguousarray(x)).cuda() for x in
+               (out["values"], out["variance"], out["niterations"], out["computation_time"], model)]
+        torch.cuda.synchronize()
+        c = k.results(0, nv, ne, 4, 12, p.conn)
+        c.add(elems, *dev, rank=3)
+        gc = c.get()
+        for key in ga:
+            np.testing.assert_array_equal(gc[key], gb[key], err_msg=key)
+    finally:
+        k.close()
+
+
+def test_load_get_and_checks():
+    import mc_kernel_amd as mk
+    k, p = _kernel()
+    try:
+        rng = np.random.default_rng(5)
+        res = k.results(1, 50, 50, 1, 0)
+        arrays = dict(kernel=rng.standard_normal((50, k.nk)), variance=rng.random((50, k.nk)),
+                      niterations=rng.integers(0, 9, (50, k.nk)).astype(np.int32),
+                      computation_time=rng.random(50), element_proc=rng.integers(1, 4, 50).astype(np.int32))
+        res.load(**arrays)
+        got = res.get()
+        for key, v in arrays.items():
+            np.testing.assert_array_equal(got[key], v, err_msg=key)
+        with pytest.raises(mk.MckError):                  # element id out of range
+            res.add([51], np.zeros((1, 1, k.nk)), np.zeros((1, 1, k.nk)), np.zeros((1, k.nk), np.int32))
+        with pytest.raises(mk.MckError):                  # no slave_work of that size yet
+            res.add(np.arange(1, 8, dtype=np.int32))
+        with pytest.raises(mk.MckError):                  # no model integrals accumulated
+            res.export(abi.MCK_RES_MODEL)
+        with pytest.raises(mk.MckError):                  # onvertices needs the connectivity
+            k.results(0, 10, 10, 4, 0)
+    finally:
+        k.close()
+
+
+def test_finalize_files_and_resume(tmp_path):
+    """slave_work -> device accumulators -> finalize's files; the intermediate results
+    written from them resume into fresh accumulators (read_intermediate converts the
+    single-precision file back to double, master_queue.f90:628-650)."""
+    import shutil
+    import subprocess
+    from mc_kernel_amd import output
+    k, p = _kernel()
+    try:
+        k.slave_work(p.verts, p.conn, nbasis=4)
+        ne = len(p.conn)
+        verts = np.asarray(p.verts, float).reshape(-1, 3)
+        nv = verts.shape[0]
+        res = k.results(0, nv, ne, 4, 0, p.conn)
+        res.add(np.arange(1, ne + 1, dtype=np.int32), rank=1)
+        names = ["k%d" % i for i in range(k.nk)]
+        base = str(tmp_path / "run")
+        var = output.finalize(res, base, names, "tet", verts, p.conn)
+        assert [v[0] for v in var] == ["kernel", "error", "niterations", "computation_time", "element_proc", "volume"]
+        assert os.path.exists(base + "_kernel.xdmf")
+        h5dump = shutil.which("h5dump") or "/opt/conda/bin/h5dump"
+        if os.path.exists(h5dump):
+            out = str(tmp_path / "k.bin")
+            subprocess.run([h5dump, "-d", "/node_data/kernel", "-b", "LE", "-o", out, base + "_kernel.nc"],
+                           check=True, capture_output=True)
+            np.testing.assert_array_equal(np.fromfile(out, np.float32), res.export(abi.MCK_RES_KERNEL).ravel())
+        raw = res.get()
+        path = str(tmp_path / "intermediate_results.nc")
+        output.write_intermediate(path, res.export(abi.MCK_RES_KERNEL), res.export(abi.MCK_RES_VARIANCE),
+                                  raw["niterations"].T, raw["element_proc"])
+        back = output.read_intermediate(path, nv, ne, k.nk)
+        assert back["completed"].all()
+        res2 = k.results(0, nv, ne, 4, 0, p.conn)
+        res2.load(kernel=back["kernel"].T.astype(np.float64), variance=back["variance"].T.astype(np.float64),
+                  niterations=back["niterations"].T, element_proc=back["element_proc"])
+        got = res2.get()
+        np.testing.assert_array_equal(got["kernel"], raw["kernel"].astype(np.float32).astype(np.float64))
+        np.testing.assert_array_equal(got["niterations"], raw["niterations"])
+        np.testing.assert_array_equal(res2.export(abi.MCK_RES_KERNEL), res.export(abi.MCK_RES_KERNEL))
+    finally:
+        k.close()
