@@ -218,6 +218,72 @@ def bench_autograd(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_assembly(args):
+    """Result assembly at the c4 scale (0.8M voxels, 1600 kernels; tasks of
+    --assembly-task voxels, volumetric): each step adds one task's kernel values and
+    variances, already in device memory, to the device-resident K_x and Var
+    (mck_results_add, extract_receive_buffer master_queue.f90:253-323)."""
+    import torch
+    import mc_kernel_amd as mk
+    from mc_kernel_amd import synth
+    from mc_kernel_amd.results import VOLUMETRIC
+    npoints, nk, task = args.assembly_points, 1600, args.assembly_task
+    p = synth.make_problem("tiny", receivers_deg=[20.0 + i for i in range(50)],
+                           filters=[("Gabor", (30.0 + 5.0 * i, 0.5, 0.0, 0.0)) for i in range(8)],
+                           kernels=synth._kernels(50, 8, ["CC", "AM"], ["vp", "vs"]))
+    assert p.nkernel == nk
+    cpu = None
+    if not args.no_cpu:
+        # the serial accumulation of the reference's master loop, on a bounded sample
+        from oracle import oracle
+        rng = np.random.default_rng(0)
+        ncpu = min(task, 20000)
+        ref = oracle.Results(VOLUMETRIC, ncpu, ncpu, 1, nk)
+        vals = rng.standard_normal((ncpu, 1, nk))
+        nit = np.ones((ncpu, nk), np.int32)
+        elems = np.arange(1, ncpu + 1, dtype=np.int32)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            ref.add(elems, vals, vals, nit)
+        dt = (time.perf_counter() - t0) / 3
+        cpu = {"value": ncpu * nk / dt, "unit": "kernel values/s", "cores": 1, "kind": "port",
+               "sample": "3 tasks of %d voxels x %d kernels into %d-voxel accumulators (FP64 oracle, serial)"
+                         % (ncpu, nk, ncpu)}
+    k = mk.MCKernel(p.setup)
+    res = k.results(VOLUMETRIC, npoints, npoints, 1, 0)
+    g = torch.Generator(device="cuda").manual_seed(1)
+    vals = torch.randn((task, 1, nk), dtype=torch.float64, device="cuda", generator=g)
+    var = vals.abs()
+    nit = torch.ones((task, nk), dtype=torch.int32, device="cuda")
+    ct = torch.zeros(task, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    ntask = npoints // task
+    times = []
+    for i in range(args.warmup + args.steps):
+        elems = (np.arange(task, dtype=np.int32) + (i % ntask) * task + 1).astype(np.int32)
+        t0 = time.perf_counter()
+        res.add(elems, vals, var, nit, ct, None, 1)
+        t1 = time.perf_counter()
+        if i >= args.warmup:
+            times.append(t1 - t0)
+        print("assembly step %d: %.3f ms" % (i, (t1 - t0) * 1e3), file=sys.stderr, flush=True)
+    k.close()
+    ms = float(np.mean(times)) * 1e3
+    # algorithmic bytes of one k_results_add launch (K_x and Var together): the task's
+    # values and variances read once, both accumulators' rows read and written once
+    launch_bytes = 2 * task * nk * 8 * 3
+    line = {
+        "metric": "result assembly: kernel values accumulated per second (K_x and Var)",
+        "value": task * nk / (ms * 1e-3), "unit": "kernel values/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic task results in device memory",
+        "config": {"workload": "c4 accumulators: %d voxels x %d kernels, volumetric; tasks of %d voxels"
+                               % (npoints, nk, task)},
+        "k_results_add_algorithmic_bytes": launch_bytes, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -240,6 +306,10 @@ def main():
                     help="time the differentiable filter-convolve-window-integrate stage at "
                          "the c3 size instead (forward and adjoint, one JSON line)")
     ap.add_argument("--autograd-points", type=int, default=10000)
+    ap.add_argument("--assembly", action="store_true",
+                    help="time the result assembly at the c4 scale instead (one JSON line)")
+    ap.add_argument("--assembly-points", type=int, default=800000)
+    ap.add_argument("--assembly-task", type=int, default=50000)
     ap.add_argument("--db-page-elems", type=int, default=0,
                     help="keep the databases in host memory and page them, N elements per "
                          "page (0: resident in HBM, the default workload)")
@@ -249,12 +319,15 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
-              "(voxel-parallel multi-GPU sharding is described in DESIGN.md)" % args.gpus,
+              "(the path would shard by voxels, one process per GPU; not built here)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
 
     if args.autograd:
         bench_autograd(args)
+        return
+    if args.assembly:
+        bench_assembly(args)
         return
 
     # CPU baseline first, in fresh processes, before this process touches the GPU

@@ -412,13 +412,16 @@ struct State {
         std::vector<int> conn;               // [nelements][nbasis] 1-based vertices
         DBuf<double> K, V, M, ctime;         // [npoints][nk], [npoints][nk], [npoints][nmodel]
         DBuf<int> niter, proc;               // [nelements][nk], [nelements]
-        DBuf<int> seg_start, seg_point, slots, elem_idx;
+        DBuf<int> dconn, elems, pt_in, pt_out, sl_in, sl_out, seg_point, seg_count, seg_start, nseg, elem_idx;
+        DBuf<unsigned char> tmp;             // hipcub scratch
         DBuf<double> stage;                  // host inputs staged in device memory
         DBuf<int> stage_i;
         void release() {
             K.release(); V.release(); M.release(); ctime.release(); niter.release(); proc.release();
-            seg_start.release(); seg_point.release(); slots.release(); elem_idx.release();
-            stage.release(); stage_i.release();
+            for (DBuf<int> *b : {&dconn, &elems, &pt_in, &pt_out, &sl_in, &sl_out, &seg_point, &seg_count,
+                                 &seg_start, &nseg, &elem_idx})
+                b->release();
+            tmp.release(); stage.release(); stage_i.release();
             conn.clear();
             init = false;
         }
@@ -2262,6 +2265,10 @@ int mck_results_init(int32_t int_type, int32_t npoints, int32_t nelements, int32
     HIPCHECK(hipMemsetAsync(Rs.niter.p, 0, Rs.niter.n * 4, G.stream));
     HIPCHECK(hipMemsetAsync(Rs.ctime.p, 0, Rs.ctime.n * 8, G.stream));
     HIPCHECK(hipMemsetAsync(Rs.proc.p, 0, Rs.proc.n * 4, G.stream));
+    if (int_type == 0) {
+        HIPCHECK(Rs.dconn.ensure(Rs.conn.size()));
+        HIPCHECK(hipMemcpyAsync(Rs.dconn.p, Rs.conn.data(), Rs.conn.size() * 4, hipMemcpyHostToDevice, G.stream));
+    }
     HIPCHECK(hipStreamSynchronize(G.stream));
     Rs.init = true;
     return MCK_OK;
@@ -2285,44 +2292,45 @@ int mck_results_add(int32_t nelems, const int32_t *elems, const double *kernel_v
     const bool with_model = Rs.nmodel > 0;
     if (with_model && !model && (G.last_model_nelems != nelems || G.last_model_nbasis != nb))
         return fail(MCK_EINVAL, "no model integrals of a mck_integrate_model call with these nelems and nbasis");
-    // the task's contributions grouped by point, each group in task order
-    std::vector<int> eidx(nelems, -1), pt, slot;
-    pt.reserve((size_t)nelems * nb);
-    slot.reserve((size_t)nelems * nb);
+    // element ids checked on the host; the grouping by point runs on the device
+    std::vector<int> eidx(nelems, -1);
     for (int iel = 0; iel < nelems; ++iel) {
         const int e = elems[iel];
         if (e == -1) continue;
         if (e < 1 || e > Rs.nelements) return fail(MCK_EINVAL, "element id out of range");
         eidx[iel] = e - 1;
-        for (int ib = 0; ib < nb; ++ib) {
-            pt.push_back(Rs.int_type == 0 ? Rs.conn[(size_t)(e - 1) * nb + ib] - 1 : e - 1);
-            slot.push_back(iel * nb + ib);
-        }
     }
-    std::vector<int> order(pt.size());
-    for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return pt[a] < pt[b]; });
-    std::vector<int> seg_start, seg_point, slots(order.size());
-    for (size_t j = 0; j < order.size(); ++j) {
-        if (j == 0 || pt[order[j]] != pt[order[j - 1]]) {
-            seg_start.push_back((int)j);
-            seg_point.push_back(pt[order[j]]);
-        }
-        slots[j] = slot[order[j]];
-    }
-    const int nseg = (int)seg_point.size();
-    seg_start.push_back((int)order.size());
     hipStream_t st = G.stream;
-    HIPCHECK(Rs.seg_start.ensure(seg_start.size()));
-    HIPCHECK(Rs.seg_point.ensure(std::max(nseg, 1)));
-    HIPCHECK(Rs.slots.ensure(std::max<size_t>(slots.size(), 1)));
+    const int npair = nelems * nb;
+    for (DBuf<int> *b : {&Rs.pt_in, &Rs.pt_out, &Rs.sl_in, &Rs.sl_out, &Rs.seg_point, &Rs.seg_count, &Rs.seg_start})
+        HIPCHECK(b->ensure(npair));
+    HIPCHECK(Rs.nseg.ensure(1));
+    HIPCHECK(Rs.elems.ensure(nelems));
     HIPCHECK(Rs.elem_idx.ensure(nelems));
-    HIPCHECK(hipMemcpyAsync(Rs.seg_start.p, seg_start.data(), seg_start.size() * 4, hipMemcpyHostToDevice, st));
-    if (nseg) {
-        HIPCHECK(hipMemcpyAsync(Rs.seg_point.p, seg_point.data(), (size_t)nseg * 4, hipMemcpyHostToDevice, st));
-        HIPCHECK(hipMemcpyAsync(Rs.slots.p, slots.data(), slots.size() * 4, hipMemcpyHostToDevice, st));
-    }
+    HIPCHECK(hipMemcpyAsync(Rs.elems.p, elems, (size_t)nelems * 4, hipMemcpyHostToDevice, st));
     HIPCHECK(hipMemcpyAsync(Rs.elem_idx.p, eidx.data(), (size_t)nelems * 4, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_results_pairs, dim3((npair + 255) / 256), dim3(256), 0, st, nelems, nb,
+                       Rs.int_type == 0 ? 1 : 0, Rs.npoints, Rs.elems.p, Rs.dconn.p, Rs.pt_in.p, Rs.sl_in.p);
+    HIPCHECK(hipGetLastError());
+    {
+        // stable radix sort by point (task order kept within a point), then the runs
+        int bits = 1;
+        while ((1 << bits) <= Rs.npoints) ++bits;
+        size_t n1 = 0, n2 = 0, n3 = 0;
+        HIPCHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, n1, Rs.pt_in.p, Rs.pt_out.p, Rs.sl_in.p, Rs.sl_out.p,
+                                                    npair, 0, bits, st));
+        HIPCHECK(hipcub::DeviceRunLengthEncode::Encode(nullptr, n2, Rs.pt_out.p, Rs.seg_point.p, Rs.seg_count.p,
+                                                       Rs.nseg.p, npair, st));
+        HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, n3, Rs.seg_count.p, Rs.seg_start.p, npair, st));
+        HIPCHECK(Rs.tmp.ensure(std::max({n1, n2, n3, (size_t)1})));
+        HIPCHECK(hipcub::DeviceRadixSort::SortPairs(Rs.tmp.p, n1, Rs.pt_in.p, Rs.pt_out.p, Rs.sl_in.p, Rs.sl_out.p,
+                                                    npair, 0, bits, st));
+        HIPCHECK(hipcub::DeviceRunLengthEncode::Encode(Rs.tmp.p, n2, Rs.pt_out.p, Rs.seg_point.p, Rs.seg_count.p,
+                                                       Rs.nseg.p, npair, st));
+        // counts past the last run are not read: every segment's start is the sum of the
+        // counts before it, whatever follows
+        HIPCHECK(hipcub::DeviceScan::ExclusiveSum(Rs.tmp.p, n3, Rs.seg_count.p, Rs.seg_start.p, npair, st));
+    }
     // inputs: in device memory already, or staged
     const size_t nv = (size_t)nelems * nb * nk, nm = with_model ? (size_t)nelems * nb * kModelPar : 0;
     const double *dv = G.integral.p, *dvar = G.variance.p, *dm = G.last_model.p, *dct = nullptr;
@@ -2364,21 +2372,35 @@ int mck_results_add(int32_t nelems, const int32_t *elems, const double *kernel_v
             dct = computation_time;
         }
     }
-    if (nseg) {
-        const long long nt1 = (long long)nseg * nk;
-        hipLaunchKernelGGL(k_results_add, dim3((unsigned)((nt1 + 255) / 256)), dim3(256), 0, st, nseg, nk,
-                           Rs.seg_start.p, Rs.seg_point.p, Rs.slots.p, dv, Rs.K.p);
-        hipLaunchKernelGGL(k_results_add, dim3((unsigned)((nt1 + 255) / 256)), dim3(256), 0, st, nseg, nk,
-                           Rs.seg_start.p, Rs.seg_point.p, Rs.slots.p, dvar, Rs.V.p);
-        if (with_model) {
-            const long long nt2 = (long long)nseg * kModelPar;
-            hipLaunchKernelGGL(k_results_add, dim3((unsigned)((nt2 + 255) / 256)), dim3(256), 0, st, nseg,
-                               kModelPar, Rs.seg_start.p, Rs.seg_point.p, Rs.slots.p, dm, Rs.M.p);
-        }
+    {
+        // grids for the largest segment count (one per pair); threads past the device
+        // count return at once.  K_x and Var in one pass over the segments.
+        auto add = [&](int ncol, const double *v0, double *a0, const double *v1, double *a1) {
+            const bool two = ncol % 2 == 0 && ((uintptr_t)v0 % 16) == 0 && ((uintptr_t)a0 % 16) == 0 &&
+                             (!v1 || (((uintptr_t)v1 % 16) == 0 && ((uintptr_t)a1 % 16) == 0));
+            const long long nt = (long long)npair * (two ? ncol / 2 : ncol);
+            const dim3 grid((unsigned)((nt + 255) / 256));
+            if (two)
+                hipLaunchKernelGGL(k_results_add<2>, grid, dim3(256), 0, st, Rs.nseg.p, ncol, Rs.npoints,
+                                   Rs.seg_start.p, Rs.seg_count.p, Rs.seg_point.p, Rs.sl_out.p, v0, a0, v1, a1);
+            else
+                hipLaunchKernelGGL(k_results_add<1>, grid, dim3(256), 0, st, Rs.nseg.p, ncol, Rs.npoints,
+                                   Rs.seg_start.p, Rs.seg_count.p, Rs.seg_point.p, Rs.sl_out.p, v0, a0, v1, a1);
+        };
+        add(nk, dv, Rs.K.p, dvar, Rs.V.p);
+        if (with_model) add(kModelPar, dm, Rs.M.p, nullptr, nullptr);
     }
-    const long long nt3 = (long long)nelems * nk;
-    hipLaunchKernelGGL(k_results_elem, dim3((unsigned)((nt3 + 255) / 256)), dim3(256), 0, st, nelems, nk,
-                       Rs.elem_idx.p, dni, dct, rank, Rs.niter.p, Rs.ctime.p, Rs.proc.p);
+    {
+        const bool four = nk % 4 == 0 && ((uintptr_t)dni % 16) == 0;
+        const long long nt3 = (long long)nelems * (four ? nk / 4 : nk);
+        const dim3 grid((unsigned)((nt3 + 255) / 256));
+        if (four)
+            hipLaunchKernelGGL(k_results_elem<4>, grid, dim3(256), 0, st, nelems, nk, Rs.elem_idx.p, dni, dct, rank,
+                               Rs.niter.p, Rs.ctime.p, Rs.proc.p);
+        else
+            hipLaunchKernelGGL(k_results_elem<1>, grid, dim3(256), 0, st, nelems, nk, Rs.elem_idx.p, dni, dct, rank,
+                               Rs.niter.p, Rs.ctime.p, Rs.proc.p);
+    }
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(st));
     return MCK_OK;

@@ -5,7 +5,7 @@ its FP64 accumulator one at a time in task order.
 
   * several random tasks with shared vertices, empty slots and recomputed elements,
     4 basis functions, model integrals: accumulators and all five exported variables;
-  * volumetric mode;
+  * volumetric mode; an odd kernel count;
   * the results of the last slave_work / integrate_model taken from device memory give
     the same accumulators as the same results passed from the host; device tensors as
     inputs;
@@ -25,9 +25,15 @@ pytestmark = pytest.mark.gpu
 EXPORTS = [abi.MCK_RES_KERNEL, abi.MCK_RES_VARIANCE, abi.MCK_RES_ERROR, abi.MCK_RES_MODEL, abi.MCK_RES_NITER]
 
 
-def _kernel():
+def _kernel(odd=False):
+    """The tiny problem (16 kernels), or with 3 kernels (odd column counts take the
+    one-column paths of the accumulation kernels)."""
     import mc_kernel_amd as mk
-    p = synth.make_problem("tiny", ntet=4)
+    over = {}
+    if odd:
+        over = dict(receivers_deg=[30.0], kernels=synth._kernels(1, 1, ["CC"], ["vp", "vs"])
+                    + synth._kernels(1, 1, ["AM"], ["vp"]))
+    p = synth.make_problem("tiny", ntet=4, **over)
     return mk.MCKernel(p.setup), p
 
 
@@ -60,11 +66,11 @@ def _compare(res, ref):
         np.testing.assert_array_equal(res.export(w), ref.export(w), err_msg=str(w))
 
 
-@pytest.mark.parametrize("int_type", [0, 1])
-def test_random_tasks_match_oracle(int_type):
+@pytest.mark.parametrize("int_type,odd", [(0, False), (1, False), (0, True)])
+def test_random_tasks_match_oracle(int_type, odd):
     from oracle import oracle
-    print("results case int_type", int_type, flush=True)
-    k, p = _kernel()
+    print("results case int_type", int_type, "odd", odd, flush=True)
+    k, p = _kernel(odd)
     try:
         rng = np.random.default_rng(11 + int_type)
         nv, ne, nk = 300, 500, k.nk
