@@ -303,6 +303,34 @@ int mck_integrate_model(int32_t elem_type, int32_t nvpe, int32_t nelems, int32_t
                         const int64_t *halton_offsets);
 
 /*
+ * The heterogeneity model for mck_integrate_hetero, replacing load_het_rtpv and
+ * build_hetero_kdtree (heterogeneities.f90:63-136, called by do_slave, slave.f90:126-134
+ * when INT_OVER_3D_HETEROGENEITIES is set): nhet points, rtp [nhet][3] = radius in km,
+ * latitude and longitude in degrees (the columns of the heterogeneity file; a latitude
+ * outside [-90, 90] is rejected as the reference stops on it), dlnhet [nhet][3] = dlnvp,
+ * dlnvs, dlnrho.  The points are converted to single-precision Cartesian coordinates as
+ * the reference does and held in device memory.
+ */
+int mck_set_hetero(int32_t nhet, const double *rtp, const float *dlnhet);
+
+/*
+ * The heterogeneity integrals of a task, integrate_3d_model (slave.f90:874-924, computed
+ * for every element when INT_OVER_3D_HETEROGENEITIES is set, slave.f90:459-461, and
+ * returned as slave_result%hetero_model): per element and basis function the Monte Carlo
+ * integral (volume 1, allowed error 1e-2, relative 1e-1) of dlnvp, dlnvs, dlnrho of the
+ * nearest model point (load_model_coeffs, heterogeneities.f90:141-188; equal fp32
+ * distances take the lower model index) times the basis function, 10 points per step,
+ * at most 9999 steps.  Arguments as mck_integrate_model:
+ *  hetero          out [nelems][nbasis][3]   (Fortran hetero_model(3, nbasis, nelems))
+ *  halton_step     with halton_offsets == NULL element e's points start at
+ *                  *halton_step + e * 99990 (9999 steps of 10)
+ */
+int mck_integrate_hetero(int32_t elem_type, int32_t nvpe, int32_t nelems, int32_t nbasis,
+                         int32_t nvertices, const double *vertices, const int32_t *conn,
+                         double *hetero, int32_t *nsteps, int64_t *halton_step,
+                         const int64_t *halton_offsets);
+
+/*
  * The filter-convolve-window-integrate stage of the receiver loop (slave.f90:515-534,
  * 586-640: rfft of the tapered traces, calc_basekernel kernel.f90:514-570,
  * apply_filter_2d kernel.f90:946-956, irfft and calc_misfit_kernel kernel.f90:410-510) as a
@@ -346,11 +374,13 @@ int mck_ad_backward(int32_t npts, const float *tf, const float *fw, const float 
  *  nbasis       basis functions per element (the tasks' nbasis)
  *  nmodel       0, or 12 to accumulate the background-model integrals
  *               (INT_OVER_BACKGROUND_MODEL, mck_integrate_model)
+ *  nhet         0, or 3 to accumulate the heterogeneity integrals (Het_Model,
+ *               INT_OVER_3D_HETEROGENEITIES, mck_integrate_hetero)
  *  connectivity [nelements][nbasis] 1-based vertex of each basis function (Fortran
  *               connectivity(nbasis, nelements)); NULL in volumetric mode
  */
 int mck_results_init(int32_t int_type, int32_t npoints, int32_t nelements, int32_t nbasis,
-                     int32_t nmodel, const int32_t *connectivity);
+                     int32_t nmodel, int32_t nhet, const int32_t *connectivity);
 
 /*
  * Adds one task's results, replacing extract_receive_buffer (master_queue.f90:253-323):
@@ -367,23 +397,26 @@ int mck_results_init(int32_t int_type, int32_t npoints, int32_t nelements, int32
  *  model           [nelems][nbasis][12] model integrals when nmodel = 12 (host or device;
  *                  NULL = those of the last mck_integrate_model call, kept in device
  *                  memory); ignored when nmodel = 0
+ *  hetero          [nelems][nbasis][3] heterogeneity integrals when nhet = 3, likewise
+ *                  (NULL = the last mck_integrate_hetero call's); ignored when nhet = 0
  */
 int mck_results_add(int32_t nelems, const int32_t *elems, const double *kernel_values,
                     const double *kernel_variance, const int32_t *niterations,
-                    const double *computation_time, const double *model, int32_t rank);
+                    const double *computation_time, const double *model, const double *hetero,
+                    int32_t rank);
 
 /* The accumulators themselves (each nullable), FP64 point-major as the device holds them:
- * kernel, variance [npoints][nkernel], model [npoints][nmodel], niterations
- * [nelements][nkernel], computation_time, element_proc [nelements]. */
-int mck_results_get(double *kernel, double *variance, double *model, int32_t *niterations,
-                    double *computation_time, int32_t *element_proc);
+ * kernel, variance [npoints][nkernel], model [npoints][nmodel], hetero [npoints][nhet],
+ * niterations [nelements][nkernel], computation_time, element_proc [nelements]. */
+int mck_results_get(double *kernel, double *variance, double *model, double *hetero,
+                    int32_t *niterations, double *computation_time, int32_t *element_proc);
 
 /* Replaces the accumulators (each nullable = unchanged), laid out as mck_results_get
  * returns them: resuming from intermediate results (read_intermediate,
  * master_queue.f90:615-659) or merging the partial sums of several devices. */
 int mck_results_load(const double *kernel, const double *variance, const double *model,
-                     const int32_t *niterations, const double *computation_time,
-                     const int32_t *element_proc);
+                     const double *hetero, const int32_t *niterations,
+                     const double *computation_time, const int32_t *element_proc);
 
 #define MCK_RES_KERNEL    0   /* real(K_x, sp)                        [nkernel][npoints]   */
 #define MCK_RES_VARIANCE  1   /* real(Var, sp)                        [nkernel][npoints]   */
@@ -391,6 +424,7 @@ int mck_results_load(const double *kernel, const double *variance, const double 
                                  master_queue.f90:383-387)            [nkernel][npoints]   */
 #define MCK_RES_MODEL     3   /* real(Bg_Model, sp)                   [nmodel][npoints]    */
 #define MCK_RES_NITER     4   /* real(niterations, sp)                [nkernel][nelements] */
+#define MCK_RES_HETERO    5   /* real(Het_Model, sp)                  [nhet][npoints]      */
 /* One output variable of finalize (master_queue.f90:327-581) in single precision, laid
  * out as the output files hold it: entry-major, i.e. Fortran (npoints, nentries). */
 int mck_results_export(int32_t which, float *out);

@@ -145,10 +145,50 @@ class MCKernel:
         self.halton_step = int(step[0])
         return model, nsteps
 
-    def results(self, int_type, npoints, nelements, nbasis=1, nmodel=0, connectivity=None):
+    def set_hetero(self, rtp, dlnhet):
+        """mck_set_hetero: the heterogeneity model, rtp (n, 3) = radius km, latitude,
+        longitude (degrees), dlnhet (n, 3) = dlnvp, dlnvs, dlnrho."""
+        rtp = np.ascontiguousarray(rtp, np.float64).reshape(-1, 3)
+        dln = np.ascontiguousarray(dlnhet, np.float32).reshape(-1, 3)
+        if rtp.shape[0] != dln.shape[0]:
+            raise ValueError("rtp and dlnhet differ in length")
+        _check(self.lib.mck_set_hetero(rtp.shape[0], abi.ptr(rtp, C.c_double), abi.ptr(dln, C.c_float)))
+
+    def integrate_hetero(self, verts, conn, nbasis=1, halton_offsets=None):
+        """mck_integrate_hetero: (hetero [ne][nbasis][3], nsteps [ne]); advances halton_step."""
+        verts = np.ascontiguousarray(verts, np.float64).reshape(-1, 3)
+        conn = np.ascontiguousarray(conn, np.int32)
+        ne, nvpe = conn.shape
+        out = np.zeros((ne, nbasis, 3))
+        nsteps = np.zeros(ne, np.int32)
+        step = np.array([self.halton_step], np.int64)
+        offs = None if halton_offsets is None else np.ascontiguousarray(halton_offsets, np.int64)
+        _check(self.lib.mck_integrate_hetero(4 if nvpe == 4 else 5, nvpe, ne, nbasis, verts.shape[0],
+                                             abi.ptr(verts, C.c_double), abi.ptr(conn, C.c_int32),
+                                             abi.ptr(out, C.c_double), abi.ptr(nsteps, C.c_int32),
+                                             abi.ptr(step, C.c_int64), abi.ptr(offs, C.c_int64)))
+        self.halton_step = int(step[0])
+        return out, nsteps
+
+    def results(self, int_type, npoints, nelements, nbasis=1, nmodel=0, connectivity=None, nhet=0):
         """mck_results_init: the master's accumulators in device memory (results.Results)."""
         from .results import Results
-        return Results(self, int_type, npoints, nelements, nbasis, nmodel, connectivity)
+        return Results(self, int_type, npoints, nelements, nbasis, nmodel, connectivity, nhet)
+
+
+def load_het_rtpv(path):
+    """The heterogeneity file read by load_het_rtpv (heterogeneities.f90:63-110): a first
+    line with the number of points, then one line per point with radius (km), latitude
+    and longitude (degrees) and dlnvp, dlnvs, dlnrho.  Returns (rtp (n, 3) float64,
+    dlnhet (n, 3) float32)."""
+    with open(path) as f:
+        n = int(f.readline().split()[0])
+        rows = np.loadtxt(f, ndmin=2, max_rows=n)
+    if rows.shape[0] != n or rows.shape[1] < 6:
+        raise ValueError("heterogeneity file %s: expected %d rows of 6 columns" % (path, n))
+    if np.any(np.abs(rows[:, 1]) > 90.0):
+        raise ValueError("heterogeneity file %s: latitude outside [-90, 90]" % path)
+    return rows[:, :3].astype(np.float64), rows[:, 3:6].astype(np.float32)
 
     def seismograms(self):
         """[nrec][nt] seismograms the kernels use (the set-up's, or by reciprocity)."""

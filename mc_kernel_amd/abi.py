@@ -20,6 +20,7 @@ MCK_RES_VARIANCE = 1
 MCK_RES_ERROR = 2
 MCK_RES_MODEL = 3
 MCK_RES_NITER = 4
+MCK_RES_HETERO = 5
 
 PARAMS = {"lam": 0, "vp": 1, "vs": 2, "mu": 3, "vsh": 4, "vsv": 5, "vph": 6, "vpv": 7,
           "eta": 8, "rho": 9}
@@ -94,7 +95,7 @@ def ptr(arr, ctype):
 # exported symbols of libmck.so, in the order of include/mck.h
 EXPORTS = [
     "mck_init", "mck_slave_work", "mck_slave_work_plain", "mck_locate_points",
-    "mck_load_strain_points", "mck_base_kernels", "mck_seismograms", "mck_integrate_model", "mck_ad_dims", "mck_ad_transfer",
+    "mck_load_strain_points", "mck_base_kernels", "mck_seismograms", "mck_integrate_model", "mck_set_hetero", "mck_integrate_hetero", "mck_ad_dims", "mck_ad_transfer",
     "mck_ad_forward", "mck_ad_backward", "mck_results_init", "mck_results_add", "mck_results_get",
     "mck_results_load", "mck_results_export", "mck_get_stats", "mck_finalize", "mck_last_error",
 ]
@@ -109,6 +110,10 @@ def declare(lib):
     lib.mck_integrate_model.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, f64p, i32p,
                                         f64p, i32p, i64p, i64p]
     lib.mck_integrate_model.restype = C.c_int
+    lib.mck_set_hetero.argtypes = [C.c_int32, f64p, f32p]
+    lib.mck_set_hetero.restype = C.c_int
+    lib.mck_integrate_hetero.argtypes = lib.mck_integrate_model.argtypes
+    lib.mck_integrate_hetero.restype = C.c_int
     work = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, f64p, i32p,
             f64p, f64p, i32p, f64p, i64p, i64p]
     lib.mck_slave_work.argtypes = work
@@ -130,14 +135,14 @@ def declare(lib):
     lib.mck_ad_forward.restype = C.c_int
     lib.mck_ad_backward.argtypes = [C.c_int32] + [C.c_void_p] * 8
     lib.mck_ad_backward.restype = C.c_int
-    lib.mck_results_init.argtypes = [C.c_int32] * 5 + [i32p]
+    lib.mck_results_init.argtypes = [C.c_int32] * 6 + [i32p]
     lib.mck_results_init.restype = C.c_int
     # result arrays may be host or device memory: passed as addresses
-    lib.mck_results_add.argtypes = [C.c_int32, i32p] + [C.c_void_p] * 5 + [C.c_int32]
+    lib.mck_results_add.argtypes = [C.c_int32, i32p] + [C.c_void_p] * 6 + [C.c_int32]
     lib.mck_results_add.restype = C.c_int
-    lib.mck_results_get.argtypes = [f64p, f64p, f64p, i32p, f64p, i32p]
+    lib.mck_results_get.argtypes = [f64p, f64p, f64p, f64p, i32p, f64p, i32p]
     lib.mck_results_get.restype = C.c_int
-    lib.mck_results_load.argtypes = [C.c_void_p] * 6
+    lib.mck_results_load.argtypes = [C.c_void_p] * 7
     lib.mck_results_load.restype = C.c_int
     lib.mck_results_export.argtypes = [C.c_int32, C.c_void_p]
     lib.mck_results_export.restype = C.c_int

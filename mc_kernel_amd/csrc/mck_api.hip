@@ -404,20 +404,26 @@ struct State {
     // mck_integrate_model (device: last_model), for mck_results_add without a host copy
     int last_nelems = 0, last_nbasis = 0, last_model_nelems = 0, last_model_nbasis = 0;
     std::vector<double> last_ctime;
-    DBuf<double> last_model;
+    DBuf<double> last_model, last_hetero;
+    int last_hetero_nelems = 0, last_hetero_nbasis = 0;
+    // heterogeneity model (mck_set_hetero): point cloud in a uniform grid
+    HetGrid hgrid{};
+    bool het_ready = false;
+    DBuf<float> het_xyz, het_dln;
+    DBuf<int> het_idx, het_cells;
     // the master's accumulators (mck_results_*)
     struct Results {
         bool init = false;
-        int int_type = 0, npoints = 0, nelements = 0, nbasis = 1, nmodel = 0;
+        int int_type = 0, npoints = 0, nelements = 0, nbasis = 1, nmodel = 0, nhet = 0;
         std::vector<int> conn;               // [nelements][nbasis] 1-based vertices
-        DBuf<double> K, V, M, ctime;         // [npoints][nk], [npoints][nk], [npoints][nmodel]
+        DBuf<double> K, V, M, H, ctime;      // [npoints][nk] x 2, [npoints][nmodel], [npoints][nhet]
         DBuf<int> niter, proc;               // [nelements][nk], [nelements]
         DBuf<int> dconn, elems, pt_in, pt_out, sl_in, sl_out, seg_point, seg_count, seg_start, nseg, elem_idx;
         DBuf<unsigned char> tmp;             // hipcub scratch
         DBuf<double> stage;                  // host inputs staged in device memory
         DBuf<int> stage_i;
         void release() {
-            K.release(); V.release(); M.release(); ctime.release(); niter.release(); proc.release();
+            K.release(); V.release(); M.release(); H.release(); ctime.release(); niter.release(); proc.release();
             for (DBuf<int> *b : {&dconn, &elems, &pt_in, &pt_out, &sl_in, &sl_out, &seg_point, &seg_count,
                                  &seg_start, &nseg, &elem_idx})
                 b->release();
@@ -1949,6 +1955,90 @@ hipError_t ad_launch_grad(State &G, int npts, const float *bw, const double *gba
     return e;
 }
 
+// integrate_1d_model (hetero = false) or integrate_3d_model (hetero = true) for a task:
+// one workgroup per element runs that element's whole MC loop
+int integrate_task(bool hetero, int32_t elem_type, int32_t nvpe, int32_t nelems, int32_t nbasis,
+                   int32_t nvertices, const double *vertices, const int32_t *conn, double *model,
+                   int32_t *nsteps, int64_t *halton_step, const int64_t *halton_offsets) {
+    State &G = g;
+    if (!G.init) return fail(MCK_ENOTINIT, "mck_init has not been called");
+    const bool vox = elem_type == 5;
+    if (!((elem_type == 4 && nvpe == 4) || (vox && nvpe == 8)))
+        return fail(MCK_EUNSUPPORTED, "inversion elements must be tetrahedra (type 4) or voxels (type 5)");
+    if ((!vox && nbasis != 1 && nbasis != 4) || (vox && nbasis != 1 && nbasis != 8))
+        return fail(MCK_EINVAL, "invalid number of basis functions");
+    if (nelems <= 0 || !vertices || !conn || !model || !halton_step) return fail(MCK_EINVAL, "invalid argument");
+    const int64_t kStride = hetero ? (int64_t)kHetMaxStep * kHetPts : (int64_t)kModelMaxStep * kModelPts;
+    std::vector<double> geom((size_t)nelems * 12, 0.0), abinv;
+    std::vector<uint8_t> live(nelems, 1);
+    std::vector<int64_t> base(nelems, 0);
+    if (nbasis > 1 && !vox) abinv.assign((size_t)nelems * 16, 0.0);
+    for (int e = 0; e < nelems; ++e) {
+        base[e] = halton_offsets ? halton_offsets[e] : *halton_step + (int64_t)e * kStride;
+        if (conn[nvpe * e] <= 0) { live[e] = 0; continue; }
+        double ev[24];
+        for (int i = 0; i < nvpe; ++i) {
+            const int vid = conn[nvpe * e + i] - 1;
+            if (vid < 0 || vid >= nvertices) return fail(MCK_EINVAL, "connectivity out of range");
+            for (int c = 0; c < 3; ++c) ev[3 * i + c] = vertices[3 * vid + c];
+        }
+        double *v = &geom[(size_t)e * 12];
+        if (vox) {
+            vox_range(ev, v);
+            if (!(v[0] > 0.0) || v[1] <= v[0] || v[3] <= v[2] || v[5] <= v[4])
+                return fail(MCK_EINVAL, "degenerate voxel");
+        } else {
+            std::copy(ev, ev + 12, v);
+            if (nbasis > 1) tet_barycentric_inverse(v, &abinv[(size_t)e * 16]);
+        }
+    }
+    double *d_geom = nullptr, *d_abinv = nullptr;
+    int64_t *d_base = nullptr;
+    uint8_t *d_live = nullptr;
+    int *d_nsteps = nullptr;
+    const size_t nout = (size_t)nelems * nbasis * (hetero ? kHetPar : kModelPar);
+    std::vector<int> ns(nelems, 0);
+    hipError_t e = hipMalloc((void **)&d_geom, geom.size() * 8);
+    if (e == hipSuccess && !abinv.empty()) e = hipMalloc((void **)&d_abinv, abinv.size() * 8);
+    DBuf<double> &last = hetero ? G.last_hetero : G.last_model;
+    (hetero ? G.last_hetero_nelems : G.last_model_nelems) = 0;
+    if (e == hipSuccess) e = last.ensure(nout);
+    double *d_out = last.p;
+    if (e == hipSuccess) e = hipMalloc((void **)&d_base, (size_t)nelems * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&d_live, (size_t)nelems);
+    if (e == hipSuccess) e = hipMalloc((void **)&d_nsteps, (size_t)nelems * 4);
+    if (e == hipSuccess) e = hipMemcpy(d_geom, geom.data(), geom.size() * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess && d_abinv) e = hipMemcpy(d_abinv, abinv.data(), abinv.size() * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_base, base.data(), (size_t)nelems * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_live, live.data(), (size_t)nelems, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemsetAsync(G.err.p, 0, 4, G.stream);
+    if (e == hipSuccess) {
+        if (hetero)
+            hipLaunchKernelGGL(k_hetero_integral, dim3(nelems), dim3(64), 0, G.stream, nelems, d_geom, vox ? 1 : 0,
+                               nbasis, d_abinv, d_base, d_live, G.hgrid, d_out, d_nsteps, G.err.p);
+        else
+            hipLaunchKernelGGL(k_model_integral, dim3(nelems), dim3(128), 0, G.stream, nelems, d_geom, vox ? 1 : 0,
+                               nbasis, d_abinv, d_base, d_live, G.model.p, G.S.model_ndepth, G.S.model_dr, d_out,
+                               d_nsteps, G.err.p);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(model, d_out, nout * 8, hipMemcpyDeviceToHost, G.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(ns.data(), d_nsteps, (size_t)nelems * 4, hipMemcpyDeviceToHost, G.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(G.stream);
+    int err = 0;
+    if (e == hipSuccess) e = hipMemcpy(&err, G.err.p, 4, hipMemcpyDeviceToHost);
+    (void)hipFree(d_geom); (void)hipFree(d_abinv);
+    (void)hipFree(d_base); (void)hipFree(d_live); (void)hipFree(d_nsteps);
+    if (e != hipSuccess) return fail(MCK_EHIP, std::string(hetero ? "mck_integrate_hetero: " : "mck_integrate_model: ") + hipGetErrorString(e));
+    if (err & 4) return fail(MCK_EINVAL, "basis function weight outside [0, 1] (point outside its element)");
+    if (err & 8) return fail(MCK_ENAN, "NaN in a model integral (mc_integration.f90:113-130)");
+    if (nsteps) std::copy(ns.begin(), ns.end(), nsteps);
+    if (!halton_offsets) *halton_step += (int64_t)nelems * kStride;
+    (hetero ? G.last_hetero_nelems : G.last_model_nelems) = nelems;
+    (hetero ? G.last_hetero_nbasis : G.last_model_nbasis) = nbasis;
+    return MCK_OK;
+}
+
 }  // namespace
 
 // =====================================================================================
@@ -2148,78 +2238,80 @@ int mck_ad_backward(int32_t npts, const float *tf, const float *fw, const float 
 int mck_integrate_model(int32_t elem_type, int32_t nvpe, int32_t nelems, int32_t nbasis, int32_t nvertices,
                         const double *vertices, const int32_t *conn, double *model, int32_t *nsteps,
                         int64_t *halton_step, const int64_t *halton_offsets) {
+    return integrate_task(false, elem_type, nvpe, nelems, nbasis, nvertices, vertices, conn, model, nsteps,
+                          halton_step, halton_offsets);
+}
+
+int mck_set_hetero(int32_t nhet, const double *rtp, const float *dlnhet) {
     State &G = g;
     if (!G.init) return fail(MCK_ENOTINIT, "mck_init has not been called");
-    const bool vox = elem_type == 5;
-    if (!((elem_type == 4 && nvpe == 4) || (vox && nvpe == 8)))
-        return fail(MCK_EUNSUPPORTED, "inversion elements must be tetrahedra (type 4) or voxels (type 5)");
-    if ((!vox && nbasis != 1 && nbasis != 4) || (vox && nbasis != 1 && nbasis != 8))
-        return fail(MCK_EINVAL, "invalid number of basis functions");
-    if (nelems <= 0 || !vertices || !conn || !model || !halton_step) return fail(MCK_EINVAL, "invalid argument");
-    constexpr int64_t kStride = (int64_t)kModelMaxStep * kModelPts;
-    std::vector<double> geom((size_t)nelems * 12, 0.0), abinv;
-    std::vector<uint8_t> live(nelems, 1);
-    std::vector<int64_t> base(nelems, 0);
-    if (nbasis > 1 && !vox) abinv.assign((size_t)nelems * 16, 0.0);
-    for (int e = 0; e < nelems; ++e) {
-        base[e] = halton_offsets ? halton_offsets[e] : *halton_step + (int64_t)e * kStride;
-        if (conn[nvpe * e] <= 0) { live[e] = 0; continue; }
-        double ev[24];
-        for (int i = 0; i < nvpe; ++i) {
-            const int vid = conn[nvpe * e + i] - 1;
-            if (vid < 0 || vid >= nvertices) return fail(MCK_EINVAL, "connectivity out of range");
-            for (int c = 0; c < 3; ++c) ev[3 * i + c] = vertices[3 * vid + c];
-        }
-        double *v = &geom[(size_t)e * 12];
-        if (vox) {
-            vox_range(ev, v);
-            if (!(v[0] > 0.0) || v[1] <= v[0] || v[3] <= v[2] || v[5] <= v[4])
-                return fail(MCK_EINVAL, "degenerate voxel");
-        } else {
-            std::copy(ev, ev + 12, v);
-            if (nbasis > 1) tet_barycentric_inverse(v, &abinv[(size_t)e * 16]);
-        }
+    if (nhet <= 0 || !rtp || !dlnhet) return fail(MCK_EINVAL, "invalid heterogeneity model");
+    // load_het_rtpv (heterogeneities.f90:63-110): r km, latitude and longitude in degrees
+    // to single-precision Cartesian coordinates (spherical_to_cartesian_point in FP64)
+    std::vector<float> xyz((size_t)nhet * 3);
+    for (int i = 0; i < nhet; ++i) {
+        const double r = rtp[3 * i], t = rtp[3 * i + 1], ph = rtp[3 * i + 2];
+        if (std::fabs(t) > 90.0) return fail(MCK_EINVAL, "heterogeneity latitude outside [-90, 90]");
+        const double rr = r * 1000.0, th = (90.0 - t) * kDeg2Rad, pp = ph * kDeg2Rad;
+        xyz[3 * i] = (float)(rr * std::cos(pp) * std::sin(th));
+        xyz[3 * i + 1] = (float)(rr * std::sin(pp) * std::sin(th));
+        xyz[3 * i + 2] = (float)(rr * std::cos(th));
     }
-    double *d_geom = nullptr, *d_abinv = nullptr;
-    int64_t *d_base = nullptr;
-    uint8_t *d_live = nullptr;
-    int *d_nsteps = nullptr;
-    const size_t nout = (size_t)nelems * nbasis * kModelPar;
-    std::vector<int> ns(nelems, 0);
-    hipError_t e = hipMalloc((void **)&d_geom, geom.size() * 8);
-    if (e == hipSuccess && !abinv.empty()) e = hipMalloc((void **)&d_abinv, abinv.size() * 8);
-    G.last_model_nelems = G.last_model_nbasis = 0;
-    if (e == hipSuccess) e = G.last_model.ensure(nout);
-    double *d_out = G.last_model.p;
-    if (e == hipSuccess) e = hipMalloc((void **)&d_base, (size_t)nelems * 8);
-    if (e == hipSuccess) e = hipMalloc((void **)&d_live, (size_t)nelems);
-    if (e == hipSuccess) e = hipMalloc((void **)&d_nsteps, (size_t)nelems * 4);
-    if (e == hipSuccess) e = hipMemcpy(d_geom, geom.data(), geom.size() * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess && d_abinv) e = hipMemcpy(d_abinv, abinv.data(), abinv.size() * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d_base, base.data(), (size_t)nelems * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d_live, live.data(), (size_t)nelems, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemsetAsync(G.err.p, 0, 4, G.stream);
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_model_integral, dim3(nelems), dim3(128), 0, G.stream, nelems, d_geom, vox ? 1 : 0,
-                           nbasis, d_abinv, d_base, d_live, G.model.p, G.S.model_ndepth, G.S.model_dr, d_out,
-                           d_nsteps, G.err.p);
-        e = hipGetLastError();
+    // uniform grid of cubic cells, about two model points per cell
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = 0; i < nhet; ++i)
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = std::min(lo[c], (double)xyz[3 * i + c]);
+            hi[c] = std::max(hi[c], (double)xyz[3 * i + c]);
+        }
+    double ext[3], vol = 1.0;
+    for (int c = 0; c < 3; ++c) { ext[c] = std::max(hi[c] - lo[c], 1.0); vol *= ext[c]; }
+    double h = std::cbrt(vol / std::max(1.0, nhet / 2.0));
+    int n[3];
+    for (int c = 0; c < 3; ++c) n[c] = std::max(1, std::min(512, (int)std::ceil(ext[c] / h)));
+    h = std::max({ext[0] / n[0], ext[1] / n[1], ext[2] / n[2]}) * (1.0 + 1e-9);
+    const size_t ncell = (size_t)n[0] * n[1] * n[2];
+    std::vector<int> cell(nhet), start(ncell + 1, 0);
+    for (int i = 0; i < nhet; ++i) {
+        int ci[3];
+        for (int c = 0; c < 3; ++c) {
+            int v = (int)std::floor(((double)xyz[3 * i + c] - lo[c]) / h);
+            ci[c] = v < 0 ? 0 : (v >= n[c] ? n[c] - 1 : v);
+        }
+        cell[i] = (ci[2] * n[1] + ci[1]) * n[0] + ci[0];
+        start[cell[i] + 1]++;
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(model, d_out, nout * 8, hipMemcpyDeviceToHost, G.stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(ns.data(), d_nsteps, (size_t)nelems * 4, hipMemcpyDeviceToHost, G.stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(G.stream);
-    int err = 0;
-    if (e == hipSuccess) e = hipMemcpy(&err, G.err.p, 4, hipMemcpyDeviceToHost);
-    (void)hipFree(d_geom); (void)hipFree(d_abinv);
-    (void)hipFree(d_base); (void)hipFree(d_live); (void)hipFree(d_nsteps);
-    if (e != hipSuccess) return fail(MCK_EHIP, std::string("mck_integrate_model: ") + hipGetErrorString(e));
-    if (err & 4) return fail(MCK_EINVAL, "basis function weight outside [0, 1] (point outside its element)");
-    if (err & 8) return fail(MCK_ENAN, "NaN in a model integral (mc_integration.f90:113-130)");
-    if (nsteps) std::copy(ns.begin(), ns.end(), nsteps);
-    if (!halton_offsets) *halton_step += (int64_t)nelems * kStride;
-    G.last_model_nelems = nelems;
-    G.last_model_nbasis = nbasis;
+    for (size_t c = 0; c < ncell; ++c) start[c + 1] += start[c];
+    std::vector<int> fill(start.begin(), start.end() - 1), idx(nhet);
+    std::vector<float> sorted((size_t)nhet * 3);
+    for (int i = 0; i < nhet; ++i) {
+        const int j = fill[cell[i]]++;
+        idx[j] = i;
+        for (int c = 0; c < 3; ++c) sorted[3 * (size_t)j + c] = xyz[3 * (size_t)i + c];
+    }
+    HIPCHECK(G.het_xyz.ensure(sorted.size()));
+    HIPCHECK(G.het_dln.ensure((size_t)nhet * 3));
+    HIPCHECK(G.het_idx.ensure(nhet));
+    HIPCHECK(G.het_cells.ensure(start.size()));
+    HIPCHECK(hipMemcpy(G.het_xyz.p, sorted.data(), sorted.size() * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(G.het_dln.p, dlnhet, (size_t)nhet * 3 * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(G.het_idx.p, idx.data(), (size_t)nhet * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(G.het_cells.p, start.data(), start.size() * 4, hipMemcpyHostToDevice));
+    HetGrid &hg = G.hgrid;
+    hg.n = nhet;
+    hg.nx = n[0]; hg.ny = n[1]; hg.nz = n[2];
+    hg.x0 = lo[0]; hg.y0 = lo[1]; hg.z0 = lo[2]; hg.h = h;
+    hg.xyz = G.het_xyz.p; hg.idx = G.het_idx.p; hg.cell_start = G.het_cells.p; hg.dln = G.het_dln.p;
+    G.het_ready = true;
     return MCK_OK;
+}
+
+int mck_integrate_hetero(int32_t elem_type, int32_t nvpe, int32_t nelems, int32_t nbasis, int32_t nvertices,
+                         const double *vertices, const int32_t *conn, double *hetero, int32_t *nsteps,
+                         int64_t *halton_step, const int64_t *halton_offsets) {
+    if (!g.het_ready) return fail(MCK_ENOTINIT, "mck_set_hetero has not been called");
+    return integrate_task(true, elem_type, nvpe, nelems, nbasis, nvertices, vertices, conn, hetero, nsteps,
+                          halton_step, halton_offsets);
 }
 
 int mck_seismograms(double *seis) {
@@ -2231,12 +2323,13 @@ int mck_seismograms(double *seis) {
 }
 
 int mck_results_init(int32_t int_type, int32_t npoints, int32_t nelements, int32_t nbasis, int32_t nmodel,
-                     const int32_t *connectivity) {
+                     int32_t nhet, const int32_t *connectivity) {
     State &G = g;
     auto &Rs = G.res;
     if (!G.init) return fail(MCK_ENOTINIT, "mck_init has not been called");
     if (int_type != 0 && int_type != 1) return fail(MCK_EINVAL, "int_type must be 0 (onvertices) or 1 (volumetric)");
-    if (npoints <= 0 || nelements <= 0 || nbasis <= 0 || (nmodel != 0 && nmodel != kModelPar))
+    if (npoints <= 0 || nelements <= 0 || nbasis <= 0 || (nmodel != 0 && nmodel != kModelPar) ||
+        (nhet != 0 && nhet != kHetPar))
         return fail(MCK_EINVAL, "invalid result dimensions");
     if (int_type == 1 && (npoints != nelements || nbasis != 1))
         return fail(MCK_EINVAL, "volumetric results: one basis function per element and npoints = nelements");
@@ -2252,16 +2345,19 @@ int mck_results_init(int32_t int_type, int32_t npoints, int32_t nelements, int32
     Rs.nelements = nelements;
     Rs.nbasis = nbasis;
     Rs.nmodel = nmodel;
+    Rs.nhet = nhet;
     const size_t nk = (size_t)G.nk;
     HIPCHECK(Rs.K.ensure((size_t)npoints * nk));
     HIPCHECK(Rs.V.ensure((size_t)npoints * nk));
     HIPCHECK(Rs.M.ensure((size_t)npoints * std::max(nmodel, 1)));
+    HIPCHECK(Rs.H.ensure((size_t)npoints * std::max(nhet, 1)));
     HIPCHECK(Rs.niter.ensure((size_t)nelements * nk));
     HIPCHECK(Rs.ctime.ensure(nelements));
     HIPCHECK(Rs.proc.ensure(nelements));
     HIPCHECK(hipMemsetAsync(Rs.K.p, 0, Rs.K.n * 8, G.stream));
     HIPCHECK(hipMemsetAsync(Rs.V.p, 0, Rs.V.n * 8, G.stream));
     HIPCHECK(hipMemsetAsync(Rs.M.p, 0, Rs.M.n * 8, G.stream));
+    HIPCHECK(hipMemsetAsync(Rs.H.p, 0, Rs.H.n * 8, G.stream));
     HIPCHECK(hipMemsetAsync(Rs.niter.p, 0, Rs.niter.n * 4, G.stream));
     HIPCHECK(hipMemsetAsync(Rs.ctime.p, 0, Rs.ctime.n * 8, G.stream));
     HIPCHECK(hipMemsetAsync(Rs.proc.p, 0, Rs.proc.n * 4, G.stream));
@@ -2276,7 +2372,7 @@ int mck_results_init(int32_t int_type, int32_t npoints, int32_t nelements, int32
 
 int mck_results_add(int32_t nelems, const int32_t *elems, const double *kernel_values,
                     const double *kernel_variance, const int32_t *niterations, const double *computation_time,
-                    const double *model, int32_t rank) {
+                    const double *model, const double *hetero, int32_t rank) {
     State &G = g;
     auto &Rs = G.res;
     if (!G.init) return fail(MCK_ENOTINIT, "mck_init has not been called");
@@ -2292,6 +2388,9 @@ int mck_results_add(int32_t nelems, const int32_t *elems, const double *kernel_v
     const bool with_model = Rs.nmodel > 0;
     if (with_model && !model && (G.last_model_nelems != nelems || G.last_model_nbasis != nb))
         return fail(MCK_EINVAL, "no model integrals of a mck_integrate_model call with these nelems and nbasis");
+    const bool with_het = Rs.nhet > 0;
+    if (with_het && !hetero && (G.last_hetero_nelems != nelems || G.last_hetero_nbasis != nb))
+        return fail(MCK_EINVAL, "no heterogeneity integrals of a mck_integrate_hetero call with these nelems and nbasis");
     // element ids checked on the host; the grouping by point runs on the device
     std::vector<int> eidx(nelems, -1);
     for (int iel = 0; iel < nelems; ++iel) {
@@ -2333,15 +2432,18 @@ int mck_results_add(int32_t nelems, const int32_t *elems, const double *kernel_v
     }
     // inputs: in device memory already, or staged
     const size_t nv = (size_t)nelems * nb * nk, nm = with_model ? (size_t)nelems * nb * kModelPar : 0;
-    const double *dv = G.integral.p, *dvar = G.variance.p, *dm = G.last_model.p, *dct = nullptr;
+    const size_t nh = with_het ? (size_t)nelems * nb * kHetPar : 0;
+    const double *dv = G.integral.p, *dvar = G.variance.p, *dm = G.last_model.p, *dh = G.last_hetero.p;
+    const double *dct = nullptr;
     const int *dni = G.niter.p;
     const std::vector<double> &ct_last = G.last_ctime;
     {
         size_t need = 0;
         const bool sv = !last && !device_ptr(kernel_values), svar = !last && !device_ptr(kernel_variance);
         const bool sm = with_model && model && !device_ptr(model);
+        const bool sh = with_het && hetero && !device_ptr(hetero);
         const bool sct = last || (computation_time && !device_ptr(computation_time));
-        need = (sv ? nv : 0) + (svar ? nv : 0) + (sm ? nm : 0) + (sct ? nelems : 0);
+        need = (sv ? nv : 0) + (svar ? nv : 0) + (sm ? nm : 0) + (sh ? nh : 0) + (sct ? nelems : 0);
         HIPCHECK(Rs.stage.ensure(std::max<size_t>(need, 1)));
         double *q = Rs.stage.p;
         auto put = [&](const double *src, size_t n) -> const double * {
@@ -2364,6 +2466,10 @@ int mck_results_add(int32_t nelems, const int32_t *elems, const double *kernel_v
         if (with_model && model) {
             dm = sm ? put(model, nm) : model;
             if (!dm) return fail(MCK_EHIP, "mck_results_add: staging copy failed");
+        }
+        if (with_het && hetero) {
+            dh = sh ? put(hetero, nh) : hetero;
+            if (!dh) return fail(MCK_EHIP, "mck_results_add: staging copy failed");
         }
         if (sct) {
             dct = put(last ? ct_last.data() : computation_time, nelems);
@@ -2389,6 +2495,7 @@ int mck_results_add(int32_t nelems, const int32_t *elems, const double *kernel_v
         };
         add(nk, dv, Rs.K.p, dvar, Rs.V.p);
         if (with_model) add(kModelPar, dm, Rs.M.p, nullptr, nullptr);
+        if (with_het) add(kHetPar, dh, Rs.H.p, nullptr, nullptr);
     }
     {
         const bool four = nk % 4 == 0 && ((uintptr_t)dni % 16) == 0;
@@ -2406,7 +2513,7 @@ int mck_results_add(int32_t nelems, const int32_t *elems, const double *kernel_v
     return MCK_OK;
 }
 
-int mck_results_get(double *kernel, double *variance, double *model, int32_t *niterations,
+int mck_results_get(double *kernel, double *variance, double *model, double *hetero, int32_t *niterations,
                     double *computation_time, int32_t *element_proc) {
     State &G = g;
     auto &Rs = G.res;
@@ -2416,13 +2523,14 @@ int mck_results_get(double *kernel, double *variance, double *model, int32_t *ni
     if (kernel) HIPCHECK(hipMemcpy(kernel, Rs.K.p, np * nk * 8, hipMemcpyDeviceToHost));
     if (variance) HIPCHECK(hipMemcpy(variance, Rs.V.p, np * nk * 8, hipMemcpyDeviceToHost));
     if (model && Rs.nmodel) HIPCHECK(hipMemcpy(model, Rs.M.p, np * Rs.nmodel * 8, hipMemcpyDeviceToHost));
+    if (hetero && Rs.nhet) HIPCHECK(hipMemcpy(hetero, Rs.H.p, np * Rs.nhet * 8, hipMemcpyDeviceToHost));
     if (niterations) HIPCHECK(hipMemcpy(niterations, Rs.niter.p, ne * nk * 4, hipMemcpyDeviceToHost));
     if (computation_time) HIPCHECK(hipMemcpy(computation_time, Rs.ctime.p, ne * 8, hipMemcpyDeviceToHost));
     if (element_proc) HIPCHECK(hipMemcpy(element_proc, Rs.proc.p, ne * 4, hipMemcpyDeviceToHost));
     return MCK_OK;
 }
 
-int mck_results_load(const double *kernel, const double *variance, const double *model,
+int mck_results_load(const double *kernel, const double *variance, const double *model, const double *hetero,
                      const int32_t *niterations, const double *computation_time, const int32_t *element_proc) {
     State &G = g;
     auto &Rs = G.res;
@@ -2432,6 +2540,7 @@ int mck_results_load(const double *kernel, const double *variance, const double 
     if (kernel) HIPCHECK(hipMemcpy(Rs.K.p, kernel, np * nk * 8, hipMemcpyDefault));
     if (variance) HIPCHECK(hipMemcpy(Rs.V.p, variance, np * nk * 8, hipMemcpyDefault));
     if (model && Rs.nmodel) HIPCHECK(hipMemcpy(Rs.M.p, model, np * Rs.nmodel * 8, hipMemcpyDefault));
+    if (hetero && Rs.nhet) HIPCHECK(hipMemcpy(Rs.H.p, hetero, np * Rs.nhet * 8, hipMemcpyDefault));
     if (niterations) HIPCHECK(hipMemcpy(Rs.niter.p, niterations, ne * nk * 4, hipMemcpyDefault));
     if (computation_time) HIPCHECK(hipMemcpy(Rs.ctime.p, computation_time, ne * 8, hipMemcpyDefault));
     if (element_proc) HIPCHECK(hipMemcpy(Rs.proc.p, element_proc, ne * 4, hipMemcpyDefault));
@@ -2454,6 +2563,11 @@ int mck_results_export(int32_t which, float *out) {
         if (!Rs.nmodel) return fail(MCK_EINVAL, "no model integrals accumulated (nmodel = 0)");
         src = Rs.M.p;
         ncol = Rs.nmodel;
+        break;
+    case MCK_RES_HETERO:
+        if (!Rs.nhet) return fail(MCK_EINVAL, "no heterogeneity integrals accumulated (nhet = 0)");
+        src = Rs.H.p;
+        ncol = Rs.nhet;
         break;
     case MCK_RES_NITER: src = Rs.niter.p; np = Rs.nelements; break;
     default: return fail(MCK_EINVAL, "unknown result variable");
@@ -2511,8 +2625,11 @@ void mck_finalize(void) {
     G.ad_W.release(); G.ad_Hg.release(); G.ad_Fs.release(); G.ad_U.release();
     G.ad_Bs.release(); G.ad_dFs.release(); G.ad_Ups.release();
     G.ad_part.release(); G.ad_hfilt.release();
-    G.last_model.release();
+    G.last_model.release(); G.last_hetero.release();
+    G.het_xyz.release(); G.het_dln.release(); G.het_idx.release(); G.het_cells.release();
+    G.het_ready = false;
     G.last_nelems = G.last_nbasis = G.last_model_nelems = G.last_model_nbasis = 0;
+    G.last_hetero_nelems = G.last_hetero_nbasis = 0;
     G.res.release();
     G.init = false;
 }

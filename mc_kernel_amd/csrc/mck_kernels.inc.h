@@ -387,6 +387,153 @@ __global__ __launch_bounds__(128) void k_model_integral(int nelem, const double 
 }
 
 // ------------------------------------------------------------------------------------
+// k_hetero_integral: integrate_3d_model (slave.f90:874-924), one wave per element.
+// The heterogeneity model is a point cloud (load_het_rtpv, heterogeneities.f90:63-110)
+// held in a uniform grid of cells; each random point takes the parameters of its nearest
+// model point (load_model_coeffs, heterogeneities.f90:141-188: kdtree2_n_nearest with
+// nn = 1 on single-precision coordinates, distance ((dx^2 + dy^2) + dz^2) in fp32 as
+// kdtree2.f90:1541-1543 sums it).  Equal distances take the lower model index.  10
+// points per step, 3 parameters (dlnvp, dlnvs, dlnrho) times the basis function, one
+// integrator per (basis, parameter) with volume 1, allowed error 1e-2 and relative error
+// 1e-1, at most 9999 steps.
+// ------------------------------------------------------------------------------------
+constexpr int kHetPts = 10, kHetPar = 3, kHetMaxStep = 9999;
+
+struct HetGrid {
+    int n, nx, ny, nz;
+    double x0, y0, z0, h;            // grid origin and cell size (m)
+    const float *xyz;                // [n][3] model points sorted by cell (fp32)
+    const int *idx;                  // [n] model index of each sorted point
+    const int *cell_start;           // [nx * ny * nz + 1]
+    const float *dln;                // [n][3] dlnvp, dlnvs, dlnrho by model index
+};
+
+__device__ inline int het_cell(double q, double o, double h, int n) {
+    int c = (int)floor((q - o) / h);
+    return c < 0 ? 0 : (c >= n ? n - 1 : c);
+}
+
+// Exact nearest model point: shells of cells of growing Chebyshev radius around the
+// query's cell, until every cell not yet searched lies farther than the best distance.
+__device__ inline int het_nearest(const HetGrid &g, float qx, float qy, float qz) {
+    const int cx = het_cell(qx, g.x0, g.h, g.nx), cy = het_cell(qy, g.y0, g.h, g.ny),
+              cz = het_cell(qz, g.z0, g.h, g.nz);
+    float best = 3.0e38f;
+    int bi = 0x7fffffff;
+    for (int k = 0;; ++k) {
+        const int zlo = max(cz - k, 0), zhi = min(cz + k, g.nz - 1);
+        const int ylo = max(cy - k, 0), yhi = min(cy + k, g.ny - 1);
+        const int xlo = max(cx - k, 0), xhi = min(cx + k, g.nx - 1);
+        for (int z = zlo; z <= zhi; ++z)
+            for (int y = ylo; y <= yhi; ++y) {
+                const bool face = abs(z - cz) == k || abs(y - cy) == k;
+                for (int x = xlo; x <= xhi; x += (face || k == 0) ? 1 : max(1, xhi - xlo)) {
+                    if (!face && abs(x - cx) != k) continue;
+                    const int c = (z * g.ny + y) * g.nx + x;
+                    for (int j = g.cell_start[c]; j < g.cell_start[c + 1]; ++j) {
+                        const float dx = g.xyz[3 * j] - qx, dy = g.xyz[3 * j + 1] - qy, dz = g.xyz[3 * j + 2] - qz;
+                        const float d = (dx * dx + dy * dy) + dz * dz;
+                        const int id = g.idx[j];
+                        if (d < best || (d == best && id < bi)) { best = d; bi = id; }
+                    }
+                }
+            }
+        // distance from the query to the faces of the searched block that are inside the
+        // grid; beyond them lie the cells not searched yet
+        double m = 1e300;
+        if (cx - k > 0) m = fmin(m, (double)qx - (g.x0 + (cx - k) * g.h));
+        if (cx + k < g.nx - 1) m = fmin(m, g.x0 + (cx + k + 1) * g.h - (double)qx);
+        if (cy - k > 0) m = fmin(m, (double)qy - (g.y0 + (cy - k) * g.h));
+        if (cy + k < g.ny - 1) m = fmin(m, g.y0 + (cy + k + 1) * g.h - (double)qy);
+        if (cz - k > 0) m = fmin(m, (double)qz - (g.z0 + (cz - k) * g.h));
+        if (cz + k < g.nz - 1) m = fmin(m, g.z0 + (cz + k + 1) * g.h - (double)qz);
+        if (m >= 1e300) break;                               // the block covers the grid
+        if (bi != 0x7fffffff && m > 0.0 && m * m * (1.0 - 1e-5) > (double)best) break;
+    }
+    return bi;
+}
+
+__global__ __launch_bounds__(64) void k_hetero_integral(int nelem, const double *geom, int vox, int nbasis,
+                                                         const double *abinv, const int64_t *base,
+                                                         const uint8_t *live, HetGrid g, double *out,
+                                                         int *nsteps, int *err)
+{
+    __shared__ double val[kHetPts][kHetPar];
+    __shared__ double wt[kHetPts][8];
+    __shared__ int nconv;
+    const int e = blockIdx.x, tid = threadIdx.x;
+    if (e >= nelem) return;
+    const int nint = nbasis * kHetPar;
+    const bool owner = tid < nint;
+    const int b = tid / kHetPar, k = tid - (tid / kHetPar) * kHetPar;
+    if (!live[e]) {
+        if (owner) out[((size_t)e * nbasis + b) * kHetPar + k] = 0.0;
+        if (tid == 0) nsteps[e] = 0;
+        return;
+    }
+    const double *V = geom + (size_t)e * 12;
+    double fsum = 0.0, f2sum = 0.0, integ = 0.0;
+    int64_t nmodels = 0;
+    bool conv = false;
+    int istep = 0;
+    for (;;) {
+        // allallconverged (mc_integration.f90:322-340)
+        if (tid == 0) nconv = 0;
+        __syncthreads();
+        if (owner && conv) atomicAdd(&nconv, 1);
+        __syncthreads();
+        if (nconv == nint) break;
+        if (tid < kHetPts) {
+            double x[3];
+            draw_point(base[e] + (int64_t)istep * kHetPts + tid, V, vox, 1, x);
+            const int j = het_nearest(g, (float)x[0], (float)x[1], (float)x[2]);
+#pragma unroll
+            for (int q = 0; q < kHetPar; ++q) val[tid][q] = (double)g.dln[3 * (size_t)j + q];
+            bool bad = false;
+            for (int bb = 0; bb < nbasis; ++bb) {
+                double w = 1.0;
+                if (nbasis > 1 && !vox) {
+                    w = tet_bary(abinv + (size_t)e * 16, V, x, bb);
+                    bad |= (w < -1e-9 || w > 1.0 + 1e-9);
+                }
+                wt[tid][bb] = w;
+            }
+            if (bad) atomicOr(err, 4);
+        }
+        __syncthreads();
+        if (owner) {
+            nmodels += kHetPts;
+            if (!conv) {
+                double s1 = 0.0, s2 = 0.0;
+                for (int i = 0; i < kHetPts; ++i) s1 += val[i][k] * wt[i][b];
+                for (int i = 0; i < kHetPts; ++i) {
+                    const double f = val[i][k] * wt[i][b];
+                    s2 += f * f;
+                }
+                fsum = fsum + s1;
+                f2sum = f2sum + s2;
+                integ = fsum * 1.0 / (double)nmodels;
+                double var;
+                if (nmodels > 2) {
+                    var = 1.0 * 1.0 / (double)(nmodels - 1)
+                        * ((f2sum / (double)nmodels) - (fsum / (double)nmodels) * (fsum / (double)nmodels));
+                    if (var < 1e-2 * 1e-2) conv = true;
+                    else if (sqrt(var) < 1e-1 * fabs(integ)) conv = true;
+                } else {
+                    var = integ * 1e10;
+                }
+                if (integ != integ || var != var) atomicOr(err, 8);
+            }
+        }
+        __syncthreads();
+        ++istep;
+        if (istep >= kHetMaxStep) break;
+    }
+    if (owner) out[((size_t)e * nbasis + b) * kHetPar + k] = integ;
+    if (tid == 0) nsteps[e] = istep;
+}
+
+// ------------------------------------------------------------------------------------
 // k_locate: one thread per (point, frame).  Frame 0 = source, 1 + r = receiver r.
 // active (nullable) gates bwd frames: active[slot_of_point * R + r].
 // ------------------------------------------------------------------------------------

@@ -19,7 +19,7 @@ module mck_gpu
        MCK_PARAM_VPH = 6, MCK_PARAM_VPV = 7, MCK_PARAM_ETA = 8, MCK_PARAM_RHO = 9
   integer(c_int32_t), parameter, public :: MCK_MISFIT_CC = 0, MCK_MISFIT_AM = 1
   integer(c_int32_t), parameter, public :: MCK_RES_KERNEL = 0, MCK_RES_VARIANCE = 1, &
-       MCK_RES_ERROR = 2, MCK_RES_MODEL = 3, MCK_RES_NITER = 4
+       MCK_RES_ERROR = 2, MCK_RES_MODEL = 3, MCK_RES_NITER = 4, MCK_RES_HETERO = 5
   integer(c_int32_t), parameter, public :: MCK_INT_PARSEVAL = 0, MCK_INT_TRAPEZOIDAL = 1
   integer(c_int32_t), parameter, public :: MCK_FILTER_IDENT = 0, MCK_FILTER_GABOR = 1, &
        MCK_FILTER_BUTTER_LP = 2, MCK_FILTER_BUTTER_HP = 3, MCK_FILTER_BUTTER_BP = 4, &
@@ -76,6 +76,7 @@ module mck_gpu
 
   public :: mck_init, mck_slave_work, mck_slave_work_plain, mck_locate_points, &
             mck_load_strain_points, mck_base_kernels, mck_seismograms, mck_integrate_model, &
+            mck_set_hetero, mck_integrate_hetero, &
             mck_results_init, mck_results_add, mck_results_get, mck_results_load, &
             mck_results_export, &
             mck_get_stats, mck_finalize, &
@@ -161,12 +162,34 @@ module mck_gpu
        type(c_ptr), value :: halton_offsets
      end function mck_integrate_model
 
+     ! heterogeneity model (load_het_rtpv): rtp(3, nhet) r km, lat, lon; dlnhet(3, nhet)
+     integer(c_int) function mck_set_hetero(nhet, rtp, dlnhet) bind(c, name='mck_set_hetero')
+       import :: c_int, c_int32_t, c_double, c_float
+       integer(c_int32_t), value :: nhet
+       real(c_double), intent(in) :: rtp(*)
+       real(c_float), intent(in) :: dlnhet(*)
+     end function mck_set_hetero
+
+     ! integrate_3d_model for a task: hetero(3, nbasis, nelems)
+     integer(c_int) function mck_integrate_hetero(elem_type, nvpe, nelems, nbasis, nvertices, &
+                                                  vertices, conn, hetero, nsteps, halton_step, &
+                                                  halton_offsets) bind(c, name='mck_integrate_hetero')
+       import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+       integer(c_int32_t), value :: elem_type, nvpe, nelems, nbasis, nvertices
+       real(c_double), intent(in) :: vertices(*)
+       integer(c_int32_t), intent(in) :: conn(*)
+       real(c_double), intent(out) :: hetero(*)
+       integer(c_int32_t), intent(out) :: nsteps(*)
+       integer(c_int64_t), intent(inout) :: halton_step
+       type(c_ptr), value :: halton_offsets
+     end function mck_integrate_hetero
+
      ! the master's accumulators in device memory (extract_receive_buffer,
      ! master_queue.f90:253-323); connectivity(nbasis, nelements), c_null_ptr when volumetric
      integer(c_int) function mck_results_init(int_type, npoints, nelements, nbasis, nmodel, &
-                                              connectivity) bind(c, name='mck_results_init')
+                                              nhet, connectivity) bind(c, name='mck_results_init')
        import :: c_int, c_int32_t, c_ptr
-       integer(c_int32_t), value :: int_type, npoints, nelements, nbasis, nmodel
+       integer(c_int32_t), value :: int_type, npoints, nelements, nbasis, nmodel, nhet
        type(c_ptr), value :: connectivity
      end function mck_results_init
 
@@ -174,28 +197,28 @@ module mck_gpu
      ! mck_slave_work returns them (host or device addresses), all c_null_ptr = the
      ! last mck_slave_work's results in device memory
      integer(c_int) function mck_results_add(nelems, elems, kernel_values, kernel_variance, &
-                                             niterations, computation_time, model, rank) &
+                                             niterations, computation_time, model, hetero, rank) &
                                              bind(c, name='mck_results_add')
        import :: c_int, c_int32_t, c_ptr
        integer(c_int32_t), value :: nelems, rank
        integer(c_int32_t), intent(in) :: elems(*)
-       type(c_ptr), value :: kernel_values, kernel_variance, niterations, computation_time, model
+       type(c_ptr), value :: kernel_values, kernel_variance, niterations, computation_time, model, hetero
      end function mck_results_add
 
      ! K_x, Var (nkernel, npoints), Bg_Model (nmodel, npoints), niterations
      ! (nkernel, nelements), computation_time, element_proc (nelements): c_null_ptr skips
-     integer(c_int) function mck_results_get(kernel, variance, model, niterations, &
+     integer(c_int) function mck_results_get(kernel, variance, model, hetero, niterations, &
                                              computation_time, element_proc) &
                                              bind(c, name='mck_results_get')
        import :: c_int, c_ptr
-       type(c_ptr), value :: kernel, variance, model, niterations, computation_time, element_proc
+       type(c_ptr), value :: kernel, variance, model, hetero, niterations, computation_time, element_proc
      end function mck_results_get
 
-     integer(c_int) function mck_results_load(kernel, variance, model, niterations, &
+     integer(c_int) function mck_results_load(kernel, variance, model, hetero, niterations, &
                                               computation_time, element_proc) &
                                               bind(c, name='mck_results_load')
        import :: c_int, c_ptr
-       type(c_ptr), value :: kernel, variance, model, niterations, computation_time, element_proc
+       type(c_ptr), value :: kernel, variance, model, hetero, niterations, computation_time, element_proc
      end function mck_results_load
 
      ! finalize's single-precision variables, out(npoints, nentries) as the files hold them

@@ -8,13 +8,15 @@ import os
 import numpy as np
 
 from . import abi, dbfile
-from .results import (MCK_RES_ERROR, MCK_RES_KERNEL, MCK_RES_MODEL, MCK_RES_NITER,
-                      MCK_RES_VARIANCE, ONVERTICES)
+from .results import (MCK_RES_ERROR, MCK_RES_HETERO, MCK_RES_KERNEL, MCK_RES_MODEL,
+                      MCK_RES_NITER, MCK_RES_VARIANCE, ONVERTICES)
 
 EXPORTS = ["mck_write_xdmf", "mck_write_csr", "mck_write_intermediate", "mck_read_intermediate"]
 
 # background-model parameter names of the model variable (background_model.f90:34-37)
 MODEL_NAMES = ["vp", "vs", "rho", "vph", "vpv", "vsh", "vsv", "eta", "phi", "xi", "lam", "mu"]
+# heterogeneity parameters as the output stores them (heterogeneities.f90:19-22)
+HETERO_NAMES = ["dlnvp", "dlnvs", "dlnrho"]
 
 
 class MckOutVar(C.Structure):
@@ -119,6 +121,8 @@ def finalize(results, output_file, kernel_names, element_type, vertices, connect
         var.append(("error", cells, results.export(MCK_RES_ERROR), ["err_" + n for n in kernel_names]))
         if results.nmodel:
             var.append(("model", cells, results.export(MCK_RES_MODEL), MODEL_NAMES[:results.nmodel]))
+        if results.nhet:
+            var.append(("hetero", cells, results.export(MCK_RES_HETERO), HETERO_NAMES[:results.nhet]))
     elif dump_type in ("csr", "ascii"):
         write_csr(output_file + "_kernel", results.export(MCK_RES_KERNEL), allowed_error, ascii=dump_type == "ascii")
     else:

@@ -10,8 +10,8 @@ import ctypes as C
 import numpy as np
 
 from . import abi
-from .abi import (MCK_RES_ERROR, MCK_RES_KERNEL, MCK_RES_MODEL,  # noqa: F401 (re-exported)
-                  MCK_RES_NITER, MCK_RES_VARIANCE)
+from .abi import (MCK_RES_ERROR, MCK_RES_HETERO, MCK_RES_KERNEL,  # noqa: F401 (re-exported)
+                  MCK_RES_MODEL, MCK_RES_NITER, MCK_RES_VARIANCE)
 
 ONVERTICES, VOLUMETRIC = 0, 1
 
@@ -29,18 +29,18 @@ class Results:
     """Accumulators of one inversion mesh (int_type ONVERTICES: npoints = vertices and
     connectivity [nelements][nbasis] 1-based; VOLUMETRIC: npoints = nelements)."""
 
-    def __init__(self, kernel, int_type, npoints, nelements, nbasis=1, nmodel=0, connectivity=None):
+    def __init__(self, kernel, int_type, npoints, nelements, nbasis=1, nmodel=0, connectivity=None, nhet=0):
         from . import _check
         self.k = kernel
         self.int_type, self.npoints, self.nelements = int_type, npoints, nelements
-        self.nbasis, self.nmodel, self.nk = nbasis, nmodel, kernel.nk
+        self.nbasis, self.nmodel, self.nhet, self.nk = nbasis, nmodel, nhet, kernel.nk
         conn = None if connectivity is None else np.ascontiguousarray(connectivity, np.int32)
         self._check = _check
-        _check(kernel.lib.mck_results_init(int_type, npoints, nelements, nbasis, nmodel,
+        _check(kernel.lib.mck_results_init(int_type, npoints, nelements, nbasis, nmodel, nhet,
                                            abi.ptr(conn, C.c_int32)))
 
     def add(self, elems, values=None, variance=None, niterations=None, computation_time=None,
-            model=None, rank=1):
+            model=None, rank=1, hetero=None):
         """One task (extract_receive_buffer).  values/variance/niterations/computation_time
         all None: the last slave_work's results, still in device memory.  Host arrays or
         device tensors otherwise, laid out as slave_work returns them."""
@@ -55,7 +55,7 @@ class Results:
             return a
 
         args = [arr(values, np.float64), arr(variance, np.float64), arr(niterations, np.int32),
-                arr(computation_time, np.float64), arr(model, np.float64)]
+                arr(computation_time, np.float64), arr(model, np.float64), arr(hetero, np.float64)]
         self._check(self.k.lib.mck_results_add(len(elems), abi.ptr(elems, C.c_int32),
                                                *[_addr(a) for a in args], rank))
 
@@ -64,19 +64,21 @@ class Results:
         npt, ne, nk = self.npoints, self.nelements, self.nk
         out = dict(kernel=np.zeros((npt, nk)), variance=np.zeros((npt, nk)),
                    model=np.zeros((npt, self.nmodel)) if self.nmodel else None,
+                   hetero=np.zeros((npt, self.nhet)) if self.nhet else None,
                    niterations=np.zeros((ne, nk), np.int32), computation_time=np.zeros(ne),
                    element_proc=np.zeros(ne, np.int32))
         self._check(self.k.lib.mck_results_get(
             abi.ptr(out["kernel"], C.c_double), abi.ptr(out["variance"], C.c_double),
-            abi.ptr(out["model"], C.c_double), abi.ptr(out["niterations"], C.c_int32),
+            abi.ptr(out["model"], C.c_double), abi.ptr(out["hetero"], C.c_double),
+            abi.ptr(out["niterations"], C.c_int32),
             abi.ptr(out["computation_time"], C.c_double), abi.ptr(out["element_proc"], C.c_int32)))
         return out
 
     def load(self, kernel=None, variance=None, model=None, niterations=None, computation_time=None,
-             element_proc=None):
+             element_proc=None, hetero=None):
         """Replaces accumulators (resume from intermediate results, or merge partial sums)."""
         keep = [None if a is None else np.ascontiguousarray(a, dt) for a, dt in
-                ((kernel, np.float64), (variance, np.float64), (model, np.float64),
+                ((kernel, np.float64), (variance, np.float64), (model, np.float64), (hetero, np.float64),
                  (niterations, np.int32), (computation_time, np.float64), (element_proc, np.int32))]
         self._check(self.k.lib.mck_results_load(*[_addr(a) for a in keep]))
 
@@ -86,6 +88,8 @@ class Results:
             shape = (self.nk, self.nelements)
         elif which == MCK_RES_MODEL:
             shape = (self.nmodel, self.npoints)
+        elif which == MCK_RES_HETERO:
+            shape = (self.nhet, self.npoints)
         else:
             shape = (self.nk, self.npoints)
         out = np.zeros(shape, np.float32)

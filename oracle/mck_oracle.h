@@ -128,6 +128,15 @@ int      ora_results_add(int int_type, int nbasis, int nk, int nmodel, int npoin
  * ia for niterations */
 void     ora_results_export(int which, int np, int ncol, const double *a, const double *kx,
                             const int32_t *ia, float *out);
+/* load_het_rtpv's coordinates (heterogeneities.f90:63-110), the brute-force nearest
+ * model point of load_model_coeffs (heterogeneities.f90:141-188), and
+ * integrate_3d_model (slave.f90:874-924) for a task: hetero[nelems][nbasis][3] */
+int      ora_het_xyz(int nhet, const double *rtp, float *xyz);
+int      ora_het_nearest(int nhet, const float *xyz, float qx, float qy, float qz);
+int      ora_integrate_hetero(int nhet, const double *rtp, const float *dlnhet, int elem_type, int nvpe,
+                              int nelems, int nbasis, const double *vertices, const int32_t *conn,
+                              double *hetero, int32_t *nsteps, int64_t *halton_step,
+                              const int64_t *halton_offsets);
 /* Test hook: which 2 = xyz_earth -> xyz_src (rotations.f90:615-671), 3 = xyz -> src
  * (rotations.f90:423-455); t[6] Voigt, in place. */
 void     ora_rotate_voigt_inv(int which, double *t, double phi, double theta);

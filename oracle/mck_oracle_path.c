@@ -1350,6 +1350,118 @@ int ora_integrate_model(ora_ctx *C, int elem_type, int nvpe, int nelems, int nba
     return 0;
 }
 
+/* load_het_rtpv (heterogeneities.f90:63-110): r km, latitude, longitude in degrees to
+ * single-precision Cartesian coordinates (spherical_to_cartesian_point,
+ * random_voxel.f90:214-227, in double, then real(sp)); returns nonzero for a latitude
+ * outside [-90, 90] */
+int ora_het_xyz(int nhet, const double *rtp, float *xyz)
+{
+    int i;
+    for (i = 0; i < nhet; ++i) {
+        const double r = rtp[3 * i], t = rtp[3 * i + 1], p = rtp[3 * i + 2];
+        double sph[3];
+        if (fabs(t) > 90.0) return MCK_EINVAL;
+        sph[0] = r * 1000.0;
+        sph[1] = (90.0 - t) * DEG2RAD;
+        sph[2] = p * DEG2RAD;
+        xyz[3 * i] = (float)(sph[0] * cos(sph[2]) * sin(sph[1]));
+        xyz[3 * i + 1] = (float)(sph[0] * sin(sph[2]) * sin(sph[1]));
+        xyz[3 * i + 2] = (float)(sph[0] * cos(sph[1]));
+    }
+    return 0;
+}
+
+/* the nearest model point of load_model_coeffs (heterogeneities.f90:141-188,
+ * kdtree2_n_nearest with nn = 1), by brute force over all points: single-precision
+ * squared distance summed as kdtree2.f90:1541-1543 does, the lower index on equal
+ * distances; 0-based */
+int ora_het_nearest(int nhet, const float *xyz, float qx, float qy, float qz)
+{
+    float best = 3.0e38f;
+    int bi = -1, j;
+    for (j = 0; j < nhet; ++j) {
+        const float dx = xyz[3 * j] - qx, dy = xyz[3 * j + 1] - qy, dz = xyz[3 * j + 2] - qz;
+        const float d = (dx * dx + dy * dy) + dz * dz;
+        if (d < best) { best = d; bi = j; }
+    }
+    return bi;
+}
+
+/* integrate_3d_model (slave.f90:874-924) for a task: hetero[nelems][nbasis][3] MC
+ * integrals (volume 1, allowed error 1e-2, relative 1e-1) of dlnvp, dlnvs, dlnrho of the
+ * nearest model point times the basis function, 10 points per step (Halton in
+ * tetrahedra, the pseudo-random stream in voxels), at most 9999 steps */
+int ora_integrate_hetero(int nhet, const double *rtp, const float *dlnhet, int elem_type, int nvpe,
+                         int nelems, int nbasis, const double *vertices, const int32_t *conn,
+                         double *hetero, int32_t *nsteps, int64_t *halton_step,
+                         const int64_t *halton_offsets)
+{
+    enum { NP = 10, NF = 3, MAXSTEP = 9999 };
+    const int vox = elem_type == 5;
+    static const int64_t seed[3] = {20, 20, 20}, leap[3] = {2, 2, 2};
+    static const int32_t bases[3] = {2, 3, 5};
+    double pts[NP * 3], hal[NP * 3], wts[NP], vals[NP * NF];
+    int near[NP];
+    ora_mc mcs[8];
+    int e, i, k, b, rc = 0;
+    float *xyz;
+    if (!((elem_type == 4 && nvpe == 4 && (nbasis == 1 || nbasis == 4))
+          || (vox && nvpe == 8 && (nbasis == 1 || nbasis == 8))) || nhet <= 0) return MCK_EINVAL;
+    xyz = (float *)malloc((size_t)nhet * 3 * sizeof(float));
+    if (ora_het_xyz(nhet, rtp, xyz)) { free(xyz); return MCK_EINVAL; }
+    for (e = 0; e < nelems && !rc; ++e) {
+        double v[24], A[16];
+        int64_t pos = halton_offsets ? halton_offsets[e] : *halton_step + (int64_t)e * MAXSTEP * NP;
+        int istep = 0, all;
+        if (conn[e * nvpe] <= 0) {
+            for (k = 0; k < nbasis * NF; ++k) hetero[(size_t)e * nbasis * NF + k] = 0.0;
+            if (nsteps) nsteps[e] = 0;
+            continue;
+        }
+        for (i = 0; i < nvpe; ++i) {
+            int vid = conn[e * nvpe + i] - 1;
+            v[3 * i] = vertices[3 * vid]; v[3 * i + 1] = vertices[3 * vid + 1]; v[3 * i + 2] = vertices[3 * vid + 2];
+        }
+        if (!vox && nbasis > 1) tet_abinv(v, A);
+        for (b = 0; b < nbasis; ++b) mc_init(&mcs[b], NF, 1.0, 1e-2, 1e-1);
+        for (;;) {
+            all = 1;
+            for (b = 0; b < nbasis; ++b) for (k = 0; k < NF; ++k) if (!mcs[b].conv[k]) all = 0;
+            if (all) break;
+            if (!vox) ora_halton(3, seed, bases, leap, pos, NP, hal);
+            else ora_prng_points(pos, NP, hal);
+            pos += NP;
+            if (vox) ora_vox_points(v, NP, hal, pts);
+            else ora_tet_points(v, NP, hal, pts);
+            for (i = 0; i < NP; ++i)
+                near[i] = ora_het_nearest(nhet, xyz, (float)pts[3 * i], (float)pts[3 * i + 1], (float)pts[3 * i + 2]);
+            for (b = 0; b < nbasis && !rc; ++b) {
+                if (nbasis == 1 || vox) {
+                    for (i = 0; i < NP; ++i) wts[i] = 1.0;
+                } else if (tet_weights(v, A, b, NP, pts, wts)) {
+                    rc = MCK_EINVAL;
+                    break;
+                }
+                for (i = 0; i < NP; ++i)
+                    for (k = 0; k < NF; ++k) vals[i * NF + k] = (double)dlnhet[3 * (size_t)near[i] + k] * wts[i];
+                rc = mc_check(&mcs[b], NP, vals);
+            }
+            if (rc) break;
+            istep += 1;
+            if (istep >= MAXSTEP) break;
+        }
+        for (b = 0; b < nbasis; ++b) {
+            for (k = 0; k < NF; ++k) hetero[((size_t)e * nbasis + b) * NF + k] = mcs[b].integral[k];
+            mc_free(&mcs[b]);
+        }
+        if (nsteps) nsteps[e] = istep;
+    }
+    free(xyz);
+    if (rc) return rc;
+    if (!halton_offsets) *halton_step += (int64_t)nelems * MAXSTEP * NP;
+    return 0;
+}
+
 /* G of every filter [nfilter][nomega][2] */
 void ora_ad_transfer(ora_ctx *C, double *G)
 {

@@ -62,6 +62,7 @@ int ora_results_add(int int_type, int nbasis, int nk, int nmodel, int npoints, i
 
 void ora_results_export(int which, int np, int ncol, const double *a, const double *kx,
                         const int32_t *ia, float *out) {
+    /* MCK_RES_KERNEL, _VARIANCE, _MODEL, _HETERO: real(a, sp); _ERROR; _NITER */
     for (int p = 0; p < np; ++p)
         for (int c = 0; c < ncol; ++c) {
             const int64_t i = (int64_t)p * ncol + c;

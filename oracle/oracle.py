@@ -84,6 +84,10 @@ def lib():
         L.ora_seismograms.argtypes = [C.c_void_p, f64p]
         L.ora_integrate_model.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, f64p, i32p,
                                           f64p, i32p, i64p, i64p]
+        L.ora_het_xyz.argtypes = [C.c_int, f64p, f32p]
+        L.ora_het_nearest.argtypes = [C.c_int, f32p, C.c_float, C.c_float, C.c_float]
+        L.ora_integrate_hetero.argtypes = [C.c_int, f64p, f32p, C.c_int, C.c_int, C.c_int, C.c_int, f64p, i32p,
+                                           f64p, i32p, i64p, i64p]
         L.ora_results_add.argtypes = [C.c_int] * 6 + [i32p, C.c_int, i32p, f64p, f64p, i32p, f64p, f64p,
                                                        C.c_int, f64p, f64p, f64p, i32p, f64p, i32p]
         L.ora_results_export.argtypes = [C.c_int, C.c_int, C.c_int, f64p, f64p, i32p, f32p]
@@ -486,8 +490,10 @@ class Results:
     the checker of mck_results_*: K, V [npoints][nk], M [npoints][nmodel], niter
     [nelements][nk], ctime, proc [nelements]."""
 
-    def __init__(self, int_type, npoints, nelements, nbasis, nk, nmodel=0, connectivity=None):
+    def __init__(self, int_type, npoints, nelements, nbasis, nk, nmodel=0, connectivity=None, nhet=0):
         self.int_type, self.np, self.ne, self.nb, self.nk, self.nm = int_type, npoints, nelements, nbasis, nk, nmodel
+        self.nh = nhet
+        self.H = np.zeros((npoints, max(nhet, 1)))
         self.conn = None if connectivity is None else np.ascontiguousarray(connectivity, np.int32)
         self.K = np.zeros((npoints, nk))
         self.V = np.zeros((npoints, nk))
@@ -496,7 +502,7 @@ class Results:
         self.ctime = np.zeros(nelements)
         self.proc = np.zeros(nelements, np.int32)
 
-    def add(self, elems, values, variance, niterations, computation_time=None, model=None, rank=1):
+    def add(self, elems, values, variance, niterations, computation_time=None, model=None, rank=1, hetero=None):
         elems = np.ascontiguousarray(elems, np.int32)
         values = np.ascontiguousarray(values, float)
         variance = np.ascontiguousarray(variance, float)
@@ -511,6 +517,15 @@ class Results:
                                    P(self.proc, C.c_int32))
         if rc:
             raise RuntimeError("oracle results_add: id out of range")
+        if self.nh and hetero is not None:
+            # Het_Model by the same loop (master_queue.f90:301-304): no kernel columns
+            hd = np.ascontiguousarray(hetero, float)
+            z = np.zeros(1)
+            lib().ora_results_add(self.int_type, self.nb, 0, self.nh, self.np, self.ne,
+                                  None if self.conn is None else P(self.conn, C.c_int32), len(elems),
+                                  P(elems, C.c_int32), P(z), P(z), P(niterations, C.c_int32),
+                                  None if ct is None else P(ct), P(hd), rank, P(z), P(z), P(self.H),
+                                  P(self.niter, C.c_int32), P(self.ctime), P(self.proc, C.c_int32))
 
     def export(self, which):
         from mc_kernel_amd import abi as A
@@ -518,9 +533,46 @@ class Results:
             a, ncol, n = None, self.nk, self.ne
         elif which == A.MCK_RES_MODEL:
             a, ncol, n = self.M, self.nm, self.np
+        elif which == A.MCK_RES_HETERO:
+            a, ncol, n = self.H, self.nh, self.np
         else:
             a, ncol, n = (self.K if which == A.MCK_RES_KERNEL else self.V), self.nk, self.np
         out = np.zeros((ncol, n), np.float32)
         lib().ora_results_export(which, n, ncol, None if a is None else P(np.ascontiguousarray(a)),
                                  P(self.K), P(self.niter, C.c_int32), P(out, C.c_float))
         return out
+
+
+def het_xyz(rtp):
+    """load_het_rtpv's single-precision Cartesian coordinates of (r km, lat, lon) rows."""
+    rtp = np.ascontiguousarray(rtp, float).reshape(-1, 3)
+    out = np.zeros((rtp.shape[0], 3), np.float32)
+    if lib().ora_het_xyz(rtp.shape[0], P(rtp), P(out, C.c_float)):
+        raise ValueError("latitude outside [-90, 90]")
+    return out
+
+
+def het_nearest(xyz, q):
+    """0-based nearest model point of each query row (brute force, fp32 distances)."""
+    xyz = np.ascontiguousarray(xyz, np.float32)
+    return np.array([lib().ora_het_nearest(xyz.shape[0], P(xyz, C.c_float), float(a), float(b), float(c))
+                     for a, b, c in np.asarray(q, np.float32)])
+
+
+def integrate_hetero(rtp, dln, verts, conn, nbasis=1, halton_step=0, halton_offsets=None):
+    """integrate_3d_model for a task: (hetero [ne][nbasis][3], nsteps [ne], new step)."""
+    rtp = np.ascontiguousarray(rtp, float).reshape(-1, 3)
+    dln = np.ascontiguousarray(dln, np.float32).reshape(-1, 3)
+    verts = np.ascontiguousarray(verts, float).reshape(-1, 3)
+    conn = np.ascontiguousarray(conn, np.int32)
+    ne, nvpe = conn.shape
+    out = np.zeros((ne, nbasis, 3))
+    nsteps = np.zeros(ne, np.int32)
+    step = np.array([halton_step], np.int64)
+    offs = None if halton_offsets is None else np.ascontiguousarray(halton_offsets, np.int64)
+    rc = lib().ora_integrate_hetero(rtp.shape[0], P(rtp), P(dln, C.c_float), 4 if nvpe == 4 else 5, nvpe, ne,
+                                    nbasis, P(verts), P(conn, C.c_int32), P(out), P(nsteps, C.c_int32),
+                                    P(step, C.c_int64), None if offs is None else P(offs, C.c_int64))
+    if rc:
+        raise RuntimeError("oracle integrate_hetero failed (%d)" % rc)
+    return out, nsteps, int(step[0])

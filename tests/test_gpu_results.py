@@ -4,7 +4,8 @@ in tests/test_oracle_results.py), bit for bit: both add each point's contributio
 its FP64 accumulator one at a time in task order.
 
   * several random tasks with shared vertices, empty slots and recomputed elements,
-    4 basis functions, model integrals: accumulators and all five exported variables;
+    4 basis functions, model and heterogeneity integrals: accumulators and all six
+    exported variables;
   * volumetric mode; an odd kernel count;
   * the results of the last slave_work / integrate_model taken from device memory give
     the same accumulators as the same results passed from the host; device tensors as
@@ -22,7 +23,8 @@ from mc_kernel_amd import abi, synth
 
 pytestmark = pytest.mark.gpu
 
-EXPORTS = [abi.MCK_RES_KERNEL, abi.MCK_RES_VARIANCE, abi.MCK_RES_ERROR, abi.MCK_RES_MODEL, abi.MCK_RES_NITER]
+EXPORTS = [abi.MCK_RES_KERNEL, abi.MCK_RES_VARIANCE, abi.MCK_RES_ERROR, abi.MCK_RES_MODEL, abi.MCK_RES_NITER,
+           abi.MCK_RES_HETERO]
 
 
 def _kernel(odd=False):
@@ -47,7 +49,8 @@ def _tasks(rng, ne, nk, nb, ntask=5, size=40):
         nit = rng.integers(1, 64, (size, nk)).astype(np.int32)
         ct = rng.random(size)
         model = rng.standard_normal((size, nb, 12))
-        out.append((t, vals, var, nit, ct, model, int(rng.integers(1, 9))))
+        het = rng.standard_normal((size, nb, 3)) * 1e-2
+        out.append((t, vals, var, nit, ct, model, int(rng.integers(1, 9)), het))
     return out
 
 
@@ -57,11 +60,13 @@ def _compare(res, ref):
     np.testing.assert_array_equal(got["variance"], ref.V)
     if ref.nm:
         np.testing.assert_array_equal(got["model"], ref.M)
+    if ref.nh:
+        np.testing.assert_array_equal(got["hetero"], ref.H)
     np.testing.assert_array_equal(got["niterations"], ref.niter)
     np.testing.assert_array_equal(got["computation_time"], ref.ctime)
     np.testing.assert_array_equal(got["element_proc"], ref.proc)
     for w in EXPORTS:
-        if w == abi.MCK_RES_MODEL and not ref.nm:
+        if (w == abi.MCK_RES_MODEL and not ref.nm) or (w == abi.MCK_RES_HETERO and not ref.nh):
             continue
         np.testing.assert_array_equal(res.export(w), ref.export(w), err_msg=str(w))
 
@@ -78,11 +83,11 @@ def test_random_tasks_match_oracle(int_type, odd):
         npt = nv if int_type == 0 else ne
         conn = np.array([rng.choice(nv, 4, replace=False) + 1 for _ in range(ne)], np.int32)
         c = conn if int_type == 0 else None
-        res = k.results(int_type, npt, ne, nb, 12, c)
-        ref = oracle.Results(int_type, npt, ne, nb, nk, 12, c)
-        for t, vals, var, nit, ct, model, rank in _tasks(rng, ne, nk, nb):
-            res.add(t, vals, var, nit, ct, model, rank)
-            ref.add(t, vals, var, nit, ct, model, rank)
+        res = k.results(int_type, npt, ne, nb, 12, c, nhet=3)
+        ref = oracle.Results(int_type, npt, ne, nb, nk, 12, c, nhet=3)
+        for t, vals, var, nit, ct, model, rank, het in _tasks(rng, ne, nk, nb):
+            res.add(t, vals, var, nit, ct, model, rank, het)
+            ref.add(t, vals, var, nit, ct, model, rank, het)
         _compare(res, ref)
     finally:
         k.close()
