@@ -284,6 +284,66 @@ def bench_assembly(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_integrals(args):
+    """The model integrals of a c1-sized task (50,000 tetrahedra, one basis function):
+    integrate_1d_model (mck_integrate_model, 100 points per step) and integrate_3d_model
+    (mck_integrate_hetero, 10 points per step) over a global heterogeneity model on a
+    1-degree grid at 10 radii (648,000 points); the FP64 oracle on a bounded sample of
+    the same elements as the CPU baseline."""
+    import mc_kernel_amd as mk
+    from mc_kernel_amd import synth
+    p = synth.make_problem("tiny", ntet=args.integrals_tets, tet_edge=300e3)
+    verts, conn = np.asarray(p.verts, float).reshape(-1, 3), np.asarray(p.conn, np.int32)
+    r = np.linspace(3500.0, 6371.0, 10)
+    lat = np.arange(-89.5, 90.0, 1.0)
+    lon = np.arange(-179.5, 180.0, 1.0)
+    rtp = np.stack(np.meshgrid(r, lat, lon, indexing="ij"), -1).reshape(-1, 3)
+    rng = np.random.default_rng(2)
+    dln = (rng.standard_normal((len(rtp), 3)) * 0.05).astype(np.float32)
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle
+        ns = min(len(conn), 200)
+        o = oracle.Oracle(p.setup)
+        t0 = time.perf_counter()
+        o.integrate_model(verts, conn[:ns])
+        t1 = time.perf_counter()
+        oracle.integrate_hetero(rtp, dln, verts, conn[:20])
+        t2 = time.perf_counter()
+        cpu = {"model": {"value": ns / (t1 - t0), "unit": "elements/s", "cores": 1, "kind": "port",
+                         "sample": "%d elements, FP64 oracle" % ns},
+               "hetero": {"value": 20 / (t2 - t1), "unit": "elements/s", "cores": 1, "kind": "port",
+                          "sample": "20 elements, FP64 oracle with brute-force nearest points"}}
+    k = mk.MCKernel(p.setup)
+    k.set_hetero(rtp, dln)
+    tm, th, steps_m, steps_h = [], [], 0, 0
+    for i in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        _, nsm = k.integrate_model(verts, conn)
+        t1 = time.perf_counter()
+        _, nsh = k.integrate_hetero(verts, conn)
+        t2 = time.perf_counter()
+        if i >= args.warmup:
+            tm.append(t1 - t0)
+            th.append(t2 - t1)
+            steps_m, steps_h = int(nsm.sum()), int(nsh.sum())
+        print("integrals step %d: model %.1f ms, hetero %.1f ms" % (i, (t1 - t0) * 1e3, (t2 - t1) * 1e3),
+              file=sys.stderr, flush=True)
+    k.close()
+    m, h = float(np.mean(tm)), float(np.mean(th))
+    line = {
+        "metric": "model integrals: elements integrated per second (1-D background, 3-D heterogeneities)",
+        "value": len(conn) / m, "unit": "elements/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": (m + h) * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (fp32 model values)", "data": "synthetic tetrahedra and heterogeneity grid",
+        "config": {"workload": "%d tetrahedra, 1 basis function; heterogeneity model %d points (1 deg x 10 radii)"
+                               % (len(conn), len(rtp))},
+        "model_ms": m * 1e3, "hetero_ms": h * 1e3, "hetero_elements_per_s": len(conn) / h,
+        "model_mc_steps": steps_m, "hetero_mc_steps": steps_h, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -310,6 +370,9 @@ def main():
                     help="time the result assembly at the c4 scale instead (one JSON line)")
     ap.add_argument("--assembly-points", type=int, default=800000)
     ap.add_argument("--assembly-task", type=int, default=50000)
+    ap.add_argument("--integrals", action="store_true",
+                    help="time the model and heterogeneity integrals of a task instead (one JSON line)")
+    ap.add_argument("--integrals-tets", type=int, default=50000)
     ap.add_argument("--db-page-elems", type=int, default=0,
                     help="keep the databases in host memory and page them, N elements per "
                          "page (0: resident in HBM, the default workload)")
@@ -328,6 +391,9 @@ def main():
         return
     if args.assembly:
         bench_assembly(args)
+        return
+    if args.integrals:
+        bench_integrals(args)
         return
 
     # CPU baseline first, in fresh processes, before this process touches the GPU

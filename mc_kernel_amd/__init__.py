@@ -175,6 +175,19 @@ class MCKernel:
         from .results import Results
         return Results(self, int_type, npoints, nelements, nbasis, nmodel, connectivity, nhet)
 
+    def seismograms(self):
+        """[nrec][nt] seismograms the kernels use (the set-up's, or by reciprocity)."""
+        out = np.zeros((self.R, self.nt))
+        _check(self.lib.mck_seismograms(abi.ptr(out, C.c_double)))
+        return out
+
+    def base_kernels(self, xyz):
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        out = np.zeros((xyz.shape[0], self.nk, 6))
+        _check(self.lib.mck_base_kernels(xyz.shape[0], abi.ptr(xyz, C.c_double),
+                                         abi.ptr(out, C.c_double)))
+        return out
+
 
 def load_het_rtpv(path):
     """The heterogeneity file read by load_het_rtpv (heterogeneities.f90:63-110): a first
@@ -189,16 +202,3 @@ def load_het_rtpv(path):
     if np.any(np.abs(rows[:, 1]) > 90.0):
         raise ValueError("heterogeneity file %s: latitude outside [-90, 90]" % path)
     return rows[:, :3].astype(np.float64), rows[:, 3:6].astype(np.float32)
-
-    def seismograms(self):
-        """[nrec][nt] seismograms the kernels use (the set-up's, or by reciprocity)."""
-        out = np.zeros((self.R, self.nt))
-        _check(self.lib.mck_seismograms(abi.ptr(out, C.c_double)))
-        return out
-
-    def base_kernels(self, xyz):
-        xyz = np.ascontiguousarray(xyz, np.float64)
-        out = np.zeros((xyz.shape[0], self.nk, 6))
-        _check(self.lib.mck_base_kernels(xyz.shape[0], abi.ptr(xyz, C.c_double),
-                                         abi.ptr(out, C.c_double)))
-        return out

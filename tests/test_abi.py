@@ -113,3 +113,12 @@ def test_fortran_binding_layout(tmp_path):
     for (cname, field), v in got.items():
         if field != "size":
             assert v == getattr(STRUCTS[cname], field).offset, (cname, field)
+
+
+def test_python_binding_methods():
+    """The Python binding keeps one method per entry point (no GPU call)."""
+    import mc_kernel_amd as mk
+    for m in ("slave_work", "stats", "locate", "load_strain", "integrate_model", "set_hetero",
+              "integrate_hetero", "results", "seismograms", "base_kernels", "close"):
+        assert callable(getattr(mk.MCKernel, m, None)), m
+    assert callable(mk.load_het_rtpv)
