@@ -617,7 +617,11 @@ int build_kernels(State &G) {
         }
     // seismograms per receiver (cut_and_add_seismogram, kernel.f90:181-385)
     std::map<std::tuple<int, int, int, int>, int> hidx;  // (rec, filter, misfit, window) -> H
-    std::vector<std::vector<cd>> Hs, Ws;
+    // per distinct window: the weighted samples of the filtered seismogram in the window
+    // (concatenated), first sample, length, normalisation, filter; the window spectra
+    // themselves are computed on the device (k_window_spectra)
+    std::vector<double> w_all, h_norm;
+    std::vector<int> w_off, h_first, h_ncut;
     std::vector<int> hfilt;
     std::vector<RecDesc> recs(G.R);
     std::vector<int> h_of(G.nk, 0);
@@ -686,21 +690,12 @@ int build_kernels(State &G) {
                 }
                 w[n] = dt * om * xs[ifirst + n];
             }
-            std::vector<cd> Hk(nw), Wk(nw);
-            for (int q = 0; q < nw; ++q) {
-                cd Sk = 0.0;
-                for (int n = 0; n < ncut; ++n) {
-                    long long m = ((long long)q * (ifirst + n)) % N;
-                    double ang = 2 * kPi * (double)m / (double)N;
-                    Sk += w[n] * cd(std::cos(ang), std::sin(ang));
-                }
-                double ck = (q == 0 || q == nw - 1) ? 1.0 : 2.0;
-                Hk[q] = -norm * df * ck * Sk * tff[fi][q] * tfb[fi][q] * ts[q];
-                Wk[q] = -norm * df * ck * Sk * rest[q] * ts[q];
-            }
-            h = (int)Hs.size();
-            Hs.push_back(Hk);
-            Ws.push_back(Wk);
+            h = (int)hfilt.size();
+            w_off.push_back((int)w_all.size());
+            w_all.insert(w_all.end(), w.begin(), w.end());
+            h_first.push_back(ifirst);
+            h_ncut.push_back(ncut);
+            h_norm.push_back(norm);
             hfilt.push_back(fi);
             hidx[key] = h;
         }
@@ -794,16 +789,59 @@ int build_kernels(State &G) {
         if ((lrc = layout(true, rc2, kerns, G.pc))) return lrc;
         recs = rc2;
     }
-    std::vector<float2> Hh(Hs.size() * nw);
-    for (size_t h = 0; h < Hs.size(); ++h)
-        for (int q = 0; q < nw; ++q) Hh[h * nw + q] = make_float2((float)Hs[h][q].real(), (float)Hs[h][q].imag());
-    HIPCHECK(G.H.ensure(Hh.size()));
-    HIPCHECK(hipMemcpy(G.H.p, Hh.data(), Hh.size() * sizeof(float2), hipMemcpyHostToDevice));
-    G.nH = (int)Hs.size();
-    for (size_t h = 0; h < Ws.size(); ++h)
-        for (int q = 0; q < nw; ++q) Hh[h * nw + q] = make_float2((float)Ws[h][q].real(), (float)Ws[h][q].imag());
-    HIPCHECK(G.ad_W.ensure(Hh.size()));
-    HIPCHECK(hipMemcpy(G.ad_W.p, Hh.data(), Hh.size() * sizeof(float2), hipMemcpyHostToDevice));
+    {
+        // window spectra on the device: exact angle table 2 pi m / N, the filters' transfer
+        // functions, time shifts and STF factors in FP64
+        const int nH = (int)hfilt.size();
+        std::vector<double> cosT(N), sinT(N);
+        for (int m = 0; m < N; ++m) {
+            const double ang = 2 * kPi * (double)m / (double)N;
+            cosT[m] = std::cos(ang);
+            sinT[m] = std::sin(ang);
+        }
+        std::vector<double2> tf2((size_t)S.nfilter * nw * 2), tr2((size_t)nw * 2);
+        for (int i = 0; i < S.nfilter; ++i)
+            for (int q = 0; q < nw; ++q) {
+                tf2[(size_t)i * nw + q] = make_double2(tff[i][q].real(), tff[i][q].imag());
+                tf2[((size_t)S.nfilter + i) * nw + q] = make_double2(tfb[i][q].real(), tfb[i][q].imag());
+            }
+        for (int q = 0; q < nw; ++q) {
+            tr2[q] = make_double2(ts[q].real(), ts[q].imag());
+            tr2[(size_t)nw + q] = make_double2(rest[q].real(), rest[q].imag());
+        }
+        DBuf<double> d_w, d_tab, d_norm;
+        DBuf<double2> d_tf, d_tr;
+        DBuf<int> d_idx;
+        HIPCHECK(d_w.ensure(std::max<size_t>(w_all.size(), 1)));
+        HIPCHECK(d_tab.ensure(2 * (size_t)N));
+        HIPCHECK(d_norm.ensure(nH));
+        HIPCHECK(d_tf.ensure(tf2.size()));
+        HIPCHECK(d_tr.ensure(tr2.size()));
+        HIPCHECK(d_idx.ensure(4 * (size_t)nH));
+        std::vector<int> idx(4 * (size_t)nH);
+        std::copy(w_off.begin(), w_off.end(), idx.begin());
+        std::copy(h_first.begin(), h_first.end(), idx.begin() + nH);
+        std::copy(h_ncut.begin(), h_ncut.end(), idx.begin() + 2 * nH);
+        std::copy(hfilt.begin(), hfilt.end(), idx.begin() + 3 * nH);
+        if (!w_all.empty()) HIPCHECK(hipMemcpy(d_w.p, w_all.data(), w_all.size() * 8, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(d_tab.p, cosT.data(), (size_t)N * 8, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(d_tab.p + N, sinT.data(), (size_t)N * 8, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(d_norm.p, h_norm.data(), (size_t)nH * 8, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(d_tf.p, tf2.data(), tf2.size() * sizeof(double2), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(d_tr.p, tr2.data(), tr2.size() * sizeof(double2), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(d_idx.p, idx.data(), idx.size() * 4, hipMemcpyHostToDevice));
+        HIPCHECK(G.H.ensure((size_t)nH * nw));
+        HIPCHECK(G.ad_W.ensure((size_t)nH * nw));
+        const long long nth = (long long)nH * nw;
+        hipLaunchKernelGGL(k_window_spectra, dim3((unsigned)((nth + 255) / 256)), dim3(256), 0, G.stream, nH, nw, N,
+                           df, d_w.p, d_idx.p, d_idx.p + nH, d_idx.p + 2 * nH, d_norm.p, d_idx.p + 3 * nH, d_tab.p,
+                           d_tab.p + N, d_tf.p, d_tf.p + (size_t)S.nfilter * nw, d_tr.p, d_tr.p + nw, G.H.p,
+                           G.ad_W.p);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(G.stream));
+        d_w.release(); d_tab.release(); d_norm.release(); d_tf.release(); d_tr.release(); d_idx.release();
+        G.nH = nH;
+    }
     HIPCHECK(G.ad_hfilt.ensure(hfilt.size()));
     HIPCHECK(hipMemcpy(G.ad_hfilt.p, hfilt.data(), hfilt.size() * sizeof(int), hipMemcpyHostToDevice));
     HIPCHECK(G.recs.ensure(G.R));

@@ -387,6 +387,49 @@ __global__ __launch_bounds__(128) void k_model_integral(int nelem, const double 
 }
 
 // ------------------------------------------------------------------------------------
+// k_window_spectra: the frequency-domain window integral of every kernel's filtered
+// seismogram window (cut_and_add_seismogram and the misfit normalisation,
+// kernel.f90:181-385).  One thread per (distinct window h, bin q):
+//   S = sum_n w_h[n] e^{2 pi i q (ifirst_h + n) / N}   (FP64, samples in order, angles
+//       from an exact table of 2 pi m / N)
+//   H = -norm df c_q S TF_fwd(q) TF_bwd(q) ts(q),  W = -norm df c_q S rest(q) ts(q)
+// with c_q = 1 at DC and Nyquist, else 2; both stored in single precision.
+// ------------------------------------------------------------------------------------
+__device__ inline double2 cmul_d(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__global__ __launch_bounds__(256) void k_window_spectra(int nH, int nw, int N, double df, const double *w_all,
+                                                         const int *w_off, const int *ifirst, const int *ncut,
+                                                         const double *norm, const int *filt,
+                                                         const double *cosT, const double *sinT,
+                                                         const double2 *tff, const double2 *tfb,
+                                                         const double2 *ts, const double2 *rest, float2 *H,
+                                                         float2 *W)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)nH * nw) return;
+    const int h = (int)(i / nw), q = (int)(i - (long long)h * nw);
+    const double *w = w_all + w_off[h];
+    int m = (int)(((long long)q * ifirst[h]) % N);
+    double sr = 0.0, si = 0.0;
+    for (int n = 0; n < ncut[h]; ++n) {
+        sr += w[n] * cosT[m];
+        si += w[n] * sinT[m];
+        m += q;
+        if (m >= N) m -= N;
+    }
+    const double ck = (q == 0 || q == nw - 1) ? 1.0 : 2.0;
+    const double sc = -norm[h] * df * ck;
+    const double2 S = make_double2(sc * sr, sc * si);
+    const int f = filt[h];
+    const double2 hv = cmul_d(cmul_d(cmul_d(S, tff[(size_t)f * nw + q]), tfb[(size_t)f * nw + q]), ts[q]);
+    const double2 wv = cmul_d(cmul_d(S, rest[q]), ts[q]);
+    H[i] = make_float2((float)hv.x, (float)hv.y);
+    W[i] = make_float2((float)wv.x, (float)wv.y);
+}
+
+// ------------------------------------------------------------------------------------
 // k_hetero_integral: integrate_3d_model (slave.f90:874-924), one wave per element.
 // The heterogeneity model is a point cloud (load_het_rtpv, heterogeneities.f90:63-110)
 // held in a uniform grid of cells; each random point takes the parameters of its nearest
