@@ -24,7 +24,11 @@
  * tet volume of tests/vertices.TEST; the 12-nearest order against the reference's own
  * kdtree2.f90 (oracle/_ref); the differentiable filter-convolve-window-integrate stage:
  * its forward equals ora_base_kernels bit for bit, its adjoint equals FP64 central
- * differences of the forward (exact for a function linear/quadratic in every input).
+ * differences of the forward (exact for a function linear/quadratic in every input);
+ * the heterogeneity model's nearest point against the reference's own 3-D kd-tree
+ * (oracle/_ref), index and fp32 distance, and the heterogeneity integrals' closed forms
+ * (tests/test_oracle_hetero.py); the result assembly's hand sums, scatter formulation and
+ * error transform (tests/test_oracle_results.py, mck_oracle_results.c).
  *
  * Parity unpinned (restated from the reference text, no fixture or closed form checks
  * them): calc_basekernel (kernel.f90:514-570) including the k_c product as written; the
