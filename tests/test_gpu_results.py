@@ -190,3 +190,28 @@ def test_finalize_files_and_resume(tmp_path):
         np.testing.assert_array_equal(res2.export(abi.MCK_RES_KERNEL), res.export(abi.MCK_RES_KERNEL))
     finally:
         k.close()
+
+
+def test_run_job_tasks_equal_one_task(tmp_path):
+    """run_job over tasks of 3 elements gives the accumulators of one task holding every
+    element (each element keeps its own stream range) and writes finalize's files; with
+    the background-model integrals the model accumulators fill as well."""
+    import mc_kernel_amd as mk
+    from mc_kernel_amd.job import run_job
+    p = synth.make_problem("tiny", ntet=8)
+    k = mk.MCKernel(p.setup)
+    try:
+        a = run_job(k, p.verts, p.conn, nbasis=4, nelems_per_task=3, output_file=str(tmp_path / "job"))
+        ga = a.get()
+        k.halton_step = 0
+        b = run_job(k, p.verts, p.conn, nbasis=4, nelems_per_task=len(p.conn))
+        gb = b.get()
+        c = run_job(k, p.verts, p.conn, nbasis=4, nelems_per_task=3, int_over_background=True)
+        gc = c.get()
+    finally:
+        k.close()
+    for key in ("kernel", "variance", "niterations", "element_proc"):
+        np.testing.assert_array_equal(ga[key], gb[key], err_msg=key)
+    assert np.abs(ga["kernel"]).max() > 0 and (ga["element_proc"] == 1).all()
+    assert os.path.exists(str(tmp_path / "job_kernel.xdmf"))
+    assert np.abs(gc["model"]).max() > 0
