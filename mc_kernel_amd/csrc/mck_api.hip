@@ -1787,6 +1787,15 @@ int finish_task(State &G, const Task &T, double *values, double *variance, int32
         G.last_ctime[e] = tot_steps ? dev_ms * 1e-3 * steps[e] / (double)tot_steps : 0.0;
     }
     if (ctime) std::copy(G.last_ctime.begin(), G.last_ctime.end(), ctime);
+    // skipped slots hold zeros in device memory too, as the host arrays above do, for
+    // mck_results_add's device-resident path
+    for (int e = 0; e < ne; ++e) {
+        if (T.live[e]) continue;
+        HIPCHECK(hipMemsetAsync(G.integral.p + (size_t)e * nb * nk, 0, (size_t)nb * nk * 8, st));
+        HIPCHECK(hipMemsetAsync(G.variance.p + (size_t)e * nb * nk, 0, (size_t)nb * nk * 8, st));
+        HIPCHECK(hipMemsetAsync(G.niter.p + (size_t)e * nk, 0, (size_t)nk * 4, st));
+    }
+    HIPCHECK(hipStreamSynchronize(st));
     G.last_nelems = ne;
     G.last_nbasis = nb;
     // the Halton stream position after the task

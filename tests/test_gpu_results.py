@@ -215,3 +215,25 @@ def test_run_job_tasks_equal_one_task(tmp_path):
     assert np.abs(ga["kernel"]).max() > 0 and (ga["element_proc"] == 1).all()
     assert os.path.exists(str(tmp_path / "job_kernel.xdmf"))
     assert np.abs(gc["model"]).max() > 0
+
+
+def test_device_resident_skipped_slot():
+    """A task slot the slave skipped (first vertex id 0) holds zeros in device memory as
+    in the host arrays, so both paths of mck_results_add agree for it."""
+    k, p = _kernel()
+    try:
+        conn = np.array(p.conn, np.int32)
+        conn[-1] = 0
+        out = k.slave_work(p.verts, conn)
+        ne = len(conn)
+        elems = np.arange(1, ne + 1, dtype=np.int32)
+        a = k.results(1, ne, ne, 1, 0)
+        a.add(elems)
+        b = k.results(1, ne, ne, 1, 0)
+        b.add(elems, out["values"], out["variance"], out["niterations"], out["computation_time"])
+        ga, gb = a.get(), b.get()
+        for key in ("kernel", "variance", "niterations", "computation_time", "element_proc"):
+            np.testing.assert_array_equal(ga[key], gb[key], err_msg=key)
+        assert not ga["kernel"][-1].any() and np.abs(ga["kernel"][:-1]).max() > 0
+    finally:
+        k.close()
