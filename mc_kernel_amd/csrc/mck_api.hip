@@ -405,6 +405,10 @@ struct State {
     int last_nelems = 0, last_nbasis = 0, last_model_nelems = 0, last_model_nbasis = 0;
     std::vector<double> last_ctime;
     DBuf<double> last_model, last_hetero;
+    DBuf<double> int_geom, int_abinv;       // model / heterogeneity integrals of a task
+    DBuf<int64_t> int_base;
+    DBuf<uint8_t> int_live;
+    DBuf<int> int_nsteps;
     int last_hetero_nelems = 0, last_hetero_nbasis = 0;
     // heterogeneity model (mck_set_hetero): point cloud in a uniform grid
     HetGrid hgrid{};
@@ -2039,25 +2043,28 @@ int integrate_task(bool hetero, int32_t elem_type, int32_t nvpe, int32_t nelems,
             if (nbasis > 1) tet_barycentric_inverse(v, &abinv[(size_t)e * 16]);
         }
     }
-    double *d_geom = nullptr, *d_abinv = nullptr;
-    int64_t *d_base = nullptr;
-    uint8_t *d_live = nullptr;
-    int *d_nsteps = nullptr;
+    // per-task buffers kept by the library between calls (grown when a task is larger)
     const size_t nout = (size_t)nelems * nbasis * (hetero ? kHetPar : kModelPar);
     std::vector<int> ns(nelems, 0);
-    hipError_t e = hipMalloc((void **)&d_geom, geom.size() * 8);
-    if (e == hipSuccess && !abinv.empty()) e = hipMalloc((void **)&d_abinv, abinv.size() * 8);
+    hipStream_t st = G.stream;
+    hipError_t e = G.int_geom.ensure(geom.size());
+    if (e == hipSuccess && !abinv.empty()) e = G.int_abinv.ensure(abinv.size());
     DBuf<double> &last = hetero ? G.last_hetero : G.last_model;
     (hetero ? G.last_hetero_nelems : G.last_model_nelems) = 0;
     if (e == hipSuccess) e = last.ensure(nout);
     double *d_out = last.p;
-    if (e == hipSuccess) e = hipMalloc((void **)&d_base, (size_t)nelems * 8);
-    if (e == hipSuccess) e = hipMalloc((void **)&d_live, (size_t)nelems);
-    if (e == hipSuccess) e = hipMalloc((void **)&d_nsteps, (size_t)nelems * 4);
-    if (e == hipSuccess) e = hipMemcpy(d_geom, geom.data(), geom.size() * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess && d_abinv) e = hipMemcpy(d_abinv, abinv.data(), abinv.size() * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d_base, base.data(), (size_t)nelems * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d_live, live.data(), (size_t)nelems, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = G.int_base.ensure(nelems);
+    if (e == hipSuccess) e = G.int_live.ensure(nelems);
+    if (e == hipSuccess) e = G.int_nsteps.ensure(nelems);
+    double *d_geom = G.int_geom.p, *d_abinv = abinv.empty() ? nullptr : G.int_abinv.p;
+    int64_t *d_base = G.int_base.p;
+    uint8_t *d_live = G.int_live.p;
+    int *d_nsteps = G.int_nsteps.p;
+    if (e == hipSuccess) e = hipMemcpyAsync(d_geom, geom.data(), geom.size() * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && d_abinv)
+        e = hipMemcpyAsync(d_abinv, abinv.data(), abinv.size() * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_base, base.data(), (size_t)nelems * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_live, live.data(), (size_t)nelems, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemsetAsync(G.err.p, 0, 4, G.stream);
     if (e == hipSuccess) {
         if (hetero)
@@ -2074,8 +2081,6 @@ int integrate_task(bool hetero, int32_t elem_type, int32_t nvpe, int32_t nelems,
     if (e == hipSuccess) e = hipStreamSynchronize(G.stream);
     int err = 0;
     if (e == hipSuccess) e = hipMemcpy(&err, G.err.p, 4, hipMemcpyDeviceToHost);
-    (void)hipFree(d_geom); (void)hipFree(d_abinv);
-    (void)hipFree(d_base); (void)hipFree(d_live); (void)hipFree(d_nsteps);
     if (e != hipSuccess) return fail(MCK_EHIP, std::string(hetero ? "mck_integrate_hetero: " : "mck_integrate_model: ") + hipGetErrorString(e));
     if (err & 4) return fail(MCK_EINVAL, "basis function weight outside [0, 1] (point outside its element)");
     if (err & 8) return fail(MCK_ENAN, "NaN in a model integral (mc_integration.f90:113-130)");
@@ -2673,6 +2678,7 @@ void mck_finalize(void) {
     G.ad_Bs.release(); G.ad_dFs.release(); G.ad_Ups.release();
     G.ad_part.release(); G.ad_hfilt.release();
     G.last_model.release(); G.last_hetero.release();
+    G.int_geom.release(); G.int_abinv.release(); G.int_base.release(); G.int_live.release(); G.int_nsteps.release();
     G.het_xyz.release(); G.het_dln.release(); G.het_idx.release(); G.het_cells.release();
     G.het_ready = false;
     G.last_nelems = G.last_nbasis = G.last_model_nelems = G.last_model_nbasis = 0;
