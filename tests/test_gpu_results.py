@@ -137,6 +137,12 @@ def test_load_get_and_checks():
         got = res.get()
         for key, v in arrays.items():
             np.testing.assert_array_equal(got[key], v, err_msg=key)
+        # export straight into device memory
+        import torch
+        dev = torch.empty((k.nk, 50), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        mk._check(k.lib.mck_results_export(abi.MCK_RES_ERROR, dev.data_ptr()))
+        np.testing.assert_array_equal(dev.cpu().numpy(), res.export(abi.MCK_RES_ERROR))
         with pytest.raises(mk.MckError):                  # element id out of range
             res.add([51], np.zeros((1, 1, k.nk)), np.zeros((1, 1, k.nk)), np.zeros((1, k.nk), np.int32))
         with pytest.raises(mk.MckError):                  # no slave_work of that size yet
