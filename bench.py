@@ -165,6 +165,22 @@ def measured_traffic(path, kernel):
     return None
 
 
+def measured_valu_busy(path, kernel):
+    """VALU busy fraction of `kernel` from the committed counter summary (the row
+    '| <kernel> | VALU per wave | SALU | LDS | busy % | ...' of its derived table)."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for ln in f:
+            cells = [c.strip() for c in ln.strip().strip("|").split("|")]
+            if len(cells) >= 5 and cells[0] == kernel:
+                try:
+                    return round(float(cells[4]) / 100.0, 3)
+                except ValueError:
+                    continue
+    return None
+
+
 def bench_autograd(args):
     """c3: the stage of the c1 workload (nt 2048, 10 receivers, 4 Gabor filters, 160
     kernels) at --autograd-points points with their strain traces; forward and the
@@ -453,6 +469,11 @@ def main():
             "traffic": measured_traffic(args.traffic, kern) if args.config == "c1" else None,
             "kernel": kern,
             "bytes_per_launch": byt / max(launches, 1), "avg_launch_ms": ms / max(launches, 1)}
+    if args.config == "c1":
+        # what does bound the kernel: its VALU issue rate, from the committed counter run
+        busy = measured_valu_busy(os.path.join(ROOT, "profiles", "c1_valu_counters.md"), kern)
+        if busy is not None:
+            roof["valu_busy"] = busy
 
     # plain one-point-at-a-time form on a small sample of the same workload
     plain = None
