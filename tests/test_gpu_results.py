@@ -243,3 +243,37 @@ def test_device_resident_skipped_slot():
         assert not ga["kernel"][-1].any() and np.abs(ga["kernel"][:-1]).max() > 0
     finally:
         k.close()
+
+
+def test_run_job_voxels_with_model_and_hetero(tmp_path):
+    """run_job over voxels with 8 basis functions (onvertices), the background-model and
+    heterogeneity integrals and finalize's files: tasks of 3 voxels give the kernel
+    accumulators of one task (model integrals off), and the model and heterogeneity
+    accumulators fill when on."""
+    import mc_kernel_amd as mk
+    from mc_kernel_amd.job import run_job
+    p = synth.make_problem("tiny", ntet=4)
+    vv, vc = p.voxels(7, seed=2)
+    v = np.asarray(vv, float).reshape(-1, 3)
+    rng = np.random.default_rng(1)
+    a_, b_ = v.min(0), v.max(0)
+    xyz = a_ + (b_ - a_) * (-0.2 + 1.4 * rng.random((500, 3)))
+    r = np.linalg.norm(xyz, axis=1)
+    rtp = np.stack([r / 1000.0, 90.0 - np.degrees(np.arccos(xyz[:, 2] / r)),
+                    np.degrees(np.arctan2(xyz[:, 1], xyz[:, 0]))], 1)
+    dln = (rng.standard_normal((500, 3)) * 0.03).astype(np.float32)
+    k = mk.MCKernel(p.setup)
+    try:
+        a = run_job(k, vv, vc, nbasis=8, nelems_per_task=3).get()
+        k.halton_step = 0
+        b = run_job(k, vv, vc, nbasis=8, nelems_per_task=len(vc)).get()
+        c = run_job(k, vv, vc, nbasis=8, nelems_per_task=3, int_over_background=True, hetero=(rtp, dln),
+                    output_file=str(tmp_path / "vox"))
+        gc = c.get()
+    finally:
+        k.close()
+    for key in ("kernel", "variance", "niterations"):
+        np.testing.assert_array_equal(a[key], b[key], err_msg=key)
+    assert np.abs(a["kernel"]).max() > 0
+    assert np.abs(gc["model"]).max() > 0 and np.abs(gc["hetero"]).max() > 0
+    assert os.path.exists(str(tmp_path / "vox_kernel.nc"))
