@@ -621,6 +621,7 @@ int build_kernels(State &G) {
         }
     // seismograms per receiver (cut_and_add_seismogram, kernel.f90:181-385)
     std::map<std::tuple<int, int, int, int>, int> hidx;  // (rec, filter, misfit, window) -> H
+    std::map<std::tuple<int, int, int>, std::vector<double>> filtered;  // (rec, filter, misfit)
     // per distinct window: the weighted samples of the filtered seismogram in the window
     // (concatenated), first sample, length, normalisation, filter; the window spectra
     // themselves are computed on the device (k_window_spectra)
@@ -667,18 +668,25 @@ int build_kernels(State &G) {
         auto it = hidx.find(key);
         if (it != hidx.end()) h = it->second;
         else {
-            const double *seis = S.seis + (size_t)r * nt;
-            std::vector<double> x = (mf == MCK_MISFIT_AM) ? cumsum_trap(seis, nt, dt)
-                                                          : std::vector<double>(seis, seis + nt);
-            auto X = rfft(x, N, dt);
-            for (int q = 0; q < nw; ++q) {
-                if (!S.deconv_stf) {
-                    double a = 2 * kPi * f[q] * S.timeshift_fwd;
-                    X[q] *= cd(std::cos(a), std::sin(a));
+            // the filtered seismogram depends on (receiver, filter, misfit) only: computed
+            // once for all the windows cut from it
+            auto fkey = std::make_tuple(r, fi, mf);
+            auto fit = filtered.find(fkey);
+            if (fit == filtered.end()) {
+                const double *seis = S.seis + (size_t)r * nt;
+                std::vector<double> x = (mf == MCK_MISFIT_AM) ? cumsum_trap(seis, nt, dt)
+                                                              : std::vector<double>(seis, seis + nt);
+                auto X = rfft(x, N, dt);
+                for (int q = 0; q < nw; ++q) {
+                    if (!S.deconv_stf) {
+                        double a = 2 * kPi * f[q] * S.timeshift_fwd;
+                        X[q] *= cd(std::cos(a), std::sin(a));
+                    }
+                    X[q] *= tff[fi][q];
                 }
-                X[q] *= tff[fi][q];
+                fit = filtered.emplace(fkey, irfft(X, N, df)).first;
             }
-            auto xs = irfft(X, N, df);
+            const std::vector<double> &xs = fit->second;
             std::vector<double> w(ncut);
             double norm_term = 0.0;
             for (int n = 0; n < ncut; ++n) norm_term += xs[ifirst + n] * xs[ifirst + n];
