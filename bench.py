@@ -57,7 +57,8 @@ def unmasked_voxels(p):
 
 
 def _cpu_worker(args):
-    """One host core: the FP64 oracle on a bounded sample of voxels (lazy database)."""
+    """One host core: the FP64 oracle on a bounded sample of voxels (lazy database);
+    returns the evaluations, the timed seconds and the oracle's results of the voxels."""
     name, voxels, max_iter = args
     sys.path.insert(0, ROOT)
     from mc_kernel_amd import synth
@@ -82,7 +83,18 @@ def _cpu_worker(args):
     o.slave_work(p.verts, conn, halton_offsets=offs)          # generates the touched elements
     t0 = time.perf_counter()
     r = o.slave_work(p.verts, conn, halton_offsets=offs)      # timed
-    return r["evaluations"], time.perf_counter() - t0
+    return r["evaluations"], time.perf_counter() - t0, voxels, r["values"], r["niterations"]
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(name, workers, per_worker, max_iter):
@@ -99,30 +111,49 @@ def cpu_baseline(name, workers, per_worker, max_iter):
     wall = time.perf_counter() - t0
     evals = sum(r[0] for r in res)
     tmax = max(r[1] for r in res)
-    return {"value": evals / tmax, "unit": "evals/s", "cores": len(chunks), "kind": "port",
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count()
+    line = {"value": evals / tmax, "unit": "evals/s", "cores": len(chunks), "kind": "port",
             "sample": "%d unmasked voxels of %s, %d MC steps each (max_iter=%d), %d points/step; "
                       "FP64 oracle, one process per core, timed region = slave_work only"
                       % (len(sel), name, max_iter, max_iter, p.npoints_per_step),
-            "seconds": tmax, "evaluations": int(evals), "total_wall_s": wall}
+            "seconds": tmax, "evaluations": int(evals), "total_wall_s": wall,
+            "nproc": os.cpu_count(), "affinity_cpus": affinity, "cpu_model": cpu_model()}
+    # the oracle's results of the sample, for the GPU's self-check on the same voxels
+    vox = np.concatenate([r[2] for r in res])
+    vals = np.concatenate([r[3] for r in res])
+    nit = np.concatenate([r[4] for r in res])
+    return line, (vox, vals, nit, max_iter)
 
 
-def gpu_db(p):
-    from mc_kernel_amd import abi, bind_hip_runtime
-    bind_hip_runtime()
-    lib = C.CDLL(os.path.join(ROOT, "mc_kernel_amd", "libmck_synth_gpu.so"))
-    lib.synth_gpu_db.argtypes = [C.c_int, abi.i32p, C.c_int, C.c_int, C.c_double, C.c_uint64,
-                                 C.POINTER(C.c_void_p)]
-    m = p.mesh
-    out = []
-    for nvar, seed in ((10, 1001), (5, 1002)):
-        d = C.c_void_p()
-        rc = lib.synth_gpu_db(m.nelem, abi.ptr(m.sem, C.c_int32), nvar, p.nt, p.dt, seed, C.byref(d))
-        if rc:
-            raise RuntimeError("device database generation failed (%d)" % rc)
-        out.append(d)
-    p.setup.fwd.snapshots = C.cast(out[0], C.POINTER(C.c_float))
-    p.setup.bwd.snapshots = C.cast(out[1], C.POINTER(C.c_float))
-    return lib, out
+def self_check(p, sample):
+    """Runs the CPU baseline's voxels on the GPU with the oracle's stream positions and
+    MAXIMUM_ITERATIONS (the databases are bit-identical: the device generator reproduces
+    the host generator's samples) and compares: niterations bit for bit, kernel values as
+    the largest relative L2 error over kernels with a nonzero oracle norm."""
+    import mc_kernel_amd as mk
+    vox, ref_vals, ref_nit, max_iter = sample
+    offs = np.asarray(vox, np.int64) * ((max_iter + 1) * p.npoints_per_step)
+    old = p.setup.max_iter
+    p.setup.max_iter = max_iter
+    try:
+        k = mk.MCKernel(p.setup)
+        try:
+            r = k.slave_work(p.verts, p.conn[vox], halton_offsets=offs)
+        finally:
+            k.close()
+    finally:
+        p.setup.max_iter = old
+    errs = []
+    for j in range(ref_vals.shape[1]):
+        n = np.linalg.norm(ref_vals[:, j])
+        if n > 0:
+            errs.append(float(np.linalg.norm(r["values"][:, j] - ref_vals[:, j]) / n))
+    return {"voxels": int(len(vox)), "max_iter": int(max_iter), "max_rel_err": max(errs) if errs else None,
+            "kernels_compared": len(errs), "niterations_equal": bool(np.array_equal(r["niterations"], ref_nit)),
+            "tolerance": 1e-5}
 
 
 def page_db_to_host(p, lib, dbs, pinned):
@@ -369,6 +400,7 @@ def main():
     ap.add_argument("--task", type=int, default=0, help="voxels per step (0 = whole job)")
     ap.add_argument("--first-voxel", type=int, default=0,
                     help="tasks start at this voxel (the synthetic meshes are spatially clustered)")
+    # the GPU box's CPU share is 16 cores (its process guard counts workers beyond them)
     ap.add_argument("--cpu-workers", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--cpu-voxels-per-worker", type=int, default=4)
     ap.add_argument("--cpu-steps", type=int, default=8)
@@ -413,18 +445,19 @@ def main():
         return
 
     # CPU baseline first, in fresh processes, before this process touches the GPU
-    cpu = None
+    cpu, sample = None, None
     if not args.no_cpu:
-        cpu = cpu_baseline(args.config, args.cpu_workers, args.cpu_voxels_per_worker, args.cpu_steps)
+        cpu, sample = cpu_baseline(args.config, args.cpu_workers, args.cpu_voxels_per_worker, args.cpu_steps)
         print("cpu baseline:", json.dumps(cpu), file=sys.stderr, flush=True)
 
     import mc_kernel_amd as mk
+    from mc_kernel_amd import synth
     p = workload(args.config)
-    lib, dbs = gpu_db(p)
+    devdb = synth.DeviceDatabases(p)
+    lib, dbs = devdb.lib, devdb.ptrs
     if args.copy_calibration_mib > 0:
         ms = C.c_float()
         nbytes = args.copy_calibration_mib << 20
-        lib.synth_gpu_copy_calibration.argtypes = [C.c_size_t, C.c_int, C.POINTER(C.c_float)]
         if lib.synth_gpu_copy_calibration(nbytes, 5, C.byref(ms)):
             raise RuntimeError("copy calibration failed")
         print("copy calibration: %d bytes read + %d written per launch, %.3f ms, %.1f GB/s"
@@ -463,12 +496,18 @@ def main():
     else:
         kern, byt, ms = "k_fwd_spectra", fwd_b, fwd_ms
     achieved = (byt / launches) / (ms / launches * 1e-3) / 1e9 if ms > 0 else 0.0
+    # the committed counter summary is of the c1 workload
+    traffic = measured_traffic(args.traffic, kern) if args.config == "c1" else None
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            # the committed counter summary is of the c1 workload
-            "traffic": measured_traffic(args.traffic, kern) if args.config == "c1" else None,
-            "kernel": kern,
-            "bytes_per_launch": byt / max(launches, 1), "avg_launch_ms": ms / max(launches, 1)}
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kern,
+            "bytes_per_launch": byt / max(launches, 1), "avg_launch_ms": ms / max(launches, 1),
+            "algorithmic_bytes": "merged blocks once per launch: fwd 4 nt 25 x 10 variables per distinct "
+                                 "forward element (+ 48 B per point), bwd 4 nt 25 x 5 variables per distinct "
+                                 "(receiver, backward element)",
+            "alg_bytes_whole_job": sum(s[1]["alg_bytes"] for s in steps) / len(steps)}
+    if traffic:
+        # the counter-measured HBM bytes of the same kernel over its launch time
+        roof["measured_frac"] = round(traffic / (ms / launches * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     if args.config == "c1":
         # what does bound the kernel: its VALU issue rate, from the committed counter run
         busy = measured_valu_busy(os.path.join(ROOT, "profiles", "c1_valu_counters.md"), kern)
@@ -488,8 +527,12 @@ def main():
         t0 = time.perf_counter()
         rb = k.slave_work(p.verts, conn)
         tb = time.perf_counter() - t0
-        plain = {"voxels": int(len(sel)), "plain_evals_per_s": rp["stats"]["evaluations"] / tp,
+        plain = {"definition": "plain = mck_slave_work_plain, the reference's loop order: one voxel at a "
+                               "time, its MC steps one after another (the points of one step evaluated "
+                               "together); batched = mck_slave_work on the same voxels",
+                 "voxels": int(len(sel)), "plain_evals_per_s": rp["stats"]["evaluations"] / tp,
                  "batched_same_sample_evals_per_s": rb["stats"]["evaluations"] / tb,
+                 "batched_over_plain_same_sample": (rb["stats"]["evaluations"] / tb) / (rp["stats"]["evaluations"] / tp),
                  "identical_results": bool(np.array_equal(rp["values"], rb["values"]))}
         print("plain sample:", json.dumps(plain), file=sys.stderr, flush=True)
     k.close()
@@ -501,13 +544,18 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 accumulation)",
         "data": "synthetic (seeded AxiSEM-style merged DB generated in HBM)",
         "config": {"workload": "%s: %d-quad DB, nt=%d, %d receivers, %d filters, %d kernels "
-                               "(CC/AM x vp/vs), %d tets, %d pts/step, max_iter %d, rel tol %g"
+                               "(CC/AM x vp/vs), %d tets, %d pts/step, max_iter %d, rel tol %g, "
+                               "STF deconvolution %s"
                                % (args.config, p.mesh.nelem, p.nt, p.nrec, len(p.filters), p.nkernel,
-                                  p.ntet, p.npoints_per_step, p.max_iter, p.allowed_relative_error),
+                                  p.ntet, p.npoints_per_step, p.max_iter, p.allowed_relative_error,
+                                  "on" if p.deconv_stf else "off"),
                    "task_voxels": T, "first_voxel": v0, "parallelism": "none (1 GPU)",
                    "evaluations": int(tot_e), "device_ms": dev_ms},
         "roofline": roof, "cpu_baseline": cpu, "plain_form": plain,
     }
+    if sample is not None and host_dbs is None:
+        line["self_check"] = self_check(p, sample)
+        print("self-check:", json.dumps(line["self_check"]), file=sys.stderr, flush=True)
     if host_dbs is not None:
         paged = sum(s[1]["paged_bytes"] for s in steps)
         line["config"]["databases"] = ("paged from %s host memory, %d elements per page, "

@@ -191,6 +191,18 @@ typedef struct {
     double  strain_bwd_ms;     /* summed time of the bwd strain-gather/FFT kernel        */
     double  wall_ms;           /* host wall time of the whole call                       */
     int64_t paged_bytes;       /* snapshot bytes copied host -> device by paging         */
+    int32_t spt;               /* samples per thread of the strain/FFT kernels           */
+    int32_t nd_f, nd_b;        /* carried forward / backward strain components           */
+    int32_t bwd_path;          /* backward evaluation: 0 per-pair FFTs, 1 element        */
+                               /* spectra and a matrix-core contraction                  */
+    int64_t fwd_elements;      /* sum over launches of the distinct fwd elements |F|     */
+    int64_t bwd_elem_receivers;/* sum over launches of sum_r |G_r|, the distinct bwd     */
+                               /* elements located by the points in receiver r's frame   */
+    int64_t alg_bytes;         /* algorithmic bytes of the whole call, summed over its   */
+                               /* launches: 4 nt 25 (10 |F| + 5 sum_r |G_r|) + 48 P      */
+                               /* + 24 V K (merged blocks once per launch and receiver,  */
+                               /* point coordinates, MC state of the evaluated voxels)   */
+    double  spectra_ms;        /* summed time of the element-spectra kernels (path 1)    */
 } mck_stats_t;
 
 /*

@@ -82,6 +82,9 @@ class MckStats(C.Structure):
         ("device_ms", C.c_double), ("strain_fwd_ms", C.c_double),
         ("strain_bwd_ms", C.c_double), ("wall_ms", C.c_double),
         ("paged_bytes", C.c_int64),
+        ("spt", C.c_int32), ("nd_f", C.c_int32), ("nd_b", C.c_int32), ("bwd_path", C.c_int32),
+        ("fwd_elements", C.c_int64), ("bwd_elem_receivers", C.c_int64), ("alg_bytes", C.c_int64),
+        ("spectra_ms", C.c_double),
     ]
 
 

@@ -31,10 +31,12 @@ def _stale(target, sources):
 
 
 def build_synth(force=True):
+    # no FMA contraction: the wavelet arithmetic (synth_math.h) must round as the device
+    # generator's does
     src = os.path.join(CSRC, "synth.cpp")
     out = os.path.join(HERE, "libmck_synth.so")
-    if force or _stale(out, [src]):
-        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", src, "-o", out])
+    if force or _stale(out, [src, os.path.join(CSRC, "synth_math.h")]):
+        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", "-ffp-contract=off", src, "-o", out])
     return out
 
 
@@ -47,7 +49,7 @@ def build_synth_gpu(force=True):
     """Device-side generator of synthetic databases (benchmark input, not the product)."""
     src = os.path.join(CSRC, "synth_gpu.hip")
     out = os.path.join(HERE, "libmck_synth_gpu.so")
-    if force or _stale(out, [src]):
+    if force or _stale(out, [src, os.path.join(CSRC, "synth_math.h")]):
         hipcc = os.path.join(ROCM, "bin", "hipcc")
         _run([hipcc] + HIP_FLAGS + ["-shared", src, "-o", out])
     return out

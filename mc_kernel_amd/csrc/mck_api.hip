@@ -391,6 +391,7 @@ struct State {
     int chunk_points = 0;
     hipStream_t stream = nullptr;
     mck_stats_t stats{};
+    int64_t stats_vk = 0;                    // evaluated (voxel, kernel) states of a task
     // differentiable filter-convolve-window-integrate stage: window spectra without their
     // filter (W_h), the filter of each window spectrum, the filter responses G of the
     // set-up, and scratch of the forward and adjoint launches
@@ -953,6 +954,10 @@ hipError_t launch_bwd(State &G, const DevDb &db, int npairs, const int *order, c
     const size_t smem = bwd_smem(pc);
     const dim3 grid(grid_x8(npairs)), block(kStrainThreads);
     const int nf = pc.nd_f, nb = pc.nd_b;
+    G.stats.spt = spt_for(pc.nt);
+    G.stats.nd_f = nf;
+    G.stats.nd_b = nb;
+    G.stats.bwd_path = 0;
 #define BWD(SP, NF, NB)                                                                           \
     if (spt_for(pc.nt) == SP && nf == NF && nb == NB) {                                           \
         if (smem > 65536)                                                                         \
@@ -993,11 +998,12 @@ hipError_t launch_traces(State &G, const DevDb &db, int fwd, int frame, int npts
 // A page below nslot - 2 is copied into its own buffer on first use and stays; any other
 // page is copied into one of the two rotating buffers (on the copy stream: the copy of
 // the next page overlaps the kernels of this one, and a buffer is refilled only after
-// the kernels that read it).  sorted_keys: element id of every item in launch order,
-// ascending (pages then cover disjoint item ranges and untouched pages are not copied);
-// null: every page covers all items and the kernels skip the items outside it.
+// the kernels that read it).  sorted_keys: element id * key_scale + minor key of every
+// item in launch order, ascending (pages then cover disjoint item ranges and untouched
+// pages are not copied); null: every page covers all items and the kernels skip the items
+// outside it.
 template <class Launch>
-int run_paged(State &G, DbDev &D, int n, const int *sorted_keys, Launch &&launch) {
+int run_paged(State &G, DbDev &D, int n, const int *sorted_keys, Launch &&launch, int key_scale = 1) {
     if (n <= 0) return MCK_OK;
     if (!D.paged) {
         HIPCHECK(launch(D.d, 0, n));
@@ -1008,7 +1014,7 @@ int run_paged(State &G, DbDev &D, int n, const int *sorted_keys, Launch &&launch
     if (sorted_keys) {
         HIPCHECK(G.page_bounds.ensure(D.npages + 1));
         hipLaunchKernelGGL(k_page_bounds, dim3((D.npages + 256) / 256), dim3(256), 0, G.stream, n, sorted_keys,
-                           D.page_elems, D.npages, G.page_bounds.p);
+                           D.page_elems, D.npages, key_scale, G.page_bounds.p);
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpyAsync(bounds.data(), G.page_bounds.p, (D.npages + 1) * sizeof(int),
                                 hipMemcpyDeviceToHost, G.stream));
@@ -1661,7 +1667,7 @@ int eval_chunk(State &G, int P, int w0, int nw, std::vector<std::pair<hipEvent_t
     HIPCHECK(launch_setup(G, true, np, G.vals_out.p, G.keys_out.p, loc_f));
     // points with an active receiver come first in the sorted order: their count
     HIPCHECK(G.nlive.ensure(2));
-    hipLaunchKernelGGL(k_page_bounds, dim3(1), dim3(64), 0, st, np, G.keys_out.p, G.fwd.d.nelem, 1, G.nlive.p);
+    hipLaunchKernelGGL(k_page_bounds, dim3(1), dim3(64), 0, st, np, G.keys_out.p, G.fwd.d.nelem, 1, 1, G.nlive.p);
     HIPCHECK(hipGetLastError());
     hipEvent_t a, b;
     HIPCHECK(hipEventCreate(&a));
@@ -1683,10 +1689,11 @@ int eval_chunk(State &G, int P, int w0, int nw, std::vector<std::pair<hipEvent_t
     int npairs = 0;
     HIPCHECK(hipMemcpyAsync(&npairs, G.pair_count.p, 4, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
-    if ((rc = sort_by_element(G, npairs, G.bwd.d.nelem))) return rc;
+    // (element, receiver) order: the runs of equal keys count sum_r |G_r|
+    if ((rc = sort_by_element(G, npairs, G.bwd.d.nelem * R))) return rc;
     if (npairs > 0) {
         hipLaunchKernelGGL(k_count_runs, dim3((npairs + 255) / 256), dim3(256), 0, st, npairs, G.keys_out.p,
-                           G.bwd.d.nelem, G.elem_count.p + 2);
+                           G.bwd.d.nelem * R, G.elem_count.p + 2);
         HIPCHECK(hipGetLastError());
     }
     HIPCHECK(launch_setup(G, false, npairs, G.vals_out.p, nullptr, loc_b));
@@ -1699,7 +1706,7 @@ int eval_chunk(State &G, int P, int w0, int nw, std::vector<std::pair<hipEvent_t
         if ((rc = run_paged(G, G.bwd, npairs, G.keys_out.p, [&](const DevDb &d, int i0, int i1) {
                  return launch_bwd(G, d, i1 - i0, G.vals_out.p + i0, loc_b, G.opWb.p + (size_t)i0 * nwb,
                                    G.opQb.p + (size_t)i0 * 36, phys, nullptr, act, P);
-             })))
+             }, R)))
             return rc;
     }
     HIPCHECK(hipEventRecord(b, st));
@@ -1752,6 +1759,7 @@ int mc_step(State &G, const Task &T, int *nwork_out, std::vector<std::pair<hipEv
     HIPCHECK(hipGetLastError());
     G.stats.points += (int64_t)nwork * P;
     G.stats.mc_steps += 1;
+    G.stats_vk += (int64_t)nwork * nk;
     return MCK_OK;
 }
 
@@ -1773,16 +1781,19 @@ int finish_task(State &G, const Task &T, double *values, double *variance, int32
     HIPCHECK(hipStreamSynchronize(st));
     G.stats.evaluations = (int64_t)ev;
     {
-        // algorithmic bytes: every distinct database element of a launch read once, the
-        // forward spectra written once (fwd) and read once per distinct point (bwd), the
-        // kernel values of every pair written once
-        const double blk = 25.0 * G.pc.nt * sizeof(float);
-        const double spec = (double)G.pc.nd_f * G.pc.nomega * sizeof(float2);
-        const double kval = G.R ? (double)G.nk / G.R * sizeof(double) : 0.0;
+        // algorithmic bytes of the path: every merged block the reference reads, once per
+        // launch - forward blocks (10 variables) of the distinct forward elements |F|,
+        // backward blocks (5 variables) of the distinct elements of every receiver's frame
+        // (sum_r |G_r|) - the coordinates of the points (48 B) and the MC state (Sum f,
+        // Sum f^2, integral of every evaluated voxel and kernel, 24 B)
+        const int64_t blk = (int64_t)25 * G.pc.nt * sizeof(float);
         G.stats.fwd_locations = (int64_t)cnt[1];
         G.stats.bwd_locations = (int64_t)cnt[3];
-        G.stats.fwd_bytes = (int64_t)(cnt[0] * G.pc.nvar_fwd * blk + cnt[1] * spec);
-        G.stats.bwd_bytes = (int64_t)(cnt[2] * G.pc.nvar_bwd * blk + cnt[4] * spec + cnt[3] * kval);
+        G.stats.fwd_elements = (int64_t)cnt[0];
+        G.stats.bwd_elem_receivers = (int64_t)cnt[2];
+        G.stats.fwd_bytes = (int64_t)cnt[0] * 10 * blk + 48 * (int64_t)cnt[1];
+        G.stats.bwd_bytes = (int64_t)cnt[2] * 5 * blk;
+        G.stats.alg_bytes = G.stats.fwd_bytes + G.stats.bwd_bytes + 24 * G.stats_vk;
     }
     if (err & 1) return fail(MCK_ENOTFOUND, "element not found: a point is outside the 12 nearest elements");
     if (err & 2) return fail(MCK_ENAN, "NaN in a Monte Carlo integral");
@@ -1831,6 +1842,7 @@ int slave_work_impl(bool plain, int32_t elem_type, int32_t nvpe, int32_t nelems,
         return fail(MCK_EINVAL, "null argument");
     auto t0 = std::chrono::steady_clock::now();
     G.stats = mck_stats_t{};
+    G.stats_vk = 0;
     G.last_nelems = G.last_nbasis = 0;
     Task T;
     int rc;

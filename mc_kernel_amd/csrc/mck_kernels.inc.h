@@ -1737,15 +1737,17 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && NDB <= 4) ? 3 : (SPT <
     }
 }
 
-// sort keys: element of every point (fwd) / of every active pair with a located element
-// (bwd, compacted through an atomic counter); keys past the last element sort last
+// sort keys: element of every point (fwd) / element * R + receiver of every active pair
+// with a located element (bwd, compacted through an atomic counter); keys past the last
+// element sort last
 // first item of every database page in element-sorted keys: out[b] = number of keys
-// below b * page_elems, b = 0 .. npages (keys of unlocated items, nelem, sort last)
-__global__ void k_page_bounds(int n, const int *keys, int page_elems, int npages, int *out)
+// below b * page_elems * scale, b = 0 .. npages (keys are element * scale + minor key;
+// keys of unlocated items sort last)
+__global__ void k_page_bounds(int n, const int *keys, int page_elems, int npages, int scale, int *out)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b > npages) return;
-    const long long e = (long long)b * page_elems;
+    const long long e = (long long)b * page_elems * scale;
     int lo = 0, hi = n;
     while (lo < hi) {
         const int mid = lo + ((hi - lo) >> 1);
@@ -1779,7 +1781,7 @@ __global__ void k_keys_pairs(int npairs, int R, int P, const uint8_t *active, co
     int e = loc[q].elem;
     if (e < 0) return;
     int pos = atomicAdd(count, 1);
-    keys[pos] = e;
+    keys[pos] = e * R + r;
     vals[pos] = q;
 }
 

@@ -13,7 +13,7 @@
 // Traces: for every global GLL point and variable, a sum of 8 Gaussian-windowed 40 s
 // period wavelets with amplitudes N(0,1) and arrival times U(100, 900) s drawn from a
 // counter-based hash of (seed, point, variable, wavelet), evaluated in double precision
-// and rounded to float.
+// (synth_math.h) and rounded to float.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -21,6 +21,8 @@
 #include <thread>
 #include <unordered_map>
 #include <vector>
+
+#include "synth_math.h"
 
 namespace {
 
@@ -292,33 +294,13 @@ void synth_mesh_get(void *h, float *mesh_s, float *mesh_z, float *mp_s, float *m
 
 void synth_mesh_free(void *h) { delete (Mesh *)h; }
 
-// counter-based hash (splitmix64 finaliser) -> uniform double in [0, 1)
-static inline uint64_t mix64(uint64_t x) {
-    x += 0x9e3779b97f4a7c15ull;
-    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
-    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
-    return x ^ (x >> 31);
-}
-static inline double unif(uint64_t seed, uint64_t a, uint64_t b, uint64_t c) {
-    uint64_t h = mix64(seed ^ mix64(a ^ mix64(b ^ mix64(c))));
-    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
-}
-
-// one trace: sum of 8 wavelets, window |t - tau| <= 240 s
+// one trace: sum of 8 wavelets, window |t - tau| <= 240 s (synth_math.h, shared with
+// the device generator so that both produce the same floats)
 void synth_trace(uint64_t seed, int64_t point, int var, int nt, double dt, float *out) {
     std::vector<double> acc(nt, 0.0);
     for (int w = 0; w < 8; ++w) {
-        double u1 = unif(seed, (uint64_t)point, (uint64_t)var, 4 * w + 0);
-        double u2 = unif(seed, (uint64_t)point, (uint64_t)var, 4 * w + 1);
-        double u3 = unif(seed, (uint64_t)point, (uint64_t)var, 4 * w + 2);
-        double amp = std::sqrt(-2.0 * std::log(1.0 - u1)) * std::cos(2.0 * kPi * u2);
-        double tau = 100.0 + 800.0 * u3;
-        int i0 = std::max(0, (int)std::ceil((tau - 240.0) / dt));
-        int i1 = std::min(nt - 1, (int)std::floor((tau + 240.0) / dt));
-        for (int i = i0; i <= i1; ++i) {
-            double x = i * dt - tau;
-            acc[i] += amp * std::exp(-(x / 40.0) * (x / 40.0)) * std::cos(2.0 * kPi * x / 40.0);
-        }
+        const synth_math::Wavelet W = synth_math::wavelet(seed, point, var, w, nt, dt);
+        for (int i = W.i0; i <= W.i1; ++i) acc[i] += synth_math::wavelet_sample(W, i, dt);
     }
     for (int i = 0; i < nt; ++i) out[i] = (float)acc[i];
 }

@@ -1,52 +1,33 @@
 // synth_gpu.hip - device-side generator of the synthetic merged databases used by the
 // benchmark: the wavelet construction of synth.cpp (8 Gaussian-windowed 40 s wavelets
 // per GLL point and variable, amplitudes N(0,1), arrival times U(100, 900) s) evaluated
-// with single-precision device math directly into HBM.  Values agree with the CPU
-// generator to single-precision rounding, not bit for bit; parity tests use the CPU one.
+// directly into HBM with the same double-precision arithmetic (synth_math.h), so every
+// sample equals the host generator's bit for bit.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "synth_math.h"
+
 namespace {
 
-__device__ inline uint64_t mix64(uint64_t x) {
-    x += 0x9e3779b97f4a7c15ull;
-    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
-    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
-    return x ^ (x >> 31);
-}
-__device__ inline double unif(uint64_t seed, uint64_t a, uint64_t b, uint64_t c) {
-    uint64_t h = mix64(seed ^ mix64(a ^ mix64(b ^ mix64(c))));
-    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
-}
-
 // one block per (element, variable); threads stride over 25 nodes x nt samples
-__global__ void k_fill(int nelem, const int *sem, int nvar, int nt, float dt, uint64_t seed,
+__global__ void k_fill(int nelem, const int *sem, int nvar, int nt, double dt, uint64_t seed,
                        float *out) {
     const int ev = blockIdx.x;
     const int e = ev / nvar, v = ev - (ev / nvar) * nvar;
-    __shared__ float amp[25][8], tau[25][8];
+    __shared__ synth_math::Wavelet wl[25][8];
     if (threadIdx.x < 200) {
-        int n = threadIdx.x / 8, w = threadIdx.x % 8;
-        uint64_t pt = (uint64_t)sem[25 * e + n];
-        double u1 = unif(seed, pt, (uint64_t)v, 4 * w + 0);
-        double u2 = unif(seed, pt, (uint64_t)v, 4 * w + 1);
-        double u3 = unif(seed, pt, (uint64_t)v, 4 * w + 2);
-        amp[n][w] = (float)(sqrt(-2.0 * log(1.0 - u1)) * cos(2.0 * 3.1415926535898 * u2));
-        tau[n][w] = (float)(100.0 + 800.0 * u3);
+        const int n = threadIdx.x / 8, w = threadIdx.x % 8;
+        wl[n][w] = synth_math::wavelet(seed, (int64_t)sem[25 * e + n], v, w, nt, dt);
     }
     __syncthreads();
     float *dst = out + ((size_t)e * nvar + v) * 25 * nt;
     for (int i = threadIdx.x; i < 25 * nt; i += blockDim.x) {
-        int n = i / nt, t = i - (i / nt) * nt;
-        float acc = 0.0f;
-        for (int w = 0; w < 8; ++w) {
-            float x = t * dt - tau[n][w];
-            if (fabsf(x) <= 240.0f) {
-                float q = x / 40.0f;
-                acc += amp[n][w] * __expf(-q * q) * __cosf(2.0f * 3.14159265f * q);
-            }
-        }
-        dst[i] = acc;
+        const int n = i / nt, t = i - (i / nt) * nt;
+        double acc = 0.0;
+        for (int w = 0; w < 8; ++w)
+            if (t >= wl[n][w].i0 && t <= wl[n][w].i1) acc += synth_math::wavelet_sample(wl[n][w], t, dt);
+        dst[i] = (float)acc;
     }
 }
 
@@ -102,8 +83,7 @@ int synth_gpu_db(int nelem, const int *sem_host, int nvar, int nt, double dt, ui
         (void)hipFree(d_sem);
         return 3;
     }
-    hipLaunchKernelGGL(k_fill, dim3(nelem * nvar), dim3(256), 0, 0, nelem, d_sem, nvar, nt,
-                       (float)dt, seed, d);
+    hipLaunchKernelGGL(k_fill, dim3(nelem * nvar), dim3(256), 0, 0, nelem, d_sem, nvar, nt, dt, seed, d);
     hipError_t e = hipDeviceSynchronize();
     (void)hipFree(d_sem);
     if (e != hipSuccess) { (void)hipFree(d); return 4; }

@@ -72,6 +72,9 @@ module mck_gpu
      integer(c_int32_t) :: mc_steps, strain_launches
      real(c_double) :: device_ms, strain_fwd_ms, strain_bwd_ms, wall_ms
      integer(c_int64_t) :: paged_bytes
+     integer(c_int32_t) :: spt, nd_f, nd_b, bwd_path
+     integer(c_int64_t) :: fwd_elements, bwd_elem_receivers, alg_bytes
+     real(c_double) :: spectra_ms
   end type mck_stats_t
 
   public :: mck_init, mck_slave_work, mck_slave_work_plain, mck_locate_points, &

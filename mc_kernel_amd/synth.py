@@ -104,6 +104,61 @@ def fill_db(mesh, nvar, nt, dt, seed, nthreads=None):
     return out
 
 
+def synth_gpu_lib():
+    """The device-side database generator (libmck_synth_gpu.so): benchmark and test input,
+    bit-identical to fill_db."""
+    from . import bind_hip_runtime
+    path = os.path.join(_HERE, "libmck_synth_gpu.so")
+    if not os.path.exists(path):
+        raise RuntimeError("libmck_synth_gpu.so is not built; run __graft_entry__.build()")
+    bind_hip_runtime()
+    lib = C.CDLL(path)
+    lib.synth_gpu_db.argtypes = [C.c_int, abi.i32p, C.c_int, C.c_int, C.c_double, C.c_uint64,
+                                 C.POINTER(C.c_void_p)]
+    lib.synth_gpu_download.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+    lib.synth_gpu_free.argtypes = [C.c_void_p]
+    lib.synth_gpu_host_alloc.argtypes = [C.c_size_t, C.POINTER(C.c_void_p)]
+    lib.synth_gpu_copy_calibration.argtypes = [C.c_size_t, C.c_int, C.POINTER(C.c_float)]
+    return lib
+
+
+class DeviceDatabases:
+    """Both merged databases of a Problem generated in device memory (fwd 10 variables,
+    bwd 5, the seeds of fill_db); points the Problem's setup at them.  free() releases
+    them; download(which) copies one to a host array."""
+
+    def __init__(self, p):
+        self.lib = synth_gpu_lib()
+        self.p = p
+        m = p.mesh
+        self.ptrs = []
+        for nvar, seed in ((10, p.seed * 1000 + 1), (5, p.seed * 1000 + 2)):
+            d = C.c_void_p()
+            rc = self.lib.synth_gpu_db(m.nelem, abi.ptr(m.sem, C.c_int32), nvar, p.nt, p.dt, seed, C.byref(d))
+            if rc:
+                raise RuntimeError("device database generation failed (%d)" % rc)
+            self.ptrs.append(d)
+        self.attach()
+
+    def attach(self):
+        self.p.setup.fwd.snapshots = C.cast(self.ptrs[0], C.POINTER(C.c_float))
+        self.p.setup.bwd.snapshots = C.cast(self.ptrs[1], C.POINTER(C.c_float))
+
+    def download(self, which):
+        m = self.p.mesh
+        nvar = 10 if which == 0 else 5
+        out = np.empty((m.nelem, nvar, 5, 5, self.p.nt), np.float32)
+        if self.lib.synth_gpu_download(self.ptrs[which], out.ctypes.data, out.size):
+            raise RuntimeError("database download failed")
+        return out
+
+    def free(self):
+        for d in self.ptrs:
+            if d:
+                self.lib.synth_gpu_free(d)
+        self.ptrs = []
+
+
 def model_table(mesh, dz=100.0):
     lib = synth_lib()
     nd = lib.synth_model_table(mesh.npoints, abi.ptr(mesh.mesh_s, C.c_float),
@@ -195,9 +250,10 @@ class Problem:
     def __init__(self, name, nelem_target, nt, dt, receivers_deg, filters, kernels, ntet,
                  tet_edge, npoints_per_step, max_iter, allowed_error=1e-4,
                  allowed_relative_error=1e-2, component="Z", mij=None, seed=1,
-                 mesh_seed=2, src=(20.0, 30.0, 100.0), azimuth=135.0, deconv_stf=0,
+                 mesh_seed=2, src=(20.0, 30.0, 100.0), azimuth=135.0, deconv_stf=1,
                  int_scheme="parseval", db_threads=None, fill=True, unmasked_frac=0.0):
         self.name = name
+        self.seed = seed
         self.mesh = Mesh(nelem_target)
         self.nt, self.dt = nt, dt
         m = self.mesh
@@ -224,7 +280,10 @@ class Problem:
             for per in (20.0, 30.0, 40.0, 60.0):
                 seis[r] += wavelet(t, tp + 20.0, per) + 0.7 * wavelet(t, 1.8 * tp + 20.0, per)
         self.seis = np.ascontiguousarray(seis * 1e-6)
-        self.stf = np.exp(-((t - 10.0) / (2.0 * dt)) ** 2)
+        # the database's source time function (stf_dump): a Gaussian one sample wide, so
+        # that its deconvolution (add_stfs) keeps the filters vanishing at the Nyquist
+        # frequency, as the reference requires
+        self.stf = np.exp(-((t - 10.0) / dt) ** 2)
         self.stf /= self.stf.sum() * dt
         # filters: list of (class, (f1, f2, f3, f4)); kernels: list of dicts
         self.filters = (abi.MckFilter * len(filters))()

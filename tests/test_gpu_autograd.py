@@ -38,7 +38,7 @@ def rel(a, b):
 def stage(request):
     import mc_kernel_amd as mk
     from oracle import oracle
-    p = synth.make_problem("tiny", ntet=10, **VARIANTS[request.param])
+    p = synth.make_problem("tiny", ntet=10, **{"deconv_stf": 0, **VARIANTS[request.param]})
     print("autograd case", request.param, flush=True)
     o = oracle.Oracle(p.setup)
     k = mk.MCKernel(p.setup)

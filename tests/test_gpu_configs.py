@@ -1,0 +1,158 @@
+"""Parity of the HIP path with the FP64 oracle at each workload's own kernel
+instantiation.  The trace length sets the samples per thread of the strain/FFT kernels
+(nt 1024: 4, nt 2048: 8, nt 4096: 16) and the source mechanism and receiver component set
+the carried strain components (trace-only sets: 1; explosion with Z receivers: 4 and 4;
+a general moment tensor with R receivers: 6 and 6):
+
+  c0        the whole workload (500 tetrahedra, 32 points per step), against the oracle's
+            result kept in tests/fixtures/c0_oracle.npz (see make_c0_oracle.py)
+  c1        160 kernels at nt 2048, explosion / Z and general moment tensor / R
+  c2        the 1,200-kernel set (50 receivers, 5 Gabor + Butterworth band pass) at nt 2048
+  nt 4096   16 samples per thread (the c4 trace length)
+
+The databases are made in device memory by the device generator and copied back for the
+oracle; the generator is checked bit for bit against the host generator first.  Bars:
+element ids and iteration counts bit-exact; kernel values within 1e-5 relative to each
+kernel's L2 norm over voxels (base kernels: over points).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from mc_kernel_amd import abi, synth
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+HERE = os.path.dirname(os.path.abspath(__file__))
+DC = [0.3e20, 1.0e20, -1.3e20, 0.5e20, -0.4e20, 0.7e20]   # general moment tensor
+
+
+def _rel(g, r):
+    n = np.linalg.norm(r)
+    return np.linalg.norm(g - r) / n if n else np.linalg.norm(g)
+
+
+@pytest.mark.parametrize("nelem,nt,dt", [(450, 256, 4.0), (120, 2048, 1.0)])
+def test_device_generator_bit_exact(nelem, nt, dt):
+    """synth_gpu.hip reproduces synth.cpp's samples bit for bit (both databases)."""
+    lib = synth.synth_gpu_lib()
+    m = synth.Mesh(nelem)
+    for nvar, seed in ((10, 1001), (5, 1002)):
+        host = synth.fill_db(m, nvar, nt, dt, seed)
+        d = C.c_void_p()
+        assert lib.synth_gpu_db(m.nelem, abi.ptr(m.sem, C.c_int32), nvar, nt, dt, seed, C.byref(d)) == 0
+        dev = np.empty_like(host)
+        try:
+            assert lib.synth_gpu_download(d, dev.ctypes.data, dev.size) == 0
+        finally:
+            lib.synth_gpu_free(d)
+        assert np.abs(host).max() > 0
+        assert np.array_equal(host.view(np.uint32), dev.view(np.uint32)), (nvar, nt)
+
+
+def _oracle_on_device_databases(p):
+    """(oracle on host copies of the device-generated databases, device databases)."""
+    from oracle import oracle
+    dev = synth.DeviceDatabases(p)
+    host = (dev.download(0), dev.download(1))
+    p.setup.fwd.snapshots = abi.ptr(host[0], C.c_float)
+    p.setup.bwd.snapshots = abi.ptr(host[1], C.c_float)
+    o = oracle.Oracle(p.setup)
+    o._host_dbs = host
+    dev.attach()
+    return o, dev
+
+
+def _check_instantiation(stats, spt, nd_f, nd_b):
+    assert (stats["spt"], stats["nd_f"], stats["nd_b"]) == (spt, nd_f, nd_b), stats
+
+
+def _pinned_case(p, spt, nd_f, nd_b, npts=4):
+    import mc_kernel_amd as mk
+    o, dev = _oracle_on_device_databases(p)
+    try:
+        x = p.sample_points(npts, seed=9)
+        ref_b = o.base_kernels(x)
+        ref = o.slave_work(p.verts, p.conn)
+        k = mk.MCKernel(p.setup)
+        try:
+            g_b = k.base_kernels(x)
+            _check_instantiation(k.stats(), spt, nd_f, nd_b)
+            res = k.slave_work(p.verts, p.conn, halton_offsets=ref["offsets"])
+            _check_instantiation(res["stats"], spt, nd_f, nd_b)
+        finally:
+            k.close()
+    finally:
+        dev.free()
+    assert np.abs(ref_b).max() > 0
+    for j in range(p.nkernel):
+        for b in range(6):
+            if not np.any(ref_b[:, j, b]):
+                assert not np.any(g_b[:, j, b]), (j, b)
+                continue
+            assert _rel(g_b[:, j, b], ref_b[:, j, b]) < TOL, (j, b, _rel(g_b[:, j, b], ref_b[:, j, b]))
+    assert np.array_equal(res["niterations"], ref["niterations"])
+    assert res["stats"]["evaluations"] == ref["evaluations"] > 0
+    for j in range(p.nkernel):
+        assert _rel(res["values"][:, j], ref["values"][:, j]) < TOL, (j, _rel(res["values"][:, j], ref["values"][:, j]))
+    return res, ref
+
+
+C1_REDUCED = dict(nelem_target=1000, ntet=3, npoints_per_step=8, max_iter=2, unmasked_frac=1.0)
+
+
+def test_c1_explosion_z():
+    """c1's kernel set (10 receivers x 4 Gabor x CC/AM x vp/vs) at nt 2048, explosive
+    source, Z receivers: 8 samples per thread, 4 and 4 carried components."""
+    p = synth.make_problem("c1", fill=False, **C1_REDUCED)
+    assert p.nkernel == 160 and p.nt == 2048
+    _pinned_case(p, 8, 4, 4)
+
+
+def test_c1_general_moment_tensor_r():
+    """c1's kernel set with a general moment tensor and R receivers: 6 and 6 carried
+    components (all four forward simulations, the PX backward simulation)."""
+    p = synth.make_problem("c1", fill=False, mij=DC, component="R", **C1_REDUCED)
+    _pinned_case(p, 8, 6, 6)
+
+
+def test_c2_kernel_set_nt2048():
+    """c2's 1,200-kernel set (72 window-spectrum x base-kernel slots per receiver) at its
+    own nt 2048."""
+    p = synth.make_problem("c2", fill=False, nelem_target=1000, ntet=2, npoints_per_step=4, max_iter=2,
+                           unmasked_frac=1.0)
+    assert p.nkernel == 1200 and p.nt == 2048
+    _pinned_case(p, 8, 4, 4)
+
+
+def test_nt4096():
+    """nt 4096 (nfft 8192, the c4 trace length): 16 samples per thread."""
+    p = synth.make_problem("c1", fill=False, nelem_target=600, nt=4096, receivers_deg=[30.0, 60.0],
+                           kernels=synth._kernels(2, 4, ["CC", "AM"], ["vp", "vs"]), ntet=3,
+                           npoints_per_step=8, max_iter=2, unmasked_frac=1.0)
+    _pinned_case(p, 16, 4, 4)
+
+
+def test_c0_whole_workload():
+    """The whole c0 workload (trace-only kernels: 4 samples per thread, 1 and 1 carried
+    components) against the oracle's kept result, with the oracle's Halton positions."""
+    import mc_kernel_amd as mk
+    ref = np.load(os.path.join(HERE, "fixtures", "c0_oracle.npz"))
+    p = synth.make_problem("c0", fill=False)
+    dev = synth.DeviceDatabases(p)
+    try:
+        k = mk.MCKernel(p.setup)
+        try:
+            res = k.slave_work(p.verts, p.conn, halton_offsets=ref["offsets"])
+        finally:
+            k.close()
+    finally:
+        dev.free()
+    _check_instantiation(res["stats"], 4, 1, 1)
+    assert np.array_equal(res["niterations"], ref["niterations"])
+    assert res["stats"]["evaluations"] == int(ref["evaluations"]) > 0
+    for j in range(p.nkernel):
+        assert _rel(res["values"][:, j], ref["values"][:, j]) < TOL, j

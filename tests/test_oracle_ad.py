@@ -25,7 +25,7 @@ VARIANTS = {
 @pytest.fixture(scope="module", params=sorted(VARIANTS))
 def stage(request):
     from oracle import oracle
-    p = synth.make_problem("tiny", ntet=10, **VARIANTS[request.param])
+    p = synth.make_problem("tiny", ntet=10, **{"deconv_stf": 0, **VARIANTS[request.param]})
     o = oracle.Oracle(p.setup)
     x = p.sample_points(2, seed=5)
     fw = o.load_strain(0, x)
