@@ -22,6 +22,7 @@
 #include "mck_device.h"
 
 #include "mck_kernels.inc.h"
+#include "mck_mfma.inc.h"
 #include "mck_ad.inc.h"
 #include "mck_results.inc.h"
 
@@ -383,6 +384,13 @@ struct State {
     std::vector<double> seis_host;      // [nrec][nt] seismograms by reciprocity (seis NULL)
     hipStream_t copy_stream = nullptr;  // page copies of paged databases
     DBuf<int> keys_in, keys_out, vals_in, vals_out, pair_count;
+    // backward evaluation through element spectra (k_elem_spectra + k_bwd_mfma): element
+    // runs of a launch and the nodal spectra of a batch of runs
+    DBuf<int> run_first, run_elem, nrun_d;
+    DBuf<float2> Uspec;
+    bool mfma_ok = false, mfma_ok_base = false;   // kernel sets the matrix-core path serves
+    int bwd_path_env = -1;                        // MCK_BWD_PATH: 0 per-pair, 1 matrix cores
+    size_t uspec_budget = 0;                      // bytes of element spectra per batch
     DBuf<unsigned char> sort_tmp;
     DBuf<double> phys;
     DBuf<unsigned long long> evals;
@@ -865,6 +873,16 @@ int build_kernels(State &G) {
     HIPCHECK(hipMemcpy(G.recs_base.p, recs_base.data(), G.R * sizeof(RecDesc), hipMemcpyHostToDevice));
     HIPCHECK(G.kerns_base.ensure(G.nk));
     HIPCHECK(hipMemcpy(G.kerns_base.p, kerns_base.data(), G.nk * sizeof(KernDesc), hipMemcpyHostToDevice));
+    // the matrix-core backward path forms the lambda and mu base kernels only
+    {
+        bool iso = true;
+        for (int k = 0; k < G.nk; ++k) {
+            const int p = S.kernel_param[k];
+            if (!(p == MCK_PARAM_LAM || p == MCK_PARAM_VP || p == MCK_PARAM_VS || p == MCK_PARAM_MU)) iso = false;
+        }
+        G.mfma_ok = iso && (G.pc.need_bk & ~3) == 0;
+        G.mfma_ok_base = iso && (G.pc_base.need_bk & ~3) == 0;
+    }
     G.rec_first.resize(G.R);
     G.rec_count.resize(G.R);
     for (int r = 0; r < G.R; ++r) { G.rec_first[r] = recs[r].first_kernel; G.rec_count[r] = recs[r].nkernel; }
@@ -977,6 +995,88 @@ hipError_t launch_bwd(State &G, const DevDb &db, int npairs, const int *order, c
     return hipErrorInvalidValue;
 }
 
+// Backward evaluation of npairs pairs sorted by (element, receiver) (keys, order) whose
+// elements db holds, through element spectra: the runs of equal elements, then per batch of
+// runs k_elem_spectra and k_bwd_mfma.  Wb, Qb: the operator set-up of the first pair.
+int launch_bwd_mfma(State &G, const DevDb &db, int npairs, const int *keys, const int *order, const float *Wb,
+                    const float *Qb, double *phys, double *base_out, const uint8_t *active, int P) {
+    if (npairs <= 0) return MCK_OK;
+    const PathConf &pc = base_out ? G.pc_base : G.pc;
+    const RecDesc *recs = base_out ? G.recs_base.p : G.recs.p;
+    const KernDesc *kerns = base_out ? G.kerns_base.p : G.kerns.p;
+    const int R = G.R;
+    hipStream_t st = G.stream;
+    // runs: flags (keys_in), inclusive scan (vals_in), table
+    HIPCHECK(G.run_first.ensure((size_t)npairs + 1));
+    HIPCHECK(G.run_elem.ensure((size_t)npairs));
+    HIPCHECK(G.nrun_d.ensure(1));
+    hipLaunchKernelGGL(k_run_flags, dim3((npairs + 255) / 256), dim3(256), 0, st, npairs, keys, R, G.keys_in.p);
+    HIPCHECK(hipGetLastError());
+    size_t need = 0;
+    HIPCHECK(hipcub::DeviceScan::InclusiveSum(nullptr, need, G.keys_in.p, G.vals_in.p, npairs, st));
+    HIPCHECK(G.sort_tmp.ensure(need));
+    HIPCHECK(hipcub::DeviceScan::InclusiveSum(G.sort_tmp.p, need, G.keys_in.p, G.vals_in.p, npairs, st));
+    hipLaunchKernelGGL(k_run_table, dim3((npairs + 255) / 256), dim3(256), 0, st, npairs, keys, R, G.keys_in.p,
+                       G.vals_in.p, G.run_first.p, G.run_elem.p, G.nrun_d.p);
+    HIPCHECK(hipGetLastError());
+    int nrun = 0;
+    HIPCHECK(hipMemcpyAsync(&nrun, G.nrun_d.p, 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    const int J = pc.nvar_bwd * 25, KS = (J + 3) / 4;
+    const size_t per_elem = (size_t)J * pc.ustride * sizeof(float2);
+    const int batch = (int)std::max<size_t>(1, std::min<size_t>((size_t)nrun, G.uspec_budget / per_elem));
+    HIPCHECK(G.Uspec.ensure((size_t)batch * J * pc.ustride));
+    const size_t smem_s = (2 * (size_t)pc.M + tq_len(pc.M)) * sizeof(float2);
+    const size_t smem_m = (size_t)kMfmaWaves * mfma_wave_lds(KS, pc.nd_b, pc.max_slots);
+    const int spt = spt_for(pc.nt);
+    G.stats.spt = spt;
+    G.stats.nd_f = pc.nd_f;
+    G.stats.nd_b = pc.nd_b;
+    G.stats.bwd_path = 1;
+    for (int u0 = 0; u0 < nrun; u0 += batch) {
+        const dim3 grid((unsigned)std::min(batch, nrun - u0));
+        bool launched = false;
+#define SPEC(SP)                                                                                  \
+        if (spt == SP) {                                                                          \
+            if (smem_s > 65536)                                                                   \
+                (void)hipFuncSetAttribute((const void *)&k_elem_spectra<SP>,                      \
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem_s); \
+            hipLaunchKernelGGL((k_elem_spectra<SP>), grid, dim3(kStrainThreads), smem_s, st, db, pc, u0, \
+                               nrun, G.run_elem.p, G.twq.p, G.Uspec.p);                           \
+            launched = true;                                                                      \
+        }
+        SPEC(1) SPEC(2) SPEC(4) SPEC(8) SPEC(16)
+#undef SPEC
+        if (!launched) return fail(MCK_EINVAL, "no element-spectra kernel for this trace length");
+        HIPCHECK(hipGetLastError());
+        launched = false;
+#define EVAL(NF, NB, NV)                                                                          \
+        if (pc.nd_f == NF && pc.nd_b == NB && pc.nvar_bwd == NV) {                                \
+            if (smem_m > 65536)                                                                   \
+                (void)hipFuncSetAttribute((const void *)&k_bwd_mfma<NF, NB, NV>,                  \
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem_m); \
+            hipLaunchKernelGGL((k_bwd_mfma<NF, NB, NV>), grid, dim3(64 * kMfmaWaves), smem_m, st, pc, u0, nrun, \
+                               G.run_first.p, G.Uspec.p, order, R, Wb, Qb, G.opQf.p, G.Fspec.p, recs, kerns, \
+                               G.H.p, pc.max_slots, G.model_idx.p, G.model.p, G.S.model_ndepth,   \
+                               G.S.relative_kernel, G.nk, phys, base_out, G.xyz.p, active, P, G.d_rec_r.p); \
+            launched = true;                                                                      \
+        }
+        EVAL(1, 1, 2) EVAL(1, 1, 3) EVAL(4, 4, 2) EVAL(6, 4, 2) EVAL(4, 6, 3) EVAL(6, 6, 3)
+#undef EVAL
+        if (!launched) return fail(MCK_EINVAL, "no matrix-core kernel for these strain components");
+        HIPCHECK(hipGetLastError());
+    }
+    return MCK_OK;
+}
+
+// the backward path of a launch: element spectra and matrix cores when the kernel set
+// allows it (MCK_BWD_PATH=0 forces the per-pair kernel, 1 the matrix-core one)
+bool use_mfma(const State &G, bool base) {
+    const bool ok = base ? G.mfma_ok_base : G.mfma_ok;
+    if (G.bwd_path_env == 0) return false;
+    return ok;
+}
+
 hipError_t launch_traces(State &G, const DevDb &db, int fwd, int frame, int npts, const Located *loc,
                          float *out) {
     if (npts <= 0) return hipSuccess;
@@ -1002,11 +1102,18 @@ hipError_t launch_traces(State &G, const DevDb &db, int fwd, int frame, int npts
 // item in launch order, ascending (pages then cover disjoint item ranges and untouched
 // pages are not copied); null: every page covers all items and the kernels skip the items
 // outside it.
+// (a launch returns a hipError_t, or an MCK_* code whose message is already set)
+inline int launch_status(hipError_t e) {
+    return e == hipSuccess ? MCK_OK : fail(MCK_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+}
+inline int launch_status(int rc) { return rc; }
+
 template <class Launch>
 int run_paged(State &G, DbDev &D, int n, const int *sorted_keys, Launch &&launch, int key_scale = 1) {
     if (n <= 0) return MCK_OK;
+    int lrc;
     if (!D.paged) {
-        HIPCHECK(launch(D.d, 0, n));
+        if ((lrc = launch_status(launch(D.d, 0, n)))) return lrc;
         return MCK_OK;
     }
     std::vector<int> bounds(D.npages + 1, n);
@@ -1045,7 +1152,7 @@ int run_paged(State &G, DbDev &D, int n, const int *sorted_keys, Launch &&launch
         d.u = D.slot[k];
         d.u_first = e0;
         d.u_last = e1;
-        HIPCHECK(launch(d, i0, i1));
+        if ((lrc = launch_status(launch(d, i0, i1)))) return lrc;
         HIPCHECK(hipEventRecord(D.freed[k], G.stream));
     }
     return MCK_OK;
@@ -1283,6 +1390,7 @@ int init_impl(const mck_setup_t *s) {
     pc.nt = S.nt;
     pc.M = nextpow2(S.nt);
     pc.nomega = pc.M + 1;
+    pc.ustride = (pc.nomega + 15) / 16 * 16;
     pc.dt = S.dt;
     // strain type (type_parameter.f90:477-486)
     pc.full = 0;
@@ -1388,6 +1496,11 @@ int init_impl(const mck_setup_t *s) {
                      + (size_t)G.R * ((size_t)pc.nd_b * pc.nvar_bwd * 25 + 36) * sizeof(float);
     size_t budget = std::min<size_t>(free_b / 3, (size_t)64 << 30);
     G.chunk_points = (int)std::max<size_t>(1024, std::min<size_t>(budget / per_point, 1 << 21));
+    // element spectra of the matrix-core path: batches of element runs within an eighth of
+    // the free memory (at most 16 GiB)
+    G.uspec_budget = std::min<size_t>(free_b / 8, (size_t)16 << 30);
+    G.bwd_path_env = -1;
+    if (const char *e = std::getenv("MCK_BWD_PATH")) G.bwd_path_env = std::atoi(e);
     // tuning knob: points per evaluation launch (smaller launches keep the forward
     // spectra of a launch in the last-level cache)
     if (const char *e = std::getenv("MCK_CHUNK_POINTS")) {
@@ -1703,10 +1816,17 @@ int eval_chunk(State &G, int P, int w0, int nw, std::vector<std::pair<hipEvent_t
     {
         const size_t nwb = (size_t)G.pc.nd_b * G.pc.nvar_bwd * 25;
         double *phys = G.phys.p + (size_t)w0 * P * G.nk;
-        if ((rc = run_paged(G, G.bwd, npairs, G.keys_out.p, [&](const DevDb &d, int i0, int i1) {
-                 return launch_bwd(G, d, i1 - i0, G.vals_out.p + i0, loc_b, G.opWb.p + (size_t)i0 * nwb,
-                                   G.opQb.p + (size_t)i0 * 36, phys, nullptr, act, P);
-             }, R)))
+        if (use_mfma(G, false)) {
+            if ((rc = run_paged(G, G.bwd, npairs, G.keys_out.p, [&](const DevDb &d, int i0, int i1) {
+                     return launch_bwd_mfma(G, d, i1 - i0, G.keys_out.p + i0, G.vals_out.p + i0,
+                                            G.opWb.p + (size_t)i0 * nwb, G.opQb.p + (size_t)i0 * 36, phys,
+                                            nullptr, act, P);
+                 }, R)))
+                return rc;
+        } else if ((rc = run_paged(G, G.bwd, npairs, G.keys_out.p, [&](const DevDb &d, int i0, int i1) {
+                        return launch_bwd(G, d, i1 - i0, G.vals_out.p + i0, loc_b, G.opWb.p + (size_t)i0 * nwb,
+                                          G.opQb.p + (size_t)i0 * 36, phys, nullptr, act, P);
+                    }, R)))
             return rc;
     }
     HIPCHECK(hipEventRecord(b, st));
@@ -2216,21 +2336,46 @@ int mck_base_kernels(int32_t npts, const double *xyz, double *base) {
     HIPCHECK(hipGetLastError());
     double *d_base = nullptr, *d_phys = nullptr;
     HIPCHECK(hipMalloc((void **)&d_base, (size_t)npts * G.nk * 6 * 8));
-    HIPCHECK(hipMalloc((void **)&d_phys, (size_t)npts * G.nk * 8));
-    hipError_t e = hipMemsetAsync(d_base, 0, (size_t)npts * G.nk * 6 * 8, G.stream);
-    if (e == hipSuccess) e = launch_setup(G, true, npts, nullptr, nullptr, loc_f);
-    if (e == hipSuccess)
-        rc = run_paged(G, G.fwd, npts, nullptr, [&](const DevDb &d, int, int) {
-            return launch_fwd(G, d, npts, nullptr, nullptr, nullptr, loc_f, G.Fspec.p);
-        });
-    if (e == hipSuccess && rc == MCK_OK) e = launch_setup(G, false, npts * R, nullptr, nullptr, loc_b);
-    if (e == hipSuccess && rc == MCK_OK)
-        rc = run_paged(G, G.bwd, npts * R, nullptr, [&](const DevDb &d, int, int) {
+    if (hipMalloc((void **)&d_phys, (size_t)npts * G.nk * 8) != hipSuccess) {
+        (void)hipFree(d_base);
+        return fail(MCK_ENOMEM, "device allocation failed");
+    }
+    hipStream_t st = G.stream;
+    auto body = [&]() -> int {
+        int rc2;
+        HIPCHECK(hipMemsetAsync(d_base, 0, (size_t)npts * G.nk * 6 * 8, st));
+        HIPCHECK(launch_setup(G, true, npts, nullptr, nullptr, loc_f));
+        if ((rc2 = run_paged(G, G.fwd, npts, nullptr, [&](const DevDb &d, int, int) {
+                 return launch_fwd(G, d, npts, nullptr, nullptr, nullptr, loc_f, G.Fspec.p);
+             })))
+            return rc2;
+        if (use_mfma(G, true)) {
+            // the located pairs in (element, receiver) order, as slave_work evaluates them
+            HIPCHECK(hipMemsetAsync(G.pair_count.p, 0, 4, st));
+            hipLaunchKernelGGL(k_keys_pairs, dim3((unsigned)(((long long)npts * R + 255) / 256)), dim3(256), 0, st,
+                               npts * R, R, 1, nullptr, loc_b, G.keys_in.p, G.vals_in.p, G.pair_count.p);
+            HIPCHECK(hipGetLastError());
+            int npairs = 0;
+            HIPCHECK(hipMemcpyAsync(&npairs, G.pair_count.p, 4, hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipStreamSynchronize(st));
+            if ((rc2 = sort_by_element(G, npairs, G.bwd.d.nelem * R))) return rc2;
+            HIPCHECK(launch_setup(G, false, npairs, G.vals_out.p, nullptr, loc_b));
+            const size_t nwb = (size_t)G.pc.nd_b * G.pc.nvar_bwd * 25;
+            return run_paged(G, G.bwd, npairs, G.keys_out.p, [&](const DevDb &d, int i0, int i1) {
+                return launch_bwd_mfma(G, d, i1 - i0, G.keys_out.p + i0, G.vals_out.p + i0, G.opWb.p + i0 * nwb,
+                                       G.opQb.p + (size_t)i0 * 36, d_phys, d_base, nullptr, 1);
+            }, R);
+        }
+        HIPCHECK(launch_setup(G, false, npts * R, nullptr, nullptr, loc_b));
+        return run_paged(G, G.bwd, npts * R, nullptr, [&](const DevDb &d, int, int) {
             return launch_bwd(G, d, npts * R, nullptr, loc_b, G.opWb.p, G.opQb.p, d_phys, d_base, nullptr, 1);
         });
-    if (e == hipSuccess && rc == MCK_OK)
-        e = hipMemcpyAsync(base, d_base, (size_t)npts * G.nk * 6 * 8, hipMemcpyDeviceToHost, G.stream);
-    if (e == hipSuccess && rc == MCK_OK) e = hipStreamSynchronize(G.stream);
+    };
+    rc = body();
+    hipError_t e = hipSuccess;
+    if (rc == MCK_OK) e = hipMemcpyAsync(base, d_base, (size_t)npts * G.nk * 6 * 8, hipMemcpyDeviceToHost, st);
+    if (rc == MCK_OK && e == hipSuccess) e = hipStreamSynchronize(st);
+    else (void)hipStreamSynchronize(st);
     (void)hipFree(d_base);
     (void)hipFree(d_phys);
     if (rc) return rc;

@@ -64,6 +64,7 @@ struct Located {
 // Path-wide configuration passed by value to the kernels.
 struct PathConf {
     int nt, M, nomega;        // samples, FFT half length (N/2) and N/2+1
+    int ustride;              // row stride of element spectra: nomega rounded up to 16
     double dt;
     int nd;                   // 1 (trace) or 6 (full strain)
     int full;

@@ -122,7 +122,7 @@ def test_c1_general_moment_tensor_r():
 def test_c2_kernel_set_nt2048():
     """c2's 1,200-kernel set (72 window-spectrum x base-kernel slots per receiver) at its
     own nt 2048."""
-    p = synth.make_problem("c2", fill=False, nelem_target=1000, ntet=2, npoints_per_step=4, max_iter=2,
+    p = synth.make_problem("c2", fill=False, nelem_target=1000, ntet=4, npoints_per_step=4, max_iter=2,
                            unmasked_frac=1.0)
     assert p.nkernel == 1200 and p.nt == 2048
     _pinned_case(p, 8, 4, 4)

@@ -19,6 +19,8 @@ CASES = {
     "explosion_T": dict(component="T"),
     "anisotropic": dict(kernels=synth._kernels(1, 2, ["CC"], ["vsh", "vsv", "vph", "vpv", "eta", "mu"]),
                         receivers_deg=[40.0]),
+    # NO_DECONVOLVE_STF: the databases' time shifts instead of the STF deconvolution
+    "no_deconvolution": dict(deconv_stf=0),
 }
 
 
@@ -148,8 +150,10 @@ def test_c2_kernel_set():
     database and task so that the oracle runs in seconds."""
     import mc_kernel_amd as mk
     from oracle import oracle
+    # receivers 10..34.5 deg: every kernel window (at most 1.8 x P time + 70 s) inside the
+    # 1024-sample traces
     p = synth.make_problem("c2", nelem_target=2000, nt=1024, ntet=2, npoints_per_step=8, max_iter=2,
-                           unmasked_frac=1.0)
+                           unmasked_frac=1.0, receivers_deg=[10.0 + 0.5 * i for i in range(50)])
     assert p.nkernel == 1200
     o = oracle.Oracle(p.setup)
     x = p.sample_points(4, seed=9)
