@@ -265,7 +265,7 @@ __global__ __launch_bounds__(kStrainThreads) void k_ad_fspec(PathConf pc, int np
     float2 *bufA = reinterpret_cast<float2 *>(smem_raw);
     float2 *bufB = bufA + pc.M;
     float2 *Tq = bufA + 2 * pc.M;
-    __shared__ float wave_tot[kStrainThreads / 64];
+    __shared__ double wave_tot[kStrainThreads / 64];
     __shared__ float edge_x[kStrainThreads + 1];
     const int p = blockIdx.x;
     if (p >= npts) return;

@@ -351,6 +351,7 @@ struct State {
     DBuf<DevFrame> frames;
     std::vector<DevFrame> frames_host;
     DBuf<float2> twq, H;
+    DBuf<double2> twq_d;                     // the quarter twiddle table in double precision
     DBuf<RecDesc> recs, recs_base;           // slot layouts: physical kernels / base kernels
     DBuf<KernDesc> kerns, kerns_base;
     PathConf pc_base{};                      // pc with the base-kernel layout
@@ -375,7 +376,7 @@ struct State {
     DBuf<double> xyz;
     DBuf<int> model_idx;
     DBuf<Located> loc;
-    DBuf<float2> Fspec;                 // [point][nd_f][nomega] forward spectra
+    DBuf<float2> Fspec;                 // [point][nd_f][fstride] forward spectra
     // operator set-up (k_op_setup): gather weights and rotations of the points (by point
     // id) and of the pairs (by position in the pair order)
     DBuf<float> opWf, opQf, opWb, opQb;
@@ -386,10 +387,11 @@ struct State {
     DBuf<int> keys_in, keys_out, vals_in, vals_out, pair_count;
     // backward evaluation through element spectra (k_elem_spectra + k_bwd_mfma): element
     // runs of a launch and the nodal spectra of a batch of runs
-    DBuf<int> run_first, run_elem, nrun_d;
-    DBuf<float2> Uspec;
+    DBuf<int> worktab;
+    DBuf<float2> Uspec, Hp;                       // element spectra; window spectra, rows fstride
     bool mfma_ok = false, mfma_ok_base = false;   // kernel sets the matrix-core path serves
     int bwd_path_env = -1;                        // MCK_BWD_PATH: 0 per-pair, 1 matrix cores
+    int mfma_xcd = 0;                             // MCK_MFMA_XCD: groups in XCD-contiguous ranges
     size_t uspec_budget = 0;                      // bytes of element spectra per batch
     DBuf<unsigned char> sort_tmp;
     DBuf<double> phys;
@@ -859,6 +861,12 @@ int build_kernels(State &G) {
                            d_tab.p + N, d_tf.p, d_tf.p + (size_t)S.nfilter * nw, d_tr.p, d_tr.p + nw, G.H.p,
                            G.ad_W.p);
         HIPCHECK(hipGetLastError());
+        // the window spectra again with rows fstride apart, zero beyond nomega (the
+        // matrix-core backward path reads them four bins at a time)
+        HIPCHECK(G.Hp.ensure((size_t)nH * G.pc.fstride));
+        HIPCHECK(hipMemsetAsync(G.Hp.p, 0, (size_t)nH * G.pc.fstride * sizeof(float2), G.stream));
+        HIPCHECK(hipMemcpy2DAsync(G.Hp.p, (size_t)G.pc.fstride * sizeof(float2), G.H.p, (size_t)nw * sizeof(float2),
+                                  (size_t)nw * sizeof(float2), nH, hipMemcpyDeviceToDevice, G.stream));
         HIPCHECK(hipStreamSynchronize(G.stream));
         d_w.release(); d_tab.release(); d_norm.release(); d_tf.release(); d_tr.release(); d_idx.release();
         G.nH = nH;
@@ -898,7 +906,8 @@ int build_kernels(State &G) {
 }
 
 
-// LDS of the forward kernel: two FFT buffers and the quarter twiddle table
+// LDS of the forward kernel: the operator set-up, two FFT buffers and the quarter twiddle
+// table
 size_t strain_smem(const PathConf &pc) {
     return (size_t)pc.op_bytes_f + (2 * (size_t)pc.M + tq_len(pc.M)) * sizeof(float2);
 }
@@ -996,53 +1005,63 @@ hipError_t launch_bwd(State &G, const DevDb &db, int npairs, const int *order, c
 }
 
 // Backward evaluation of npairs pairs sorted by (element, receiver) (keys, order) whose
-// elements db holds, through element spectra: the runs of equal elements, then per batch of
-// runs k_elem_spectra and k_bwd_mfma.  Wb, Qb: the operator set-up of the first pair.
+// elements db holds, through element spectra: the element runs and the groups of at most
+// 16 pairs of one (element, receiver), then per batch of element runs k_elem_spectra and
+// k_bwd_mfma.  Wb, Qb: the operator set-up of the first pair.
 int launch_bwd_mfma(State &G, const DevDb &db, int npairs, const int *keys, const int *order, const float *Wb,
                     const float *Qb, double *phys, double *base_out, const uint8_t *active, int P) {
     if (npairs <= 0) return MCK_OK;
     const PathConf &pc = base_out ? G.pc_base : G.pc;
     const RecDesc *recs = base_out ? G.recs_base.p : G.recs.p;
     const KernDesc *kerns = base_out ? G.kerns_base.p : G.kerns.p;
-    const int R = G.R;
+    const int R = G.R, n = npairs;
     hipStream_t st = G.stream;
-    // runs: flags (keys_in), inclusive scan (vals_in), table
-    HIPCHECK(G.run_first.ensure((size_t)npairs + 1));
-    HIPCHECK(G.run_elem.ensure((size_t)npairs));
-    HIPCHECK(G.nrun_d.ensure(1));
-    hipLaunchKernelGGL(k_run_flags, dim3((npairs + 255) / 256), dim3(256), 0, st, npairs, keys, R, G.keys_in.p);
+    // work tables: flags and their inclusive scans, element runs, key runs, groups
+    HIPCHECK(G.worktab.ensure((size_t)12 * n + 8));
+    int *eflag = G.worktab.p, *kflag = eflag + n, *epos = kflag + n, *kpos = epos + n, *gflag = kpos + n,
+        *gpos = gflag + n, *erun_first = gpos + n, *erun_elem = erun_first + n + 1, *krun_first = erun_elem + n,
+        *group_first = krun_first + n, *group_erun = group_first + n + 1, *counts = group_erun + n;
+    const dim3 gr((n + 255) / 256), bl(256);
+    hipLaunchKernelGGL(k_run_flags, gr, bl, 0, st, n, keys, R, eflag, kflag);
     HIPCHECK(hipGetLastError());
     size_t need = 0;
-    HIPCHECK(hipcub::DeviceScan::InclusiveSum(nullptr, need, G.keys_in.p, G.vals_in.p, npairs, st));
+    HIPCHECK(hipcub::DeviceScan::InclusiveSum(nullptr, need, eflag, epos, n, st));
     HIPCHECK(G.sort_tmp.ensure(need));
-    HIPCHECK(hipcub::DeviceScan::InclusiveSum(G.sort_tmp.p, need, G.keys_in.p, G.vals_in.p, npairs, st));
-    hipLaunchKernelGGL(k_run_table, dim3((npairs + 255) / 256), dim3(256), 0, st, npairs, keys, R, G.keys_in.p,
-                       G.vals_in.p, G.run_first.p, G.run_elem.p, G.nrun_d.p);
+    HIPCHECK(hipcub::DeviceScan::InclusiveSum(G.sort_tmp.p, need, eflag, epos, n, st));
+    HIPCHECK(hipcub::DeviceScan::InclusiveSum(G.sort_tmp.p, need, kflag, kpos, n, st));
+    hipLaunchKernelGGL(k_run_table, gr, bl, 0, st, n, keys, R, eflag, epos, kflag, kpos, erun_first, erun_elem,
+                       counts, krun_first);
     HIPCHECK(hipGetLastError());
-    int nrun = 0;
-    HIPCHECK(hipMemcpyAsync(&nrun, G.nrun_d.p, 4, hipMemcpyDeviceToHost, st));
+    hipLaunchKernelGGL(k_group_flags, gr, bl, 0, st, n, kpos, krun_first, gflag);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipcub::DeviceScan::InclusiveSum(G.sort_tmp.p, need, gflag, gpos, n, st));
+    hipLaunchKernelGGL(k_group_table, gr, bl, 0, st, n, gflag, gpos, epos, group_first, group_erun, counts + 1);
+    HIPCHECK(hipGetLastError());
+    int cnt[2] = {0, 0};
+    HIPCHECK(hipMemcpyAsync(cnt, counts, 8, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
+    const int nerun = cnt[0], ngroups = cnt[1];
     const int J = pc.nvar_bwd * 25, KS = (J + 3) / 4;
     const size_t per_elem = (size_t)J * pc.ustride * sizeof(float2);
-    const int batch = (int)std::max<size_t>(1, std::min<size_t>((size_t)nrun, G.uspec_budget / per_elem));
+    const int batch = (int)std::max<size_t>(1, std::min<size_t>((size_t)nerun, G.uspec_budget / per_elem));
     HIPCHECK(G.Uspec.ensure((size_t)batch * J * pc.ustride));
-    const size_t smem_s = (2 * (size_t)pc.M + tq_len(pc.M)) * sizeof(float2);
-    const size_t smem_m = (size_t)kMfmaWaves * mfma_wave_lds(KS, pc.nd_b, pc.max_slots);
+    const size_t smem_s = 2 * (size_t)pc.M * sizeof(double2);
+    const size_t smem_m = mfma_wave_lds(KS, pc.nd_b, pc.max_slots);
     const int spt = spt_for(pc.nt);
     G.stats.spt = spt;
     G.stats.nd_f = pc.nd_f;
     G.stats.nd_b = pc.nd_b;
     G.stats.bwd_path = 1;
-    for (int u0 = 0; u0 < nrun; u0 += batch) {
-        const dim3 grid((unsigned)std::min(batch, nrun - u0));
+    for (int u0 = 0; u0 < nerun; u0 += batch) {
+        const int nb = std::min(batch, nerun - u0);
         bool launched = false;
 #define SPEC(SP)                                                                                  \
         if (spt == SP) {                                                                          \
             if (smem_s > 65536)                                                                   \
                 (void)hipFuncSetAttribute((const void *)&k_elem_spectra<SP>,                      \
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem_s); \
-            hipLaunchKernelGGL((k_elem_spectra<SP>), grid, dim3(kStrainThreads), smem_s, st, db, pc, u0, \
-                               nrun, G.run_elem.p, G.twq.p, G.Uspec.p);                           \
+            hipLaunchKernelGGL((k_elem_spectra<SP>), dim3(nb), dim3(kStrainThreads), smem_s, st, db, pc, u0, \
+                               nerun, erun_elem, G.twq_d.p, G.Uspec.p);                           \
             launched = true;                                                                      \
         }
         SPEC(1) SPEC(2) SPEC(4) SPEC(8) SPEC(16)
@@ -1055,10 +1074,11 @@ int launch_bwd_mfma(State &G, const DevDb &db, int npairs, const int *keys, cons
             if (smem_m > 65536)                                                                   \
                 (void)hipFuncSetAttribute((const void *)&k_bwd_mfma<NF, NB, NV>,                  \
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem_m); \
-            hipLaunchKernelGGL((k_bwd_mfma<NF, NB, NV>), grid, dim3(64 * kMfmaWaves), smem_m, st, pc, u0, nrun, \
-                               G.run_first.p, G.Uspec.p, order, R, Wb, Qb, G.opQf.p, G.Fspec.p, recs, kerns, \
-                               G.H.p, pc.max_slots, G.model_idx.p, G.model.p, G.S.model_ndepth,   \
-                               G.S.relative_kernel, G.nk, phys, base_out, G.xyz.p, active, P, G.d_rec_r.p); \
+            hipLaunchKernelGGL((k_bwd_mfma<NF, NB, NV>), dim3(grid_x8(ngroups)), dim3(64), smem_m, st, pc, u0, nb, \
+                               ngroups, group_first, group_erun, G.Uspec.p, order, R, Wb, Qb, G.opQf.p, \
+                               G.Fspec.p, recs, kerns, G.Hp.p, pc.max_slots, G.model_idx.p, G.model.p, \
+                               G.S.model_ndepth, G.S.relative_kernel, G.nk, phys, base_out, G.xyz.p, \
+                               active, P, G.d_rec_r.p, G.mfma_xcd);                               \
             launched = true;                                                                      \
         }
         EVAL(1, 1, 2) EVAL(1, 1, 3) EVAL(4, 4, 2) EVAL(6, 4, 2) EVAL(4, 6, 3) EVAL(6, 6, 3)
@@ -1339,6 +1359,15 @@ int setup_tables(State &G) {
     }
     HIPCHECK(G.twq.ensure(N / 4));
     HIPCHECK(hipMemcpy(G.twq.p, twq.data(), (N / 4) * sizeof(float2), hipMemcpyHostToDevice));
+    {
+        std::vector<double2> twd(N / 4);
+        for (int r = 0; r < N / 4; ++r) {
+            const double a = -2.0 * M_PI * r / N;
+            twd[r] = make_double2(std::cos(a), std::sin(a));
+        }
+        HIPCHECK(G.twq_d.ensure(N / 4));
+        HIPCHECK(hipMemcpy(G.twq_d.p, twd.data(), (N / 4) * sizeof(double2), hipMemcpyHostToDevice));
+    }
     // radial model table [6][ndepth]
     const int nd = S.model_ndepth;
     std::vector<float> mt((size_t)6 * nd);
@@ -1358,7 +1387,7 @@ int ensure_chunk(State &G, int points) {
     HIPCHECK(G.xyz.ensure((size_t)points * 3));
     HIPCHECK(G.model_idx.ensure(points));
     HIPCHECK(G.loc.ensure((size_t)points * (1 + R)));
-    HIPCHECK(G.Fspec.ensure((size_t)points * G.pc.nd_f * nw));
+    HIPCHECK(G.Fspec.ensure((size_t)points * G.pc.nd_f * G.pc.fstride));
     HIPCHECK(G.opWf.ensure((size_t)points * G.pc.nd_f * G.pc.nvar_fwd * 25));
     HIPCHECK(G.opQf.ensure((size_t)points * 36));
     HIPCHECK(G.opWb.ensure(npairs * G.pc.nd_b * G.pc.nvar_bwd * 25));
@@ -1391,6 +1420,7 @@ int init_impl(const mck_setup_t *s) {
     pc.M = nextpow2(S.nt);
     pc.nomega = pc.M + 1;
     pc.ustride = (pc.nomega + 15) / 16 * 16;
+    pc.fstride = pc.ustride;
     pc.dt = S.dt;
     // strain type (type_parameter.f90:477-486)
     pc.full = 0;
@@ -1490,7 +1520,7 @@ int init_impl(const mck_setup_t *s) {
     // points per chunk: forward spectra, locations and sort buffers within a memory budget
     size_t free_b = 0, total_b = 0;
     HIPCHECK(hipMemGetInfo(&free_b, &total_b));
-    size_t per_point = (size_t)pc.nd_f * pc.nomega * sizeof(float2) + (size_t)(1 + G.R) * sizeof(Located)
+    size_t per_point = (size_t)pc.nd_f * pc.fstride * sizeof(float2) + (size_t)(1 + G.R) * sizeof(Located)
                      + (size_t)G.R * 16 + 64
                      + ((size_t)pc.nd_f * pc.nvar_fwd * 25 + 36) * sizeof(float)
                      + (size_t)G.R * ((size_t)pc.nd_b * pc.nvar_bwd * 25 + 36) * sizeof(float);
@@ -1501,6 +1531,8 @@ int init_impl(const mck_setup_t *s) {
     G.uspec_budget = std::min<size_t>(free_b / 8, (size_t)16 << 30);
     G.bwd_path_env = -1;
     if (const char *e = std::getenv("MCK_BWD_PATH")) G.bwd_path_env = std::atoi(e);
+    G.mfma_xcd = 0;
+    if (const char *e = std::getenv("MCK_MFMA_XCD")) G.mfma_xcd = std::atoi(e) != 0;
     // tuning knob: points per evaluation launch (smaller launches keep the forward
     // spectra of a launch in the last-level cache)
     if (const char *e = std::getenv("MCK_CHUNK_POINTS")) {
@@ -2825,7 +2857,9 @@ void mck_finalize(void) {
         D->mp_s.release(); D->mp_z.release(); D->econ.release(); D->ejac.release(); D->eltype.release();
         D->axis.release(); D->cell_start.release(); D->cell_elems.release();
     }
-    G.frames.release(); G.twq.release(); G.H.release(); G.recs.release();
+    G.frames.release(); G.twq.release(); G.twq_d.release(); G.H.release(); G.recs.release();
+    G.recs_base.release(); G.kerns_base.release();
+    G.worktab.release(); G.Uspec.release(); G.Hp.release();
     G.kerns.release(); G.model.release(); G.d_k_rec.release(); G.d_k_nbk.release();
     G.d_rec_first.release(); G.d_rec_count.release(); G.d_rec_r.release(); G.tet.release(); G.volume.release();
     G.allowed_var.release(); G.hbase.release(); G.masked.release(); G.live.release();

@@ -65,6 +65,7 @@ struct Located {
 struct PathConf {
     int nt, M, nomega;        // samples, FFT half length (N/2) and N/2+1
     int ustride;              // row stride of element spectra: nomega rounded up to 16
+    int fstride;              // row stride of forward and window spectra (= ustride)
     double dt;
     int nd;                   // 1 (trace) or 6 (full strain)
     int full;

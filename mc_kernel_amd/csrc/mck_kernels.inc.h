@@ -1159,38 +1159,40 @@ __device__ inline void select_comp(const float (&acc)[ND][SPT], int c, int t0, i
 }
 
 // cumsum_trapezoidal (simple_routines.f90:314-341): c0 = x0, c1 = c0 + x1,
-// ct = c(t-1) + x(t-1) + xt, times dt/2; a block-wide scan of the per-thread totals
+// ct = c(t-1) + x(t-1) + xt, times dt/2; a block-wide scan of the per-thread totals.  The
+// running sums are kept in double precision (the reference's whole cumsum is FP64) and
+// rounded once per sample.
 template <int SPT>
 __device__ inline void cumsum_trapezoidal(float (&x)[SPT], int t0, int nt, float h, float *edge_x,
-                                          float *wave_tot)
+                                          double *wave_tot)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     edge_x[tid + 1] = x[SPT - 1];
     if (tid == 0) edge_x[0] = 0.0f;
     __syncthreads();
     float prev = edge_x[tid];
-    float d[SPT], run = 0.0f;
+    double d[SPT], run = 0.0;
 #pragma unroll
     for (int q = 0; q < SPT; ++q) {
         int t = t0 + q;
         float xm1 = q == 0 ? prev : x[q - 1];
-        d[q] = (t <= 1) ? x[q] : (xm1 + x[q]);
+        d[q] = (t <= 1) ? (double)x[q] : ((double)xm1 + (double)x[q]);
         run += d[q];
     }
-    float incl = run;
+    double incl = run;
     for (int o = 1; o < 64; o <<= 1) {
-        float y = __shfl_up(incl, o, 64);
+        double y = __shfl_up(incl, o, 64);
         if (lane >= o) incl += y;
     }
     if (lane == 63) wave_tot[wave] = incl;
     __syncthreads();
-    float wbase = 0.0f;
+    double wbase = 0.0;
     for (int w2 = 0; w2 < wave; ++w2) wbase += wave_tot[w2];
-    float cs = wbase + incl - run;
+    double cs = wbase + (incl - run);
 #pragma unroll
     for (int q = 0; q < SPT; ++q) {
         cs += d[q];
-        x[q] = (t0 + q < nt) ? cs * h : 0.0f;
+        x[q] = (t0 + q < nt) ? (float)(cs * (double)h) : 0.0f;
     }
     __syncthreads();
 }
@@ -1293,8 +1295,8 @@ __device__ inline const float2 *lds_fft(const float (&x)[SPT], int t0, int nt, i
             const float2 v0 = in[j], v1 = in[j + Mq], v2 = in[j + 2 * Mq], v3 = in[j + 3 * Mq];
             // w_M^t = w_N^(2t), N = 2M
             const int ti = kk << (lstride + 1);
-            // ti < N/4: w1 lies in the table's quadrant; w2 = w1^2, w3 = w1 w2
-            const float2 w1 = Tq[tq_slot(ti)], w2 = cmulf(w1, w1), w3 = cmulf(w1, w2);
+            // ti < N/4: w1 lies in the table's quadrant; w2 and w3 from the table too
+            const float2 w1 = Tq[tq_slot(ti)], w2 = twiddle(Tq, 2 * ti, lM - 1), w3 = twiddle(Tq, 3 * ti, lM - 1);
             const float2 a1 = make_float2(v1.x * w1.x - v1.y * w1.y, v1.x * w1.y + v1.y * w1.x);
             const float2 a2 = make_float2(v2.x * w2.x - v2.y * w2.y, v2.x * w2.y + v2.y * w2.x);
             const float2 a3 = make_float2(v3.x * w3.x - v3.y * w3.y, v3.x * w3.y + v3.y * w3.x);
@@ -1358,6 +1360,106 @@ __device__ inline double wave_sum_lane63(double v)
     return v;
 }
 
+// ---- double-precision LDS real FFT (the element spectra, computed once per element) ----
+
+// w_N^i = exp(-2 pi i i / N) from the quarter table T[r] = w_N^r, r < N/4 = 2^lq (global
+// memory): w_N^(q N/4 + r) = (-i)^q w_N^r
+__device__ inline double2 twiddle_d(const double2 *__restrict__ T, int i, int lq)
+{
+    const int q = (i >> lq) & 3;
+    const double2 w = T[i & ((1 << lq) - 1)];
+    return q == 0 ? w : q == 1 ? make_double2(w.y, -w.x) : q == 2 ? make_double2(-w.x, -w.y) : make_double2(-w.y, w.x);
+}
+
+__device__ inline double2 cmul_dd(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// lds_fft in double precision: packs the zero-padded real sequence x (length 2M) as
+// z[n] = x[2n] + i x[2n+1] and runs the length-M Stockham radix-4 FFT (one fused radix-2
+// pass first when log2 M is odd); twiddles from the table in global memory.  Returns the
+// buffer holding Z; ends with a barrier.
+template <int SPT>
+__device__ inline const double2 *lds_fft_d(const double (&x)[SPT], int t0, int nt, int M, double2 *bufA,
+                                           double2 *bufB, const double2 *__restrict__ twd)
+{
+    const int tid = threadIdx.x;
+    const int lM = 31 - __clz(M);
+    double2 *in = bufA, *ob = bufB;
+    int Ns = 1;
+    if (lM & 1) {
+        if (SPT >= 2) {
+#pragma unroll
+            for (int q = 0; q < SPT; q += 2) {
+                const int n = (t0 + q) >> 1;
+                if (n < (M >> 1)) {
+                    const double2 z = make_double2(t0 + q < nt ? x[q] : 0.0, t0 + q + 1 < nt ? x[q + 1] : 0.0);
+                    in[2 * n] = z;
+                    in[2 * n + 1] = z;
+                }
+            }
+        } else {
+            double *fa = reinterpret_cast<double *>(ob);
+            for (int n = tid; n < M; n += kStrainThreads) ob[n] = make_double2(0.0, 0.0);
+            __syncthreads();
+            if (t0 < nt) fa[t0] = x[0];
+            __syncthreads();
+            for (int n = tid; n < (M >> 1); n += kStrainThreads) {
+                const double2 z = ob[n];
+                in[2 * n] = z;
+                in[2 * n + 1] = z;
+            }
+        }
+        Ns = 2;
+    } else {
+        for (int n = tid; n < M; n += kStrainThreads) in[n] = make_double2(0.0, 0.0);
+        __syncthreads();
+        double *fa = reinterpret_cast<double *>(in);
+#pragma unroll
+        for (int q = 0; q < SPT; ++q)
+            if (t0 + q < nt) fa[t0 + q] = x[q];
+    }
+    __syncthreads();
+    const int Mq = M >> 2;
+    for (int lNs = Ns == 1 ? 0 : 1; Ns < M; Ns <<= 2, lNs += 2) {
+        const int lstride = lM - lNs - 2;            // M / (4 Ns)
+#pragma unroll 1
+        for (int j = tid; j < Mq; j += kStrainThreads) {
+            const int kk = j & (Ns - 1);
+            const double2 v0 = in[j], v1 = in[j + Mq], v2 = in[j + 2 * Mq], v3 = in[j + 3 * Mq];
+            // w_M^t = w_N^(2t), N = 2M; the index lies in the table's quadrant
+            const int ti = kk << (lstride + 1);
+            const double2 w1 = twd[ti], w2 = cmul_dd(w1, w1), w3 = cmul_dd(w1, w2);
+            const double2 a1 = cmul_dd(v1, w1), a2 = cmul_dd(v2, w2), a3 = cmul_dd(v3, w3);
+            const double2 s02 = make_double2(v0.x + a2.x, v0.y + a2.y), d02 = make_double2(v0.x - a2.x, v0.y - a2.y);
+            const double2 s13 = make_double2(a1.x + a3.x, a1.y + a3.y), d13 = make_double2(a1.x - a3.x, a1.y - a3.y);
+            const int o = ((j >> lNs) << (lNs + 2)) + kk;
+            ob[o] = make_double2(s02.x + s13.x, s02.y + s13.y);
+            ob[o + Ns] = make_double2(d02.x + d13.y, d02.y - d13.x);
+            ob[o + 2 * Ns] = make_double2(s02.x - s13.x, s02.y - s13.y);
+            ob[o + 3 * Ns] = make_double2(d02.x - d13.y, d02.y + d13.x);
+        }
+        __syncthreads();
+        double2 *tmp = in; in = ob; ob = tmp;
+    }
+    return in;
+}
+
+// real-FFT post-processing in double precision, times dt, rounded to single precision
+__device__ inline float2 rfft_bin_d(const double2 *Z, int k, int M, const double2 *__restrict__ twd, double dt)
+{
+    const double2 zk = Z[k == M ? 0 : k];
+    const double2 zm = Z[(M - k) & (M - 1)];
+    const double2 zc = make_double2(zm.x, -zm.y);
+    const double2 E = make_double2(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
+    const double2 dd = make_double2(zk.x - zc.x, zk.y - zc.y);
+    const double2 O = make_double2(0.5 * dd.y, -0.5 * dd.x);
+    const double2 w = twiddle_d(twd, k, 30 - __clz(M));     // w_N^k, log2(N/4) = log2(M) - 1
+    const double2 X = make_double2(E.x + w.x * O.x - w.y * O.y, E.y + w.x * O.y + w.y * O.x);
+    return make_float2((float)(X.x * dt), (float)(X.y * dt));
+}
+
 // ------------------------------------------------------------------------------------
 // k_strain_traces: one workgroup per point; earth-frame strain (or trace) time series
 // [point][nd][nt] of one frame (load_fw_points / load_bw_points, readfields.f90:689-1063).
@@ -1392,7 +1494,7 @@ __global__ __launch_bounds__(kStrainThreads) void k_strain_traces(
 // Fused gather + strain + interpolation + azimuthal factors (sem_derivatives.f90,
 // spectral_basis.f90:181-314, readfields.f90:1138-1465), cumsum (simple_routines.f90:
 // 314-341), taper + zero pad (fft.f90:546-601) and LDS real FFT (fft.f90:375-413).
-// F: [point][ND][nomega] spectra of the carried source-frame components.
+// F: [point][ND][fstride] spectra of the carried source-frame components (bins < nomega).
 // ------------------------------------------------------------------------------------
 template <int SPT, int ND>
 __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && ND <= 4) ? 3 : (SPT <= 8 ? 2 : 1)) void k_fwd_spectra(
@@ -1404,7 +1506,7 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && ND <= 4) ? 3 : (SPT <=
     float2 *bufA = reinterpret_cast<float2 *>(smem_raw + pc.op_bytes_f);
     float2 *bufB = bufA + pc.M;
     float2 *Tq = bufA + 2 * pc.M;
-    __shared__ float wave_tot[kStrainThreads / 64];
+    __shared__ double wave_tot[kStrainThreads / 64];
     __shared__ float edge_x[kStrainThreads + 1];
 
     // nlive (device, nullable): the sorted items before the first one keyed past the last
@@ -1439,8 +1541,9 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && ND <= 4) ? 3 : (SPT <=
 #pragma unroll
         for (int q = 0; q < SPT; ++q) x[q] *= tap[q];
         const float2 *Z = lds_fft(x, t0, nt, M, bufA, bufB, Tq);
-        float2 *dst = F + ((size_t)p * ND + c) * pc.nomega;
-        for (int k = tid; k <= M; k += kStrainThreads) dst[k] = rfft_bin(Z, k, M, Tq, fdt);
+        float2 *dst = F + ((size_t)p * ND + c) * pc.fstride;
+        for (int k = tid; k < pc.fstride; k += kStrainThreads)
+            dst[k] = k <= M ? rfft_bin(Z, k, M, Tq, fdt) : make_float2(0.0f, 0.0f);
         __syncthreads();
     }
 }
@@ -1540,7 +1643,7 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && NDB <= 4) ? 3 : (SPT <
     // forward spectra (HBM) and the backward spectra (registers) into LDS (over the FFT
     // buffers); thread tid owns k = tid + 256 j, k = M goes to thread M % 256.
     float2 *bkL = bufA;                                  // [pc.nbk_lds][M + 1]
-    const float2 *Fp = F + (size_t)p * NDF * nw;
+    const float2 *Fp = F + (size_t)p * NDF * pc.fstride;
     const int need = pc.need_bk;
     const int kM_owner = M & (kStrainThreads - 1);
     constexpr int NJ = SPT + 1;                          // bins per thread (the last: k = M)
@@ -1556,7 +1659,7 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && NDB <= 4) ? 3 : (SPT <
             const bool ok = j < SPT ? (k < M) : (j == SPT && tid == kM_owner);
 #pragma unroll
             for (int c = 0; c < NDF; ++c)
-                fv[jj][c] = (j < NJ && ok) ? Fp[(size_t)c * nw + k] : make_float2(0.f, 0.f);
+                fv[jj][c] = (j < NJ && ok) ? Fp[(size_t)c * pc.fstride + k] : make_float2(0.f, 0.f);
         }
 #pragma unroll
         for (int jj = 0; jj < JB; ++jj) {

@@ -22,87 +22,114 @@ namespace mck {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-// run boundaries of element-sorted pair keys (element * R + receiver):
-// flags[i] = 1 where a new element starts
-__global__ void k_run_flags(int n, const int *keys, int R, int *flags)
+// Work tables of element- and receiver-sorted pair keys (element * R + receiver):
+// element runs (the elements whose spectra a launch needs) and groups of at most 16
+// consecutive pairs with one key (one wave of k_bwd_mfma each).
+// flags: eflag[i] = a new element starts at i, kflag[i] = a new key starts at i
+__global__ void k_run_flags(int n, const int *keys, int R, int *eflag, int *kflag)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    flags[i] = (i == 0 || keys[i] / R != keys[i - 1] / R) ? 1 : 0;
+    eflag[i] = (i == 0 || keys[i] / R != keys[i - 1] / R) ? 1 : 0;
+    kflag[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
 }
 
-// from the inclusive scan of the flags: run_first[u] (first item of run u, with
-// run_first[nrun] = n), run_elem[u] and *nrun
-__global__ void k_run_table(int n, const int *keys, int R, const int *flags, const int *runpos, int *run_first,
-                            int *run_elem, int *nrun)
+// from the inclusive scans of the flags (epos, kpos): element runs erun_first[u] (with
+// erun_first[nerun] = n), erun_elem[u], *nerun; first item of every key run krun_first[v];
+// gflag[i] = a group starts at i (every 16th item of a key run)
+__global__ void k_run_table(int n, const int *keys, int R, const int *eflag, const int *epos, const int *kflag,
+                            const int *kpos, int *erun_first, int *erun_elem, int *nerun, int *krun_first)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (flags[i]) {
-        const int u = runpos[i] - 1;
-        run_first[u] = i;
-        run_elem[u] = keys[i] / R;
+    if (eflag[i]) {
+        const int u = epos[i] - 1;
+        erun_first[u] = i;
+        erun_elem[u] = keys[i] / R;
+    }
+    if (kflag[i]) krun_first[kpos[i] - 1] = i;
+    if (i == n - 1) {
+        *nerun = epos[i];
+        erun_first[epos[i]] = n;
+    }
+}
+
+__global__ void k_group_flags(int n, const int *kpos, const int *krun_first, int *gflag)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    gflag[i] = ((i - krun_first[kpos[i] - 1]) & 15) == 0 ? 1 : 0;
+}
+
+// groups from the inclusive scan of gflag: group_first[g] (with group_first[ngroups] = n),
+// the element run of the group, *ngroups
+__global__ void k_group_table(int n, const int *gflag, const int *gpos, const int *epos, int *group_first,
+                              int *group_erun, int *ngroups)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (gflag[i]) {
+        const int g = gpos[i] - 1;
+        group_first[g] = i;
+        group_erun[g] = epos[i] - 1;
     }
     if (i == n - 1) {
-        *nrun = runpos[i];
-        run_first[runpos[i]] = n;
+        *ngroups = gpos[i];
+        group_first[gpos[i]] = n;
     }
 }
 
 // ------------------------------------------------------------------------------------
 // k_elem_spectra: one workgroup per element run u0 + blockIdx.x; the tapered real
-// spectra (x dt) of the element's J = nvar_bwd x 25 nodal traces,
-// U[u - u0][j][k], k <= M, rows ustride apart.
+// spectra (x dt) of the element's J = nvar_bwd x 25 nodal traces, U[u - u0][j][k],
+// k <= M, rows ustride apart (zero beyond M).  Computed in double precision and rounded
+// once: the spectra of an element serve all of its pairs.
 // ------------------------------------------------------------------------------------
 template <int SPT>
 __global__ __launch_bounds__(kStrainThreads) void k_elem_spectra(DevDb db, PathConf pc, int u0, int nrun,
-                                                                 const int *run_elem, const float2 *twq,
+                                                                 const int *run_elem, const double2 *twd,
                                                                  float2 *U)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    float2 *bufA = reinterpret_cast<float2 *>(smem_raw);
-    float2 *bufB = bufA + pc.M;
-    float2 *Tq = bufA + 2 * pc.M;
+    double2 *bufA = reinterpret_cast<double2 *>(smem_raw);
+    double2 *bufB = bufA + pc.M;
     const int u = u0 + (int)blockIdx.x;
     if (u >= nrun) return;
     const int e = run_elem[u];
     if (e < db.u_first || e >= db.u_last) return;
-    load_twiddles(twq, pc.M, Tq);
-    __syncthreads();
     const int tid = threadIdx.x, nt = pc.nt, M = pc.M, t0 = tid * SPT;
     const size_t blk = (size_t)25 * nt;
     const float *ue = db.u + (size_t)(e - db.u_first) * db.nvar * blk;
     float tap[SPT];
     taper_factors(t0, nt, pc, tap);
     const int J = pc.nvar_bwd * 25;
-    const float fdt = (float)pc.dt;
     const bool full_vec = (t0 + SPT <= nt) && ((nt & 3) == 0) && (SPT % 4) == 0;
     float2 *Uu = U + (size_t)(u - u0) * J * pc.ustride;
     for (int j = 0; j < J; ++j) {
         const float *src = ue + (size_t)pc.bwd_vars[j / 25] * blk + (size_t)(j % 25) * nt + t0;
-        float x[SPT];
+        double x[SPT];
         if (full_vec) {
 #pragma unroll
             for (int q = 0; q < SPT; q += 4) {
                 const float4 v = *reinterpret_cast<const float4 *>(src + q);
-                x[q] = v.x * tap[q]; x[q + 1] = v.y * tap[q + 1]; x[q + 2] = v.z * tap[q + 2]; x[q + 3] = v.w * tap[q + 3];
+                x[q] = (double)v.x * tap[q]; x[q + 1] = (double)v.y * tap[q + 1];
+                x[q + 2] = (double)v.z * tap[q + 2]; x[q + 3] = (double)v.w * tap[q + 3];
             }
         } else {
 #pragma unroll
-            for (int q = 0; q < SPT; ++q) x[q] = (t0 + q < nt) ? src[q] * tap[q] : 0.0f;
+            for (int q = 0; q < SPT; ++q) x[q] = (t0 + q < nt) ? (double)src[q] * tap[q] : 0.0;
         }
-        const float2 *Z = lds_fft(x, t0, nt, M, bufA, bufB, Tq);
+        const double2 *Z = lds_fft_d(x, t0, nt, M, bufA, bufB, twd);
         float2 *dst = Uu + (size_t)j * pc.ustride;
         for (int k = tid; k < pc.ustride; k += kStrainThreads)
-            dst[k] = k <= M ? rfft_bin(Z, k, M, Tq, fdt) : make_float2(0.0f, 0.0f);
+            dst[k] = k <= M ? rfft_bin_d(Z, k, M, twd, pc.dt) : make_float2(0.0f, 0.0f);
         __syncthreads();
     }
 }
 
-constexpr int kMfmaWaves = 4;          // waves of a k_bwd_mfma workgroup
 constexpr int kMfmaPairs = 16;         // pairs of one wave (the matrix-core columns)
 
-// LDS bytes of one k_bwd_mfma wave: the B operands [KS][NDB][64] of its 16 pairs and the
+// LDS bytes of k_bwd_mfma (one wave): the B operands [KS][NDB][64] of its 16 pairs and the
 // FP64 slot sums [nslot][16]
 __host__ __device__ constexpr size_t mfma_wave_lds(int ks, int ndb, int nslot)
 {
@@ -110,220 +137,233 @@ __host__ __device__ constexpr size_t mfma_wave_lds(int ks, int ndb, int nslot)
 }
 
 // ------------------------------------------------------------------------------------
-// k_bwd_mfma: one workgroup per element run u0 + blockIdx.x; each wave takes groups of
-// 16 of the run's pairs.  Per 16-bin tile, the matrix cores give every pair's NDB carried
-// backward components at 16 bins (real and imaginary parts as two products); lane l
-// then holds pair l % 16 at bins 4 (l / 16) .. +3 and forms the base kernels with the
-// forward spectra of the pair's point and the slot sums Re(bk H) (FP64, four lanes of a
-// pair reduced per tile).  nslot: the largest slot count of a receiver.  Only the lambda
-// and mu base kernels (and mu - 2 lambda) are formed: the host sends kernel sets with
-// anisotropic parameters to k_bwd_eval.
+// k_bwd_mfma: one wave per group of at most 16 pairs of one (element, receiver), groups of
+// element runs u0 .. u0 + ubatch - 1 (the others are left to another launch).  Per 16-bin
+// tile the matrix cores give every pair's NDB carried backward components at 16 bins
+// (real and imaginary parts as two products); lane l then holds pair l % 16 at bins
+// 4 (l / 16) .. +3 and forms the base kernels with the forward spectra of the pair's
+// point, and the slot sums Re(bk H) of the receiver's window spectra, each product rounded
+// once in fp32 and accumulated in FP64 (the four lanes of a pair reduced per tile in a
+// fixed order).  Only the lambda and mu base kernels (and mu - 2 lambda) are formed: the
+// host sends kernel sets with anisotropic parameters to k_bwd_eval.  Hp: window spectra,
+// rows fstride apart, zero past nomega; F: forward spectra, rows fstride apart, zero past
+// nomega.  xcd: groups in XCD-contiguous ranges (the groups of an element on one XCD).
 //  phys: [point][nk] values of the receiver's kernels; base_out (optional): [point][nk][6]
 // ------------------------------------------------------------------------------------
 template <int NDF, int NDB, int NVB>
-__global__ __launch_bounds__(64 * kMfmaWaves) void k_bwd_mfma(
-    PathConf pc, int u0, int nrun, const int *run_first, const float2 *U, const int *order, int R,
-    const float *Wb, const float *Qb, const float *Qf, const float2 *F, const RecDesc *recs,
-    const KernDesc *kern, const float2 *H, int nslot, const int *model_idx, const float *model, int ndepth,
-    int relative, int nk, double *phys, double *base_out, const double *xyz, const uint8_t *active, int P,
-    const double *rec_xyz)
+__global__ __launch_bounds__(64, (NDB <= 4 && NDF <= 4) ? 2 : 1) void k_bwd_mfma(
+    PathConf pc, int u0, int ubatch, int ngroups, const int *group_first, const int *group_erun,
+    const float2 *U, const int *order, int R, const float *Wb, const float *Qb, const float *Qf,
+    const float2 *F, const RecDesc *recs, const KernDesc *kern, const float2 *Hp, int nslot,
+    const int *model_idx, const float *model, int ndepth, int relative, int nk, double *phys, double *base_out,
+    const double *xyz, const uint8_t *active, int P, const double *rec_xyz, int xcd)
 {
     constexpr int J = NVB * 25, KS = (J + 3) / 4;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int u = u0 + (int)blockIdx.x;
-    if (u >= nrun) return;
-    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int jp = l & 15, bq = l >> 4;
-    float *sW = reinterpret_cast<float *>(smem_raw + (size_t)wave * mfma_wave_lds(KS, NDB, nslot));
+    const int grp = xcd ? xcd_block_n(ngroups) : ((int)blockIdx.x < ngroups ? (int)blockIdx.x : -1);
+    if (grp < 0) return;
+    const int eu = group_erun[grp];
+    if (eu < u0 || eu >= u0 + ubatch) return;
+    const int l = threadIdx.x, jp = l & 15, bq = l >> 4;
+    float *sW = reinterpret_cast<float *>(smem_raw);
     double *sAcc = reinterpret_cast<double *>(sW + KS * NDB * 64);
-    const int pfirst = run_first[u], plast = run_first[u + 1];
-    const float2 *Ue = U + (size_t)(u - u0) * J * pc.ustride;
-    const int nomega = pc.nomega, ntile = (nomega + 15) >> 4;
-    const int ngroups = (plast - pfirst + kMfmaPairs - 1) / kMfmaPairs;
-    const int iters = (ngroups + kMfmaWaves - 1) / kMfmaWaves;
-    for (int it = 0; it < iters; ++it) {
-        const int grp = it * kMfmaWaves + wave;
-        const bool gact = grp < ngroups;
-        const int g0 = pfirst + grp * kMfmaPairs;
-        const int n = gact ? min(kMfmaPairs, plast - g0) : 0;
-        // B operands of the wave's pairs: sW[(s NDB + c) 64 + l'] = W of pair l' % 16,
-        // component c, trace 4 s + l' / 16 (zero past the run or the traces)
-        for (int idx = l; idx < KS * NDB * 64; idx += 64) {
-            const int s = idx / (NDB * 64), rem = idx - s * (NDB * 64), c = rem >> 6, ll = rem & 63;
-            const int pj = ll & 15, j = 4 * s + (ll >> 4);
-            float w = 0.0f;
-            if (pj < n && j < J) w = Wb[(size_t)(g0 + pj) * NDB * J + c * J + j];
-            sW[idx] = w;
+    const int g0 = group_first[grp], n = group_first[grp + 1] - g0;
+    const float2 *Ue = U + (size_t)(eu - u0) * J * pc.ustride;
+    const int nomega = pc.nomega, fs = pc.fstride, ntile = (nomega + 15) >> 4;
+    // B operands of the group's pairs: sW[(s NDB + c) 64 + l'] = W of pair l' % 16,
+    // component c, trace 4 s + l' / 16 (zero past the group or the traces)
+    for (int idx = l; idx < KS * NDB * 64; idx += 64) {
+        const int s = idx / (NDB * 64), rem = idx - s * (NDB * 64), c = rem >> 6, ll = rem & 63;
+        const int pj = ll & 15, j = 4 * s + (ll >> 4);
+        float w = 0.0f;
+        if (pj < n && j < J) w = Wb[(size_t)(g0 + pj) * NDB * J + c * J + j];
+        sW[idx] = w;
+    }
+    for (int s = l; s < nslot * kMfmaPairs; s += 64) sAcc[s] = 0.0;
+    __syncthreads();
+    const bool valid = jp < n;
+    // the group's pairs share the receiver
+    const int r = order[g0] % R;
+    int p = 0;
+    if (valid) p = order[g0 + jp] / R;
+    // per-pair coefficients: F.B = f^T Amu b, traces uf.f and ub.b (earth frame through
+    // the rotations of the point (Qf) and of the pair (Qb))
+    float uf[NDF], ub[NDB], Amu[NDF][NDB];
+    if (NDF > 1) {
+        float qf[6][NDF], qb[6][NDB];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+#pragma unroll
+            for (int m = 0; m < NDF; ++m) qf[i][m] = valid ? Qf[(size_t)p * 36 + i * 6 + m] : 0.0f;
+#pragma unroll
+            for (int c = 0; c < NDB; ++c) qb[i][c] = valid ? Qb[(size_t)(g0 + jp) * 36 + i * 6 + c] : 0.0f;
         }
-        if (bq == 0)
-            for (int s = 0; s < nslot; ++s) sAcc[s * kMfmaPairs + jp] = 0.0;
-        __syncthreads();
-        const bool valid = jp < n;
-        int p = 0, r = 0;
-        if (valid) {
-            const int q = order[g0 + jp];
-            p = q / R;
-            r = q - p * R;
-        }
-        // per-pair coefficients: F.B = f^T Amu b, traces uf.f and ub.b (earth frame through
-        // the rotations of the point (Qf) and of the pair (Qb))
-        float uf[NDF], ub[NDB], Amu[NDF][NDB];
-        if (NDF > 1) {
-            float qf[6][NDF], qb[6][NDB];
 #pragma unroll
-            for (int i = 0; i < 6; ++i) {
-#pragma unroll
-                for (int m = 0; m < NDF; ++m) qf[i][m] = valid ? Qf[(size_t)p * 36 + i * 6 + m] : 0.0f;
-#pragma unroll
-                for (int c = 0; c < NDB; ++c) qb[i][c] = valid ? Qb[(size_t)(g0 + jp) * 36 + i * 6 + c] : 0.0f;
-            }
-#pragma unroll
-            for (int m = 0; m < NDF; ++m) {
-                uf[m] = qf[0][m] + qf[1][m] + qf[2][m];
-#pragma unroll
-                for (int c = 0; c < NDB; ++c) {
-                    float a = 0.0f;
-#pragma unroll
-                    for (int i = 0; i < 6; ++i) a += qf[i][m] * qb[i][c];
-                    Amu[m][c] = a;
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < NDB; ++c) ub[c] = qb[0][c] + qb[1][c] + qb[2][c];
-        } else {
-            uf[0] = 1.0f; ub[0] = 1.0f; Amu[0][0] = 1.0f;
-        }
-        const RecDesc &rd = recs[r];
-        const float2 *Fp = F + (size_t)p * NDF * nomega;
-        for (int t = 0; t < (gact ? ntile : 0); ++t) {
-            const int b0 = t << 4;
-            v4f accR[NDB], accI[NDB];
+        for (int m = 0; m < NDF; ++m) {
+            uf[m] = qf[0][m] + qf[1][m] + qf[2][m];
 #pragma unroll
             for (int c = 0; c < NDB; ++c) {
-                accR[c] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-                accI[c] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+                float a = 0.0f;
+#pragma unroll
+                for (int i = 0; i < 6; ++i) a += qf[i][m] * qb[i][c];
+                Amu[m][c] = a;
             }
-            // A operands: lane l gives trace 4 s + l / 16 at bin b0 + l % 16 (rows past
-            // nomega are never read back; traces past J are zero)
-            float2 a[KS];
+        }
 #pragma unroll
-            for (int s = 0; s < KS; ++s) {
-                const int j = 4 * s + bq;
-                a[s] = (j < J) ? Ue[(size_t)j * pc.ustride + b0 + jp] : make_float2(0.0f, 0.0f);
+        for (int c = 0; c < NDB; ++c) ub[c] = qb[0][c] + qb[1][c] + qb[2][c];
+    } else {
+        uf[0] = 1.0f; ub[0] = 1.0f; Amu[0][0] = 1.0f;
+    }
+    const RecDesc &rd = recs[r];
+    const int ns = rd.nslot;
+    const float2 *Fp = F + (size_t)p * NDF * fs;
+    for (int t = 0; t < ntile; ++t) {
+        const int b0 = t << 4, kb = b0 + 4 * bq;
+        // loads of the tile: A operands (lane l gives trace 4 s + l / 16 at bin b0 + l % 16;
+        // bins past nomega hold zeros, traces past J are zero) and the forward spectra of
+        // this lane's pair at its bins kb .. kb + 3
+        float2 a[KS];
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const int j = 4 * s + bq;
+            a[s] = (j < J) ? Ue[(size_t)j * pc.ustride + b0 + jp] : make_float2(0.0f, 0.0f);
+        }
+        float4 f4[NDF][2];
+#pragma unroll
+        for (int m = 0; m < NDF; ++m) {
+            const float4 *src = reinterpret_cast<const float4 *>(Fp + (size_t)m * fs + kb);
+            f4[m][0] = valid ? src[0] : make_float4(0.f, 0.f, 0.f, 0.f);
+            f4[m][1] = valid ? src[1] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        v4f accR[NDB], accI[NDB];
+#pragma unroll
+        for (int c = 0; c < NDB; ++c) {
+            accR[c] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+            accI[c] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+#pragma unroll
+            for (int c = 0; c < NDB; ++c) {
+                const float w = sW[(s * NDB + c) * 64 + l];
+                accR[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s].x, w, accR[c], 0, 0, 0);
+                accI[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s].y, w, accI[c], 0, 0, 0);
             }
+        }
+        // base kernels of this lane's pair at bins kb + v
+        float2 bk0[4], bk1[4];
 #pragma unroll
-            for (int s = 0; s < KS; ++s) {
-#pragma unroll
-                for (int c = 0; c < NDB; ++c) {
-                    const float w = sW[(s * NDB + c) * 64 + l];
-                    accR[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s].x, w, accR[c], 0, 0, 0);
-                    accI[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s].y, w, accI[c], 0, 0, 0);
-                }
-            }
-            // base kernels of this lane's pair at bins b0 + 4 bq + v
-            float2 bk0[4], bk1[4];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
+        for (int v = 0; v < 4; ++v) {
 #pragma clang fp contract(fast)
-                const int k = b0 + 4 * bq + v;
-                bk0[v] = make_float2(0.0f, 0.0f);
-                bk1[v] = bk0[v];
-                if (!valid || k >= nomega) continue;
-                float2 f[NDF], b[NDB];
+            float2 f[NDF], b[NDB];
 #pragma unroll
-                for (int m = 0; m < NDF; ++m) f[m] = Fp[(size_t)m * nomega + k];
+            for (int m = 0; m < NDF; ++m)
+                f[m] = (v & 1) ? make_float2(f4[m][v >> 1].z, f4[m][v >> 1].w) : make_float2(f4[m][v >> 1].x, f4[m][v >> 1].y);
 #pragma unroll
-                for (int c = 0; c < NDB; ++c) b[c] = make_float2(accR[c][v], accI[c][v]);
-                if (NDF == 1) {
-                    const float2 pr = cmulf(f[0], b[0]);
-                    bk0[v] = make_float2(-pr.x, -pr.y);
-                    bk1[v] = make_float2(-2.0f * pr.x, -2.0f * pr.y);
-                } else {
-                    float2 trf = make_float2(0.f, 0.f), trb = trf;
+            for (int c = 0; c < NDB; ++c) b[c] = make_float2(accR[c][v], accI[c][v]);
+            if (NDF == 1) {
+                const float2 pr = cmulf(f[0], b[0]);
+                bk0[v] = make_float2(-pr.x, -pr.y);
+                bk1[v] = make_float2(-2.0f * pr.x, -2.0f * pr.y);
+            } else {
+                float2 trf = make_float2(0.f, 0.f), trb = trf;
 #pragma unroll
-                    for (int m = 0; m < NDF; ++m) { trf.x += uf[m] * f[m].x; trf.y += uf[m] * f[m].y; }
+                for (int m = 0; m < NDF; ++m) { trf.x += uf[m] * f[m].x; trf.y += uf[m] * f[m].y; }
 #pragma unroll
-                    for (int c = 0; c < NDB; ++c) { trb.x += ub[c] * b[c].x; trb.y += ub[c] * b[c].y; }
-                    const float2 lam = cmulf(trf, trb);
-                    bk0[v] = make_float2(-lam.x, -lam.y);
-                    if (pc.need_bk & 2) {
-                        float2 mu = make_float2(0.f, 0.f);
+                for (int c = 0; c < NDB; ++c) { trb.x += ub[c] * b[c].x; trb.y += ub[c] * b[c].y; }
+                const float2 lam = cmulf(trf, trb);
+                bk0[v] = make_float2(-lam.x, -lam.y);
+                bk1[v] = make_float2(0.f, 0.f);
+                if (pc.need_bk & 2) {
+                    float2 mu = make_float2(0.f, 0.f);
 #pragma unroll
-                        for (int m = 0; m < NDF; ++m) {
-                            float2 gm = make_float2(0.f, 0.f);
+                    for (int m = 0; m < NDF; ++m) {
+                        float2 gm = make_float2(0.f, 0.f);
 #pragma unroll
-                            for (int c = 0; c < NDB; ++c) { gm.x += Amu[m][c] * b[c].x; gm.y += Amu[m][c] * b[c].y; }
-                            const float2 tt = cmulf(f[m], gm);
-                            mu.x += tt.x; mu.y += tt.y;
-                        }
-                        bk1[v] = make_float2(-2.0f * mu.x, -2.0f * mu.y);
+                        for (int c = 0; c < NDB; ++c) { gm.x += Amu[m][c] * b[c].x; gm.y += Amu[m][c] * b[c].y; }
+                        const float2 tt = cmulf(f[m], gm);
+                        mu.x += tt.x; mu.y += tt.y;
                     }
+                    bk1[v] = make_float2(-2.0f * mu.x, -2.0f * mu.y);
                 }
             }
-            // slot sums: sum_k Re(bk H) with each product rounded once in fp32 and added in
-            // FP64; the four lanes of a pair are reduced in a fixed order
-            for (int s = 0; s < nslot; ++s) {
-                const bool has = valid && s < rd.nslot;
-                const int code = has ? rd.slot_bk[s] : 0;
-                const float2 *Hh = H + (size_t)(has ? rd.slot_h[s] : 0) * nomega;
-                double part = 0.0;
+        }
+        // slot sums, four slots at a time (their window-spectrum loads in flight together),
+        // the four lanes of a pair reduced in a fixed order
+        for (int s0 = 0; s0 < ns; s0 += 4) {
+            float4 h4[4][2];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int s = s0 + q < ns ? s0 + q : s0;
+                const float4 *src = reinterpret_cast<const float4 *>(Hp + (size_t)rd.slot_h[s] * fs + kb);
+                h4[q][0] = src[0];
+                h4[q][1] = src[1];
+            }
+            double part[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int code = rd.slot_bk[s0 + q < ns ? s0 + q : s0];
+                part[q] = 0.0;
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    const int k = b0 + 4 * bq + v;
-                    if (!has || k >= nomega) continue;
                     const float2 bk = code == 0 ? bk0[v]
                                     : code == 1 ? bk1[v]
                                                 : make_float2(bk1[v].x - 2.0f * bk0[v].x, bk1[v].y - 2.0f * bk0[v].y);
-                    const float2 h = Hh[k];
-                    part += (double)__builtin_fmaf(bk.x, h.x, -(bk.y * h.y));
-                }
-                part += __shfl_xor(part, 16, 64);
-                part += __shfl_xor(part, 32, 64);
-                if (bq == 0 && valid) sAcc[s * kMfmaPairs + jp] += part;
-            }
-        }
-        // physical kernels (kernel.f90:665-832) with the model at the point, after the
-        // damping of every receiver processed in this step (readfields.f90:3184-3204)
-        if (bq == 0 && valid) {
-            const int mi = model_idx[p];
-            const float vp = model[0 * ndepth + mi], vs = model[1 * ndepth + mi], rho_f = model[2 * ndepth + mi];
-            const double rho = rho_f;
-            double damp = 1.0;
-            if (pc.damp_radius > 0.0 && active) {
-                const double x0 = xyz[3 * (size_t)p], x1 = xyz[3 * (size_t)p + 1], x2 = xyz[3 * (size_t)p + 2];
-                const double ds = sqrt((x0 - pc.src_r[0]) * (x0 - pc.src_r[0]) + (x1 - pc.src_r[1]) * (x1 - pc.src_r[1])
-                                       + (x2 - pc.src_r[2]) * (x2 - pc.src_r[2]));
-                const double fs = ds < pc.damp_radius ? ds / pc.damp_radius : 1.0;
-                for (int r2 = r; r2 < R; ++r2) {
-                    if (!active[(size_t)(p / P) * R + r2]) continue;
-                    const double *c = rec_xyz + 3 * r2;
-                    const double dr = sqrt((x0 - c[0]) * (x0 - c[0]) + (x1 - c[1]) * (x1 - c[1]) + (x2 - c[2]) * (x2 - c[2]));
-                    damp = damp * (dr < pc.damp_radius ? dr / pc.damp_radius : 1.0) * fs;
+                    const float4 hh = h4[q][v >> 1];
+                    const float2 h = (v & 1) ? make_float2(hh.z, hh.w) : make_float2(hh.x, hh.y);
+                    part[q] += (double)__builtin_fmaf(bk.x, h.x, -(bk.y * h.y));
                 }
             }
-            for (int kk = 0; kk < rd.nkernel; ++kk) {
-                const int k = rd.first_kernel + kk;
-                const KernDesc &kd = kern[k];
-                double b[kNumBk];
 #pragma unroll
-                for (int qq = 0; qq < kNumBk; ++qq)
-                    b[qq] = kd.slot[qq] >= 0 ? sAcc[kd.slot[qq] * kMfmaPairs + jp] * damp : 0.0;
-                if (base_out) {
-                    for (int qq = 0; qq < 6; ++qq) base_out[((size_t)p * nk + k) * 6 + qq] = b[qq];
-                    continue;
-                }
-                double v = 0.0;
-                switch (kd.param) {
-                case 0: v = relative ? b[0] * (double)(rho_f * (vp * vp - 2 * (vs * vs))) : b[0]; break;
-                case 3: v = relative ? b[1] * (double)(rho_f * (vs * vs)) : b[1]; break;
-                case 1: v = relative ? 2.0 * rho * (double)(vp * vp) * b[0] : 2.0 * rho * (double)vp * b[0]; break;
-                case 2: v = (relative ? 2.0 * rho * (double)(vs * vs) : 2.0 * rho * (double)vs) * b[6]; break;
-                default: v = 0.0;
-                }
-                phys[(size_t)p * nk + k] = v;
+            for (int q = 0; q < 4; ++q) part[q] += __shfl_xor(part[q], 16, 64);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) part[q] += __shfl_xor(part[q], 32, 64);
+            if (bq == 0 && valid)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (s0 + q < ns) sAcc[(s0 + q) * kMfmaPairs + jp] += part[q];
+        }
+    }
+    // physical kernels (kernel.f90:665-832) with the model at the point, after the damping
+    // of every receiver processed in this step (readfields.f90:3184-3204)
+    if (bq == 0 && valid) {
+        const int mi = model_idx[p];
+        const float vp = model[0 * ndepth + mi], vs = model[1 * ndepth + mi], rho_f = model[2 * ndepth + mi];
+        const double rho = rho_f;
+        double damp = 1.0;
+        if (pc.damp_radius > 0.0 && active) {
+            const double x0 = xyz[3 * (size_t)p], x1 = xyz[3 * (size_t)p + 1], x2 = xyz[3 * (size_t)p + 2];
+            const double ds = sqrt((x0 - pc.src_r[0]) * (x0 - pc.src_r[0]) + (x1 - pc.src_r[1]) * (x1 - pc.src_r[1])
+                                   + (x2 - pc.src_r[2]) * (x2 - pc.src_r[2]));
+            const double fsrc = ds < pc.damp_radius ? ds / pc.damp_radius : 1.0;
+            for (int r2 = r; r2 < R; ++r2) {
+                if (!active[(size_t)(p / P) * R + r2]) continue;
+                const double *c = rec_xyz + 3 * r2;
+                const double dr = sqrt((x0 - c[0]) * (x0 - c[0]) + (x1 - c[1]) * (x1 - c[1]) + (x2 - c[2]) * (x2 - c[2]));
+                damp = damp * (dr < pc.damp_radius ? dr / pc.damp_radius : 1.0) * fsrc;
             }
         }
-        __syncthreads();
+        for (int kk = 0; kk < rd.nkernel; ++kk) {
+            const int k = rd.first_kernel + kk;
+            const KernDesc &kd = kern[k];
+            double b[kNumBk];
+#pragma unroll
+            for (int qq = 0; qq < kNumBk; ++qq)
+                b[qq] = kd.slot[qq] >= 0 ? sAcc[kd.slot[qq] * kMfmaPairs + jp] * damp : 0.0;
+            if (base_out) {
+                for (int qq = 0; qq < 6; ++qq) base_out[((size_t)p * nk + k) * 6 + qq] = b[qq];
+                continue;
+            }
+            double v = 0.0;
+            switch (kd.param) {
+            case 0: v = relative ? b[0] * (double)(rho_f * (vp * vp - 2 * (vs * vs))) : b[0]; break;
+            case 3: v = relative ? b[1] * (double)(rho_f * (vs * vs)) : b[1]; break;
+            case 1: v = relative ? 2.0 * rho * (double)(vp * vp) * b[0] : 2.0 * rho * (double)vp * b[0]; break;
+            case 2: v = (relative ? 2.0 * rho * (double)(vs * vs) : 2.0 * rho * (double)vs) * b[6]; break;
+            default: v = 0.0;
+            }
+            phys[(size_t)p * nk + k] = v;
+        }
     }
 }
 

@@ -121,8 +121,9 @@ def test_c1_general_moment_tensor_r():
 
 def test_c2_kernel_set_nt2048():
     """c2's 1,200-kernel set (72 window-spectrum x base-kernel slots per receiver) at its
-    own nt 2048."""
-    p = synth.make_problem("c2", fill=False, nelem_target=1000, ntet=4, npoints_per_step=4, max_iter=2,
+    own nt 2048.  24 voxels: the kernels' L2 norms over the voxels are not left to the two
+    or three voxels that a receiver's travel-time mask keeps."""
+    p = synth.make_problem("c2", fill=False, nelem_target=1000, ntet=24, npoints_per_step=4, max_iter=2,
                            unmasked_frac=1.0)
     assert p.nkernel == 1200 and p.nt == 2048
     _pinned_case(p, 8, 4, 4)

@@ -74,11 +74,12 @@ def test_page_larger_than_database_is_resident():
     assert got["stats"]["paged_bytes"] == 0
 
 
-def test_fixed_page_buffers_are_copied_once():
-    """Unsorted launches (base kernels) visit every page in order.  With 2 buffers every
-    call copies every page of both databases; with 7, pages 0-4 keep their buffers and
-    pages 5-12 rotate through the other two, so a second call copies exactly 5 pages per
-    database less."""
+def test_fixed_page_buffers_are_copied_once(monkeypatch):
+    """Unsorted launches (base kernels on the per-pair backward kernel) visit every page
+    in order.  With 2 buffers every call copies every page of both databases; with 7,
+    pages 0-4 keep their buffers and pages 5-12 rotate through the other two, so a second
+    call copies exactly 5 pages per database less."""
+    monkeypatch.setenv("MCK_BWD_PATH", "0")
     def two_calls(p, k):
         x = p.sample_points(40, seed=9)
         k.base_kernels(x)
