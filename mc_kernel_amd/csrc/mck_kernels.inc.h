@@ -1602,10 +1602,10 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && NDB <= 4) ? 3 : (SPT <
             E.Amu[m][n] = a;
         } else if (tid >= 64 && tid < 64 + NDF) {
             const int m = tid - 64;
-            E.uf[m] = E.Qf[0][m] + E.Qf[1][m] + E.Qf[2][m];
+            E.uf[m] = pc.comps_f[m] < 3 ? 1.0f : 0.0f;     // the trace: rotation invariant
         } else if (tid >= 128 && tid < 128 + NDB) {
             const int n = tid - 128;
-            E.ub[n] = S.Q[0][n] + S.Q[1][n] + S.Q[2][n];
+            E.ub[n] = pc.comps_b[n] < 3 ? 1.0f : 0.0f;
         }
         // (made visible by the barriers of the FFTs below)
     }

@@ -186,8 +186,8 @@ __global__ __launch_bounds__(64, (NDB <= 4 && NDF <= 4) ? 2 : 1) void k_bwd_mfma
     const int r = order[g0] % R;
     int p = 0;
     if (valid) p = order[g0 + jp] / R;
-    // per-pair coefficients: F.B = f^T Amu b, traces uf.f and ub.b (earth frame through
-    // the rotations of the point (Qf) and of the pair (Qb))
+    // per-pair coefficients: F.B = f^T Amu b (earth frame through the rotations of the
+    // point (Qf) and of the pair (Qb)); traces uf.f and ub.b
     float uf[NDF], ub[NDB], Amu[NDF][NDB];
     if (NDF > 1) {
         float qf[6][NDF], qb[6][NDB];
@@ -200,7 +200,9 @@ __global__ __launch_bounds__(64, (NDB <= 4 && NDF <= 4) ? 2 : 1) void k_bwd_mfma
         }
 #pragma unroll
         for (int m = 0; m < NDF; ++m) {
-            uf[m] = qf[0][m] + qf[1][m] + qf[2][m];
+            // the trace is invariant under the rotations: exactly 1 for the diagonal
+            // components (ss, pp, zz), 0 for the others
+            uf[m] = pc.comps_f[m] < 3 ? 1.0f : 0.0f;
 #pragma unroll
             for (int c = 0; c < NDB; ++c) {
                 float a = 0.0f;
@@ -210,7 +212,7 @@ __global__ __launch_bounds__(64, (NDB <= 4 && NDF <= 4) ? 2 : 1) void k_bwd_mfma
             }
         }
 #pragma unroll
-        for (int c = 0; c < NDB; ++c) ub[c] = qb[0][c] + qb[1][c] + qb[2][c];
+        for (int c = 0; c < NDB; ++c) ub[c] = pc.comps_b[c] < 3 ? 1.0f : 0.0f;
     } else {
         uf[0] = 1.0f; ub[0] = 1.0f; Amu[0][0] = 1.0f;
     }
