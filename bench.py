@@ -391,6 +391,25 @@ def bench_integrals(args):
     print(json.dumps(line), flush=True)
 
 
+def task_curve(k, p, sizes, v0, nvox):
+    """evals/s of the same voxels [v0, v0 + nvox) run as tasks of each size (the master
+    hands out tasks of ELEMENTS_PER_TASK voxels, master_queue.f90:223-249); one warm-up
+    task of each size first."""
+    out = []
+    for T in sizes:
+        T = min(T, nvox)
+        k.slave_work(p.verts, p.conn[v0:v0 + T])
+        evals, t0 = 0, time.perf_counter()
+        for a in range(v0, v0 + nvox, T):
+            r = k.slave_work(p.verts, p.conn[a:min(a + T, v0 + nvox)])
+            evals += r["stats"]["evaluations"]
+        dt = time.perf_counter() - t0
+        out.append({"task_voxels": T, "tasks": (nvox + T - 1) // T, "seconds": dt, "evals_per_s": evals / dt})
+        print("task curve: %d-voxel tasks: %.1f M evals/s (%.2f s)" % (T, evals / dt / 1e6, dt),
+              file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -421,6 +440,10 @@ def main():
     ap.add_argument("--integrals", action="store_true",
                     help="time the model and heterogeneity integrals of a task instead (one JSON line)")
     ap.add_argument("--integrals-tets", type=int, default=50000)
+    ap.add_argument("--task-curve", default="",
+                    help="comma-separated task sizes: evals/s of the voxels [--first-voxel, "
+                         "+--curve-voxels) run as tasks of each size (one JSON field)")
+    ap.add_argument("--curve-voxels", type=int, default=10000)
     ap.add_argument("--db-page-elems", type=int, default=0,
                     help="keep the databases in host memory and page them, N elements per "
                          "page (0: resident in HBM, the default workload)")
@@ -514,6 +537,11 @@ def main():
         if busy is not None:
             roof["valu_busy"] = busy
 
+    curve = None
+    if args.task_curve:
+        sizes = [int(x) for x in args.task_curve.split(",") if x]
+        curve = task_curve(k, p, sizes, v0, min(args.curve_voxels, p.ntet - v0))
+
     # plain one-point-at-a-time form on a small sample of the same workload
     plain = None
     if args.plain_voxels > 0:
@@ -553,6 +581,8 @@ def main():
                    "evaluations": int(tot_e), "device_ms": dev_ms},
         "roofline": roof, "cpu_baseline": cpu, "plain_form": plain,
     }
+    if curve is not None:
+        line["task_curve"] = curve
     if sample is not None and host_dbs is None:
         line["self_check"] = self_check(p, sample)
         print("self-check:", json.dumps(line["self_check"]), file=sys.stderr, flush=True)
