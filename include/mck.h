@@ -417,6 +417,22 @@ int mck_results_add(int32_t nelems, const int32_t *elems, const double *kernel_v
                     const double *computation_time, const double *model, const double *hetero,
                     int32_t rank);
 
+/*
+ * mck_results_add without any allocation (no counterpart in the reference): for callers
+ * that cannot allocate device memory - code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc.  Same arguments and result as mck_results_add, with
+ * two limits: kernel_values, kernel_variance, niterations, model and hetero are device
+ * arrays or NULL (the device-resident results of the last mck_slave_work /
+ * mck_integrate_model / mck_integrate_hetero), and a task holds at most 131072 (element,
+ * basis function) pairs (MCK_EINVAL beyond).  Its work arrays live in one fixed 16 MiB
+ * device array per GPU, kept until the process ends and shared by every call on that
+ * GPU: calls that overlap on one GPU race.
+ */
+int mck_results_add_noalloc(int32_t nelems, const int32_t *elems, const double *kernel_values,
+                            const double *kernel_variance, const int32_t *niterations,
+                            const double *computation_time, const double *model, const double *hetero,
+                            int32_t rank);
+
 /* The accumulators themselves (each nullable), FP64 point-major as the device holds them:
  * kernel, variance [npoints][nkernel], model [npoints][nmodel], hetero [npoints][nhet],
  * niterations [nelements][nkernel], computation_time, element_proc [nelements]. */

@@ -99,7 +99,8 @@ def ptr(arr, ctype):
 EXPORTS = [
     "mck_init", "mck_slave_work", "mck_slave_work_plain", "mck_locate_points",
     "mck_load_strain_points", "mck_base_kernels", "mck_seismograms", "mck_integrate_model", "mck_set_hetero", "mck_integrate_hetero", "mck_ad_dims", "mck_ad_transfer",
-    "mck_ad_forward", "mck_ad_backward", "mck_results_init", "mck_results_add", "mck_results_get",
+    "mck_ad_forward", "mck_ad_backward", "mck_results_init", "mck_results_add", "mck_results_add_noalloc",
+    "mck_results_get",
     "mck_results_load", "mck_results_export", "mck_get_stats", "mck_finalize", "mck_last_error",
 ]
 
@@ -143,6 +144,8 @@ def declare(lib):
     # result arrays may be host or device memory: passed as addresses
     lib.mck_results_add.argtypes = [C.c_int32, i32p] + [C.c_void_p] * 6 + [C.c_int32]
     lib.mck_results_add.restype = C.c_int
+    lib.mck_results_add_noalloc.argtypes = [C.c_int32, i32p] + [C.c_void_p] * 6 + [C.c_int32]
+    lib.mck_results_add_noalloc.restype = C.c_int
     lib.mck_results_get.argtypes = [f64p, f64p, f64p, f64p, i32p, f64p, i32p]
     lib.mck_results_get.restype = C.c_int
     lib.mck_results_load.argtypes = [C.c_void_p] * 7

@@ -1382,7 +1382,7 @@ int setup_tables(State &G) {
 }
 
 int ensure_chunk(State &G, int points) {
-    const int R = G.R, nw = G.pc.nomega;
+    const int R = G.R;
     const size_t npairs = (size_t)points * std::max(R, 1);
     HIPCHECK(G.xyz.ensure((size_t)points * 3));
     HIPCHECK(G.model_idx.ensure(points));
@@ -1399,6 +1399,8 @@ int ensure_chunk(State &G, int points) {
     HIPCHECK(G.pair_count.ensure(1));
     return MCK_OK;
 }
+
+int scratch_address(unsigned char **out);
 
 int init_impl(const mck_setup_t *s) {
     if (!s) return fail(MCK_EINVAL, "null setup");
@@ -1538,6 +1540,11 @@ int init_impl(const mck_setup_t *s) {
     if (const char *e = std::getenv("MCK_CHUNK_POINTS")) {
         const long v = std::atol(e);
         if (v > 0) G.chunk_points = (int)std::min<long>(v, G.chunk_points);
+    }
+    // the address of the no-allocation entry point's fixed device array on this device
+    {
+        unsigned char *sp = nullptr;
+        if ((rc = scratch_address(&sp))) return rc;
     }
     G.init = true;
     return MCK_OK;
@@ -2619,9 +2626,60 @@ int mck_results_init(int32_t int_type, int32_t npoints, int32_t nelements, int32
     return MCK_OK;
 }
 
-int mck_results_add(int32_t nelems, const int32_t *elems, const double *kernel_values,
-                    const double *kernel_variance, const int32_t *niterations, const double *computation_time,
-                    const double *model, const double *hetero, int32_t rank) {
+}  // extern "C"
+
+// The no-allocation result assembly (mck_results_add_noalloc): its device arrays live in
+// `scratch`, a fixed array that every GPU the library runs on holds until the process
+// ends.  kNoAllocScratch bytes bound the task: kNoAllocMaxPairs (element, basis function)
+// pairs.  Host code takes the array's address per device with hipGetSymbolAddress (the
+// name itself is a host-side stand-in).
+constexpr size_t kNoAllocScratch = (size_t)16 << 20;
+constexpr int kNoAllocMaxPairs = 131072;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratch];
+
+namespace {
+
+// Device arrays of one result-assembly call: the (point, slot) pairs and their sort, the
+// runs of equal points, the task's element ids, the cub temporary storage and the staging
+// area of host inputs (double) and host iteration counts (int)
+struct AddScratch {
+    int *pt_in = nullptr, *pt_out = nullptr, *sl_in = nullptr, *sl_out = nullptr;
+    int *seg_point = nullptr, *seg_count = nullptr, *seg_start = nullptr, *nseg = nullptr;
+    int *elems = nullptr, *elem_idx = nullptr, *stage_i = nullptr;
+    void *tmp = nullptr;
+    size_t tmp_bytes = 0;
+    double *stage = nullptr;
+    size_t stage_len = 0;
+};
+
+std::map<int, unsigned char *> g_scratch_ptr;   // device -> address of its `scratch`
+
+// the address of `scratch` on the current device, taken once per device (mck_init)
+int scratch_address(unsigned char **out) {
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    auto it = g_scratch_ptr.find(dev);
+    if (it == g_scratch_ptr.end()) {
+        void *p = nullptr;
+        HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        it = g_scratch_ptr.emplace(dev, (unsigned char *)p).first;
+    }
+    *out = it->second;
+    return MCK_OK;
+}
+
+// carves n elements of T, 256-byte aligned, from [*cur, end); null when they do not fit
+template <typename T>
+T *carve(unsigned char **cur, unsigned char *end, size_t n) {
+    uintptr_t a = ((uintptr_t)*cur + 255) & ~(uintptr_t)255;
+    if (a + n * sizeof(T) > (uintptr_t)end) return nullptr;
+    *cur = (unsigned char *)(a + n * sizeof(T));
+    return (T *)a;
+}
+
+int results_add_impl(int32_t nelems, const int32_t *elems, const double *kernel_values,
+                     const double *kernel_variance, const int32_t *niterations, const double *computation_time,
+                     const double *model, const double *hetero, int32_t rank, bool noalloc) {
     State &G = g;
     auto &Rs = G.res;
     if (!G.init) return fail(MCK_ENOTINIT, "mck_init has not been called");
@@ -2640,6 +2698,16 @@ int mck_results_add(int32_t nelems, const int32_t *elems, const double *kernel_v
     const bool with_het = Rs.nhet > 0;
     if (with_het && !hetero && (G.last_hetero_nelems != nelems || G.last_hetero_nbasis != nb))
         return fail(MCK_EINVAL, "no heterogeneity integrals of a mck_integrate_hetero call with these nelems and nbasis");
+    const int npair = nelems * nb;
+    if (noalloc) {
+        if (npair > kNoAllocMaxPairs)
+            return fail(MCK_EINVAL, "mck_results_add_noalloc: more than " + std::to_string(kNoAllocMaxPairs)
+                                    + " (element, basis function) pairs per task");
+        if ((!last && (!device_ptr(kernel_values) || !device_ptr(kernel_variance) || !device_ptr(niterations)))
+            || (with_model && model && !device_ptr(model)) || (with_het && hetero && !device_ptr(hetero)))
+            return fail(MCK_EINVAL, "mck_results_add_noalloc takes its kernel values, variances, iteration "
+                                    "counts and model integrals in device memory (or NULL)");
+    }
     // element ids checked on the host; the grouping by point runs on the device
     std::vector<int> eidx(nelems, -1);
     for (int iel = 0; iel < nelems; ++iel) {
@@ -2649,52 +2717,78 @@ int mck_results_add(int32_t nelems, const int32_t *elems, const double *kernel_v
         eidx[iel] = e - 1;
     }
     hipStream_t st = G.stream;
-    const int npair = nelems * nb;
-    for (DBuf<int> *b : {&Rs.pt_in, &Rs.pt_out, &Rs.sl_in, &Rs.sl_out, &Rs.seg_point, &Rs.seg_count, &Rs.seg_start})
-        HIPCHECK(b->ensure(npair));
-    HIPCHECK(Rs.nseg.ensure(1));
-    HIPCHECK(Rs.elems.ensure(nelems));
-    HIPCHECK(Rs.elem_idx.ensure(nelems));
-    HIPCHECK(hipMemcpyAsync(Rs.elems.p, elems, (size_t)nelems * 4, hipMemcpyHostToDevice, st));
-    HIPCHECK(hipMemcpyAsync(Rs.elem_idx.p, eidx.data(), (size_t)nelems * 4, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_results_pairs, dim3((npair + 255) / 256), dim3(256), 0, st, nelems, nb,
-                       Rs.int_type == 0 ? 1 : 0, Rs.npoints, Rs.elems.p, Rs.dconn.p, Rs.pt_in.p, Rs.sl_in.p);
-    HIPCHECK(hipGetLastError());
-    {
-        // stable radix sort by point (task order kept within a point), then the runs
-        int bits = 1;
-        while ((1 << bits) <= Rs.npoints) ++bits;
-        size_t n1 = 0, n2 = 0, n3 = 0;
-        HIPCHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, n1, Rs.pt_in.p, Rs.pt_out.p, Rs.sl_in.p, Rs.sl_out.p,
-                                                    npair, 0, bits, st));
-        HIPCHECK(hipcub::DeviceRunLengthEncode::Encode(nullptr, n2, Rs.pt_out.p, Rs.seg_point.p, Rs.seg_count.p,
-                                                       Rs.nseg.p, npair, st));
-        HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, n3, Rs.seg_count.p, Rs.seg_start.p, npair, st));
-        HIPCHECK(Rs.tmp.ensure(std::max({n1, n2, n3, (size_t)1})));
-        HIPCHECK(hipcub::DeviceRadixSort::SortPairs(Rs.tmp.p, n1, Rs.pt_in.p, Rs.pt_out.p, Rs.sl_in.p, Rs.sl_out.p,
-                                                    npair, 0, bits, st));
-        HIPCHECK(hipcub::DeviceRunLengthEncode::Encode(Rs.tmp.p, n2, Rs.pt_out.p, Rs.seg_point.p, Rs.seg_count.p,
-                                                       Rs.nseg.p, npair, st));
-        // counts past the last run are not read: every segment's start is the sum of the
-        // counts before it, whatever follows
-        HIPCHECK(hipcub::DeviceScan::ExclusiveSum(Rs.tmp.p, n3, Rs.seg_count.p, Rs.seg_start.p, npair, st));
-    }
-    // inputs: in device memory already, or staged
+    int bits = 1;
+    while ((1 << bits) <= Rs.npoints) ++bits;
+    size_t n1 = 0, n2 = 0, n3 = 0;
+    // (size queries: no device work, no allocation)
+    HIPCHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, n1, (int *)nullptr, (int *)nullptr, (int *)nullptr,
+                                                (int *)nullptr, npair, 0, bits, st));
+    HIPCHECK(hipcub::DeviceRunLengthEncode::Encode(nullptr, n2, (int *)nullptr, (int *)nullptr, (int *)nullptr,
+                                                   (int *)nullptr, npair, st));
+    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, n3, (int *)nullptr, (int *)nullptr, npair, st));
+    const size_t tmp_bytes = std::max({n1, n2, n3, (size_t)1});
+    // staging of host inputs
     const size_t nv = (size_t)nelems * nb * nk, nm = with_model ? (size_t)nelems * nb * kModelPar : 0;
     const size_t nh = with_het ? (size_t)nelems * nb * kHetPar : 0;
+    const bool sv = !last && !device_ptr(kernel_values), svar = !last && !device_ptr(kernel_variance);
+    const bool sm = with_model && model && !device_ptr(model);
+    const bool sh = with_het && hetero && !device_ptr(hetero);
+    const bool sct = last || (computation_time && !device_ptr(computation_time));
+    const bool sni = !last && !device_ptr(niterations);
+    const size_t need = (sv ? nv : 0) + (svar ? nv : 0) + (sm ? nm : 0) + (sh ? nh : 0) + (sct ? nelems : 0);
+    AddScratch A;
+    if (noalloc) {
+        unsigned char *base = nullptr;
+        int rc;
+        if ((rc = scratch_address(&base))) return rc;
+        unsigned char *cur = base, *end = base + kNoAllocScratch;
+        int **ints[] = {&A.pt_in, &A.pt_out, &A.sl_in, &A.sl_out, &A.seg_point, &A.seg_count, &A.seg_start};
+        bool ok = true;
+        for (int **q : ints) ok = ok && (*q = carve<int>(&cur, end, npair)) != nullptr;
+        ok = ok && (A.nseg = carve<int>(&cur, end, 1)) != nullptr;
+        ok = ok && (A.elems = carve<int>(&cur, end, nelems)) != nullptr;
+        ok = ok && (A.elem_idx = carve<int>(&cur, end, nelems)) != nullptr;
+        ok = ok && (A.stage = carve<double>(&cur, end, std::max<size_t>(need, 1))) != nullptr;
+        ok = ok && (A.tmp = carve<unsigned char>(&cur, end, tmp_bytes)) != nullptr;
+        if (!ok) return fail(MCK_EINVAL, "mck_results_add_noalloc: the task does not fit the fixed device array");
+    } else {
+        for (DBuf<int> *b : {&Rs.pt_in, &Rs.pt_out, &Rs.sl_in, &Rs.sl_out, &Rs.seg_point, &Rs.seg_count, &Rs.seg_start})
+            HIPCHECK(b->ensure(npair));
+        HIPCHECK(Rs.nseg.ensure(1));
+        HIPCHECK(Rs.elems.ensure(nelems));
+        HIPCHECK(Rs.elem_idx.ensure(nelems));
+        HIPCHECK(Rs.tmp.ensure(tmp_bytes));
+        HIPCHECK(Rs.stage.ensure(std::max<size_t>(need, 1)));
+        if (sni) HIPCHECK(Rs.stage_i.ensure((size_t)nelems * nk));
+        A.pt_in = Rs.pt_in.p; A.pt_out = Rs.pt_out.p; A.sl_in = Rs.sl_in.p; A.sl_out = Rs.sl_out.p;
+        A.seg_point = Rs.seg_point.p; A.seg_count = Rs.seg_count.p; A.seg_start = Rs.seg_start.p;
+        A.nseg = Rs.nseg.p; A.elems = Rs.elems.p; A.elem_idx = Rs.elem_idx.p; A.tmp = Rs.tmp.p;
+        A.stage = Rs.stage.p; A.stage_i = Rs.stage_i.p;
+    }
+    A.tmp_bytes = tmp_bytes;
+    HIPCHECK(hipMemcpyAsync(A.elems, elems, (size_t)nelems * 4, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(A.elem_idx, eidx.data(), (size_t)nelems * 4, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_results_pairs, dim3((npair + 255) / 256), dim3(256), 0, st, nelems, nb,
+                       Rs.int_type == 0 ? 1 : 0, Rs.npoints, A.elems, Rs.dconn.p, A.pt_in, A.sl_in);
+    HIPCHECK(hipGetLastError());
+    {
+        // stable radix sort by point (task order kept within a point), then the runs;
+        // counts past the last run are not read: every segment's start is the sum of the
+        // counts before it, whatever follows
+        size_t b1 = tmp_bytes, b2 = tmp_bytes, b3 = tmp_bytes;
+        HIPCHECK(hipcub::DeviceRadixSort::SortPairs(A.tmp, b1, A.pt_in, A.pt_out, A.sl_in, A.sl_out, npair, 0, bits,
+                                                    st));
+        HIPCHECK(hipcub::DeviceRunLengthEncode::Encode(A.tmp, b2, A.pt_out, A.seg_point, A.seg_count, A.nseg, npair,
+                                                       st));
+        HIPCHECK(hipcub::DeviceScan::ExclusiveSum(A.tmp, b3, A.seg_count, A.seg_start, npair, st));
+    }
+    // inputs: in device memory already, or staged
     const double *dv = G.integral.p, *dvar = G.variance.p, *dm = G.last_model.p, *dh = G.last_hetero.p;
     const double *dct = nullptr;
     const int *dni = G.niter.p;
     const std::vector<double> &ct_last = G.last_ctime;
     {
-        size_t need = 0;
-        const bool sv = !last && !device_ptr(kernel_values), svar = !last && !device_ptr(kernel_variance);
-        const bool sm = with_model && model && !device_ptr(model);
-        const bool sh = with_het && hetero && !device_ptr(hetero);
-        const bool sct = last || (computation_time && !device_ptr(computation_time));
-        need = (sv ? nv : 0) + (svar ? nv : 0) + (sm ? nm : 0) + (sh ? nh : 0) + (sct ? nelems : 0);
-        HIPCHECK(Rs.stage.ensure(std::max<size_t>(need, 1)));
-        double *q = Rs.stage.p;
+        double *q = A.stage;
         auto put = [&](const double *src, size_t n) -> const double * {
             double *d = q;
             q += n;
@@ -2704,12 +2798,11 @@ int mck_results_add(int32_t nelems, const int32_t *elems, const double *kernel_v
             dv = sv ? put(kernel_values, nv) : kernel_values;
             dvar = svar ? put(kernel_variance, nv) : kernel_variance;
             if (!dv || !dvar) return fail(MCK_EHIP, "mck_results_add: staging copy failed");
-            if (device_ptr(niterations)) {
+            if (!sni) {
                 dni = niterations;
             } else {
-                HIPCHECK(Rs.stage_i.ensure((size_t)nelems * nk));
-                HIPCHECK(hipMemcpyAsync(Rs.stage_i.p, niterations, (size_t)nelems * nk * 4, hipMemcpyHostToDevice, st));
-                dni = Rs.stage_i.p;
+                HIPCHECK(hipMemcpyAsync(A.stage_i, niterations, (size_t)nelems * nk * 4, hipMemcpyHostToDevice, st));
+                dni = A.stage_i;
             }
         }
         if (with_model && model) {
@@ -2736,11 +2829,11 @@ int mck_results_add(int32_t nelems, const int32_t *elems, const double *kernel_v
             const long long nt = (long long)npair * (two ? ncol / 2 : ncol);
             const dim3 grid((unsigned)((nt + 255) / 256));
             if (two)
-                hipLaunchKernelGGL(k_results_add<2>, grid, dim3(256), 0, st, Rs.nseg.p, ncol, Rs.npoints,
-                                   Rs.seg_start.p, Rs.seg_count.p, Rs.seg_point.p, Rs.sl_out.p, v0, a0, v1, a1);
+                hipLaunchKernelGGL(k_results_add<2>, grid, dim3(256), 0, st, A.nseg, ncol, Rs.npoints,
+                                   A.seg_start, A.seg_count, A.seg_point, A.sl_out, v0, a0, v1, a1);
             else
-                hipLaunchKernelGGL(k_results_add<1>, grid, dim3(256), 0, st, Rs.nseg.p, ncol, Rs.npoints,
-                                   Rs.seg_start.p, Rs.seg_count.p, Rs.seg_point.p, Rs.sl_out.p, v0, a0, v1, a1);
+                hipLaunchKernelGGL(k_results_add<1>, grid, dim3(256), 0, st, A.nseg, ncol, Rs.npoints,
+                                   A.seg_start, A.seg_count, A.seg_point, A.sl_out, v0, a0, v1, a1);
         };
         add(nk, dv, Rs.K.p, dvar, Rs.V.p);
         if (with_model) add(kModelPar, dm, Rs.M.p, nullptr, nullptr);
@@ -2751,15 +2844,38 @@ int mck_results_add(int32_t nelems, const int32_t *elems, const double *kernel_v
         const long long nt3 = (long long)nelems * (four ? nk / 4 : nk);
         const dim3 grid((unsigned)((nt3 + 255) / 256));
         if (four)
-            hipLaunchKernelGGL(k_results_elem<4>, grid, dim3(256), 0, st, nelems, nk, Rs.elem_idx.p, dni, dct, rank,
+            hipLaunchKernelGGL(k_results_elem<4>, grid, dim3(256), 0, st, nelems, nk, A.elem_idx, dni, dct, rank,
                                Rs.niter.p, Rs.ctime.p, Rs.proc.p);
         else
-            hipLaunchKernelGGL(k_results_elem<1>, grid, dim3(256), 0, st, nelems, nk, Rs.elem_idx.p, dni, dct, rank,
+            hipLaunchKernelGGL(k_results_elem<1>, grid, dim3(256), 0, st, nelems, nk, A.elem_idx, dni, dct, rank,
                                Rs.niter.p, Rs.ctime.p, Rs.proc.p);
     }
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(st));
+    // device-resident results are consumed: a later call must bring its own
+    if (last) G.last_nelems = G.last_nbasis = 0;
+    if (with_model && !model) G.last_model_nelems = G.last_model_nbasis = 0;
+    if (with_het && !hetero) G.last_hetero_nelems = G.last_hetero_nbasis = 0;
     return MCK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mck_results_add(int32_t nelems, const int32_t *elems, const double *kernel_values,
+                    const double *kernel_variance, const int32_t *niterations, const double *computation_time,
+                    const double *model, const double *hetero, int32_t rank) {
+    return results_add_impl(nelems, elems, kernel_values, kernel_variance, niterations, computation_time, model,
+                            hetero, rank, false);
+}
+
+int mck_results_add_noalloc(int32_t nelems, const int32_t *elems, const double *kernel_values,
+                            const double *kernel_variance, const int32_t *niterations,
+                            const double *computation_time, const double *model, const double *hetero,
+                            int32_t rank) {
+    return results_add_impl(nelems, elems, kernel_values, kernel_variance, niterations, computation_time, model,
+                            hetero, rank, true);
 }
 
 int mck_results_get(double *kernel, double *variance, double *model, double *hetero, int32_t *niterations,

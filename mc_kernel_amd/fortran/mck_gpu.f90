@@ -80,7 +80,7 @@ module mck_gpu
   public :: mck_init, mck_slave_work, mck_slave_work_plain, mck_locate_points, &
             mck_load_strain_points, mck_base_kernels, mck_seismograms, mck_integrate_model, &
             mck_set_hetero, mck_integrate_hetero, &
-            mck_results_init, mck_results_add, mck_results_get, mck_results_load, &
+            mck_results_init, mck_results_add, mck_results_add_noalloc, mck_results_get, mck_results_load, &
             mck_results_export, &
             mck_get_stats, mck_finalize, &
             mck_last_error, mck_error_message, mck_check
@@ -207,6 +207,18 @@ module mck_gpu
        integer(c_int32_t), intent(in) :: elems(*)
        type(c_ptr), value :: kernel_values, kernel_variance, niterations, computation_time, model, hetero
      end function mck_results_add
+
+     ! mck_results_add without any allocation, for callers that cannot allocate (no
+     ! counterpart in the reference): the four result arrays, model and hetero in device
+     ! memory or c_null_ptr, at most 131072 (element, basis function) pairs per call
+     integer(c_int) function mck_results_add_noalloc(nelems, elems, kernel_values, kernel_variance, &
+                                                     niterations, computation_time, model, hetero, rank) &
+                                                     bind(c, name='mck_results_add_noalloc')
+       import :: c_int, c_int32_t, c_ptr
+       integer(c_int32_t), value :: nelems, rank
+       integer(c_int32_t), intent(in) :: elems(*)
+       type(c_ptr), value :: kernel_values, kernel_variance, niterations, computation_time, model, hetero
+     end function mck_results_add_noalloc
 
      ! K_x, Var (nkernel, npoints), Bg_Model (nmodel, npoints), niterations
      ! (nkernel, nelements), computation_time, element_proc (nelements): c_null_ptr skips
