@@ -28,13 +28,19 @@
  * the heterogeneity model's nearest point against the reference's own 3-D kd-tree
  * (oracle/_ref), index and fp32 distance, and the heterogeneity integrals' closed forms
  * (tests/test_oracle_hetero.py); the result assembly's hand sums, scatter formulation and
- * error transform (tests/test_oracle_results.py, mck_oracle_results.c).
+ * error transform (tests/test_oracle_results.py, mck_oracle_results.c); in
+ * tests/test_oracle_pins.py: dampen_field by the reference's known answer
+ * (test_readfields.f90:1189-1217), calc_basekernel's lambda and mu by branch equality,
+ * the isotropic-strain ratio K_mu = 2/3 K_lambda, symmetry and unit strains, add_stfs by
+ * time-domain convolutions with the database and source STFs and a circular shift, and
+ * its stop at the highest frequency, and the slave_work loop by iteration counts, draws
+ * and evaluations counted by hand (max_iter exits, travel-time masks, a receiver inside
+ * the element).
  *
  * Parity unpinned (restated from the reference text, no fixture or closed form checks
- * them): calc_basekernel (kernel.f90:514-570) including the k_c product as written; the
- * anisotropic physical kernels vsh, vsv, vph, vpv, eta (kernel.f90:745-822); add_stfs
- * (filtering.f90:241-470, deconv_stf); dampen_field (readfields.f90:3184-3204); the
- * slave_work loop itself (slave.f90:425-806), checked end to end only against itself.
+ * them): calc_basekernel's k_a, k_b, k_c (kernel.f90:553-567, k_c's product as written),
+ * which no kernel of the benchmark sets uses; the anisotropic physical kernels vsh, vsv,
+ * vph, vpv, eta (kernel.f90:745-822).
  */
 #ifndef MCK_ORACLE_H
 #define MCK_ORACLE_H
@@ -164,6 +170,15 @@ int      ora_param_needs(int param, int32_t *nb);
 double   ora_azim_factor(double phi, const double *mij, int isim, int ikind);
 /* cut_timewindow on a time axis t[n] (test hook and kernel set-up) */
 void     ora_cut_window(int n, const double *t, double w0, double w1, int *ifirst, int *ncut);
+/* calc_basekernel alone (test hook): F, B [nd][nw][2] -> out [nw][2]; ibk 0..5 */
+void     ora_basekernel(int ibk, int full, int nd, int nw, const double *F, const double *B, double *out);
+/* dampen_field alone (test hook): base[npts][nk][6] scaled by |x - c| / rmax inside rmax */
+void     ora_dampen_field(double *base, int npts, int nk, const double *pts, const double *c, double rmax);
+/* add_stfs alone (test hook): tff in = filter response, out = forward; tfb out = backward;
+ * stf_src NULL = unit impulse; -1 where the reference stops (response at the highest
+ * frequency above 1e-2 of its maximum) */
+int      ora_add_stfs(int nt, double dt, int N, const double *stf_sem, double amplitude, const double *stf_src,
+                      double stf_shift, double *tff, double *tfb, char *err, int errlen);
 /* integrate_parseval alone (test hook) */
 double   ora_integrate_parseval(int n, double dt, const double *f, const double *g);
 /* the Monte Carlo integrator alone (test hook): vals[nsteps][npts][n] */

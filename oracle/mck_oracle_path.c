@@ -110,36 +110,44 @@ static void make_timeshift(int nomega, double df, double tau, double *ts)
     }
 }
 
-/* add_stfs (filtering.f90:241-470) */
-static int add_stfs(ora_ctx *C, double *tff, double *tfb, char *err, int errlen)
+/* add_stfs (filtering.f90:241-470) on one transfer function tff[nomega][2] (in: the
+ * filter's response; out: the forward one) and tfb (out: the backward one): divide by the
+ * spectrum of the database STF (tapered, ntaper 2), scale by the source magnitude, apply
+ * the earthquake STF (tapered at its end only; NULL = a unit impulse at t = 0) and its
+ * time shift to the forward one, then the squared 12th-order Butterworth low pass at
+ * 0.75 x the highest frequency to both; NaNs become 0.  Returns -1 (with the message)
+ * when either response at the highest frequency exceeds 1e-2 x its maximum, where the
+ * reference stops. */
+int ora_add_stfs(int nt, double dt, int N, const double *stf_sem, double amplitude, const double *stf_src,
+                 double stf_shift, double *tff, double *tfb, char *err, int errlen)
 {
-    const mck_setup_t *S = &C->S;
-    int N = C->N, nw = C->nomega, k;
+    int nw = N / 2 + 1, k;
+    double df = 1.0 / (dt * N);
     double *x = (double *)calloc(N, sizeof(double)), *pad = (double *)calloc(N, sizeof(double));
     double *Fs = (double *)malloc(2 * nw * sizeof(double)), *Fe = (double *)malloc(2 * nw * sizeof(double));
     double *f = (double *)malloc(nw * sizeof(double));
     mck_filter_t lpf;
     double *lp = (double *)malloc(2 * nw * sizeof(double));
     double maxf = 0, maxb = 0, rf, rb;
-    for (k = 0; k < nw; ++k) f[k] = C->df * k;
-    ora_taperandzeropad(S->nt, N, 2, 0, S->stf_fwd, pad);
-    ora_rfft(N, S->dt, pad, Fs);
-    for (k = 0; k < S->nt; ++k) x[k] = S->stf_src ? S->stf_src[k] : (k == 0 ? 1.0 : 0.0);
-    ora_taperandzeropad(S->nt, N, 2, 1, x, pad);
-    ora_rfft(N, S->dt, pad, Fe);
+    for (k = 0; k < nw; ++k) f[k] = df * k;
+    ora_taperandzeropad(nt, N, 2, 0, stf_sem, pad);
+    ora_rfft(N, dt, pad, Fs);
+    for (k = 0; k < nt; ++k) x[k] = stf_src ? stf_src[k] : (k == 0 ? 1.0 : 0.0);
+    ora_taperandzeropad(nt, N, 2, 1, x, pad);
+    ora_rfft(N, dt, pad, Fe);
     lpf.filterclass = MCK_FILTER_BUTTER_LP;
     lpf.frequencies[0] = 1.0 / (f[nw - 1] * 0.75);
     lpf.frequencies[1] = 0; lpf.frequencies[2] = 12; lpf.frequencies[3] = 0;
-    ora_filter_tf(&lpf, nw, C->df, lp);
+    ora_filter_tf(&lpf, nw, df, lp);
     for (k = 0; k < nw; ++k) {
         cz t = {tff[2 * k], tff[2 * k + 1]}, s = {Fs[2 * k], Fs[2 * k + 1]};
         cz e = {Fe[2 * k], Fe[2 * k + 1]}, l = {lp[2 * k], lp[2 * k + 1]}, lc = {lp[2 * k], -lp[2 * k + 1]};
         cz fw, bw, sh;
-        double th = 2 * ORA_PI * f[k] * S->stf_src_shift;
+        double th = 2 * ORA_PI * f[k] * stf_shift;
         sh.re = cos(th); sh.im = sin(th);
         l = zmul(l, lc);
-        fw = zdiv(t, s); fw.re *= S->fwd.amplitude; fw.im *= S->fwd.amplitude;
-        bw = zdiv(t, s); bw.re *= S->fwd.amplitude; bw.im *= S->fwd.amplitude;
+        fw = zdiv(t, s); fw.re *= amplitude; fw.im *= amplitude;
+        bw = zdiv(t, s); bw.re *= amplitude; bw.im *= amplitude;
         fw = zmul(fw, e);
         fw = zmul(fw, sh);
         fw = zmul(fw, l);
@@ -155,10 +163,17 @@ static int add_stfs(ora_ctx *C, double *tff, double *tfb, char *err, int errlen)
     rb = hypot(tfb[2 * (nw - 1)], tfb[2 * (nw - 1) + 1]) / maxb;
     free(x); free(pad); free(Fs); free(Fe); free(f); free(lp);
     if (rf > 1e-2 || rb > 1e-2) {
-        snprintf(err, errlen, "filter not vanishing at the highest frequency (%g, %g)", rf, rb);
+        if (err) snprintf(err, errlen, "filter not vanishing at the highest frequency (%g, %g)", rf, rb);
         return -1;
     }
     return 0;
+}
+
+static int add_stfs(ora_ctx *C, double *tff, double *tfb, char *err, int errlen)
+{
+    const mck_setup_t *S = &C->S;
+    return ora_add_stfs(S->nt, S->dt, C->N, S->stf_fwd, S->fwd.amplitude, S->stf_src, S->stf_src_shift, tff, tfb,
+                        err, errlen);
 }
 
 /* integrate_parseval (kernel.f90:975-1020): Re[(ab_1 + ab_n + 2 sum ab_mid) df] */
@@ -775,8 +790,10 @@ static void fields_to_fd(ora_ctx *C, int ntr, const double *td, double *fd, cons
     free(pad);
 }
 
-/* calc_basekernel (kernel.f90:514-570); F, B [nd][nomega][2] -> out [nomega][2] */
-static void basekernel(int ibk, int full, int nd, int nw, const double *F, const double *B, double *out)
+/* calc_basekernel (kernel.f90:514-570); F, B [nd][nomega][2] -> out [nomega][2];
+ * ibk 0..5 = lambda, mu, rho, a, b, c; full: the strain types carry the 6 Voigt
+ * components (else the trace alone, nd 1) */
+void ora_basekernel(int ibk, int full, int nd, int nw, const double *F, const double *B, double *out)
 {
     int k, c;
 #define FF(c) (cz){F[((c) * nw + k) * 2], F[((c) * nw + k) * 2 + 1]}
@@ -949,7 +966,7 @@ static int receiver_base(ora_ctx *C, int r, int npts, const double *pts, const d
             if (!C->K[k].needs_bk[b]) continue;
             if (conv_flags && conv_flags[k]) continue;
             for (p = 0; p < npts; ++p) {
-                basekernel(b, C->full, nd, nw, F + (size_t)p * nd * nw * 2, B + (size_t)p * nd * nw * 2, conv);
+                ora_basekernel(b, C->full, nd, nw, F + (size_t)p * nd * nw * 2, B + (size_t)p * nd * nw * 2, conv);
                 base[((size_t)p * S->nkernel + k) * 6 + b] = misfit_value(C, k, conv, work);
             }
             if (niter) niter[k] += 1;
@@ -977,7 +994,7 @@ int ora_base_kernels(ora_ctx *C, int npts, const double *xyz, double *base)
 }
 
 /* dampen_field (readfields.f90:3184-3204) on base[npts][nkernel][6] */
-static void dampen(double *base, int npts, int nk, const double *pts, const double *c, double rmax)
+void ora_dampen_field(double *base, int npts, int nk, const double *pts, const double *c, double rmax)
 {
     int p, i;
     if (!(rmax > 0.0)) return;
@@ -1217,8 +1234,8 @@ int ora_slave_work(ora_ctx *C, int elem_type, int nvpe, int nelems, int nbasis,
                 if (allconv) continue;
                 rc = receiver_base(C, r, P, pts, F, conv_all, base, evaluations, niter);
                 if (rc) { for (b = 0; b < nbasis; ++b) mc_free(&mcs[b]); goto out; }
-                dampen(base, P, nk, pts, C->R[r].r, S->damp_radius);
-                dampen(base, P, nk, pts, C->src_r, S->damp_radius);
+                ora_dampen_field(base, P, nk, pts, C->R[r].r, S->damp_radius);
+                ora_dampen_field(base, P, nk, pts, C->src_r, S->damp_radius);
             }
             /* per basis function: weights, mult3d_1d, physical kernels, MC check
              * (slave.f90:723-744) */
@@ -1511,7 +1528,7 @@ int ora_ad_forward(ora_ctx *C, int npts, const double *G, const double *fw, cons
                 if (!C->R[r].needs_bk[b]) continue;
                 for (k = C->R[r].first; k <= C->R[r].last; ++k) {
                     if (!C->K[k].needs_bk[b]) continue;
-                    basekernel(b, C->full, nd, nw, F + p * fs, B + ((size_t)p * R + r) * fs, conv);
+                    ora_basekernel(b, C->full, nd, nw, F + p * fs, B + ((size_t)p * R + r) * fs, conv);
                     ad_tf(C, G, C->K[k].filter, ta, tb);
                     base[((size_t)p * nk + k) * 6 + b] = misfit_value_tf(C, k, conv, ta, tb, work);
                 }
@@ -1615,7 +1632,7 @@ int ora_ad_backward(ora_ctx *C, int npts, const double *G, const double *fw, con
                     const double *rf = C->rest_f, *rb = C->rest_b;
                     double *gGf = gG + (size_t)K->filter * nw * 2;
                     if (!K->needs_bk[b] || gv == 0.0) continue;
-                    basekernel(b, C->full, nd, nw, Fp, Bp, conv);
+                    ora_basekernel(b, C->full, nd, nw, Fp, Bp, conv);
                     ad_tf(C, G, K->filter, ta, tb);
                     /* value = -norm dt sum_n w_n x(ifirst + n) s(n): the Parseval integral of
                      * the zero-padded window (integrate_parseval_b_complex) and the

@@ -71,6 +71,10 @@ def lib():
         L.ora_strain_nd.argtypes = [C.c_void_p]
         L.ora_set_lazy_fill.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.ora_integrate_parseval.argtypes = [C.c_int, C.c_double, f64p, f64p]
+        L.ora_basekernel.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, f64p, f64p, f64p]
+        L.ora_dampen_field.argtypes = [f64p, C.c_int, C.c_int, f64p, f64p, C.c_double]
+        L.ora_add_stfs.argtypes = [C.c_int, C.c_double, C.c_int, f64p, C.c_double, f64p, C.c_double,
+                                   f64p, f64p, C.c_char_p, C.c_int]
         L.ora_integrate_parseval.restype = C.c_double
         L.ora_physical.argtypes = [C.c_int, f64p] + [C.c_float] * 6 + [C.c_int]
         L.ora_physical.restype = C.c_double
@@ -271,6 +275,39 @@ def integrate_parseval(f, g, dt):
     f = np.ascontiguousarray(f, float)
     g = np.ascontiguousarray(g, float)
     return lib().ora_integrate_parseval(len(f), dt, P(f), P(g))
+
+
+def basekernel(ibk, full, F, B):
+    """calc_basekernel: F, B complex [nd][nw] -> complex [nw]"""
+    F = np.ascontiguousarray(F, np.complex128)
+    B = np.ascontiguousarray(B, np.complex128)
+    nd, nw = F.shape
+    out = np.zeros(nw, np.complex128)
+    lib().ora_basekernel(ibk, int(full), nd, nw, P(F.view(float)), P(B.view(float)), P(out.view(float)))
+    return out
+
+
+def dampen_field(base, pts, c, rmax):
+    """dampen_field on base [npts][nk][6] (a damped copy)"""
+    b = np.array(base, float, order="C")
+    pts = np.ascontiguousarray(pts, float)
+    c = np.ascontiguousarray(c, float)
+    lib().ora_dampen_field(P(b), b.shape[0], b.shape[1], P(pts), P(c), rmax)
+    return b
+
+
+def add_stfs(tf, stf_sem, dt, amplitude=1.0, stf_src=None, stf_shift=0.0):
+    """add_stfs on a filter response tf [nw] (N = 2 (nw - 1)); (rc, tf_fwd, tf_bwd, message)"""
+    stf_sem = np.ascontiguousarray(stf_sem, float)
+    nt = len(stf_sem)
+    tff = np.array(tf, np.complex128)
+    tfb = np.zeros_like(tff)
+    N = 2 * (len(tff) - 1)
+    src = None if stf_src is None else np.ascontiguousarray(stf_src, float)
+    msg = C.create_string_buffer(256)
+    rc = lib().ora_add_stfs(nt, dt, N, P(stf_sem), amplitude, P(src), stf_shift, P(tff.view(float)),
+                            P(tfb.view(float)), msg, 256)
+    return rc, tff, tfb, msg.value.decode()
 
 
 def physical(param, base, vp, vs, rho, phi=1.0, xi=1.0, eta=1.0, relative=True):

@@ -22,11 +22,12 @@ pytestmark = pytest.mark.gpu
 DC = [0.3e20, 1.0e20, -1.3e20, 0.5e20, -0.4e20, 0.7e20]
 CASES = {"explosion_Z": {}, "general_mij_R": dict(mij=DC, component="R")}
 PAGE = 37          # elements per page: 13 pages of the tiny databases (462 elements)
+TOL = 1e-5
 
 
-def _with_kernel(page, kw, fn, device_pages=0):
+def _with_kernel(page, kw, fn, device_pages=0, p=None):
     import mc_kernel_amd as mk
-    p = synth.make_problem("tiny", ntet=12, **kw)
+    p = p or synth.make_problem("tiny", ntet=12, **kw)
     p.setup.db_page_elems = page
     p.setup.db_device_pages = device_pages
     k = mk.MCKernel(p.setup)
@@ -36,8 +37,26 @@ def _with_kernel(page, kw, fn, device_pages=0):
         k.close()
 
 
-def _task(p, k):
-    res = k.slave_work(p.verts, p.conn)
+_ORACLE = {}
+
+
+def _oracle(case):
+    """The oracle's slave_work and base kernels of a case's task (kept per case)."""
+    if case not in _ORACLE:
+        from oracle import oracle
+        p = synth.make_problem("tiny", ntet=12, **CASES[case])
+        o = oracle.Oracle(p.setup)
+        _ORACLE[case] = (o.slave_work(p.verts, p.conn), o.base_kernels(p.sample_points(40, seed=9)))
+    return _ORACLE[case]
+
+
+def _rel(g, r):
+    n = np.linalg.norm(r)
+    return np.linalg.norm(g - r) / n if n else np.linalg.norm(g)
+
+
+def _task(p, k, offsets=None):
+    res = k.slave_work(p.verts, p.conn, halton_offsets=offsets)
     x = p.sample_points(40, seed=9)
     traces = [k.load_strain(f, x) for f in (0, 1)]
     return res, k.base_kernels(x), traces
@@ -46,11 +65,15 @@ def _task(p, k):
 @pytest.mark.parametrize("device_pages", [2, 7])
 @pytest.mark.parametrize("case", sorted(CASES))
 def test_paged_equals_resident(case, device_pages):
+    """Paged = resident bit for bit, and both = the oracle on the oracle's stream
+    positions (element ids and iteration counts exact, kernels to 1e-5)."""
     print("paging case", case, device_pages, flush=True)
-    p, (ref, base_ref, tr_ref) = _with_kernel(0, CASES[case], _task)
+    oref, obase = _oracle(case)
+    task = lambda p, k: _task(p, k, oref["offsets"])
+    p, (ref, base_ref, tr_ref) = _with_kernel(0, CASES[case], task)
     nelem = p.setup.fwd.nelem
     assert nelem > 4 * PAGE
-    _, (got, base_got, tr_got) = _with_kernel(PAGE, CASES[case], _task, device_pages)
+    _, (got, base_got, tr_got) = _with_kernel(PAGE, CASES[case], task, device_pages)
     assert np.abs(ref["values"]).max() > 0
     for key in ("values", "variance", "niterations"):
         np.testing.assert_array_equal(got[key], ref[key], err_msg=key)
@@ -58,6 +81,13 @@ def test_paged_equals_resident(case, device_pages):
     np.testing.assert_array_equal(base_got, base_ref)
     for a, b in zip(tr_got, tr_ref):
         np.testing.assert_array_equal(a, b)
+    # against the oracle
+    assert np.array_equal(got["niterations"], oref["niterations"])
+    assert got["stats"]["evaluations"] == oref["evaluations"]
+    for j in range(p.nkernel):
+        assert _rel(got["values"][:, j], oref["values"][:, j]) < TOL, j
+        for b in range(6):
+            assert _rel(base_got[:, j, b], obase[:, j, b]) < TOL, (j, b)
     # the task's launches copied pages, and never more than every page once per launch
     st, st0 = got["stats"], ref["stats"]
     assert st0["paged_bytes"] == 0
@@ -127,7 +157,16 @@ def test_slave_work_from_database_files(tmp_path, page):
             got = k.slave_work(p.verts, p.conn)
         finally:
             k.close()
+        from oracle import oracle
+        p.setup.db_page_elems = 0
+        o = oracle.Oracle(p.setup)
+        oref = o.slave_work(p.verts, p.conn, halton_offsets=np.arange(len(p.conn), dtype=np.int64)
+                            * (p.setup.max_iter + 1) * p.setup.npoints_per_step)
     assert np.abs(ref["values"]).max() > 0
     for key in ("values", "variance", "niterations"):
         np.testing.assert_array_equal(got[key], ref[key], err_msg=key)
     assert (got["stats"]["paged_bytes"] > 0) == (page > 0)
+    # the oracle on the databases read back from the files, on the task's stream positions
+    assert np.array_equal(got["niterations"], oref["niterations"])
+    for j in range(p.nkernel):
+        assert _rel(got["values"][:, j], oref["values"][:, j]) < TOL, j
