@@ -254,6 +254,25 @@ int mck_slave_work_plain(int32_t elem_type, int32_t nvpe, int32_t nelems, int32_
                          int64_t *halton_step, const int64_t *halton_offsets);
 
 /*
+ * The work unit on the reference's own Halton stream (slave_work with the module-level
+ * position of halton_sequence.f90:109-153, advanced by every draw, including the draw
+ * before the max_iter exit, slave.f90:501-511): elements one after another in the
+ * plain form's order, element e starting where element e-1's draws ended, so that a task
+ * gives the reference's points without positions computed elsewhere.
+ *  halton_step     in: the position at which the first element starts;
+ *                  out: the position after the task's last draw
+ *  offsets_used    optional out [nelems]: the position at which each element started
+ *                  (passed as mck_slave_work's halton_offsets, the batched form
+ *                  evaluates the same points)
+ * Other arguments and results as mck_slave_work.
+ */
+int mck_slave_work_sequential(int32_t elem_type, int32_t nvpe, int32_t nelems, int32_t nbasis,
+                              int32_t nvertices, const double *vertices, const int32_t *conn,
+                              double *kernel_values, double *kernel_variance,
+                              int32_t *niterations, double *computation_time,
+                              int64_t *halton_step, int64_t *offsets_used);
+
+/*
  * Point location, replacing the front half of load_fw_points / load_bw_points
  * (readfields.f90:1192-1226, 1359-1418) and find_element (readfields.f90:1052-1134).
  *  frame      0 = fwd (source frame), 1+r = bwd frame of receiver r

@@ -81,10 +81,12 @@ class MCKernel:
             self.lib.mck_finalize()
             self.lib = None
 
-    def slave_work(self, verts, conn, halton_offsets=None, plain=False, nbasis=1):
+    def slave_work(self, verts, conn, halton_offsets=None, plain=False, nbasis=1, sequential=False):
         """mck_slave_work for a task of tetrahedra (conn (ne, 4)) or voxels (conn (ne, 8));
         returns a dict of numpy arrays.  values/variance are (ne, nk) for volumetric
-        elements (nbasis 1) and (ne, nbasis, nk) for basis functions on the vertices."""
+        elements (nbasis 1) and (ne, nbasis, nk) for basis functions on the vertices.
+        plain: mck_slave_work_plain; sequential: mck_slave_work_sequential (the reference's
+        own stream from self.halton_step, each element's start returned as "offsets")."""
         verts = np.ascontiguousarray(verts, np.float64)
         conn = np.ascontiguousarray(conn, np.int32)
         ne, nvpe = conn.shape
@@ -96,14 +98,23 @@ class MCKernel:
         tm = np.zeros(ne)
         step = np.array([self.halton_step], np.int64)
         offs = None if halton_offsets is None else np.ascontiguousarray(halton_offsets, np.int64)
-        fn = self.lib.mck_slave_work_plain if plain else self.lib.mck_slave_work
+        if sequential:
+            if offs is not None:
+                raise ValueError("the sequential stream takes no halton_offsets")
+            offs = np.zeros(ne, np.int64)
+            fn = self.lib.mck_slave_work_sequential
+        else:
+            fn = self.lib.mck_slave_work_plain if plain else self.lib.mck_slave_work
         _check(fn(etype, nvpe, ne, nbasis, verts.shape[0], abi.ptr(verts, C.c_double),
                   abi.ptr(conn, C.c_int32), abi.ptr(vals, C.c_double), abi.ptr(var, C.c_double),
                   abi.ptr(nit, C.c_int32), abi.ptr(tm, C.c_double), abi.ptr(step, C.c_int64),
                   abi.ptr(offs, C.c_int64)))
         self.halton_step = int(step[0])
-        return dict(values=vals, variance=var, niterations=nit, computation_time=tm,
-                    halton_step=int(step[0]), stats=self.stats())
+        out = dict(values=vals, variance=var, niterations=nit, computation_time=tm,
+                   halton_step=int(step[0]), stats=self.stats())
+        if sequential:
+            out["offsets"] = offs
+        return out
 
     def stats(self):
         s = abi.MckStats()

@@ -97,7 +97,7 @@ def ptr(arr, ctype):
 
 # exported symbols of libmck.so, in the order of include/mck.h
 EXPORTS = [
-    "mck_init", "mck_slave_work", "mck_slave_work_plain", "mck_locate_points",
+    "mck_init", "mck_slave_work", "mck_slave_work_plain", "mck_slave_work_sequential", "mck_locate_points",
     "mck_load_strain_points", "mck_base_kernels", "mck_seismograms", "mck_integrate_model", "mck_set_hetero", "mck_integrate_hetero", "mck_ad_dims", "mck_ad_transfer",
     "mck_ad_forward", "mck_ad_backward", "mck_results_init", "mck_results_add", "mck_results_add_noalloc",
     "mck_results_get",
@@ -124,6 +124,8 @@ def declare(lib):
     lib.mck_slave_work.restype = C.c_int
     lib.mck_slave_work_plain.argtypes = work
     lib.mck_slave_work_plain.restype = C.c_int
+    lib.mck_slave_work_sequential.argtypes = work
+    lib.mck_slave_work_sequential.restype = C.c_int
     lib.mck_locate_points.argtypes = [C.c_int32, C.c_int32, f64p, i32p, f64p, f64p, i32p]
     lib.mck_locate_points.restype = C.c_int
     lib.mck_load_strain_points.argtypes = [C.c_int32, C.c_int32, f64p, f32p]

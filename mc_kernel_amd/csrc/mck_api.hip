@@ -1658,6 +1658,10 @@ struct Task {
     std::vector<double> vol, allowed_var;
     std::vector<uint8_t> masked, live;
     std::vector<int64_t> hbase;
+    // the reference's sequential stream (mck_slave_work_sequential): each element starts
+    // where the previous one's draws ended; seq_end is the position after the task
+    bool sequential = false;
+    int64_t seq_end = 0;
 };
 
 int prepare_task(State &G, Task &T, int elem_type, int nvpe, int nelems, int nbasis, int nvert,
@@ -1959,29 +1963,18 @@ int finish_task(State &G, const Task &T, double *values, double *variance, int32
     if (err & 4) return fail(MCK_EINVAL, "basis-function weight outside [0, 1]: point outside its element");
     long long tot_steps = 0;
     for (int e = 0; e < ne; ++e) tot_steps += steps[e];
+    // skipped slots (padding, the source's element) are zeroed with the whole task at its
+    // start and never written: the device arrays and their host copies hold zeros there
     G.last_ctime.assign(ne, 0.0);
-    for (int e = 0; e < ne; ++e) {
-        if (!T.live[e]) {
-            for (int k = 0; k < nb * nk; ++k) { values[(size_t)e * nb * nk + k] = 0; variance[(size_t)e * nb * nk + k] = 0; }
-            for (int k = 0; k < nk; ++k) niterations[(size_t)e * nk + k] = 0;
-            continue;
-        }
-        G.last_ctime[e] = tot_steps ? dev_ms * 1e-3 * steps[e] / (double)tot_steps : 0.0;
-    }
+    for (int e = 0; e < ne; ++e)
+        if (T.live[e]) G.last_ctime[e] = tot_steps ? dev_ms * 1e-3 * steps[e] / (double)tot_steps : 0.0;
     if (ctime) std::copy(G.last_ctime.begin(), G.last_ctime.end(), ctime);
-    // skipped slots hold zeros in device memory too, as the host arrays above do, for
-    // mck_results_add's device-resident path
-    for (int e = 0; e < ne; ++e) {
-        if (T.live[e]) continue;
-        HIPCHECK(hipMemsetAsync(G.integral.p + (size_t)e * nb * nk, 0, (size_t)nb * nk * 8, st));
-        HIPCHECK(hipMemsetAsync(G.variance.p + (size_t)e * nb * nk, 0, (size_t)nb * nk * 8, st));
-        HIPCHECK(hipMemsetAsync(G.niter.p + (size_t)e * nk, 0, (size_t)nk * 4, st));
-    }
-    HIPCHECK(hipStreamSynchronize(st));
     G.last_nelems = ne;
     G.last_nbasis = nb;
     // the Halton stream position after the task
-    if (halton_offsets) {
+    if (T.sequential) {
+        *halton_step = T.seq_end;
+    } else if (halton_offsets) {
         int last = -1;
         for (int e = 0; e < ne; ++e) if (T.live[e]) last = e;
         if (last >= 0) *halton_step = T.hbase[last] + (int64_t)draws[last] * T.P;
@@ -1991,10 +1984,15 @@ int finish_task(State &G, const Task &T, double *values, double *variance, int32
     return MCK_OK;
 }
 
-int slave_work_impl(bool plain, int32_t elem_type, int32_t nvpe, int32_t nelems, int32_t nbasis,
-                    int32_t nvertices, const double *vertices, const int32_t *conn, double *values,
-                    double *variance, int32_t *niterations, double *ctime, int64_t *halton_step,
-                    const int64_t *halton_offsets) {
+// One task in the batched form, or (plain) in the reference's loop order: one element at
+// a time, its MC steps one after another.  sequential (plain only): the reference's own
+// Halton stream, halton_sequence.f90:109-153 with one global position that every draw
+// advances (slave.f90:506-511 draws before the max_iter exit): element e starts where
+// element e-1's draws ended; offsets_out (nullable) receives every element's start.
+int slave_work_body(Task &T, bool plain, bool sequential, int32_t elem_type, int32_t nvpe, int32_t nelems,
+                    int32_t nbasis, int32_t nvertices, const double *vertices, const int32_t *conn,
+                    double *values, double *variance, int32_t *niterations, double *ctime,
+                    int64_t *halton_step, const int64_t *halton_offsets, int64_t *offsets_out) {
     State &G = g;
     if (!G.init) return fail(MCK_ENOTINIT, "mck_init has not been called");
     if (!vertices || !conn || !values || !variance || !niterations || !halton_step)
@@ -2003,11 +2001,12 @@ int slave_work_impl(bool plain, int32_t elem_type, int32_t nvpe, int32_t nelems,
     G.stats = mck_stats_t{};
     G.stats_vk = 0;
     G.last_nelems = G.last_nbasis = 0;
-    Task T;
     int rc;
     if ((rc = prepare_task(G, T, elem_type, nvpe, nelems, nbasis, nvertices, vertices, conn, halton_step,
                            halton_offsets)))
         return rc;
+    T.sequential = sequential;
+    T.seq_end = *halton_step;
     hipEvent_t e0, e1;
     HIPCHECK(hipEventCreate(&e0));
     HIPCHECK(hipEventCreate(&e1));
@@ -2024,15 +2023,27 @@ int slave_work_impl(bool plain, int32_t elem_type, int32_t nvpe, int32_t nelems,
         // reference loop order: one element at a time, one MC step at a time
         std::vector<uint8_t> only(nelems, 0);
         for (int e = 0; e < nelems; ++e) {
+            if (sequential) {
+                T.hbase[e] = T.seq_end;
+                if (offsets_out) offsets_out[e] = T.seq_end;
+            }
             if (!T.live[e]) continue;
             only.assign(nelems, 0);
             only[e] = 1;
             HIPCHECK(hipMemcpyAsync(G.live.p, only.data(), nelems, hipMemcpyHostToDevice, G.stream));
+            if (sequential)
+                HIPCHECK(hipMemcpyAsync(G.hbase.p + e, &T.hbase[e], 8, hipMemcpyHostToDevice, G.stream));
             for (;;) {
                 int nwork = 0;
                 if ((rc = mc_step(G, T, &nwork, evs))) return rc;
                 if ((rc = collect_events(G, evs))) return rc;
                 if (nwork == 0) break;
+            }
+            if (sequential) {
+                int d = 0;
+                HIPCHECK(hipMemcpyAsync(&d, G.draws.p + e, 4, hipMemcpyDeviceToHost, G.stream));
+                HIPCHECK(hipStreamSynchronize(G.stream));
+                T.seq_end += (int64_t)d * T.P;
             }
         }
     }
@@ -2046,6 +2057,20 @@ int slave_work_impl(bool plain, int32_t elem_type, int32_t nvpe, int32_t nelems,
     if ((rc = finish_task(G, T, values, variance, niterations, ctime, halton_step, halton_offsets, ms))) return rc;
     G.stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return MCK_OK;
+}
+
+int slave_work_impl(bool plain, bool sequential, int32_t elem_type, int32_t nvpe, int32_t nelems, int32_t nbasis,
+                    int32_t nvertices, const double *vertices, const int32_t *conn, double *values,
+                    double *variance, int32_t *niterations, double *ctime, int64_t *halton_step,
+                    const int64_t *halton_offsets, int64_t *offsets_out) {
+    Task T;
+    const int rc = slave_work_body(T, plain, sequential, elem_type, nvpe, nelems, nbasis, nvertices, vertices,
+                                   conn, values, variance, niterations, ctime, halton_step, halton_offsets,
+                                   offsets_out);
+    // an error return can leave copies from the task's host arrays in flight: let them
+    // end before the arrays go
+    if (rc && g.init) (void)hipStreamSynchronize(g.stream);
+    return rc;
 }
 
 // point-level evaluation shared by mck_locate_points / mck_load_strain_points /
@@ -2260,7 +2285,11 @@ int integrate_task(bool hetero, int32_t elem_type, int32_t nvpe, int32_t nelems,
     if (e == hipSuccess) e = hipStreamSynchronize(G.stream);
     int err = 0;
     if (e == hipSuccess) e = hipMemcpy(&err, G.err.p, 4, hipMemcpyDeviceToHost);
-    if (e != hipSuccess) return fail(MCK_EHIP, std::string(hetero ? "mck_integrate_hetero: " : "mck_integrate_model: ") + hipGetErrorString(e));
+    if (e != hipSuccess) {
+        // copies from the local arrays may still be in flight
+        (void)hipStreamSynchronize(G.stream);
+        return fail(MCK_EHIP, std::string(hetero ? "mck_integrate_hetero: " : "mck_integrate_model: ") + hipGetErrorString(e));
+    }
     if (err & 4) return fail(MCK_EINVAL, "basis function weight outside [0, 1] (point outside its element)");
     if (err & 8) return fail(MCK_ENAN, "NaN in a model integral (mc_integration.f90:113-130)");
     if (nsteps) std::copy(ns.begin(), ns.end(), nsteps);
@@ -2288,8 +2317,8 @@ int mck_slave_work(int32_t elem_type, int32_t nvpe, int32_t nelems, int32_t nbas
                    const double *vertices, const int32_t *conn, double *kernel_values,
                    double *kernel_variance, int32_t *niterations, double *computation_time,
                    int64_t *halton_step, const int64_t *halton_offsets) {
-    return slave_work_impl(false, elem_type, nvpe, nelems, nbasis, nvertices, vertices, conn, kernel_values,
-                           kernel_variance, niterations, computation_time, halton_step, halton_offsets);
+    return slave_work_impl(false, false, elem_type, nvpe, nelems, nbasis, nvertices, vertices, conn, kernel_values,
+                           kernel_variance, niterations, computation_time, halton_step, halton_offsets, nullptr);
 }
 
 int mck_slave_work_plain(int32_t elem_type, int32_t nvpe, int32_t nelems, int32_t nbasis,
@@ -2297,8 +2326,16 @@ int mck_slave_work_plain(int32_t elem_type, int32_t nvpe, int32_t nelems, int32_
                          double *kernel_values, double *kernel_variance, int32_t *niterations,
                          double *computation_time, int64_t *halton_step,
                          const int64_t *halton_offsets) {
-    return slave_work_impl(true, elem_type, nvpe, nelems, nbasis, nvertices, vertices, conn, kernel_values,
-                           kernel_variance, niterations, computation_time, halton_step, halton_offsets);
+    return slave_work_impl(true, false, elem_type, nvpe, nelems, nbasis, nvertices, vertices, conn, kernel_values,
+                           kernel_variance, niterations, computation_time, halton_step, halton_offsets, nullptr);
+}
+
+int mck_slave_work_sequential(int32_t elem_type, int32_t nvpe, int32_t nelems, int32_t nbasis,
+                              int32_t nvertices, const double *vertices, const int32_t *conn,
+                              double *kernel_values, double *kernel_variance, int32_t *niterations,
+                              double *computation_time, int64_t *halton_step, int64_t *offsets_used) {
+    return slave_work_impl(true, true, elem_type, nvpe, nelems, nbasis, nvertices, vertices, conn, kernel_values,
+                           kernel_variance, niterations, computation_time, halton_step, nullptr, offsets_used);
 }
 
 int mck_locate_points(int32_t frame, int32_t npts, const double *xyz, int32_t *elem, double *xi,

@@ -77,7 +77,7 @@ module mck_gpu
      real(c_double) :: spectra_ms
   end type mck_stats_t
 
-  public :: mck_init, mck_slave_work, mck_slave_work_plain, mck_locate_points, &
+  public :: mck_init, mck_slave_work, mck_slave_work_plain, mck_slave_work_sequential, mck_locate_points, &
             mck_load_strain_points, mck_base_kernels, mck_seismograms, mck_integrate_model, &
             mck_set_hetero, mck_integrate_hetero, &
             mck_results_init, mck_results_add, mck_results_add_noalloc, mck_results_get, mck_results_load, &
@@ -118,6 +118,23 @@ module mck_gpu
        integer(c_int64_t), intent(inout) :: halton_step
        type(c_ptr), value :: halton_offsets
      end function mck_slave_work_plain
+
+     ! the reference's own Halton stream: element e starts where element e-1's draws
+     ! ended; offsets_used (c_null_ptr or c_loc of an int64 array(nelems)) receives the
+     ! starts, halton_step the position after the task
+     integer(c_int) function mck_slave_work_sequential(elem_type, nvpe, nelems, nbasis, nvertices, &
+          vertices, conn, kernel_values, kernel_variance, niterations, computation_time, &
+          halton_step, offsets_used) bind(c, name='mck_slave_work_sequential')
+       import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+       integer(c_int32_t), value :: elem_type, nvpe, nelems, nbasis, nvertices
+       real(c_double), intent(in) :: vertices(*)
+       integer(c_int32_t), intent(in) :: conn(*)
+       real(c_double), intent(out) :: kernel_values(*), kernel_variance(*)
+       integer(c_int32_t), intent(out) :: niterations(*)
+       real(c_double), intent(out) :: computation_time(*)
+       integer(c_int64_t), intent(inout) :: halton_step
+       type(c_ptr), value :: offsets_used
+     end function mck_slave_work_sequential
 
      integer(c_int) function mck_locate_points(frame, npts, xyz, elem, xi, eta, ties) &
           bind(c, name='mck_locate_points')

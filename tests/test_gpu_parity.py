@@ -80,6 +80,37 @@ def test_slave_work_pinned(tiny, gpu):
         assert err < TOL, (k, err)
 
 
+@pytest.mark.parametrize("start", [0, 12345])
+def test_sequential_stream_is_the_reference_stream(tiny, gpu, start):
+    """mck_slave_work_sequential walks the reference's own Halton stream (one position
+    advanced by every draw, halton_sequence.f90:151, slave.f90:506-511) without positions
+    computed elsewhere: the same element starts, iteration counts, evaluations and end
+    position as the oracle's sequential loop; the batched form on those starts gives the
+    same results bit for bit."""
+    from oracle import oracle
+    o = oracle.Oracle(tiny.setup)
+    o.halton_step = start
+    ref = o.slave_work(tiny.verts, tiny.conn)
+    gpu.halton_step = start
+    seq = gpu.slave_work(tiny.verts, tiny.conn, sequential=True)
+    assert np.array_equal(seq["offsets"], ref["offsets"])
+    assert seq["halton_step"] == ref["halton_step"] > start
+    assert np.array_equal(seq["niterations"], ref["niterations"])
+    assert seq["stats"]["evaluations"] == ref["evaluations"]
+    for k in range(tiny.nkernel):
+        r = ref["values"][:, k]
+        nrm = np.linalg.norm(r)
+        if nrm == 0:
+            assert not np.any(seq["values"][:, k])
+            continue
+        assert np.linalg.norm(seq["values"][:, k] - r) / nrm < TOL, k
+    b = gpu.slave_work(tiny.verts, tiny.conn, halton_offsets=seq["offsets"])
+    assert np.array_equal(b["values"], seq["values"])
+    assert np.array_equal(b["variance"], seq["variance"])
+    assert np.array_equal(b["niterations"], seq["niterations"])
+    assert b["halton_step"] == seq["halton_step"]
+
+
 def test_plain_equals_batched(tiny, gpu):
     gpu.halton_step = 0
     a = gpu.slave_work(tiny.verts, tiny.conn)
