@@ -196,19 +196,22 @@ def measured_traffic(path, kernel):
     return None
 
 
-def measured_valu_busy(path, kernel):
-    """VALU busy fraction of `kernel` from the committed counter summary (the row
-    '| <kernel> | VALU per wave | SALU | LDS | busy % | ...' of its derived table)."""
+def measured_counters(path, kernel):
+    """Issue rates of `kernel` from the committed counter summary: the row
+    '| <kernel> | VALU busy % | MFMA busy % | VALU instructions per wave | MFMA instructions
+    per wave |' of its derived table."""
     if not os.path.exists(path):
         return None
+    keys = ("valu_busy", "mfma_busy", "valu_insts_per_wave", "mfma_insts_per_wave")
     with open(path) as f:
         for ln in f:
             cells = [c.strip() for c in ln.strip().strip("|").split("|")]
             if len(cells) >= 5 and cells[0] == kernel:
                 try:
-                    return round(float(cells[4]) / 100.0, 3)
+                    v = [float(c) for c in cells[1:5]]
                 except ValueError:
                     continue
+                return {keys[0]: round(v[0] / 100, 3), keys[1]: round(v[1] / 100, 3), keys[2]: v[2], keys[3]: v[3]}
     return None
 
 
@@ -514,8 +517,19 @@ def main():
     bwd_ms = sum(s[1]["strain_bwd_ms"] for s in steps)
     launches = sum(s[1]["strain_launches"] for s in steps) // 2
     dev_ms = sum(s[1]["device_ms"] for s in steps)
-    if bwd_ms >= fwd_ms:
-        kern, byt, ms = "k_bwd_eval", bwd_b, bwd_ms
+    sp_ms = sum(s[1]["spectra_ms"] for s in steps)
+    mf_ms = sum(s[1]["mfma_ms"] for s in steps)
+    mf_launches = sum(s[1]["mfma_launches"] for s in steps)
+    mfma_path = all(s[1]["bwd_path"] == 1 for s in steps)
+    # the backward stage dominates: with the matrix-core path it is two kernels, the
+    # element spectra (which read the merged blocks) and the matrix-core evaluation; its
+    # algorithmic bytes are the backward blocks of every receiver's element set
+    if mfma_path:
+        bwd_kern, bwd_kms = "k_elem_spectra+k_bwd_mfma", sp_ms + mf_ms
+    else:
+        bwd_kern, bwd_kms = "k_bwd_eval", bwd_ms
+    if bwd_kms >= fwd_ms:
+        kern, byt, ms = bwd_kern, bwd_b, bwd_kms
     else:
         kern, byt, ms = "k_fwd_spectra", fwd_b, fwd_ms
     achieved = (byt / launches) / (ms / launches * 1e-3) / 1e9 if ms > 0 else 0.0
@@ -524,18 +538,27 @@ def main():
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kern,
             "bytes_per_launch": byt / max(launches, 1), "avg_launch_ms": ms / max(launches, 1),
+            "launches": launches,
             "algorithmic_bytes": "merged blocks once per launch: fwd 4 nt 25 x 10 variables per distinct "
                                  "forward element (+ 48 B per point), bwd 4 nt 25 x 5 variables per distinct "
                                  "(receiver, backward element)",
-            "alg_bytes_whole_job": sum(s[1]["alg_bytes"] for s in steps) / len(steps)}
+            "alg_bytes_whole_job": sum(s[1]["alg_bytes"] for s in steps) / len(steps),
+            "timing": "HIP events on the library's stream around each kernel of every launch"}
+    if mfma_path:
+        roof["kernels"] = {
+            "k_elem_spectra": {"avg_launch_ms": sp_ms / max(mf_launches, 1), "launches": mf_launches},
+            "k_bwd_mfma": {"avg_launch_ms": mf_ms / max(mf_launches, 1), "launches": mf_launches,
+                           "waves_per_launch": sum(s[1]["mfma_groups"] for s in steps) / max(mf_launches, 1)},
+            "k_fwd_spectra": {"avg_launch_ms": fwd_ms / max(launches, 1), "launches": launches},
+            "backward_stage_ms_per_launch": bwd_ms / max(launches, 1)}
     if traffic:
-        # the counter-measured HBM bytes of the same kernel over its launch time
+        # the counter-measured HBM bytes of the same kernels over their launch time
         roof["measured_frac"] = round(traffic / (ms / launches * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     if args.config == "c1":
-        # what does bound the kernel: its VALU issue rate, from the committed counter run
-        busy = measured_valu_busy(os.path.join(ROOT, "profiles", "c1_valu_counters.md"), kern)
-        if busy is not None:
-            roof["valu_busy"] = busy
+        # what does bound the matrix-core kernel: its issue rates, from the committed counters
+        busy = measured_counters(os.path.join(ROOT, "profiles", "c1_bwd_mfma_counters.md"), "k_bwd_mfma")
+        if busy:
+            roof["k_bwd_mfma_counters"] = busy
 
     curve = None
     if args.task_curve:

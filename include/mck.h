@@ -203,6 +203,10 @@ typedef struct {
                                /* + 24 V K (merged blocks once per launch and receiver,  */
                                /* point coordinates, MC state of the evaluated voxels)   */
     double  spectra_ms;        /* summed time of the element-spectra kernels (path 1)    */
+    double  mfma_ms;           /* summed time of the matrix-core evaluation kernels      */
+                               /* (path 1); both timed with events on the library stream */
+    int32_t mfma_launches;     /* launches of the matrix-core evaluation kernel          */
+    int32_t mfma_groups;       /* waves (groups of 16 pairs) over those launches         */
 } mck_stats_t;
 
 /*

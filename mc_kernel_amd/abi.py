@@ -84,7 +84,8 @@ class MckStats(C.Structure):
         ("paged_bytes", C.c_int64),
         ("spt", C.c_int32), ("nd_f", C.c_int32), ("nd_b", C.c_int32), ("bwd_path", C.c_int32),
         ("fwd_elements", C.c_int64), ("bwd_elem_receivers", C.c_int64), ("alg_bytes", C.c_int64),
-        ("spectra_ms", C.c_double),
+        ("spectra_ms", C.c_double), ("mfma_ms", C.c_double),
+        ("mfma_launches", C.c_int32), ("mfma_groups", C.c_int32),
     ]
 
 

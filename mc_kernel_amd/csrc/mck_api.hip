@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <initializer_list>
+#include <array>
 #include <map>
 #include <string>
 #include <tuple>
@@ -390,6 +391,9 @@ struct State {
     DBuf<int> worktab;
     DBuf<float2> Uspec, Hp;                       // element spectra; window spectra, rows fstride
     bool mfma_ok = false, mfma_ok_base = false;   // kernel sets the matrix-core path serves
+    // events around the element-spectra and matrix-core launches of the current step
+    // (start, after the spectra, after the matrix-core kernel), read by collect_events
+    std::vector<std::array<hipEvent_t, 3>> kev;
     int bwd_path_env = -1;                        // MCK_BWD_PATH: 0 per-pair, 1 matrix cores
     int mfma_xcd = 0;                             // MCK_MFMA_XCD: groups in XCD-contiguous ranges
     size_t uspec_budget = 0;                      // bytes of element spectra per batch
@@ -1064,10 +1068,15 @@ int launch_bwd_mfma(State &G, const DevDb &db, int npairs, const int *keys, cons
                                nerun, erun_elem, G.twq_d.p, G.Uspec.p);                           \
             launched = true;                                                                      \
         }
+        std::array<hipEvent_t, 3> ev{};
+        for (auto &e : ev) HIPCHECK(hipEventCreate(&e));
+        G.kev.push_back(ev);
+        HIPCHECK(hipEventRecord(ev[0], st));
         SPEC(1) SPEC(2) SPEC(4) SPEC(8) SPEC(16)
 #undef SPEC
         if (!launched) return fail(MCK_EINVAL, "no element-spectra kernel for this trace length");
         HIPCHECK(hipGetLastError());
+        HIPCHECK(hipEventRecord(ev[1], st));
         launched = false;
 #define EVAL(NF, NB, NV)                                                                          \
         if (pc.nd_f == NF && pc.nd_b == NB && pc.nvar_bwd == NV) {                                \
@@ -1085,6 +1094,9 @@ int launch_bwd_mfma(State &G, const DevDb &db, int npairs, const int *keys, cons
 #undef EVAL
         if (!launched) return fail(MCK_EINVAL, "no matrix-core kernel for these strain components");
         HIPCHECK(hipGetLastError());
+        HIPCHECK(hipEventRecord(ev[2], st));
+        G.stats.mfma_launches += 1;
+        G.stats.mfma_groups += ngroups;
     }
     return MCK_OK;
 }
@@ -1878,6 +1890,27 @@ int eval_chunk(State &G, int P, int w0, int nw, std::vector<std::pair<hipEvent_t
     return MCK_OK;
 }
 
+// the element-spectra and matrix-core times of the launches so far (their events are
+// then released); keep = false releases them unread, after an error
+int fold_kernel_events(State &G, bool keep = true) {
+    int rc = MCK_OK;
+    for (auto &ev : G.kev) {
+        float a = 0.f, b = 0.f;
+        if (keep && rc == MCK_OK) {
+            hipError_t e = hipEventSynchronize(ev[2]);
+            if (e == hipSuccess) e = hipEventElapsedTime(&a, ev[0], ev[1]);
+            if (e == hipSuccess) e = hipEventElapsedTime(&b, ev[1], ev[2]);
+            if (e != hipSuccess) rc = fail(MCK_EHIP, hipGetErrorString(e));
+            G.stats.spectra_ms += a;
+            G.stats.mfma_ms += b;
+        }
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    G.kev.clear();
+    return rc;
+}
+
 int collect_events(State &G, std::vector<std::pair<hipEvent_t, hipEvent_t>> &evs) {
     for (size_t i = 0; i < evs.size(); ++i) {
         float ms = 0.f;
@@ -1888,7 +1921,7 @@ int collect_events(State &G, std::vector<std::pair<hipEvent_t, hipEvent_t>> &evs
         (void)hipEventDestroy(evs[i].second);
     }
     evs.clear();
-    return MCK_OK;
+    return fold_kernel_events(G);
 }
 
 // One MC step of every voxel that is still iterating.  Returns the number of voxels
@@ -1999,6 +2032,7 @@ int slave_work_body(Task &T, bool plain, bool sequential, int32_t elem_type, int
         return fail(MCK_EINVAL, "null argument");
     auto t0 = std::chrono::steady_clock::now();
     G.stats = mck_stats_t{};
+    (void)fold_kernel_events(G, false);   // events an earlier failed call left
     G.stats_vk = 0;
     G.last_nelems = G.last_nbasis = 0;
     int rc;
@@ -2452,6 +2486,10 @@ int mck_base_kernels(int32_t npts, const double *xyz, double *base) {
     if (rc == MCK_OK) e = hipMemcpyAsync(base, d_base, (size_t)npts * G.nk * 6 * 8, hipMemcpyDeviceToHost, st);
     if (rc == MCK_OK && e == hipSuccess) e = hipStreamSynchronize(st);
     else (void)hipStreamSynchronize(st);
+    {
+        const int r3 = fold_kernel_events(G, rc == MCK_OK && e == hipSuccess);
+        if (rc == MCK_OK && e == hipSuccess && r3) rc = r3;
+    }
     (void)hipFree(d_base);
     (void)hipFree(d_phys);
     if (rc) return rc;
@@ -3003,6 +3041,7 @@ int mck_get_stats(mck_stats_t *stats) {
 void mck_finalize(void) {
     State &G = g;
     if (G.stream) (void)hipStreamSynchronize(G.stream);
+    (void)fold_kernel_events(G, false);
     for (DbDev *D : {&G.fwd, &G.bwd}) {
         if (D->u_owned) (void)hipFree(D->u_owned);
         D->u_owned = nullptr;

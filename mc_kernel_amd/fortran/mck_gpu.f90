@@ -74,7 +74,8 @@ module mck_gpu
      integer(c_int64_t) :: paged_bytes
      integer(c_int32_t) :: spt, nd_f, nd_b, bwd_path
      integer(c_int64_t) :: fwd_elements, bwd_elem_receivers, alg_bytes
-     real(c_double) :: spectra_ms
+     real(c_double) :: spectra_ms, mfma_ms
+     integer(c_int32_t) :: mfma_launches, mfma_groups
   end type mck_stats_t
 
   public :: mck_init, mck_slave_work, mck_slave_work_plain, mck_slave_work_sequential, mck_locate_points, &

@@ -26,6 +26,7 @@ program layout_check
   write(*, '(a, 1x, i0)') 'mck_stats_t paged_bytes', off(c_loc(st%paged_bytes), c_loc(st))
   write(*, '(a, 1x, i0)') 'mck_stats_t alg_bytes', off(c_loc(st%alg_bytes), c_loc(st))
   write(*, '(a, 1x, i0)') 'mck_stats_t spectra_ms', off(c_loc(st%spectra_ms), c_loc(st))
+  write(*, '(a, 1x, i0)') 'mck_stats_t mfma_groups', off(c_loc(st%mfma_groups), c_loc(st))
   write(*, '(a, 1x, a)') 'last_error', mck_error_message()
 contains
   integer function off(a, b)
