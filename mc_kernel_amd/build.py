@@ -121,6 +121,58 @@ def build_dbio(force=True):
     return out
 
 
+def flang():
+    return shutil.which("flang") or shutil.which("amdflang") or (
+        os.path.join(ROCM, "llvm", "bin", "amdflang") if os.path.exists(os.path.join(ROCM, "llvm", "bin", "amdflang"))
+        else None)
+
+
+def build_fortran_e2e(force=True):
+    """tests/fortran/slave_e2e: a Fortran program that drives libmck.so through the
+    iso_c_binding module (mc_kernel_amd/fortran/mck_gpu.f90), test infrastructure of the
+    Fortran binding; skipped when no Fortran compiler is found."""
+    fc = flang()
+    if fc is None:
+        print("slave_e2e skipped: no Fortran compiler", file=sys.stderr)
+        return None
+    mod = os.path.join(HERE, "fortran", "mck_gpu.f90")
+    prog = os.path.join(ROOT, "tests", "fortran", "slave_e2e.f90")
+    out = os.path.join(ROOT, "tests", "fortran", "slave_e2e")
+    if force or _stale(out, [mod, prog]):
+        bdir = os.path.join(HERE, "_build", "fortran")
+        os.makedirs(bdir, exist_ok=True)
+        obj = os.path.join(bdir, "mck_gpu.o")
+        _run([fc, "-O1", "-c", mod, "-o", obj, "-module-dir", bdir])
+        _run([fc, "-O1", prog, obj, "-I", bdir, "-L" + HERE, "-lmck", "-Wl,-rpath,$ORIGIN/../../mc_kernel_amd",
+              "-o", out])
+    return out
+
+
+# host code only (gcc / g++, no HIP source on these lines)
+SANITIZE = ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+            "-fno-sanitize-recover=undefined"]
+
+
+def build_sanitized(outdir):
+    """The oracle and libmck_dbio.so built with AddressSanitizer and
+    UndefinedBehaviorSanitizer into outdir (test infrastructure); returns their paths
+    (the second None without HDF5)."""
+    os.makedirs(outdir, exist_ok=True)
+    odir = os.path.join(ROOT, "oracle")
+    osrc = [os.path.join(odir, f) for f in ("mck_oracle_core.c", "mck_oracle_path.c", "mck_oracle_results.c")]
+    ora = os.path.join(outdir, "liboracle_san.so")
+    _run(["gcc"] + SANITIZE + ["-std=gnu11", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
+                               "-fexcess-precision=standard"] + osrc + ["-o", ora, "-lm"])
+    dbio = None
+    inc, lib = os.path.join(HDF5_PREFIX, "include"), os.path.join(HDF5_PREFIX, "lib")
+    if os.path.exists(os.path.join(inc, "hdf5.h")):
+        dbio = os.path.join(outdir, "libmck_dbio_san.so")
+        _run(["g++"] + SANITIZE + ["-std=c++17", "-fPIC", "-shared", "-I", os.path.join(ROOT, "include"), "-I", inc,
+                                   os.path.join(CSRC, "mck_dbio.cpp"), "-o", dbio, os.path.join(lib, "libhdf5.so"),
+                                   "-Wl,-rpath," + lib])
+    return ora, dbio
+
+
 def build_all(force=True):
     build_synth(force)
     build_oracle(force)
@@ -131,6 +183,7 @@ def build_all(force=True):
     build_hip(force)
     build_synth_gpu(force)
     build_dbio(force)
+    build_fortran_e2e(force)
 
 
 if __name__ == "__main__":

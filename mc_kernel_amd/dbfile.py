@@ -31,7 +31,8 @@ EXPORTS = ["mck_db_open", "mck_db_close", "mck_db_write", "mck_dbio_last_error"]
 
 
 def library_path():
-    return os.path.join(HERE, "libmck_dbio.so")
+    # MCK_DBIO_LIB: another build of the same sources (the sanitizer build of the tests)
+    return os.environ.get("MCK_DBIO_LIB") or os.path.join(HERE, "libmck_dbio.so")
 
 
 def load_library():

@@ -18,7 +18,8 @@ def lib():
     global _LIB
     if _LIB is None:
         from mc_kernel_amd import build
-        path = build.build_oracle(force=False)      # rebuilds only when sources are newer
+        # MCK_ORACLE_LIB: another build of the same sources (the sanitizer build)
+        path = os.environ.get("MCK_ORACLE_LIB") or build.build_oracle(force=False)
         L = C.CDLL(path)
         f64p, f32p, i32p, i64p = abi.f64p, abi.f32p, abi.i32p, abi.i64p
         L.ora_gll_points.argtypes = [C.c_int, f64p]
