@@ -86,6 +86,77 @@ def _cpu_worker(args):
     return r["evaluations"], time.perf_counter() - t0, voxels, r["values"], r["niterations"]
 
 
+def c4_task(layer, nvox):
+    """The c4 problem and its task: the nvox voxels of the 1.25 deg grid's layer `layer`
+    nearest to the epicentre (corners [8 nvox][3], connectivity [nvox][8])."""
+    from mc_kernel_amd import synth
+    p = synth.make_problem("c4")
+    v, c, L, _ = synth.voxel_grid()
+    sel = synth.voxel_slice(v, c, L, layer, p.src_lat, p.src_lon, nvox)
+    verts = np.ascontiguousarray(v.reshape(-1, 8, 3)[sel].reshape(-1, 3))
+    conn = np.arange(1, 8 * len(sel) + 1, dtype=np.int32).reshape(-1, 8)
+    return p, verts, conn
+
+
+def _cpu_worker_c4(args):
+    """One host core: the FP64 oracle on voxels of the c4 slice, its databases generated
+    element by element as they are touched into sparse host mappings; the same return
+    as _cpu_worker."""
+    layer, nvox, voxels, max_iter = args
+    sys.path.insert(0, ROOT)
+    from mc_kernel_amd import synth
+    from oracle import oracle
+    p, verts, conn = c4_task(layer, nvox)
+    p.max_iter = max_iter
+    p.setup.max_iter = max_iter
+    m = p.mesh
+    maps = []
+    for db, nvar in ((p.setup.fwd, 10), (p.setup.bwd, 5)):
+        n = m.nelem * nvar * 25 * p.nt * 4
+        a = synth.sparse_mapping(n)
+        maps.append((a, n))
+        db.snapshots = C.cast(C.c_void_p(a), C.POINTER(C.c_float))
+    lib = synth.synth_lib()
+    from mc_kernel_amd import abi
+    lib.synth_lazy_create.argtypes = [abi.i32p, C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                                      C.c_uint64, C.c_uint64]
+    lib.synth_lazy_create.restype = C.c_void_p
+    user = lib.synth_lazy_create(abi.ptr(m.sem, C.c_int32), 10, 5, p.nt, p.dt, 1001, 1002)
+    o = oracle.Oracle(p.setup)
+    o.set_lazy_fill(C.cast(lib.synth_fill_element, C.c_void_p), user)
+    sub = conn[voxels]
+    offs = np.asarray(voxels, np.int64) * ((max_iter + 1) * p.npoints_per_step)
+    o.slave_work(verts, sub, halton_offsets=offs)                # generates the touched elements
+    t0 = time.perf_counter()
+    r = o.slave_work(verts, sub, halton_offsets=offs)            # timed
+    dt = time.perf_counter() - t0
+    return r["evaluations"], dt, voxels, r["values"], r["niterations"]
+
+
+def cpu_baseline_c4(workers, layer, nvox, max_iter):
+    voxels = np.linspace(0, nvox - 1, workers).astype(int)
+    ctx = mp.get_context("spawn")
+    t0 = time.perf_counter()
+    with ctx.Pool(workers) as pool:
+        res = pool.map(_cpu_worker_c4, [(layer, nvox, np.array([v]), max_iter) for v in voxels])
+    wall = time.perf_counter() - t0
+    evals = sum(r[0] for r in res)
+    tmax = max(r[1] for r in res)
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count()
+    line = {"value": evals / tmax, "unit": "evals/s", "cores": workers, "kind": "port",
+            "sample": "%d voxels of the c4 slice (one per core), %d MC step(s) each (max_iter=%d), 16 points/step; "
+                      "FP64 oracle, one process per core, timed region = slave_work only" % (workers, max_iter, max_iter),
+            "seconds": tmax, "evaluations": int(evals), "total_wall_s": wall,
+            "nproc": os.cpu_count(), "affinity_cpus": affinity, "cpu_model": cpu_model()}
+    vox = np.concatenate([r[2] for r in res])
+    vals = np.concatenate([r[3] for r in res])
+    nit = np.concatenate([r[4] for r in res])
+    return line, (vox, vals, nit, max_iter)
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -394,6 +465,156 @@ def bench_integrals(args):
     print(json.dumps(line), flush=True)
 
 
+def roofline(steps, config, traffic_path):
+    """The roofline object of the dominant stage over the timed steps' statistics."""
+    launches = sum(s[1]["strain_launches"] for s in steps) // 2
+    fwd_b = sum(s[1]["fwd_bytes"] for s in steps)
+    bwd_b = sum(s[1]["bwd_bytes"] for s in steps)
+    fwd_ms = sum(s[1]["strain_fwd_ms"] for s in steps)
+    bwd_ms = sum(s[1]["strain_bwd_ms"] for s in steps)
+    sp_ms = sum(s[1]["spectra_ms"] for s in steps)
+    mf_ms = sum(s[1]["mfma_ms"] for s in steps)
+    mf_launches = sum(s[1]["mfma_launches"] for s in steps)
+    mfma_path = all(s[1]["bwd_path"] == 1 for s in steps)
+    # the backward stage dominates: with the matrix-core path it is two kernels, the
+    # element spectra (which read the merged blocks) and the matrix-core evaluation; its
+    # algorithmic bytes are the backward blocks of every receiver's element set
+    if mfma_path:
+        bwd_kern, bwd_kms = "k_elem_spectra+k_bwd_mfma", sp_ms + mf_ms
+    else:
+        bwd_kern, bwd_kms = "k_bwd_eval", bwd_ms
+    if bwd_kms >= fwd_ms:
+        kern, byt, ms = bwd_kern, bwd_b, bwd_kms
+    else:
+        kern, byt, ms = "k_fwd_spectra", fwd_b, fwd_ms
+    achieved = (byt / launches) / (ms / launches * 1e-3) / 1e9 if ms > 0 else 0.0
+    # the committed counter summary is of the c1 workload
+    traffic = measured_traffic(traffic_path, kern) if config == "c1" else None
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kern,
+            "bytes_per_launch": byt / max(launches, 1), "avg_launch_ms": ms / max(launches, 1),
+            "launches": launches,
+            "algorithmic_bytes": "merged blocks once per launch: fwd 4 nt 25 x 10 variables per distinct "
+                                 "forward element (+ 48 B per point), bwd 4 nt 25 x 5 variables per distinct "
+                                 "(receiver, backward element)",
+            "alg_bytes_whole_job": sum(s[1]["alg_bytes"] for s in steps) / len(steps),
+            "timing": "HIP events on the library's stream around each kernel of every launch"}
+    if mfma_path:
+        roof["kernels"] = {
+            "k_elem_spectra": {"avg_launch_ms": sp_ms / max(mf_launches, 1), "launches": mf_launches},
+            "k_bwd_mfma": {"avg_launch_ms": mf_ms / max(mf_launches, 1), "launches": mf_launches,
+                           "waves_per_launch": sum(s[1]["mfma_groups"] for s in steps) / max(mf_launches, 1)},
+            "k_fwd_spectra": {"avg_launch_ms": fwd_ms / max(launches, 1), "launches": launches},
+            "backward_stage_ms_per_launch": bwd_ms / max(launches, 1)}
+    if traffic:
+        # the counter-measured HBM bytes of the same kernels over their launch time
+        roof["measured_frac"] = round(traffic / (ms / launches * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    if config == "c1":
+        # what does bound the matrix-core kernel: its issue rates, from the committed counters
+        busy = measured_counters(os.path.join(ROOT, "profiles", "c1_bwd_mfma_counters.md"), "k_bwd_mfma")
+        if busy:
+            roof["k_bwd_mfma_counters"] = busy
+
+    return roof
+
+
+def bench_c4(args):
+    """c4: 200 receivers x 4 Gabor x CC x vp/vs (1600 kernels), 100,000-quad databases at
+    nt 4096 (614 GB, more than HBM and than the host memory a run may take).  Both
+    databases are reserved as sparse host mappings; the element pages that the voxel
+    slice can touch in any frame (synth.touched_elements) are generated there and the
+    library pages them (db_page_elems).  A step is the slice as one task."""
+    import mc_kernel_amd as mk
+    from mc_kernel_amd import synth
+    cpu, sample = None, None
+    if not args.no_cpu:
+        cpu, sample = cpu_baseline_c4(args.cpu_workers, args.c4_layer, args.c4_voxels, args.c4_cpu_steps)
+        print("cpu baseline:", json.dumps(cpu), file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    p, verts, conn = c4_task(args.c4_layer, args.c4_voxels)
+    src = synth.geo_xyz(p.src_lat, p.src_lon, 1.0)
+    recs = np.array([synth.geo_xyz(a, b, 1.0) for a, b in zip(p.rec_lat, p.rec_lon)])
+    masks = (synth.touched_elements(p.mesh, src[None], verts, conn),
+             synth.touched_elements(p.mesh, recs, verts, conn))
+    page = args.db_page_elems if args.db_page_elems > 0 else 256
+    dbs = synth.PagedSyntheticDatabases(p, masks, page)
+    npages = [len(x) for x in dbs.pages]
+    p.setup.db_device_pages = args.db_device_pages if args.db_device_pages > 0 else max(npages) + 2
+    setup_s = time.perf_counter() - t0
+    print("c4: %d voxels of layer %d, %d + %d pages of %d elements generated (%.1f GB) in %.1f s"
+          % (len(conn), args.c4_layer, npages[0], npages[1], page, dbs.generated_bytes / 1e9, setup_s),
+          file=sys.stderr, flush=True)
+    k = mk.MCKernel(p.setup)
+    steps, warm_paged = [], 0
+    for i in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        r = k.slave_work(verts, conn)
+        dt = time.perf_counter() - t0
+        if i >= args.warmup:
+            steps.append((dt, r["stats"]))
+        else:
+            warm_paged += r["stats"]["paged_bytes"]
+        print("step %d: %.3f s, %d evals, %d MC steps, %.1f GB paged" % (
+              i, dt, r["stats"]["evaluations"], r["stats"]["mc_steps"], r["stats"]["paged_bytes"] / 1e9),
+              file=sys.stderr, flush=True)
+    k.close()
+    tot_t = sum(s[0] for s in steps)
+    tot_e = sum(s[1]["evaluations"] for s in steps)
+    roof = roofline(steps, "c4", None)
+    check = None
+    if sample is not None:
+        check = self_check_task(p, verts, conn, sample)
+        print("self-check:", json.dumps(check), file=sys.stderr, flush=True)
+    dbs.free()
+    line = {
+        "metric": METRIC, "value": tot_e / tot_t, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": tot_t / len(steps) * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 accumulation)",
+        "data": "synthetic (seeded AxiSEM-style merged DB; the pages the slice touches generated into sparse host "
+                "mappings and paged to HBM)",
+        "config": {"workload": "c4: %d-quad DB, nt=%d (nfft %d), 200 receivers at the Gibraltar benchmark "
+                               "stations, Gibraltar CMT, 4 Gabor filters, CC x vp/vs (1600 kernels), slice of "
+                               "the 30-layer 1.25 deg voxel grid: the %d voxels of layer %d nearest the "
+                               "epicentre, 16 pts/step, max_iter 64, rel tol 0.001, STF deconvolution on"
+                               % (p.mesh.nelem, p.nt, 2 * p.nt, len(conn), args.c4_layer),
+                   "task_voxels": int(len(conn)), "parallelism": "none (1 GPU)", "evaluations": int(tot_e)},
+        "paging": {"page_elems": page, "pages_generated": {"fwd": npages[0], "bwd": npages[1]},
+                   "generated_gb": dbs.generated_bytes / 1e9,
+                   "db_bytes_total": float(p.mesh.nelem * 15 * 25 * p.nt * 4),
+                   "device_pages_per_db": int(p.setup.db_device_pages),
+                   "paged_gb_warmup": warm_paged / 1e9,
+                   "paged_gb_timed": sum(s[1]["paged_bytes"] for s in steps) / 1e9},
+        "roofline": roof, "cpu_baseline": cpu, "self_check": check,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def self_check_task(p, verts, conn, sample):
+    """The CPU sample's voxels of a task on the GPU with the oracle's stream positions and
+    MAXIMUM_ITERATIONS; niterations bit for bit and the largest relative L2 error."""
+    import mc_kernel_amd as mk
+    vox, ref_vals, ref_nit, max_iter = sample
+    offs = np.asarray(vox, np.int64) * ((max_iter + 1) * p.npoints_per_step)
+    old = p.setup.max_iter
+    p.setup.max_iter = max_iter
+    try:
+        k = mk.MCKernel(p.setup)
+        try:
+            r = k.slave_work(verts, conn[vox], halton_offsets=offs)
+        finally:
+            k.close()
+    finally:
+        p.setup.max_iter = old
+    errs = []
+    for j in range(ref_vals.shape[1]):
+        n = np.linalg.norm(ref_vals[:, j])
+        if n > 0:
+            errs.append(float(np.linalg.norm(r["values"][:, j] - ref_vals[:, j]) / n))
+    return {"voxels": int(len(vox)), "max_iter": int(max_iter), "max_rel_err": max(errs) if errs else None,
+            "kernels_compared": len(errs), "niterations_equal": bool(np.array_equal(r["niterations"], ref_nit)),
+            "tolerance": 1e-5}
+
+
 def task_curve(k, p, sizes, v0, nvox):
     """evals/s of the same voxels [v0, v0 + nvox) run as tasks of each size (the master
     hands out tasks of ELEMENTS_PER_TASK voxels, master_queue.f90:223-249); one warm-up
@@ -452,7 +673,10 @@ def main():
                          "page (0: resident in HBM, the default workload)")
     ap.add_argument("--db-host-memory", choices=["pinned", "pageable"], default="pinned")
     ap.add_argument("--db-device-pages", type=int, default=0,
-                    help="device page buffers per database when paging (0: 2)")
+                    help="device page buffers per database when paging (0: 2; c4: every page generated)")
+    ap.add_argument("--c4-layer", type=int, default=15, help="c4: layer of the voxel slice (0 = top)")
+    ap.add_argument("--c4-voxels", type=int, default=1024, help="c4: voxels of the slice")
+    ap.add_argument("--c4-cpu-steps", type=int, default=1, help="c4: MC steps of the CPU sample's voxels")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -468,6 +692,9 @@ def main():
         return
     if args.integrals:
         bench_integrals(args)
+        return
+    if args.config == "c4":
+        bench_c4(args)
         return
 
     # CPU baseline first, in fresh processes, before this process touches the GPU
@@ -511,54 +738,8 @@ def main():
               file=sys.stderr, flush=True)
     tot_t = sum(s[0] for s in steps)
     tot_e = sum(s[1]["evaluations"] for s in steps)
-    fwd_b = sum(s[1]["fwd_bytes"] for s in steps)
-    bwd_b = sum(s[1]["bwd_bytes"] for s in steps)
-    fwd_ms = sum(s[1]["strain_fwd_ms"] for s in steps)
-    bwd_ms = sum(s[1]["strain_bwd_ms"] for s in steps)
-    launches = sum(s[1]["strain_launches"] for s in steps) // 2
     dev_ms = sum(s[1]["device_ms"] for s in steps)
-    sp_ms = sum(s[1]["spectra_ms"] for s in steps)
-    mf_ms = sum(s[1]["mfma_ms"] for s in steps)
-    mf_launches = sum(s[1]["mfma_launches"] for s in steps)
-    mfma_path = all(s[1]["bwd_path"] == 1 for s in steps)
-    # the backward stage dominates: with the matrix-core path it is two kernels, the
-    # element spectra (which read the merged blocks) and the matrix-core evaluation; its
-    # algorithmic bytes are the backward blocks of every receiver's element set
-    if mfma_path:
-        bwd_kern, bwd_kms = "k_elem_spectra+k_bwd_mfma", sp_ms + mf_ms
-    else:
-        bwd_kern, bwd_kms = "k_bwd_eval", bwd_ms
-    if bwd_kms >= fwd_ms:
-        kern, byt, ms = bwd_kern, bwd_b, bwd_kms
-    else:
-        kern, byt, ms = "k_fwd_spectra", fwd_b, fwd_ms
-    achieved = (byt / launches) / (ms / launches * 1e-3) / 1e9 if ms > 0 else 0.0
-    # the committed counter summary is of the c1 workload
-    traffic = measured_traffic(args.traffic, kern) if args.config == "c1" else None
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kern,
-            "bytes_per_launch": byt / max(launches, 1), "avg_launch_ms": ms / max(launches, 1),
-            "launches": launches,
-            "algorithmic_bytes": "merged blocks once per launch: fwd 4 nt 25 x 10 variables per distinct "
-                                 "forward element (+ 48 B per point), bwd 4 nt 25 x 5 variables per distinct "
-                                 "(receiver, backward element)",
-            "alg_bytes_whole_job": sum(s[1]["alg_bytes"] for s in steps) / len(steps),
-            "timing": "HIP events on the library's stream around each kernel of every launch"}
-    if mfma_path:
-        roof["kernels"] = {
-            "k_elem_spectra": {"avg_launch_ms": sp_ms / max(mf_launches, 1), "launches": mf_launches},
-            "k_bwd_mfma": {"avg_launch_ms": mf_ms / max(mf_launches, 1), "launches": mf_launches,
-                           "waves_per_launch": sum(s[1]["mfma_groups"] for s in steps) / max(mf_launches, 1)},
-            "k_fwd_spectra": {"avg_launch_ms": fwd_ms / max(launches, 1), "launches": launches},
-            "backward_stage_ms_per_launch": bwd_ms / max(launches, 1)}
-    if traffic:
-        # the counter-measured HBM bytes of the same kernels over their launch time
-        roof["measured_frac"] = round(traffic / (ms / launches * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
-    if args.config == "c1":
-        # what does bound the matrix-core kernel: its issue rates, from the committed counters
-        busy = measured_counters(os.path.join(ROOT, "profiles", "c1_bwd_mfma_counters.md"), "k_bwd_mfma")
-        if busy:
-            roof["k_bwd_mfma_counters"] = busy
+    roof = roofline(steps, args.config, args.traffic)
 
     curve = None
     if args.task_curve:

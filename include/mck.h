@@ -169,9 +169,13 @@ typedef struct {
      * points touch, db_page_elems elements each, into device page buffers.  Results
      * are identical to the resident mode.                                               */
     int32_t db_page_elems;
-    /* Device page buffers per database when paging (0 or < 2: 2).  The first
-     * db_device_pages - 2 pages keep a buffer of their own once loaded; the others
-     * rotate through the last two buffers (copies overlap the kernels).                 */
+    /* Device page buffers per database when paging (0 or < 2: 2; at most the number
+     * of pages + 2 are made).  The first db_device_pages - 2 distinct pages used keep a
+     * buffer of their own once loaded (a working set of at most that many pages is
+     * copied once, wherever its pages lie); the others rotate through the last two
+     * buffers (copies overlap the kernels).  A launch whose pages all keep a buffer of
+     * their own runs once over all its points, through a table of the page buffers;
+     * otherwise once per page.                                                          */
     int32_t db_device_pages;
 } mck_setup_t;
 

@@ -316,14 +316,21 @@ struct DbDev {
     DevDb d{};
     float *u_owned = nullptr;      // device copy of the snapshots (null if caller-resident)
     // paged snapshots (databases larger than device memory): the caller's host copy is
-    // paged through nslot device buffers of page_elems elements.  Pages below nslot - 2
-    // keep a buffer of their own once loaded; the others rotate through the last two.
+    // paged through nslot device buffers of page_elems elements.  The first nslot - 2
+    // distinct pages used keep a buffer of their own once loaded (a working set that fits
+    // stays resident wherever its pages lie); the others rotate through the last two.
     bool paged = false;
     int page_elems = 0, npages = 0, nslot = 0;
     size_t elem_floats = 0;        // floats per element: nvar x 25 x nt
     const float *host_u = nullptr;
     std::vector<float *> slot;
     std::vector<int> slot_page;    // page held by each buffer (-1: none)
+    std::vector<int> page_slot;    // own buffer of each page (-1: none yet, or rotating)
+    int nfixed_used = 0;           // own buffers handed out
+    // page table of the own buffers (device copy and its host image): launches whose
+    // pages all have one run once over all their items
+    DBuf<const float *> ptable;
+    std::vector<const float *> ptable_host;
     std::vector<hipEvent_t> copied, freed;
     int rot = 0;                   // next rotating buffer
     void release_pages() {
@@ -332,8 +339,9 @@ struct DbDev {
             if (copied[i]) (void)hipEventDestroy(copied[i]);
             if (freed[i]) (void)hipEventDestroy(freed[i]);
         }
-        slot.clear(); slot_page.clear(); copied.clear(); freed.clear();
-        paged = false; page_elems = npages = nslot = rot = 0; host_u = nullptr;
+        slot.clear(); slot_page.clear(); page_slot.clear(); copied.clear(); freed.clear();
+        ptable.release(); ptable_host.clear();
+        paged = false; page_elems = npages = nslot = rot = nfixed_used = 0; host_u = nullptr;
     }
     DBuf<float> mp_s, mp_z;
     DBuf<double> econ, ejac;
@@ -521,10 +529,15 @@ int upload_db(DbDev &D, const mck_db_t &db, int nt, const char *name, int page_e
         D.paged = true;
         D.page_elems = page_elems;
         D.npages = (ne + page_elems - 1) / page_elems;
-        D.nslot = std::max(2, std::min(nslot, D.npages));
+        D.nslot = std::max(2, std::min(nslot, D.npages + 2));
         D.host_u = db.snapshots;
         D.slot.assign(D.nslot, nullptr);
         D.slot_page.assign(D.nslot, -1);
+        D.page_slot.assign(D.npages, -1);
+        D.nfixed_used = 0;
+        D.ptable_host.assign(D.npages, nullptr);
+        HIPCHECK(D.ptable.ensure(D.npages));
+        HIPCHECK(hipMemcpy(D.ptable.p, D.ptable_host.data(), D.npages * sizeof(float *), hipMemcpyHostToDevice));
         D.copied.assign(D.nslot, nullptr);
         D.freed.assign(D.nslot, nullptr);
         for (int i = 0; i < D.nslot; ++i) {
@@ -1127,8 +1140,8 @@ hipError_t launch_traces(State &G, const DevDb &db, int fwd, int frame, int npts
 
 // Runs launch(db, i0, i1) over the items [i0, i1) of a launch whose elements the device
 // view db holds: once over [0, n) when database D is resident; otherwise page by page.
-// A page below nslot - 2 is copied into its own buffer on first use and stays; any other
-// page is copied into one of the two rotating buffers (on the copy stream: the copy of
+// The first nslot - 2 distinct pages used are copied into buffers of their own on first
+// use and stay; any other page is copied into one of the two rotating buffers (on the copy stream: the copy of
 // the next page overlaps the kernels of this one, and a buffer is refilled only after
 // the kernels that read it).  sorted_keys: element id * key_scale + minor key of every
 // item in launch order, ascending (pages then cover disjoint item ranges and untouched
@@ -1160,10 +1173,59 @@ int run_paged(State &G, DbDev &D, int n, const int *sorted_keys, Launch &&launch
         HIPCHECK(hipStreamSynchronize(G.stream));
     }
     const int nfixed = D.nslot - 2;
+    if (sorted_keys) {
+        // every page of the launch in a buffer of its own (already, or one still free):
+        // the missing ones are copied, the page table updated, and one launch runs over
+        // all the items
+        int need = 0;
+        for (int b = 0; b < D.npages; ++b)
+            if (bounds[b + 1] > bounds[b] && D.page_slot[b] < 0) ++need;
+        if (need <= nfixed - D.nfixed_used) {
+            bool table = false;
+            for (int b = 0; b < D.npages; ++b) {
+                if (bounds[b + 1] <= bounds[b]) continue;
+                int k = D.page_slot[b];
+                if (k < 0) {
+                    k = D.nfixed_used++;
+                    D.page_slot[b] = k;
+                }
+                if (D.slot_page[k] != b) {
+                    const int e0 = b * D.page_elems, e1 = std::min(D.d.nelem, e0 + D.page_elems);
+                    HIPCHECK(hipStreamWaitEvent(G.copy_stream, D.freed[k], 0));
+                    HIPCHECK(hipMemcpyAsync(D.slot[k], D.host_u + (size_t)e0 * D.elem_floats,
+                                            (size_t)(e1 - e0) * D.elem_floats * sizeof(float), hipMemcpyHostToDevice,
+                                            G.copy_stream));
+                    HIPCHECK(hipEventRecord(D.copied[k], G.copy_stream));
+                    D.slot_page[k] = b;
+                    G.stats.paged_bytes += (int64_t)((size_t)(e1 - e0) * D.elem_floats * sizeof(float));
+                }
+                // (the host image changes only here, after the stream synchronisation
+                // above: an earlier copy of it has completed)
+                if (D.ptable_host[b] != D.slot[k]) {
+                    D.ptable_host[b] = D.slot[k];
+                    table = true;
+                }
+                HIPCHECK(hipStreamWaitEvent(G.stream, D.copied[k], 0));
+            }
+            if (table)
+                HIPCHECK(hipMemcpyAsync(D.ptable.p, D.ptable_host.data(), D.npages * sizeof(float *),
+                                        hipMemcpyHostToDevice, G.stream));
+            DevDb d = D.d;
+            d.u = nullptr;
+            d.u_first = d.u_last = 0;
+            d.page_u = D.ptable.p;
+            d.page_elems = D.page_elems;
+            return launch_status(launch(d, 0, n));
+        }
+    }
     for (int b = 0; b < D.npages; ++b) {
         const int i0 = sorted_keys ? bounds[b] : 0, i1 = sorted_keys ? bounds[b + 1] : n;
         if (i1 <= i0) continue;
-        int k = b < nfixed ? b : -1;
+        int k = D.page_slot[b];
+        if (k < 0 && D.nfixed_used < nfixed) {
+            k = D.nfixed_used++;
+            D.page_slot[b] = k;
+        }
         if (k < 0) {
             // a rotating buffer: the one that already holds the page, else the next one
             for (int j = nfixed; j < D.nslot; ++j) if (D.slot_page[j] == b) k = j;

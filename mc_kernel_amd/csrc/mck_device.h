@@ -37,6 +37,10 @@ struct DevDb {
     int nelem, nvar, nt;
     const float *u;           // [u_last - u_first][nvar][5][5][nt]: elements u_first ..
     int u_first, u_last;      // u_last - 1 (all elements when resident, one page when paged)
+    // paged database whose pages of this launch are all in device buffers: the buffer of
+    // each page (null: not held), page_elems elements each; u is then unused
+    const float *const *page_u;
+    int page_elems;
     const float *mp_s, *mp_z; // element midpoints
     const double *econ;       // [nelem][16] element constants (see host setup)
     const double *ejac;       // [nelem][25][5] inverse Jacobian and 1/s at the nodes

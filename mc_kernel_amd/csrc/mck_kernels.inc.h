@@ -34,6 +34,20 @@ __constant__ Basis c_basis;
 //  E[0..3] corner s, E[4..7] corner z (fp32 values), spheroid: E[8..11] r, E[12..15]
 //  theta; semino/semiso: E[8] a, E[9] b, E[10] theta_bar, E[11] dtheta.
 // ------------------------------------------------------------------------------------
+// The snapshots [nvar][5][5][nt] of element e in the view db, or null when the view does
+// not hold it (a point not located, an element of another page)
+__device__ inline const float *elem_u(const DevDb &db, int e)
+{
+    const size_t stride = (size_t)db.nvar * 25 * db.nt;
+    if (db.page_u) {
+        if (e < 0 || e >= db.nelem) return nullptr;
+        const float *b = db.page_u[e / db.page_elems];
+        return b ? b + (size_t)(e % db.page_elems) * stride : nullptr;
+    }
+    if (e < db.u_first || e >= db.u_last) return nullptr;
+    return db.u + (size_t)(e - db.u_first) * stride;
+}
+
 __device__ inline void d_mapping(double xi, double eta, const double *E, int type, double *sz)
 {
     if (type == 0) {
@@ -923,7 +937,8 @@ __global__ __launch_bounds__(kStrainThreads) void k_seis(DevDb db, PathConf pc, 
     __shared__ double Wd[6 * 3 * 25];
     const int r = blockIdx.x, tid = threadIdx.x;
     const Located L = loc[r];
-    if (L.elem < db.u_first || L.elem >= db.u_last) return;
+    const float *ue = elem_u(db, L.elem);
+    if (!ue) return;
     DevFrame none;
     none.cos_lon = 1.0; none.sin_lon = 0.0; none.cos_colat = 1.0; none.sin_colat = 0.0;
     // element tables and Lagrange bases only (no output-transform roles)
@@ -938,7 +953,6 @@ __global__ __launch_bounds__(kStrainThreads) void k_seis(DevDb db, PathConf pc, 
     __syncthreads();
     const int nt = pc.nt;
     const size_t blk = (size_t)25 * nt;
-    const float *ue = db.u + (size_t)(L.elem - db.u_first) * db.nvar * blk;
     const double *m = mb + 6 * r;
     for (int t = tid; t < nt; t += kStrainThreads) {
         double e[6] = {0, 0, 0, 0, 0, 0};
@@ -1473,14 +1487,15 @@ __global__ __launch_bounds__(kStrainThreads) void k_strain_traces(
     const int p = blockIdx.x;
     if (p >= npts) return;
     const Located L = loc[p];
-    if (L.elem < db.u_first || L.elem >= db.u_last) return;   // not located / not in this page
+    const float *ue = elem_u(db, L.elem);
+    if (!ue) return;                                           // not located / not in this page
     const int nvar = fwd ? pc.nvar_fwd : pc.nvar_bwd;
     const int *vars = fwd ? pc.fwd_vars : pc.bwd_vars;
     build_weights(S, db, pc, frames[frame], L, fwd != 0, nvar, vars, pc.nd, nullptr, false);
     const int nt = pc.nt, nd = pc.nd, t0 = threadIdx.x * SPT;
     const size_t blk = (size_t)25 * nt;
     float acc[6][SPT];
-    gather_strain<SPT, 6>(S, db.u + (size_t)(L.elem - db.u_first) * db.nvar * blk, blk, nvar, vars, t0, nt, acc);
+    gather_strain<SPT, 6>(S, ue, blk, nvar, vars, t0, nt, acc);
 #pragma unroll
     for (int c = 0; c < 6; ++c)
         if (c < nd)
@@ -1516,7 +1531,8 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && ND <= 4) ? 3 : (SPT <=
     if (keys && keys[lb] >= db.nelem) return;                  // spectra nobody reads
     const int p = order ? order[lb] : lb;
     const Located L = loc[p];
-    if (L.elem < db.u_first || L.elem >= db.u_last) return;   // not located / not in this page
+    const float *ue = elem_u(db, L.elem);
+    if (!ue) return;                                           // not located / not in this page
     load_twiddles(twq, pc.M, Tq);
     {
         // gather weights of the point (k_op_setup)
@@ -1529,7 +1545,7 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && ND <= 4) ? 3 : (SPT <=
     const int tid = threadIdx.x, nt = pc.nt, M = pc.M, t0 = tid * SPT;
     const size_t blk = (size_t)25 * nt;
     float acc[ND][SPT];
-    gather_strain<SPT, ND>(S, db.u + (size_t)(L.elem - db.u_first) * db.nvar * blk, blk, pc.nvar_fwd, pc.fwd_vars, t0, nt, acc);
+    gather_strain<SPT, ND>(S, ue, blk, pc.nvar_fwd, pc.fwd_vars, t0, nt, acc);
     const float fdt = (float)pc.dt, h = (float)(pc.dt * 0.5);
     float tap[SPT];
     taper_factors(t0, nt, pc, tap);
@@ -1580,7 +1596,8 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && NDB <= 4) ? 3 : (SPT <
     const int q = order ? order[lb] : lb;
     const int p = q / R, r = q - (q / R) * R;
     const Located L = loc_b[q];
-    if (L.elem < db.u_first || L.elem >= db.u_last) return;   // not located / not in this page
+    const float *ue = elem_u(db, L.elem);
+    if (!ue) return;                                           // not located / not in this page
     load_twiddles(twq, pc.M, Tq);
     const int tid = threadIdx.x;
     {
@@ -1615,7 +1632,7 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && NDB <= 4) ? 3 : (SPT <
     float2 spec[NDB][SPT], specM[NDB];
     {
         float acc[NDB][SPT];
-        gather_strain<SPT, NDB>(S, db.u + (size_t)(L.elem - db.u_first) * db.nvar * blk, blk, pc.nvar_bwd, pc.bwd_vars, t0, nt, acc);
+        gather_strain<SPT, NDB>(S, ue, blk, pc.nvar_bwd, pc.bwd_vars, t0, nt, acc);
         const float fdt = (float)pc.dt;
         float tap[SPT];
         taper_factors(t0, nt, pc, tap);

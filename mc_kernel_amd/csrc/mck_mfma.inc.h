@@ -96,10 +96,10 @@ __global__ __launch_bounds__(kStrainThreads) void k_elem_spectra(DevDb db, PathC
     const int u = u0 + (int)blockIdx.x;
     if (u >= nrun) return;
     const int e = run_elem[u];
-    if (e < db.u_first || e >= db.u_last) return;
+    const float *ue = elem_u(db, e);
+    if (!ue) return;
     const int tid = threadIdx.x, nt = pc.nt, M = pc.M, t0 = tid * SPT;
     const size_t blk = (size_t)25 * nt;
-    const float *ue = db.u + (size_t)(e - db.u_first) * db.nvar * blk;
     float tap[SPT];
     taper_factors(t0, nt, pc, tap);
     const int J = pc.nvar_bwd * 25;

@@ -179,6 +179,13 @@ def destination(lat, lon, delta, az):
     return math.degrees(la2), (math.degrees(lo2) + 540.0) % 360.0 - 180.0
 
 
+def epicentral_distance(lat1, lon1, lat2, lon2):
+    """Great-circle distance (deg) between two points given in degrees."""
+    a1, o1, a2, o2 = map(math.radians, (lat1, lon1, lat2, lon2))
+    c = math.sin(a1) * math.sin(a2) + math.cos(a1) * math.cos(a2) * math.cos(o2 - o1)
+    return math.degrees(math.acos(max(-1.0, min(1.0, c))))
+
+
 def morton3(q):
     """Morton codes of integer coordinates q[n][3] (21 bits each)."""
     def spread(v):
@@ -251,7 +258,8 @@ class Problem:
                  tet_edge, npoints_per_step, max_iter, allowed_error=1e-4,
                  allowed_relative_error=1e-2, component="Z", mij=None, seed=1,
                  mesh_seed=2, src=(20.0, 30.0, 100.0), azimuth=135.0, deconv_stf=1,
-                 int_scheme="parseval", db_threads=None, fill=True, unmasked_frac=0.0):
+                 int_scheme="parseval", db_threads=None, fill=True, unmasked_frac=0.0,
+                 receivers_latlon=None):
         self.name = name
         self.seed = seed
         self.mesh = Mesh(nelem_target)
@@ -265,10 +273,18 @@ class Problem:
             mij = [1.0e20, 1.0e20, 1.0e20, 0.0, 0.0, 0.0]   # explosion, N m
         self.mij = np.array(mij, np.float64)
         lats, lons = [], []
-        for d in receivers_deg:
-            la, lo = destination(self.src_lat, self.src_lon, d, azimuth)
-            lats.append(la)
-            lons.append(lo)
+        if receivers_latlon is not None:
+            # receivers given by their coordinates: distances from the source
+            for la, lo in receivers_latlon:
+                lats.append(float(la))
+                lons.append(float(lo))
+            receivers_deg = [epicentral_distance(self.src_lat, self.src_lon, la, lo) for la, lo in receivers_latlon]
+        else:
+            for d in receivers_deg:
+                la, lo = destination(self.src_lat, self.src_lon, d, azimuth)
+                lats.append(la)
+                lons.append(lo)
+        self.receivers_deg = list(receivers_deg)
         self.rec_lat = np.array(lats, np.float64)
         self.rec_lon = np.array(lons, np.float64)
         self.nrec = len(receivers_deg)
@@ -447,6 +463,19 @@ def make_problem(name, **over):
                   kernels=_kernels(50, 6, ["CC", "AM"], ["vp", "vs"]), ntet=200000,
                   tet_edge=300e3, npoints_per_step=16, max_iter=256,
                   allowed_error=1e-30, allowed_relative_error=1e-3)
+    elif name == "c4":
+        # the reference's Gibraltar benchmark event (CMTSOLUTION: 35.74 N, 3.72 W, 12 km,
+        # moment tensor in dyne cm -> N m) and its 200 stations' coordinates (3.8..174.3
+        # deg); 4 Gabor filters x CC x vp/vs = 1600 kernels; 100,000-quad databases at
+        # nt 4096 (614 GB: paged, see PagedSyntheticDatabases); the inversion mesh is the
+        # 30-layer 1.25 deg voxel grid (voxel_grid), of which a task takes a slice
+        kw = dict(nelem_target=100000, nt=4096, dt=1.0, receivers_deg=None,
+                  receivers_latlon=gibraltar_receivers(), src=(35.74, -3.72, 12.0),
+                  mij=[x * 1e-7 for x in (-3.85e24, -2.76e25, 3.145e25, 1.188e25, 1.345e25, -1.649e25)],
+                  filters=[("Gabor", (p, 0.5, 0.0, 0.0)) for p in (20.0, 30.0, 40.0, 60.0)],
+                  kernels=_kernels(200, 4, ["CC"], ["vp", "vs"]), ntet=1, tet_edge=300e3,
+                  npoints_per_step=16, max_iter=64, allowed_error=1e-30, allowed_relative_error=1e-3,
+                  fill=False)
     elif name == "tiny":
         kw = dict(nelem_target=450, nt=256, dt=4.0, receivers_deg=[30.0, 60.0],
                   filters=[("Gabor", (40.0, 0.5, 0.0, 0.0)), ("Gabor", (60.0, 0.5, 0.0, 0.0))],
@@ -456,3 +485,206 @@ def make_problem(name, **over):
         raise ValueError(name)
     kw.update(over)
     return Problem(name, **kw)
+
+
+# --------------------------------------------------------------------------------------
+# c4: receivers, voxel grid, databases paged from a sparse host mapping
+# --------------------------------------------------------------------------------------
+
+def gibraltar_receivers():
+    """(lat, lon) in degrees of the reference's 200-station Gibraltar benchmark
+    (data/gibraltar_receivers_200.txt, extracted by data/extract_receivers.py)."""
+    d = np.loadtxt(os.path.join(_HERE, "data", "gibraltar_receivers_200.txt"))
+    return [tuple(r) for r in d]
+
+
+def voxel_grid(nlayers=30, dlat=1.25, rmin=R_CMB, rmax=R_EARTH):
+    """The global equal-area voxel grid: nlayers shells of equal thickness between rmin
+    and rmax, colatitude bands of dlat degrees, band i cut into round(360 / dlat x
+    sin(mid colatitude)) equal longitude blocks from -180 deg.  Voxels are blocks in
+    radius, colatitude and longitude given by their 8 corners (the inversion mesh's TYPE=VOX
+    elements, random_voxel.f90): bottom face (colat0, lon0), (colat0, lon1), (colat1, lon1),
+    (colat1, lon0), then the top face.  Order: layer (top down), band (north to south),
+    longitude.  Returns corners [8 nvox][3] in m, 1-based connectivity [nvox][8] and the
+    (layer, band) of every voxel."""
+    radii = np.linspace(rmax, rmin, nlayers + 1)
+    nb = int(round(180.0 / dlat))
+    th_edges = np.radians(np.arange(nb + 1) * dlat)
+    blocks = []
+    for i in range(nb):
+        nl = max(1, int(round(360.0 / dlat * math.sin(0.5 * (th_edges[i] + th_edges[i + 1])))))
+        ph = np.radians(np.linspace(-180.0, 180.0, nl + 1))
+        blocks.append((th_edges[i], th_edges[i + 1], ph))
+    corners, lay, band = [], [], []
+    for k in range(nlayers):
+        r_top, r_bot = radii[k], radii[k + 1]
+        for i, (t0, t1, ph) in enumerate(blocks):
+            n = len(ph) - 1
+            p0, p1 = ph[:-1], ph[1:]
+            th = np.array([t0, t0, t1, t1])
+            c = np.empty((n, 8, 3))
+            for j, r in enumerate((r_bot, r_top)):
+                for q, (tq, pq) in enumerate(zip(th, (p0, p1, p1, p0))):
+                    c[:, 4 * j + q, 0] = r * np.sin(tq) * np.cos(pq)
+                    c[:, 4 * j + q, 1] = r * np.sin(tq) * np.sin(pq)
+                    c[:, 4 * j + q, 2] = r * np.cos(tq)
+            corners.append(c)
+            lay.append(np.full(n, k, np.int32))
+            band.append(np.full(n, i, np.int32))
+    c = np.concatenate(corners)
+    nv = len(c)
+    verts = np.ascontiguousarray(c.reshape(-1, 3))
+    conn = np.arange(1, 8 * nv + 1, dtype=np.int32).reshape(nv, 8)
+    return verts, conn, np.concatenate(lay), np.concatenate(band)
+
+
+def write_vox_mesh(path, verts, conn):
+    """The inversion mesh in the abaqus text format of the reference's voxel meshes
+    (Meshes/vox_15l_5deg.dat: *NODE in km, *ELEMENT, TYPE=VOX with 8 node ids)."""
+    with open(path, "w") as f:
+        f.write("*HEADING\nvoxel grid\n**\n*PART, NAME=Part-Default\n**\n*NODE, NSET=ALLNODES\n")
+        for i, (x, y, z) in enumerate(np.asarray(verts) / 1e3):
+            f.write("%d, %.9e, %.9e, %.9e\n" % (i + 1, x, y, z))
+        f.write("**\n*ELEMENT, TYPE=VOX, ELSET=EB2\n")
+        for i, row in enumerate(np.asarray(conn)):
+            f.write("%d, %s\n" % (i + 1, ", ".join(str(int(v)) for v in row)))
+        f.write("**\n*END PART\n")
+
+
+def read_vox_mesh(path):
+    """(verts [n][3] in m, conn [nvox][8] 1-based) of an abaqus TYPE=VOX mesh file."""
+    import gzip
+    op = gzip.open if path.endswith(".gz") else open
+    verts, conn, mode = [], [], None
+    with op(path, "rt") as f:
+        for ln in f:
+            if ln.startswith("*"):
+                u = ln.upper()
+                mode = "n" if u.startswith("*NODE") else ("e" if u.startswith("*ELEMENT") and "VOX" in u else None)
+                continue
+            parts = [x for x in ln.replace(",", " ").split()]
+            if mode == "n" and len(parts) == 4:
+                verts.append([float(x) for x in parts[1:]])
+            elif mode == "e" and len(parts) == 9:
+                conn.append([int(x) for x in parts[1:]])
+    return np.array(verts) * 1e3, np.array(conn, np.int32)
+
+
+def voxel_slice(verts, conn, layer_of, layer, lat, lon, n):
+    """The n voxels of one layer whose corner centroids lie nearest to (lat, lon), in
+    grid order."""
+    sel = np.nonzero(layer_of == layer)[0]
+    c = np.asarray(verts).reshape(-1, 8, 3)[sel].mean(axis=1)
+    c /= np.linalg.norm(c, axis=1)[:, None]
+    p = geo_xyz(lat, lon, 1.0)
+    order = np.argsort(-(c @ p), kind="stable")[:n]
+    return np.sort(sel[order])
+
+
+def _frame_angles(pole, pts):
+    """angular distance (rad) of unit vectors pts [n][3] from the unit vector pole"""
+    return np.arccos(np.clip(pts @ pole, -1.0, 1.0))
+
+
+def touched_elements(mesh, poles, verts, conn, margin_deg=0.5):
+    """Elements of the (s, z) mesh that points of the voxels (verts, conn) can reach in
+    any of the frames whose poles (unit vectors: source, receivers) are given: elements
+    whose radius range meets the voxels' and whose angle from the pole range meets the
+    voxels' angular range from that pole (each voxel's range sampled on a 9 x 9 grid over
+    its colatitude-longitude face, widened by the sampling step and margin_deg).
+    Returns a bool mask [nelem]."""
+    c = np.asarray(verts).reshape(-1, 8, 3)
+    r = np.linalg.norm(c, axis=2)
+    rmin, rmax = r.min(), r.max()
+    th = np.arccos(np.clip(c[..., 2] / r, -1, 1))
+    ph = np.arctan2(c[..., 1], c[..., 0])
+    t0, t1, p0, p1 = th.min(1), th.max(1), ph.min(1), ph.max(1)
+    g = np.linspace(0.0, 1.0, 9)
+    tt = t0[:, None, None] + (t1 - t0)[:, None, None] * g[None, :, None]
+    pp = p0[:, None, None] + (p1 - p0)[:, None, None] * g[None, None, :]
+    pts = np.stack([np.sin(tt) * np.cos(pp), np.sin(tt) * np.sin(pp), np.cos(tt) * np.ones_like(pp)], -1).reshape(-1, 3)
+    step = max(float((t1 - t0).max()), float(((p1 - p0) * np.sin(0.5 * (t0 + t1))).max())) / 8.0
+    widen = step + math.radians(margin_deg)
+    # element extents in the frame's polar coordinates, from its GLL points
+    s_ = mesh.mesh_s.astype(np.float64)[mesh.sem.reshape(mesh.nelem, 25)]
+    z_ = mesh.mesh_z.astype(np.float64)[mesh.sem.reshape(mesh.nelem, 25)]
+    er = np.hypot(s_, z_)
+    ea = np.arctan2(s_, z_)
+    er0, er1, ea0, ea1 = er.min(1), er.max(1), ea.min(1), ea.max(1)
+    pad_r = 0.25 * (er1 - er0)
+    pad_a = 0.25 * (ea1 - ea0)
+    in_r = (er1 + pad_r >= rmin - 50e3) & (er0 - pad_r <= rmax + 50e3)
+    mask = np.zeros(mesh.nelem, bool)
+    for pole in np.atleast_2d(poles):
+        pole = pole / np.linalg.norm(pole)
+        a = _frame_angles(pole, pts)
+        a0, a1 = a.min() - widen, a.max() + widen
+        mask |= in_r & (ea1 + pad_a >= a0) & (ea0 - pad_a <= a1)
+    return mask
+
+
+def _libc():
+    libc = C.CDLL(None, use_errno=True)
+    libc.mmap.restype = C.c_void_p
+    libc.mmap.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_long]
+    libc.munmap.argtypes = [C.c_void_p, C.c_size_t]
+    return libc
+
+
+def sparse_mapping(nbytes):
+    """Address of a private anonymous mapping of nbytes that reserves nothing: memory is
+    taken only where written (pages never written read as zeros)."""
+    addr = _libc().mmap(None, nbytes, 0x3, 0x02 | 0x20 | 0x4000, -1, 0)
+    if addr is None or addr == C.c_void_p(-1).value:
+        raise OSError(C.get_errno(), "sparse mapping of %d bytes failed" % nbytes)
+    return addr
+
+
+def release_mapping(addr, nbytes):
+    _libc().munmap(addr, nbytes)
+
+
+class PagedSyntheticDatabases:
+    """Both merged databases of a Problem too large for device or host memory: the whole
+    address range of each is reserved as a sparse host mapping (memory is taken only where
+    written), and only the element pages of page_elems elements that hold an element of
+    masks[which] are generated (the device generator, bit-identical to fill_db) and
+    written into it.  The setup's snapshots point at the mappings and page from them
+    (db_page_elems = page_elems).  generated_bytes: the bytes written."""
+
+    def __init__(self, p, masks, page_elems, chunk_elems=2048):
+        self.lib = synth_gpu_lib()
+        self.p = p
+        m = p.mesh
+        self.maps, self.generated_bytes, self.pages = [], 0, []
+        for which, (nvar, seed) in enumerate(((10, p.seed * 1000 + 1), (5, p.seed * 1000 + 2))):
+            elem_floats = nvar * 25 * p.nt
+            size = m.nelem * elem_floats * 4
+            addr = sparse_mapping(size)
+            self.maps.append((addr, size))
+            pages = np.unique(np.nonzero(masks[which])[0] // page_elems)
+            self.pages.append(pages)
+            # runs of consecutive pages, generated chunk_elems elements at a time
+            for pg in pages:
+                e0, e1 = int(pg) * page_elems, min(m.nelem, (int(pg) + 1) * page_elems)
+                for a in range(e0, e1, chunk_elems):
+                    b = min(e1, a + chunk_elems)
+                    d = C.c_void_p()
+                    sem = np.ascontiguousarray(m.sem[a:b])
+                    rc = self.lib.synth_gpu_db(b - a, abi.ptr(sem, C.c_int32), nvar, p.nt, p.dt, seed, C.byref(d))
+                    if rc:
+                        raise RuntimeError("device database generation failed (%d)" % rc)
+                    try:
+                        if self.lib.synth_gpu_download(d, addr + a * elem_floats * 4, (b - a) * elem_floats):
+                            raise RuntimeError("database download failed")
+                    finally:
+                        self.lib.synth_gpu_free(d)
+                    self.generated_bytes += (b - a) * elem_floats * 4
+        p.setup.fwd.snapshots = C.cast(C.c_void_p(self.maps[0][0]), C.POINTER(C.c_float))
+        p.setup.bwd.snapshots = C.cast(C.c_void_p(self.maps[1][0]), C.POINTER(C.c_float))
+        p.setup.db_page_elems = page_elems
+
+    def free(self):
+        for addr, size in self.maps:
+            release_mapping(addr, size)
+        self.maps = []

@@ -62,11 +62,13 @@ def _task(p, k, offsets=None):
     return res, k.base_kernels(x), traces
 
 
-@pytest.mark.parametrize("device_pages", [2, 7])
+@pytest.mark.parametrize("device_pages", [2, 7, 15])
 @pytest.mark.parametrize("case", sorted(CASES))
 def test_paged_equals_resident(case, device_pages):
     """Paged = resident bit for bit, and both = the oracle on the oracle's stream
-    positions (element ids and iteration counts exact, kernels to 1e-5)."""
+    positions (element ids and iteration counts exact, kernels to 1e-5).  2 and 7 buffers:
+    element-sorted launches run page by page; 15 (every one of the 13 pages can keep a
+    buffer): each launch runs once over all its items through the page table."""
     print("paging case", case, device_pages, flush=True)
     oref, obase = _oracle(case)
     task = lambda p, k: _task(p, k, oref["offsets"])
@@ -107,8 +109,8 @@ def test_page_larger_than_database_is_resident():
 def test_fixed_page_buffers_are_copied_once(monkeypatch):
     """Unsorted launches (base kernels on the per-pair backward kernel) visit every page
     in order.  With 2 buffers every call copies every page of both databases; with 7,
-    pages 0-4 keep their buffers and pages 5-12 rotate through the other two, so a second
-    call copies exactly 5 pages per database less."""
+    the first 5 pages used (0-4) keep their buffers and pages 5-12 rotate through the
+    other two, so a second call copies exactly 5 pages per database less."""
     monkeypatch.setenv("MCK_BWD_PATH", "0")
     def two_calls(p, k):
         x = p.sample_points(40, seed=9)
