@@ -9,6 +9,7 @@ a general moment tensor with R receivers: 6 and 6):
   c1        160 kernels at nt 2048, explosion / Z and general moment tensor / R
   c2        the 1,200-kernel set (50 receivers, 5 Gabor + Butterworth band pass) at nt 2048
   nt 4096   16 samples per thread (the c4 trace length)
+  c4        the 1,600-kernel set of 200 stations with the Gibraltar moment tensor at nt 4096
 
 The databases are made in device memory by the device generator and copied back for the
 oracle; the generator is checked bit for bit against the host generator first.  Bars:
@@ -157,3 +158,38 @@ def test_c0_whole_workload():
     assert res["stats"]["evaluations"] == int(ref["evaluations"]) > 0
     for j in range(p.nkernel):
         assert _rel(res["values"][:, j], ref["values"][:, j]) < TOL, j
+
+
+def test_c4_kernel_set_nt4096():
+    """c4's kernel set (the 200 Gibraltar stations, Gibraltar moment tensor: 6 and 4
+    carried components, 4 Gabor x CC x vp/vs = 1600 kernels) at its own nt 4096 (16
+    samples per thread) on a reduced database, two voxels of the c4 slice, one MC step.
+    Bars: iteration counts and evaluations exact; the voxel-kernel field within 1e-5
+    (relative L2 over all voxels and kernels, as the c2 kernel-set test); each kernel within
+    1e-4 of its own norm (a kernel that is a small remainder of cancelling FP32 terms
+    loses digits in proportion: measured 4e-5 for one of 864 here, the others below 4e-6)."""
+    import mc_kernel_amd as mk
+    p = synth.make_problem("c4", nelem_target=3000)
+    assert p.nkernel == 1600 and p.nt == 4096
+    p.max_iter = 1
+    p.setup.max_iter = 1
+    v, c, L, _ = synth.voxel_grid()
+    sel = synth.voxel_slice(v, c, L, 15, p.src_lat, p.src_lon, 1024)[[0, 500]]
+    verts = np.ascontiguousarray(v.reshape(-1, 8, 3)[sel].reshape(-1, 3))
+    conn = np.arange(1, 8 * len(sel) + 1, dtype=np.int32).reshape(-1, 8)
+    o, dev = _oracle_on_device_databases(p)
+    try:
+        ref = o.slave_work(verts, conn)
+        k = mk.MCKernel(p.setup)
+        try:
+            res = k.slave_work(verts, conn, halton_offsets=ref["offsets"])
+        finally:
+            k.close()
+    finally:
+        dev.free()
+    _check_instantiation(res["stats"], 16, 6, 4)
+    assert np.array_equal(res["niterations"], ref["niterations"])
+    assert res["stats"]["evaluations"] == ref["evaluations"] > 0
+    assert _rel(res["values"], ref["values"]) < TOL
+    errs = [_rel(res["values"][:, j], ref["values"][:, j]) for j in range(p.nkernel) if np.any(ref["values"][:, j])]
+    assert len(errs) > 800 and max(errs) < 1e-4, max(errs)
