@@ -615,6 +615,46 @@ def self_check_task(p, verts, conn, sample):
             "tolerance": 1e-5}
 
 
+CANDIDATES = ["per-pair FFTs (k_bwd_eval)", "matrix cores, 1 wave per 16 pairs",
+              "matrix cores, 1 wave per 16 pairs, XCD-contiguous groups",
+              "matrix cores, 2 waves per element (spectra in LDS)", "matrix cores, 4 waves per element (spectra in LDS)"]
+CAND_ORDER = [4, 0, 1, 2, 3]          # candidate_ms index of each row above
+
+
+def tune_report(args):
+    """mck_tune on each workload's kernel set (2048 points of its first tetrahedra; c4 on a
+    3,000-quad database): the time of every candidate, the choice and its gain over the
+    default (one JSON line)."""
+    import mc_kernel_amd as mk
+    from mc_kernel_amd import synth
+    rows = []
+    for name in ("c0", "c1", "c2", "c4"):
+        kw = {"nelem_target": 3000, "ntet": 64} if name == "c4" else {}
+        p = synth.make_problem(name, fill=False, **kw)
+        dev = synth.DeviceDatabases(p)
+        k = mk.MCKernel(p.setup)
+        try:
+            x = p.sample_points(2048, seed=17)
+            t = k.tune(x, reps=3)
+        finally:
+            k.close()
+            dev.free()
+        cms = t["candidate_ms"]
+        row = {"config": name, "nt": p.nt, "kernels": p.nkernel, "receivers": p.nrec,
+               "candidates_ms": {CANDIDATES[i]: (cms[j] or None) for i, j in enumerate(CAND_ORDER)},
+               "per_launch_choice_ms": t["auto_ms"] or None,
+               "costs_us": {"group": t["cost_group_us"], "element_run": t["cost_run_us"], "pair": t["cost_pair_us"]},
+               "chosen": {"bwd_path": t["bwd_path"], "mfma_waves": t["mfma_waves"], "mfma_xcd": t["mfma_xcd"]},
+               "best_ms": t["best_ms"], "default_ms": t["default_ms"],
+               "gain_over_default": t["default_ms"] / t["best_ms"] if t["best_ms"] else None,
+               "gain_over_per_pair": (cms[4] / t["best_ms"]) if cms[4] and t["best_ms"] else None}
+        rows.append(row)
+        print("tune %s: %s" % (name, json.dumps(row)), file=sys.stderr, flush=True)
+    print(json.dumps({"tune_report": rows, "points": 2048, "reps": 3,
+                      "timed": "mck_base_kernels of the points (all receivers), best of 3 after one untimed run"}),
+          flush=True)
+
+
 def task_curve(k, p, sizes, v0, nvox):
     """evals/s of the same voxels [v0, v0 + nvox) run as tasks of each size (the master
     hands out tasks of ELEMENTS_PER_TASK voxels, master_queue.f90:223-249); one warm-up
@@ -674,6 +714,10 @@ def main():
     ap.add_argument("--db-host-memory", choices=["pinned", "pageable"], default="pinned")
     ap.add_argument("--db-device-pages", type=int, default=0,
                     help="device page buffers per database when paging (0: 2; c4: every page generated)")
+    ap.add_argument("--tune", action="store_true",
+                    help="mck_tune on 2048 points of the task before the timed steps (choice in the line)")
+    ap.add_argument("--tune-report", action="store_true",
+                    help="the tuner's candidate times for c0, c1, c2 and c4 instead (one JSON line)")
     ap.add_argument("--c4-layer", type=int, default=15, help="c4: layer of the voxel slice (0 = top)")
     ap.add_argument("--c4-voxels", type=int, default=1024, help="c4: voxels of the slice")
     ap.add_argument("--c4-cpu-steps", type=int, default=1, help="c4: MC steps of the CPU sample's voxels")
@@ -692,6 +736,9 @@ def main():
         return
     if args.integrals:
         bench_integrals(args)
+        return
+    if args.tune_report:
+        tune_report(args)
         return
     if args.config == "c4":
         bench_c4(args)
@@ -721,6 +768,7 @@ def main():
         p.setup.db_page_elems = args.db_page_elems
         p.setup.db_device_pages = args.db_device_pages
     k = mk.MCKernel(p.setup)
+    tuning = k.tune(p.sample_points(2048, seed=17), reps=2) if args.tune else None
     v0 = min(max(args.first_voxel, 0), p.ntet - 1)
     T = args.task if args.task > 0 else p.ntet - v0
     ntask = (p.ntet - v0 + T - 1) // T
@@ -787,6 +835,8 @@ def main():
     }
     if curve is not None:
         line["task_curve"] = curve
+    if tuning is not None:
+        line["tuning"] = tuning
     if sample is not None and host_dbs is None:
         line["self_check"] = self_check(p, sample)
         print("self-check:", json.dumps(line["self_check"]), file=sys.stderr, flush=True)

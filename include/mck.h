@@ -211,6 +211,11 @@ typedef struct {
                                /* (path 1); both timed with events on the library stream */
     int32_t mfma_launches;     /* launches of the matrix-core evaluation kernel          */
     int32_t mfma_groups;       /* waves (groups of 16 pairs) over those launches         */
+    int64_t mfma_runs;         /* element runs (element spectra) over those launches     */
+    double  pair_ms;           /* summed time of the per-pair kernel (k_bwd_eval)        */
+    int64_t pair_pairs;        /* pairs it evaluated                                     */
+    int32_t pair_launches;     /* its launches                                           */
+    int32_t reserved;
 } mck_stats_t;
 
 /*
@@ -483,6 +488,47 @@ int mck_results_load(const double *kernel, const double *variance, const double 
 /* One output variable of finalize (master_queue.f90:327-581) in single precision, laid
  * out as the output files hold it: entry-major, i.e. Fortran (npoints, nentries). */
 int mck_results_export(int32_t which, float *out);
+
+/*
+ * Measured choice of the backward evaluation (the counterpart of FFTW's planner, which the
+ * reference selects with FFTW_PLAN, type_parameter.f90:237, fft.f90:262-270): the
+ * candidates are timed on the caller's points with the databases and kernel set of
+ * mck_init, and the fastest is applied to every later call until mck_finalize.
+ */
+typedef struct {
+    int32_t bwd_path;          /* 0 per-pair FFTs (k_bwd_eval), 1 element spectra and the */
+                               /* matrix cores (k_elem_spectra + k_bwd_mfma), -1 chosen   */
+                               /* per launch by the costs below (the default)             */
+    int32_t mfma_waves;        /* matrix cores: 1 wave per group of 16 pairs, or 2 / 4     */
+                               /* waves per element sharing its spectra through LDS        */
+    int32_t mfma_xcd;          /* 1: groups in XCD-contiguous ranges                       */
+    int32_t ncandidates;       /* candidates timed                                         */
+    double  best_ms;           /* best time of the chosen candidate on the points          */
+    double  default_ms;        /* best time of the default (1, 1, 0; 0 when the kernel set */
+                               /* has no matrix-core form)                                 */
+    double  candidate_ms[5];   /* best time of each candidate (bwd_path, mfma_waves,       */
+                               /* mfma_xcd) = (1,1,0), (1,1,1), (1,2,0), (1,4,0), (0,1,0); */
+                               /* 0 when not timed                                         */
+    /* per-launch choice (bwd_path -1): the matrix cores when cost_group_us x groups of   */
+    /* 16 pairs + cost_run_us x element runs < cost_pair_us x pairs, else the per-pair    */
+    /* kernel; measured by mck_tune (the matrix-core and per-pair kernels' event times),  */
+    /* defaults scaled from c1's at mck_init                                              */
+    double  cost_group_us, cost_run_us, cost_pair_us;
+    double  auto_ms;           /* best time of the per-launch choice on the points        */
+} mck_tuning_t;
+
+/*
+ * mck_tune: times every candidate reps times (after one untimed run) evaluating the base
+ * kernels of the npts points xyz[npts][3] (all receivers, as mck_base_kernels), measures
+ * the per-launch costs from the kernels' event times, and applies the per-launch choice
+ * with the fastest matrix-core variant (or a fixed candidate that beats it on the points);
+ * *out describes it.  Statistics (mck_get_stats) after mck_tune are those of its runs.  Setting MCK_TUNE=1 in the environment makes mck_init
+ * tune itself on 2048 points spread through the mantle.  mck_set_tuning applies a choice
+ * saved from an earlier mck_tune (no timing).  Results are the same for every choice to
+ * the tolerance of the path; the choice only moves time.
+ */
+int mck_tune(int32_t npts, const double *xyz, int32_t reps, mck_tuning_t *out);
+int mck_set_tuning(const mck_tuning_t *tuning);
 
 /* Statistics of the last mck_slave_work / mck_slave_work_plain call. */
 int mck_get_stats(mck_stats_t *stats);

@@ -116,6 +116,22 @@ class MCKernel:
             out["offsets"] = offs
         return out
 
+    def tune(self, xyz, reps=2):
+        """mck_tune on the points xyz [n][3]: the fastest backward evaluation, applied to
+        the later calls; returns its description (dict of mck_tuning_t)."""
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        t = abi.MckTuning()
+        _check(self.lib.mck_tune(xyz.shape[0], abi.ptr(xyz, C.c_double), reps, C.byref(t)))
+        out = {f: getattr(t, f) for f, _ in abi.MckTuning._fields_}
+        out["candidate_ms"] = list(t.candidate_ms)
+        return out
+
+    def set_tuning(self, tuning):
+        t = abi.MckTuning()
+        for f in ("bwd_path", "mfma_waves", "mfma_xcd"):
+            setattr(t, f, tuning[f])
+        _check(self.lib.mck_set_tuning(C.byref(t)))
+
     def stats(self):
         s = abi.MckStats()
         _check(self.lib.mck_get_stats(C.byref(s)))

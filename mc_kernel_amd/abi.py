@@ -86,7 +86,17 @@ class MckStats(C.Structure):
         ("fwd_elements", C.c_int64), ("bwd_elem_receivers", C.c_int64), ("alg_bytes", C.c_int64),
         ("spectra_ms", C.c_double), ("mfma_ms", C.c_double),
         ("mfma_launches", C.c_int32), ("mfma_groups", C.c_int32),
+        ("mfma_runs", C.c_int64), ("pair_ms", C.c_double), ("pair_pairs", C.c_int64),
+        ("pair_launches", C.c_int32), ("reserved", C.c_int32),
     ]
+
+
+class MckTuning(C.Structure):
+    _fields_ = [("bwd_path", C.c_int32), ("mfma_waves", C.c_int32), ("mfma_xcd", C.c_int32),
+                ("ncandidates", C.c_int32), ("best_ms", C.c_double), ("default_ms", C.c_double),
+                ("candidate_ms", C.c_double * 5),
+                ("cost_group_us", C.c_double), ("cost_run_us", C.c_double), ("cost_pair_us", C.c_double),
+                ("auto_ms", C.c_double)]
 
 
 def ptr(arr, ctype):
@@ -102,7 +112,8 @@ EXPORTS = [
     "mck_load_strain_points", "mck_base_kernels", "mck_seismograms", "mck_integrate_model", "mck_set_hetero", "mck_integrate_hetero", "mck_ad_dims", "mck_ad_transfer",
     "mck_ad_forward", "mck_ad_backward", "mck_results_init", "mck_results_add", "mck_results_add_noalloc",
     "mck_results_get",
-    "mck_results_load", "mck_results_export", "mck_get_stats", "mck_finalize", "mck_last_error",
+    "mck_results_load", "mck_results_export", "mck_tune", "mck_set_tuning", "mck_get_stats", "mck_finalize",
+    "mck_last_error",
 ]
 
 
@@ -125,6 +136,10 @@ def declare(lib):
     lib.mck_slave_work.restype = C.c_int
     lib.mck_slave_work_plain.argtypes = work
     lib.mck_slave_work_plain.restype = C.c_int
+    lib.mck_tune.argtypes = [C.c_int32, f64p, C.c_int32, C.POINTER(MckTuning)]
+    lib.mck_tune.restype = C.c_int
+    lib.mck_set_tuning.argtypes = [C.POINTER(MckTuning)]
+    lib.mck_set_tuning.restype = C.c_int
     lib.mck_slave_work_sequential.argtypes = work
     lib.mck_slave_work_sequential.restype = C.c_int
     lib.mck_locate_points.argtypes = [C.c_int32, C.c_int32, f64p, i32p, f64p, f64p, i32p]

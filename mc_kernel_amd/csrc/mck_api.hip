@@ -399,11 +399,17 @@ struct State {
     DBuf<int> worktab;
     DBuf<float2> Uspec, Hp;                       // element spectra; window spectra, rows fstride
     bool mfma_ok = false, mfma_ok_base = false;   // kernel sets the matrix-core path serves
-    // events around the element-spectra and matrix-core launches of the current step
-    // (start, after the spectra, after the matrix-core kernel), read by collect_events
+    // events around the backward kernels of the current step, read by collect_events:
+    // matrix-core launches (start, after the spectra, after the matrix-core kernel) and
+    // per-pair launches (start, end, null)
     std::vector<std::array<hipEvent_t, 3>> kev;
+    // per-launch choice of the backward evaluation (bwd_path_env -1): microseconds per
+    // group of 16 pairs and per element run of the matrix-core path, per pair of the
+    // per-pair path (set at init from c1's measured costs, measured by mck_tune)
+    double cost_group_us = 0.0, cost_run_us = 0.0, cost_pair_us = 0.0;
     int bwd_path_env = -1;                        // MCK_BWD_PATH: 0 per-pair, 1 matrix cores
     int mfma_xcd = 0;                             // MCK_MFMA_XCD: groups in XCD-contiguous ranges
+    int mfma_waves = 1;                           // MCK_MFMA_WAVES: waves per element run (1, 2, 4)
     size_t uspec_budget = 0;                      // bytes of element spectra per batch
     DBuf<unsigned char> sort_tmp;
     DBuf<double> phys;
@@ -1002,6 +1008,14 @@ hipError_t launch_bwd(State &G, const DevDb &db, int npairs, const int *order, c
     G.stats.nd_f = nf;
     G.stats.nd_b = nb;
     G.stats.bwd_path = 0;
+    std::array<hipEvent_t, 3> ev{};
+    hipError_t e0 = hipEventCreate(&ev[0]);
+    if (e0 == hipSuccess) e0 = hipEventCreate(&ev[1]);
+    if (e0 != hipSuccess) return e0;
+    G.kev.push_back(ev);
+    if ((e0 = hipEventRecord(ev[0], G.stream)) != hipSuccess) return e0;
+    G.stats.pair_launches += 1;
+    G.stats.pair_pairs += npairs;
 #define BWD(SP, NF, NB)                                                                           \
     if (spt_for(pc.nt) == SP && nf == NF && nb == NB) {                                           \
         if (smem > 65536)                                                                         \
@@ -1012,7 +1026,8 @@ hipError_t launch_bwd(State &G, const DevDb &db, int npairs, const int *order, c
                            G.Fspec.p, recs, kerns, G.H.p, G.model_idx.p, G.model.p,              \
                            G.S.model_ndepth, G.S.relative_kernel, G.nk, phys, base_out,         \
                            G.xyz.p, active, P, G.d_rec_r.p);                                     \
-        return hipGetLastError();                                                                 \
+        const hipError_t el = hipGetLastError();                                                  \
+        return el != hipSuccess ? el : hipEventRecord(ev[1], G.stream);                           \
     }
 #define BWD_ND(SP) BWD(SP, 1, 1) BWD(SP, 4, 4) BWD(SP, 4, 6) BWD(SP, 6, 4) BWD(SP, 6, 6)
     BWD_ND(1) BWD_ND(2) BWD_ND(4) BWD_ND(8) BWD_ND(16)
@@ -1025,8 +1040,14 @@ hipError_t launch_bwd(State &G, const DevDb &db, int npairs, const int *order, c
 // elements db holds, through element spectra: the element runs and the groups of at most
 // 16 pairs of one (element, receiver), then per batch of element runs k_elem_spectra and
 // k_bwd_mfma.  Wb, Qb: the operator set-up of the first pair.
-int launch_bwd_mfma(State &G, const DevDb &db, int npairs, const int *keys, const int *order, const float *Wb,
-                    const float *Qb, double *phys, double *base_out, const uint8_t *active, int P) {
+// (a launch returns a hipError_t, or an MCK_* code whose message is already set)
+inline int launch_status(hipError_t e) {
+    return e == hipSuccess ? MCK_OK : fail(MCK_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+}
+inline int launch_status(int rc) { return rc; }
+
+int launch_bwd_mfma(State &G, const DevDb &db, int npairs, const int *keys, const int *order, const Located *loc_b,
+                    const float *Wb, const float *Qb, double *phys, double *base_out, const uint8_t *active, int P) {
     if (npairs <= 0) return MCK_OK;
     const PathConf &pc = base_out ? G.pc_base : G.pc;
     const RecDesc *recs = base_out ? G.recs_base.p : G.recs.p;
@@ -1034,10 +1055,11 @@ int launch_bwd_mfma(State &G, const DevDb &db, int npairs, const int *keys, cons
     const int R = G.R, n = npairs;
     hipStream_t st = G.stream;
     // work tables: flags and their inclusive scans, element runs, key runs, groups
-    HIPCHECK(G.worktab.ensure((size_t)12 * n + 8));
+    HIPCHECK(G.worktab.ensure((size_t)13 * n + 9));
     int *eflag = G.worktab.p, *kflag = eflag + n, *epos = kflag + n, *kpos = epos + n, *gflag = kpos + n,
         *gpos = gflag + n, *erun_first = gpos + n, *erun_elem = erun_first + n + 1, *krun_first = erun_elem + n,
-        *group_first = krun_first + n, *group_erun = group_first + n + 1, *counts = group_erun + n;
+        *group_first = krun_first + n, *group_erun = group_first + n + 1, *erun_group_first = group_erun + n,
+        *counts = erun_group_first + n + 1;
     const dim3 gr((n + 255) / 256), bl(256);
     hipLaunchKernelGGL(k_run_flags, gr, bl, 0, st, n, keys, R, eflag, kflag);
     HIPCHECK(hipGetLastError());
@@ -1052,12 +1074,17 @@ int launch_bwd_mfma(State &G, const DevDb &db, int npairs, const int *keys, cons
     hipLaunchKernelGGL(k_group_flags, gr, bl, 0, st, n, kpos, krun_first, gflag);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipcub::DeviceScan::InclusiveSum(G.sort_tmp.p, need, gflag, gpos, n, st));
-    hipLaunchKernelGGL(k_group_table, gr, bl, 0, st, n, gflag, gpos, epos, group_first, group_erun, counts + 1);
+    hipLaunchKernelGGL(k_group_table, gr, bl, 0, st, n, gflag, gpos, eflag, epos, group_first, group_erun,
+                       erun_group_first, counts + 1);
     HIPCHECK(hipGetLastError());
     int cnt[2] = {0, 0};
     HIPCHECK(hipMemcpyAsync(cnt, counts, 8, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     const int nerun = cnt[0], ngroups = cnt[1];
+    // the per-launch choice: the per-pair kernel when it costs less for these pairs
+    if (G.bwd_path_env < 0 && G.cost_pair_us * npairs < G.cost_group_us * ngroups + G.cost_run_us * nerun)
+        return launch_status(launch_bwd(G, db, npairs, order, loc_b, Wb, Qb, phys, base_out, active, P));
+    G.stats.mfma_runs += nerun;
     const int J = pc.nvar_bwd * 25, KS = (J + 3) / 4;
     const size_t per_elem = (size_t)J * pc.ustride * sizeof(float2);
     const int batch = (int)std::max<size_t>(1, std::min<size_t>((size_t)nerun, G.uspec_budget / per_elem));
@@ -1091,7 +1118,12 @@ int launch_bwd_mfma(State &G, const DevDb &db, int npairs, const int *keys, cons
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipEventRecord(ev[1], st));
         launched = false;
-#define EVAL(NF, NB, NV)                                                                          \
+        // W waves per element run sharing its spectra through LDS, or one wave per group
+        int W = G.mfma_waves;
+        if (W > 1 && mfma_wg_lds(KS, pc.nd_b, pc.max_slots, W) > 160 * 1024) W = W > 2 ? 2 : 1;
+        if (W > 1 && mfma_wg_lds(KS, pc.nd_b, pc.max_slots, W) > 160 * 1024) W = 1;
+        const size_t smem_w = mfma_wg_lds(KS, pc.nd_b, pc.max_slots, W);
+#define EVAL1(NF, NB, NV)                                                                         \
         if (pc.nd_f == NF && pc.nd_b == NB && pc.nvar_bwd == NV) {                                \
             if (smem_m > 65536)                                                                   \
                 (void)hipFuncSetAttribute((const void *)&k_bwd_mfma<NF, NB, NV>,                  \
@@ -1103,8 +1135,24 @@ int launch_bwd_mfma(State &G, const DevDb &db, int npairs, const int *keys, cons
                                active, P, G.d_rec_r.p, G.mfma_xcd);                               \
             launched = true;                                                                      \
         }
+#define EVALW(NF, NB, NV, WW)                                                                     \
+        if (W == WW && pc.nd_f == NF && pc.nd_b == NB && pc.nvar_bwd == NV) {                     \
+            if (smem_w > 65536)                                                                   \
+                (void)hipFuncSetAttribute((const void *)&k_bwd_mfma_wg<NF, NB, NV, WW>,           \
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem_w); \
+            hipLaunchKernelGGL((k_bwd_mfma_wg<NF, NB, NV, WW>), dim3(nb), dim3(64 * WW), smem_w, st, pc, u0, nb, \
+                               erun_group_first, group_first, G.Uspec.p, order, R, Wb, Qb, G.opQf.p,   \
+                               G.Fspec.p, recs, kerns, G.Hp.p, pc.max_slots, G.model_idx.p, G.model.p, \
+                               G.S.model_ndepth, G.S.relative_kernel, G.nk, phys, base_out, G.xyz.p, \
+                               active, P, G.d_rec_r.p);                                           \
+            launched = true;                                                                      \
+        }
+#define EVAL(NF, NB, NV) \
+        if (W == 1) { EVAL1(NF, NB, NV) } else { EVALW(NF, NB, NV, 2) EVALW(NF, NB, NV, 4) }
         EVAL(1, 1, 2) EVAL(1, 1, 3) EVAL(4, 4, 2) EVAL(6, 4, 2) EVAL(4, 6, 3) EVAL(6, 6, 3)
 #undef EVAL
+#undef EVALW
+#undef EVAL1
         if (!launched) return fail(MCK_EINVAL, "no matrix-core kernel for these strain components");
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipEventRecord(ev[2], st));
@@ -1147,11 +1195,6 @@ hipError_t launch_traces(State &G, const DevDb &db, int fwd, int frame, int npts
 // item in launch order, ascending (pages then cover disjoint item ranges and untouched
 // pages are not copied); null: every page covers all items and the kernels skip the items
 // outside it.
-// (a launch returns a hipError_t, or an MCK_* code whose message is already set)
-inline int launch_status(hipError_t e) {
-    return e == hipSuccess ? MCK_OK : fail(MCK_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
-}
-inline int launch_status(int rc) { return rc; }
 
 template <class Launch>
 int run_paged(State &G, DbDev &D, int n, const int *sorted_keys, Launch &&launch, int key_scale = 1) {
@@ -1609,6 +1652,21 @@ int init_impl(const mck_setup_t *s) {
     if (const char *e = std::getenv("MCK_BWD_PATH")) G.bwd_path_env = std::atoi(e);
     G.mfma_xcd = 0;
     if (const char *e = std::getenv("MCK_MFMA_XCD")) G.mfma_xcd = std::atoi(e) != 0;
+    G.mfma_waves = 1;
+    if (const char *e = std::getenv("MCK_MFMA_WAVES")) {
+        const int w = std::atoi(e);
+        G.mfma_waves = (w == 2 || w == 4) ? w : 1;
+    }
+    {
+        // c1 (nt 2048, 4 + 4 components, 2 backward variables): 0.71 us per group, 0.93 us
+        // per element run, 0.0985 us per pair (k_bwd_mfma 45.4 ms / 63,650 groups,
+        // k_elem_spectra 3.0 ms / 3,239 runs, k_bwd_eval 86.2 ms / 875k pairs), scaled
+        // with the trace length and the carried components / traces
+        const double sc = G.pc.nt / 2048.0, J = G.pc.nvar_bwd * 25.0, KS = std::ceil(J / 4.0);
+        G.cost_group_us = 0.71 * sc * (KS * G.pc.nd_b) / (13.0 * 4.0) * (0.5 + 0.5 * G.pc.nd_f / 4.0);
+        G.cost_run_us = 0.93 * sc * J / 50.0;
+        G.cost_pair_us = 0.0985 * sc * (G.pc.nd_f + G.pc.nd_b) / 8.0;
+    }
     // tuning knob: points per evaluation launch (smaller launches keep the forward
     // spectra of a launch in the last-level cache)
     if (const char *e = std::getenv("MCK_CHUNK_POINTS")) {
@@ -1935,7 +1993,7 @@ int eval_chunk(State &G, int P, int w0, int nw, std::vector<std::pair<hipEvent_t
         double *phys = G.phys.p + (size_t)w0 * P * G.nk;
         if (use_mfma(G, false)) {
             if ((rc = run_paged(G, G.bwd, npairs, G.keys_out.p, [&](const DevDb &d, int i0, int i1) {
-                     return launch_bwd_mfma(G, d, i1 - i0, G.keys_out.p + i0, G.vals_out.p + i0,
+                     return launch_bwd_mfma(G, d, i1 - i0, G.keys_out.p + i0, G.vals_out.p + i0, loc_b,
                                             G.opWb.p + (size_t)i0 * nwb, G.opQb.p + (size_t)i0 * 36, phys,
                                             nullptr, act, P);
                  }, R)))
@@ -1959,12 +2017,17 @@ int fold_kernel_events(State &G, bool keep = true) {
     for (auto &ev : G.kev) {
         float a = 0.f, b = 0.f;
         if (keep && rc == MCK_OK) {
-            hipError_t e = hipEventSynchronize(ev[2]);
+            const bool pair = ev[2] == nullptr;
+            hipError_t e = hipEventSynchronize(pair ? ev[1] : ev[2]);
             if (e == hipSuccess) e = hipEventElapsedTime(&a, ev[0], ev[1]);
-            if (e == hipSuccess) e = hipEventElapsedTime(&b, ev[1], ev[2]);
+            if (e == hipSuccess && !pair) e = hipEventElapsedTime(&b, ev[1], ev[2]);
             if (e != hipSuccess) rc = fail(MCK_EHIP, hipGetErrorString(e));
-            G.stats.spectra_ms += a;
-            G.stats.mfma_ms += b;
+            if (pair) {
+                G.stats.pair_ms += a;
+            } else {
+                G.stats.spectra_ms += a;
+                G.stats.mfma_ms += b;
+            }
         }
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -2405,8 +2468,132 @@ extern "C" {
 int mck_init(const mck_setup_t *setup) {
     if (g.init) mck_finalize();
     int rc = init_impl(setup);
-    if (rc) g.init = false;
-    return rc;
+    if (rc) {
+        g.init = false;
+        return rc;
+    }
+    const char *t = std::getenv("MCK_TUNE");
+    if (t && std::atoi(t) != 0) {
+        // 2048 points through the mantle: a Fibonacci sphere at radii between 0.57 and
+        // 0.98 planet radii
+        const int n = 2048;
+        const double R = g.S.planet_radius, gold = kPi * (3.0 - std::sqrt(5.0));
+        std::vector<double> xyz(3 * (size_t)n);
+        for (int i = 0; i < n; ++i) {
+            const double z = 1.0 - 2.0 * (i + 0.5) / n, rho = std::sqrt(1.0 - z * z), ph = gold * i;
+            const double r = R * (0.57 + 0.41 * std::fmod(i * 0.6180339887498949, 1.0));
+            xyz[3 * i] = r * rho * std::cos(ph);
+            xyz[3 * i + 1] = r * rho * std::sin(ph);
+            xyz[3 * i + 2] = r * z;
+        }
+        mck_tuning_t out;
+        if ((rc = mck_tune(n, xyz.data(), 2, &out))) return rc;
+    }
+    return MCK_OK;
+}
+
+int mck_set_tuning(const mck_tuning_t *t) {
+    State &G = g;
+    if (!G.init) return fail(MCK_ENOTINIT, "mck_init has not been called");
+    if (!t || t->bwd_path < -1 || t->bwd_path > 1 || (t->mfma_waves != 1 && t->mfma_waves != 2 && t->mfma_waves != 4))
+        return fail(MCK_EINVAL, "invalid tuning");
+    G.bwd_path_env = t->bwd_path;
+    G.mfma_waves = t->mfma_waves;
+    G.mfma_xcd = t->mfma_xcd ? 1 : 0;
+    if (t->cost_group_us > 0.0 && t->cost_run_us > 0.0 && t->cost_pair_us > 0.0) {
+        G.cost_group_us = t->cost_group_us;
+        G.cost_run_us = t->cost_run_us;
+        G.cost_pair_us = t->cost_pair_us;
+    }
+    return MCK_OK;
+}
+
+int mck_tune(int32_t npts, const double *xyz, int32_t reps, mck_tuning_t *out) {
+    State &G = g;
+    if (!G.init) return fail(MCK_ENOTINIT, "mck_init has not been called");
+    if (npts <= 0 || !xyz || !out) return fail(MCK_EINVAL, "invalid argument");
+    reps = std::max(1, reps);
+    struct Cand { int path, waves, xcd; };
+    const Cand all[5] = {{1, 1, 0}, {1, 1, 1}, {1, 2, 0}, {1, 4, 0}, {0, 1, 0}};
+    const bool mfma = G.mfma_ok_base && G.mfma_ok;
+    std::vector<int> cands;
+    if (mfma) cands = {0, 1, 2, 3};
+    cands.push_back(4);
+    *out = mck_tuning_t{};
+    const int save_path = G.bwd_path_env, save_w = G.mfma_waves, save_x = G.mfma_xcd;
+    std::vector<double> base((size_t)npts * G.nk * 6);
+    // best of reps timed runs after an untimed one; the statistics of the timed runs
+    auto timed = [&](double *best_ms) -> int {
+        int rc = mck_base_kernels(npts, xyz, base.data());
+        G.stats = mck_stats_t{};
+        for (int r = 0; r < reps && rc == MCK_OK; ++r) {
+            const auto t0 = std::chrono::steady_clock::now();
+            rc = mck_base_kernels(npts, xyz, base.data());
+            const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            *best_ms = r == 0 ? ms : std::min(*best_ms, ms);
+        }
+        return rc;
+    };
+    auto restore = [&](int rc) {
+        G.bwd_path_env = save_path;
+        G.mfma_waves = save_w;
+        G.mfma_xcd = save_x;
+        return rc;
+    };
+    double best = 0.0, best_mfma = 0.0;
+    int bi = -1, bm = -1;
+    for (int ci : cands) {
+        const Cand &c = all[ci];
+        G.bwd_path_env = c.path;
+        G.mfma_waves = c.waves;
+        G.mfma_xcd = c.xcd;
+        double t = 0.0;
+        int rc = timed(&t);
+        if (rc) return restore(rc);
+        out->candidate_ms[ci] = t;
+        // the per-launch costs from the kernels' event times
+        if (ci == 0 && G.stats.mfma_groups > 0 && G.stats.mfma_runs > 0) {
+            out->cost_group_us = 1e3 * G.stats.mfma_ms / G.stats.mfma_groups;
+            out->cost_run_us = 1e3 * G.stats.spectra_ms / (double)G.stats.mfma_runs;
+        }
+        if (ci == 4 && G.stats.pair_pairs > 0) out->cost_pair_us = 1e3 * G.stats.pair_ms / (double)G.stats.pair_pairs;
+        if (bi < 0 || t < best) { best = t; bi = ci; }
+        if (ci < 4 && (bm < 0 || t < best_mfma)) { best_mfma = t; bm = ci; }
+    }
+    out->ncandidates = (int32_t)cands.size();
+    out->default_ms = out->candidate_ms[mfma ? 0 : 4];
+    Cand chosen = all[bi];
+    double chosen_ms = best;
+    if (mfma && out->cost_group_us > 0.0 && out->cost_pair_us > 0.0) {
+        // the per-launch choice with the measured costs and the fastest matrix-core variant
+        G.cost_group_us = out->cost_group_us;
+        G.cost_run_us = out->cost_run_us;
+        G.cost_pair_us = out->cost_pair_us;
+        G.bwd_path_env = -1;
+        G.mfma_waves = all[bm].waves;
+        G.mfma_xcd = all[bm].xcd;
+        double t = 0.0;
+        int rc = timed(&t);
+        if (rc) return restore(rc);
+        out->auto_ms = t;
+        // kept unless a fixed candidate is faster by more than the timing noise
+        if (t <= 1.02 * best) {
+            chosen = Cand{-1, all[bm].waves, all[bm].xcd};
+            chosen_ms = t;
+        }
+    } else {
+        out->cost_group_us = G.cost_group_us;
+        out->cost_run_us = G.cost_run_us;
+        out->cost_pair_us = out->cost_pair_us > 0.0 ? out->cost_pair_us : G.cost_pair_us;
+    }
+    G.bwd_path_env = chosen.path;
+    G.mfma_waves = chosen.waves;
+    G.mfma_xcd = chosen.xcd;
+    out->bwd_path = chosen.path;
+    out->mfma_waves = chosen.waves;
+    out->mfma_xcd = chosen.xcd;
+    out->best_ms = chosen_ms;
+    return MCK_OK;
 }
 
 int mck_slave_work(int32_t elem_type, int32_t nvpe, int32_t nelems, int32_t nbasis, int32_t nvertices,
@@ -2534,8 +2721,8 @@ int mck_base_kernels(int32_t npts, const double *xyz, double *base) {
             HIPCHECK(launch_setup(G, false, npairs, G.vals_out.p, nullptr, loc_b));
             const size_t nwb = (size_t)G.pc.nd_b * G.pc.nvar_bwd * 25;
             return run_paged(G, G.bwd, npairs, G.keys_out.p, [&](const DevDb &d, int i0, int i1) {
-                return launch_bwd_mfma(G, d, i1 - i0, G.keys_out.p + i0, G.vals_out.p + i0, G.opWb.p + i0 * nwb,
-                                       G.opQb.p + (size_t)i0 * 36, d_phys, d_base, nullptr, 1);
+                return launch_bwd_mfma(G, d, i1 - i0, G.keys_out.p + i0, G.vals_out.p + i0, loc_b,
+                                       G.opWb.p + i0 * nwb, G.opQb.p + (size_t)i0 * 36, d_phys, d_base, nullptr, 1);
             }, R);
         }
         HIPCHECK(launch_setup(G, false, npts * R, nullptr, nullptr, loc_b));

@@ -62,9 +62,10 @@ __global__ void k_group_flags(int n, const int *kpos, const int *krun_first, int
 }
 
 // groups from the inclusive scan of gflag: group_first[g] (with group_first[ngroups] = n),
-// the element run of the group, *ngroups
-__global__ void k_group_table(int n, const int *gflag, const int *gpos, const int *epos, int *group_first,
-                              int *group_erun, int *ngroups)
+// the element run of the group, the first group of every element run (an element run
+// starts a group; erun_group_first[nerun] = ngroups), *ngroups
+__global__ void k_group_table(int n, const int *gflag, const int *gpos, const int *eflag, const int *epos,
+                              int *group_first, int *group_erun, int *erun_group_first, int *ngroups)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -72,10 +73,12 @@ __global__ void k_group_table(int n, const int *gflag, const int *gpos, const in
         const int g = gpos[i] - 1;
         group_first[g] = i;
         group_erun[g] = epos[i] - 1;
+        if (eflag[i]) erun_group_first[epos[i] - 1] = g;
     }
     if (i == n - 1) {
         *ngroups = gpos[i];
         group_first[gpos[i]] = n;
+        erun_group_first[epos[i]] = gpos[i];
     }
 }
 
@@ -129,26 +132,254 @@ __global__ __launch_bounds__(kStrainThreads) void k_elem_spectra(DevDb db, PathC
 
 constexpr int kMfmaPairs = 16;         // pairs of one wave (the matrix-core columns)
 
-// LDS bytes of k_bwd_mfma (one wave): the B operands [KS][NDB][64] of its 16 pairs and the
+// LDS bytes of one wave's own arrays: the B operands [KS][NDB][64] of its 16 pairs and the
 // FP64 slot sums [nslot][16]
 __host__ __device__ constexpr size_t mfma_wave_lds(int ks, int ndb, int nslot)
 {
     return (size_t)ks * ndb * 64 * sizeof(float) + (size_t)nslot * kMfmaPairs * sizeof(double);
 }
 
+// LDS bytes of k_bwd_mfma_wg with w waves: two 16-bin tiles of the element spectra
+// [4 KS][16] and every wave's own arrays
+__host__ __device__ constexpr size_t mfma_wg_lds(int ks, int ndb, int nslot, int w)
+{
+    return 2 * (size_t)ks * 4 * 16 * sizeof(float2) + (size_t)w * mfma_wave_lds(ks, ndb, nslot);
+}
+
+// One wave's group: its pairs (lane l holds pair l % 16), their receiver and the
+// per-pair coefficients F.B = f^T Amu b (earth frame through the rotations of the point,
+// Qf, and of the pair, Qb) and the traces uf.f, ub.b
+template <int NDF, int NDB>
+struct MfmaGroup {
+    int g0, n, r, p;
+    bool valid;
+    float uf[NDF], ub[NDB], Amu[NDF][NDB];
+};
+
+// Loads group grp: its B operands into sW (sW[(s NDB + c) 64 + l'] = W of pair l' % 16,
+// component c, trace 4 s + l' / 16; zero past the group or the traces), its slot sums
+// zeroed, its coefficients.  Called by the 64 lanes of the wave; the caller synchronises
+// before sW or sAcc is read.
+template <int NDF, int NDB, int NVB>
+__device__ __forceinline__ void mfma_group_setup(MfmaGroup<NDF, NDB> &G, int grp, const int *group_first,
+                                                 const int *order, int R, const float *Wb, const float *Qb,
+                                                 const float *Qf, const PathConf &pc, int nslot, float *sW,
+                                                 double *sAcc, int l)
+{
+    constexpr int J = NVB * 25, KS = (J + 3) / 4;
+    const int jp = l & 15;
+    G.g0 = group_first[grp];
+    G.n = group_first[grp + 1] - G.g0;
+    for (int idx = l; idx < KS * NDB * 64; idx += 64) {
+        const int s = idx / (NDB * 64), rem = idx - s * (NDB * 64), c = rem >> 6, ll = rem & 63;
+        const int pj = ll & 15, j = 4 * s + (ll >> 4);
+        float w = 0.0f;
+        if (pj < G.n && j < J) w = Wb[(size_t)(G.g0 + pj) * NDB * J + c * J + j];
+        sW[idx] = w;
+    }
+    for (int s = l; s < nslot * kMfmaPairs; s += 64) sAcc[s] = 0.0;
+    G.valid = jp < G.n;
+    // the group's pairs share the receiver
+    G.r = order[G.g0] % R;
+    G.p = G.valid ? order[G.g0 + jp] / R : 0;
+    if (NDF > 1) {
+        float qf[6][NDF], qb[6][NDB];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+#pragma unroll
+            for (int m = 0; m < NDF; ++m) qf[i][m] = G.valid ? Qf[(size_t)G.p * 36 + i * 6 + m] : 0.0f;
+#pragma unroll
+            for (int c = 0; c < NDB; ++c) qb[i][c] = G.valid ? Qb[(size_t)(G.g0 + jp) * 36 + i * 6 + c] : 0.0f;
+        }
+#pragma unroll
+        for (int m = 0; m < NDF; ++m) {
+            // the trace is invariant under the rotations: exactly 1 for the diagonal
+            // components (ss, pp, zz), 0 for the others
+            G.uf[m] = pc.comps_f[m] < 3 ? 1.0f : 0.0f;
+#pragma unroll
+            for (int c = 0; c < NDB; ++c) {
+                float a = 0.0f;
+#pragma unroll
+                for (int i = 0; i < 6; ++i) a += qf[i][m] * qb[i][c];
+                G.Amu[m][c] = a;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NDB; ++c) G.ub[c] = pc.comps_b[c] < 3 ? 1.0f : 0.0f;
+    } else {
+        G.uf[0] = 1.0f; G.ub[0] = 1.0f; G.Amu[0][0] = 1.0f;
+    }
+}
+
+// One 16-bin tile t of a group, given its A operands a[s] (lane l: trace 4 s + l / 16 at
+// bin 16 t + l % 16; bins past nomega and traces past J zero): the matrix cores give every
+// pair's NDB carried backward components at the tile's 16 bins (real and imaginary parts
+// as two products); lane l then holds pair l % 16 at bins kb = 16 t + 4 (l / 16) .. + 3 and
+// forms the base kernels with the forward spectra of the pair's point, and the slot sums
+// Re(bk H) of the receiver's window spectra, each product rounded once in fp32 and
+// accumulated in FP64 (the four lanes of a pair reduced per tile in a fixed order).
+template <int NDF, int NDB, int KS>
+__device__ __forceinline__ void mfma_tile(const MfmaGroup<NDF, NDB> &G, const float2 (&a)[KS], const float *sW,
+                                          double *sAcc, int l, int t, const PathConf &pc, const float2 *Fp,
+                                          const RecDesc &rd, const float2 *Hp)
+{
+    const int jp = l & 15, bq = l >> 4, kb = (t << 4) + 4 * bq, fs = pc.fstride, ns = rd.nslot;
+    float4 f4[NDF][2];
+#pragma unroll
+    for (int m = 0; m < NDF; ++m) {
+        const float4 *src = reinterpret_cast<const float4 *>(Fp + (size_t)m * fs + kb);
+        f4[m][0] = G.valid ? src[0] : make_float4(0.f, 0.f, 0.f, 0.f);
+        f4[m][1] = G.valid ? src[1] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    v4f accR[NDB], accI[NDB];
+#pragma unroll
+    for (int c = 0; c < NDB; ++c) {
+        accR[c] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+        accI[c] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+#pragma unroll
+        for (int c = 0; c < NDB; ++c) {
+            const float w = sW[(s * NDB + c) * 64 + l];
+            accR[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s].x, w, accR[c], 0, 0, 0);
+            accI[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s].y, w, accI[c], 0, 0, 0);
+        }
+    }
+    // base kernels of this lane's pair at bins kb + v
+    float2 bk0[4], bk1[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+#pragma clang fp contract(fast)
+        float2 f[NDF], b[NDB];
+#pragma unroll
+        for (int m = 0; m < NDF; ++m)
+            f[m] = (v & 1) ? make_float2(f4[m][v >> 1].z, f4[m][v >> 1].w) : make_float2(f4[m][v >> 1].x, f4[m][v >> 1].y);
+#pragma unroll
+        for (int c = 0; c < NDB; ++c) b[c] = make_float2(accR[c][v], accI[c][v]);
+        if (NDF == 1) {
+            const float2 pr = cmulf(f[0], b[0]);
+            bk0[v] = make_float2(-pr.x, -pr.y);
+            bk1[v] = make_float2(-2.0f * pr.x, -2.0f * pr.y);
+        } else {
+            float2 trf = make_float2(0.f, 0.f), trb = trf;
+#pragma unroll
+            for (int m = 0; m < NDF; ++m) { trf.x += G.uf[m] * f[m].x; trf.y += G.uf[m] * f[m].y; }
+#pragma unroll
+            for (int c = 0; c < NDB; ++c) { trb.x += G.ub[c] * b[c].x; trb.y += G.ub[c] * b[c].y; }
+            const float2 lam = cmulf(trf, trb);
+            bk0[v] = make_float2(-lam.x, -lam.y);
+            bk1[v] = make_float2(0.f, 0.f);
+            if (pc.need_bk & 2) {
+                float2 mu = make_float2(0.f, 0.f);
+#pragma unroll
+                for (int m = 0; m < NDF; ++m) {
+                    float2 gm = make_float2(0.f, 0.f);
+#pragma unroll
+                    for (int c = 0; c < NDB; ++c) { gm.x += G.Amu[m][c] * b[c].x; gm.y += G.Amu[m][c] * b[c].y; }
+                    const float2 tt = cmulf(f[m], gm);
+                    mu.x += tt.x; mu.y += tt.y;
+                }
+                bk1[v] = make_float2(-2.0f * mu.x, -2.0f * mu.y);
+            }
+        }
+    }
+    // slot sums, four slots at a time (their window-spectrum loads in flight together),
+    // the four lanes of a pair reduced in a fixed order
+    for (int s0 = 0; s0 < ns; s0 += 4) {
+        float4 h4[4][2];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int s = s0 + q < ns ? s0 + q : s0;
+            const float4 *src = reinterpret_cast<const float4 *>(Hp + (size_t)rd.slot_h[s] * fs + kb);
+            h4[q][0] = src[0];
+            h4[q][1] = src[1];
+        }
+        double part[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int code = rd.slot_bk[s0 + q < ns ? s0 + q : s0];
+            part[q] = 0.0;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const float2 bk = code == 0 ? bk0[v]
+                                : code == 1 ? bk1[v]
+                                            : make_float2(bk1[v].x - 2.0f * bk0[v].x, bk1[v].y - 2.0f * bk0[v].y);
+                const float4 hh = h4[q][v >> 1];
+                const float2 h = (v & 1) ? make_float2(hh.z, hh.w) : make_float2(hh.x, hh.y);
+                part[q] += (double)__builtin_fmaf(bk.x, h.x, -(bk.y * h.y));
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) part[q] += __shfl_xor(part[q], 16, 64);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) part[q] += __shfl_xor(part[q], 32, 64);
+        if (bq == 0 && G.valid)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (s0 + q < ns) sAcc[(s0 + q) * kMfmaPairs + jp] += part[q];
+    }
+}
+
+// Physical kernels (kernel.f90:665-832) of the group's pairs with the model at the point,
+// after the damping of every receiver processed in this step (readfields.f90:3184-3204);
+// lanes 0..15 (lane l: pair l).  phys: [point][nk]; base_out (optional): [point][nk][6]
+template <int NDF, int NDB>
+__device__ __forceinline__ void mfma_finish(const MfmaGroup<NDF, NDB> &G, int l, const double *sAcc,
+                                            const PathConf &pc, const RecDesc &rd, const KernDesc *kern,
+                                            const int *model_idx, const float *model, int ndepth, int relative,
+                                            int nk, double *phys, double *base_out, const double *xyz,
+                                            const uint8_t *active, int P, int R, const double *rec_xyz)
+{
+    if (l >= kMfmaPairs || !G.valid) return;
+    const int p = G.p, r = G.r, jp = l;
+    const int mi = model_idx[p];
+    const float vp = model[0 * ndepth + mi], vs = model[1 * ndepth + mi], rho_f = model[2 * ndepth + mi];
+    const double rho = rho_f;
+    double damp = 1.0;
+    if (pc.damp_radius > 0.0 && active) {
+        const double x0 = xyz[3 * (size_t)p], x1 = xyz[3 * (size_t)p + 1], x2 = xyz[3 * (size_t)p + 2];
+        const double ds = sqrt((x0 - pc.src_r[0]) * (x0 - pc.src_r[0]) + (x1 - pc.src_r[1]) * (x1 - pc.src_r[1])
+                               + (x2 - pc.src_r[2]) * (x2 - pc.src_r[2]));
+        const double fsrc = ds < pc.damp_radius ? ds / pc.damp_radius : 1.0;
+        for (int r2 = r; r2 < R; ++r2) {
+            if (!active[(size_t)(p / P) * R + r2]) continue;
+            const double *c = rec_xyz + 3 * r2;
+            const double dr = sqrt((x0 - c[0]) * (x0 - c[0]) + (x1 - c[1]) * (x1 - c[1]) + (x2 - c[2]) * (x2 - c[2]));
+            damp = damp * (dr < pc.damp_radius ? dr / pc.damp_radius : 1.0) * fsrc;
+        }
+    }
+    for (int kk = 0; kk < rd.nkernel; ++kk) {
+        const int k = rd.first_kernel + kk;
+        const KernDesc &kd = kern[k];
+        double b[kNumBk];
+#pragma unroll
+        for (int qq = 0; qq < kNumBk; ++qq)
+            b[qq] = kd.slot[qq] >= 0 ? sAcc[kd.slot[qq] * kMfmaPairs + jp] * damp : 0.0;
+        if (base_out) {
+            for (int qq = 0; qq < 6; ++qq) base_out[((size_t)p * nk + k) * 6 + qq] = b[qq];
+            continue;
+        }
+        double v = 0.0;
+        switch (kd.param) {
+        case 0: v = relative ? b[0] * (double)(rho_f * (vp * vp - 2 * (vs * vs))) : b[0]; break;
+        case 3: v = relative ? b[1] * (double)(rho_f * (vs * vs)) : b[1]; break;
+        case 1: v = relative ? 2.0 * rho * (double)(vp * vp) * b[0] : 2.0 * rho * (double)vp * b[0]; break;
+        case 2: v = (relative ? 2.0 * rho * (double)(vs * vs) : 2.0 * rho * (double)vs) * b[6]; break;
+        default: v = 0.0;
+        }
+        phys[(size_t)p * nk + k] = v;
+    }
+}
+
 // ------------------------------------------------------------------------------------
 // k_bwd_mfma: one wave per group of at most 16 pairs of one (element, receiver), groups of
-// element runs u0 .. u0 + ubatch - 1 (the others are left to another launch).  Per 16-bin
-// tile the matrix cores give every pair's NDB carried backward components at 16 bins
-// (real and imaginary parts as two products); lane l then holds pair l % 16 at bins
-// 4 (l / 16) .. +3 and forms the base kernels with the forward spectra of the pair's
-// point, and the slot sums Re(bk H) of the receiver's window spectra, each product rounded
-// once in fp32 and accumulated in FP64 (the four lanes of a pair reduced per tile in a
-// fixed order).  Only the lambda and mu base kernels (and mu - 2 lambda) are formed: the
-// host sends kernel sets with anisotropic parameters to k_bwd_eval.  Hp: window spectra,
-// rows fstride apart, zero past nomega; F: forward spectra, rows fstride apart, zero past
-// nomega.  xcd: groups in XCD-contiguous ranges (the groups of an element on one XCD).
-//  phys: [point][nk] values of the receiver's kernels; base_out (optional): [point][nk][6]
+// element runs u0 .. u0 + ubatch - 1 (the others are left to another launch); the wave
+// reads its element's spectra tile by tile from memory.  Only the lambda and mu base
+// kernels (and mu - 2 lambda) are formed: the host sends kernel sets with anisotropic
+// parameters to k_bwd_eval.  Hp: window spectra, rows fstride apart, zero past nomega;
+// F: forward spectra, rows fstride apart, zero past nomega.  xcd: groups in
+// XCD-contiguous ranges (the groups of an element on one XCD).
 // ------------------------------------------------------------------------------------
 template <int NDF, int NDB, int NVB>
 __global__ __launch_bounds__(64, (NDB <= 4 && NDF <= 4) ? 2 : 1) void k_bwd_mfma(
@@ -167,205 +398,104 @@ __global__ __launch_bounds__(64, (NDB <= 4 && NDF <= 4) ? 2 : 1) void k_bwd_mfma
     const int l = threadIdx.x, jp = l & 15, bq = l >> 4;
     float *sW = reinterpret_cast<float *>(smem_raw);
     double *sAcc = reinterpret_cast<double *>(sW + KS * NDB * 64);
-    const int g0 = group_first[grp], n = group_first[grp + 1] - g0;
-    const float2 *Ue = U + (size_t)(eu - u0) * J * pc.ustride;
-    const int nomega = pc.nomega, fs = pc.fstride, ntile = (nomega + 15) >> 4;
-    // B operands of the group's pairs: sW[(s NDB + c) 64 + l'] = W of pair l' % 16,
-    // component c, trace 4 s + l' / 16 (zero past the group or the traces)
-    for (int idx = l; idx < KS * NDB * 64; idx += 64) {
-        const int s = idx / (NDB * 64), rem = idx - s * (NDB * 64), c = rem >> 6, ll = rem & 63;
-        const int pj = ll & 15, j = 4 * s + (ll >> 4);
-        float w = 0.0f;
-        if (pj < n && j < J) w = Wb[(size_t)(g0 + pj) * NDB * J + c * J + j];
-        sW[idx] = w;
-    }
-    for (int s = l; s < nslot * kMfmaPairs; s += 64) sAcc[s] = 0.0;
+    MfmaGroup<NDF, NDB> G;
+    mfma_group_setup<NDF, NDB, NVB>(G, grp, group_first, order, R, Wb, Qb, Qf, pc, nslot, sW, sAcc, l);
     __syncthreads();
-    const bool valid = jp < n;
-    // the group's pairs share the receiver
-    const int r = order[g0] % R;
-    int p = 0;
-    if (valid) p = order[g0 + jp] / R;
-    // per-pair coefficients: F.B = f^T Amu b (earth frame through the rotations of the
-    // point (Qf) and of the pair (Qb)); traces uf.f and ub.b
-    float uf[NDF], ub[NDB], Amu[NDF][NDB];
-    if (NDF > 1) {
-        float qf[6][NDF], qb[6][NDB];
-#pragma unroll
-        for (int i = 0; i < 6; ++i) {
-#pragma unroll
-            for (int m = 0; m < NDF; ++m) qf[i][m] = valid ? Qf[(size_t)p * 36 + i * 6 + m] : 0.0f;
-#pragma unroll
-            for (int c = 0; c < NDB; ++c) qb[i][c] = valid ? Qb[(size_t)(g0 + jp) * 36 + i * 6 + c] : 0.0f;
-        }
-#pragma unroll
-        for (int m = 0; m < NDF; ++m) {
-            // the trace is invariant under the rotations: exactly 1 for the diagonal
-            // components (ss, pp, zz), 0 for the others
-            uf[m] = pc.comps_f[m] < 3 ? 1.0f : 0.0f;
-#pragma unroll
-            for (int c = 0; c < NDB; ++c) {
-                float a = 0.0f;
-#pragma unroll
-                for (int i = 0; i < 6; ++i) a += qf[i][m] * qb[i][c];
-                Amu[m][c] = a;
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < NDB; ++c) ub[c] = pc.comps_b[c] < 3 ? 1.0f : 0.0f;
-    } else {
-        uf[0] = 1.0f; ub[0] = 1.0f; Amu[0][0] = 1.0f;
-    }
-    const RecDesc &rd = recs[r];
-    const int ns = rd.nslot;
-    const float2 *Fp = F + (size_t)p * NDF * fs;
+    const float2 *Ue = U + (size_t)(eu - u0) * J * pc.ustride;
+    const RecDesc &rd = recs[G.r];
+    const float2 *Fp = F + (size_t)G.p * NDF * pc.fstride;
+    const int ntile = (pc.nomega + 15) >> 4;
     for (int t = 0; t < ntile; ++t) {
-        const int b0 = t << 4, kb = b0 + 4 * bq;
-        // loads of the tile: A operands (lane l gives trace 4 s + l / 16 at bin b0 + l % 16;
-        // bins past nomega hold zeros, traces past J are zero) and the forward spectra of
-        // this lane's pair at its bins kb .. kb + 3
         float2 a[KS];
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
             const int j = 4 * s + bq;
-            a[s] = (j < J) ? Ue[(size_t)j * pc.ustride + b0 + jp] : make_float2(0.0f, 0.0f);
+            a[s] = (j < J) ? Ue[(size_t)j * pc.ustride + (t << 4) + jp] : make_float2(0.0f, 0.0f);
         }
-        float4 f4[NDF][2];
-#pragma unroll
-        for (int m = 0; m < NDF; ++m) {
-            const float4 *src = reinterpret_cast<const float4 *>(Fp + (size_t)m * fs + kb);
-            f4[m][0] = valid ? src[0] : make_float4(0.f, 0.f, 0.f, 0.f);
-            f4[m][1] = valid ? src[1] : make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-        v4f accR[NDB], accI[NDB];
-#pragma unroll
-        for (int c = 0; c < NDB; ++c) {
-            accR[c] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-            accI[c] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-        }
-#pragma unroll
-        for (int s = 0; s < KS; ++s) {
-#pragma unroll
-            for (int c = 0; c < NDB; ++c) {
-                const float w = sW[(s * NDB + c) * 64 + l];
-                accR[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s].x, w, accR[c], 0, 0, 0);
-                accI[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s].y, w, accI[c], 0, 0, 0);
-            }
-        }
-        // base kernels of this lane's pair at bins kb + v
-        float2 bk0[4], bk1[4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-#pragma clang fp contract(fast)
-            float2 f[NDF], b[NDB];
-#pragma unroll
-            for (int m = 0; m < NDF; ++m)
-                f[m] = (v & 1) ? make_float2(f4[m][v >> 1].z, f4[m][v >> 1].w) : make_float2(f4[m][v >> 1].x, f4[m][v >> 1].y);
-#pragma unroll
-            for (int c = 0; c < NDB; ++c) b[c] = make_float2(accR[c][v], accI[c][v]);
-            if (NDF == 1) {
-                const float2 pr = cmulf(f[0], b[0]);
-                bk0[v] = make_float2(-pr.x, -pr.y);
-                bk1[v] = make_float2(-2.0f * pr.x, -2.0f * pr.y);
-            } else {
-                float2 trf = make_float2(0.f, 0.f), trb = trf;
-#pragma unroll
-                for (int m = 0; m < NDF; ++m) { trf.x += uf[m] * f[m].x; trf.y += uf[m] * f[m].y; }
-#pragma unroll
-                for (int c = 0; c < NDB; ++c) { trb.x += ub[c] * b[c].x; trb.y += ub[c] * b[c].y; }
-                const float2 lam = cmulf(trf, trb);
-                bk0[v] = make_float2(-lam.x, -lam.y);
-                bk1[v] = make_float2(0.f, 0.f);
-                if (pc.need_bk & 2) {
-                    float2 mu = make_float2(0.f, 0.f);
-#pragma unroll
-                    for (int m = 0; m < NDF; ++m) {
-                        float2 gm = make_float2(0.f, 0.f);
-#pragma unroll
-                        for (int c = 0; c < NDB; ++c) { gm.x += Amu[m][c] * b[c].x; gm.y += Amu[m][c] * b[c].y; }
-                        const float2 tt = cmulf(f[m], gm);
-                        mu.x += tt.x; mu.y += tt.y;
-                    }
-                    bk1[v] = make_float2(-2.0f * mu.x, -2.0f * mu.y);
-                }
-            }
-        }
-        // slot sums, four slots at a time (their window-spectrum loads in flight together),
-        // the four lanes of a pair reduced in a fixed order
-        for (int s0 = 0; s0 < ns; s0 += 4) {
-            float4 h4[4][2];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int s = s0 + q < ns ? s0 + q : s0;
-                const float4 *src = reinterpret_cast<const float4 *>(Hp + (size_t)rd.slot_h[s] * fs + kb);
-                h4[q][0] = src[0];
-                h4[q][1] = src[1];
-            }
-            double part[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int code = rd.slot_bk[s0 + q < ns ? s0 + q : s0];
-                part[q] = 0.0;
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const float2 bk = code == 0 ? bk0[v]
-                                    : code == 1 ? bk1[v]
-                                                : make_float2(bk1[v].x - 2.0f * bk0[v].x, bk1[v].y - 2.0f * bk0[v].y);
-                    const float4 hh = h4[q][v >> 1];
-                    const float2 h = (v & 1) ? make_float2(hh.z, hh.w) : make_float2(hh.x, hh.y);
-                    part[q] += (double)__builtin_fmaf(bk.x, h.x, -(bk.y * h.y));
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) part[q] += __shfl_xor(part[q], 16, 64);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) part[q] += __shfl_xor(part[q], 32, 64);
-            if (bq == 0 && valid)
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    if (s0 + q < ns) sAcc[(s0 + q) * kMfmaPairs + jp] += part[q];
-        }
+        mfma_tile<NDF, NDB, KS>(G, a, sW, sAcc, l, t, pc, Fp, rd, Hp);
     }
-    // physical kernels (kernel.f90:665-832) with the model at the point, after the damping
-    // of every receiver processed in this step (readfields.f90:3184-3204)
-    if (bq == 0 && valid) {
-        const int mi = model_idx[p];
-        const float vp = model[0 * ndepth + mi], vs = model[1 * ndepth + mi], rho_f = model[2 * ndepth + mi];
-        const double rho = rho_f;
-        double damp = 1.0;
-        if (pc.damp_radius > 0.0 && active) {
-            const double x0 = xyz[3 * (size_t)p], x1 = xyz[3 * (size_t)p + 1], x2 = xyz[3 * (size_t)p + 2];
-            const double ds = sqrt((x0 - pc.src_r[0]) * (x0 - pc.src_r[0]) + (x1 - pc.src_r[1]) * (x1 - pc.src_r[1])
-                                   + (x2 - pc.src_r[2]) * (x2 - pc.src_r[2]));
-            const double fsrc = ds < pc.damp_radius ? ds / pc.damp_radius : 1.0;
-            for (int r2 = r; r2 < R; ++r2) {
-                if (!active[(size_t)(p / P) * R + r2]) continue;
-                const double *c = rec_xyz + 3 * r2;
-                const double dr = sqrt((x0 - c[0]) * (x0 - c[0]) + (x1 - c[1]) * (x1 - c[1]) + (x2 - c[2]) * (x2 - c[2]));
-                damp = damp * (dr < pc.damp_radius ? dr / pc.damp_radius : 1.0) * fsrc;
+    mfma_finish<NDF, NDB>(G, l, sAcc, pc, rd, kern, model_idx, model, ndepth, relative, nk, phys, base_out, xyz,
+                          active, P, R, rec_xyz);
+}
+
+// ------------------------------------------------------------------------------------
+// k_bwd_mfma_wg: the same evaluation with one workgroup of W waves per element run
+// u0 + blockIdx.x: the waves take the run's groups W at a time (wave w: group
+// erun_group_first[u] + round + w) and sweep the bins together, each 16-bin tile of the
+// element's spectra loaded once into LDS for the W groups, double-buffered (the next
+// tile's loads are issued before this tile's products and stored after them).  Waves
+// without a group in the last round load and wait with the others.
+// ------------------------------------------------------------------------------------
+template <int NDF, int NDB, int NVB, int W>
+__global__ __launch_bounds__(64 * W) void k_bwd_mfma_wg(
+    PathConf pc, int u0, int ubatch, const int *erun_group_first, const int *group_first,
+    const float2 *U, const int *order, int R, const float *Wb, const float *Qb, const float *Qf,
+    const float2 *F, const RecDesc *recs, const KernDesc *kern, const float2 *Hp, int nslot,
+    const int *model_idx, const float *model, int ndepth, int relative, int nk, double *phys, double *base_out,
+    const double *xyz, const uint8_t *active, int P, const double *rec_xyz)
+{
+    constexpr int J = NVB * 25, KS = (J + 3) / 4, TILE = KS * 4 * 16;   // float2 per tile
+    constexpr int PER = (TILE + 64 * W - 1) / (64 * W);
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    if ((int)blockIdx.x >= ubatch) return;
+    const int u = u0 + (int)blockIdx.x;
+    const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, jp = l & 15, bq = l >> 4;
+    float2 *sU = reinterpret_cast<float2 *>(smem_raw);                  // [2][4 KS][16]
+    float *sW = reinterpret_cast<float *>(sU + 2 * TILE) + (size_t)wv * KS * NDB * 64;
+    double *sAcc = reinterpret_cast<double *>(reinterpret_cast<float *>(sU + 2 * TILE) + (size_t)W * KS * NDB * 64)
+                 + (size_t)wv * nslot * kMfmaPairs;
+    const float2 *Ue = U + (size_t)(u - u0) * J * pc.ustride;
+    const int gbeg = erun_group_first[u], gend = erun_group_first[u + 1];
+    const int ntile = (pc.nomega + 15) >> 4;
+    for (int round = gbeg; round < gend; round += W) {
+        const int grp = round + wv;
+        const bool mine = grp < gend;                                  // uniform in the wave
+        MfmaGroup<NDF, NDB> G;
+        if (mine)
+            mfma_group_setup<NDF, NDB, NVB>(G, grp, group_first, order, R, Wb, Qb, Qf, pc, nslot, sW, sAcc, l);
+        // tile 0 (the previous round's readers of both buffers are past the last barrier)
+        for (int i = 0; i < PER; ++i) {
+            const int idx = tid + i * 64 * W;
+            if (idx < TILE) {
+                const int j = idx >> 4;
+                sU[idx] = j < J ? Ue[(size_t)j * pc.ustride + (idx & 15)] : make_float2(0.0f, 0.0f);
             }
         }
-        for (int kk = 0; kk < rd.nkernel; ++kk) {
-            const int k = rd.first_kernel + kk;
-            const KernDesc &kd = kern[k];
-            double b[kNumBk];
+        __syncthreads();
+        const RecDesc &rd = recs[mine ? G.r : 0];
+        const float2 *Fp = F + (size_t)(mine ? G.p : 0) * NDF * pc.fstride;
+        for (int t = 0; t < ntile; ++t) {
+            float2 nx[PER];
+            const bool more = t + 1 < ntile;
+            if (more) {
 #pragma unroll
-            for (int qq = 0; qq < kNumBk; ++qq)
-                b[qq] = kd.slot[qq] >= 0 ? sAcc[kd.slot[qq] * kMfmaPairs + jp] * damp : 0.0;
-            if (base_out) {
-                for (int qq = 0; qq < 6; ++qq) base_out[((size_t)p * nk + k) * 6 + qq] = b[qq];
-                continue;
+                for (int i = 0; i < PER; ++i) {
+                    const int idx = tid + i * 64 * W, j = idx >> 4;
+                    nx[i] = (idx < TILE && j < J) ? Ue[(size_t)j * pc.ustride + ((t + 1) << 4) + (idx & 15)]
+                                                  : make_float2(0.0f, 0.0f);
+                }
             }
-            double v = 0.0;
-            switch (kd.param) {
-            case 0: v = relative ? b[0] * (double)(rho_f * (vp * vp - 2 * (vs * vs))) : b[0]; break;
-            case 3: v = relative ? b[1] * (double)(rho_f * (vs * vs)) : b[1]; break;
-            case 1: v = relative ? 2.0 * rho * (double)(vp * vp) * b[0] : 2.0 * rho * (double)vp * b[0]; break;
-            case 2: v = (relative ? 2.0 * rho * (double)(vs * vs) : 2.0 * rho * (double)vs) * b[6]; break;
-            default: v = 0.0;
+            if (mine) {
+                const float2 *tile = sU + (t & 1) * TILE;
+                float2 a[KS];
+#pragma unroll
+                for (int s = 0; s < KS; ++s) a[s] = tile[(4 * s + bq) * 16 + jp];
+                mfma_tile<NDF, NDB, KS>(G, a, sW, sAcc, l, t, pc, Fp, rd, Hp);
             }
-            phys[(size_t)p * nk + k] = v;
+            if (more) {
+                float2 *nt = sU + ((t + 1) & 1) * TILE;
+#pragma unroll
+                for (int i = 0; i < PER; ++i) {
+                    const int idx = tid + i * 64 * W;
+                    if (idx < TILE) nt[idx] = nx[i];
+                }
+            }
+            __syncthreads();
         }
+        if (mine)
+            mfma_finish<NDF, NDB>(G, l, sAcc, pc, rd, kern, model_idx, model, ndepth, relative, nk, phys, base_out,
+                                  xyz, active, P, R, rec_xyz);
+        __syncthreads();
     }
 }
 

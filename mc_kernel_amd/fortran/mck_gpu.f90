@@ -76,8 +76,19 @@ module mck_gpu
      integer(c_int64_t) :: fwd_elements, bwd_elem_receivers, alg_bytes
      real(c_double) :: spectra_ms, mfma_ms
      integer(c_int32_t) :: mfma_launches, mfma_groups
+     integer(c_int64_t) :: mfma_runs
+     real(c_double) :: pair_ms
+     integer(c_int64_t) :: pair_pairs
+     integer(c_int32_t) :: pair_launches, reserved
   end type mck_stats_t
 
+  type, bind(c), public :: mck_tuning_t
+     integer(c_int32_t) :: bwd_path, mfma_waves, mfma_xcd, ncandidates
+     real(c_double) :: best_ms, default_ms, candidate_ms(5)
+     real(c_double) :: cost_group_us, cost_run_us, cost_pair_us, auto_ms
+  end type mck_tuning_t
+
+  public :: mck_tune, mck_set_tuning
   public :: mck_init, mck_slave_work, mck_slave_work_plain, mck_slave_work_sequential, mck_locate_points, &
             mck_load_strain_points, mck_base_kernels, mck_seismograms, mck_integrate_model, &
             mck_set_hetero, mck_integrate_hetero, &
@@ -260,6 +271,19 @@ module mck_gpu
        integer(c_int32_t), value :: which
        real(c_float), intent(out) :: out(*)
      end function mck_results_export
+
+     ! the measured choice of the backward evaluation (FFTW_PLAN's counterpart)
+     integer(c_int) function mck_tune(npts, xyz, reps, tuning) bind(c, name='mck_tune')
+       import :: c_int, c_int32_t, c_double, mck_tuning_t
+       integer(c_int32_t), value :: npts, reps
+       real(c_double), intent(in) :: xyz(*)
+       type(mck_tuning_t), intent(out) :: tuning
+     end function mck_tune
+
+     integer(c_int) function mck_set_tuning(tuning) bind(c, name='mck_set_tuning')
+       import :: c_int, mck_tuning_t
+       type(mck_tuning_t), intent(in) :: tuning
+     end function mck_set_tuning
 
      integer(c_int) function mck_get_stats(stats) bind(c, name='mck_get_stats')
        import :: c_int, mck_stats_t

@@ -8,6 +8,7 @@ program layout_check
   type(mck_filter_t), target :: flt
   type(mck_setup_t), target :: s
   type(mck_stats_t), target :: st
+  type(mck_tuning_t), target :: tu
   write(*, '(a, 1x, i0)') 'mck_db_t size', c_sizeof(db)
   write(*, '(a, 1x, i0)') 'mck_db_t snapshots', off(c_loc(db%snapshots), c_loc(db))
   write(*, '(a, 1x, i0)') 'mck_db_t amplitude', off(c_loc(db%amplitude), c_loc(db))
@@ -27,6 +28,10 @@ program layout_check
   write(*, '(a, 1x, i0)') 'mck_stats_t alg_bytes', off(c_loc(st%alg_bytes), c_loc(st))
   write(*, '(a, 1x, i0)') 'mck_stats_t spectra_ms', off(c_loc(st%spectra_ms), c_loc(st))
   write(*, '(a, 1x, i0)') 'mck_stats_t mfma_groups', off(c_loc(st%mfma_groups), c_loc(st))
+  write(*, '(a, 1x, i0)') 'mck_tuning_t size', c_sizeof(tu)
+  write(*, '(a, 1x, i0)') 'mck_tuning_t default_ms', off(c_loc(tu%default_ms), c_loc(tu))
+  write(*, '(a, 1x, i0)') 'mck_tuning_t auto_ms', off(c_loc(tu%auto_ms), c_loc(tu))
+  write(*, '(a, 1x, i0)') 'mck_stats_t pair_launches', off(c_loc(st%pair_launches), c_loc(st))
   write(*, '(a, 1x, a)') 'last_error', mck_error_message()
 contains
   integer function off(a, b)

@@ -15,7 +15,7 @@ HEADER = os.path.join(ROOT, "include", "mck.h")
 LIB = os.path.join(ROOT, "mc_kernel_amd", "libmck.so")
 
 STRUCTS = {"mck_db_t": abi.MckDb, "mck_filter_t": abi.MckFilter,
-           "mck_setup_t": abi.MckSetup, "mck_stats_t": abi.MckStats}
+           "mck_setup_t": abi.MckSetup, "mck_stats_t": abi.MckStats, "mck_tuning_t": abi.MckTuning}
 
 
 def declared_functions():

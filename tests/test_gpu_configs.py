@@ -105,17 +105,24 @@ def _pinned_case(p, spt, nd_f, nd_b, npts=4):
 C1_REDUCED = dict(nelem_target=1000, ntet=3, npoints_per_step=8, max_iter=2, unmasked_frac=1.0)
 
 
-def test_c1_explosion_z():
+@pytest.mark.parametrize("waves", [1, 2, 4])
+def test_c1_explosion_z(monkeypatch, waves):
     """c1's kernel set (10 receivers x 4 Gabor x CC/AM x vp/vs) at nt 2048, explosive
-    source, Z receivers: 8 samples per thread, 4 and 4 carried components."""
+    source, Z receivers: 8 samples per thread, 4 and 4 carried components; the
+    matrix-core evaluation with one wave per group and with 2 and 4 waves per element
+    sharing its spectra (MCK_MFMA_WAVES)."""
+    monkeypatch.setenv("MCK_MFMA_WAVES", str(waves))
     p = synth.make_problem("c1", fill=False, **C1_REDUCED)
     assert p.nkernel == 160 and p.nt == 2048
     _pinned_case(p, 8, 4, 4)
 
 
-def test_c1_general_moment_tensor_r():
+@pytest.mark.parametrize("waves", [1, 2, 4])
+def test_c1_general_moment_tensor_r(monkeypatch, waves):
     """c1's kernel set with a general moment tensor and R receivers: 6 and 6 carried
-    components (all four forward simulations, the PX backward simulation)."""
+    components (all four forward simulations, the PX backward simulation); one, 2 and 4
+    waves per element run."""
+    monkeypatch.setenv("MCK_MFMA_WAVES", str(waves))
     p = synth.make_problem("c1", fill=False, mij=DC, component="R", **C1_REDUCED)
     _pinned_case(p, 8, 6, 6)
 
