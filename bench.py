@@ -663,14 +663,18 @@ def task_curve(k, p, sizes, v0, nvox):
     for T in sizes:
         T = min(T, nvox)
         k.slave_work(p.verts, p.conn[v0:v0 + T])
-        evals, t0 = 0, time.perf_counter()
+        evals, dev_ms, steps, t0 = 0, 0.0, 0, time.perf_counter()
         for a in range(v0, v0 + nvox, T):
             r = k.slave_work(p.verts, p.conn[a:min(a + T, v0 + nvox)])
             evals += r["stats"]["evaluations"]
+            dev_ms += r["stats"]["device_ms"]
+            steps += r["stats"]["mc_steps"]
         dt = time.perf_counter() - t0
-        out.append({"task_voxels": T, "tasks": (nvox + T - 1) // T, "seconds": dt, "evals_per_s": evals / dt})
-        print("task curve: %d-voxel tasks: %.1f M evals/s (%.2f s)" % (T, evals / dt / 1e6, dt),
-              file=sys.stderr, flush=True)
+        out.append({"task_voxels": T, "tasks": (nvox + T - 1) // T, "seconds": dt, "evals_per_s": evals / dt,
+                    "device_seconds": dev_ms * 1e-3, "mc_steps": steps,
+                    "host_ms_per_mc_step": (dt - dev_ms * 1e-3) * 1e3 / max(steps, 1)})
+        print("task curve: %d-voxel tasks: %.1f M evals/s (%.2f s, %.2f s on the device, %d MC steps)"
+              % (T, evals / dt / 1e6, dt, dev_ms * 1e-3, steps), file=sys.stderr, flush=True)
     return out
 
 
