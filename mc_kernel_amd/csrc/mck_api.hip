@@ -820,6 +820,12 @@ int build_kernels(State &G) {
                 ++rd.ngroup;
                 q = e;
             }
+            int n = 0;
+            for (int c = 0; c < 3; ++c) {
+                for (int q = 0; q < rd.nslot; ++q)
+                    if ((rd.slot_bk[q] == 0 ? 0 : rd.slot_bk[q] == 1 ? 1 : 2) == c) rd.cls_order[n++] = q;
+                rd.cls_end[c] = n;
+            }
         }
         pc.max_slots = 1;
         for (int r = 0; r < G.R; ++r) pc.max_slots = std::max(pc.max_slots, lr[r].nslot);

@@ -144,6 +144,10 @@ struct RecDesc {
     // slots ordered by window spectrum, in groups sharing one spectrum (<= kSlotGroup)
     int ngroup;
     int grp_h[kMaxSlots], grp_first[kMaxSlots], grp_n[kMaxSlots];
+    // the slots by base-kernel class for the matrix-core kernel: lambda (slot_bk 0), mu
+    // (1), combinations; class c is cls_order[c ? cls_end[c - 1] : 0 .. cls_end[c] - 1]
+    int cls_order[kMaxSlots];
+    int cls_end[3];
 };
 
 // kernel: model parameter and the slot of every base kernel it needs (-1 = none)

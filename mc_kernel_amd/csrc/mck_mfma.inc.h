@@ -211,6 +211,45 @@ __device__ __forceinline__ void mfma_group_setup(MfmaGroup<NDF, NDB> &G, int grp
     }
 }
 
+// The slot sums of the slots cls_order[b .. e - 1], which share the base kernel bk of
+// this lane's pair at bins kb .. kb + 3: Re(bk H) per bin rounded once in fp32, summed in
+// FP64 over the lane's 4 bins and the pair's 4 lanes, added to sAcc (lanes 0..15)
+__device__ __forceinline__ void mfma_class_sums(const float2 (&bk)[4], int b, int e, bool valid, int l, int kb,
+                                                int fs, const RecDesc &rd, const float2 *Hp, double *sAcc)
+{
+    const int jp = l & 15, bq = l >> 4;
+    for (int s0 = b; s0 < e; s0 += 4) {
+        float4 h4[4][2];
+        int sl[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            sl[q] = rd.cls_order[s0 + q < e ? s0 + q : s0];
+            const float4 *src = reinterpret_cast<const float4 *>(Hp + (size_t)rd.slot_h[sl[q]] * fs + kb);
+            h4[q][0] = src[0];
+            h4[q][1] = src[1];
+        }
+        double part[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            part[q] = 0.0;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const float4 hh = h4[q][v >> 1];
+                const float2 h = (v & 1) ? make_float2(hh.z, hh.w) : make_float2(hh.x, hh.y);
+                part[q] += (double)__builtin_fmaf(bk[v].x, h.x, -(bk[v].y * h.y));
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) part[q] += __shfl_xor(part[q], 16, 64);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) part[q] += __shfl_xor(part[q], 32, 64);
+        if (bq == 0 && valid)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (s0 + q < e) sAcc[sl[q] * kMfmaPairs + jp] += part[q];
+    }
+}
+
 // One 16-bin tile t of a group, given its A operands a[s] (lane l: trace 4 s + l / 16 at
 // bin 16 t + l % 16; bins past nomega and traces past J zero): the matrix cores give every
 // pair's NDB carried backward components at the tile's 16 bins (real and imaginary parts
@@ -223,7 +262,7 @@ __device__ __forceinline__ void mfma_tile(const MfmaGroup<NDF, NDB> &G, const fl
                                           double *sAcc, int l, int t, const PathConf &pc, const float2 *Fp,
                                           const RecDesc &rd, const float2 *Hp)
 {
-    const int jp = l & 15, bq = l >> 4, kb = (t << 4) + 4 * bq, fs = pc.fstride, ns = rd.nslot;
+    const int bq = l >> 4, kb = (t << 4) + 4 * bq, fs = pc.fstride;
     float4 f4[NDF][2];
 #pragma unroll
     for (int m = 0; m < NDF; ++m) {
@@ -284,41 +323,15 @@ __device__ __forceinline__ void mfma_tile(const MfmaGroup<NDF, NDB> &G, const fl
             }
         }
     }
-    // slot sums, four slots at a time (their window-spectrum loads in flight together),
-    // the four lanes of a pair reduced in a fixed order
-    for (int s0 = 0; s0 < ns; s0 += 4) {
-        float4 h4[4][2];
+    // slot sums class by class (lambda, mu, mu - 2 lambda: loops without a per-slot choice),
+    // four slots at a time (their window-spectrum loads in flight together), the four
+    // lanes of a pair reduced in a fixed order
+    float2 bk2[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int s = s0 + q < ns ? s0 + q : s0;
-            const float4 *src = reinterpret_cast<const float4 *>(Hp + (size_t)rd.slot_h[s] * fs + kb);
-            h4[q][0] = src[0];
-            h4[q][1] = src[1];
-        }
-        double part[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int code = rd.slot_bk[s0 + q < ns ? s0 + q : s0];
-            part[q] = 0.0;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const float2 bk = code == 0 ? bk0[v]
-                                : code == 1 ? bk1[v]
-                                            : make_float2(bk1[v].x - 2.0f * bk0[v].x, bk1[v].y - 2.0f * bk0[v].y);
-                const float4 hh = h4[q][v >> 1];
-                const float2 h = (v & 1) ? make_float2(hh.z, hh.w) : make_float2(hh.x, hh.y);
-                part[q] += (double)__builtin_fmaf(bk.x, h.x, -(bk.y * h.y));
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) part[q] += __shfl_xor(part[q], 16, 64);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) part[q] += __shfl_xor(part[q], 32, 64);
-        if (bq == 0 && G.valid)
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (s0 + q < ns) sAcc[(s0 + q) * kMfmaPairs + jp] += part[q];
-    }
+    for (int v = 0; v < 4; ++v) bk2[v] = make_float2(bk1[v].x - 2.0f * bk0[v].x, bk1[v].y - 2.0f * bk0[v].y);
+    mfma_class_sums(bk0, 0, rd.cls_end[0], G.valid, l, kb, fs, rd, Hp, sAcc);
+    mfma_class_sums(bk1, rd.cls_end[0], rd.cls_end[1], G.valid, l, kb, fs, rd, Hp, sAcc);
+    mfma_class_sums(bk2, rd.cls_end[1], rd.cls_end[2], G.valid, l, kb, fs, rd, Hp, sAcc);
 }
 
 // Physical kernels (kernel.f90:665-832) of the group's pairs with the model at the point,
