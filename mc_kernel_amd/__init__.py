@@ -25,7 +25,9 @@ class MckError(RuntimeError):
 
 
 def library_path():
-    return os.path.join(_HERE, "libmck.so")
+    """The in-tree libmck.so, or the build that MCK_LIB names (an A/B build of the same
+    sources with other compile-time settings)."""
+    return os.environ.get("MCK_LIB") or os.path.join(_HERE, "libmck.so")
 
 
 def bind_hip_runtime():
