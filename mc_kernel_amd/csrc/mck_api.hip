@@ -822,13 +822,22 @@ int build_kernels(State &G) {
             }
             int n = 0;
             for (int c = 0; c < 3; ++c) {
+                const int b = n;
                 for (int q = 0; q < rd.nslot; ++q)
-                    if ((rd.slot_bk[q] == 0 ? 0 : rd.slot_bk[q] == 1 ? 1 : 2) == c) rd.cls_order[n++] = q;
+                    if ((rd.slot_bk[q] == 0 ? 0 : rd.slot_bk[q] == 1 ? 1 : 2) == c) {
+                        rd.cls_pos[q] = n;
+                        rd.cls_h[n++] = rd.slot_h[q];
+                    }
+                while ((n - b) & 3) { rd.cls_h[n] = rd.cls_h[b]; ++n; }
                 rd.cls_end[c] = n;
             }
         }
         pc.max_slots = 1;
-        for (int r = 0; r < G.R; ++r) pc.max_slots = std::max(pc.max_slots, lr[r].nslot);
+        pc.max_cls = 4;
+        for (int r = 0; r < G.R; ++r) {
+            pc.max_slots = std::max(pc.max_slots, lr[r].nslot);
+            pc.max_cls = std::max(pc.max_cls, lr[r].cls_end[2]);
+        }
         return MCK_OK;
     };
     std::vector<RecDesc> recs_base;
@@ -1096,7 +1105,7 @@ int launch_bwd_mfma(State &G, const DevDb &db, int npairs, const int *keys, cons
     const int batch = (int)std::max<size_t>(1, std::min<size_t>((size_t)nerun, G.uspec_budget / per_elem));
     HIPCHECK(G.Uspec.ensure((size_t)batch * J * pc.ustride));
     const size_t smem_s = 2 * (size_t)pc.M * sizeof(double2);
-    const size_t smem_m = mfma_wave_lds(KS, pc.nd_b, pc.max_slots);
+    const size_t smem_m = mfma_wave_lds(KS, pc.nd_b, pc.max_cls);
     const int spt = spt_for(pc.nt);
     G.stats.spt = spt;
     G.stats.nd_f = pc.nd_f;
@@ -1126,9 +1135,9 @@ int launch_bwd_mfma(State &G, const DevDb &db, int npairs, const int *keys, cons
         launched = false;
         // W waves per element run sharing its spectra through LDS, or one wave per group
         int W = G.mfma_waves;
-        if (W > 1 && mfma_wg_lds(KS, pc.nd_b, pc.max_slots, W) > 160 * 1024) W = W > 2 ? 2 : 1;
-        if (W > 1 && mfma_wg_lds(KS, pc.nd_b, pc.max_slots, W) > 160 * 1024) W = 1;
-        const size_t smem_w = mfma_wg_lds(KS, pc.nd_b, pc.max_slots, W);
+        if (W > 1 && mfma_wg_lds(KS, pc.nd_b, pc.max_cls, W) > 160 * 1024) W = W > 2 ? 2 : 1;
+        if (W > 1 && mfma_wg_lds(KS, pc.nd_b, pc.max_cls, W) > 160 * 1024) W = 1;
+        const size_t smem_w = mfma_wg_lds(KS, pc.nd_b, pc.max_cls, W);
 #define EVAL1(NF, NB, NV)                                                                         \
         if (pc.nd_f == NF && pc.nd_b == NB && pc.nvar_bwd == NV) {                                \
             if (smem_m > 65536)                                                                   \
@@ -1136,7 +1145,7 @@ int launch_bwd_mfma(State &G, const DevDb &db, int npairs, const int *keys, cons
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem_m); \
             hipLaunchKernelGGL((k_bwd_mfma<NF, NB, NV>), dim3(grid_x8(ngroups)), dim3(64), smem_m, st, pc, u0, nb, \
                                ngroups, group_first, group_erun, G.Uspec.p, order, R, Wb, Qb, G.opQf.p, \
-                               G.Fspec.p, recs, kerns, G.Hp.p, pc.max_slots, G.model_idx.p, G.model.p, \
+                               G.Fspec.p, recs, kerns, G.Hp.p, pc.max_cls, G.model_idx.p, G.model.p, \
                                G.S.model_ndepth, G.S.relative_kernel, G.nk, phys, base_out, G.xyz.p, \
                                active, P, G.d_rec_r.p, G.mfma_xcd);                               \
             launched = true;                                                                      \
@@ -1148,7 +1157,7 @@ int launch_bwd_mfma(State &G, const DevDb &db, int npairs, const int *keys, cons
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem_w); \
             hipLaunchKernelGGL((k_bwd_mfma_wg<NF, NB, NV, WW>), dim3(nb), dim3(64 * WW), smem_w, st, pc, u0, nb, \
                                erun_group_first, group_first, G.Uspec.p, order, R, Wb, Qb, G.opQf.p,   \
-                               G.Fspec.p, recs, kerns, G.Hp.p, pc.max_slots, G.model_idx.p, G.model.p, \
+                               G.Fspec.p, recs, kerns, G.Hp.p, pc.max_cls, G.model_idx.p, G.model.p, \
                                G.S.model_ndepth, G.S.relative_kernel, G.nk, phys, base_out, G.xyz.p, \
                                active, P, G.d_rec_r.p);                                           \
             launched = true;                                                                      \

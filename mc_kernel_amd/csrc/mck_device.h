@@ -97,6 +97,7 @@ struct PathConf {
     int bk_lds[kNumBk];       // LDS array of (combined) base kernel b (-1 = not stored)
     int nbk_lds;
     int max_slots;            // largest slot count of a receiver
+    int max_cls;              // largest padded class-position count of a receiver
     int op_bytes_f, op_bytes_b;   // LDS of the per-point operator set-up, fwd / bwd
     // dampen_field (readfields.f90:3184-3204): radius (<= 0 disables), source position
     double damp_radius;
@@ -145,8 +146,12 @@ struct RecDesc {
     int ngroup;
     int grp_h[kMaxSlots], grp_first[kMaxSlots], grp_n[kMaxSlots];
     // the slots by base-kernel class for the matrix-core kernel: lambda (slot_bk 0), mu
-    // (1), combinations; class c is cls_order[c ? cls_end[c - 1] : 0 .. cls_end[c] - 1]
-    int cls_order[kMaxSlots];
+    // (1), combinations; class c holds the positions c ? cls_end[c - 1] : 0 .. cls_end[c] - 1,
+    // each class padded to a multiple of 4 positions (padding: another position's window
+    // spectrum, a sum nobody reads); cls_h: window spectrum of a position, cls_pos:
+    // position of a slot
+    int cls_h[kMaxSlots + 12];
+    int cls_pos[kMaxSlots];
     int cls_end[3];
 };
 

@@ -133,17 +133,47 @@ __global__ __launch_bounds__(kStrainThreads) void k_elem_spectra(DevDb db, PathC
 constexpr int kMfmaPairs = 16;         // pairs of one wave (the matrix-core columns)
 
 // LDS bytes of one wave's own arrays: the B operands [KS][NDB][64] of its 16 pairs and the
-// FP64 slot sums [nslot][16]
-__host__ __device__ constexpr size_t mfma_wave_lds(int ks, int ndb, int nslot)
+// FP64 slot sums [npos][16] by class position
+__host__ __device__ constexpr size_t mfma_wave_lds(int ks, int ndb, int npos)
 {
-    return (size_t)ks * ndb * 64 * sizeof(float) + (size_t)nslot * kMfmaPairs * sizeof(double);
+    return (size_t)ks * ndb * 64 * sizeof(float) + (size_t)npos * kMfmaPairs * sizeof(double);
+}
+
+// row and half-wave swaps of two doubles (both dwords): x keeps its own row (half) and
+// takes y's, y takes x's
+__device__ __forceinline__ void swap16(double &x, double &y)
+{
+    auto a = __builtin_amdgcn_permlane16_swap(__double2loint(x), __double2loint(y), false, false);
+    auto b = __builtin_amdgcn_permlane16_swap(__double2hiint(x), __double2hiint(y), false, false);
+    x = __hiloint2double(b[0], a[0]);
+    y = __hiloint2double(b[1], a[1]);
+}
+__device__ __forceinline__ void swap32(double &x, double &y)
+{
+    auto a = __builtin_amdgcn_permlane32_swap(__double2loint(x), __double2loint(y), false, false);
+    auto b = __builtin_amdgcn_permlane32_swap(__double2hiint(x), __double2hiint(y), false, false);
+    x = __hiloint2double(b[0], a[0]);
+    y = __hiloint2double(b[1], a[1]);
+}
+
+// The four lanes of a pair added, for four values at once: returns in lane l the sum of
+// p[l / 16] over the lanes l % 16 + 16 r, as (r0 + r1) + (r2 + r3) (rows of 16 lanes)
+__device__ __forceinline__ double mfma_quad_sum(double (&p)[4])
+{
+    swap16(p[0], p[1]);                        // p0: rows (0,1) of p0 | p1 rows (0,1) ...
+    const double s = p[0] + p[1];              // [p0 r0+r1, p1 r0+r1, p0 r2+r3, p1 r2+r3]
+    swap16(p[2], p[3]);
+    double t = p[2] + p[3];                    // the same for p2, p3
+    double u = s;
+    swap32(u, t);                              // u: [r0+r1 of p0..p3], t: [r2+r3 of p0..p3]
+    return u + t;
 }
 
 // LDS bytes of k_bwd_mfma_wg with w waves: two 16-bin tiles of the element spectra
 // [4 KS][16] and every wave's own arrays
-__host__ __device__ constexpr size_t mfma_wg_lds(int ks, int ndb, int nslot, int w)
+__host__ __device__ constexpr size_t mfma_wg_lds(int ks, int ndb, int npos, int w)
 {
-    return 2 * (size_t)ks * 4 * 16 * sizeof(float2) + (size_t)w * mfma_wave_lds(ks, ndb, nslot);
+    return 2 * (size_t)ks * 4 * 16 * sizeof(float2) + (size_t)w * mfma_wave_lds(ks, ndb, npos);
 }
 
 // One wave's group: its pairs (lane l holds pair l % 16), their receiver and the
@@ -163,7 +193,7 @@ struct MfmaGroup {
 template <int NDF, int NDB, int NVB>
 __device__ __forceinline__ void mfma_group_setup(MfmaGroup<NDF, NDB> &G, int grp, const int *group_first,
                                                  const int *order, int R, const float *Wb, const float *Qb,
-                                                 const float *Qf, const PathConf &pc, int nslot, float *sW,
+                                                 const float *Qf, const PathConf &pc, int npos, float *sW,
                                                  double *sAcc, int l)
 {
     constexpr int J = NVB * 25, KS = (J + 3) / 4;
@@ -177,7 +207,7 @@ __device__ __forceinline__ void mfma_group_setup(MfmaGroup<NDF, NDB> &G, int grp
         if (pj < G.n && j < J) w = Wb[(size_t)(G.g0 + pj) * NDB * J + c * J + j];
         sW[idx] = w;
     }
-    for (int s = l; s < nslot * kMfmaPairs; s += 64) sAcc[s] = 0.0;
+    for (int s = l; s < npos * kMfmaPairs; s += 64) sAcc[s] = 0.0;
     G.valid = jp < G.n;
     // the group's pairs share the receiver
     G.r = order[G.g0] % R;
@@ -211,42 +241,36 @@ __device__ __forceinline__ void mfma_group_setup(MfmaGroup<NDF, NDB> &G, int grp
     }
 }
 
-// The slot sums of the slots cls_order[b .. e - 1], which share the base kernel bk of
-// this lane's pair at bins kb .. kb + 3: Re(bk H) per bin rounded once in fp32, summed in
-// FP64 over the lane's 4 bins and the pair's 4 lanes, added to sAcc (lanes 0..15)
+// The slot sums of the class positions b .. e - 1 (a multiple of 4 apart), which share the
+// base kernel bk of this lane's pair at bins kb .. kb + 3: Re(bk H) per bin rounded once in
+// fp32, summed in FP64 over the lane's 4 bins and the pair's 4 lanes, added to sAcc; four
+// positions at a time (their window-spectrum loads in flight together; lane l adds the sum
+// of position + l / 16)
 __device__ __forceinline__ void mfma_class_sums(const float2 (&bk)[4], int b, int e, bool valid, int l, int kb,
                                                 int fs, const RecDesc &rd, const float2 *Hp, double *sAcc)
 {
     const int jp = l & 15, bq = l >> 4;
     for (int s0 = b; s0 < e; s0 += 4) {
         float4 h4[4][2];
-        int sl[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            sl[q] = rd.cls_order[s0 + q < e ? s0 + q : s0];
-            const float4 *src = reinterpret_cast<const float4 *>(Hp + (size_t)rd.slot_h[sl[q]] * fs + kb);
+            const float4 *src = reinterpret_cast<const float4 *>(Hp + (size_t)rd.cls_h[s0 + q] * fs + kb);
             h4[q][0] = src[0];
             h4[q][1] = src[1];
         }
         double part[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            part[q] = 0.0;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const float4 hh = h4[q][v >> 1];
                 const float2 h = (v & 1) ? make_float2(hh.z, hh.w) : make_float2(hh.x, hh.y);
-                part[q] += (double)__builtin_fmaf(bk[v].x, h.x, -(bk[v].y * h.y));
+                const double t = (double)__builtin_fmaf(bk[v].x, h.x, -(bk[v].y * h.y));
+                part[q] = v ? part[q] + t : t;
             }
         }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) part[q] += __shfl_xor(part[q], 16, 64);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) part[q] += __shfl_xor(part[q], 32, 64);
-        if (bq == 0 && valid)
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (s0 + q < e) sAcc[sl[q] * kMfmaPairs + jp] += part[q];
+        const double tot = mfma_quad_sum(part);        // position s0 + bq, pair jp
+        if (valid) sAcc[(s0 + bq) * kMfmaPairs + jp] += tot;
     }
 }
 
@@ -368,7 +392,7 @@ __device__ __forceinline__ void mfma_finish(const MfmaGroup<NDF, NDB> &G, int l,
         double b[kNumBk];
 #pragma unroll
         for (int qq = 0; qq < kNumBk; ++qq)
-            b[qq] = kd.slot[qq] >= 0 ? sAcc[kd.slot[qq] * kMfmaPairs + jp] * damp : 0.0;
+            b[qq] = kd.slot[qq] >= 0 ? sAcc[rd.cls_pos[kd.slot[qq]] * kMfmaPairs + jp] * damp : 0.0;
         if (base_out) {
             for (int qq = 0; qq < 6; ++qq) base_out[((size_t)p * nk + k) * 6 + qq] = b[qq];
             continue;
@@ -398,7 +422,7 @@ template <int NDF, int NDB, int NVB>
 __global__ __launch_bounds__(64, (NDB <= 4 && NDF <= 4) ? 2 : 1) void k_bwd_mfma(
     PathConf pc, int u0, int ubatch, int ngroups, const int *group_first, const int *group_erun,
     const float2 *U, const int *order, int R, const float *Wb, const float *Qb, const float *Qf,
-    const float2 *F, const RecDesc *recs, const KernDesc *kern, const float2 *Hp, int nslot,
+    const float2 *F, const RecDesc *recs, const KernDesc *kern, const float2 *Hp, int npos,
     const int *model_idx, const float *model, int ndepth, int relative, int nk, double *phys, double *base_out,
     const double *xyz, const uint8_t *active, int P, const double *rec_xyz, int xcd)
 {
@@ -412,7 +436,7 @@ __global__ __launch_bounds__(64, (NDB <= 4 && NDF <= 4) ? 2 : 1) void k_bwd_mfma
     float *sW = reinterpret_cast<float *>(smem_raw);
     double *sAcc = reinterpret_cast<double *>(sW + KS * NDB * 64);
     MfmaGroup<NDF, NDB> G;
-    mfma_group_setup<NDF, NDB, NVB>(G, grp, group_first, order, R, Wb, Qb, Qf, pc, nslot, sW, sAcc, l);
+    mfma_group_setup<NDF, NDB, NVB>(G, grp, group_first, order, R, Wb, Qb, Qf, pc, npos, sW, sAcc, l);
     __syncthreads();
     const float2 *Ue = U + (size_t)(eu - u0) * J * pc.ustride;
     const RecDesc &rd = recs[G.r];
@@ -443,7 +467,7 @@ template <int NDF, int NDB, int NVB, int W>
 __global__ __launch_bounds__(64 * W) void k_bwd_mfma_wg(
     PathConf pc, int u0, int ubatch, const int *erun_group_first, const int *group_first,
     const float2 *U, const int *order, int R, const float *Wb, const float *Qb, const float *Qf,
-    const float2 *F, const RecDesc *recs, const KernDesc *kern, const float2 *Hp, int nslot,
+    const float2 *F, const RecDesc *recs, const KernDesc *kern, const float2 *Hp, int npos,
     const int *model_idx, const float *model, int ndepth, int relative, int nk, double *phys, double *base_out,
     const double *xyz, const uint8_t *active, int P, const double *rec_xyz)
 {
@@ -456,7 +480,7 @@ __global__ __launch_bounds__(64 * W) void k_bwd_mfma_wg(
     float2 *sU = reinterpret_cast<float2 *>(smem_raw);                  // [2][4 KS][16]
     float *sW = reinterpret_cast<float *>(sU + 2 * TILE) + (size_t)wv * KS * NDB * 64;
     double *sAcc = reinterpret_cast<double *>(reinterpret_cast<float *>(sU + 2 * TILE) + (size_t)W * KS * NDB * 64)
-                 + (size_t)wv * nslot * kMfmaPairs;
+                 + (size_t)wv * npos * kMfmaPairs;
     const float2 *Ue = U + (size_t)(u - u0) * J * pc.ustride;
     const int gbeg = erun_group_first[u], gend = erun_group_first[u + 1];
     const int ntile = (pc.nomega + 15) >> 4;
@@ -465,7 +489,7 @@ __global__ __launch_bounds__(64 * W) void k_bwd_mfma_wg(
         const bool mine = grp < gend;                                  // uniform in the wave
         MfmaGroup<NDF, NDB> G;
         if (mine)
-            mfma_group_setup<NDF, NDB, NVB>(G, grp, group_first, order, R, Wb, Qb, Qf, pc, nslot, sW, sAcc, l);
+            mfma_group_setup<NDF, NDB, NVB>(G, grp, group_first, order, R, Wb, Qb, Qf, pc, npos, sW, sAcc, l);
         // tile 0 (the previous round's readers of both buffers are past the last barrier)
         for (int i = 0; i < PER; ++i) {
             const int idx = tid + i * 64 * W;
