@@ -386,6 +386,7 @@ struct State {
     DBuf<int> model_idx;
     DBuf<Located> loc;
     DBuf<float2> Fspec;                 // [point][nd_f][fstride] forward spectra
+    DBuf<float> Ftr;                    // [sorted point][nd_f][nt] forward strain traces
     // operator set-up (k_op_setup): gather weights and rotations of the points (by point
     // id) and of the pairs (by position in the pair order)
     DBuf<float> opWf, opQf, opWb, opQb;
@@ -397,7 +398,8 @@ struct State {
     // backward evaluation through element spectra (k_elem_spectra + k_bwd_mfma): element
     // runs of a launch and the nodal spectra of a batch of runs
     DBuf<int> worktab;
-    DBuf<float2> Uspec, Hp;                       // element spectra; window spectra, rows fstride
+    DBuf<float2> Uspec;                           // element spectra
+    DBuf<float2> Hp;                              // window spectra (re, -im), rows fstride
     bool mfma_ok = false, mfma_ok_base = false;   // kernel sets the matrix-core path serves
     // events around the backward kernels of the current step, read by collect_events:
     // matrix-core launches (start, after the spectra, after the matrix-core kernel) and
@@ -828,12 +830,12 @@ int build_kernels(State &G) {
                         rd.cls_pos[q] = n;
                         rd.cls_h[n++] = rd.slot_h[q];
                     }
-                while ((n - b) & 3) { rd.cls_h[n] = rd.cls_h[b]; ++n; }
+                while ((n - b) & 15) { rd.cls_h[n] = rd.cls_h[b]; ++n; }
                 rd.cls_end[c] = n;
             }
         }
         pc.max_slots = 1;
-        pc.max_cls = 4;
+        pc.max_cls = 16;
         for (int r = 0; r < G.R; ++r) {
             pc.max_slots = std::max(pc.max_slots, lr[r].nslot);
             pc.max_cls = std::max(pc.max_cls, lr[r].cls_end[2]);
@@ -893,18 +895,16 @@ int build_kernels(State &G) {
         HIPCHECK(hipMemcpy(d_idx.p, idx.data(), idx.size() * 4, hipMemcpyHostToDevice));
         HIPCHECK(G.H.ensure((size_t)nH * nw));
         HIPCHECK(G.ad_W.ensure((size_t)nH * nw));
+        // and for the matrix-core backward path as (re, -im), rows fstride apart, zero
+        // beyond nomega
+        HIPCHECK(G.Hp.ensure((size_t)nH * G.pc.fstride));
+        HIPCHECK(hipMemsetAsync(G.Hp.p, 0, (size_t)nH * G.pc.fstride * sizeof(float2), G.stream));
         const long long nth = (long long)nH * nw;
         hipLaunchKernelGGL(k_window_spectra, dim3((unsigned)((nth + 255) / 256)), dim3(256), 0, G.stream, nH, nw, N,
                            df, d_w.p, d_idx.p, d_idx.p + nH, d_idx.p + 2 * nH, d_norm.p, d_idx.p + 3 * nH, d_tab.p,
                            d_tab.p + N, d_tf.p, d_tf.p + (size_t)S.nfilter * nw, d_tr.p, d_tr.p + nw, G.H.p,
-                           G.ad_W.p);
+                           G.ad_W.p, G.Hp.p, G.pc.fstride);
         HIPCHECK(hipGetLastError());
-        // the window spectra again with rows fstride apart, zero beyond nomega (the
-        // matrix-core backward path reads them four bins at a time)
-        HIPCHECK(G.Hp.ensure((size_t)nH * G.pc.fstride));
-        HIPCHECK(hipMemsetAsync(G.Hp.p, 0, (size_t)nH * G.pc.fstride * sizeof(float2), G.stream));
-        HIPCHECK(hipMemcpy2DAsync(G.Hp.p, (size_t)G.pc.fstride * sizeof(float2), G.H.p, (size_t)nw * sizeof(float2),
-                                  (size_t)nw * sizeof(float2), nH, hipMemcpyDeviceToDevice, G.stream));
         HIPCHECK(hipStreamSynchronize(G.stream));
         d_w.release(); d_tab.release(); d_norm.release(); d_tf.release(); d_tr.release(); d_idx.release();
         G.nH = nH;
@@ -944,10 +944,9 @@ int build_kernels(State &G) {
 }
 
 
-// LDS of the forward kernel: the operator set-up, two FFT buffers and the quarter twiddle
-// table
+// LDS of the forward kernel: two FFT buffers and the quarter twiddle table
 size_t strain_smem(const PathConf &pc) {
-    return (size_t)pc.op_bytes_f + (2 * (size_t)pc.M + tq_len(pc.M)) * sizeof(float2);
+    return (2 * (size_t)pc.M + tq_len(pc.M)) * sizeof(float2);
 }
 
 // samples per thread of the strain kernels for a trace length
@@ -978,14 +977,32 @@ hipError_t launch_fwd(State &G, const DevDb &db, int npts, const int *nlive, con
     if (npts <= 0) return hipSuccess;
     const size_t smem = strain_smem(G.pc);
     const dim3 grid(grid_x8(npts)), block(kStrainThreads);
-    const int nd = G.pc.nd_f;
+    const int nd = G.pc.nd_f, nv = G.pc.nvar_fwd;
+    // the strain traces of the points (16 / nd points per wave), then their spectra
+    bool traced = false;
+#define TRC(ND, NV)                                                                               \
+    if (nd == ND && nv == NV) {                                                                   \
+        hipLaunchKernelGGL((k_fwd_traces<ND, NV>), dim3(grid_x8((npts + 16 / ND - 1) / (16 / ND))), \
+                           dim3(64), 0, G.stream, db, G.pc, npts, nlive, order, keys, loc, G.opWf.p, \
+                           G.Ftr.p);                                                              \
+        traced = true;                                                                            \
+    }
+#define TRC_NV(ND) TRC(ND, 2) TRC(ND, 3) TRC(ND, 4) TRC(ND, 5) TRC(ND, 6) TRC(ND, 7) TRC(ND, 8) TRC(ND, 10)
+    TRC_NV(1) TRC(4, 2) TRC(4, 4) TRC_NV(6)                    // (monopole fields: 2 or 4 variables)
+#undef TRC_NV
+#undef TRC
+    if (!traced) return hipErrorInvalidValue;
+    {
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
 #define FWD(SP, ND)                                                                               \
     if (spt_for(G.pc.nt) == SP && nd == ND) {                                                     \
         if (smem > 65536)                                                                         \
             (void)hipFuncSetAttribute((const void *)&k_fwd_spectra<SP, ND>,                       \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);     \
         hipLaunchKernelGGL((k_fwd_spectra<SP, ND>), grid, block, smem, G.stream, db, G.pc,        \
-                           npts, nlive, order, keys, loc, G.opWf.p, G.twq.p, F);                 \
+                           npts, nlive, order, keys, loc, G.Ftr.p, G.twq.p, F);                  \
         return hipGetLastError();                                                                 \
     }
 #define FWD_ND(SP) FWD(SP, 1) FWD(SP, 4) FWD(SP, 6)
@@ -1520,6 +1537,7 @@ int ensure_chunk(State &G, int points) {
     HIPCHECK(G.model_idx.ensure(points));
     HIPCHECK(G.loc.ensure((size_t)points * (1 + R)));
     HIPCHECK(G.Fspec.ensure((size_t)points * G.pc.nd_f * G.pc.fstride));
+    HIPCHECK(G.Ftr.ensure((size_t)points * G.pc.nd_f * G.pc.nt));
     HIPCHECK(G.opWf.ensure((size_t)points * G.pc.nd_f * G.pc.nvar_fwd * 25));
     HIPCHECK(G.opQf.ensure((size_t)points * 36));
     HIPCHECK(G.opWb.ensure(npairs * G.pc.nd_b * G.pc.nvar_bwd * 25));
@@ -1654,7 +1672,8 @@ int init_impl(const mck_setup_t *s) {
     // points per chunk: forward spectra, locations and sort buffers within a memory budget
     size_t free_b = 0, total_b = 0;
     HIPCHECK(hipMemGetInfo(&free_b, &total_b));
-    size_t per_point = (size_t)pc.nd_f * pc.fstride * sizeof(float2) + (size_t)(1 + G.R) * sizeof(Located)
+    size_t per_point = (size_t)pc.nd_f * pc.fstride * sizeof(float2) + (size_t)pc.nd_f * pc.nt * sizeof(float)
+                     + (size_t)(1 + G.R) * sizeof(Located)
                      + (size_t)G.R * 16 + 64
                      + ((size_t)pc.nd_f * pc.nvar_fwd * 25 + 36) * sizeof(float)
                      + (size_t)G.R * ((size_t)pc.nd_b * pc.nvar_bwd * 25 + 36) * sizeof(float);
@@ -1673,12 +1692,12 @@ int init_impl(const mck_setup_t *s) {
         G.mfma_waves = (w == 2 || w == 4) ? w : 1;
     }
     {
-        // c1 (nt 2048, 4 + 4 components, 2 backward variables): 0.71 us per group, 0.93 us
-        // per element run, 0.0985 us per pair (k_bwd_mfma 45.4 ms / 63,650 groups,
+        // c1 (nt 2048, 4 + 4 components, 2 backward variables): 0.45 us per group, 0.93 us
+        // per element run, 0.0985 us per pair (k_bwd_mfma 28.4 ms / 63,650 groups,
         // k_elem_spectra 3.0 ms / 3,239 runs, k_bwd_eval 86.2 ms / 875k pairs), scaled
         // with the trace length and the carried components / traces
         const double sc = G.pc.nt / 2048.0, J = G.pc.nvar_bwd * 25.0, KS = std::ceil(J / 4.0);
-        G.cost_group_us = 0.71 * sc * (KS * G.pc.nd_b) / (13.0 * 4.0) * (0.5 + 0.5 * G.pc.nd_f / 4.0);
+        G.cost_group_us = 0.45 * sc * (KS * G.pc.nd_b) / (13.0 * 4.0) * (0.5 + 0.5 * G.pc.nd_f / 4.0);
         G.cost_run_us = 0.93 * sc * J / 50.0;
         G.cost_pair_us = 0.0985 * sc * (G.pc.nd_f + G.pc.nd_b) / 8.0;
     }
@@ -3323,7 +3342,7 @@ void mck_finalize(void) {
     G.fsum.release(); G.f2sum.release(); G.integral.release(); G.variance.release();
     G.conv.release(); G.done.release(); G.niter.release(); G.draws.release(); G.steps.release();
     G.nmodels.release(); G.xyz.release(); G.model_idx.release(); G.loc.release();
-    G.Fspec.release(); G.phys.release(); G.evals.release(); G.err.release();
+    G.Fspec.release(); G.Ftr.release(); G.phys.release(); G.evals.release(); G.err.release();
     G.opWf.release(); G.opQf.release(); G.opWb.release(); G.opQb.release(); G.page_bounds.release();
     G.nlive.release();
     G.abinv.release(); G.bweights.release();

@@ -12,6 +12,10 @@ constexpr int kMaxKernelsPerRec = 256;
 constexpr int kNN = 12;          // nearest midpoints tested (readfields.f90:1067)
 constexpr int kStrainThreads = 256;
 constexpr int kEvalThreads = 256;
+// accumulators of the matrix-core products (fp32 and FP64 16 x 16 blocks: 4 per lane)
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v4d __attribute__((ext_vector_type(4)));
+
 constexpr int kMaxSlots = 256;   // distinct (window-filter, base kernel) pairs per receiver
 // base kernels of the frequency-domain contraction: lambda, mu, rho (unused), a, b, c
 // (calc_basekernel, kernel.f90:514-570) and the model-independent combinations that the
@@ -147,10 +151,10 @@ struct RecDesc {
     int grp_h[kMaxSlots], grp_first[kMaxSlots], grp_n[kMaxSlots];
     // the slots by base-kernel class for the matrix-core kernel: lambda (slot_bk 0), mu
     // (1), combinations; class c holds the positions c ? cls_end[c - 1] : 0 .. cls_end[c] - 1,
-    // each class padded to a multiple of 4 positions (padding: another position's window
-    // spectrum, a sum nobody reads); cls_h: window spectrum of a position, cls_pos:
-    // position of a slot
-    int cls_h[kMaxSlots + 12];
+    // each class padded to a multiple of 16 positions (blocks of 16 matrix-core columns;
+    // padding: another position's window spectrum, a sum nobody reads); cls_h: window
+    // spectrum of a position, cls_pos: position of a slot
+    int cls_h[kMaxSlots + 48];
     int cls_pos[kMaxSlots];
     int cls_end[3];
 };

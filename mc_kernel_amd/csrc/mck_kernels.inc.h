@@ -8,11 +8,12 @@
 //                 midpoints with single-precision distances (kdtree2.f90:1541-1552) and
 //                 the Newton inverse mappings (finite_elem_mapping.f90) - bit-identical
 //                 element indices
-//  k_fwd_spectra  fused gather + strain + interpolation + azimuthal factors
-//                 (sem_derivatives.f90, spectral_basis.f90:181-314, readfields.f90:
-//                 1138-1465), cumsum (simple_routines.f90:314-341), taper + zero pad
-//                 (fft.f90:546-601) and LDS Stockham real FFT (replacing FFTW in
-//                 fft.f90:375-413) of the forward field, points in element order
+//  k_fwd_traces   gather + strain + interpolation + azimuthal factors of the forward
+//                 field (sem_derivatives.f90, spectral_basis.f90:181-314, readfields.f90:
+//                 1138-1465) on the matrix cores, the points of one element together
+//  k_fwd_spectra  cumsum (simple_routines.f90:314-341), taper + zero pad (fft.f90:
+//                 546-601) and LDS Stockham real FFT (replacing FFTW in fft.f90:375-413)
+//                 of those traces, points in element order
 //  k_bwd_eval     the same for the backward field of every active (point, receiver) pair,
 //                 fused with the fwd x bwd base kernels (kernel.f90:514-570), the filtered
 //                 Parseval window integral (kernel.f90:410-510, 946-1020) in the frequency
@@ -407,7 +408,8 @@ __global__ __launch_bounds__(128) void k_model_integral(int nelem, const double 
 //   S = sum_n w_h[n] e^{2 pi i q (ifirst_h + n) / N}   (FP64, samples in order, angles
 //       from an exact table of 2 pi m / N)
 //   H = -norm df c_q S TF_fwd(q) TF_bwd(q) ts(q),  W = -norm df c_q S rest(q) ts(q)
-// with c_q = 1 at DC and Nyquist, else 2; both stored in single precision.
+// with c_q = 1 at DC and Nyquist, else 2; both stored in single precision, and H once
+// more as (re, -im) in Hm (rows fs apart; nullable).
 // ------------------------------------------------------------------------------------
 __device__ inline double2 cmul_d(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -419,7 +421,7 @@ __global__ __launch_bounds__(256) void k_window_spectra(int nH, int nw, int N, d
                                                          const double *cosT, const double *sinT,
                                                          const double2 *tff, const double2 *tfb,
                                                          const double2 *ts, const double2 *rest, float2 *H,
-                                                         float2 *W)
+                                                         float2 *W, float2 *Hm, int fs)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long long)nH * nw) return;
@@ -441,6 +443,7 @@ __global__ __launch_bounds__(256) void k_window_spectra(int nH, int nw, int N, d
     const double2 wv = cmul_d(cmul_d(S, rest[q]), ts[q]);
     H[i] = make_float2((float)hv.x, (float)hv.y);
     W[i] = make_float2((float)wv.x, (float)wv.y);
+    if (Hm) Hm[(size_t)h * fs + q] = make_float2((float)hv.x, -(float)hv.y);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1505,20 +1508,169 @@ __global__ __launch_bounds__(kStrainThreads) void k_strain_traces(
 }
 
 // ------------------------------------------------------------------------------------
+// k_fwd_traces: the forward strain gather (sem_derivatives.f90, spectral_basis.f90:181-314,
+// readfields.f90:1138-1465: u_c(t) = sum_j W_cj u_j(t) over the nvar x 25 nodal traces j
+// of the point's element, the weights from k_op_setup) on the matrix cores.  One wave per
+// window of PW = 16 / ND sorted items (windows in XCD-contiguous ranges): rows r = ND i + c
+// (item i0 + i, component c) of a 16-row block, K = the traces in steps of 4,
+// v_mfma_f32_16x16x4f32 with A = W of row l % 16 at trace 4 s + l / 16 and B = that trace
+// at 16 samples; lane l loads 4 samples
+// 64 t + 4 (l % 16) .. + 3 of its trace and feeds them to 4 products (sample 64 t + 4 j + q
+// in column j of product q), so that a row's 4 consecutive samples land in one lane.  The
+// window's items of one element share every trace load; a window that spans several
+// elements takes one pass per element with the other rows' weights zero.
+// T: [sorted item][ND][nt]; items not located in db (or keyed past the last element) are
+// left unwritten, as k_fwd_spectra skips them.
+// ------------------------------------------------------------------------------------
+// a float4 of global memory at base + off (base: a generic pointer into device memory;
+// global loads take a 32-bit offset from a uniform base)
+typedef float vec4f __attribute__((ext_vector_type(4)));
+typedef const float __attribute__((address_space(1))) gfloat;
+typedef const vec4f __attribute__((address_space(1))) gvec4f;
+__device__ __forceinline__ float4 gload4(const float *base, int off)
+{
+    const vec4f v = *(gvec4f *)((gfloat *)base + (unsigned)off);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// One element pass of k_fwd_traces over all super-tiles: A operands from registers (wa,
+// KS <= 32) or from LDS (sWa[s][64]); WHOLE: nt a multiple of 64 (no partial tile).  The
+// trace loads of half the K steps (8 when the A operands are in LDS) are issued together,
+// then multiplied (two batches per super-tile: fewer registers, more waves in flight).
+template <int ND, int KS, bool WHOLE>
+__device__ __forceinline__ void fwd_traces_pass(const float *ue, const int *toff, const float *wa,
+                                                const float *sWa, float mf, int e, const int (&eout)[4],
+                                                int i0, int l, int nt, float *T)
+{
+    constexpr int PW = 16 / ND;
+    constexpr bool WREG = KS <= 32;
+    constexpr int CH = WREG ? (KS + 1) / 2 : 8;                // K steps loaded together
+    const int r = l & 15, kq = l >> 4, nst = (nt + 63) >> 6;
+    for (int t = 0; t < nst; ++t) {
+        const int s0 = (t << 6) + 4 * r;                       // this lane's 4 samples
+        v4f d[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) d[q] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 1
+        for (int c0 = 0; c0 < KS; c0 += CH) {
+            float4 b[CH];
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                const int s = c0 + c;
+                b[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (s < KS) {
+                    const int off = toff[4 * s + kq] + s0;
+                    if (WHOLE) {
+                        b[c] = gload4(ue, off);
+                    } else {
+                        if (s0 < nt) b[c].x = ue[off];
+                        if (s0 + 1 < nt) b[c].y = ue[off + 1];
+                        if (s0 + 2 < nt) b[c].z = ue[off + 2];
+                        if (s0 + 3 < nt) b[c].w = ue[off + 3];
+                    }
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);                 // all loads in flight first
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                const int s = c0 + c;
+                if (s >= KS) break;
+                const float a = (WREG ? wa[s < KS ? s : 0] : sWa[s * 64 + l]) * mf;
+                d[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[c].x, d[0], 0, 0, 0);
+                d[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[c].y, d[1], 0, 0, 0);
+                d[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[c].z, d[2], 0, 0, 0);
+                d[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[c].w, d[3], 0, 0, 0);
+            }
+        }
+        // lane l: rows 4 kq + v, samples s0 .. s0 + 3
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int ro = 4 * kq + v, io = ro / ND;
+            if (io >= PW || eout[v] != e) continue;
+            float *dst = T + ((size_t)(i0 + io) * ND + (ro - io * ND)) * nt + s0;
+            if (WHOLE) *reinterpret_cast<float4 *>(dst) = make_float4(d[0][v], d[1][v], d[2][v], d[3][v]);
+            else
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (s0 + q < nt) dst[q] = d[q][v];
+        }
+    }
+}
+
+template <int ND, int NV>
+__global__ __launch_bounds__(64, 2) void k_fwd_traces(DevDb db, PathConf pc, int npts, const int *nlive,
+                                                     const int *order, const int *keys, const Located *loc,
+                                                     const float *Wf, float *T)
+{
+    constexpr int J = NV * 25, KS = (J + 3) / 4, PW = 16 / ND;
+    constexpr bool WREG = KS <= 32;
+    __shared__ int toff[4 * KS];                               // trace j -> offset in the element
+    __shared__ float sWa[WREG ? 1 : KS * 64];                  // A operands when not in registers
+    // windows in XCD-contiguous ranges: the windows of an element share one L2
+    const int n = nlive ? min(*nlive, npts) : npts;
+    const int w = xcd_block_n((n + PW - 1) / PW);
+    if (w < 0) return;
+    const int i0 = w * PW;
+    const int l = threadIdx.x, r = l & 15, kq = l >> 4, nt = pc.nt;
+    // (traces past J: offset 0 with weight 0)
+    for (int j = l; j < 4 * KS; j += 64) {
+        const int v = j / 25;
+        toff[j] = j < J ? (pc.fwd_vars[v] * 25 + (j - v * 25)) * nt : 0;
+    }
+    // this lane's A row: item i0 + r / ND, component r % ND
+    const int ir = r / ND, cr = r - ir * ND, item = i0 + ir;
+    int erow = -1;
+    const float *wrow = nullptr;
+    if (ir < PW && item < n && !(keys && keys[item] >= db.nelem)) {
+        const int p = order ? order[item] : item;
+        erow = loc[p].elem;
+        wrow = Wf + ((size_t)p * ND + cr) * J;
+    }
+    float wa[WREG ? KS : 1];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        const int j = 4 * s + kq;
+        const float w = (wrow && j < J) ? wrow[j] : 0.0f;
+        if constexpr (WREG) wa[s] = w;
+        else sWa[s * 64 + l] = w;
+    }
+    // element of the output rows 4 kq + v of this lane
+    int eout[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) eout[v] = __shfl(erow, 4 * kq + v, 64);
+    __syncthreads();
+    const bool whole = (nt & 63) == 0;                         // super-tiles of 64 samples
+    int prev = -1;
+    for (;;) {
+        // the next element of the window (the wave's smallest above prev)
+        int e = erow > prev ? erow : INT_MAX;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) e = min(e, __shfl_xor(e, o, 64));
+        e = __builtin_amdgcn_readfirstlane(e);                 // (uniform: a scalar base)
+        if (e == INT_MAX) break;
+        prev = e;
+        const float *ue = elem_u(db, e);
+        if (!ue) continue;                                     // not in this page
+        const float mf = erow == e ? 1.0f : 0.0f;
+        if (whole) fwd_traces_pass<ND, KS, true>(ue, toff, wa, sWa, mf, e, eout, i0, l, nt, T);
+        else fwd_traces_pass<ND, KS, false>(ue, toff, wa, sWa, mf, e, eout, i0, l, nt, T);
+    }
+}
+
+// ------------------------------------------------------------------------------------
 // k_fwd_spectra: one workgroup per point of the forward frame, points in element order.
-// Fused gather + strain + interpolation + azimuthal factors (sem_derivatives.f90,
-// spectral_basis.f90:181-314, readfields.f90:1138-1465), cumsum (simple_routines.f90:
-// 314-341), taper + zero pad (fft.f90:546-601) and LDS real FFT (fft.f90:375-413).
+// The point's strain traces (k_fwd_traces: T[sorted item][ND][nt]), cumsum
+// (simple_routines.f90:314-341), taper + zero pad (fft.f90:546-601) and LDS real FFT
+// (fft.f90:375-413).
 // F: [point][ND][fstride] spectra of the carried source-frame components (bins < nomega).
 // ------------------------------------------------------------------------------------
 template <int SPT, int ND>
 __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && ND <= 4) ? 3 : (SPT <= 8 ? 2 : 1)) void k_fwd_spectra(
     DevDb db, PathConf pc, int npts, const int *nlive, const int *order, const int *keys, const Located *loc,
-    const float *Wf, const float2 *twq, float2 *F)
+    const float *T, const float2 *twq, float2 *F)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    OpShared &S = *reinterpret_cast<OpShared *>(smem_raw);
-    float2 *bufA = reinterpret_cast<float2 *>(smem_raw + pc.op_bytes_f);
+    float2 *bufA = reinterpret_cast<float2 *>(smem_raw);
     float2 *bufB = bufA + pc.M;
     float2 *Tq = bufA + 2 * pc.M;
     __shared__ double wave_tot[kStrainThreads / 64];
@@ -1534,18 +1686,28 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && ND <= 4) ? 3 : (SPT <=
     const float *ue = elem_u(db, L.elem);
     if (!ue) return;                                           // not located / not in this page
     load_twiddles(twq, pc.M, Tq);
-    {
-        // gather weights of the point (k_op_setup)
-        const int nw = ND * pc.nvar_fwd * 25;
-        const float *src = Wf + (size_t)p * nw;
-        for (int i = threadIdx.x; i < nw; i += kStrainThreads) S.W[i] = src[i];
-    }
     __syncthreads();
 
     const int tid = threadIdx.x, nt = pc.nt, M = pc.M, t0 = tid * SPT;
-    const size_t blk = (size_t)25 * nt;
+    // the point's strain traces (k_fwd_traces)
     float acc[ND][SPT];
-    gather_strain<SPT, ND>(S, ue, blk, pc.nvar_fwd, pc.fwd_vars, t0, nt, acc);
+    {
+        const float *tr = T + (size_t)lb * ND * nt + t0;
+        const bool full_vec = (t0 + SPT <= nt) && ((nt & 3) == 0) && (SPT % 4) == 0;
+#pragma unroll
+        for (int c = 0; c < ND; ++c) {
+            if (full_vec) {
+#pragma unroll
+                for (int q = 0; q < SPT; q += 4) {
+                    const float4 v = *reinterpret_cast<const float4 *>(tr + (size_t)c * nt + q);
+                    acc[c][q] = v.x; acc[c][q + 1] = v.y; acc[c][q + 2] = v.z; acc[c][q + 3] = v.w;
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < SPT; ++q) acc[c][q] = (t0 + q < nt) ? tr[(size_t)c * nt + q] : 0.0f;
+            }
+        }
+    }
     const float fdt = (float)pc.dt, h = (float)(pc.dt * 0.5);
     float tap[SPT];
     taper_factors(t0, nt, pc, tap);

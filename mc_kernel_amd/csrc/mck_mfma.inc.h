@@ -20,7 +20,6 @@
 
 namespace mck {
 
-typedef float v4f __attribute__((ext_vector_type(4)));
 
 // Work tables of element- and receiver-sorted pair keys (element * R + receiver):
 // element runs (the elements whose spectra a launch needs) and groups of at most 16
@@ -133,40 +132,17 @@ __global__ __launch_bounds__(kStrainThreads) void k_elem_spectra(DevDb db, PathC
 constexpr int kMfmaPairs = 16;         // pairs of one wave (the matrix-core columns)
 
 // LDS bytes of one wave's own arrays: the B operands [KS][NDB][64] of its 16 pairs and the
-// FP64 slot sums [npos][16] by class position
+// FP64 slot sums of its npos class positions (npos / 16 blocks of 16 x 16)
 __host__ __device__ constexpr size_t mfma_wave_lds(int ks, int ndb, int npos)
 {
     return (size_t)ks * ndb * 64 * sizeof(float) + (size_t)npos * kMfmaPairs * sizeof(double);
 }
 
-// row and half-wave swaps of two doubles (both dwords): x keeps its own row (half) and
-// takes y's, y takes x's
-__device__ __forceinline__ void swap16(double &x, double &y)
+// the slot sum of class position pos for pair jp, from the blocks of matrix-core results
+// (block pos / 16: D[pair][pos % 16] in lane pos % 16 + 16 (pair % 4), element pair / 4)
+__device__ __forceinline__ double mfma_slot_sum(const double *sAcc, int pos, int jp)
 {
-    auto a = __builtin_amdgcn_permlane16_swap(__double2loint(x), __double2loint(y), false, false);
-    auto b = __builtin_amdgcn_permlane16_swap(__double2hiint(x), __double2hiint(y), false, false);
-    x = __hiloint2double(b[0], a[0]);
-    y = __hiloint2double(b[1], a[1]);
-}
-__device__ __forceinline__ void swap32(double &x, double &y)
-{
-    auto a = __builtin_amdgcn_permlane32_swap(__double2loint(x), __double2loint(y), false, false);
-    auto b = __builtin_amdgcn_permlane32_swap(__double2hiint(x), __double2hiint(y), false, false);
-    x = __hiloint2double(b[0], a[0]);
-    y = __hiloint2double(b[1], a[1]);
-}
-
-// The four lanes of a pair added, for four values at once: returns in lane l the sum of
-// p[l / 16] over the lanes l % 16 + 16 r, as (r0 + r1) + (r2 + r3) (rows of 16 lanes)
-__device__ __forceinline__ double mfma_quad_sum(double (&p)[4])
-{
-    swap16(p[0], p[1]);                        // p0: rows (0,1) of p0 | p1 rows (0,1) ...
-    const double s = p[0] + p[1];              // [p0 r0+r1, p1 r0+r1, p0 r2+r3, p1 r2+r3]
-    swap16(p[2], p[3]);
-    double t = p[2] + p[3];                    // the same for p2, p3
-    double u = s;
-    swap32(u, t);                              // u: [r0+r1 of p0..p3], t: [r2+r3 of p0..p3]
-    return u + t;
+    return sAcc[(size_t)(pos >> 4) * 256 + 4 * ((pos & 15) + 16 * (jp & 3)) + (jp >> 2)];
 }
 
 // LDS bytes of k_bwd_mfma_wg with w waves: two 16-bin tiles of the element spectra
@@ -241,36 +217,45 @@ __device__ __forceinline__ void mfma_group_setup(MfmaGroup<NDF, NDB> &G, int grp
     }
 }
 
-// The slot sums of the class positions b .. e - 1 (a multiple of 4 apart), which share the
-// base kernel bk of this lane's pair at bins kb .. kb + 3: Re(bk H) per bin rounded once in
-// fp32, summed in FP64 over the lane's 4 bins and the pair's 4 lanes, added to sAcc; four
-// positions at a time (their window-spectrum loads in flight together; lane l adds the sum
-// of position + l / 16)
-__device__ __forceinline__ void mfma_class_sums(const float2 (&bk)[4], int b, int e, bool valid, int l, int kb,
-                                                int fs, const RecDesc &rd, const float2 *Hp, double *sAcc)
+// The slot sums of one tile on the FP64 matrix cores: for every block of 16 class
+// positions (one class, columns c = 0..15: window spectrum cls_h[16 b + c]),
+// D[pair][c] += sum over the tile's 16 bins of Re(bk H) = bk.x H.x + bk.y (-H.y); the
+// fp32 base kernels and window spectra multiply exactly in FP64 and the products are
+// summed in FP64.  This lane holds bk of pair l % 16 at bins kb .. kb + 3 (the A
+// operands: pair l % 16, k = l / 16) and loads column l % 16's spectra at the same bins
+// (the B operands).  D of block b: sAcc[b 256 + 4 l + r] = D[l / 16 + 4 r][l % 16] (the
+// FP64 product's result layout).
+__device__ __forceinline__ void mfma_window_sums(const float2 (&bk0)[4], const float2 (&bk1)[4], int l, int kb,
+                                                 int fs, const RecDesc &rd, const float2 *Hp, double *sAcc)
 {
-    const int jp = l & 15, bq = l >> 4;
-    for (int s0 = b; s0 < e; s0 += 4) {
-        float4 h4[4][2];
+    const int nblk = rd.cls_end[2] >> 4;
+    for (int b = 0; b < nblk; ++b) {
+        const int p0 = b << 4;
+        const int cls = p0 < rd.cls_end[0] ? 0 : (p0 < rd.cls_end[1] ? 1 : 2);
+        const float4 *src = reinterpret_cast<const float4 *>(Hp + (size_t)rd.cls_h[p0 + (l & 15)] * fs + kb);
+        const float4 q0 = src[0], q1 = src[1];
+        const double2 h[4] = {make_double2(q0.x, q0.y), make_double2(q0.z, q0.w), make_double2(q1.x, q1.y),
+                              make_double2(q1.z, q1.w)};
+        double ax[4], ay[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const float4 *src = reinterpret_cast<const float4 *>(Hp + (size_t)rd.cls_h[s0 + q] * fs + kb);
-            h4[q][0] = src[0];
-            h4[q][1] = src[1];
-        }
-        double part[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const float4 hh = h4[q][v >> 1];
-                const float2 h = (v & 1) ? make_float2(hh.z, hh.w) : make_float2(hh.x, hh.y);
-                const double t = (double)__builtin_fmaf(bk[v].x, h.x, -(bk[v].y * h.y));
-                part[q] = v ? part[q] + t : t;
+        for (int v = 0; v < 4; ++v) {
+            if (cls == 0) {
+                ax[v] = bk0[v].x; ay[v] = bk0[v].y;
+            } else if (cls == 1) {
+                ax[v] = bk1[v].x; ay[v] = bk1[v].y;
+            } else {                                            // mu - 2 lambda
+                ax[v] = (double)bk1[v].x - 2.0 * (double)bk0[v].x;
+                ay[v] = (double)bk1[v].y - 2.0 * (double)bk0[v].y;
             }
         }
-        const double tot = mfma_quad_sum(part);        // position s0 + bq, pair jp
-        if (valid) sAcc[(s0 + bq) * kMfmaPairs + jp] += tot;
+        v4d *acc = reinterpret_cast<v4d *>(sAcc + (size_t)b * 256 + 4 * l);
+        v4d dx = *acc, dy = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            dx = __builtin_amdgcn_mfma_f64_16x16x4f64(ax[v], h[v].x, dx, 0, 0, 0);
+            dy = __builtin_amdgcn_mfma_f64_16x16x4f64(ay[v], h[v].y, dy, 0, 0, 0);
+        }
+        *acc = dx + dy;
     }
 }
 
@@ -278,9 +263,8 @@ __device__ __forceinline__ void mfma_class_sums(const float2 (&bk)[4], int b, in
 // bin 16 t + l % 16; bins past nomega and traces past J zero): the matrix cores give every
 // pair's NDB carried backward components at the tile's 16 bins (real and imaginary parts
 // as two products); lane l then holds pair l % 16 at bins kb = 16 t + 4 (l / 16) .. + 3 and
-// forms the base kernels with the forward spectra of the pair's point, and the slot sums
-// Re(bk H) of the receiver's window spectra, each product rounded once in fp32 and
-// accumulated in FP64 (the four lanes of a pair reduced per tile in a fixed order).
+// forms the base kernels with the forward spectra of the pair's point; then the slot sums
+// Re(bk H) with the receiver's window spectra on the FP64 matrix cores.
 template <int NDF, int NDB, int KS>
 __device__ __forceinline__ void mfma_tile(const MfmaGroup<NDF, NDB> &G, const float2 (&a)[KS], const float *sW,
                                           double *sAcc, int l, int t, const PathConf &pc, const float2 *Fp,
@@ -347,15 +331,7 @@ __device__ __forceinline__ void mfma_tile(const MfmaGroup<NDF, NDB> &G, const fl
             }
         }
     }
-    // slot sums class by class (lambda, mu, mu - 2 lambda: loops without a per-slot choice),
-    // four slots at a time (their window-spectrum loads in flight together), the four
-    // lanes of a pair reduced in a fixed order
-    float2 bk2[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) bk2[v] = make_float2(bk1[v].x - 2.0f * bk0[v].x, bk1[v].y - 2.0f * bk0[v].y);
-    mfma_class_sums(bk0, 0, rd.cls_end[0], G.valid, l, kb, fs, rd, Hp, sAcc);
-    mfma_class_sums(bk1, rd.cls_end[0], rd.cls_end[1], G.valid, l, kb, fs, rd, Hp, sAcc);
-    mfma_class_sums(bk2, rd.cls_end[1], rd.cls_end[2], G.valid, l, kb, fs, rd, Hp, sAcc);
+    mfma_window_sums(bk0, bk1, l, kb, fs, rd, Hp, sAcc);
 }
 
 // Physical kernels (kernel.f90:665-832) of the group's pairs with the model at the point,
@@ -392,7 +368,7 @@ __device__ __forceinline__ void mfma_finish(const MfmaGroup<NDF, NDB> &G, int l,
         double b[kNumBk];
 #pragma unroll
         for (int qq = 0; qq < kNumBk; ++qq)
-            b[qq] = kd.slot[qq] >= 0 ? sAcc[rd.cls_pos[kd.slot[qq]] * kMfmaPairs + jp] * damp : 0.0;
+            b[qq] = kd.slot[qq] >= 0 ? mfma_slot_sum(sAcc, rd.cls_pos[kd.slot[qq]], jp) * damp : 0.0;
         if (base_out) {
             for (int qq = 0; qq < 6; ++qq) base_out[((size_t)p * nk + k) * 6 + qq] = b[qq];
             continue;
@@ -451,6 +427,7 @@ __global__ __launch_bounds__(64, (NDB <= 4 && NDF <= 4) ? 2 : 1) void k_bwd_mfma
         }
         mfma_tile<NDF, NDB, KS>(G, a, sW, sAcc, l, t, pc, Fp, rd, Hp);
     }
+    __syncthreads();                                           // the lanes' blocks of sums
     mfma_finish<NDF, NDB>(G, l, sAcc, pc, rd, kern, model_idx, model, ndepth, relative, nk, phys, base_out, xyz,
                           active, P, R, rec_xyz);
 }
