@@ -978,13 +978,16 @@ hipError_t launch_fwd(State &G, const DevDb &db, int npts, const int *nlive, con
     const size_t smem = strain_smem(G.pc);
     const dim3 grid(grid_x8(npts)), block(kStrainThreads);
     const int nd = G.pc.nd_f, nv = G.pc.nvar_fwd;
-    // the strain traces of the points (16 / nd points per wave), then their spectra
+    // the strain traces of the points (16 / nd points per wave; a launch of fewer than
+    // 8,192 windows splits each window's samples over up to nt / 64 waves), then their
+    // spectra
     bool traced = false;
+    const int nwin = (npts + 16 / nd - 1) / (16 / nd);
+    const int split = std::max(1, std::min((G.pc.nt + 63) / 64, 8192 / std::max(nwin, 1)));
 #define TRC(ND, NV)                                                                               \
     if (nd == ND && nv == NV) {                                                                   \
-        hipLaunchKernelGGL((k_fwd_traces<ND, NV>), dim3(grid_x8((npts + 16 / ND - 1) / (16 / ND))), \
-                           dim3(64), 0, G.stream, db, G.pc, npts, nlive, order, keys, loc, G.opWf.p, \
-                           G.Ftr.p);                                                              \
+        hipLaunchKernelGGL((k_fwd_traces<ND, NV>), dim3(grid_x8(nwin), split), dim3(64), 0, G.stream, \
+                           db, G.pc, npts, nlive, order, keys, loc, G.opWf.p, G.Ftr.p);           \
         traced = true;                                                                            \
     }
 #define TRC_NV(ND) TRC(ND, 2) TRC(ND, 3) TRC(ND, 4) TRC(ND, 5) TRC(ND, 6) TRC(ND, 7) TRC(ND, 8) TRC(ND, 10)

@@ -1511,7 +1511,8 @@ __global__ __launch_bounds__(kStrainThreads) void k_strain_traces(
 // k_fwd_traces: the forward strain gather (sem_derivatives.f90, spectral_basis.f90:181-314,
 // readfields.f90:1138-1465: u_c(t) = sum_j W_cj u_j(t) over the nvar x 25 nodal traces j
 // of the point's element, the weights from k_op_setup) on the matrix cores.  One wave per
-// window of PW = 16 / ND sorted items (windows in XCD-contiguous ranges): rows r = ND i + c
+// window of PW = 16 / ND sorted items (windows in XCD-contiguous ranges; blockIdx.y: a share
+// of the samples when a launch has few windows): rows r = ND i + c
 // (item i0 + i, component c) of a 16-row block, K = the traces in steps of 4,
 // v_mfma_f32_16x16x4f32 with A = W of row l % 16 at trace 4 s + l / 16 and B = that trace
 // at 16 samples; lane l loads 4 samples
@@ -1540,13 +1541,13 @@ __device__ __forceinline__ float4 gload4(const float *base, int off)
 template <int ND, int KS, bool WHOLE>
 __device__ __forceinline__ void fwd_traces_pass(const float *ue, const int *toff, const float *wa,
                                                 const float *sWa, float mf, int e, const int (&eout)[4],
-                                                int i0, int l, int nt, float *T)
+                                                int i0, int l, int nt, int tb, int te, float *T)
 {
     constexpr int PW = 16 / ND;
     constexpr bool WREG = KS <= 32;
     constexpr int CH = WREG ? (KS + 1) / 2 : 8;                // K steps loaded together
-    const int r = l & 15, kq = l >> 4, nst = (nt + 63) >> 6;
-    for (int t = 0; t < nst; ++t) {
+    const int r = l & 15, kq = l >> 4;
+    for (int t = tb; t < te; ++t) {
         const int s0 = (t << 6) + 4 * r;                       // this lane's 4 samples
         v4f d[4];
 #pragma unroll
@@ -1640,6 +1641,10 @@ __global__ __launch_bounds__(64, 2) void k_fwd_traces(DevDb db, PathConf pc, int
     for (int v = 0; v < 4; ++v) eout[v] = __shfl(erow, 4 * kq + v, 64);
     __syncthreads();
     const bool whole = (nt & 63) == 0;                         // super-tiles of 64 samples
+    // this block's share of the super-tiles (gridDim.y blocks per window: small launches
+    // spread a window's samples over several waves)
+    const int nst = (nt + 63) >> 6, tb = (int)blockIdx.y * nst / (int)gridDim.y,
+              te = ((int)blockIdx.y + 1) * nst / (int)gridDim.y;
     int prev = -1;
     for (;;) {
         // the next element of the window (the wave's smallest above prev)
@@ -1652,8 +1657,8 @@ __global__ __launch_bounds__(64, 2) void k_fwd_traces(DevDb db, PathConf pc, int
         const float *ue = elem_u(db, e);
         if (!ue) continue;                                     // not in this page
         const float mf = erow == e ? 1.0f : 0.0f;
-        if (whole) fwd_traces_pass<ND, KS, true>(ue, toff, wa, sWa, mf, e, eout, i0, l, nt, T);
-        else fwd_traces_pass<ND, KS, false>(ue, toff, wa, sWa, mf, e, eout, i0, l, nt, T);
+        if (whole) fwd_traces_pass<ND, KS, true>(ue, toff, wa, sWa, mf, e, eout, i0, l, nt, tb, te, T);
+        else fwd_traces_pass<ND, KS, false>(ue, toff, wa, sWa, mf, e, eout, i0, l, nt, tb, te, T);
     }
 }
 
