@@ -265,19 +265,37 @@ __device__ __forceinline__ void mfma_window_sums(const float2 (&bk0)[4], const f
 // as two products); lane l then holds pair l % 16 at bins kb = 16 t + 4 (l / 16) .. + 3 and
 // forms the base kernels with the forward spectra of the pair's point; then the slot sums
 // Re(bk H) with the receiver's window spectra on the FP64 matrix cores.
-template <int NDF, int NDB, int KS>
-__device__ __forceinline__ void mfma_tile(const MfmaGroup<NDF, NDB> &G, const float2 (&a)[KS], const float *sW,
-                                          double *sAcc, int l, int t, const PathConf &pc, const float2 *Fp,
-                                          const RecDesc &rd, const float2 *Hp)
+// this lane's forward spectra of tile t (pair l % 16's point, bins kb .. kb + 3)
+template <int NDF, int NDB>
+__device__ __forceinline__ void mfma_load_f(const MfmaGroup<NDF, NDB> &G, const float2 *Fp, int l, int t, int fs,
+                                            float4 (&f4)[NDF][2])
 {
-    const int bq = l >> 4, kb = (t << 4) + 4 * bq, fs = pc.fstride;
-    float4 f4[NDF][2];
+    const int kb = (t << 4) + 4 * (l >> 4);
 #pragma unroll
     for (int m = 0; m < NDF; ++m) {
         const float4 *src = reinterpret_cast<const float4 *>(Fp + (size_t)m * fs + kb);
         f4[m][0] = G.valid ? src[0] : make_float4(0.f, 0.f, 0.f, 0.f);
         f4[m][1] = G.valid ? src[1] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
+}
+
+// this lane's element spectra of tile t (the A operands: trace 4 s + l / 16, bin 16 t + l % 16)
+template <int KS, int J>
+__device__ __forceinline__ void mfma_load_u(const float2 *Ue, int us, int l, int t, float2 (&a)[KS])
+{
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        const int j = 4 * s + (l >> 4);
+        a[s] = (j < J) ? Ue[(size_t)j * us + (t << 4) + (l & 15)] : make_float2(0.0f, 0.0f);
+    }
+}
+
+template <int NDF, int NDB, int KS>
+__device__ __forceinline__ void mfma_tile(const MfmaGroup<NDF, NDB> &G, const float2 (&a)[KS],
+                                          const float4 (&f4)[NDF][2], const float *sW, double *sAcc, int l, int t,
+                                          const PathConf &pc, const RecDesc &rd, const float2 *Hp)
+{
+    const int bq = l >> 4, kb = (t << 4) + 4 * bq, fs = pc.fstride;
     v4f accR[NDB], accI[NDB];
 #pragma unroll
     for (int c = 0; c < NDB; ++c) {
@@ -408,7 +426,7 @@ __global__ __launch_bounds__(64, (NDB <= 4 && NDF <= 4) ? 2 : 1) void k_bwd_mfma
     if (grp < 0) return;
     const int eu = group_erun[grp];
     if (eu < u0 || eu >= u0 + ubatch) return;
-    const int l = threadIdx.x, jp = l & 15, bq = l >> 4;
+    const int l = threadIdx.x;
     float *sW = reinterpret_cast<float *>(smem_raw);
     double *sAcc = reinterpret_cast<double *>(sW + KS * NDB * 64);
     MfmaGroup<NDF, NDB> G;
@@ -418,14 +436,33 @@ __global__ __launch_bounds__(64, (NDB <= 4 && NDF <= 4) ? 2 : 1) void k_bwd_mfma
     const RecDesc &rd = recs[G.r];
     const float2 *Fp = F + (size_t)G.p * NDF * pc.fstride;
     const int ntile = (pc.nomega + 15) >> 4;
-    for (int t = 0; t < ntile; ++t) {
-        float2 a[KS];
+    if constexpr (NDF <= 4 && NDB <= 4) {
+        // the next tile's element and forward spectra are loaded while this one is
+        // multiplied (the registers for it fit two waves per SIMD up to 4 components)
+        float2 an[KS];
+        float4 fn[NDF][2];
+        mfma_load_u<KS, J>(Ue, pc.ustride, l, 0, an);
+        mfma_load_f<NDF, NDB>(G, Fp, l, 0, pc.fstride, fn);
+        for (int t = 0; t < ntile; ++t) {
+            float2 a[KS];
+            float4 f4[NDF][2];
 #pragma unroll
-        for (int s = 0; s < KS; ++s) {
-            const int j = 4 * s + bq;
-            a[s] = (j < J) ? Ue[(size_t)j * pc.ustride + (t << 4) + jp] : make_float2(0.0f, 0.0f);
+            for (int s = 0; s < KS; ++s) a[s] = an[s];
+#pragma unroll
+            for (int m = 0; m < NDF; ++m) { f4[m][0] = fn[m][0]; f4[m][1] = fn[m][1]; }
+            const int tn = t + 1 < ntile ? t + 1 : t;
+            mfma_load_u<KS, J>(Ue, pc.ustride, l, tn, an);
+            mfma_load_f<NDF, NDB>(G, Fp, l, tn, pc.fstride, fn);
+            mfma_tile<NDF, NDB, KS>(G, a, f4, sW, sAcc, l, t, pc, rd, Hp);
         }
-        mfma_tile<NDF, NDB, KS>(G, a, sW, sAcc, l, t, pc, Fp, rd, Hp);
+    } else {
+        for (int t = 0; t < ntile; ++t) {
+            float2 a[KS];
+            float4 f4[NDF][2];
+            mfma_load_u<KS, J>(Ue, pc.ustride, l, t, a);
+            mfma_load_f<NDF, NDB>(G, Fp, l, t, pc.fstride, f4);
+            mfma_tile<NDF, NDB, KS>(G, a, f4, sW, sAcc, l, t, pc, rd, Hp);
+        }
     }
     __syncthreads();                                           // the lanes' blocks of sums
     mfma_finish<NDF, NDB>(G, l, sAcc, pc, rd, kern, model_idx, model, ndepth, relative, nk, phys, base_out, xyz,
@@ -494,7 +531,9 @@ __global__ __launch_bounds__(64 * W) void k_bwd_mfma_wg(
                 float2 a[KS];
 #pragma unroll
                 for (int s = 0; s < KS; ++s) a[s] = tile[(4 * s + bq) * 16 + jp];
-                mfma_tile<NDF, NDB, KS>(G, a, sW, sAcc, l, t, pc, Fp, rd, Hp);
+                float4 f4[NDF][2];
+                mfma_load_f<NDF, NDB>(G, Fp, l, t, pc.fstride, f4);
+                mfma_tile<NDF, NDB, KS>(G, a, f4, sW, sAcc, l, t, pc, rd, Hp);
             }
             if (more) {
                 float2 *nt = sU + ((t + 1) & 1) * TILE;
