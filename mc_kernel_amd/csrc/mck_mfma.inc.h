@@ -455,6 +455,19 @@ __global__ __launch_bounds__(64, (NDB <= 4 && NDF <= 4) ? 2 : 1) void k_bwd_mfma
             mfma_load_f<NDF, NDB>(G, Fp, l, tn, pc.fstride, fn);
             mfma_tile<NDF, NDB, KS>(G, a, f4, sW, sAcc, l, t, pc, rd, Hp);
         }
+    } else if constexpr (NVB == 2) {
+        // (more components: the element spectra only)
+        float2 an[KS];
+        mfma_load_u<KS, J>(Ue, pc.ustride, l, 0, an);
+        for (int t = 0; t < ntile; ++t) {
+            float2 a[KS];
+            float4 f4[NDF][2];
+#pragma unroll
+            for (int s = 0; s < KS; ++s) a[s] = an[s];
+            mfma_load_u<KS, J>(Ue, pc.ustride, l, t + 1 < ntile ? t + 1 : t, an);
+            mfma_load_f<NDF, NDB>(G, Fp, l, t, pc.fstride, f4);
+            mfma_tile<NDF, NDB, KS>(G, a, f4, sW, sAcc, l, t, pc, rd, Hp);
+        }
     } else {
         for (int t = 0; t < ntile; ++t) {
             float2 a[KS];
