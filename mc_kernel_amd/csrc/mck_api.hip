@@ -410,7 +410,7 @@ struct State {
     // per-pair path (set at init from c1's measured costs, measured by mck_tune)
     double cost_group_us = 0.0, cost_run_us = 0.0, cost_pair_us = 0.0;
     int bwd_path_env = -1;                        // MCK_BWD_PATH: 0 per-pair, 1 matrix cores
-    int mfma_xcd = 0;                             // MCK_MFMA_XCD: groups in XCD-contiguous ranges
+    int mfma_xcd = 1;                             // MCK_MFMA_XCD: groups in XCD-contiguous ranges
     int mfma_waves = 1;                           // MCK_MFMA_WAVES: waves per element run (1, 2, 4)
     size_t uspec_budget = 0;                      // bytes of element spectra per batch
     DBuf<unsigned char> sort_tmp;
@@ -1687,7 +1687,9 @@ int init_impl(const mck_setup_t *s) {
     G.uspec_budget = std::min<size_t>(free_b / 8, (size_t)16 << 30);
     G.bwd_path_env = -1;
     if (const char *e = std::getenv("MCK_BWD_PATH")) G.bwd_path_env = std::atoi(e);
-    G.mfma_xcd = 0;
+    // groups in XCD-contiguous ranges (the groups of an element share one L2) unless
+    // MCK_MFMA_XCD=0
+    G.mfma_xcd = 1;
     if (const char *e = std::getenv("MCK_MFMA_XCD")) G.mfma_xcd = std::atoi(e) != 0;
     G.mfma_waves = 1;
     if (const char *e = std::getenv("MCK_MFMA_WAVES")) {
