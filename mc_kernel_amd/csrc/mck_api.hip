@@ -1573,6 +1573,9 @@ int init_impl(const mck_setup_t *s) {
     pc = PathConf{};
     pc.nt = S.nt;
     pc.M = nextpow2(S.nt);
+    // the LDS FFTs take M = 256 samples-per-thread as a compile-time size from 2 samples
+    // per thread on
+    if (spt_for(pc.nt) >= 2 && pc.M != 256 * spt_for(pc.nt)) return fail(MCK_EUNSUPPORTED, "FFT size");
     pc.nomega = pc.M + 1;
     pc.ustride = (pc.nomega + 15) / 16 * 16;
     pc.fstride = pc.ustride;

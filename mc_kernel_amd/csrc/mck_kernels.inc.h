@@ -1259,12 +1259,18 @@ __device__ inline void load_twiddles(const float2 *twq, int M, float2 *Tq)
 // the length-M Stockham autosort FFT in LDS (radix 4, after one radix-2 pass when log2 M
 // is odd).  z[n] vanishes for n >= M/2 (nt <= M), so the radix-2 pass reduces to
 // out[2n] = out[2n+1] = z[n] and is fused with the packing.  Returns the buffer holding
-// Z; ends with a barrier.
+// Z; ends with a barrier.  From 2 samples per thread on, M = 256 SPT (the host sizes both
+// from nt), so the passes' sizes and strides are compile-time constants.
 template <int SPT>
-__device__ inline const float2 *lds_fft(const float (&x)[SPT], int t0, int nt, int M, float2 *bufA,
+constexpr int fft_size_of_spt() { return SPT >= 2 ? 256 * SPT : 0; }
+
+template <int SPT>
+__device__ inline const float2 *lds_fft(const float (&x)[SPT], int t0, int nt, int M_, float2 *bufA,
                                         float2 *bufB, const float2 *Tq)
 {
 #pragma clang fp contract(fast)
+    constexpr int MC = fft_size_of_spt<SPT>();
+    const int M = MC ? MC : M_;
     const int tid = threadIdx.x;
     const int lM = 31 - __clz(M);
     float2 *in = bufA, *ob = bufB;
@@ -1304,9 +1310,12 @@ __device__ inline const float2 *lds_fft(const float (&x)[SPT], int t0, int nt, i
     }
     __syncthreads();
     const int Mq = M >> 2;
+    // (a compile-time size unrolls the passes and each thread's butterflies)
+    constexpr int UP = MC ? 8 : 1, UJ = MC ? 4 : 1;
+#pragma unroll UP
     for (int lNs = Ns == 1 ? 0 : 1; Ns < M; Ns <<= 2, lNs += 2) {
         const int lstride = lM - lNs - 2;            // M / (4 Ns)
-#pragma unroll 1
+#pragma unroll UJ
         for (int j = tid; j < Mq; j += kStrainThreads) {
             const int kk = j & (Ns - 1);
             const float2 v0 = in[j], v1 = in[j + Mq], v2 = in[j + 2 * Mq], v3 = in[j + 3 * Mq];
@@ -1398,9 +1407,11 @@ __device__ inline double2 cmul_dd(double2 a, double2 b)
 // pass first when log2 M is odd); twiddles from the table in global memory.  Returns the
 // buffer holding Z; ends with a barrier.
 template <int SPT>
-__device__ inline const double2 *lds_fft_d(const double (&x)[SPT], int t0, int nt, int M, double2 *bufA,
+__device__ inline const double2 *lds_fft_d(const double (&x)[SPT], int t0, int nt, int M_, double2 *bufA,
                                            double2 *bufB, const double2 *__restrict__ twd)
 {
+    constexpr int MC = fft_size_of_spt<SPT>();
+    const int M = MC ? MC : M_;
     const int tid = threadIdx.x;
     const int lM = 31 - __clz(M);
     double2 *in = bufA, *ob = bufB;
@@ -1439,9 +1450,12 @@ __device__ inline const double2 *lds_fft_d(const double (&x)[SPT], int t0, int n
     }
     __syncthreads();
     const int Mq = M >> 2;
+    // (a compile-time size unrolls the passes and each thread's butterflies)
+    constexpr int UP = MC ? 8 : 1, UJ = MC ? 4 : 1;
+#pragma unroll UP
     for (int lNs = Ns == 1 ? 0 : 1; Ns < M; Ns <<= 2, lNs += 2) {
         const int lstride = lM - lNs - 2;            // M / (4 Ns)
-#pragma unroll 1
+#pragma unroll UJ
         for (int j = tid; j < Mq; j += kStrainThreads) {
             const int kk = j & (Ns - 1);
             const double2 v0 = in[j], v1 = in[j + Mq], v2 = in[j + 2 * Mq], v3 = in[j + 3 * Mq];
