@@ -1707,7 +1707,8 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && ND <= 4) ? 3 : (SPT <=
     load_twiddles(twq, pc.M, Tq);
     __syncthreads();
 
-    const int tid = threadIdx.x, nt = pc.nt, M = pc.M, t0 = tid * SPT;
+    const int tid = threadIdx.x, nt = pc.nt, t0 = tid * SPT;
+    const int M = fft_size_of_spt<SPT>() ? fft_size_of_spt<SPT>() : pc.M;      // (= pc.M)
     // the point's strain traces (k_fwd_traces)
     float acc[ND][SPT];
     {

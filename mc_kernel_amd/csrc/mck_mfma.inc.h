@@ -100,7 +100,8 @@ __global__ __launch_bounds__(kStrainThreads) void k_elem_spectra(DevDb db, PathC
     const int e = run_elem[u];
     const float *ue = elem_u(db, e);
     if (!ue) return;
-    const int tid = threadIdx.x, nt = pc.nt, M = pc.M, t0 = tid * SPT;
+    const int tid = threadIdx.x, nt = pc.nt, t0 = tid * SPT;
+    const int M = fft_size_of_spt<SPT>() ? fft_size_of_spt<SPT>() : pc.M;      // (= pc.M)
     const size_t blk = (size_t)25 * nt;
     float tap[SPT];
     taper_factors(t0, nt, pc, tap);
