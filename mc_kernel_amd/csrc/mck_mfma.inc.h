@@ -226,37 +226,65 @@ __device__ __forceinline__ void mfma_group_setup(MfmaGroup<NDF, NDB> &G, int grp
 // operands: pair l % 16, k = l / 16) and loads column l % 16's spectra at the same bins
 // (the B operands).  D of block b: sAcc[b 256 + 4 l + r] = D[l / 16 + 4 r][l % 16] (the
 // FP64 product's result layout).
-__device__ __forceinline__ void mfma_window_sums(const float2 (&bk0)[4], const float2 (&bk1)[4], int l, int kb,
-                                                 int fs, const RecDesc &rd, const float2 *Hp, double *sAcc)
+// blocks of slot sums kept in registers across the tiles (the rest in LDS): 2 where the
+// registers still fit two waves per SIMD (4 or 6 forward components, 2 backward variables)
+template <int NDF, int NVB>
+constexpr int mfma_dreg() { return (NDF >= 4 && NVB == 2) ? 2 : 0; }
+
+// block b of the tile's slot sums: d + the block's products
+__device__ __forceinline__ v4d mfma_window_block(const float2 (&bk0)[4], const float2 (&bk1)[4], int b, int l,
+                                                 int kb, int fs, const RecDesc &rd, const float2 *Hp, v4d d)
+{
+    const int p0 = b << 4;
+    const int cls = p0 < rd.cls_end[0] ? 0 : (p0 < rd.cls_end[1] ? 1 : 2);
+    const float4 *src = reinterpret_cast<const float4 *>(Hp + (size_t)rd.cls_h[p0 + (l & 15)] * fs + kb);
+    const float4 q0 = src[0], q1 = src[1];
+    const double2 h[4] = {make_double2(q0.x, q0.y), make_double2(q0.z, q0.w), make_double2(q1.x, q1.y),
+                          make_double2(q1.z, q1.w)};
+    double ax[4], ay[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        if (cls == 0) {
+            ax[v] = bk0[v].x; ay[v] = bk0[v].y;
+        } else if (cls == 1) {
+            ax[v] = bk1[v].x; ay[v] = bk1[v].y;
+        } else {                                                // mu - 2 lambda
+            ax[v] = (double)bk1[v].x - 2.0 * (double)bk0[v].x;
+            ay[v] = (double)bk1[v].y - 2.0 * (double)bk0[v].y;
+        }
+    }
+    v4d dy = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        d = __builtin_amdgcn_mfma_f64_16x16x4f64(ax[v], h[v].x, d, 0, 0, 0);
+        dy = __builtin_amdgcn_mfma_f64_16x16x4f64(ay[v], h[v].y, dy, 0, 0, 0);
+    }
+    return d + dy;
+}
+
+// the register blocks into sAcc (before the lanes read each other's sums)
+template <int DR>
+__device__ __forceinline__ void mfma_store_dreg(const RecDesc &rd, int l, double *sAcc, const v4d (&dreg)[DR ? DR : 1])
 {
     const int nblk = rd.cls_end[2] >> 4;
-    for (int b = 0; b < nblk; ++b) {
-        const int p0 = b << 4;
-        const int cls = p0 < rd.cls_end[0] ? 0 : (p0 < rd.cls_end[1] ? 1 : 2);
-        const float4 *src = reinterpret_cast<const float4 *>(Hp + (size_t)rd.cls_h[p0 + (l & 15)] * fs + kb);
-        const float4 q0 = src[0], q1 = src[1];
-        const double2 h[4] = {make_double2(q0.x, q0.y), make_double2(q0.z, q0.w), make_double2(q1.x, q1.y),
-                              make_double2(q1.z, q1.w)};
-        double ax[4], ay[4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            if (cls == 0) {
-                ax[v] = bk0[v].x; ay[v] = bk0[v].y;
-            } else if (cls == 1) {
-                ax[v] = bk1[v].x; ay[v] = bk1[v].y;
-            } else {                                            // mu - 2 lambda
-                ax[v] = (double)bk1[v].x - 2.0 * (double)bk0[v].x;
-                ay[v] = (double)bk1[v].y - 2.0 * (double)bk0[v].y;
-            }
-        }
+    for (int b = 0; b < DR; ++b)
+        if (b < nblk) *reinterpret_cast<v4d *>(sAcc + (size_t)b * 256 + 4 * l) = dreg[b];
+}
+
+// dreg: blocks 0 .. DR - 1 (registers across the tiles); the others in sAcc
+template <int DR>
+__device__ __forceinline__ void mfma_window_sums(const float2 (&bk0)[4], const float2 (&bk1)[4], int l, int kb,
+                                                 int fs, const RecDesc &rd, const float2 *Hp, double *sAcc,
+                                                 v4d (&dreg)[DR ? DR : 1])
+{
+    const int nblk = rd.cls_end[2] >> 4;
+#pragma unroll
+    for (int b = 0; b < DR; ++b)
+        if (b < nblk) dreg[b] = mfma_window_block(bk0, bk1, b, l, kb, fs, rd, Hp, dreg[b]);
+    for (int b = DR; b < nblk; ++b) {
         v4d *acc = reinterpret_cast<v4d *>(sAcc + (size_t)b * 256 + 4 * l);
-        v4d dx = *acc, dy = v4d{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            dx = __builtin_amdgcn_mfma_f64_16x16x4f64(ax[v], h[v].x, dx, 0, 0, 0);
-            dy = __builtin_amdgcn_mfma_f64_16x16x4f64(ay[v], h[v].y, dy, 0, 0, 0);
-        }
-        *acc = dx + dy;
+        *acc = mfma_window_block(bk0, bk1, b, l, kb, fs, rd, Hp, *acc);
     }
 }
 
@@ -291,10 +319,11 @@ __device__ __forceinline__ void mfma_load_u(const float2 *Ue, int us, int l, int
     }
 }
 
-template <int NDF, int NDB, int KS>
+template <int NDF, int NDB, int KS, int DR>
 __device__ __forceinline__ void mfma_tile(const MfmaGroup<NDF, NDB> &G, const float2 (&a)[KS],
                                           const float4 (&f4)[NDF][2], const float *sW, double *sAcc, int l, int t,
-                                          const PathConf &pc, const RecDesc &rd, const float2 *Hp)
+                                          const PathConf &pc, const RecDesc &rd, const float2 *Hp,
+                                          v4d (&dreg)[DR ? DR : 1])
 {
     const int bq = l >> 4, kb = (t << 4) + 4 * bq, fs = pc.fstride;
     v4f accR[NDB], accI[NDB];
@@ -350,7 +379,7 @@ __device__ __forceinline__ void mfma_tile(const MfmaGroup<NDF, NDB> &G, const fl
             }
         }
     }
-    mfma_window_sums(bk0, bk1, l, kb, fs, rd, Hp, sAcc);
+    mfma_window_sums<DR>(bk0, bk1, l, kb, fs, rd, Hp, sAcc, dreg);
 }
 
 // Physical kernels (kernel.f90:665-832) of the group's pairs with the model at the point,
@@ -437,6 +466,10 @@ __global__ __launch_bounds__(64, (NDB <= 4 && NDF <= 4) ? 2 : 1) void k_bwd_mfma
     const RecDesc &rd = recs[G.r];
     const float2 *Fp = F + (size_t)G.p * NDF * pc.fstride;
     const int ntile = (pc.nomega + 15) >> 4;
+    constexpr int DR = mfma_dreg<NDF, NVB>();
+    v4d dreg[DR ? DR : 1];
+#pragma unroll
+    for (int b = 0; b < (DR ? DR : 1); ++b) dreg[b] = v4d{0.0, 0.0, 0.0, 0.0};
     if constexpr (NDF <= 4 && NDB <= 4) {
         // the next tile's element and forward spectra are loaded while this one is
         // multiplied (the registers for it fit two waves per SIMD up to 4 components)
@@ -454,7 +487,7 @@ __global__ __launch_bounds__(64, (NDB <= 4 && NDF <= 4) ? 2 : 1) void k_bwd_mfma
             const int tn = t + 1 < ntile ? t + 1 : t;
             mfma_load_u<KS, J>(Ue, pc.ustride, l, tn, an);
             mfma_load_f<NDF, NDB>(G, Fp, l, tn, pc.fstride, fn);
-            mfma_tile<NDF, NDB, KS>(G, a, f4, sW, sAcc, l, t, pc, rd, Hp);
+            mfma_tile<NDF, NDB, KS, DR>(G, a, f4, sW, sAcc, l, t, pc, rd, Hp, dreg);
         }
     } else if constexpr (NVB == 2) {
         // (more components: the element spectra only)
@@ -467,7 +500,7 @@ __global__ __launch_bounds__(64, (NDB <= 4 && NDF <= 4) ? 2 : 1) void k_bwd_mfma
             for (int s = 0; s < KS; ++s) a[s] = an[s];
             mfma_load_u<KS, J>(Ue, pc.ustride, l, t + 1 < ntile ? t + 1 : t, an);
             mfma_load_f<NDF, NDB>(G, Fp, l, t, pc.fstride, f4);
-            mfma_tile<NDF, NDB, KS>(G, a, f4, sW, sAcc, l, t, pc, rd, Hp);
+            mfma_tile<NDF, NDB, KS, DR>(G, a, f4, sW, sAcc, l, t, pc, rd, Hp, dreg);
         }
     } else {
         for (int t = 0; t < ntile; ++t) {
@@ -475,9 +508,10 @@ __global__ __launch_bounds__(64, (NDB <= 4 && NDF <= 4) ? 2 : 1) void k_bwd_mfma
             float4 f4[NDF][2];
             mfma_load_u<KS, J>(Ue, pc.ustride, l, t, a);
             mfma_load_f<NDF, NDB>(G, Fp, l, t, pc.fstride, f4);
-            mfma_tile<NDF, NDB, KS>(G, a, f4, sW, sAcc, l, t, pc, rd, Hp);
+            mfma_tile<NDF, NDB, KS, DR>(G, a, f4, sW, sAcc, l, t, pc, rd, Hp, dreg);
         }
     }
+    mfma_store_dreg<DR>(rd, l, sAcc, dreg);
     __syncthreads();                                           // the lanes' blocks of sums
     mfma_finish<NDF, NDB>(G, l, sAcc, pc, rd, kern, model_idx, model, ndepth, relative, nk, phys, base_out, xyz,
                           active, P, R, rec_xyz);
@@ -516,6 +550,10 @@ __global__ __launch_bounds__(64 * W) void k_bwd_mfma_wg(
         const int grp = round + wv;
         const bool mine = grp < gend;                                  // uniform in the wave
         MfmaGroup<NDF, NDB> G;
+        constexpr int DR = mfma_dreg<NDF, NVB>();
+        v4d dreg[DR ? DR : 1];
+#pragma unroll
+        for (int b = 0; b < (DR ? DR : 1); ++b) dreg[b] = v4d{0.0, 0.0, 0.0, 0.0};
         if (mine)
             mfma_group_setup<NDF, NDB, NVB>(G, grp, group_first, order, R, Wb, Qb, Qf, pc, npos, sW, sAcc, l);
         // tile 0 (the previous round's readers of both buffers are past the last barrier)
@@ -547,7 +585,7 @@ __global__ __launch_bounds__(64 * W) void k_bwd_mfma_wg(
                 for (int s = 0; s < KS; ++s) a[s] = tile[(4 * s + bq) * 16 + jp];
                 float4 f4[NDF][2];
                 mfma_load_f<NDF, NDB>(G, Fp, l, t, pc.fstride, f4);
-                mfma_tile<NDF, NDB, KS>(G, a, f4, sW, sAcc, l, t, pc, rd, Hp);
+                mfma_tile<NDF, NDB, KS, DR>(G, a, f4, sW, sAcc, l, t, pc, rd, Hp, dreg);
             }
             if (more) {
                 float2 *nt = sU + ((t + 1) & 1) * TILE;
@@ -559,6 +597,8 @@ __global__ __launch_bounds__(64 * W) void k_bwd_mfma_wg(
             }
             __syncthreads();
         }
+        if (mine) mfma_store_dreg<DR>(rd, l, sAcc, dreg);
+        __syncthreads();                                               // the lanes' blocks of sums
         if (mine)
             mfma_finish<NDF, NDB>(G, l, sAcc, pc, rd, kern, model_idx, model, ndepth, relative, nk, phys, base_out,
                                   xyz, active, P, R, rec_xyz);
