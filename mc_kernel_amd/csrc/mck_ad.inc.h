@@ -282,7 +282,7 @@ __global__ __launch_bounds__(kStrainThreads) void k_ad_fspec(PathConf pc, int np
         cumsum_trapezoidal(x, t0, nt, h, edge_x, wave_tot);
 #pragma unroll
         for (int q = 0; q < SPT; ++q) x[q] *= tap[q];
-        const float2 *Z = lds_fft(x, t0, nt, M, bufA, bufB, Tq);
+        const float2 *Z = lds_fft<SPT, false>(x, t0, nt, M, bufA, bufB, Tq);
         float2 *dst = Fs + ((size_t)p * nd + c) * nw;
         for (int k = tid; k <= M; k += kStrainThreads) dst[k] = rfft_bin(Z, k, M, Tq, fdt);
         __syncthreads();
@@ -301,7 +301,7 @@ __device__ inline void ad_bspec(const PathConf &pc, const float *src, const floa
         float x[SPT];
 #pragma unroll
         for (int q = 0; q < SPT; ++q) x[q] = t0 + q < nt ? src[(size_t)c * nt + t0 + q] * tap[q] : 0.0f;
-        const float2 *Z = lds_fft(x, t0, nt, M, bufA, bufB, Tq);
+        const float2 *Z = lds_fft<SPT, false>(x, t0, nt, M, bufA, bufB, Tq);
 #pragma unroll
         for (int j = 0; j < SPT; ++j) {
             const int k = tid + kStrainThreads * j;
@@ -427,7 +427,7 @@ __global__ __launch_bounds__(kStrainThreads) void k_ad_bstage(PathConf pc, int n
         float x[SPT];
 #pragma unroll
         for (int i = 0; i < SPT; ++i) x[i] = t0 + i < nt ? src[t0 + i] * tap[i] : 0.0f;
-        const float2 *Z = lds_fft(x, t0, nt, M, bufA, bufB, Tq);
+        const float2 *Z = lds_fft<SPT, false>(x, t0, nt, M, bufA, bufB, Tq);
         float2 *dst = Bs + ((size_t)q * nd + c) * nw;
         for (int k = tid; k <= M; k += kStrainThreads) dst[k] = rfft_bin(Z, k, M, Tq, fdt);
         __syncthreads();

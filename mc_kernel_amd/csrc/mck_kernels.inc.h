@@ -1260,16 +1260,18 @@ __device__ inline void load_twiddles(const float2 *twq, int M, float2 *Tq)
 // is odd).  z[n] vanishes for n >= M/2 (nt <= M), so the radix-2 pass reduces to
 // out[2n] = out[2n+1] = z[n] and is fused with the packing.  Returns the buffer holding
 // Z; ends with a barrier.  From 2 samples per thread on, M = 256 SPT (the host sizes both
-// from nt), so the passes' sizes and strides are compile-time constants.
+// from nt), so the passes' sizes and strides can be compile-time constants: UNROLL takes
+// them so and unrolls the passes and each thread's butterflies (more registers: off for
+// kernels that hold other spectra in registers meanwhile).
 template <int SPT>
 constexpr int fft_size_of_spt() { return SPT >= 2 ? 256 * SPT : 0; }
 
-template <int SPT>
+template <int SPT, bool UNROLL = true>
 __device__ inline const float2 *lds_fft(const float (&x)[SPT], int t0, int nt, int M_, float2 *bufA,
                                         float2 *bufB, const float2 *Tq)
 {
 #pragma clang fp contract(fast)
-    constexpr int MC = fft_size_of_spt<SPT>();
+    constexpr int MC = UNROLL ? fft_size_of_spt<SPT>() : 0;
     const int M = MC ? MC : M_;
     const int tid = threadIdx.x;
     const int lM = 31 - __clz(M);
@@ -1823,7 +1825,7 @@ __global__ __launch_bounds__(kStrainThreads, (SPT <= 8 && NDB <= 4) ? 3 : (SPT <
             select_comp(acc, c, t0, nt, x);
 #pragma unroll
             for (int q = 0; q < SPT; ++q) x[q] *= tap[q];
-            const float2 *Z = lds_fft(x, t0, nt, M, bufA, bufB, Tq);
+            const float2 *Z = lds_fft<SPT, false>(x, t0, nt, M, bufA, bufB, Tq);
 #pragma unroll
             for (int j = 0; j < SPT; ++j) {
                 const int k = tid + kStrainThreads * j;
