@@ -1620,7 +1620,6 @@ int init_impl(const mck_setup_t *s) {
     };
     carry(fwd_mono, &pc.nd_f, pc.comps_f);
     carry(bwd_mono, &pc.nd_b, pc.comps_b);
-    pc.op_bytes_f = (int)op_bytes(pc.nd_f * pc.nvar_fwd * 25);
     pc.op_bytes_b = (int)op_bytes(pc.nd_b * pc.nvar_bwd * 25);
     if (pc.nd_f == 1 && pc.nd_b != 1) return fail(MCK_EINVAL, "inconsistent strain types");
     pc.ntaper = 10;

@@ -102,7 +102,7 @@ struct PathConf {
     int nbk_lds;
     int max_slots;            // largest slot count of a receiver
     int max_cls;              // largest padded class-position count of a receiver
-    int op_bytes_f, op_bytes_b;   // LDS of the per-point operator set-up, fwd / bwd
+    int op_bytes_b;           // LDS of a pair's operator set-up in k_bwd_eval
     // dampen_field (readfields.f90:3184-3204): radius (<= 0 disables), source position
     double damp_radius;
     double src_r[3];
