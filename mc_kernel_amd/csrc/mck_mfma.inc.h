@@ -188,7 +188,7 @@ __device__ __forceinline__ void mfma_group_setup(MfmaGroup<NDF, NDB> &G, int grp
     G.valid = jp < G.n;
     // the group's pairs share the receiver
     G.r = order[G.g0] % R;
-    G.p = G.valid ? order[G.g0 + jp] / R : 0;
+    G.p = order[G.g0 + (G.valid ? jp : 0)] / R;                 // (no pair: the group's first point)
     if (NDF > 1) {
         float qf[6][NDF], qb[6][NDB];
 #pragma unroll
@@ -294,7 +294,9 @@ __device__ __forceinline__ void mfma_window_sums(const float2 (&bk0)[4], const f
 // as two products); lane l then holds pair l % 16 at bins kb = 16 t + 4 (l / 16) .. + 3 and
 // forms the base kernels with the forward spectra of the pair's point; then the slot sums
 // Re(bk H) with the receiver's window spectra on the FP64 matrix cores.
-// this lane's forward spectra of tile t (pair l % 16's point, bins kb .. kb + 3)
+// this lane's forward spectra of tile t (pair l % 16's point, bins kb .. kb + 3).  A lane
+// without a pair loads the group's first point (no branch): its weights are zero, so its
+// base kernels and rows of the slot sums are zero, and nothing reads them.
 template <int NDF, int NDB>
 __device__ __forceinline__ void mfma_load_f(const MfmaGroup<NDF, NDB> &G, const float2 *Fp, int l, int t, int fs,
                                             float4 (&f4)[NDF][2])
@@ -303,19 +305,20 @@ __device__ __forceinline__ void mfma_load_f(const MfmaGroup<NDF, NDB> &G, const 
 #pragma unroll
     for (int m = 0; m < NDF; ++m) {
         const float4 *src = reinterpret_cast<const float4 *>(Fp + (size_t)m * fs + kb);
-        f4[m][0] = G.valid ? src[0] : make_float4(0.f, 0.f, 0.f, 0.f);
-        f4[m][1] = G.valid ? src[1] : make_float4(0.f, 0.f, 0.f, 0.f);
+        f4[m][0] = src[0];
+        f4[m][1] = src[1];
     }
 }
 
-// this lane's element spectra of tile t (the A operands: trace 4 s + l / 16, bin 16 t + l % 16)
+// this lane's element spectra of tile t (the A operands: trace 4 s + l / 16, bin 16 t + l % 16;
+// traces past J load trace J - 1, whose B operands there are zero)
 template <int KS, int J>
 __device__ __forceinline__ void mfma_load_u(const float2 *Ue, int us, int l, int t, float2 (&a)[KS])
 {
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
-        const int j = 4 * s + (l >> 4);
-        a[s] = (j < J) ? Ue[(size_t)j * us + (t << 4) + (l & 15)] : make_float2(0.0f, 0.0f);
+        const int j = min(4 * s + (l >> 4), J - 1);
+        a[s] = Ue[(size_t)j * us + (t << 4) + (l & 15)];
     }
 }
 
