@@ -191,7 +191,7 @@ typedef struct {
     int32_t mc_steps;          /* step-synchronous MC iterations performed               */
     int32_t strain_launches;   /* launches of the strain-gather/FFT kernel               */
     double  device_ms;         /* device time of the whole call (hipEvent)               */
-    double  strain_fwd_ms;     /* summed time of the fwd strain-gather/FFT kernel        */
+    double  strain_fwd_ms;     /* summed time of the fwd strain-gather and FFT kernels   */
     double  strain_bwd_ms;     /* summed time of the bwd strain-gather/FFT kernel        */
     double  wall_ms;           /* host wall time of the whole call                       */
     int64_t paged_bytes;       /* snapshot bytes copied host -> device by paging         */
