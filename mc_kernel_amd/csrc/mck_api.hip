@@ -1120,7 +1120,7 @@ int launch_bwd_mfma(State &G, const DevDb &db, int npairs, const int *keys, cons
     if (G.bwd_path_env < 0 && G.cost_pair_us * npairs < G.cost_group_us * ngroups + G.cost_run_us * nerun)
         return launch_status(launch_bwd(G, db, npairs, order, loc_b, Wb, Qb, phys, base_out, active, P));
     G.stats.mfma_runs += nerun;
-    const int J = pc.nvar_bwd * 25, KS = (J + 3) / 4;
+    const int J = pc.nvar_bwd * 25, KS = mfma_ks(J);
     const size_t per_elem = (size_t)J * pc.ustride * sizeof(float2);
     const int batch = (int)std::max<size_t>(1, std::min<size_t>((size_t)nerun, G.uspec_budget / per_elem));
     HIPCHECK(G.Uspec.ensure((size_t)batch * J * pc.ustride));

@@ -132,6 +132,31 @@ __global__ __launch_bounds__(kStrainThreads) void k_elem_spectra(DevDb db, PathC
 
 constexpr int kMfmaPairs = 16;         // pairs of one wave (the matrix-core columns)
 
+// The contraction U W of the element spectra with the pair weights on the fp16 matrix
+// cores (v_mfma_f32_16x16x32_f16, 16 times the fp32 products' rate): every fp32 operand x
+// is split into x_hi = fp16(x) and x_lo = fp16(x - x_hi), 22 of its 24 significant bits,
+// and U W = U_hi W_hi + U_hi W_lo + U_lo W_hi (fp16 products are exact in fp32; the
+// dropped U_lo W_lo is below 2^-22 of a product).  The operands are scaled by powers of
+// two into fp16's range first (W per pair, U per tile of 16 bins, to below 2^14) and the
+// sums scaled back.  K: the traces in blocks of 32 (8 per lane).  false: the fp32
+// products (v_mfma_f32_16x16x4f32, K in steps of 4).
+constexpr bool kMfmaSplit = true;
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+
+// A operands per lane for J traces (trace mfma_trace(s, g) in slot s of lane group g)
+__host__ __device__ constexpr int mfma_ks(int J) { return kMfmaSplit ? 8 * ((J + 31) / 32) : (J + 3) / 4; }
+__device__ __forceinline__ int mfma_trace(int s, int g)
+{
+    return kMfmaSplit ? 32 * (s >> 3) + 8 * g + (s & 7) : 4 * s + g;
+}
+
+// x (scaled into fp16's range) = hi + lo to 22 bits
+__device__ __forceinline__ void mfma_split(float x, _Float16 &hi, _Float16 &lo)
+{
+    hi = (_Float16)x;
+    lo = (_Float16)(x - (float)hi);
+}
+
 // LDS bytes of one wave's own arrays: the B operands [KS][NDB][64] of its 16 pairs and the
 // FP64 slot sums of its npos class positions (npos / 16 blocks of 16 x 16)
 __host__ __device__ constexpr size_t mfma_wave_lds(int ks, int ndb, int npos)
@@ -159,30 +184,64 @@ __host__ __device__ constexpr size_t mfma_wg_lds(int ks, int ndb, int npos, int 
 template <int NDF, int NDB>
 struct MfmaGroup {
     int g0, n, r, p;
+    int ew;                             // the pair's weights scaled by 2^ew (split products)
     bool valid;
     float uf[NDF], ub[NDB], Amu[NDF][NDB];
 };
 
-// Loads group grp: its B operands into sW (sW[(s NDB + c) 64 + l'] = W of pair l' % 16,
-// component c, trace 4 s + l' / 16; zero past the group or the traces), its slot sums
-// zeroed, its coefficients.  Called by the 64 lanes of the wave; the caller synchronises
-// before sW or sAcc is read.
+// Loads group grp: its B operands into sW (fp32 products: sW[(s NDB + c) 64 + l'] = W of
+// pair l' % 16, component c, trace 4 s + l' / 16; split products: the v8h at
+// ((kb NDB + c) 2 + part) 64 + l' holds part hi / lo of 2^ew W of pair l' % 16, component c,
+// traces 32 kb + 8 (l' / 16) + q, q = 0..7; zero past the group or the traces), its slot
+// sums zeroed, its coefficients.  Called by the 64 lanes of the wave; the caller
+// synchronises before sW or sAcc is read.
 template <int NDF, int NDB, int NVB>
 __device__ __forceinline__ void mfma_group_setup(MfmaGroup<NDF, NDB> &G, int grp, const int *group_first,
                                                  const int *order, int R, const float *Wb, const float *Qb,
                                                  const float *Qf, const PathConf &pc, int npos, float *sW,
                                                  double *sAcc, int l)
 {
-    constexpr int J = NVB * 25, KS = (J + 3) / 4;
+    constexpr int J = NVB * 25, KS = mfma_ks(J);
     const int jp = l & 15;
     G.g0 = group_first[grp];
     G.n = group_first[grp + 1] - G.g0;
-    for (int idx = l; idx < KS * NDB * 64; idx += 64) {
-        const int s = idx / (NDB * 64), rem = idx - s * (NDB * 64), c = rem >> 6, ll = rem & 63;
-        const int pj = ll & 15, j = 4 * s + (ll >> 4);
-        float w = 0.0f;
-        if (pj < G.n && j < J) w = Wb[(size_t)(G.g0 + pj) * NDB * J + c * J + j];
-        sW[idx] = w;
+    if constexpr (kMfmaSplit) {
+        // the pair's scale: its largest weight to below 2^14 (lanes jp, jp + 16, .. share it)
+        const float *wp = Wb + (size_t)(G.g0 + jp) * NDB * J;
+        float mx = 0.0f;
+        if (jp < G.n)
+            for (int i = l >> 4; i < NDB * J; i += 4) mx = fmaxf(mx, fabsf(wp[i]));
+        mx = fmaxf(mx, __shfl_xor(mx, 16, 64));
+        mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
+        int e;
+        (void)frexpf(mx, &e);
+        G.ew = 14 - e;
+        v8h *sWh = reinterpret_cast<v8h *>(sW);
+        const int g = l >> 4;
+        for (int kc = 0; kc < (KS / 8) * NDB; ++kc) {          // kc = kb NDB + c
+            const int kb = kc / NDB, c = kc - kb * NDB;
+            v8h hi, lo;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int j = 32 * kb + 8 * g + q;
+                const float w = (jp < G.n && j < J) ? ldexpf(wp[c * J + j], G.ew) : 0.0f;
+                _Float16 h, o;
+                mfma_split(w, h, o);
+                hi[q] = h;
+                lo[q] = o;
+            }
+            sWh[(2 * kc) * 64 + l] = hi;
+            sWh[(2 * kc + 1) * 64 + l] = lo;
+        }
+    } else {
+        G.ew = 0;
+        for (int idx = l; idx < KS * NDB * 64; idx += 64) {
+            const int s = idx / (NDB * 64), rem = idx - s * (NDB * 64), c = rem >> 6, ll = rem & 63;
+            const int pj = ll & 15, j = 4 * s + (ll >> 4);
+            float w = 0.0f;
+            if (pj < G.n && j < J) w = Wb[(size_t)(G.g0 + pj) * NDB * J + c * J + j];
+            sW[idx] = w;
+        }
     }
     for (int s = l; s < npos * kMfmaPairs; s += 64) sAcc[s] = 0.0;
     G.valid = jp < G.n;
@@ -310,14 +369,14 @@ __device__ __forceinline__ void mfma_load_f(const MfmaGroup<NDF, NDB> &G, const 
     }
 }
 
-// this lane's element spectra of tile t (the A operands: trace 4 s + l / 16, bin 16 t + l % 16;
-// traces past J load trace J - 1, whose B operands there are zero)
+// this lane's element spectra of tile t (the A operands: trace mfma_trace(s, l / 16), bin
+// 16 t + l % 16; traces past J load trace J - 1, whose B operands there are zero)
 template <int KS, int J>
 __device__ __forceinline__ void mfma_load_u(const float2 *Ue, int us, int l, int t, float2 (&a)[KS])
 {
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
-        const int j = min(4 * s + (l >> 4), J - 1);
+        const int j = min(mfma_trace(s, l >> 4), J - 1);
         a[s] = Ue[(size_t)j * us + (t << 4) + (l & 15)];
     }
 }
@@ -335,13 +394,59 @@ __device__ __forceinline__ void mfma_tile(const MfmaGroup<NDF, NDB> &G, const fl
         accR[c] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
         accI[c] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
     }
+    if constexpr (kMfmaSplit) {
+        // the tile's scale: its largest |U| to below 2^14
+        float mx = 0.0f;
 #pragma unroll
-    for (int s = 0; s < KS; ++s) {
+        for (int s = 0; s < KS; ++s) mx = fmaxf(mx, fmaxf(fabsf(a[s].x), fabsf(a[s].y)));
 #pragma unroll
-        for (int c = 0; c < NDB; ++c) {
-            const float w = sW[(s * NDB + c) * 64 + l];
-            accR[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s].x, w, accR[c], 0, 0, 0);
-            accI[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s].y, w, accI[c], 0, 0, 0);
+        for (int o = 1; o < 64; o <<= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+        int e;
+        (void)frexpf(mx, &e);
+        const int eu = __builtin_amdgcn_readfirstlane(14 - e);
+        const v8h *sWh = reinterpret_cast<const v8h *>(sW);
+#pragma unroll
+        for (int kb = 0; kb < KS / 8; ++kb) {
+            v8h rh, rl, ih, il;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                _Float16 h, o;
+                mfma_split(ldexpf(a[8 * kb + q].x, eu), h, o);
+                rh[q] = h;
+                rl[q] = o;
+                mfma_split(ldexpf(a[8 * kb + q].y, eu), h, o);
+                ih[q] = h;
+                il[q] = o;
+            }
+#pragma unroll
+            for (int c = 0; c < NDB; ++c) {
+                const v8h wh = sWh[(2 * (kb * NDB + c)) * 64 + l], wl = sWh[(2 * (kb * NDB + c) + 1) * 64 + l];
+                accR[c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(rh, wh, accR[c], 0, 0, 0);
+                accI[c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ih, wh, accI[c], 0, 0, 0);
+                accR[c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(rh, wl, accR[c], 0, 0, 0);
+                accI[c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ih, wl, accI[c], 0, 0, 0);
+                accR[c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(rl, wh, accR[c], 0, 0, 0);
+                accI[c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(il, wh, accI[c], 0, 0, 0);
+            }
+        }
+        // back to the spectra's scale (column l % 16: this lane's pair)
+        const int es = -(eu + G.ew);
+#pragma unroll
+        for (int c = 0; c < NDB; ++c)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                accR[c][v] = ldexpf(accR[c][v], es);
+                accI[c][v] = ldexpf(accI[c][v], es);
+            }
+    } else {
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+#pragma unroll
+            for (int c = 0; c < NDB; ++c) {
+                const float w = sW[(s * NDB + c) * 64 + l];
+                accR[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s].x, w, accR[c], 0, 0, 0);
+                accI[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s].y, w, accI[c], 0, 0, 0);
+            }
         }
     }
     // base kernels of this lane's pair at bins kb + v
@@ -453,7 +558,7 @@ __global__ __launch_bounds__(64, (NDB <= 4 && NDF <= 4) ? 2 : 1) void k_bwd_mfma
     const int *model_idx, const float *model, int ndepth, int relative, int nk, double *phys, double *base_out,
     const double *xyz, const uint8_t *active, int P, const double *rec_xyz, int xcd)
 {
-    constexpr int J = NVB * 25, KS = (J + 3) / 4;
+    constexpr int J = NVB * 25, KS = mfma_ks(J);
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int grp = xcd ? xcd_block_n(ngroups) : ((int)blockIdx.x < ngroups ? (int)blockIdx.x : -1);
     if (grp < 0) return;
@@ -536,7 +641,7 @@ __global__ __launch_bounds__(64 * W) void k_bwd_mfma_wg(
     const int *model_idx, const float *model, int ndepth, int relative, int nk, double *phys, double *base_out,
     const double *xyz, const uint8_t *active, int P, const double *rec_xyz)
 {
-    constexpr int J = NVB * 25, KS = (J + 3) / 4, TILE = KS * 4 * 16;   // float2 per tile
+    constexpr int J = NVB * 25, KS = mfma_ks(J), TILE = KS * 4 * 16;   // float2 per tile
     constexpr int PER = (TILE + 64 * W - 1) / (64 * W);
     extern __shared__ __align__(16) unsigned char smem_raw[];
     if ((int)blockIdx.x >= ubatch) return;
@@ -585,7 +690,7 @@ __global__ __launch_bounds__(64 * W) void k_bwd_mfma_wg(
                 const float2 *tile = sU + (t & 1) * TILE;
                 float2 a[KS];
 #pragma unroll
-                for (int s = 0; s < KS; ++s) a[s] = tile[(4 * s + bq) * 16 + jp];
+                for (int s = 0; s < KS; ++s) a[s] = tile[mfma_trace(s, bq) * 16 + jp];
                 float4 f4[NDF][2];
                 mfma_load_f<NDF, NDB>(G, Fp, l, t, pc.fstride, f4);
                 mfma_tile<NDF, NDB, KS, DR>(G, a, f4, sW, sAcc, l, t, pc, rd, Hp, dreg);
