@@ -1699,12 +1699,12 @@ int init_impl(const mck_setup_t *s) {
         G.mfma_waves = (w == 2 || w == 4) ? w : 1;
     }
     {
-        // c1 (nt 2048, 4 + 4 components, 2 backward variables): 0.45 us per group, 0.93 us
-        // per element run, 0.0985 us per pair (k_bwd_mfma 28.4 ms / 63,650 groups,
+        // c1 (nt 2048, 4 + 4 components, 2 backward variables): 0.355 us per group, 0.93 us
+        // per element run, 0.0985 us per pair (k_bwd_mfma 22.6 ms / 63,650 groups,
         // k_elem_spectra 3.0 ms / 3,239 runs, k_bwd_eval 86.2 ms / 875k pairs), scaled
         // with the trace length and the carried components / traces
         const double sc = G.pc.nt / 2048.0, J = G.pc.nvar_bwd * 25.0, KS = std::ceil(J / 4.0);
-        G.cost_group_us = 0.45 * sc * (KS * G.pc.nd_b) / (13.0 * 4.0) * (0.5 + 0.5 * G.pc.nd_f / 4.0);
+        G.cost_group_us = 0.355 * sc * (KS * G.pc.nd_b) / (13.0 * 4.0) * (0.5 + 0.5 * G.pc.nd_f / 4.0);
         G.cost_run_us = 0.93 * sc * J / 50.0;
         G.cost_pair_us = 0.0985 * sc * (G.pc.nd_f + G.pc.nd_b) / 8.0;
     }
