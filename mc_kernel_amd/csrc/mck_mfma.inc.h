@@ -9,8 +9,8 @@
 //     B_c(k) = sum_j W_c,j U_j(k),    U_j = rfft(taper . u_j)
 // The nodal spectra U of every backward element a launch touches are computed once
 // (k_elem_spectra, one workgroup per element); the pairs of an element then form their
-// spectra as one matrix product [bins x traces] x [traces x pairs] on the fp32 matrix
-// cores (v_mfma_f32_16x16x4f32), fused with the frequency-domain contraction of
+// spectra as one matrix product [bins x traces] x [traces x pairs] on the matrix cores
+// (fp32 operands as split fp16 products, kMfmaSplit), fused with the frequency-domain contraction of
 // k_bwd_eval: base kernels per bin (kernel.f90:514-570) against the forward spectra of
 // the pair's point, the filtered window integrals sum_k Re(bk_k H_k) accumulated in FP64
 // (kernel.f90:410-510, 946-1020), and the physical kernels (kernel.f90:665-832).
@@ -551,7 +551,7 @@ __device__ __forceinline__ void mfma_finish(const MfmaGroup<NDF, NDB> &G, int l,
 // XCD-contiguous ranges (the groups of an element on one XCD).
 // ------------------------------------------------------------------------------------
 template <int NDF, int NDB, int NVB>
-__global__ __launch_bounds__(64, (NDB <= 4 && NDF <= 4) ? 2 : 1) void k_bwd_mfma(
+__global__ __launch_bounds__(64, NDB <= 4 ? 2 : 1) void k_bwd_mfma(
     PathConf pc, int u0, int ubatch, int ngroups, const int *group_first, const int *group_erun,
     const float2 *U, const int *order, int R, const float *Wb, const float *Qb, const float *Qf,
     const float2 *F, const RecDesc *recs, const KernDesc *kern, const float2 *Hp, int npos,
