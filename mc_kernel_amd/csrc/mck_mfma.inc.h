@@ -150,6 +150,26 @@ __device__ __forceinline__ int mfma_trace(int s, int g)
     return kMfmaSplit ? 32 * (s >> 3) + 8 * g + (s & 7) : 4 * s + g;
 }
 
+// the largest of a non-negative value over the wave (all 64 lanes active): within each
+// row of 16 lanes by DPP (lane swaps 1 and 2, rotations by 4 and 8), then across the 4
+// rows on the bit patterns (non-negative floats order as unsigned integers)
+__device__ __forceinline__ float mfma_wave_max(float x)
+{
+    int v = __builtin_bit_cast(int, x);
+    v = __builtin_bit_cast(int, fmaxf(__builtin_bit_cast(float, v),
+                                      __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false))));
+    v = __builtin_bit_cast(int, fmaxf(__builtin_bit_cast(float, v),
+                                      __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false))));
+    v = __builtin_bit_cast(int, fmaxf(__builtin_bit_cast(float, v),
+                                      __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, v, 0x124, 0xF, 0xF, false))));
+    v = __builtin_bit_cast(int, fmaxf(__builtin_bit_cast(float, v),
+                                      __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, v, 0x128, 0xF, 0xF, false))));
+    const unsigned r0 = (unsigned)__builtin_amdgcn_readlane(v, 0), r1 = (unsigned)__builtin_amdgcn_readlane(v, 16);
+    const unsigned r2 = (unsigned)__builtin_amdgcn_readlane(v, 32), r3 = (unsigned)__builtin_amdgcn_readlane(v, 48);
+    const unsigned m = r0 > r1 ? r0 : r1, n = r2 > r3 ? r2 : r3;
+    return __builtin_bit_cast(float, m > n ? m : n);
+}
+
 // x (scaled into fp16's range) = hi + lo to 22 bits
 __device__ __forceinline__ void mfma_split(float x, _Float16 &hi, _Float16 &lo)
 {
@@ -399,8 +419,7 @@ __device__ __forceinline__ void mfma_tile(const MfmaGroup<NDF, NDB> &G, const fl
         float mx = 0.0f;
 #pragma unroll
         for (int s = 0; s < KS; ++s) mx = fmaxf(mx, fmaxf(fabsf(a[s].x), fabsf(a[s].y)));
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+        mx = mfma_wave_max(mx);
         int e;
         (void)frexpf(mx, &e);
         const int eu = __builtin_amdgcn_readfirstlane(14 - e);
